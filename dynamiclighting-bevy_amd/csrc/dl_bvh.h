@@ -1,0 +1,35 @@
+// Occluder BVH in device layout: 4-wide nodes of 128 bytes (one cache line pair), child boxes in
+// SoA order so one node is eight float4 loads, and triangles in leaf order as three float4 each
+// (v0 | occluder index, e1 = v1 - v0, e2 = v2 - v0).
+//
+// Boxes are grown by a relative 2^-15 of their coordinate magnitude, so the slab test, itself
+// widened by a relative 2^-16, never culls a triangle that Moller-Trumbore reports as hit: any-hit
+// results equal a scan over all occluders and closest-hit results equal the lowest (t, index)
+// over all occluders (occluder.rs:180-223 with ties resolved as at occluder.rs:205-209).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace dl {
+
+struct alignas(16) Node4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];  // >= 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1))
+  int32_t count;     // children in use (1..4)
+  int32_t pad[3];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
+
+struct Bvh4 {
+  std::vector<Node4> nodes;
+  std::vector<float> tris;      // 12 floats per triangle, leaf order
+  std::vector<uint32_t> order;  // leaf position -> occluder index
+  uint32_t depth = 0;           // node levels on the longest root-to-leaf path
+};
+
+// Builds over `n` triangles given as 9 floats each (v0, v1, v2).
+void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out);
+
+}  // namespace dl

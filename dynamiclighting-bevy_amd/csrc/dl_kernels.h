@@ -1,0 +1,135 @@
+// Launch wrappers for the bake kernels (dl_kernels.hip).  Everything here is plain data: device
+// pointers and sizes.  All wrappers enqueue on `stream` and return the hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dl_bvh.h"
+#include "dl_math.h"
+#include "dl_trig.h"
+
+namespace dl {
+
+// One baked light in device form (per BVH slot); host-derived per-light quantities included.
+struct DevLight {
+  float pos[3];
+  float radius;
+  float bmin[3];  // sphere bounds Aabb::from_sphere (light_bvh.rs:74-78)
+  float bmax[3];
+  int32_t shape;
+  uint32_t bits;
+  uint32_t alpha;
+  uint32_t bakes_bounce;
+  uint32_t distant;
+  float reach;    // shape_reach (light_shape.rs:22-28)
+  float ax[3], ay[3], af[3];  // area light axes x, y, forward (light/area.rs:46-57)
+  float hx, hy;
+  uint32_t rect;
+  float toward[3];  // sun direction toward the sun
+  float srad, stan, ssin;
+  uint32_t bounce_samples;
+  float bounce_intensity;
+  uint64_t id;
+};
+
+// Occluder scene on the device.
+struct DevScene {
+  const Node4* nodes;
+  const float4* tris;
+  uint32_t ntris;
+  uint32_t stack;  // LDS stack entries per lane
+  uint32_t* overflow;
+};
+
+// Receiver triangles of the meshes being baked (SoA, one entry per triangle).
+struct DevTris {
+  uint32_t n;
+  const float* v;       // 9 per triangle
+  const float* nrm;     // 9 per triangle (vertex normals or the face normal three times)
+  const float* uv;      // 6 per triangle (sanitized UV1)
+  const uint32_t* mesh; // mesh index
+  const uint32_t* local;// triangle index within its mesh
+  float* face;          // 3 per triangle (written by setup)
+  uint32_t* crop;       // 4 per triangle (written by setup)
+};
+
+struct DevMeshes {
+  uint32_t n;
+  const uint32_t* res;
+  const uint64_t* id;
+  const uint64_t* owned_off;    // word offset of the owned bit grid
+  const uint64_t* visited_off;  // word offset of the visited bit grid
+  uint32_t* owned;              // word pool
+  uint32_t* visited;            // word pool
+  unsigned long long* rays;     // per mesh counters
+  unsigned long long* samples;
+};
+
+struct DevTexels {
+  uint32_t* xy;     // x | y << 16
+  float4* pos;
+  float4* nrm;
+  uint32_t* tri;
+};
+
+struct DevRegions {
+  const uint32_t* rect;   // 4 per region
+  const uint32_t* bits;
+  const uint64_t* off;    // word offset in pool
+  const uint32_t* mesh;
+  uint32_t* pool;
+};
+
+struct DevEntries {
+  uint64_t n;
+  const uint32_t* light;
+  const uint32_t* region;
+  const uint32_t* tri;
+  const uint64_t* start;  // per triangle, n_tris + 1
+};
+
+hipError_t launch_setup(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, uint32_t nlights,
+                        uint32_t local_lights, uint64_t* light_count, uint64_t* trace_count, uint64_t* touched_count);
+hipError_t launch_assoc_fill(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, uint32_t nlights,
+                             uint32_t local_lights, const uint64_t* ent_start, uint32_t* ent_light, uint32_t* ent_tri);
+hipError_t launch_owned(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_start, uint64_t total);
+hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_count,
+                         const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
+                         const uint64_t* tex_start, DevTexels out);
+hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
+                         uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept_count,
+                         uint32_t block);
+hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint8_t* buried, const uint64_t* start,
+                          unsigned long long* cursor, DevTexels out);
+struct TraceParams {
+  uint64_t seed;
+  uint32_t area_samples, area_min, sky_samples;
+};
+hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights,
+                        DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevRegions reg,
+                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block);
+hipError_t launch_optimize(hipStream_t s, DevTris tris, DevMeshes meshes, DevEntries ent, DevRegions reg,
+                           const uint8_t* has_bounce, uint8_t* keep);
+hipError_t launch_pad_seams(hipStream_t s, DevMeshes meshes, DevRegions reg, const uint32_t* used_regions,
+                            const uint64_t* area_start, uint32_t nused, uint64_t total);
+// Crop + dilate + pack of kept entries into the output words (shadows.rs:148-160, seams.rs:43-95).
+hipError_t launch_crop(hipStream_t s, DevTris tris, DevEntries ent, DevRegions reg, const uint64_t* kept_entry,
+                       const uint64_t* kept_word_start, uint64_t nkept, uint64_t total_words,
+                       const uint64_t* kept_out, uint32_t* out);
+hipError_t launch_serialize(hipStream_t s, DevTris tris, const uint64_t* mesh_base, const uint32_t* ld_off,
+                            const uint64_t* tri_kept_start, const uint64_t* kept_entry, const uint32_t* kept_shadow_off,
+                            const uint32_t* kept_bounce_off, const uint32_t* ent_light, uint32_t stride,
+                            const uint32_t* tri_first_in_mesh, uint32_t* out);
+hipError_t launch_distance_cubes(hipStream_t s, DevScene scene, const DevLight* lights, uint32_t nlocal, uint32_t* out);
+
+// Batched closest-hit / any-hit queries (tests and maps): rays as 8 floats (o, d, max, pad).
+hipError_t launch_ray_queries(hipStream_t s, DevScene scene, const float* rays, uint64_t n, int closest,
+                              float* t_out, uint32_t* idx_out);
+// f32 sin/cos through the exception tables, for the exhaustive host-libm check.
+hipError_t launch_trig_eval(hipStream_t s, TrigTables trig, const float* x, uint64_t n, float* sin_out, float* cos_out);
+
+// Exclusive prefix sum over n+1 entries of `in` (in[n] may be anything; out[n] = total of in[0..n)).
+hipError_t scan_u64(hipStream_t s, const uint64_t* in, uint64_t* out, uint64_t n, void** tmp, size_t* tmp_bytes);
+
+}  // namespace dl

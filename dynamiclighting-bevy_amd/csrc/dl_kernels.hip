@@ -1,0 +1,916 @@
+// gfx950 kernels of the bake.  Each kernel cites the reference function it computes; the f32
+// arithmetic it shares with the host lives in dl_math.h.
+//
+// Ray queries traverse the 4-wide occluder BVH one ray per lane, with the traversal stack in
+// LDS (stack[i * blockDim + lane], conflict-free); the nearest hit child is taken next without
+// a push.  Hot loops are wave-uniform in their (triangle, light) work unit so neighbouring lanes
+// trace neighbouring texels toward the same light.
+#include <hipcub/hipcub.hpp>
+
+#include "dl_kernels.h"
+
+namespace dl {
+
+static constexpr float kBoxEps = 1.0f / 65536.0f;
+static constexpr uint32_t kThreads = 256;
+
+__device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ start, uint64_t n, uint64_t idx) {
+  uint64_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (start[mid] <= idx) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float inv_dir(float d) {
+  float c = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+  return 1.0f / c;
+}
+
+struct Hit {
+  float t, u, v;
+  uint32_t occ;  // occluder index (lowest wins ties)
+  uint32_t pos;  // leaf position of the triangle
+};
+
+// One ray against the occluder BVH.  ANY: true when some occluder has 0 <= t <= tmax (t finite).
+// !ANY: the lowest (t, occluder index) hit with t <= tmax in `hit`.
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
+                                         float tmax, Hit* hit) {
+  if (sc.ntris == 0) return false;
+  const float ix = inv_dir(d.x), iy = inv_dir(d.y), iz = inv_dir(d.z);
+  bool have = false;
+  float bt = tmax;
+  uint32_t sp = 0;
+  int32_t node = 0;
+  for (;;) {
+    const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
+    const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
+    const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
+    const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
+    const float limit = (ANY || !have ? tmax : bt) * (1.0f + kBoxEps);
+    int32_t next = -1;
+    float next_t = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= cnt) break;
+      const float lx = k == 0 ? lox.x : k == 1 ? lox.y : k == 2 ? lox.z : lox.w;
+      const float hx = k == 0 ? hix.x : k == 1 ? hix.y : k == 2 ? hix.z : hix.w;
+      const float ly = k == 0 ? loy.x : k == 1 ? loy.y : k == 2 ? loy.z : loy.w;
+      const float hy = k == 0 ? hiy.x : k == 1 ? hiy.y : k == 2 ? hiy.z : hiy.w;
+      const float lz = k == 0 ? loz.x : k == 1 ? loz.y : k == 2 ? loz.z : loz.w;
+      const float hz = k == 0 ? hiz.x : k == 1 ? hiz.y : k == 2 ? hiz.z : hiz.w;
+      const int32_t c = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
+      const float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
+      const float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
+      const float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
+      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+      tn = tn - fabsf(tn) * kBoxEps;
+      tf = tf + fabsf(tf) * kBoxEps;
+      if (!(tn <= tf && tf >= 0.0f && tn <= limit)) continue;
+      if (c < 0) {
+        const uint32_t v = (uint32_t)(-1 - c);
+        const uint32_t first = v >> 2, count = (v & 3u) + 1u;
+        for (uint32_t i = first; i < first + count; ++i) {
+          const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+          float t, u, vv;
+          if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+          if (!finite_(t)) continue;
+          if (ANY) return true;
+          const uint32_t occ = __float_as_uint(a.w);
+          if (!have || t < bt || (t == bt && occ < hit->occ)) {
+            have = true;
+            bt = t;
+            hit->t = t;
+            hit->u = u;
+            hit->v = vv;
+            hit->occ = occ;
+            hit->pos = i;
+          }
+        }
+      } else if (next < 0) {
+        next = c;
+        next_t = tn;
+      } else if (tn < next_t) {
+        stack[sp * stride] = (uint32_t)next;
+        sp++;
+        next = c;
+        next_t = tn;
+      } else {
+        stack[sp * stride] = (uint32_t)c;
+        sp++;
+      }
+    }
+    if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+    if (next >= 0) {
+      node = next;
+      continue;
+    }
+    if (sp == 0) break;
+    sp--;
+    node = (int32_t)stack[sp * stride];
+  }
+  return have;
+}
+
+__device__ __forceinline__ f3 tri_face_normal(const DevScene& sc, uint32_t pos) {
+  const float4 b = sc.tris[3 * pos + 1], c = sc.tris[3 * pos + 2];
+  return normalize_or_zero(cross(mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z)));
+}
+
+__device__ __forceinline__ f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+// ---- light association and per-triangle rectangles: mesh_bake.rs:141-173 ----------------------
+// light_shape.rs:47-65
+__device__ bool may_face(const DevLight& L, const f3* v, f3 normal) {
+  f3 center = divs(add(add(v[0], v[1]), v[2]), 3.0f);
+  if (L.shape == 4) return true;  // sky
+  if (L.shape == 3) return dot(normal, ld3(L.toward)) + L.ssin > kNdotlTolerance;
+  if (L.shape == 0) return dot(normal, normalize_or_zero(sub(ld3(L.pos), center))) > kNdotlTolerance;
+  f3 c = ld3(L.pos), x = ld3(L.ax), y = ld3(L.ay), fw = ld3(L.af);
+  if (L.rect) {
+    bool lit_side = false;
+    for (int k = 0; k < 3; ++k) lit_side |= dot(fw, sub(v[k], c)) > 0.0f;
+    if (!lit_side) return false;
+  }
+  f3 hx = mul(x, L.hx), hy = mul(y, L.hy);
+  f3 corners[4] = {sub(sub(c, hx), hy), sub(add(c, hx), hy), add(add(c, hx), hy), add(sub(c, hx), hy)};
+  for (int k = 0; k < 4; ++k)
+    if (dot(normal, normalize_or_zero(sub(corners[k], center))) > kNdotlTolerance) return true;
+  return false;
+}
+
+__device__ __forceinline__ bool light_keeps(const DevLight& L, uint32_t slot, uint32_t local_lights, const f3* v,
+                                            f3 face, bool valid, bool flat, f3 bmin, f3 bmax) {
+  if (slot < local_lights) {
+    // light BVH query: sphere bounds overlap the triangle bounds (light_bvh.rs:196-220)
+    if (!(L.bmin[0] <= bmax.x && L.bmin[1] <= bmax.y && L.bmin[2] <= bmax.z && bmin.x <= L.bmax[0] &&
+          bmin.y <= L.bmax[1] && bmin.z <= L.bmax[2]))
+      return false;
+  }
+  if (valid && flat && !L.bakes_bounce && !may_face(L, v, face)) return false;
+  return L.distant || sphere_touches(ld3(L.pos), L.radius, v[0], v[1], v[2]);
+}
+
+__device__ __forceinline__ void load_tri(const DevTris& tr, uint32_t t, f3* v, f3* n, f2* uv) {
+  for (int k = 0; k < 3; ++k) {
+    v[k] = ld3(tr.v + (size_t)t * 9 + 3 * k);
+    n[k] = ld3(tr.nrm + (size_t)t * 9 + 3 * k);
+    uv[k] = mk2(tr.uv[(size_t)t * 6 + 2 * k], tr.uv[(size_t)t * 6 + 2 * k + 1]);
+  }
+}
+
+__global__ void k_setup(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, uint32_t nlights,
+                        uint32_t local_lights, uint64_t* light_count, uint64_t* trace_count, uint64_t* touched_count) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tr.n; t += gridDim.x * blockDim.x) {
+    f3 v[3], n[3];
+    f2 uv[3];
+    load_tri(tr, t, v, n, uv);
+    f3 face = normalize_or_zero(cross(sub(v[1], v[0]), sub(v[2], v[0])));
+    bool valid = !is_zero(face);
+    bool flat = approx_eq3(n[0], n[1], 1e-5f) && approx_eq3(n[1], n[2], 1e-5f);
+    f3 bmin = vmin(vmin(v[0], v[1]), v[2]), bmax = vmax(vmax(v[0], v[1]), v[2]);
+    uint32_t cnt = 0;
+    for (uint32_t s = 0; s < nlights; ++s) cnt += light_keeps(lights[s], s, local_lights, v, face, valid, flat, bmin, bmax);
+    light_count[t] = cnt;
+    uint32_t res = ms.res[tr.mesh[t]];
+    Rect crop = crop_rect(uv, res);
+    tr.crop[4 * t] = crop.x0;
+    tr.crop[4 * t + 1] = crop.y0;
+    tr.crop[4 * t + 2] = crop.x1;
+    tr.crop[4 * t + 3] = crop.y1;
+    tr.face[3 * t] = face.x;
+    tr.face[3 * t + 1] = face.y;
+    tr.face[3 * t + 2] = face.z;
+    uint64_t a = 0, b = 0;
+    if (valid) {
+      Rect r = center_rect(uv, res, 0.0f);
+      a = (uint64_t)(r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
+      Rect q;
+      if (touched_range(uv, res, &q)) b = (uint64_t)(q.x1 - q.x0 + 1) * (q.y1 - q.y0 + 1);
+    }
+    trace_count[t] = a;
+    touched_count[t] = a + b;
+  }
+}
+
+__global__ void k_assoc_fill(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, uint32_t nlights,
+                             uint32_t local_lights, const uint64_t* __restrict__ ent_start, uint32_t* ent_light,
+                             uint32_t* ent_tri) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tr.n; t += gridDim.x * blockDim.x) {
+    f3 v[3], n[3];
+    f2 uv[3];
+    load_tri(tr, t, v, n, uv);
+    f3 face = ld3(tr.face + 3 * (size_t)t);
+    bool valid = !is_zero(face);
+    bool flat = approx_eq3(n[0], n[1], 1e-5f) && approx_eq3(n[1], n[2], 1e-5f);
+    f3 bmin = vmin(vmin(v[0], v[1]), v[2]), bmax = vmax(vmax(v[0], v[1]), v[2]);
+    uint64_t w = ent_start[t];
+    for (uint32_t s = 0; s < nlights; ++s)
+      if (light_keeps(lights[s], s, local_lights, v, face, valid, flat, bmin, bmax)) {
+        ent_light[w] = s;
+        ent_tri[w] = t;
+        w++;
+      }
+  }
+}
+
+// ---- owned texels: mesh_texels.rs:47-55 over triangle_texels (texels.rs:48-72) ------------------
+__global__ void k_owned(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ start, uint64_t total) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t t = (uint32_t)owner_of(start, tr.n, c);
+    uint64_t local = c - start[t];
+    uint32_t m = tr.mesh[t], res = ms.res[m];
+    f2 uv[3];
+    for (int k = 0; k < 3; ++k) uv[k] = mk2(tr.uv[(size_t)t * 6 + 2 * k], tr.uv[(size_t)t * 6 + 2 * k + 1]);
+    Rect r = center_rect(uv, res, 0.0f);
+    uint32_t w = r.x1 - r.x0 + 1;
+    uint32_t x = r.x0 + (uint32_t)(local % w), y = r.y0 + (uint32_t)(local / w);
+    f3 bw;
+    if (uv_bary(uv, texel_center(x, y, res), &bw)) {
+      uint64_t i = (uint64_t)x + (uint64_t)y * res;
+      atomicOr(&ms.owned[ms.owned_off[m] + i / 32], 1u << (i % 32));
+    }
+  }
+}
+
+// ---- traced texels: centered then touched (texels.rs:48-121) -----------------------------------
+__global__ void k_texels(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ trace_count,
+                         const uint64_t* __restrict__ start, uint64_t total, int fill, unsigned long long* counts,
+                         const uint64_t* __restrict__ tex_start, DevTexels out) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t t = (uint32_t)owner_of(start, tr.n, c);
+    uint64_t local = c - start[t];
+    uint32_t m = tr.mesh[t], res = ms.res[m];
+    f3 v[3], n[3];
+    f2 uv[3];
+    load_tri(tr, t, v, n, uv);
+    f3 face = ld3(tr.face + 3 * (size_t)t);
+    uint32_t x, y;
+    f3 w;
+    uint64_t a = trace_count[t];
+    if (local < a) {
+      Rect r = center_rect(uv, res, 0.0f);
+      uint32_t rw = r.x1 - r.x0 + 1;
+      x = r.x0 + (uint32_t)(local % rw);
+      y = r.y0 + (uint32_t)(local / rw);
+      if (!uv_bary(uv, texel_center(x, y, res), &w)) continue;
+    } else {
+      local -= a;
+      Rect q;
+      touched_range(uv, res, &q);
+      uint32_t qw = q.x1 - q.x0 + 1;
+      x = q.x0 + (uint32_t)(local % qw);
+      y = q.y0 + (uint32_t)(local / qw);
+      uint64_t i = (uint64_t)x + (uint64_t)y * res;
+      if ((ms.owned[ms.owned_off[m] + i / 32] >> (i % 32)) & 1u) continue;
+      f2 center = texel_center(x, y, res);
+      if (!nearest_weights(uv, center, &w)) continue;
+      float nx = (uv[0].x * w.x + uv[1].x * w.y) + uv[2].x * w.z;
+      float ny = (uv[0].y * w.x + uv[1].y * w.y) + uv[2].y * w.z;
+      float dx = nx - center.x, dy = ny - center.y;
+      if (maxf_(fabsf(dx), fabsf(dy)) > 0.5f / (float)res) continue;
+    }
+    if (!fill) {
+      atomicAdd(&counts[t], 1ull);
+      continue;
+    }
+    uint64_t slot = tex_start[t] + atomicAdd(&counts[t], 1ull);
+    f3 p = lerp3w(v, w);
+    f3 nn;
+    if (!try_normalize(lerp3w(n, w), &nn)) nn = face;
+    out.xy[slot] = x | (y << 16);
+    out.pos[slot] = make_float4(p.x, p.y, p.z, 0.0f);
+    out.nrm[slot] = make_float4(nn.x, nn.y, nn.z, 0.0f);
+    out.tri[slot] = t;
+  }
+}
+
+// ---- burial: mesh_texels.rs:57-87, is_buried mesh_texels.rs:111-135 ----------------------------
+__global__ void k_burial(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, uint64_t ntex, uint64_t seed,
+                         TrigTables trig, uint8_t* buried, unsigned long long* kept) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntex; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t t = tx.tri[i];
+    f3 normal = ld3(tr.face + 3 * (size_t)t);
+    float4 p4 = tx.pos[i];
+    uint32_t xy = tx.xy[i];
+    f3 origin = add(mk3(p4.x, p4.y, p4.z), mul(normal, kRayOffset));
+    uint32_t m = tr.mesh[t];
+    Pcg rng = stream6(seed, ms.id[m], kBurialStream, tr.local[t], xy & 0xffffu, xy >> 16);
+    float turn = pcg_f32(rng);
+    f3 tangent, bitangent;
+    orthonormal_pair(normal, &tangent, &bitangent);
+    uint32_t back = 0, front = 0;
+    bool is_buried = false;
+    for (uint32_t k = 0; k < 8; ++k) {
+      float ring = (float)((k * 5u) % 8u) + 0.5f;
+      float sin_theta = sqrtf(ring / 8.0f);
+      float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
+      float phi = (((float)k + turn) * kTau) / 8.0f;
+      f3 dir = add(mul(add(mul(tangent, lcosf(trig, phi)), mul(bitangent, lsinf(trig, phi))), sin_theta), mul(normal, cos_theta));
+      Hit h;
+      bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
+      if (bf) back++;
+      else front++;
+      if (back >= 5) { is_buried = true; break; }
+      if (front > 3) break;
+    }
+    buried[i] = is_buried ? 1 : 0;
+    if (!is_buried) atomicAdd(&kept[t], 1ull);
+  }
+}
+
+__global__ void k_compact(DevTexels in, uint64_t ntex, const uint8_t* __restrict__ buried,
+                          const uint64_t* __restrict__ start, unsigned long long* cursor, DevTexels out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntex; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (buried[i]) continue;
+    uint32_t t = in.tri[i];
+    uint64_t slot = start[t] + atomicAdd(&cursor[t], 1ull);
+    out.xy[slot] = in.xy[i];
+    out.pos[slot] = in.pos[i];
+    out.nrm[slot] = in.nrm[i];
+    out.tri[slot] = t;
+  }
+}
+
+// ---- per-texel shadows: texel_shadow.rs:20-90, area.rs:61-146, sun.rs:21-56, sky.rs:20-48 ------
+__device__ __forceinline__ f2 concentric_disk(const TrigTables& trig, f2 sq) {
+  float px = sq.x * 2.0f - 1.0f, py = sq.y * 2.0f - 1.0f;
+  if (px == 0.0f && py == 0.0f) return mk2(0.0f, 0.0f);
+  float radius, angle;
+  if (fabsf(px) > fabsf(py)) {
+    radius = px;
+    angle = kFracPi4 * (py / px);
+  } else {
+    radius = py;
+    angle = 2.0f * kFracPi4 - kFracPi4 * (px / py);
+  }
+  return mk2(lcosf(trig, angle) * radius, lsinf(trig, angle) * radius);
+}
+
+__device__ __forceinline__ f3 cosine_dir(const TrigTables& trig, f3 normal, f2 sq) {
+  f2 disk = concentric_disk(trig, sq);
+  float up = sqrtf(maxf_(1.0f - (disk.x * disk.x + disk.y * disk.y), 0.0f));
+  f3 side, across;
+  orthonormal_pair(normal, &side, &across);
+  return normalize(add(add(mul(side, disk.x), mul(across, disk.y)), mul(normal, up)));
+}
+
+struct Share {
+  bool any_lit = false, any_blocked = false;
+  float lit = 0.0f, total = 0.0f;
+};
+__device__ __forceinline__ bool share_result(const Share& s, float* fraction) {
+  if (s.any_lit != s.any_blocked) {
+    *fraction = s.any_lit ? 1.0f : 0.0f;
+    return true;
+  }
+  if (s.total > 0.0f) {
+    *fraction = s.lit / s.total;
+    return true;
+  }
+  return false;
+}
+
+// area.rs:61-90 with adaptive_share area.rs:98-126 and toward area.rs:130-146
+__device__ bool area_visibility(const DevScene& sc, uint32_t* stack, uint32_t stride, const DevLight& L, f3 origin, f3 pos,
+                                f3 normal, Pcg& rng, uint32_t samples, uint32_t minimum, float* fraction, uint32_t* rays) {
+  f3 c = ld3(L.pos), ax = ld3(L.ax), ay = ld3(L.ay), fw = ld3(L.af);
+  f3 local = sub(pos, c);
+  float npx = clampf_(dot(local, ax), -L.hx, L.hx), npy = clampf_(dot(local, ay), -L.hy, L.hy);
+  f3 nearest = add(add(c, mul(ax, npx)), mul(ay, npy));
+  if (L.reach <= 0.0f || len_sq(sub(pos, nearest)) > L.reach * L.reach) return false;
+  if (L.rect && dot(fw, sub(pos, c)) <= 0.0f) return false;
+  uint32_t nr = 0;
+  Share s;
+  auto trace = [&](f2 uv, bool probe) {
+    float ox = (uv.x * 2.0f - 1.0f) * L.hx, oy = (uv.y * 2.0f - 1.0f) * L.hy;
+    f3 point = add(add(c, mul(ax, ox)), mul(ay, oy));
+    f3 to_point = sub(point, pos);
+    float dsq = len_sq(to_point);
+    if (dsq < 1.0e-12f) return;
+    float distance = sqrtf(dsq);
+    f3 dir = divs(to_point, distance);
+    float receiver = dot(normal, dir) - kNdotlTolerance;
+    float emitter = L.rect ? -dot(fw, dir) : 1.0f;
+    if (!(receiver > 0.0f && emitter > 0.0f)) return;
+    float weight = (receiver * emitter) / dsq;
+    nr++;
+    Hit h;
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, maxf_(distance - kEndGap, 0.0f), &h);
+    if (!probe) {
+      s.total += weight;
+      s.lit += lit ? weight : 0.0f;
+    }
+    s.any_lit |= lit;
+    s.any_blocked |= !lit;
+  };
+  if (L.rect) {
+    trace(mk2(0.0f, 0.0f), true);
+    trace(mk2(1.0f, 0.0f), true);
+    trace(mk2(1.0f, 1.0f), true);
+    trace(mk2(0.0f, 1.0f), true);
+    trace(mk2(0.5f, 0.5f), true);
+  } else {
+    trace(mk2(0.0f, 0.5f), true);
+    trace(mk2(1.0f, 0.5f), true);
+    trace(mk2(0.5f, 0.5f), true);
+  }
+  f2 offset;
+  offset.x = pcg_f32(rng);
+  offset.y = pcg_f32(rng);
+  for (uint32_t taken = 0; taken < samples; ++taken) {
+    if (taken == minimum && s.any_lit != s.any_blocked) break;
+    trace(spread_pt(taken, offset, L.rect != 0), false);
+  }
+  *rays = nr;
+  return share_result(s, fraction);
+}
+
+// sun.rs:21-56
+__device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t stride, const TrigTables& trig,
+                               const DevLight& L, f3 origin, f3 normal, Pcg& rng, uint32_t samples, uint32_t minimum,
+                               float* fraction, uint32_t* rays) {
+  f3 toward = ld3(L.toward);
+  Hit h;
+  if (L.srad <= 0.0f) {
+    if (dot(normal, toward) <= kNdotlTolerance) return false;
+    *fraction = traverse<true>(sc, stack, stride, origin, toward, INFINITY, &h) ? 0.0f : 1.0f;
+    *rays = 1;
+    return true;
+  }
+  f3 side, up;
+  orthonormal_pair(toward, &side, &up);
+  uint32_t nr = 0;
+  Share s;
+  auto trace = [&](f2 disk, bool probe) {
+    f3 dir = normalize(add(toward, mul(add(mul(side, disk.x), mul(up, disk.y)), L.stan)));
+    float receiver = dot(normal, dir) - kNdotlTolerance;
+    if (receiver <= 0.0f) return;
+    nr++;
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, INFINITY, &h);
+    if (!probe) {
+      s.total += receiver;
+      s.lit += lit ? receiver : 0.0f;
+    }
+    s.any_lit |= lit;
+    s.any_blocked |= !lit;
+  };
+  trace(mk2(0.0f, 0.0f), true);
+  trace(mk2(1.0f, 0.0f), true);
+  trace(mk2(0.0f, 1.0f), true);
+  trace(mk2(-1.0f, 0.0f), true);
+  trace(mk2(0.0f, -1.0f), true);
+  f2 offset;
+  offset.x = pcg_f32(rng);
+  offset.y = pcg_f32(rng);
+  for (uint32_t taken = 0; taken < samples; ++taken) {
+    if (taken == minimum && s.any_lit != s.any_blocked) break;
+    trace(concentric_disk(trig, spread_pt(taken, offset, true)), false);
+  }
+  *rays = nr;
+  return share_result(s, fraction);
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+__device__ __forceinline__ void raise_level(uint32_t* pool, uint64_t off, uint32_t bits, uint64_t i, uint32_t level) {
+  if (level == 0) return;
+  if (bits == 1) {
+    atomicOr(&pool[off + i / 32], 1u << (i % 32));
+    return;
+  }
+  uint32_t* w = &pool[off + i / 4];
+  uint32_t shift = 8u * (uint32_t)(i % 4);
+  uint32_t cur = *w;
+  for (;;) {
+    uint32_t have = (cur >> shift) & 0xffu;
+    if (have >= level) break;
+    uint32_t want = (cur & ~(0xffu << shift)) | (level << shift);
+    uint32_t prev = atomicCAS(w, cur, want);
+    if (prev == cur) break;
+    cur = prev;
+  }
+}
+
+__device__ __forceinline__ uint32_t region_level(const DevRegions& rg, uint32_t r, int64_t x, int64_t y) {
+  const uint32_t* rc = rg.rect + 4 * (size_t)r;
+  if (x < (int64_t)rc[0] || y < (int64_t)rc[1] || x > (int64_t)rc[2] || y > (int64_t)rc[3]) return 0;
+  uint64_t w = rc[2] - rc[0] + 1;
+  uint64_t i = (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * w;
+  uint64_t off = rg.off[r];
+  if (rg.bits[r] == 1) return (rg.pool[off + i / 32] >> (i % 32)) & 1u;
+  return (rg.pool[off + i / 4] >> (8u * (uint32_t)(i % 4))) & 0xffu;
+}
+
+// The (triangle x light x texel) loop of mesh_bake.rs:203-267; one wave per (triangle, light,
+// 64-texel chunk).
+__global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
+                        const uint64_t* __restrict__ tex_start, DevEntries ent, DevRegions rg,
+                        const uint64_t* __restrict__ unit_start, uint64_t units, TraceParams p, TrigTables trig) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = blockDim.x >> 6;
+  for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
+    uint32_t t = (uint32_t)owner_of(unit_start, tr.n, unit);
+    uint64_t local = unit - unit_start[t];
+    uint64_t t0 = tex_start[t], nt = tex_start[t + 1] - t0;
+    uint64_t chunks = (nt + 63) / 64;
+    uint64_t k = local / chunks, chunk = local % chunks;
+    uint64_t e = ent.start[t] + k;
+    uint32_t slot = ent.light[e], r = ent.region[e];
+    const DevLight& L = lights[slot];
+    uint64_t i = chunk * 64 + lane;
+    uint32_t rays = 0, some_n = 0;
+    if (i < nt) {
+      uint64_t j = t0 + i;
+      uint32_t xy = tx.xy[j];
+      uint32_t x = xy & 0xffffu, y = xy >> 16;
+      float4 p4 = tx.pos[j], n4 = tx.nrm[j];
+      f3 pos = mk3(p4.x, p4.y, p4.z), nrm = mk3(n4.x, n4.y, n4.z);
+      f3 face = ld3(tr.face + 3 * (size_t)t);
+      f3 origin = add(pos, mul(face, kRayOffset));
+      uint32_t m = tr.mesh[t];
+      bool some = false;
+      uint32_t level = 0;
+      Hit h;
+      if (L.shape == 0) {
+        f3 to_light = sub(ld3(L.pos), pos);
+        float dsq = len_sq(to_light);
+        if (dsq <= L.radius * L.radius) {
+          float distance = sqrtf(dsq);
+          if (distance == 0.0f) {
+            some = true;
+            level = 1;
+          } else {
+            f3 dir = divs(to_light, distance);
+            if (dot(nrm, dir) > kNdotlTolerance) {
+              some = true;
+              rays = 1;
+              level = traverse<true>(sc, stack, blockDim.x, origin, dir, distance, &h) ? 0u : 1u;
+            }
+          }
+        }
+      } else {
+        Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
+        float fraction = 0.0f;
+        if (L.shape == 1 || L.shape == 2) {
+          some = area_visibility(sc, stack, blockDim.x, L, origin, pos, nrm, rng, p.area_samples, p.area_min, &fraction, &rays);
+        } else if (L.shape == 3) {
+          some = sun_visibility(sc, stack, blockDim.x, trig, L, origin, nrm, rng, p.area_samples, p.area_min, &fraction, &rays);
+        } else {
+          f2 offset;
+          offset.x = pcg_f32(rng);
+          offset.y = pcg_f32(rng);
+          uint32_t open = 0;
+          for (uint32_t taken = 0; taken < p.sky_samples; ++taken) {
+            f3 dir = cosine_dir(trig, nrm, spread_pt(taken, offset, true));
+            open += traverse<true>(sc, stack, blockDim.x, origin, dir, INFINITY, &h) ? 0u : 1u;
+          }
+          fraction = (float)open / (float)(p.sky_samples > 1 ? p.sky_samples : 1u);
+          rays = p.sky_samples;
+          some = true;
+        }
+        if (some) level = level_of(fraction, L.bits);
+      }
+      if (some) {
+        some_n = 1;
+        uint32_t res = ms.res[m];
+        uint64_t vi = (uint64_t)x + (uint64_t)y * res;
+        atomicOr(&ms.visited[ms.visited_off[m] + vi / 32], 1u << (vi % 32));
+        const uint32_t* rc = rg.rect + 4 * (size_t)r;
+        uint64_t ri = (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * (rc[2] - rc[0] + 1);
+        raise_level(rg.pool, rg.off[r], rg.bits[r], ri, level);
+      }
+    }
+    uint32_t m = tr.mesh[t];
+    wave_add(&ms.rays[m], rays);
+    wave_add(&ms.samples[m], some_n);
+  }
+}
+
+// ---- optimize: shadows.rs:54-74 with any_light shadows.rs:136-146 -------------------------------
+__global__ void k_optimize(DevTris tr, DevMeshes ms, DevEntries ent, DevRegions rg, const uint8_t* __restrict__ has_bounce,
+                           uint8_t* keep) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ent.n; e += (uint64_t)gridDim.x * blockDim.x) {
+    if (has_bounce && has_bounce[e]) {
+      keep[e] = 1;
+      continue;
+    }
+    uint32_t t = ent.tri[e], r = ent.region[e];
+    f2 uv[3];
+    for (int k = 0; k < 3; ++k) uv[k] = mk2(tr.uv[(size_t)t * 6 + 2 * k], tr.uv[(size_t)t * 6 + 2 * k + 1]);
+    Rect o = center_rect(uv, ms.res[tr.mesh[t]], 1.0f);
+    const uint32_t* rc = rg.rect + 4 * (size_t)r;
+    uint32_t x0 = max(o.x0, rc[0]), x1 = min(o.x1, rc[2]), y0 = max(o.y0, rc[1]), y1 = min(o.y1, rc[3]);
+    bool any = false;
+    if (x0 <= x1 && y0 <= y1) {
+      for (uint32_t y = y0; y <= y1 && !any; ++y)
+        for (uint32_t x = x0; x <= x1; ++x)
+          if (region_level(rg, r, x, y)) {
+            any = true;
+            break;
+          }
+    }
+    keep[e] = any ? 1 : 0;
+  }
+}
+
+// ---- pad_seams: seams.rs:15-39 over every used region ---------------------------------------
+__device__ __forceinline__ bool visited_at(const DevMeshes& ms, uint32_t m, uint32_t res, int64_t x, int64_t y) {
+  if (x < 0 || y < 0 || x >= (int64_t)res || y >= (int64_t)res) return false;
+  uint64_t i = (uint64_t)x + (uint64_t)y * res;
+  return (ms.visited[ms.visited_off[m] + i / 32] >> (i % 32)) & 1u;
+}
+__global__ void k_pad_seams(DevMeshes ms, DevRegions rg, const uint32_t* __restrict__ used, const uint64_t* __restrict__ start,
+                            uint32_t nused, uint64_t total) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = (uint32_t)owner_of(start, nused, c);
+    uint32_t r = used[u];
+    uint64_t local = c - start[u];
+    const uint32_t* rc = rg.rect + 4 * (size_t)r;
+    uint32_t w = rc[2] - rc[0] + 1;
+    int64_t x = rc[0] + (int64_t)(local % w), y = rc[1] + (int64_t)(local / w);
+    uint32_t m = rg.mesh[r], res = ms.res[m];
+    if (visited_at(ms, m, res, x, y)) continue;
+    uint32_t level = 0;
+    const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int64_t nx = x + dx[k], ny = y + dy[k];
+      if (visited_at(ms, m, res, nx, ny)) level = max(level, region_level(rg, r, nx, ny));
+      else if (visited_at(ms, m, res, x + 2 * dx[k], y + 2 * dy[k]))
+        level = max(level, region_level(rg, r, x + 2 * dx[k], y + 2 * dy[k]));
+    }
+    raise_level(rg.pool, rg.off[r], rg.bits[r], (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * w, level);
+  }
+}
+
+// ---- crop_shadows + dilate_shadows + into_words: shadows.rs:148-160, seams.rs:43-95 ------------
+struct CropView {
+  const DevRegions* rg;
+  uint32_t r, cx0, cy0, W, H;
+  __device__ uint32_t orig(int32_t px, int32_t py) const {
+    if (px < 2 || py < 2 || px >= (int32_t)W - 2 || py >= (int32_t)H - 2) return 0;
+    return region_level(*rg, r, (int64_t)cx0 + px - 2, (int64_t)cy0 + py - 2);
+  }
+  __device__ uint32_t m1(int32_t px, int32_t py) const {
+    uint32_t v = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) v = max(v, orig(px + dx, py + dy));
+    return v;
+  }
+  __device__ bool ring0(int32_t px, int32_t py) const { return px == 0 || py == 0 || px == (int32_t)W - 1 || py == (int32_t)H - 1; }
+  __device__ bool ring1(int32_t px, int32_t py) const { return px == 1 || py == 1 || px == (int32_t)W - 2 || py == (int32_t)H - 2; }
+  // the grid after the first dilation pass
+  __device__ uint32_t p1(int32_t px, int32_t py) const {
+    if (px < 0 || py < 0 || px >= (int32_t)W || py >= (int32_t)H) return 0;
+    return ring1(px, py) ? m1(px, py) : orig(px, py);
+  }
+  __device__ uint32_t final_(int32_t px, int32_t py) const {
+    if (ring0(px, py)) {
+      uint32_t v = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) v = max(v, p1(px + dx, py + dy));
+      return v;
+    }
+    return p1(px, py);
+  }
+};
+
+__global__ void k_crop(DevTris tr, DevEntries ent, DevRegions rg, const uint64_t* __restrict__ kept_entry,
+                       const uint64_t* __restrict__ word_start, uint64_t nkept, uint64_t total,
+                       const uint64_t* __restrict__ kept_out, uint32_t* out) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t j = owner_of(word_start, nkept, c);
+    uint64_t wi = c - word_start[j];
+    uint64_t e = kept_entry[j];
+    uint32_t t = ent.tri[e], r = ent.region[e];
+    CropView cv;
+    cv.rg = &rg;
+    cv.r = r;
+    cv.cx0 = tr.crop[4 * (size_t)t];
+    cv.cy0 = tr.crop[4 * (size_t)t + 1];
+    cv.W = tr.crop[4 * (size_t)t + 2] - cv.cx0 + 1 + 4;
+    cv.H = tr.crop[4 * (size_t)t + 3] - cv.cy0 + 1 + 4;
+    uint32_t bits = rg.bits[r];
+    uint32_t per = 32 / bits;
+    uint64_t n = (uint64_t)cv.W * cv.H;
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+      uint64_t i = wi * per + k;
+      if (i >= n) break;
+      int32_t px = (int32_t)(i % cv.W), py = (int32_t)(i / cv.W);
+      uint32_t v = cv.final_(px, py);
+      word |= v << (k * bits);
+    }
+    out[kept_out[j] + wi] = word;
+  }
+}
+
+// ---- serialize_triangles: triangle_words.rs:47-89 (headers and light data) ----------------------
+__global__ void k_serialize(DevTris tr, const uint64_t* __restrict__ mesh_base, const uint32_t* __restrict__ ld_off,
+                            const uint64_t* __restrict__ tri_kept_start, const uint64_t* __restrict__ kept_entry,
+                            const uint32_t* __restrict__ shadow_off, const uint32_t* __restrict__ bounce_off,
+                            const uint32_t* __restrict__ ent_light, uint32_t stride, const uint32_t* __restrict__ first_in_mesh,
+                            uint32_t* out) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tr.n; t += gridDim.x * blockDim.x) {
+    uint32_t m = tr.mesh[t];
+    uint32_t lt = t - first_in_mesh[m];
+    uint32_t* w = out + mesh_base[m];
+    uint32_t off = ld_off[t];
+    uint32_t cx0 = tr.crop[4 * (size_t)t], cy0 = tr.crop[4 * (size_t)t + 1], cx1 = tr.crop[4 * (size_t)t + 2];
+    w[4 * lt] = off;
+    w[4 * lt + 1] = cx0 - 2u;
+    w[4 * lt + 2] = cy0 - 2u;
+    w[4 * lt + 3] = cx1 - cx0 + 1 + 4;
+    uint64_t k0 = tri_kept_start[t], k1 = tri_kept_start[t + 1];
+    w[off] = (uint32_t)(k1 - k0);
+    for (uint64_t j = k0; j < k1; ++j) {
+      uint32_t* le = w + off + 1 + (j - k0) * stride;
+      le[0] = ent_light[kept_entry[j]];
+      le[1] = shadow_off[j];
+      if (stride == 3) le[2] = bounce_off ? bounce_off[j] : 0u;
+    }
+  }
+}
+
+// ---- distance cubes: distance_cubes.rs:29-72 ---------------------------------------------------
+__global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ lights, uint32_t nlocal, uint32_t* out) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  const uint32_t res = 64, per_face = res * res / 2;
+  uint64_t total = (uint64_t)nlocal * 6 * per_face;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t slot = (uint32_t)(c / (6 * per_face));
+    uint32_t rem = (uint32_t)(c % (6 * per_face));
+    uint32_t face = rem / per_face, pair = rem % per_face;
+    const DevLight& L = lights[slot];
+    f3 o = ld3(L.pos);
+    float d[2];
+    for (int k = 0; k < 2; ++k) {
+      uint32_t texel = pair * 2 + k;
+      f3 dir = cube_texel_dir(face, texel % res, texel / res, res);
+      Hit h;
+      d[k] = traverse<false>(sc, stack, blockDim.x, o, dir, L.radius, &h) ? h.t : L.radius;
+    }
+    out[c] = (f32_to_f16_bits(d[1]) << 16) | f32_to_f16_bits(d[0]);
+  }
+}
+
+__global__ void k_ray_queries(DevScene sc, const float* __restrict__ rays, uint64_t n, int closest, float* t_out, uint32_t* idx_out) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const float* r = rays + 8 * i;
+    Hit h;
+    if (closest) {
+      bool hit = traverse<false>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h);
+      t_out[i] = hit ? h.t : -1.0f;
+      idx_out[i] = hit ? h.occ : 0xffffffffu;
+    } else {
+      idx_out[i] = traverse<true>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h) ? 1u : 0u;
+    }
+  }
+}
+
+__global__ void k_trig_eval(TrigTables trig, const float* __restrict__ x, uint64_t n, float* s, float* c) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    s[i] = lsinf(trig, x[i]);
+    c[i] = lcosf(trig, x[i]);
+  }
+}
+
+// ---- launchers ------------------------------------------------------------------------------
+static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
+  uint64_t g = (n + block - 1) / block;
+  if (g == 0) g = 1;
+  return (uint32_t)(g < cap ? g : cap);
+}
+static inline size_t stack_bytes(const DevScene& sc, uint32_t block) { return (size_t)(sc.stack + 4) * block * 4; }
+
+hipError_t launch_setup(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, uint32_t nlights,
+                        uint32_t local_lights, uint64_t* light_count, uint64_t* trace_count, uint64_t* touched_count) {
+  if (tris.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(tris.n, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, nlights,
+                     local_lights, light_count, trace_count, touched_count);
+  return hipGetLastError();
+}
+hipError_t launch_assoc_fill(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, uint32_t nlights,
+                             uint32_t local_lights, const uint64_t* ent_start, uint32_t* ent_light, uint32_t* ent_tri) {
+  if (tris.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assoc_fill, dim3(grid_for(tris.n, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, nlights,
+                     local_lights, ent_start, ent_light, ent_tri);
+  return hipGetLastError();
+}
+hipError_t launch_owned(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_start, uint64_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_owned, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, tris, meshes, trace_start, total);
+  return hipGetLastError();
+}
+hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_count,
+                         const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
+                         const uint64_t* tex_start, DevTexels out) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_texels, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, tris, meshes, trace_count, cand_start,
+                     total, fill, counts, tex_start, out);
+  return hipGetLastError();
+}
+hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
+                         uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept_count, uint32_t block) {
+  if (ntex == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_burial, dim3(grid_for(ntex, block)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
+                     tx, ntex, seed, trig, buried, kept_count);
+  return hipGetLastError();
+}
+hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint8_t* buried, const uint64_t* start,
+                          unsigned long long* cursor, DevTexels out) {
+  if (ntex == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(ntex, kThreads)), dim3(kThreads), 0, s, in, ntex, buried, start, cursor, out);
+  return hipGetLastError();
+}
+hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights,
+                        DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevRegions reg,
+                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block) {
+  if (units == 0) return hipSuccess;
+  uint32_t waves = block / 64;
+  hipLaunchKernelGGL(k_trace, dim3(grid_for(units, waves)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
+                     lights, tx, tex_start, ent, reg, unit_start, units, p, trig);
+  return hipGetLastError();
+}
+hipError_t launch_optimize(hipStream_t s, DevTris tris, DevMeshes meshes, DevEntries ent, DevRegions reg,
+                           const uint8_t* has_bounce, uint8_t* keep) {
+  if (ent.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_optimize, dim3(grid_for(ent.n, kThreads)), dim3(kThreads), 0, s, tris, meshes, ent, reg, has_bounce, keep);
+  return hipGetLastError();
+}
+hipError_t launch_pad_seams(hipStream_t s, DevMeshes meshes, DevRegions reg, const uint32_t* used_regions,
+                            const uint64_t* area_start, uint32_t nused, uint64_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pad_seams, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, meshes, reg, used_regions,
+                     area_start, nused, total);
+  return hipGetLastError();
+}
+hipError_t launch_crop(hipStream_t s, DevTris tris, DevEntries ent, DevRegions reg, const uint64_t* kept_entry,
+                       const uint64_t* kept_word_start, uint64_t nkept, uint64_t total_words, const uint64_t* kept_out,
+                       uint32_t* out) {
+  if (total_words == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_crop, dim3(grid_for(total_words, kThreads)), dim3(kThreads), 0, s, tris, ent, reg, kept_entry,
+                     kept_word_start, nkept, total_words, kept_out, out);
+  return hipGetLastError();
+}
+hipError_t launch_serialize(hipStream_t s, DevTris tris, const uint64_t* mesh_base, const uint32_t* ld_off,
+                            const uint64_t* tri_kept_start, const uint64_t* kept_entry, const uint32_t* kept_shadow_off,
+                            const uint32_t* kept_bounce_off, const uint32_t* ent_light, uint32_t stride,
+                            const uint32_t* tri_first_in_mesh, uint32_t* out) {
+  if (tris.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_serialize, dim3(grid_for(tris.n, kThreads)), dim3(kThreads), 0, s, tris, mesh_base, ld_off,
+                     tri_kept_start, kept_entry, kept_shadow_off, kept_bounce_off, ent_light, stride, tri_first_in_mesh, out);
+  return hipGetLastError();
+}
+hipError_t launch_distance_cubes(hipStream_t s, DevScene scene, const DevLight* lights, uint32_t nlocal, uint32_t* out) {
+  if (nlocal == 0) return hipSuccess;
+  uint64_t total = (uint64_t)nlocal * 6 * 2048;
+  hipLaunchKernelGGL(k_distance_cubes, dim3(grid_for(total, 128)), dim3(128), stack_bytes(scene, 128), s, scene, lights,
+                     nlocal, out);
+  return hipGetLastError();
+}
+hipError_t launch_ray_queries(hipStream_t s, DevScene scene, const float* rays, uint64_t n, int closest, float* t_out,
+                              uint32_t* idx_out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ray_queries, dim3(grid_for(n, 128)), dim3(128), stack_bytes(scene, 128), s, scene, rays, n, closest,
+                     t_out, idx_out);
+  return hipGetLastError();
+}
+hipError_t launch_trig_eval(hipStream_t s, TrigTables trig, const float* x, uint64_t n, float* sin_out, float* cos_out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trig_eval, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, trig, x, n, sin_out, cos_out);
+  return hipGetLastError();
+}
+
+hipError_t scan_u64(hipStream_t s, const uint64_t* in, uint64_t* out, uint64_t n, void** tmp, size_t* tmp_bytes) {
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)(n + 1), s);
+  if (e != hipSuccess) return e;
+  if (need > *tmp_bytes) {
+    if (*tmp) (void)hipFree(*tmp);
+    *tmp = nullptr;
+    e = hipMalloc(tmp, need);
+    if (e != hipSuccess) { *tmp_bytes = 0; return e; }
+    *tmp_bytes = need;
+  }
+  return hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, (int)(n + 1), s);
+}
+
+}  // namespace dl

@@ -1,0 +1,129 @@
+"""MI355X bake backend: Python face of libdynlight_hip.so.
+
+Mirrors the reference's pure bake API (src/bake/baker.rs:27-31):
+
+    scene = dynlight_hip.bake(inp, options)          # BakedScene, or raises BakeError
+
+`inp` is an abi.BakeInput (BakeMesh/BakeLight data), `options` an abi.dl_bake_options
+(BakeOptions; `abi.default_options()` is BakeOptions::default()).  The work runs in the HIP
+library on the GPU; there is no CPU fallback, and loading fails loudly when the library or a
+device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import abi
+from .abi import BakeError, BakeInput, LightData, MeshData, default_options  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdynlight_hip.so")
+PKG_DIR = os.path.dirname(HERE)
+
+_lib = None
+
+
+def build() -> None:
+    """Compiles libdynlight_hip.so for gfx950 with the package Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", PKG_DIR, "-j8"], check=True)
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.dl_ctx_create.restype = C.c_int
+    lib.dl_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+    lib.dl_ctx_destroy.restype = None
+    lib.dl_ctx_destroy.argtypes = [P]
+    lib.dl_last_error.restype = C.c_char_p
+    lib.dl_last_error.argtypes = [P]
+    lib.dl_default_options.restype = None
+    lib.dl_default_options.argtypes = [C.POINTER(abi.dl_bake_options)]
+    lib.dl_bake.restype = C.c_int
+    lib.dl_bake.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options),
+                            C.POINTER(abi.dl_progress), C.POINTER(P)]
+    lib.dl_baked_scene_free.restype = None
+    lib.dl_baked_scene_free.argtypes = [P]
+    lib.dl_last_timings.restype = C.c_int
+    lib.dl_last_timings.argtypes = [P, C.POINTER(abi.dl_timings)]
+    abi.bind_scene_api(lib, "dl_")
+    f32p, u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+    lib.dl_test_ray_queries.restype = C.c_int
+    lib.dl_test_ray_queries.argtypes = [P, f32p, C.c_uint32, f32p, C.c_uint64, C.c_int, f32p, u32p]
+    lib.dl_test_trig.restype = C.c_int
+    lib.dl_test_trig.argtypes = [P, f32p, C.c_uint64, f32p, f32p]
+    lib.dl_test_trig_host.restype = C.c_int
+    lib.dl_test_trig_host.argtypes = [f32p, C.c_uint64, f32p, f32p]
+    lib.dl_test_trig_table_sizes.restype = None
+    lib.dl_test_trig_table_sizes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.dl_set_launch.restype = C.c_int
+    lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
+    _lib = lib
+    return lib
+
+
+class Context:
+    """One HIP device with its stream and resident sin/cos tables (dl_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.dl_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.lib.dl_last_error(None).decode())
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dl_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def error(self) -> str:
+        return self.lib.dl_last_error(self.handle).decode()
+
+    def bake(self, inp: BakeInput, options: abi.dl_bake_options | None = None,
+             progress: abi.dl_progress | None = None) -> abi.BakedScene:
+        """bake() (src/bake/baker.rs:27-203) on the GPU."""
+        options = options if options is not None else default_options()
+        h = C.c_void_p()
+        st = self.lib.dl_bake(self.handle, C.byref(inp.struct), C.byref(options),
+                              C.byref(progress) if progress is not None else None, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.error())
+        try:
+            return abi.read_scene(self.lib, "dl_", h)
+        finally:
+            self.lib.dl_baked_scene_free(h)
+
+    def timings(self) -> dict:
+        t = abi.dl_timings()
+        self.lib.dl_last_timings(self.handle, C.byref(t))
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def set_launch(self, trace_block: int = 0, burial_block: int = 0) -> None:
+        st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)
+        if st != 0:
+            raise BakeError(st, self.error())
+
+
+_default_ctx = None
+
+
+def bake(inp: BakeInput, options: abi.dl_bake_options | None = None,
+         progress: abi.dl_progress | None = None) -> abi.BakedScene:
+    """bake() on device 0 with a process-wide context."""
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx.bake(inp, options, progress)

@@ -1,0 +1,297 @@
+"""Synthetic bake inputs: the reference's test-scene helpers and the seeded benchmark scenes.
+
+Helpers follow src/bake/tests/mod.rs:23-137 (`floor`, `cuboid` with its 3x2 UV1 atlas at 0.8
+fill, `point_light`, `area_light`, `sun_light`, `sky_light`).  The procedural scenes are:
+
+* ``box_room``      -- closed 10x4x10 m room (12 inward triangles) plus 2 seeded boxes, 4 point
+                        lights of range 6-12 m, 16 texels/m, 1-bit shadows, no bounce.
+* ``interior``      -- 8x8 grid of 6x3x6 m rooms with 1x2 m door openings, seeded boxes and
+                        pillars up to ~200k triangles, albedo U(0.1,0.9)^3, 256 point/spot
+                        lights of range 4-12 m with one colored bounce, 32 texels/m.
+
+Randomness is PCG32 seeded through SplitMix64, restated here so scenes are identical on every
+machine.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .abi import (BakeInput, LightData, MeshData, ILLUMINATION_DIRECT,
+                  ILLUMINATION_SINGLE_BOUNCE, SHAPE_POINT, SHAPE_RECTANGLE, SHAPE_SKY,
+                  SHAPE_SUN, SHAPE_TUBE, default_options, DYNAMIC_ANGULAR)
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(v: int) -> int:
+    v = (v + 0x9E3779B97F4A7C15) & MASK64
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & MASK64
+    return v ^ (v >> 31)
+
+
+class Pcg32:
+    """PCG32 XSH-RR (same recurrence as src/effects/rng.rs:48-71)."""
+
+    def __init__(self, seed: int, stream: int = 54):
+        self.state = 0
+        self.inc = ((stream << 1) | 1) & MASK64
+        self.next_u32()
+        self.state = (self.state + seed) & MASK64
+        self.next_u32()
+
+    @classmethod
+    def seeded(cls, seed: int) -> "Pcg32":
+        s = splitmix64(seed)
+        return cls(s, splitmix64(s))
+
+    def next_u32(self) -> int:
+        old = self.state
+        self.state = (old * 6364136223846793005 + self.inc) & MASK64
+        x = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((x >> rot) | (x << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u32() >> 8) / 16777216.0)
+
+    def below(self, n: int) -> int:
+        return self.next_u32() % n
+
+    def uniforms(self, count: int, lo=0.0, hi=1.0) -> np.ndarray:
+        return np.array([self.uniform(lo, hi) for _ in range(count)], dtype=np.float64)
+
+
+# ---- reference test helpers (src/bake/tests/mod.rs:23-137) -----------------------------------
+def floor(mesh_id: int, center, half: float) -> MeshData:
+    corners = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
+    c = np.asarray(center, dtype=np.float32)
+    pos = np.array([[c[0] + x * half, c[1], c[2] + z * half] for x, z in corners], dtype=np.float32)
+    uv1 = np.array([[(x + 1) * 0.5, (z + 1) * 0.5] for x, z in corners], dtype=np.float32)
+    return MeshData(mesh_id, pos, np.array([0, 2, 1, 0, 3, 2], dtype=np.uint32), uv1,
+                    normals=np.tile(np.array([0, 1, 0], np.float32), (4, 1)))
+
+
+_FACES = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+
+
+def _cuboid_template():
+    """Unit-cube corners (local in [-1,1]), normals, indices and UV1 of `cuboid`."""
+    local, normals, uv1, idx = [], [], [], []
+    for face, n in enumerate(_FACES):
+        u = np.array([1, 0, 0], np.float32) if abs(n[1]) > 0.5 else np.array([0, 1, 0], np.float32)
+        v = np.cross(n, u).astype(np.float32)
+        base = len(local)
+        cell = np.array([face % 3, face // 3], np.float32)
+        for a, b in [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]:
+            local.append(n + u * np.float32(a) + v * np.float32(b))
+            normals.append(n)
+            inner = np.array([a + 1.0, b + 1.0], np.float32) * np.float32(0.5) * np.float32(0.8) + np.float32(0.1)
+            uv1.append((cell + inner) / np.array([3.0, 2.0], np.float32))
+        idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return (np.array(local, np.float32), np.array(normals, np.float32),
+            np.array(idx, np.uint32), np.array(uv1, np.float32))
+
+
+_TEMPLATE = _cuboid_template()
+
+
+def cuboid(mesh_id: int, mn, mx, inward: bool = False, albedo=(1.0, 1.0, 1.0)) -> MeshData:
+    """An axis-aligned box with outward faces (inward for room shells)."""
+    local, normals, idx, uv1 = _TEMPLATE
+    mn = np.asarray(mn, np.float32)
+    mx = np.asarray(mx, np.float32)
+    pos = ((mn + mx) * np.float32(0.5) + local * (mx - mn) * np.float32(0.5)).astype(np.float32)
+    if inward:
+        idx = idx.reshape(-1, 3)[:, [0, 2, 1]].reshape(-1).copy()
+        normals = -normals
+    return MeshData(mesh_id, pos, idx.copy(), uv1.copy(), normals=normals.copy(), albedo=tuple(albedo),
+                    content_hash=splitmix64(mesh_id))
+
+
+def point_light(light_id: int, position, radius: float, bounce: bool = False) -> LightData:
+    return LightData(light_id, tuple(float(x) for x in position), float(radius),
+                     illumination=ILLUMINATION_SINGLE_BOUNCE if bounce else ILLUMINATION_DIRECT)
+
+
+def quat_from_rotation_x(angle: float):
+    s, c = math.sin(angle * 0.5), math.cos(angle * 0.5)
+    return (s, 0.0, 0.0, c)
+
+
+def quat_from_rotation_arc(frm, to):
+    """Quat::from_rotation_arc for unit vectors (used by sun_light)."""
+    frm = np.asarray(frm, np.float64)
+    to = np.asarray(to, np.float64)
+    to = to / np.linalg.norm(to)
+    d = float(np.dot(frm, to))
+    if d < -1.0 + 1e-6:
+        axis = np.cross(frm, [1.0, 0, 0])
+        if np.linalg.norm(axis) < 1e-6:
+            axis = np.cross(frm, [0, 1.0, 0])
+        axis /= np.linalg.norm(axis)
+        return (float(axis[0]), float(axis[1]), float(axis[2]), 0.0)
+    c = np.cross(frm, to)
+    q = np.array([c[0], c[1], c[2], 1.0 + d])
+    q /= np.linalg.norm(q)
+    return tuple(float(x) for x in q)
+
+
+def area_light(light_id: int, position, radius: float, rotation, size) -> LightData:
+    shape = SHAPE_RECTANGLE if size[1] > 0 else SHAPE_TUBE
+    return LightData(light_id, tuple(position), float(radius), shape=shape,
+                     rotation=tuple(rotation), size=tuple(size))
+
+
+def sun_light(light_id: int, toward, degrees: float) -> LightData:
+    return LightData(light_id, (500.0, -300.0, 70.0), 0.5, shape=SHAPE_SUN,
+                     rotation=quat_from_rotation_arc([0, 0, 1.0], toward), angular_diameter=degrees)
+
+
+def sky_light(light_id: int) -> LightData:
+    return LightData(light_id, (-40.0, 900.0, 3.0), 0.5, shape=SHAPE_SKY)
+
+
+# ---- box room: closed room, 2 boxes, 4 point lights ------------------------------------------
+def box_room(seed: int = 1):
+    """Returns (BakeInput, options) of the closed-room scene."""
+    rng = Pcg32.seeded(seed)
+    meshes = [cuboid(1, (-5.0, 0.0, -5.0), (5.0, 4.0, 5.0), inward=True)]
+    for k in range(2):
+        ext = [rng.uniform(0.2, 3.0) for _ in range(3)]
+        cx = rng.uniform(-5.0 + ext[0] / 2, 5.0 - ext[0] / 2)
+        cz = rng.uniform(-5.0 + ext[2] / 2, 5.0 - ext[2] / 2)
+        meshes.append(cuboid(2 + k, (cx - ext[0] / 2, 0.0, cz - ext[2] / 2),
+                             (cx + ext[0] / 2, ext[1], cz + ext[2] / 2)))
+    lights = []
+    for k in range(4):
+        p = (rng.uniform(-4.9, 4.9), rng.uniform(0.1, 3.9), rng.uniform(-4.9, 4.9))
+        lights.append(point_light(100 + k, p, rng.uniform(6.0, 12.0)))
+    opts = default_options()
+    opts.pixel_density = 16.0
+    opts.dynamic_geometry = DYNAMIC_ANGULAR
+    return BakeInput(meshes, lights), opts
+
+
+# ---- interior: 8x8 rooms with doors, boxes/pillars, 256 bounce lights -------------------------
+def _room_shell(ox: float, oz: float, w: float, h: float, d: float, doors: dict):
+    """Inward-facing room shell with 1x2 m door openings; one 3x2 atlas cell per face."""
+    pos, nrm, uv, idx = [], [], [], []
+
+    def quad(p0, p1, p2, p3, n, cell, uvq):
+        base = len(pos)
+        for p, q in zip((p0, p1, p2, p3), uvq):
+            pos.append(p)
+            nrm.append(n)
+            uv.append(((cell[0] + 0.1 + 0.8 * q[0]) / 3.0, (cell[1] + 0.1 + 0.8 * q[1]) / 2.0))
+        idx.extend([base, base + 1, base + 2, base, base + 2, base + 3])
+
+    x0, x1, z0, z1 = ox, ox + w, oz, oz + d
+    # floor (normal +y) and ceiling (normal -y): windings chosen so cross(b-a, c-a) = normal
+    quad((x0, 0, z0), (x0, 0, z1), (x1, 0, z1), (x1, 0, z0), (0, 1, 0), (0, 0),
+         [(0, 0), (0, 1), (1, 1), (1, 0)])
+    quad((x0, h, z0), (x1, h, z0), (x1, h, z1), (x0, h, z1), (0, -1, 0), (1, 0),
+         [(0, 0), (1, 0), (1, 1), (0, 1)])
+    # walls: (name, origin, along, normal, length, cell)
+    walls = [("-x", (x0, 0, z0), (0, 0, 1), (1, 0, 0), d, (2, 0)),
+             ("+x", (x1, 0, z1), (0, 0, -1), (-1, 0, 0), d, (0, 1)),
+             ("-z", (x1, 0, z0), (-1, 0, 0), (0, 0, 1), w, (1, 1)),
+             ("+z", (x0, 0, z1), (1, 0, 0), (0, 0, -1), w, (2, 1))]
+    for name, o, a, n, length, cell in walls:
+        o = np.array(o, np.float64)
+        a = np.array(a, np.float64)
+        up = np.array([0.0, 1.0, 0.0])
+
+        def wall_quad(s0, s1, t0, t1):
+            p = lambda s, t: tuple(o + a * s + up * t)  # noqa: E731
+            # cross(along, up) points into the room for these orientations
+            quad(p(s0, t0), p(s1, t0), p(s1, t1), p(s0, t1), n, cell,
+                 [(s0 / length, t0 / h), (s1 / length, t0 / h), (s1 / length, t1 / h), (s0 / length, t1 / h)])
+
+        if doors.get(name):
+            dl, dr = length / 2 - 0.5, length / 2 + 0.5
+            wall_quad(0.0, dl, 0.0, h)
+            wall_quad(dr, length, 0.0, h)
+            wall_quad(dl, dr, 2.0, h)
+        else:
+            wall_quad(0.0, length, 0.0, h)
+    pos = np.array(pos, np.float32)
+    nrm = np.array(nrm, np.float32)
+    idx = np.array(idx, np.uint32)
+    # fix windings so every face normal points along the stored normal
+    tri = idx.reshape(-1, 3)
+    a, b, c = pos[tri[:, 0]], pos[tri[:, 1]], pos[tri[:, 2]]
+    fn = np.cross(b - a, c - a)
+    flip = np.einsum("ij,ij->i", fn, nrm[tri[:, 0]]) < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    return pos, nrm, tri.reshape(-1).astype(np.uint32), np.array(uv, np.float32)
+
+
+def interior(seed: int = 2, target_triangles: int = 200_000, lights: int = 256,
+             pillar_share: float = 0.3, grid: int = 8, density: float = 32.0):
+    """Returns (BakeInput, options) of the procedural interior scene."""
+    rng = Pcg32.seeded(seed)
+    W, H, D = 6.0, 3.0, 6.0
+    positions, normals, indices, uvs, ids, albedo = [], [], [], [], [], []
+    vcount = [0]
+    icount = [0]
+
+    def add(pos, nrm, idx, uv, mesh_id, alb):
+        positions.append(pos)
+        normals.append(nrm)
+        indices.append(idx)
+        uvs.append(uv)
+        ids.append(mesh_id)
+        albedo.append(alb)
+        vcount.append(vcount[-1] + len(pos))
+        icount.append(icount[-1] + len(idx))
+
+    mesh_id = 1
+    for gz in range(grid):
+        for gx in range(grid):
+            doors = {"-x": gx > 0, "+x": gx < grid - 1, "-z": gz > 0, "+z": gz < grid - 1}
+            p, n, i, u = _room_shell(gx * W, gz * D, W, H, D, doors)
+            add(p, n, i, u, mesh_id, (rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9)))
+            mesh_id += 1
+    shell_tris = icount[-1] // 3
+    boxes = max(0, (target_triangles - shell_tris) // 12)
+    local, tnrm, tidx, tuv = _TEMPLATE
+    for _ in range(boxes):
+        room = rng.below(grid * grid)
+        rx, rz = (room % grid) * W, (room // grid) * D
+        if rng.uniform() < pillar_share:
+            ext = [rng.uniform(0.2, 1.0), H, 0.0]
+            ext[2] = rng.uniform(0.2, 1.0)
+        else:
+            ext = [rng.uniform(0.2, 3.0), rng.uniform(0.2, 3.0), rng.uniform(0.2, 3.0)]
+        cx = rx + rng.uniform(ext[0] / 2, W - ext[0] / 2)
+        cz = rz + rng.uniform(ext[2] / 2, D - ext[2] / 2)
+        mn = np.array([cx - ext[0] / 2, 0.0, cz - ext[2] / 2], np.float32)
+        mx = np.array([cx + ext[0] / 2, ext[1], cz + ext[2] / 2], np.float32)
+        pos = ((mn + mx) * np.float32(0.5) + local * (mx - mn) * np.float32(0.5)).astype(np.float32)
+        add(pos, tnrm, tidx, tuv, mesh_id, (rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9)))
+        mesh_id += 1
+    light_list = []
+    for k in range(lights):
+        room = rng.below(grid * grid)
+        rx, rz = (room % grid) * W, (room // grid) * D
+        p = (rx + rng.uniform(0.1, W - 0.1), rng.uniform(0.1, H - 0.1), rz + rng.uniform(0.1, D - 0.1))
+        light_list.append(point_light(10_000 + k, p, rng.uniform(4.0, 12.0), bounce=True))
+    inp = BakeInput.from_arrays(
+        ids=np.array(ids, np.uint64), positions=np.concatenate(positions), indices=np.concatenate(indices),
+        uv1=np.concatenate(uvs), normals=np.concatenate(normals),
+        vertex_offsets=np.array(vcount, np.int64), index_offsets=np.array(icount, np.int64),
+        albedo=np.array(albedo, np.float32), content_hash=np.array([splitmix64(i) for i in ids], np.uint64),
+        lights=light_list)
+    opts = default_options()
+    opts.pixel_density = density
+    opts.colored_bounce = 1
+    opts.dynamic_geometry = DYNAMIC_ANGULAR
+    return inp, opts
+
+
+def mesh_sample_mask(ids: np.ndarray, modulus: int) -> np.ndarray:
+    """Deterministic receiver subset: meshes with splitmix64(id) % modulus == 0."""
+    return np.array([splitmix64(int(i)) % modulus == 0 for i in ids], dtype=bool)

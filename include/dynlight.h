@@ -1,0 +1,252 @@
+/*
+ * dynlight.h -- C ABI of the MI355X bake backend (libdynlight_hip.so).
+ *
+ * A drop-in for the reference crate's pure bake entry point and its plain-data boundary:
+ *
+ *   pub fn bake(input: &BakeInput, options: &BakeOptions, progress: &BakeProgress)
+ *       -> Result<BakedScene, BakeError>                 (src/bake/baker.rs:27-31)
+ *
+ * Every struct here is plain C: pointers and sizes, borrowed for the duration of one call.
+ * Results are owned by the library and released with the matching *_free function.  Nothing
+ * unwinds across this boundary; failures return a negative dl_status and leave a message that
+ * dl_last_error() returns.
+ */
+#ifndef DYNLIGHT_H
+#define DYNLIGHT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: BakeError (src/bake/input.rs:198-225) plus device classes ------------- */
+typedef enum dl_status {
+    DL_OK = 0,
+    DL_E_CANCELLED = -1,       /* BakeError::Cancelled                                        */
+    DL_E_DUPLICATE_ID = -2,    /* BakeError::DuplicateId { kind, id }                         */
+    DL_E_INVALID_MESH = -3,    /* BakeError::InvalidMesh { id, reason }                       */
+    DL_E_INVALID_LIGHT = -4,   /* BakeError::InvalidLight { id, reason }                      */
+    DL_E_INVALID_OPTIONS = -5, /* BakeError::InvalidOptions(reason)                           */
+    DL_E_DEVICE = -16,         /* a HIP runtime call failed                                   */
+    DL_E_OOM = -17,            /* device or host allocation failed                            */
+    DL_E_ARGUMENT = -18,       /* null pointer or out-of-range argument at the ABI itself     */
+    DL_E_FORMAT = -19          /* FormatError of decode (src/bake/format/mod.rs:53-77)        */
+} dl_status;
+
+/* ---- enums mirroring src/light/kinds.rs:97-158 and src/shared/settings.rs:229-237 -------- */
+enum { DL_ILLUMINATION_DIRECT = 0, DL_ILLUMINATION_SINGLE_BOUNCE = 1 };
+enum { DL_TRANSPARENCY_DISABLED = 0, DL_TRANSPARENCY_ENABLED = 1, DL_TRANSPARENCY_ENABLED_MAX = 2 };
+/* BounceCompression; DL_BOUNCE_DEFAULT means "None: use BakeOptions::bounce_compression". */
+enum { DL_BOUNCE_BITS8 = 0, DL_BOUNCE_BITS6 = 1, DL_BOUNCE_BITS5 = 2, DL_BOUNCE_BITS4 = 3,
+       DL_BOUNCE_DEFAULT = -1 };
+/* LightShape (src/light/shape.rs:19-103): Point, Area(Rectangle|Tube), Sun, Sky. */
+enum { DL_SHAPE_POINT = 0, DL_SHAPE_RECTANGLE = 1, DL_SHAPE_TUBE = 2, DL_SHAPE_SUN = 3,
+       DL_SHAPE_SKY = 4 };
+/* DynamicGeometryMode: LightingOnly bakes no distance cubes, sun maps or sky probes. */
+enum { DL_DYNAMIC_LIGHTING_ONLY = 0, DL_DYNAMIC_ANGULAR = 1, DL_DYNAMIC_DISTANCE_CUBES = 2 };
+
+/* AlphaTexture (src/bake/input.rs:66-97). */
+typedef struct dl_alpha_texture {
+    uint32_t width, height;
+    const uint8_t* rgba8; /* width*height RGBA8 texels, rows top to bottom */
+    uint64_t rgba8_len;   /* bytes at rgba8; must be width*height*4 */
+    float cutoff;         /* Henry uses 0.5 */
+} dl_alpha_texture;
+
+/* BakeMesh (src/bake/input.rs:24-45), world space. */
+typedef struct dl_mesh {
+    uint64_t id;                    /* BakeId                                               */
+    uint64_t content_hash;          /* stored for stale detection                           */
+    uint32_t vertex_count;
+    const float* positions;         /* vertex_count * 3                                     */
+    const float* normals;           /* vertex_count * 3, or NULL for flat normals           */
+    uint32_t index_count;           /* multiple of 3                                        */
+    const uint32_t* indices;
+    const float* uv1;               /* vertex_count * 2                                     */
+    const float* uv0;               /* vertex_count * 2 or NULL                             */
+    const dl_alpha_texture* alpha;  /* NULL or an alpha texture (needs uv0)                 */
+    uint32_t occludes;              /* 0 = every ray passes through the mesh                */
+    float albedo[3];                /* mean linear reflectance, (1,1,1) by default          */
+    uint32_t uv1_count;             /* must equal vertex_count (validation mirrors input.rs) */
+    uint32_t uv0_count;
+    uint32_t normal_count;          /* 0 for flat normals                                   */
+} dl_mesh;
+
+/* BakeLight (src/bake/input.rs:101-132) with its LightShape. */
+typedef struct dl_light {
+    uint64_t id;
+    float position[3];
+    float radius;
+    int32_t illumination;   /* DL_ILLUMINATION_*                          */
+    int32_t transparency;   /* DL_TRANSPARENCY_*                          */
+    uint32_t bounce_samples; /* BakeBounce::samples (>= 2 used)            */
+    float bounce_intensity;
+    int32_t bounce_compression; /* DL_BOUNCE_* or DL_BOUNCE_DEFAULT        */
+    int32_t shape;           /* DL_SHAPE_*                                */
+    float rotation[4];       /* x, y, z, w: area light / sun rotation     */
+    float size[2];           /* area light width, height (tube: height 0) */
+    float angular_diameter;  /* sun, degrees                              */
+} dl_light;
+
+typedef struct dl_bake_input {
+    const dl_mesh* meshes;
+    uint32_t mesh_count;
+    const dl_light* lights;
+    uint32_t light_count;
+} dl_bake_input;
+
+/* BakeOptions (src/bake/input.rs:136-164). */
+typedef struct dl_bake_options {
+    float pixel_density;
+    uint32_t max_lightmap_size;
+    int32_t bounce_compression;     /* DL_BOUNCE_BITS* */
+    int32_t dynamic_geometry;       /* DL_DYNAMIC_*    */
+    uint32_t skip_bounce;
+    uint32_t photon_cube_resolution;
+    uint32_t area_light_samples;
+    uint32_t area_light_min_samples;
+    uint32_t sun_map_resolution;
+    uint32_t sky_samples;
+    float sky_probe_spacing;
+    uint32_t colored_bounce;
+    uint64_t seed;
+} dl_bake_options;
+
+/* BakeProgress (src/bake/progress.rs:38-112): written by the library, read by any thread.
+ * Setting `cancel` to non-zero makes the bake stop at the next check and return
+ * DL_E_CANCELLED. */
+typedef struct dl_progress {
+    volatile uint32_t stage; /* 0 Preparing, 1 PhotonCubes, 2 Meshes, 3 DistanceCubes, 4 Done */
+    volatile uint64_t done;
+    volatile uint64_t total;
+    volatile uint32_t cancel;
+} dl_progress;
+
+/* BakeStats (src/bake/format/types.rs:209-223). */
+typedef struct dl_bake_stats {
+    uint32_t meshes;
+    uint64_t triangles;
+    uint64_t texels;
+    uint64_t rays;
+    uint64_t lights_removed;
+    uint64_t bounce_rays;
+    uint64_t triangle_words;
+    /* Not in the reference's stats: texel-light visibility samples (texel_shadow returned
+     * Some), the unit of the throughput metric. */
+    uint64_t samples;
+} dl_bake_stats;
+
+/* LightBvhNode (src/bake/light_bvh.rs:17-25), 32 bytes as the shaders read it. */
+typedef struct dl_light_bvh_node {
+    float aabb_min[3];
+    uint32_t left_first;
+    float aabb_max[3];
+    uint32_t count;
+} dl_light_bvh_node;
+
+/* BakedLight (src/bake/format/types.rs). */
+typedef struct dl_baked_light {
+    uint64_t id;
+    float origin[3];
+    float radius;
+    int32_t shape;
+    float rotation[4];
+    float size[2];
+    float angular_diameter;
+    uint32_t shadow_bits_per_texel;
+    int32_t bounce_compression; /* DL_BOUNCE_* or -1 when the light bakes no bounce */
+} dl_baked_light;
+
+/* A view of BakedMesh (src/bake/format/types.rs): words stay owned by the scene. */
+typedef struct dl_baked_mesh_view {
+    uint64_t id;
+    uint64_t content_hash;
+    uint32_t resolution;
+    uint32_t occludes;
+    const uint32_t* triangles; /* Henry's dynamic_triangles words (triangle_words.rs:1-12) */
+    uint64_t triangle_word_count;
+} dl_baked_mesh_view;
+
+/* SunMap / SkyProbes views. */
+typedef struct dl_sun_map_view {
+    float rows[12];            /* three Vec4 rows, world to map space */
+    uint32_t resolution;
+    const uint32_t* texels;    /* unorm16 pairs */
+    uint64_t texel_word_count;
+} dl_sun_map_view;
+
+typedef struct dl_sky_probes_view {
+    uint32_t present;
+    float origin[3];
+    float spacing;
+    uint32_t counts[3];
+    const uint32_t* words;
+    uint64_t word_count;
+} dl_sky_probes_view;
+
+/* BakedSettings. */
+typedef struct dl_baked_settings {
+    uint32_t bounce;
+    uint32_t colored_bounce;
+    int32_t dynamic_geometry;
+    float pixel_density;
+    uint32_t max_lightmap_size;
+    uint32_t distance_cube_resolution;
+} dl_baked_settings;
+
+typedef struct dl_ctx dl_ctx;
+typedef struct dl_baked_scene dl_baked_scene;
+
+/* ---- context ------------------------------------------------------------------------------
+ * One context owns one HIP device, its streams and the resident lookup tables. Calls on one
+ * context are serialized internally; contexts are independent. */
+int dl_ctx_create(int device, dl_ctx** out);
+void dl_ctx_destroy(dl_ctx* ctx);
+/* The message of the last failed call on this context (thread-local when ctx is NULL). */
+const char* dl_last_error(const dl_ctx* ctx);
+/* BakeOptions::default() (src/bake/input.rs:166-171 with BakeSettings::default(),
+ * src/bake/settings.rs:67-81). */
+void dl_default_options(dl_bake_options* out);
+
+/* ---- the bake (replaces bake(), src/bake/baker.rs:27-203) --------------------------------- */
+int dl_bake(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options,
+            dl_progress* progress, dl_baked_scene** out);
+void dl_baked_scene_free(dl_baked_scene* scene);
+
+/* ---- BakedScene accessors (src/bake/format/types.rs:13-37) --------------------------------- */
+void dl_scene_settings(const dl_baked_scene* scene, dl_baked_settings* out);
+void dl_scene_stats(const dl_baked_scene* scene, dl_bake_stats* out);
+uint32_t dl_scene_light_count(const dl_baked_scene* scene);
+int dl_scene_light(const dl_baked_scene* scene, uint32_t index, dl_baked_light* out);
+uint32_t dl_scene_light_bvh(const dl_baked_scene* scene, const dl_light_bvh_node** nodes);
+uint32_t dl_scene_mesh_count(const dl_baked_scene* scene);
+int dl_scene_mesh(const dl_baked_scene* scene, uint32_t index, dl_baked_mesh_view* out);
+uint64_t dl_scene_distance_cubes(const dl_baked_scene* scene, const uint32_t** words);
+uint32_t dl_scene_sun_map_count(const dl_baked_scene* scene);
+int dl_scene_sun_map(const dl_baked_scene* scene, uint32_t index, dl_sun_map_view* out);
+void dl_scene_sky_probes(const dl_baked_scene* scene, dl_sky_probes_view* out);
+
+/* ---- timing of the last dl_bake on a context (milliseconds, device events) ----------------- */
+typedef struct dl_timings {
+    double total_ms;      /* wall time of dl_bake                                          */
+    double upload_ms;     /* host-to-device copies                                         */
+    double texels_ms;     /* owned/touched texel enumeration and burial                    */
+    double trace_ms;      /* shadow rays (the trace pass)                                  */
+    double bounce_ms;     /* photon cubes and gather                                       */
+    double post_ms;       /* optimize, seams, crop, dilate, serialize                      */
+    double maps_ms;       /* distance cubes, sun maps, sky probes                          */
+    double download_ms;   /* device-to-host copies                                         */
+    double trace_kernel_ms; /* summed device time of the shadow-ray kernel launches        */
+    uint64_t trace_launches;
+    uint64_t trace_rays;  /* rays cast by the shadow-ray kernel                             */
+    uint64_t bvh_nodes;   /* occluder BVH node count                                       */
+    uint64_t occluder_triangles;
+} dl_timings;
+int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYNLIGHT_H */

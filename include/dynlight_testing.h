@@ -1,0 +1,41 @@
+/*
+ * dynlight_testing.h -- diagnostic entry points of libdynlight_hip.so.
+ *
+ * No counterpart in the reference: these expose single device passes so tests can check them
+ * in isolation (ray queries against a scan of every occluder, f32 sin/cos against the host's
+ * libm).  They run on the context's device and stream and allocate their own scratch.
+ */
+#ifndef DYNLIGHT_TESTING_H
+#define DYNLIGHT_TESTING_H
+
+#include "dynlight.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Builds the occluder BVH over `tri_count` triangles (9 floats each) and answers `ray_count`
+ * rays given as 8 floats (origin, direction, max distance, unused).  closest = 0: hit_or_index[i]
+ * is 1 for a hit; closest = 1: t_out[i] is the nearest distance (-1 on a miss) and
+ * hit_or_index[i] the occluder index of the nearest hit (lowest index on ties). */
+int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, const float* rays8,
+                        uint64_t ray_count, int closest, float* t_out, uint32_t* hit_or_index);
+
+/* Evaluates the device f32 sin and cos at `n` inputs. */
+int dl_test_trig(dl_ctx* ctx, const float* x, uint64_t n, float* sin_out, float* cos_out);
+
+/* The same f32 sin/cos evaluated on the host from the tables this process builds (no device
+ * needed): the check that the tables reproduce the host's libm. */
+int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out);
+
+/* Number of exception entries of the sin and cos tables built for this context. */
+void dl_test_trig_table_sizes(const dl_ctx* ctx, uint64_t* sin_entries, uint64_t* cos_entries);
+
+/* Launch shape of the traversal kernels (0 keeps the current value): threads per block of the
+ * shadow-ray and burial kernels. */
+int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+"""Parity of the HIP bake with the oracle (bit-exact words, stats and distance cubes)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from dynlight_hip import abi, scenes
+from helpers import assert_scenes_equal
+
+pytestmark = pytest.mark.gpu
+
+F32P = C.POINTER(C.c_float)
+U32P = C.POINTER(C.c_uint32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(F32P)
+
+
+def test_trig_matches_host_libm(ctx):
+    """Device f32 sin/cos equal the host's libm on the bake's argument domain."""
+    rng = np.random.default_rng(7)
+    pos = rng.integers(0x37800000, 0x41000000, 4_000_000, dtype=np.uint32)
+    neg = rng.integers(0xB7800000, 0xBF4CCCCD, 2_000_000, dtype=np.uint32)
+    turn = np.arange(0, 1 << 24, 13, dtype=np.uint32).astype(np.float32) * np.float32(1.0 / 16777216.0)
+    burial = [((np.float32(i) + turn) * np.float32(6.2831855)).astype(np.float32) / np.float32(8.0) for i in range(8)]
+    x = np.ascontiguousarray(np.concatenate([pos.view(np.float32), neg.view(np.float32)] + burial).astype(np.float32))
+    ds, dc = np.zeros_like(x), np.zeros_like(x)
+    hs, hc = np.zeros_like(x), np.zeros_like(x)
+    assert ctx.lib.dl_test_trig(ctx.handle, _fp(x), x.size, _fp(ds), _fp(dc)) == 0, ctx.error()
+    assert ctx.lib.dl_test_trig_host(_fp(x), x.size, _fp(hs), _fp(hc)) == 0
+    assert np.array_equal(ds.view(np.uint32), hs.view(np.uint32))
+    assert np.array_equal(dc.view(np.uint32), hc.view(np.uint32))
+    ol = oracle.load()
+    pick = rng.integers(0, x.size, 20000)
+    gs = np.array([ol.dlo_sinf(float(v)) for v in x[pick]], dtype=np.float32)
+    gc = np.array([ol.dlo_cosf(float(v)) for v in x[pick]], dtype=np.float32)
+    assert np.array_equal(gs.view(np.uint32), ds[pick].view(np.uint32))
+    assert np.array_equal(gc.view(np.uint32), dc[pick].view(np.uint32))
+
+
+def _room_triangles():
+    inp, _ = scenes.box_room(1)
+    tris = []
+    for m in inp.meshes:
+        idx = np.asarray(m.indices).reshape(-1, 3)
+        tris.append(np.asarray(m.positions, np.float32)[idx].reshape(-1, 9))
+    return np.ascontiguousarray(np.concatenate(tris).astype(np.float32))
+
+
+@pytest.mark.parametrize("closest", [0, 1])
+def test_ray_queries_equal_a_scan_of_every_occluder(ctx, closest):
+    tris = _room_triangles()
+    rng = np.random.default_rng(3 + closest)
+    n = 200_000
+    o = rng.uniform([-5.5, -0.5, -5.5], [5.5, 4.5, 5.5], (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[: n // 10, 1] = 0.0  # axis-parallel components
+    d[n // 10: n // 5, 0] = -0.0
+    mx = rng.choice(np.array([np.inf, 3.0, 0.5, 12.0], np.float32), n)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 3:6], rays[:, 6] = o, d, mx
+    rays = np.ascontiguousarray(rays)
+    t_gpu = np.zeros(n, np.float32)
+    i_gpu = np.zeros(n, np.uint32)
+    st = ctx.lib.dl_test_ray_queries(ctx.handle, _fp(tris), tris.shape[0], _fp(rays), n, closest, _fp(t_gpu),
+                                     i_gpu.ctypes.data_as(U32P))
+    assert st == 0, ctx.error()
+    ol = oracle.load()
+    r7 = np.ascontiguousarray(rays[:, :7])
+    t_ref = np.zeros(n, np.float32)
+    i_ref = np.zeros(n, np.uint32)
+    ol.dlo_rays_batch(_fp(tris), tris.shape[0], _fp(r7), n, closest, 1, _fp(t_ref), i_ref.ctypes.data_as(U32P))
+    assert np.array_equal(i_gpu, i_ref)
+    if closest:
+        assert np.array_equal(t_gpu.view(np.uint32), t_ref.view(np.uint32))
+
+
+def test_box_room_bake_matches_oracle(ctx):
+    inp, opts = scenes.box_room(1)
+    got = ctx.bake(inp, opts)
+    want = oracle.bake(inp, opts, threads=0)
+    assert_scenes_equal(got, want)
+    assert got.stats["samples"] > 100_000
