@@ -1,0 +1,120 @@
+"""Benchmark: texel-light visibility samples/s of the MI355X bake on the procedural interior.
+
+One step is one full bake (dl_bake: BVH build, association, texels, burial, shadow rays,
+post-passes, serialized words, distance cubes) of the interior scene: an 8x8 grid of 6x3x6 m
+rooms with door openings, seeded boxes and pillars to 200k triangles, 256 point/spot lights of
+range 4-12 m, 32 texels/m, 1-bit shadows.  A sample is one (texel, light) pair for which the
+reference's texel_shadow returns Some (src/bake/mesh_bake.rs:241-247).
+
+Prints one JSON line.  `roofline` prices the dominant kernel (the shadow-ray kernel) at its
+algorithmic bytes: per sample, 128 B per BVH node fetched plus 48 B per triangle tested plus the
+mask bits, with node and triangle counts from an instrumented (untimed) bake.  `cpu_baseline`
+times the CPU oracle on a 1/64 receiver sample of the same scene (every mesh still occludes).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "dynamiclighting-bevy_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(inp, opts, modulus=64):
+    import numpy as np
+
+    import oracle
+    from dynlight_hip import abi, scenes
+
+    mask = scenes.mesh_sample_mask(inp.mesh_array["id"], modulus)
+    o = abi.dl_bake_options.from_buffer_copy(opts)
+    o.dynamic_geometry = abi.DYNAMIC_LIGHTING_ONLY
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    s = oracle.bake_receivers(inp, o, mask, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": s.stats["samples"] / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle, receivers splitmix64(id) % {modulus} == 0 ({int(np.sum(mask))} of "
+                      f"{len(mask)} meshes) against all 200k occluders and 256 lights, no distance cubes; "
+                      f"{s.stats['samples']} samples in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the bake shards receiver meshes across GPUs only by design, see DESIGN.md)", flush=True)
+        sys.exit(2)
+
+    import dynlight_hip
+    from dynlight_hip import scenes
+
+    inp, opts = scenes.interior(2)
+    opts.skip_bounce = 1
+    ctx = dynlight_hip.Context(0)
+    for _ in range(args.warmup):
+        ctx.bake(inp, opts)
+    # instrumented bake: node and triangle counts of the shadow-ray kernel (untimed)
+    ctx.set_counting(True)
+    scene = ctx.bake(inp, opts)
+    counts = ctx.timings()
+    ctx.set_counting(False)
+    samples = scene.stats["samples"]
+    t_total, trace_ms = 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = ctx.bake(inp, opts)
+        tm = ctx.timings()
+        trace_ms += tm["trace_kernel_ms"]
+        assert s.stats["samples"] == samples
+    t_total = time.perf_counter() - t0
+    ms_per_step = 1000.0 * t_total / args.steps
+    value = samples * args.steps / t_total
+    nodes, tris = counts["trace_node_visits"], counts["trace_tri_tests"]
+    b_alg = 128.0 * nodes + 48.0 * tris + samples / 8.0
+    kernel_s = trace_ms / args.steps / 1000.0
+    achieved = b_alg / kernel_s / 1e9
+    out = {
+        "metric": "texel-light visibility samples/s (1 GPU)",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "interior: 8x8 rooms 6x3x6 m with doors, 200k triangles, 256 point/spot "
+                               "lights r 4-12 m, 32 texels/m, 1-bit shadows, bounce skipped",
+                   "meshes": inp.mesh_count, "triangles": inp.triangle_count,
+                   "texels": scene.stats["texels"], "samples_per_step": samples,
+                   "rays_per_step": scene.stats["rays"]},
+        "trace_kernel_ms": trace_ms / args.steps,
+        "trace_samples_per_s": samples / kernel_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": b_alg, "node_fetches": nodes, "triangle_tests": tris},
+        "stages_ms": {k: counts[k] for k in ("upload_ms", "texels_ms", "trace_ms", "post_ms", "maps_ms",
+                                             "download_ms", "total_ms")},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(inp, opts)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

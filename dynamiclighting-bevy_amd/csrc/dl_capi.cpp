@@ -19,14 +19,16 @@ int set_err(dl_ctx* ctx, int code, const std::string& msg) {
 int upload_trig(dl_ctx* ctx) {
   using namespace dl;
   const TrigHost& t = ctx->trig_host;
-  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() + 4;
+  // six arrays, each followed by one spare word
+  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() + 6;
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
   size_t off = 0;
+  hipError_t copy = hipSuccess;
   auto put = [&](const std::vector<uint32_t>& v) -> const uint32_t* {
     const uint32_t* p = base + off;
-    if (!v.empty()) (void)hipMemcpy(base + off, v.data(), v.size() * 4, hipMemcpyHostToDevice);
+    if (!v.empty() && copy == hipSuccess) copy = hipMemcpy(base + off, v.data(), v.size() * 4, hipMemcpyHostToDevice);
     off += v.size() + 1;
     return p;
   };
@@ -36,6 +38,7 @@ int upload_trig(dl_ctx* ctx) {
   ctx->trig.cos.bucket = put(t.cos_bucket);
   ctx->trig.cos.keys = put(t.cos_keys);
   ctx->trig.cos.vals = put(t.cos_vals);
+  if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
 }
 }  // namespace
@@ -183,6 +186,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   sc.ntris = tri_count;
   sc.stack = 3 * std::max(1u, bvh.depth) + 1;
   sc.overflow = ovf.p;
+  sc.counters = nullptr;
   if ((e = launch_ray_queries(ctx->stream, sc, r.p, ray_count, closest, tout.p, idx.p))) return fail(e);
   if ((e = hipStreamSynchronize(ctx->stream))) return fail(e);
   if (ray_count) {
@@ -232,6 +236,13 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
 void dl_test_trig_table_sizes(const dl_ctx* ctx, uint64_t* s, uint64_t* c) {
   *s = ctx ? ctx->trig_host.sin_keys.size() : 0;
   *c = ctx ? ctx->trig_host.cos_keys.size() : 0;
+}
+
+int dl_set_counting(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->counting = enable != 0;
+  return DL_OK;
 }
 
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {

@@ -486,6 +486,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   sc.ntris = (uint32_t)(occ9.size() / 9);
   sc.stack = 3 * std::max(1u, bvh.depth) + 1;
   sc.overflow = d_overflow.p;
+  sc.counters = nullptr;
   DBuf<DevLight> d_lights;
   HIP_TRY(upload(ctx, d_lights, hlights));
   DBuf<float> d_v, d_n, d_uv, d_face;
@@ -694,7 +695,13 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(hipEventRecord(ev0, s));
-  HIP_TRY(launch_trace(s, sc, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nt], tp, ctx->trig,
+  DBuf<unsigned long long> d_counters;
+  DevScene sct = sc;
+  if (ctx->counting) {
+    HIP_TRY(zeroed(ctx, d_counters, 2));
+    sct.counters = d_counters.p;
+  }
+  HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nt], tp, ctx->trig,
                        ctx->trace_block));
   HIP_TRY(hipEventRecord(ev1, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -703,6 +710,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   ctx->timings.trace_kernel_ms = kms;
+  if (ctx->counting) {
+    unsigned long long c[2] = {0, 0};
+    HIP_TRY(hipMemcpy(c, d_counters.p, 16, hipMemcpyDeviceToHost));
+    ctx->timings.trace_node_visits = c[0];
+    ctx->timings.trace_tri_tests = c[1];
+  }
   ctx->timings.trace_launches = 1;
   ctx->timings.trace_ms = ms_since(t_trace);
   if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }

@@ -65,6 +65,7 @@ struct dl_ctx {
   uint32_t trace_block = 256;
   uint32_t burial_block = 128;
   unsigned host_threads = 1;
+  bool counting = false;
 };
 
 struct dl_baked_mesh_out {

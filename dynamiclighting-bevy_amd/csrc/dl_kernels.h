@@ -40,6 +40,7 @@ struct DevScene {
   uint32_t ntris;
   uint32_t stack;  // LDS stack entries per lane
   uint32_t* overflow;
+  unsigned long long* counters;  // null, or [node fetches, triangle tests] for instrumented runs
 };
 
 // Receiver triangles of the meshes being baked (SoA, one entry per triangle).

@@ -46,7 +46,9 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
   float bt = tmax;
   uint32_t sp = 0;
   int32_t node = 0;
+  uint32_t nvisit = 0, ntest = 0;
   for (;;) {
+    nvisit++;
     const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
     const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
     const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
@@ -78,9 +80,16 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
         for (uint32_t i = first; i < first + count; ++i) {
           const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
           float t, u, vv;
+          ntest++;
           if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
           if (!finite_(t)) continue;
-          if (ANY) return true;
+          if (ANY) {
+            if (sc.counters) {
+              atomicAdd(&sc.counters[0], (unsigned long long)nvisit);
+              atomicAdd(&sc.counters[1], (unsigned long long)ntest);
+            }
+            return true;
+          }
           const uint32_t occ = __float_as_uint(a.w);
           if (!have || t < bt || (t == bt && occ < hit->occ)) {
             have = true;
@@ -113,6 +122,10 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
     if (sp == 0) break;
     sp--;
     node = (int32_t)stack[sp * stride];
+  }
+  if (sc.counters) {
+    atomicAdd(&sc.counters[0], (unsigned long long)nvisit);
+    atomicAdd(&sc.counters[1], (unsigned long long)ntest);
   }
   return have;
 }

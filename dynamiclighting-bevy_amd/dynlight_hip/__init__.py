@@ -64,6 +64,8 @@ def load() -> C.CDLL:
     lib.dl_test_trig_host.argtypes = [f32p, C.c_uint64, f32p, f32p]
     lib.dl_test_trig_table_sizes.restype = None
     lib.dl_test_trig_table_sizes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.dl_set_counting.restype = C.c_int
+    lib.dl_set_counting.argtypes = [P, C.c_int]
     lib.dl_set_launch.restype = C.c_int
     lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
     _lib = lib
@@ -110,6 +112,10 @@ class Context:
         t = abi.dl_timings()
         self.lib.dl_last_timings(self.handle, C.byref(t))
         return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def set_counting(self, enable: bool) -> None:
+        """Counts BVH node fetches and triangle tests of the shadow-ray kernel in later bakes."""
+        self.lib.dl_set_counting(self.handle, 1 if enable else 0)
 
     def set_launch(self, trace_block: int = 0, burial_block: int = 0) -> None:
         st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)

@@ -243,6 +243,8 @@ typedef struct dl_timings {
     uint64_t trace_rays;  /* rays cast by the shadow-ray kernel                             */
     uint64_t bvh_nodes;   /* occluder BVH node count                                       */
     uint64_t occluder_triangles;
+    uint64_t trace_node_visits; /* BVH nodes fetched by the shadow-ray kernel (counting on)  */
+    uint64_t trace_tri_tests;   /* triangles tested by the shadow-ray kernel (counting on)  */
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 

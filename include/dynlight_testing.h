@@ -31,6 +31,10 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
 /* Number of exception entries of the sin and cos tables built for this context. */
 void dl_test_trig_table_sizes(const dl_ctx* ctx, uint64_t* sin_entries, uint64_t* cos_entries);
 
+/* Counting mode: when enabled, later bakes on this context count the BVH nodes fetched and the
+ * triangles tested by the shadow-ray kernel (dl_timings.trace_node_visits / trace_tri_tests). */
+int dl_set_counting(dl_ctx* ctx, int enable);
+
 /* Launch shape of the traversal kernels (0 keeps the current value): threads per block of the
  * shadow-ray and burial kernels. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);

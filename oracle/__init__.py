@@ -39,6 +39,9 @@ def load() -> C.CDLL:
     lib.dlo_bake.argtypes = [C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
                              C.c_uint32, C.POINTER(P)]
     lib.dlo_last_error.restype = C.c_char_p
+    lib.dlo_bake_receivers.restype = C.c_int
+    lib.dlo_bake_receivers.argtypes = [C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
+                                       C.POINTER(C.c_uint8), C.POINTER(P)]
     abi.bind_scene_api(lib, "dlo_")
     f32p = C.POINTER(C.c_float)
     u32p = C.POINTER(C.c_uint32)
@@ -84,6 +87,22 @@ def load() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def bake_receivers(inp: "abi.BakeInput", options: "abi.dl_bake_options", receivers, threads: int = 0):
+    """Bakes only the meshes where `receivers` is true; every mesh still occludes."""
+    import numpy as np
+    lib = load()
+    mask = np.ascontiguousarray(np.asarray(receivers, dtype=np.uint8))
+    h = C.c_void_p()
+    st = lib.dlo_bake_receivers(C.byref(inp.struct), C.byref(options), threads,
+                                mask.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(h))
+    if st != 0:
+        raise abi.BakeError(st, lib.dlo_last_error().decode())
+    try:
+        return abi.read_scene(lib, "dlo_", h)
+    finally:
+        lib.dlo_scene_free(h)
 
 
 def bake(inp: "abi.BakeInput", options: "abi.dl_bake_options", threads: int = 0,

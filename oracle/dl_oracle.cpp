@@ -2047,8 +2047,21 @@ extern "C" {
 
 const char* dlo_last_error(void) { return g_err.c_str(); }
 
+static int bake_impl(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads, uint32_t brute_force,
+                     const uint8_t* receivers, dlo_scene** out);
+
 int dlo_bake(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads, uint32_t brute_force,
              dlo_scene** out) {
+  return bake_impl(input, options, threads, brute_force, nullptr, out);
+}
+
+int dlo_bake_receivers(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads,
+                       const uint8_t* receivers, dlo_scene** out) {
+  return bake_impl(input, options, threads, 0, receivers, out);
+}
+
+static int bake_impl(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads, uint32_t brute_force,
+                     const uint8_t* receivers, dlo_scene** out) {
   using namespace o;
   if (!input || !options || !out) return fail(DL_E_ARGUMENT, "null argument");
   *out = nullptr;
@@ -2115,7 +2128,8 @@ int dlo_bake(const dl_bake_input* input, const dl_bake_options* options, uint32_
     build_photon_cube(*sc, L, opt.photon_cube_resolution, *sc->cubes[l]);
   }
   std::vector<Prepared> prepared;
-  for (auto* m : meshes) prepared.push_back(prepare(m, opt));
+  for (auto* m : meshes)
+    if (!receivers || receivers[m - input->meshes]) prepared.push_back(prepare(m, opt));
   std::unique_ptr<dlo_scene> res(new dlo_scene());
   res->meshes.resize(prepared.size());
   if (prepared.size() >= g_threads) {

@@ -24,6 +24,10 @@ typedef struct dlo_scene dlo_scene;
  * scan every occluder per ray instead of using the oracle's BVH. */
 int dlo_bake(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads,
              uint32_t brute_force, dlo_scene** out);
+/* The bench's CPU-baseline sample: bakes only the meshes with receivers[i] != 0 (indexed like
+ * input->meshes) while every mesh still occludes.  Not a reference entry point. */
+int dlo_bake_receivers(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads,
+                       const uint8_t* receivers, dlo_scene** out);
 const char* dlo_last_error(void);
 void dlo_scene_free(dlo_scene* scene);
 void dlo_scene_settings(const dlo_scene* scene, dl_baked_settings* out);
