@@ -1,0 +1,814 @@
+// Orchestration of one bake (baker.rs:27-203).
+//
+// Scene-level passes run once: validation and ordering on the host, the occluder BVH (host
+// build, uploaded once), photon cubes of every bounce light, and light association of every
+// receiver triangle.  Receiver meshes are then processed in batches sized to an HBM budget:
+//
+//   owned texels -> traced texels -> burial -> compaction -> shadow rays -> bounce gather,
+//   dilate, blur -> optimize -> seam padding -> crop/dilate/pack + serialization
+//
+// Host work between passes is O(triangles + light entries) integer bookkeeping (regions,
+// windows and word offsets); every ray and texel is processed on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "dl_host.h"
+
+using namespace dl;
+
+namespace {
+
+struct Batch {
+  uint32_t m0, m1;  // meshes [m0, m1)
+};
+
+// Gaussian weights of the bounce blur (bounce_map.rs:140-150): 7 taps, sigma 5, glibc expf.
+BlurWeights blur_weights() {
+  BlurWeights b;
+  const float sigma2 = 5.0f * 5.0f;
+  float sum = 0.0f;
+  for (int i = -3; i <= 3; ++i) {
+    b.w[i + 3] = (1.0f / sqrtf((2.0f * kPi) * sigma2)) * expf(-(float)(i * i) / (2.0f * sigma2));
+    sum += b.w[i + 3];
+  }
+  float s2 = 0.0f;
+  for (int i = 0; i < 7; ++i) {
+    b.w[i] = b.w[i] / sum;
+    s2 += b.w[i];
+  }
+  b.sum = s2;
+  return b;
+}
+
+uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOUNCE_BITS6 ? 6u : c == DL_BOUNCE_BITS5 ? 5u : 4u; }
+
+}  // namespace
+
+int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
+                 dl_baked_scene** out) {
+  auto t_start = std::chrono::steady_clock::now();
+  ctx->timings = dl_timings{};
+  if (!input || !options || !out) { ctx->err = "null argument"; return DL_E_ARGUMENT; }
+  *out = nullptr;
+  const dl_bake_options& opt = *options;
+  auto cancelled = [&]() { return progress && progress->cancel; };
+  auto stage = [&](uint32_t s, uint64_t total) {
+    if (progress) { progress->stage = s; progress->done = 0; progress->total = total; }
+  };
+  auto advance = [&](uint64_t n) {
+    if (progress) progress->done = progress->done + n;
+  };
+  stage(0, 0);
+
+  // ---- validation and ordering (baker.rs:31-50, 205-233) -------------------------------------
+  int st = check_options(ctx, opt);
+  if (st) return st;
+  if (input->mesh_count && !input->meshes) { ctx->err = "null mesh array"; return DL_E_ARGUMENT; }
+  if (input->light_count && !input->lights) { ctx->err = "null light array"; return DL_E_ARGUMENT; }
+  for (uint32_t i = 0; i < input->light_count; ++i)
+    if (input->lights[i].shape < 0 || input->lights[i].shape > 4) { ctx->err = "unknown light shape"; return DL_E_ARGUMENT; }
+  std::vector<const dl_mesh*> meshes;
+  for (uint32_t i = 0; i < input->mesh_count; ++i) meshes.push_back(&input->meshes[i]);
+  std::stable_sort(meshes.begin(), meshes.end(), [](const dl_mesh* a, const dl_mesh* b) { return a->id < b->id; });
+  for (size_t i = 1; i < meshes.size(); ++i)
+    if (meshes[i - 1]->id == meshes[i]->id) { ctx->err = "two meshes share BakeId " + hex_id(meshes[i]->id); return DL_E_DUPLICATE_ID; }
+  for (auto* m : meshes)
+    if ((st = check_mesh(ctx, *m))) return st;
+  std::vector<const dl_light*> lights_sorted;
+  for (uint32_t i = 0; i < input->light_count; ++i) lights_sorted.push_back(&input->lights[i]);
+  std::stable_sort(lights_sorted.begin(), lights_sorted.end(), [](const dl_light* a, const dl_light* b) { return a->id < b->id; });
+  for (size_t i = 1; i < lights_sorted.size(); ++i)
+    if (lights_sorted[i - 1]->id == lights_sorted[i]->id) { ctx->err = "two lights share BakeId " + hex_id(lights_sorted[i]->id); return DL_E_DUPLICATE_ID; }
+  for (auto* l : lights_sorted)
+    if ((st = check_light(ctx, *l))) return st;
+
+  std::vector<const dl_light*> local, suns, skies;
+  for (auto* l : lights_sorted) {
+    if (l->shape == DL_SHAPE_SUN) suns.push_back(l);
+    else if (l->shape == DL_SHAPE_SKY) skies.push_back(l);
+    else local.push_back(l);
+  }
+  std::vector<f3> lpos;
+  std::vector<float> lrad;
+  for (auto* l : local) { lpos.push_back(mk3(l->position[0], l->position[1], l->position[2])); lrad.push_back(l->radius); }
+  LightBvhOut lbvh;
+  build_light_bvh(lpos, lrad, lbvh);
+  std::vector<const dl_light*> lights;
+  for (uint32_t i : lbvh.order) lights.push_back(local[i]);
+  for (auto* l : suns) lights.push_back(l);
+  for (auto* l : skies) lights.push_back(l);
+  const uint32_t nl = (uint32_t)lights.size(), nlocal = (uint32_t)local.size();
+  std::vector<DevLight> hlights;
+  std::vector<int32_t> comp_of_slot(nl + 1, 0), cube_of_slot(nl + 1, -1);
+  std::vector<uint32_t> cube_slot;
+  bool bounce_in_scene = false, dithered_bounce = false;
+  for (uint32_t i = 0; i < nl; ++i) {
+    const dl_light* l = lights[i];
+    hlights.push_back(make_dev_light(*l, opt));
+    bool bb = hlights.back().bakes_bounce != 0;
+    bounce_in_scene |= bb;
+    comp_of_slot[i] = l->bounce_compression >= 0 ? l->bounce_compression : opt.bounce_compression;
+    if (bb) {
+      dithered_bounce |= comp_of_slot[i] != DL_BOUNCE_BITS8;
+      if (l->shape != DL_SHAPE_SUN) {
+        cube_of_slot[i] = (int32_t)cube_slot.size();
+        cube_slot.push_back(i);
+      }
+    }
+  }
+  for (auto* m : meshes)
+    if (m->alpha && m->occludes)
+      for (auto* l : lights)
+        if (l->transparency != DL_TRANSPARENCY_DISABLED) {
+          ctx->err = "alpha-tested shadows are not supported by this build";
+          return DL_E_ARGUMENT;
+        }
+
+  // ---- occluders (occluder.rs:80-115) with their normals and albedo --------------------------
+  std::vector<float> occ9, occ_nrm, occ_alb;
+  std::vector<uint32_t> occ_flags;
+  for (auto* m : meshes) {
+    if (!m->occludes) continue;
+    bool has_n = m->normals && m->normal_count;
+    for (uint32_t t = 0; t < m->index_count / 3; ++t) {
+      f3 v[3];
+      for (int k = 0; k < 3; ++k) {
+        const float* p = m->positions + 3 * (size_t)m->indices[t * 3 + k];
+        v[k] = mk3(p[0], p[1], p[2]);
+      }
+      f3 n = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+      if (!(len_sq(n) > 0.0f && finite_(n.x) && finite_(n.y) && finite_(n.z))) continue;
+      for (int k = 0; k < 3; ++k) {
+        occ9.push_back(v[k].x); occ9.push_back(v[k].y); occ9.push_back(v[k].z);
+        for (int a = 0; a < 3; ++a) occ_nrm.push_back(has_n ? m->normals[3 * (size_t)m->indices[t * 3 + k] + a] : 0.0f);
+      }
+      occ_flags.push_back(has_n ? 1u : 0u);
+      for (int a = 0; a < 3; ++a) occ_alb.push_back(m->albedo[a]);
+    }
+  }
+  const uint32_t nocc = (uint32_t)occ_flags.size();
+  Bvh4 bvh;
+  build_bvh4(occ9.data(), nocc, ctx->host_threads, bvh);
+  ctx->timings.bvh_nodes = bvh.nodes.size();
+  ctx->timings.occluder_triangles = nocc;
+
+  // ---- prepared meshes and receiver triangles (mesh_builder.rs:21-80) -----------------------
+  const uint32_t nm = (uint32_t)meshes.size();
+  std::vector<uint32_t> res(nm), first_tri(nm + 1, 0);
+  std::vector<uint64_t> mid(nm), owned_off(nm + 1, 0);
+  uint64_t ntri = 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    const dl_mesh* m = meshes[i];
+    float area = -0.0f;
+    for (uint32_t t = 0; t < m->index_count / 3; ++t) {
+      const float* a = m->positions + 3 * (size_t)m->indices[t * 3];
+      const float* b = m->positions + 3 * (size_t)m->indices[t * 3 + 1];
+      const float* c = m->positions + 3 * (size_t)m->indices[t * 3 + 2];
+      area = area + tri_area(mk3(a[0], a[1], a[2]), mk3(b[0], b[1], b[2]), mk3(c[0], c[1], c[2]));
+    }
+    res[i] = lightmap_size(area, opt.pixel_density, opt.max_lightmap_size);
+    if (res[i] > 65535) { ctx->err = "mesh " + hex_id(m->id) + ": shadow map side above 65535 texels"; return DL_E_ARGUMENT; }
+    mid[i] = m->id;
+    first_tri[i] = (uint32_t)ntri;
+    ntri += m->index_count / 3;
+    owned_off[i + 1] = owned_off[i] + ((uint64_t)res[i] * res[i] + 31) / 32;
+  }
+  first_tri[nm] = (uint32_t)ntri;
+  if (ntri >= (1ull << 31)) { ctx->err = "too many triangles"; return DL_E_ARGUMENT; }
+  const uint32_t nt = (uint32_t)ntri;
+  std::vector<float> hv((size_t)nt * 9), hn((size_t)nt * 9), huv((size_t)nt * 6);
+  std::vector<uint32_t> hmesh(nt), hlocal(nt);
+  for (uint32_t i = 0; i < nm; ++i) {
+    const dl_mesh* m = meshes[i];
+    bool flat = !m->normals || m->normal_count == 0;
+    for (uint32_t t = 0; t < m->index_count / 3; ++t) {
+      uint32_t g = first_tri[i] + t;
+      hmesh[g] = i;
+      hlocal[g] = t;
+      f3 v[3];
+      for (int k = 0; k < 3; ++k) {
+        uint32_t c = m->indices[t * 3 + k];
+        const float* p = m->positions + 3 * (size_t)c;
+        v[k] = mk3(p[0], p[1], p[2]);
+        for (int a = 0; a < 3; ++a) hv[(size_t)g * 9 + 3 * k + a] = p[a];
+        float u = m->uv1[2 * (size_t)c], w = m->uv1[2 * (size_t)c + 1];
+        if (!(finite_(u) && finite_(w))) { u = 0.0f; w = 0.0f; }
+        huv[(size_t)g * 6 + 2 * k] = u;
+        huv[(size_t)g * 6 + 2 * k + 1] = w;
+        if (!flat)
+          for (int a = 0; a < 3; ++a) hn[(size_t)g * 9 + 3 * k + a] = m->normals[3 * (size_t)c + a];
+      }
+      if (flat) {
+        f3 f = normalize_or_zero(cross(sub(v[1], v[0]), sub(v[2], v[0])));
+        for (int k = 0; k < 3; ++k) {
+          hn[(size_t)g * 9 + 3 * k] = f.x;
+          hn[(size_t)g * 9 + 3 * k + 1] = f.y;
+          hn[(size_t)g * 9 + 3 * k + 2] = f.z;
+        }
+      }
+    }
+  }
+  if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
+
+  // ---- scene-level device state --------------------------------------------------------------
+  auto t_up = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
+  DBuf<Node4> d_nodes;
+  DBuf<float> d_occ, d_occ_nrm, d_occ_alb;
+  DBuf<uint32_t> d_overflow, d_occ_flags;
+  HIP_TRY(upload(ctx, d_nodes, bvh.nodes));
+  HIP_TRY(upload(ctx, d_occ, bvh.tris));
+  HIP_TRY(zeroed(ctx, d_overflow, 1));
+  DevScene sc;
+  sc.nodes = d_nodes.p;
+  sc.tris = reinterpret_cast<const float4*>(d_occ.p);
+  sc.ntris = nocc;
+  sc.stack = 3 * std::max(1u, bvh.depth) + 1;
+  sc.overflow = d_overflow.p;
+  sc.counters = nullptr;
+  DevOcc occ{nullptr, nullptr, nullptr};
+  if (bounce_in_scene) {
+    HIP_TRY(upload(ctx, d_occ_nrm, occ_nrm));
+    HIP_TRY(upload(ctx, d_occ_alb, occ_alb));
+    HIP_TRY(upload(ctx, d_occ_flags, occ_flags));
+    occ = DevOcc{d_occ_nrm.p, d_occ_alb.p, d_occ_flags.p};
+  }
+  DBuf<DevLight> d_lights;
+  HIP_TRY(upload(ctx, d_lights, hlights));
+  DBuf<int32_t> d_comp, d_cube_of;
+  HIP_TRY(upload(ctx, d_comp, comp_of_slot));
+  HIP_TRY(upload(ctx, d_cube_of, cube_of_slot));
+  DBuf<float> d_v, d_n, d_uv, d_face;
+  DBuf<uint32_t> d_mesh, d_local, d_crop;
+  HIP_TRY(upload(ctx, d_v, hv));
+  HIP_TRY(upload(ctx, d_n, hn));
+  HIP_TRY(upload(ctx, d_uv, huv));
+  HIP_TRY(upload(ctx, d_mesh, hmesh));
+  HIP_TRY(upload(ctx, d_local, hlocal));
+  HIP_TRY(d_face.alloc((size_t)nt * 3 + 3));
+  HIP_TRY(d_crop.alloc((size_t)nt * 4 + 4));
+  DevTris trall;
+  trall.n = nt;
+  trall.v = d_v.p;
+  trall.nrm = d_n.p;
+  trall.uv = d_uv.p;
+  trall.mesh = d_mesh.p;
+  trall.local = d_local.p;
+  trall.face = d_face.p;
+  trall.crop = d_crop.p;
+  DBuf<uint32_t> d_res, d_owned, d_visited;
+  DBuf<uint64_t> d_mid, d_owned_off;
+  DBuf<unsigned long long> d_mrays, d_msamples, d_mbrays;
+  HIP_TRY(upload(ctx, d_res, res));
+  HIP_TRY(upload(ctx, d_mid, mid));
+  HIP_TRY(upload(ctx, d_owned_off, owned_off));
+  HIP_TRY(zeroed(ctx, d_owned, owned_off[nm] + 1));
+  HIP_TRY(zeroed(ctx, d_visited, owned_off[nm] + 1));
+  HIP_TRY(zeroed(ctx, d_mrays, nm + 1));
+  HIP_TRY(zeroed(ctx, d_msamples, nm + 1));
+  HIP_TRY(zeroed(ctx, d_mbrays, nm + 1));
+  DevMeshes ms;
+  ms.n = nm;
+  ms.res = d_res.p;
+  ms.id = d_mid.p;
+  ms.owned_off = d_owned_off.p;
+  ms.visited_off = d_owned_off.p;
+  ms.owned = d_owned.p;
+  ms.visited = d_visited.p;
+  ms.rays = d_mrays.p;
+  ms.samples = d_msamples.p;
+  HIP_TRY(hipStreamSynchronize(s));
+  ctx->timings.upload_ms = ms_since(t_up);
+
+  // ---- photon cubes (bounce.rs:126-151) ------------------------------------------------------
+  auto t_bounce0 = std::chrono::steady_clock::now();
+  stage(1, cube_slot.size());
+  const uint32_t cres = opt.photon_cube_resolution;
+  DBuf<uint32_t> d_cube_slot, d_photons;
+  if (!cube_slot.empty()) {
+    uint64_t photons = 6ull * cres * cres * cube_slot.size();
+    HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
+    HIP_TRY(d_photons.alloc(photons * 3));
+    HIP_TRY(launch_photons(s, sc, occ, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, 128));
+    HIP_TRY(hipStreamSynchronize(s));
+    advance(cube_slot.size());
+  }
+  double bounce_ms = ms_since(t_bounce0);
+  if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
+
+  // ---- association and rectangles of every receiver triangle (mesh_bake.rs:141-173) ------------
+  auto t_tex = std::chrono::steady_clock::now();
+  stage(2, nm);
+  DBuf<uint64_t> d_lc, d_tc, d_tcc, d_ent_start;
+  HIP_TRY(zeroed(ctx, d_lc, (size_t)nt + 1));
+  HIP_TRY(zeroed(ctx, d_tc, (size_t)nt + 1));
+  HIP_TRY(zeroed(ctx, d_tcc, (size_t)nt + 1));
+  HIP_TRY(d_ent_start.alloc((size_t)nt + 1));
+  HIP_TRY(launch_setup(s, trall, ms, d_lights.p, nl, nlocal, d_lc.p, d_tc.p, d_tcc.p));
+  HIP_TRY(scan_u64(s, d_lc.p, d_ent_start.p, nt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+  std::vector<uint64_t> h_ent_start, h_tc, h_tcc;
+  HIP_TRY(download(ctx, h_ent_start, d_ent_start.p, (size_t)nt + 1));
+  HIP_TRY(download(ctx, h_tc, d_tc.p, (size_t)nt));
+  HIP_TRY(download(ctx, h_tcc, d_tcc.p, (size_t)nt));
+  const uint64_t nent = h_ent_start[nt];
+  DBuf<uint32_t> d_ent_light, d_ent_tri;
+  HIP_TRY(d_ent_light.alloc(nent + 1));
+  HIP_TRY(d_ent_tri.alloc(nent + 1));
+  HIP_TRY(launch_assoc_fill(s, trall, ms, d_lights.p, nl, nlocal, d_ent_start.p, d_ent_light.p, d_ent_tri.p));
+  std::vector<uint32_t> h_ent_light, h_crop;
+  HIP_TRY(download(ctx, h_ent_light, d_ent_light.p, nent));
+  HIP_TRY(download(ctx, h_crop, d_crop.p, (size_t)nt * 4));
+  d_ent_tri.release();
+
+  // ---- regions (mesh_bake.rs:175-192) and bounce windows (bounce_map.rs:30-43) -----------------
+  std::vector<uint32_t> reg_rect, reg_bits, reg_mesh, reg_light, h_ent_region(nent);
+  std::vector<uint64_t> reg_words;
+  std::vector<uint32_t> mesh_reg_first(nm + 1, 0);
+  std::vector<int64_t> win_of_reg;
+  std::vector<uint32_t> win_rect, win_res;
+  std::vector<uint64_t> win_area;
+  std::vector<uint32_t> mesh_win_first(nm + 1, 0);
+  {
+    std::vector<int32_t> slot_of(nl, -1);
+    std::vector<uint32_t> rect(4 * (size_t)nl + 4);
+    std::vector<uint32_t> touched;
+    for (uint32_t i = 0; i < nm; ++i) {
+      mesh_reg_first[i] = (uint32_t)reg_bits.size();
+      mesh_win_first[i] = (uint32_t)win_area.size();
+      touched.clear();
+      for (uint32_t t = first_tri[i]; t < first_tri[i + 1]; ++t)
+        for (uint64_t e = h_ent_start[t]; e < h_ent_start[t + 1]; ++e) {
+          uint32_t l = h_ent_light[e];
+          const uint32_t* c = &h_crop[4 * (size_t)t];
+          if (slot_of[l] < 0) {
+            slot_of[l] = 0;
+            touched.push_back(l);
+            std::memcpy(&rect[4 * (size_t)l], c, 16);
+          } else {
+            uint32_t* r = &rect[4 * (size_t)l];
+            r[0] = std::min(r[0], c[0]); r[1] = std::min(r[1], c[1]);
+            r[2] = std::max(r[2], c[2]); r[3] = std::max(r[3], c[3]);
+          }
+        }
+      std::sort(touched.begin(), touched.end());
+      for (uint32_t l : touched) {
+        slot_of[l] = (int32_t)reg_bits.size();
+        const uint32_t* r = &rect[4 * (size_t)l];
+        reg_rect.insert(reg_rect.end(), r, r + 4);
+        uint32_t bits = hlights[l].bits;
+        reg_bits.push_back(bits);
+        reg_mesh.push_back(i);
+        reg_light.push_back(l);
+        uint64_t texels = (uint64_t)(r[2] - r[0] + 1) * (r[3] - r[1] + 1);
+        reg_words.push_back(bits == 1 ? (texels + 31) / 32 : (texels + 3) / 4);
+        if (hlights[l].bakes_bounce) {
+          win_of_reg.push_back((int64_t)win_area.size());
+          uint32_t last = res[i] - 1;
+          uint32_t w[4] = {r[0] >= 4 ? r[0] - 4 : 0, r[1] >= 4 ? r[1] - 4 : 0, std::min(r[2] + 4, last), std::min(r[3] + 4, last)};
+          win_rect.insert(win_rect.end(), w, w + 4);
+          win_res.push_back(res[i]);
+          win_area.push_back((uint64_t)(w[2] - w[0] + 1) * (w[3] - w[1] + 1));
+        } else {
+          win_of_reg.push_back(-1);
+        }
+      }
+      for (uint32_t t = first_tri[i]; t < first_tri[i + 1]; ++t)
+        for (uint64_t e = h_ent_start[t]; e < h_ent_start[t + 1]; ++e) h_ent_region[e] = (uint32_t)slot_of[h_ent_light[e]];
+      for (uint32_t l : touched) slot_of[l] = -1;
+    }
+    mesh_reg_first[nm] = (uint32_t)reg_bits.size();
+    mesh_win_first[nm] = (uint32_t)win_area.size();
+  }
+
+  // ---- batches of meshes under the HBM budget ---------------------------------------------------
+  uint64_t budget = 48ull << 30;
+  if (const char* env = std::getenv("DL_BATCH_BYTES")) budget = std::max<uint64_t>(1ull << 20, strtoull(env, nullptr, 10));
+  std::vector<Batch> batches;
+  {
+    uint64_t acc = 0;
+    uint32_t start = 0;
+    for (uint32_t i = 0; i < nm; ++i) {
+      uint64_t est = 0;
+      for (uint32_t t = first_tri[i]; t < first_tri[i + 1]; ++t) {
+        est += h_tcc[t] * 80;  // staged and final texel records
+        const uint32_t* c = &h_crop[4 * (size_t)t];
+        uint64_t padded = (uint64_t)(c[2] - c[0] + 5) * (c[3] - c[1] + 5);
+        est += (h_ent_start[t + 1] - h_ent_start[t]) * (48 + padded * (bounce_in_scene ? 4 : 1) / 4);
+      }
+      for (uint32_t r = mesh_reg_first[i]; r < mesh_reg_first[i + 1]; ++r) est += reg_words[r] * 4;
+      for (uint32_t w = mesh_win_first[i]; w < mesh_win_first[i + 1]; ++w) est += win_area[w] * 24;
+      if (i > start && acc + est > budget) {
+        batches.push_back({start, i});
+        start = i;
+        acc = 0;
+      }
+      acc += est;
+    }
+    if (nm > start) batches.push_back({start, nm});
+  }
+  ctx->timings.texels_ms = ms_since(t_tex);
+
+  std::unique_ptr<dl_baked_scene> scene(new dl_baked_scene());
+  scene->meshes.resize(nm);
+  DBuf<unsigned long long> d_counters;
+  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 2));
+  const BlurWeights blur = blur_weights();
+  const uint32_t stride = bounce_in_scene ? 3 : 2;
+  float trace_kernel_ms = 0.0f;
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  struct EventGuard {
+    hipEvent_t a, b;
+    ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  } guard{ev0, ev1};
+  double trace_ms = 0, post_ms = 0, texels_ms = 0;
+
+  for (const Batch& B : batches) {
+    const uint32_t t0 = first_tri[B.m0], t1 = first_tri[B.m1], nbt = t1 - t0;
+    const uint64_t e0 = h_ent_start[t0], e1 = h_ent_start[t1], nbe = e1 - e0;
+    const uint32_t r0 = mesh_reg_first[B.m0], r1 = mesh_reg_first[B.m1], nbr = r1 - r0;
+    const uint32_t w0 = mesh_win_first[B.m0], w1 = mesh_win_first[B.m1], nbw = w1 - w0;
+    auto tb = std::chrono::steady_clock::now();
+    DevTris tr = trall;
+    tr.n = nbt;
+    tr.v = trall.v + (size_t)t0 * 9;
+    tr.nrm = trall.nrm + (size_t)t0 * 9;
+    tr.uv = trall.uv + (size_t)t0 * 6;
+    tr.mesh = trall.mesh + t0;
+    tr.local = trall.local + t0;
+    tr.face = trall.face + (size_t)t0 * 3;
+    tr.crop = trall.crop + (size_t)t0 * 4;
+
+    // entries and regions of the batch, rebased
+    std::vector<uint64_t> b_ent_start(nbt + 1), b_trace_start(nbt + 1, 0), b_cand_start(nbt + 1, 0);
+    for (uint32_t t = 0; t <= nbt; ++t) b_ent_start[t] = h_ent_start[t0 + t] - e0;
+    for (uint32_t t = 0; t < nbt; ++t) {
+      b_trace_start[t + 1] = b_trace_start[t] + h_tc[t0 + t];
+      b_cand_start[t + 1] = b_cand_start[t] + h_tcc[t0 + t];
+    }
+    std::vector<uint32_t> b_ent_light(h_ent_light.begin() + (long)e0, h_ent_light.begin() + (long)e1), b_ent_tri(nbe), b_ent_region(nbe);
+    std::vector<uint32_t> b_ent_window(nbe, 0xffffffffu);
+    for (uint32_t t = 0; t < nbt; ++t)
+      for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) {
+        b_ent_tri[e] = t;
+        uint32_t r = h_ent_region[e0 + e];
+        b_ent_region[e] = r - r0;
+        if (win_of_reg[r] >= 0) b_ent_window[e] = (uint32_t)(win_of_reg[r] - w0);
+      }
+    std::vector<uint32_t> b_reg_rect(reg_rect.begin() + 4 * (size_t)r0, reg_rect.begin() + 4 * (size_t)r1);
+    std::vector<uint32_t> b_reg_bits(reg_bits.begin() + r0, reg_bits.begin() + r1), b_reg_mesh(reg_mesh.begin() + r0, reg_mesh.begin() + r1);
+    std::vector<uint64_t> b_reg_off(nbr + 1, 0);
+    for (uint32_t r = 0; r < nbr; ++r) b_reg_off[r + 1] = b_reg_off[r] + reg_words[r0 + r];
+    DBuf<uint64_t> d_bes, d_bts, d_bcs, d_reg_off;
+    DBuf<uint32_t> d_bel, d_bet, d_ber, d_reg_rect, d_reg_bits, d_reg_mesh, d_pool;
+    HIP_TRY(upload(ctx, d_bes, b_ent_start));
+    HIP_TRY(upload(ctx, d_bts, b_trace_start));
+    HIP_TRY(upload(ctx, d_bcs, b_cand_start));
+    HIP_TRY(upload(ctx, d_bel, b_ent_light));
+    HIP_TRY(upload(ctx, d_bet, b_ent_tri));
+    HIP_TRY(upload(ctx, d_ber, b_ent_region));
+    HIP_TRY(upload(ctx, d_reg_rect, b_reg_rect));
+    HIP_TRY(upload(ctx, d_reg_bits, b_reg_bits));
+    HIP_TRY(upload(ctx, d_reg_mesh, b_reg_mesh));
+    HIP_TRY(upload(ctx, d_reg_off, b_reg_off));
+    HIP_TRY(zeroed(ctx, d_pool, b_reg_off[nbr] + 1));
+    DevRegions rg{d_reg_rect.p, d_reg_bits.p, d_reg_off.p, d_reg_mesh.p, d_pool.p};
+    DevEntries ent{nbe, d_bel.p, d_ber.p, d_bet.p, d_bes.p};
+
+    // ---- texels: owned, traced, burial, compaction (mesh_texels.rs:41-108) -------------------
+    DBuf<unsigned long long> d_cnt;
+    HIP_TRY(zeroed(ctx, d_cnt, (size_t)nbt + 1));
+    HIP_TRY(launch_owned(s, tr, ms, d_bts.p, b_trace_start[nbt]));
+    DevTexels none{};
+    const uint64_t* tc_b = d_tc.p + t0;
+    HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 0, d_cnt.p, nullptr, none));
+    DBuf<uint64_t> d_tex0_start;
+    HIP_TRY(d_tex0_start.alloc((size_t)nbt + 1));
+    HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex0_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+    uint64_t ntex0 = 0;
+    HIP_TRY(hipMemcpyAsync(&ntex0, d_tex0_start.p + nbt, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    DBuf<uint32_t> d_t0xy, d_t0tri;
+    DBuf<float4> d_t0pos, d_t0nrm;
+    HIP_TRY(d_t0xy.alloc(ntex0 + 1));
+    HIP_TRY(d_t0tri.alloc(ntex0 + 1));
+    HIP_TRY(d_t0pos.alloc(ntex0 + 1));
+    HIP_TRY(d_t0nrm.alloc(ntex0 + 1));
+    DevTexels tx0{d_t0xy.p, d_t0pos.p, d_t0nrm.p, d_t0tri.p};
+    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+    HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 1, d_cnt.p, d_tex0_start.p, tx0));
+    DBuf<uint8_t> d_buried;
+    HIP_TRY(d_buried.alloc(ntex0 + 1));
+    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+    HIP_TRY(launch_burial(s, sc, tr, ms, tx0, ntex0, opt.seed, ctx->trig, d_buried.p, d_cnt.p, ctx->burial_block));
+    DBuf<uint64_t> d_tex_start;
+    HIP_TRY(d_tex_start.alloc((size_t)nbt + 1));
+    HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+    std::vector<uint64_t> h_tex_start;
+    HIP_TRY(download(ctx, h_tex_start, d_tex_start.p, (size_t)nbt + 1));
+    const uint64_t ntex = h_tex_start[nbt];
+    DBuf<uint32_t> d_txy, d_ttri;
+    DBuf<float4> d_tpos, d_tnrm;
+    HIP_TRY(d_txy.alloc(ntex + 1));
+    HIP_TRY(d_ttri.alloc(ntex + 1));
+    HIP_TRY(d_tpos.alloc(ntex + 1));
+    HIP_TRY(d_tnrm.alloc(ntex + 1));
+    DevTexels tx{d_txy.p, d_tpos.p, d_tnrm.p, d_ttri.p};
+    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+    HIP_TRY(launch_compact(s, tx0, ntex0, d_buried.p, d_tex_start.p, d_cnt.p, tx));
+    HIP_TRY(hipStreamSynchronize(s));
+    d_t0xy.release();
+    d_t0tri.release();
+    d_t0pos.release();
+    d_t0nrm.release();
+    d_buried.release();
+    for (uint32_t t = 0; t < nbt; ++t) scene->stats.texels += h_tex_start[t + 1] - h_tex_start[t];
+    texels_ms += ms_since(tb);
+    if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
+
+    // ---- shadow rays (mesh_bake.rs:203-267) ----------------------------------------------------
+    auto t_trace = std::chrono::steady_clock::now();
+    std::vector<uint64_t> h_unit_start(nbt + 1, 0);
+    for (uint32_t t = 0; t < nbt; ++t) {
+      uint64_t texels = h_tex_start[t + 1] - h_tex_start[t];
+      h_unit_start[t + 1] = h_unit_start[t] + (b_ent_start[t + 1] - b_ent_start[t]) * ((texels + 63) / 64);
+    }
+    DBuf<uint64_t> d_unit_start;
+    HIP_TRY(upload(ctx, d_unit_start, h_unit_start));
+    TraceParams tp;
+    tp.seed = opt.seed;
+    tp.area_samples = opt.area_light_samples;
+    tp.area_min = opt.area_light_min_samples;
+    tp.sky_samples = opt.sky_samples;
+    DevScene sct = sc;
+    sct.counters = ctx->counting ? d_counters.p : nullptr;
+    HIP_TRY(hipEventRecord(ev0, s));
+    HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nbt], tp, ctx->trig,
+                         ctx->trace_block));
+    HIP_TRY(hipEventRecord(ev1, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float kms = 0.0f;
+    (void)hipEventElapsedTime(&kms, ev0, ev1);
+    trace_kernel_ms += kms;
+    ctx->timings.trace_launches++;
+    trace_ms += ms_since(t_trace);
+    if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
+
+    // ---- bounce gather, dilate, blur (bounce.rs:155-217) -----------------------------------------
+    auto t_b = std::chrono::steady_clock::now();
+    DBuf<uint8_t> d_has;
+    DBuf<float> d_map_a, d_map_b;
+    DBuf<uint32_t> d_win_rect, d_win_res, d_ent_window;
+    DBuf<uint64_t> d_win_off, d_win_start, d_be, d_be_units;
+    DevBounce bb{};
+    if (bounce_in_scene && nbw > 0) {
+      std::vector<uint64_t> win_off(nbw + 1, 0);
+      for (uint32_t w = 0; w < nbw; ++w) win_off[w + 1] = win_off[w] + win_area[w0 + w];
+      std::vector<uint32_t> b_win_rect(win_rect.begin() + 4 * (size_t)w0, win_rect.begin() + 4 * (size_t)w1);
+      std::vector<uint32_t> b_win_res(win_res.begin() + w0, win_res.begin() + w1);
+      std::vector<uint64_t> be, be_units(1, 0);
+      for (uint32_t t = 0; t < nbt; ++t) {
+        uint64_t chunks = (h_tex_start[t + 1] - h_tex_start[t] + 63) / 64;
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
+          if (hlights[b_ent_light[e]].bakes_bounce) {
+            be.push_back(e);
+            be_units.push_back(be_units.back() + chunks);
+          }
+      }
+      HIP_TRY(upload(ctx, d_win_rect, b_win_rect));
+      HIP_TRY(upload(ctx, d_win_res, b_win_res));
+      HIP_TRY(upload(ctx, d_win_off, win_off));
+      HIP_TRY(upload(ctx, d_win_start, win_off));
+      HIP_TRY(upload(ctx, d_ent_window, b_ent_window));
+      HIP_TRY(upload(ctx, d_be, be));
+      HIP_TRY(upload(ctx, d_be_units, be_units));
+      HIP_TRY(zeroed(ctx, d_map_a, win_off[nbw] * 3 + 3));
+      HIP_TRY(zeroed(ctx, d_map_b, win_off[nbw] * 3 + 3));
+      bb.n = be.size();
+      bb.entry = d_be.p;
+      bb.unit_start = d_be_units.p;
+      bb.win_rect = d_win_rect.p;
+      bb.win_off = d_win_off.p;
+      bb.ent_window = d_ent_window.p;
+      bb.cube_of_slot = d_cube_of.p;
+      bb.map_a = d_map_a.p;
+      bb.map_b = d_map_b.p;
+      bb.rays = d_mbrays.p;
+      BounceParams bp;
+      bp.seed = opt.seed;
+      bp.colored = opt.colored_bounce ? 1u : 0u;
+      bp.cube_res = cres;
+      bp.photons = d_photons.p;
+      HIP_TRY(launch_bounce_owner(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back()));
+      HIP_TRY(launch_bounce_gather(s, sc, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
+                                   ctx->trace_block));
+      HIP_TRY(launch_bounce_filter(s, bb, d_win_res.p, d_win_start.p, nbw, win_off[nbw], blur));
+      HIP_TRY(d_has.alloc(nbe + 1));
+      HIP_TRY(launch_bounce_has(s, tr, ent, bb, d_has.p));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+    bounce_ms += ms_since(t_b);
+    if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
+
+    // ---- optimize, seams, crop, pack, serialize (shadows.rs, seams.rs, triangle_words.rs) ------
+    auto t_post = std::chrono::steady_clock::now();
+    DBuf<uint8_t> d_keep;
+    HIP_TRY(d_keep.alloc(nbe + 1));
+    HIP_TRY(launch_optimize(s, tr, ms, ent, rg, d_has.p, d_keep.p));
+    std::vector<uint8_t> h_keep, h_has;
+    HIP_TRY(download(ctx, h_keep, d_keep.p, nbe));
+    if (d_has.p) HIP_TRY(download(ctx, h_has, d_has.p, nbe));
+    std::vector<uint8_t> used(nbr, 0);
+    std::vector<uint64_t> mesh_base(nm + 1, 0);
+    std::vector<uint32_t> ld_off(nbt, 0), kept_shadow_off, kept_bounce_off, first_adj(nm, 0);
+    std::vector<uint64_t> tri_kept_start(nbt + 1, 0), kept_entry, kept_word_start, kept_out;
+    std::vector<uint64_t> kb_entry, kb_word_start, kb_out, kd_entry, kd_word_start, kd_out;
+    uint64_t flat = 0, kb_flat = 0, kd_flat = 0, bw = 0;
+    for (uint32_t i = B.m0; i < B.m1; ++i) {
+      first_adj[i] = first_tri[i] - t0;
+      uint32_t tm0 = first_tri[i] - t0, tm1 = first_tri[i + 1] - t0;
+      uint64_t w = 4ull * (tm1 - tm0);
+      uint64_t removed = 0;
+      for (uint32_t t = tm0; t < tm1; ++t) {
+        ld_off[t] = (uint32_t)w;
+        uint64_t kept = 0;
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) kept += h_keep[e];
+        removed += (b_ent_start[t + 1] - b_ent_start[t]) - kept;
+        w += 1 + stride * kept;
+      }
+      scene->stats.lights_removed += removed;
+      for (uint32_t t = tm0; t < tm1; ++t) {
+        tri_kept_start[t] = kept_entry.size();
+        const uint32_t* c = &h_crop[4 * (size_t)(t0 + t)];
+        uint64_t texels = (uint64_t)(c[2] - c[0] + 5) * (c[3] - c[1] + 5);
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) {
+          if (!h_keep[e]) continue;
+          uint32_t r = b_ent_region[e];
+          used[r] = 1;
+          uint64_t words = (texels + (32 / b_reg_bits[r]) - 1) / (32 / b_reg_bits[r]);
+          kept_entry.push_back(e);
+          kept_word_start.push_back(flat);
+          kept_out.push_back(bw + w);
+          kept_shadow_off.push_back((uint32_t)w);
+          flat += words;
+          w += words;
+          uint32_t boff = 0;
+          if (!h_has.empty() && h_has[e]) {
+            int32_t comp = comp_of_slot[b_ent_light[e]];
+            uint32_t bits = bounce_bits(comp);
+            uint64_t bwords = opt.colored_bounce ? (texels * 3 * bits + 31) / 32 : (texels + (32 / bits) - 1) / (32 / bits);
+            boff = (uint32_t)w;
+            if (comp == DL_BOUNCE_BITS8) {
+              kb_entry.push_back(e);
+              kb_word_start.push_back(kb_flat);
+              kb_out.push_back(bw + w);
+              kb_flat += bwords;
+            } else {
+              kd_entry.push_back(e);
+              kd_word_start.push_back(kd_flat);
+              kd_out.push_back(bw + w);
+              kd_flat += bwords;
+            }
+            w += bwords;
+          }
+          kept_bounce_off.push_back(boff);
+        }
+      }
+      mesh_base[i] = bw;
+      bw += w;
+      mesh_base[i + 1] = bw;
+    }
+    tri_kept_start[nbt] = kept_entry.size();
+    kept_word_start.push_back(flat);
+    kb_word_start.push_back(kb_flat);
+    kd_word_start.push_back(kd_flat);
+    std::vector<uint32_t> used_list;
+    std::vector<uint64_t> used_area(1, 0);
+    for (uint32_t r = 0; r < nbr; ++r)
+      if (used[r]) {
+        used_list.push_back(r);
+        const uint32_t* q = &b_reg_rect[4 * (size_t)r];
+        used_area.push_back(used_area.back() + (uint64_t)(q[2] - q[0] + 1) * (q[3] - q[1] + 1));
+      }
+    DBuf<uint32_t> d_used;
+    DBuf<uint64_t> d_used_area;
+    HIP_TRY(upload(ctx, d_used, used_list));
+    HIP_TRY(upload(ctx, d_used_area, used_area));
+    HIP_TRY(launch_pad_seams(s, ms, rg, d_used.p, d_used_area.p, (uint32_t)used_list.size(), used_area.back()));
+    const uint64_t nkept = kept_entry.size();
+    DBuf<uint32_t> d_out, d_ld_off, d_kso, d_kbo, d_first;
+    DBuf<uint64_t> d_mesh_base, d_tks, d_ke, d_kws, d_kout, d_kbe, d_kbws, d_kbout, d_kde, d_kdws, d_kdout;
+    HIP_TRY(d_out.alloc(bw + 1));
+    HIP_TRY(upload(ctx, d_mesh_base, mesh_base));
+    HIP_TRY(upload(ctx, d_ld_off, ld_off));
+    HIP_TRY(upload(ctx, d_tks, tri_kept_start));
+    HIP_TRY(upload(ctx, d_ke, kept_entry));
+    HIP_TRY(upload(ctx, d_kws, kept_word_start));
+    HIP_TRY(upload(ctx, d_kout, kept_out));
+    HIP_TRY(upload(ctx, d_kso, kept_shadow_off));
+    HIP_TRY(upload(ctx, d_kbo, kept_bounce_off));
+    HIP_TRY(upload(ctx, d_first, first_adj));
+    HIP_TRY(launch_serialize(s, tr, d_mesh_base.p, d_ld_off.p, d_tks.p, d_ke.p, d_kso.p, d_kbo.p, d_bel.p, stride, d_first.p,
+                             d_out.p));
+    HIP_TRY(launch_crop(s, tr, ent, rg, d_ke.p, d_kws.p, nkept, flat, d_kout.p, d_out.p));
+    if (!kb_entry.empty()) {
+      HIP_TRY(upload(ctx, d_kbe, kb_entry));
+      HIP_TRY(upload(ctx, d_kbws, kb_word_start));
+      HIP_TRY(upload(ctx, d_kbout, kb_out));
+      HIP_TRY(launch_bounce_pack(s, tr, ms, d_lights.p, ent, bb, d_kbe.p, d_kbws.p, kb_entry.size(), kb_flat, d_kbout.p, d_comp.p,
+                                 opt.colored_bounce ? 1u : 0u, opt.seed, 0, d_out.p));
+    }
+    if (!kd_entry.empty()) {
+      HIP_TRY(upload(ctx, d_kde, kd_entry));
+      HIP_TRY(upload(ctx, d_kdws, kd_word_start));
+      HIP_TRY(upload(ctx, d_kdout, kd_out));
+      HIP_TRY(launch_bounce_pack(s, tr, ms, d_lights.p, ent, bb, d_kde.p, d_kdws.p, kd_entry.size(), kd_flat, d_kdout.p, d_comp.p,
+                                 opt.colored_bounce ? 1u : 0u, opt.seed, 1, d_out.p));
+    }
+    std::vector<uint32_t> words;
+    HIP_TRY(download(ctx, words, d_out.p, bw));
+    for (uint32_t i = B.m0; i < B.m1; ++i) {
+      dl_baked_mesh_out& mo = scene->meshes[i];
+      mo.id = meshes[i]->id;
+      mo.content_hash = meshes[i]->content_hash;
+      mo.resolution = res[i];
+      mo.occludes = meshes[i]->occludes;
+      mo.words.assign(words.begin() + (long)mesh_base[i], words.begin() + (long)mesh_base[i + 1]);
+    }
+    post_ms += ms_since(t_post);
+    advance(B.m1 - B.m0);
+    (void)dithered_bounce;
+  }
+  ctx->timings.texels_ms += texels_ms;
+  ctx->timings.trace_ms = trace_ms;
+  ctx->timings.trace_kernel_ms = trace_kernel_ms;
+  ctx->timings.bounce_ms = bounce_ms;
+  ctx->timings.post_ms = post_ms;
+  if (ctx->counting) {
+    unsigned long long c[2] = {0, 0};
+    HIP_TRY(hipMemcpy(c, d_counters.p, 16, hipMemcpyDeviceToHost));
+    ctx->timings.trace_node_visits = c[0];
+    ctx->timings.trace_tri_tests = c[1];
+  }
+
+  // ---- distance cubes (distance_cubes.rs:29-72) -----------------------------------------------
+  auto t_maps = std::chrono::steady_clock::now();
+  stage(3, (uint64_t)nlocal * 6);
+  if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY && nlocal > 0) {
+    DBuf<uint32_t> d_cubes;
+    HIP_TRY(d_cubes.alloc((size_t)nlocal * 6 * 2048));
+    HIP_TRY(launch_distance_cubes(s, sc, d_lights.p, nlocal, d_cubes.p));
+    HIP_TRY(download(ctx, scene->distance_cubes, d_cubes.p, (size_t)nlocal * 6 * 2048));
+  }
+  ctx->timings.maps_ms = ms_since(t_maps);
+
+  // ---- stats, settings, lights --------------------------------------------------------------
+  auto t_down = std::chrono::steady_clock::now();
+  std::vector<unsigned long long> h_rays, h_samples, h_brays;
+  HIP_TRY(download(ctx, h_rays, d_mrays.p, nm));
+  HIP_TRY(download(ctx, h_samples, d_msamples.p, nm));
+  HIP_TRY(download(ctx, h_brays, d_mbrays.p, nm));
+  uint32_t overflow = 0;
+  HIP_TRY(hipMemcpy(&overflow, d_overflow.p, 4, hipMemcpyDeviceToHost));
+  if (overflow) { ctx->err = "BVH traversal stack overflow"; return DL_E_DEVICE; }
+  ctx->timings.download_ms = ms_since(t_down);
+
+  scene->settings.bounce = bounce_in_scene;
+  scene->settings.colored_bounce = bounce_in_scene && opt.colored_bounce;
+  scene->settings.dynamic_geometry = opt.dynamic_geometry;
+  scene->settings.pixel_density = opt.pixel_density;
+  scene->settings.max_lightmap_size = opt.max_lightmap_size;
+  scene->settings.distance_cube_resolution = 64;
+  for (size_t i = 0; i < lights.size(); ++i) {
+    const dl_light* l = lights[i];
+    dl_baked_light b{};
+    b.id = l->id;
+    std::memcpy(b.origin, l->position, 12);
+    b.radius = l->radius;
+    b.shape = l->shape;
+    std::memcpy(b.rotation, l->rotation, 16);
+    std::memcpy(b.size, l->size, 8);
+    b.angular_diameter = l->angular_diameter;
+    b.shadow_bits_per_texel = hlights[i].bits;
+    b.bounce_compression = hlights[i].bakes_bounce ? comp_of_slot[i] : -1;
+    scene->lights.push_back(b);
+  }
+  scene->bvh = lbvh.nodes;
+  scene->stats.meshes = nm;
+  scene->stats.triangles = ntri;
+  for (uint32_t i = 0; i < nm; ++i) {
+    scene->stats.triangle_words += scene->meshes[i].words.size();
+    scene->stats.rays += h_rays[i];
+    scene->stats.samples += h_samples[i];
+    scene->stats.bounce_rays += h_brays[i];
+  }
+  ctx->timings.trace_rays = scene->stats.rays;
+  stage(4, 0);
+  ctx->timings.total_ms = ms_since(t_start);
+  *out = scene.release();
+  return DL_OK;
+}

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -85,3 +86,50 @@ struct dl_baked_scene {
 
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
                  dl_baked_scene** out);
+
+namespace dl {
+
+// Validation with the reference's messages (input.rs:233-373); errors go to ctx->err.
+std::string hex_id(uint64_t id);
+int check_options(dl_ctx* ctx, const dl_bake_options& o);
+int check_mesh(dl_ctx* ctx, const dl_mesh& m);
+int check_light(dl_ctx* ctx, const dl_light& l);
+DevLight make_dev_light(const dl_light& l, const dl_bake_options& o);
+// lightmap_size (geometry.rs:66-70)
+uint32_t lightmap_size(float area, float density, uint32_t max_size);
+
+inline double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <class T>
+hipError_t upload(dl_ctx* ctx, DBuf<T>& b, const std::vector<T>& h) {
+  hipError_t e = b.alloc(h.size());
+  if (e != hipSuccess || h.empty()) return e;
+  return hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+}
+template <class T>
+hipError_t download(dl_ctx* ctx, std::vector<T>& h, const T* p, size_t n) {
+  h.resize(n);
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+template <class T>
+hipError_t zeroed(dl_ctx* ctx, DBuf<T>& b, size_t n) {
+  hipError_t e = b.alloc(n);
+  if (e != hipSuccess || n == 0) return e;
+  return hipMemsetAsync(b.p, 0, n * sizeof(T), ctx->stream);
+}
+
+}  // namespace dl
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);          \
+      return e_ == hipErrorOutOfMemory ? (int)DL_E_OOM : (int)DL_E_DEVICE;   \
+    }                                                                        \
+  } while (0)

@@ -124,6 +124,53 @@ hipError_t launch_serialize(hipStream_t s, DevTris tris, const uint64_t* mesh_ba
                             const uint32_t* tri_first_in_mesh, uint32_t* out);
 hipError_t launch_distance_cubes(hipStream_t s, DevScene scene, const DevLight* lights, uint32_t nlocal, uint32_t* out);
 
+// ---- bounce (bounce.rs, sun_bounce.rs, bounce_map.rs, bounce_pack.rs) -------------------------
+// Per occluder: vertex normals (9 floats, used when flags bit 0), albedo (3 floats).
+struct DevOcc {
+  const float* nrm;
+  const float* albedo;
+  const uint32_t* flags;
+};
+struct BounceParams {
+  uint64_t seed;
+  uint32_t colored;
+  uint32_t cube_res;
+  const uint32_t* photons;  // 3 words per photon, 6 * res^2 per cube
+};
+// Bounce work: entries (triangle-light pairs) whose light bakes bounce, each with its window.
+struct DevBounce {
+  uint64_t n;                 // bounce entries
+  const uint64_t* entry;      // entry index of each bounce entry
+  const uint64_t* unit_start; // n + 1: 64-texel chunks per bounce entry
+  const uint32_t* win_rect;   // 4 per window
+  const uint64_t* win_off;    // float3 offset of each window
+  const uint32_t* ent_window; // window of each entry (all entries; ~0u when none)
+  const int32_t* cube_of_slot;// cube index per light slot, -1 when none
+  float* map_a;               // 3 floats per window texel
+  float* map_b;
+  unsigned long long* rays;   // per mesh
+};
+hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
+                          uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block);
+hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
+                               DevEntries ent, DevBounce b, uint64_t units);
+hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,
+                                const DevLight* lights, DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevBounce b,
+                                uint64_t units, BounceParams p, TrigTables trig, uint32_t block);
+struct BlurWeights {
+  float w[7];
+  float sum;
+};
+hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,
+                                uint32_t nwin, uint64_t total, BlurWeights weights);
+hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
+// Packs the bounce of kept entries into `out` (bounce_pack.rs:18-86): per-word when no dither is
+// drawn, one thread per entry otherwise.
+hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
+                              DevBounce b, const uint64_t* kept_entry, const uint64_t* word_start, uint64_t nkept,
+                              uint64_t total_words, const uint64_t* kept_out, const int32_t* comp_of_slot,
+                              uint32_t colored, uint64_t seed, int dithered, uint32_t* out);
+
 // Batched closest-hit / any-hit queries (tests and maps): rays as 8 floats (o, d, max, pad).
 hipError_t launch_ray_queries(hipStream_t s, DevScene scene, const float* rays, uint64_t n, int closest,
                               float* t_out, uint32_t* idx_out);

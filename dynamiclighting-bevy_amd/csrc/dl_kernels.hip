@@ -596,6 +596,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
           some = true;
         }
         if (some) level = level_of(fraction, L.bits);
+        else rays = 0;  // rays of a texel the light does not reach are not counted (mesh_bake.rs:241-245)
       }
       if (some) {
         some_n = 1;
@@ -924,6 +925,491 @@ hipError_t scan_u64(hipStream_t s, const uint64_t* in, uint64_t* out, uint64_t n
     *tmp_bytes = need;
   }
   return hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, (int)(n + 1), s);
+}
+
+// =============================================================================================
+// Bounce
+// =============================================================================================
+
+// Occluders::surface_normal (occluder.rs:131-145)
+__device__ __forceinline__ f3 surface_normal(const DevScene& sc, const DevOcc& occ, const Hit& h) {
+  f3 face = tri_face_normal(sc, h.pos);
+  if (!(occ.flags[h.occ] & 1u)) return face;
+  const float* n = occ.nrm + 9 * (size_t)h.occ;
+  float wx = (1.0f - h.u) - h.v;
+  f3 s = add(add(mul(ld3(n), wx), mul(ld3(n + 3), h.u)), mul(ld3(n + 6), h.v));
+  f3 r;
+  return try_normalize(s, &r) ? r : face;
+}
+
+// PhotonCube::build / photon (bounce.rs:48-115)
+__global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ lights, const uint32_t* __restrict__ cube_slot,
+                          uint32_t ncubes, uint32_t res, uint32_t* photons) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  const uint64_t per = 6ull * res * res;
+  const uint64_t total = per * ncubes;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = (uint32_t)(i / per);
+    uint64_t rem = i % per;
+    uint32_t face = (uint32_t)(rem / ((uint64_t)res * res));
+    uint32_t yx = (uint32_t)(rem % ((uint64_t)res * res));
+    uint32_t y = yx / res, x = yx % res;
+    const DevLight& L = lights[cube_slot[c]];
+    f3 lp = ld3(L.pos);
+    f3 dir = cube_texel_dir(face, x, y, res);
+    Hit h;
+    uint32_t w0, w1 = 0, w2 = 0;
+    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h)) {
+      w0 = __float_as_uint(L.radius);
+    } else {
+      f3 normal = surface_normal(sc, occ, h);
+      f3 point = add(lp, mul(dir, h.t));
+      float bias = maxf_(h.t * 0.001f, 0.001f);
+      f3 nudged = add(sub(point, mul(dir, bias)), mul(normal, bias));
+      bool lit_side = dot(tri_face_normal(sc, h.pos), dir) < 0.0f;
+      w0 = __float_as_uint(len(sub(nudged, lp)));
+      if (lit_side)
+        w1 = ((uint32_t)sat_i8(roundf(normal.x * 127.0f)) & 0xffu) | (((uint32_t)sat_i8(roundf(normal.y * 127.0f)) & 0xffu) << 8) |
+             (((uint32_t)sat_i8(roundf(normal.z * 127.0f)) & 0xffu) << 16);
+      const float* a = occ.albedo + 3 * (size_t)h.occ;
+      w2 = sat_u8(roundf(clampf_(a[0], 0.0f, 1.0f) * 255.0f)) | (sat_u8(roundf(clampf_(a[1], 0.0f, 1.0f) * 255.0f)) << 8) |
+           (sat_u8(roundf(clampf_(a[2], 0.0f, 1.0f) * 255.0f)) << 16);
+    }
+    photons[3 * i] = w0;
+    photons[3 * i + 1] = w1;
+    photons[3 * i + 2] = w2;
+  }
+}
+
+// glam acos_approx (DirectXMath XMScalarAcos polynomial)
+__device__ __forceinline__ float acos_approx(float v) {
+  bool nonneg = v >= 0.0f;
+  float x = fabsf(v);
+  float omx = 1.0f - x;
+  if (omx < 0.0f) omx = 0.0f;
+  float root = sqrtf(omx);
+  float r = ((((((-0.0012624911f * x + 0.0066700901f) * x - 0.0170881256f) * x + 0.0308918810f) * x - 0.0501743046f) * x +
+              0.0889789874f) * x - 0.2145988016f) * x + 1.5707963050f;
+  r *= root;
+  return nonneg ? r : kPi - r;
+}
+// glam Vec3::slerp
+__device__ f3 slerp3(const TrigTables& trig, f3 a, f3 b, float s) {
+  float la = len(a), lb = len(b);
+  float d = dot(a, b) / (la * lb);
+  if (fabsf(d) < 1.0f - 3.0e-7f) {
+    float theta = acos_approx(d);
+    float sin_theta = lsinf(trig, theta);
+    float t1 = lsinf(trig, theta * (1.0f - s));
+    float t2 = lsinf(trig, theta * s);
+    float rl = la + (lb - la) * s;
+    return mul(add(mul(mul(a, rl / la), t1), mul(mul(b, rl / lb), t2)), 1.0f / sin_theta);
+  }
+  if (d < 0.0f) {
+    f3 o = fabsf(a.x) > fabsf(a.y) ? mk3(-a.z, 0.0f, a.x) : mk3(0.0f, a.z, -a.y);
+    f3 axis = normalize(o);
+    float half = kPi * s * 0.5f;
+    float sn = lsinf(trig, half), cs = lcosf(trig, half);
+    float q[4] = {axis.x * sn, axis.y * sn, axis.z * sn, cs};
+    float rl = la + (lb - la) * s;
+    return mul(quat_rotate(q, a), rl / la);
+  }
+  return add(a, mul(sub(b, a), s));
+}
+// random::on_unit_sphere (random.rs:22-28)
+__device__ __forceinline__ f3 on_unit_sphere(const TrigTables& trig, Pcg& rng) {
+  float z = pcg_f32(rng) * 2.0f - 1.0f;
+  float angle = pcg_f32(rng) * kTau;
+  float r = sqrtf(maxf_(1.0f - z * z, 0.0f));
+  return mk3(r * lcosf(trig, angle), r * lsinf(trig, angle), z);
+}
+
+struct BounceUnit {
+  uint32_t t;      // batch-local triangle
+  uint64_t e;      // entry
+  uint64_t i;      // texel index within the triangle
+  uint64_t nt;     // texels of the triangle
+  uint64_t t0;     // first texel
+};
+__device__ __forceinline__ BounceUnit bounce_unit(const DevBounce& b, const DevEntries& ent, const uint64_t* tex_start, uint64_t unit,
+                                                  uint32_t lane) {
+  BounceUnit u;
+  uint64_t j = owner_of(b.unit_start, b.n, unit);
+  u.e = b.entry[j];
+  u.t = ent.tri[u.e];
+  u.t0 = tex_start[u.t];
+  u.nt = tex_start[u.t + 1] - u.t0;
+  u.i = (unit - b.unit_start[j]) * 64 + lane;
+  return u;
+}
+
+// "Later triangles overwrite shared texels" (bounce.rs:202-209): the highest triangle that
+// gathers a texel owns it.
+__global__ void k_bounce_owner(DevTris tr, const DevLight* __restrict__ lights, DevTexels tx, const uint64_t* __restrict__ tex_start,
+                               DevEntries ent, DevBounce b, uint64_t units) {
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
+    BounceUnit u = bounce_unit(b, ent, tex_start, unit, lane);
+    if (u.i >= u.nt) continue;
+    const DevLight& L = lights[ent.light[u.e]];
+    uint64_t j = u.t0 + u.i;
+    if (L.shape != 3) {
+      float4 p4 = tx.pos[j];
+      f3 to_texel = sub(mk3(p4.x, p4.y, p4.z), ld3(L.pos));
+      if (len_sq(to_texel) > L.radius * L.radius) continue;
+    }
+    uint32_t w = b.ent_window[u.e];
+    const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+    uint32_t xy = tx.xy[j];
+    uint64_t k = (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
+    unsigned int* owner = reinterpret_cast<unsigned int*>(b.map_b) + b.win_off[w] * 3;
+    atomicMax(&owner[k], u.t + 1);
+  }
+}
+
+// gather_triangle (bounce.rs:235-292) and gather_sun (sun_bounce.rs:24-80)
+__global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
+                                const uint64_t* __restrict__ tex_start, DevEntries ent, DevBounce b, uint64_t units,
+                                BounceParams p, TrigTables trig) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
+    BounceUnit u = bounce_unit(b, ent, tex_start, unit, lane);
+    uint32_t m = tr.mesh[u.t];
+    uint32_t rays = 0;
+    if (u.i < u.nt) {
+      const DevLight& L = lights[ent.light[u.e]];
+      uint64_t j = u.t0 + u.i;
+      uint32_t xy = tx.xy[j];
+      uint32_t x = xy & 0xffffu, y = xy >> 16;
+      float4 p4 = tx.pos[j];
+      f3 pos = mk3(p4.x, p4.y, p4.z);
+      f3 normal = ld3(tr.face + 3 * (size_t)u.t);
+      uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
+      bool write = true;
+      f3 sum = mk3(0.0f, 0.0f, 0.0f);
+      Hit h;
+      if (L.shape != 3) {
+        f3 lp = ld3(L.pos);
+        f3 to_texel = sub(pos, lp);
+        if (len_sq(to_texel) > L.radius * L.radius) {
+          write = false;
+        } else {
+          const uint32_t* cube = p.photons + (size_t)b.cube_of_slot[ent.light[u.e]] * 6 * p.cube_res * p.cube_res * 3;
+          float step = 0.9f / (float)(samples - 1);
+          f3 direction = normalize_or(to_texel, mk3(0.0f, 1.0f, 0.0f));
+          f3 origin = add(pos, mul(normal, kRayOffset));
+          Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
+          for (uint32_t k = 0; k < samples; ++k) {
+            float spread = 0.1f + (float)k * step;
+            f3 rnd = on_unit_sphere(trig, rng);
+            f3 sample = normalize_or(slerp3(trig, direction, rnd, spread), direction);
+            uint32_t ci = cube_texel_of(sample, p.cube_res);
+            uint32_t w0 = cube[3 * ci], w1 = cube[3 * ci + 1], w2 = cube[3 * ci + 2];
+            f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
+                                          (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
+            f3 photon = add(lp, mul(sample, __uint_as_float(w0)));
+            f3 to_photon = sub(photon, origin);
+            float distance = len(to_photon);
+            if (is_zero(pn) || distance <= 0.0f) continue;
+            f3 d = divs(to_photon, distance);
+            rays++;
+            if (!traverse<true>(sc, stack, blockDim.x, origin, d, distance, &h)) {
+              float weight = maxf_(dot(normal, d), 0.0f) * maxf_(dot(pn, neg(d)), 0.0f);
+              f3 alb = p.colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f,
+                                       (float)((w2 >> 16) & 0xffu) / 255.0f)
+                                 : mk3(1.0f, 1.0f, 1.0f);
+              sum = add(sum, mul(alb, weight));
+            }
+          }
+        }
+      } else {
+        f3 toward = ld3(L.toward);
+        f3 origin = add(pos, mul(normal, kRayOffset));
+        Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
+        f2 offset;
+        offset.x = pcg_f32(rng);
+        offset.y = pcg_f32(rng);
+        for (uint32_t k = 0; k < samples; ++k) {
+          f3 direction = cosine_dir(trig, normal, spread_pt(k, offset, true));
+          rays++;
+          if (!traverse<false>(sc, stack, blockDim.x, origin, direction, INFINITY, &h)) continue;
+          f3 face = tri_face_normal(sc, h.pos);
+          if (dot(face, direction) >= 0.0f) continue;
+          float lit = dot(surface_normal(sc, occ, h), toward);
+          if (lit <= 0.0f || dot(face, toward) <= 0.0f) continue;
+          f3 point = add(add(origin, mul(direction, h.t)), mul(face, kRayOffset));
+          rays++;
+          Hit h2;
+          if (!traverse<true>(sc, stack, blockDim.x, point, toward, INFINITY, &h2)) {
+            f3 alb = mk3(1.0f, 1.0f, 1.0f);
+            if (p.colored) {
+              const float* a = occ.albedo + 3 * (size_t)h.occ;
+              alb = mk3(a[0], a[1], a[2]);
+            }
+            sum = add(sum, mul(alb, lit));
+          }
+        }
+      }
+      if (write) {
+        uint32_t w = b.ent_window[u.e];
+        const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+        uint64_t k = (uint64_t)(x - wr[0]) + (uint64_t)(y - wr[1]) * (wr[2] - wr[0] + 1);
+        const unsigned int* owner = reinterpret_cast<const unsigned int*>(b.map_b) + b.win_off[w] * 3;
+        if (owner[k] == u.t + 1) {
+          f3 v = divs(mul(sum, L.bounce_intensity), (float)samples);
+          float* dst = b.map_a + (b.win_off[w] + k) * 3;
+          dst[0] = v.x;
+          dst[1] = v.y;
+          dst[2] = v.z;
+        }
+      }
+    }
+    wave_add(&b.rays[m], rays);
+  }
+}
+
+struct WinView {
+  const float* map;
+  uint32_t x0, y0, x1, y1;
+  __device__ f3 get(int64_t x, int64_t y) const {
+    if (x < (int64_t)x0 || y < (int64_t)y0 || x > (int64_t)x1 || y > (int64_t)y1) return mk3(0.0f, 0.0f, 0.0f);
+    const float* p = map + 3 * ((uint64_t)(x - x0) + (uint64_t)(y - y0) * (x1 - x0 + 1));
+    return mk3(p[0], p[1], p[2]);
+  }
+};
+
+// BounceMap::dilate (pass 0) and blur (passes 1, 2) (bounce_map.rs:60-125)
+__global__ void k_bounce_filter(DevBounce b, const uint32_t* __restrict__ win_res, const uint64_t* __restrict__ win_start,
+                                uint32_t nwin, uint64_t total, int pass, BlurWeights bw) {
+  const float* src = pass == 1 ? b.map_b : b.map_a;
+  float* dst = pass == 1 ? b.map_a : b.map_b;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t w = (uint32_t)owner_of(win_start, nwin, c);
+    uint64_t local = c - win_start[w];
+    const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+    uint32_t ww = wr[2] - wr[0] + 1;
+    int64_t x = wr[0] + (int64_t)(local % ww), y = wr[1] + (int64_t)(local / ww);
+    WinView v{src + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+    f3 r = mk3(0.0f, 0.0f, 0.0f);
+    if (pass == 0) {
+      const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1}, oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+      for (int k = 0; k < 9; ++k) {
+        f3 s = v.get(x + ox[k], y + oy[k]);
+        if (max_elem(s) > 0.0f) {
+          r = s;
+          break;
+        }
+      }
+    } else {
+      int64_t last = (int64_t)win_res[w] - 1;
+      f3 sum = mk3(0.0f, 0.0f, 0.0f);
+      for (int k = -3; k <= 3; ++k) {
+        f3 s;
+        if (pass == 2) {
+          int64_t yy = y + k;
+          yy = yy < 0 ? 0 : (yy > last ? last : yy);
+          s = v.get(x, yy);
+        } else {
+          int64_t xx = x + k;
+          xx = xx < 0 ? 0 : (xx > last ? last : xx);
+          s = v.get(xx, y);
+        }
+        sum = add(sum, mul(s, bw.w[k + 3]));
+      }
+      r = divs(sum, bw.sum);
+    }
+    float* d = dst + 3 * (b.win_off[w] + local);
+    d[0] = r.x;
+    d[1] = r.y;
+    d[2] = r.z;
+  }
+}
+
+// BounceMap::crop's lit test (bounce_map.rs:128-137): any positive texel inside the crop.
+__global__ void k_bounce_has(DevTris tr, DevEntries ent, DevBounce b, uint8_t* has) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ent.n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t w = b.ent_window[e];
+    if (w == 0xffffffffu) {
+      has[e] = 0;
+      continue;
+    }
+    uint32_t t = ent.tri[e];
+    const uint32_t* c = tr.crop + 4 * (size_t)t;
+    const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+    WinView v{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+    bool any = false;
+    for (uint32_t y = c[1]; y <= c[3] && !any; ++y)
+      for (uint32_t x = c[0]; x <= c[2]; ++x)
+        if (max_elem(v.get(x, y)) > 0.0f) {
+          any = true;
+          break;
+        }
+    has[e] = any ? 1 : 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t comp_bits(int32_t c) { return c == 0 ? 8u : c == 1 ? 6u : c == 2 ? 5u : 4u; }
+__device__ __forceinline__ float comp_amp(int32_t c) { return c == 0 ? 0.0f : c == 1 ? 0.004f : c == 2 ? 0.008f : 0.02f; }
+// quantize (bounce_pack.rs:75-86) without dither
+__device__ __forceinline__ uint32_t quantize0(float value, uint32_t bits) {
+  float mx = (float)((1u << bits) - 1u);
+  float color = value != value ? 0.0f : clampf_(value, 0.0f, 1.0f);
+  return sat_u32(sqrtf(color) * mx);
+}
+__device__ __forceinline__ uint32_t quantize_d(float value, uint32_t bits, float amp, Pcg& rng) {
+  float mx = (float)((1u << bits) - 1u);
+  float color = value != value ? 0.0f : clampf_(value, 0.0f, 1.0f);
+  if (amp > 0.0f && color > 0.001f) color = minf_(color + pcg_f32(rng) * amp, 1.0f);
+  return sat_u32(sqrtf(color) * mx);
+}
+
+struct PackView {
+  WinView v;
+  uint32_t cx0, cy0, W, H;
+  __device__ f3 pixel(uint64_t i) const {
+    int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
+    if (px < 2 || py < 2 || px >= (int32_t)W - 2 || py >= (int32_t)H - 2) return mk3(0.0f, 0.0f, 0.0f);
+    return v.get((int64_t)cx0 + px - 2, (int64_t)cy0 + py - 2);
+  }
+};
+
+__device__ __forceinline__ PackView pack_view(const DevTris& tr, const DevEntries& ent, const DevBounce& b, uint64_t e) {
+  uint32_t t = ent.tri[e], w = b.ent_window[e];
+  const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+  PackView pv;
+  pv.v = WinView{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+  pv.cx0 = tr.crop[4 * (size_t)t];
+  pv.cy0 = tr.crop[4 * (size_t)t + 1];
+  pv.W = tr.crop[4 * (size_t)t + 2] - pv.cx0 + 5;
+  pv.H = tr.crop[4 * (size_t)t + 3] - pv.cy0 + 5;
+  return pv;
+}
+
+__global__ void k_bounce_pack_words(DevTris tr, DevEntries ent, DevBounce b, const uint64_t* __restrict__ kept_entry,
+                                    const uint64_t* __restrict__ word_start, uint64_t nkept, uint64_t total,
+                                    const uint64_t* __restrict__ kept_out, const int32_t* __restrict__ comp_of_slot,
+                                    uint32_t colored, uint32_t* out) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t j = owner_of(word_start, nkept, c);
+    uint64_t wi = c - word_start[j];
+    uint64_t e = kept_entry[j];
+    PackView pv = pack_view(tr, ent, b, e);
+    uint32_t bits = comp_bits(comp_of_slot[ent.light[e]]);
+    uint64_t n = (uint64_t)pv.W * pv.H;
+    uint32_t word = 0;
+    if (colored) {
+      uint64_t pb = 3 * bits;
+      uint64_t lo = wi * 32, hi = lo + 32;
+      for (uint64_t i = lo / pb; i < n && i * pb < hi; ++i) {
+        f3 v = pv.pixel(i);
+        uint64_t pixel = quantize0(v.x, bits) | (quantize0(v.y, bits) << bits) | (quantize0(v.z, bits) << (2 * bits));
+        int64_t shift = (int64_t)(i * pb) - (int64_t)lo;
+        word |= shift >= 0 ? (uint32_t)(pixel << shift) : (uint32_t)(pixel >> (-shift));
+      }
+    } else {
+      uint32_t per = 32 / bits;
+      for (uint32_t k = 0; k < per; ++k) {
+        uint64_t i = wi * per + k;
+        if (i >= n) break;
+        word |= quantize0(pv.pixel(i).x, bits) << (k * bits);
+      }
+    }
+    out[kept_out[j] + wi] = word;
+  }
+}
+
+// Dithered precisions draw from one stream per (mesh, light, triangle) in pixel order.
+__global__ void k_bounce_pack_seq(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevEntries ent, DevBounce b,
+                                  const uint64_t* __restrict__ kept_entry, uint64_t nkept, const uint64_t* __restrict__ kept_out,
+                                  const uint64_t* __restrict__ word_start, const int32_t* __restrict__ comp_of_slot,
+                                  uint32_t colored, uint64_t seed, uint32_t* out) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nkept; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t e = kept_entry[j];
+    PackView pv = pack_view(tr, ent, b, e);
+    int32_t comp = comp_of_slot[ent.light[e]];
+    uint32_t bits = comp_bits(comp);
+    float amp = comp_amp(comp);
+    uint32_t t = ent.tri[e];
+    Pcg rng = stream4(seed, ms.id[tr.mesh[t]], lights[ent.light[e]].id, tr.local[t]);
+    uint64_t n = (uint64_t)pv.W * pv.H;
+    uint64_t words = word_start[j + 1] - word_start[j];
+    uint32_t* o = out + kept_out[j];
+    for (uint64_t k = 0; k < words; ++k) o[k] = 0;
+    if (colored) {
+      uint64_t pb = 3 * bits;
+      for (uint64_t i = 0; i < n; ++i) {
+        f3 v = pv.pixel(i);
+        uint32_t px = quantize_d(v.x, bits, amp, rng);
+        px |= quantize_d(v.y, bits, amp, rng) << bits;
+        px |= quantize_d(v.z, bits, amp, rng) << (2 * bits);
+        uint64_t off = i * pb, w = off / 32;
+        uint32_t shift = (uint32_t)(off % 32);
+        o[w] |= px << shift;
+        if (shift + pb > 32) o[w + 1] |= px >> (32 - shift);
+      }
+    } else {
+      uint32_t per = 32 / bits;
+      for (uint64_t i = 0; i < n; ++i) o[i / per] |= quantize_d(pv.pixel(i).x, bits, amp, rng) << ((i % per) * bits);
+    }
+  }
+}
+
+hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
+                          uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block) {
+  uint64_t total = 6ull * res * res * ncubes;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_photons, dim3(grid_for(total, block)), dim3(block), stack_bytes(scene, block), s, scene, occ, lights,
+                     cube_slot, ncubes, res, photons);
+  return hipGetLastError();
+}
+hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
+                               DevEntries ent, DevBounce b, uint64_t units) {
+  if (units == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bounce_owner, dim3(grid_for(units, kThreads / 64)), dim3(kThreads), 0, s, tris, lights, tx, tex_start,
+                     ent, b, units);
+  return hipGetLastError();
+}
+hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,
+                                const DevLight* lights, DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevBounce b,
+                                uint64_t units, BounceParams p, TrigTables trig, uint32_t block) {
+  if (units == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bounce_gather, dim3(grid_for(units, block / 64)), dim3(block), stack_bytes(scene, block), s, scene, occ,
+                     tris, meshes, lights, tx, tex_start, ent, b, units, p, trig);
+  return hipGetLastError();
+}
+hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,
+                                uint32_t nwin, uint64_t total, BlurWeights weights) {
+  if (total == 0) return hipSuccess;
+  for (int pass = 0; pass < 3; ++pass) {
+    hipLaunchKernelGGL(k_bounce_filter, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, b, win_mesh_res, win_start, nwin,
+                       total, pass, weights);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce) {
+  if (ent.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bounce_has, dim3(grid_for(ent.n, kThreads)), dim3(kThreads), 0, s, tris, ent, b, has_bounce);
+  return hipGetLastError();
+}
+hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
+                              DevBounce b, const uint64_t* kept_entry, const uint64_t* word_start, uint64_t nkept,
+                              uint64_t total_words, const uint64_t* kept_out, const int32_t* comp_of_slot,
+                              uint32_t colored, uint64_t seed, int dithered, uint32_t* out) {
+  if (nkept == 0) return hipSuccess;
+  if (dithered) {
+    hipLaunchKernelGGL(k_bounce_pack_seq, dim3(grid_for(nkept, 64)), dim3(64), 0, s, tris, meshes, lights, ent, b, kept_entry,
+                       nkept, kept_out, word_start, comp_of_slot, colored, seed, out);
+  } else {
+    if (total_words == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bounce_pack_words, dim3(grid_for(total_words, kThreads)), dim3(kThreads), 0, s, tris, ent, b,
+                       kept_entry, word_start, nkept, total_words, kept_out, comp_of_slot, colored, out);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace dl

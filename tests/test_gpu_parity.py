@@ -84,3 +84,43 @@ def test_box_room_bake_matches_oracle(ctx):
     want = oracle.bake(inp, opts, threads=0)
     assert_scenes_equal(got, want)
     assert got.stats["samples"] > 100_000
+
+
+import scene_cases  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(scene_cases.CASES))
+def test_light_shapes_and_edge_cases_match_oracle(ctx, name):
+    inp, opts = scene_cases.CASES[name]()
+    assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
+
+
+def _errors(bake, inp, opts):
+    try:
+        bake(inp, opts)
+    except abi.BakeError as e:
+        return e.code, str(e)
+    return None
+
+
+def test_errors_match_oracle(ctx):
+    from dynlight_hip.abi import BakeInput
+    a, b = scenes.floor(5, (0, 0, 0), 1.0), scenes.floor(5, (0, 1, 0), 1.0)
+    bad_mesh = scenes.floor(1, (0, 0, 0), 1.0)
+    bad_mesh.indices = np.array([0, 1, 9], np.uint32)
+    cases = [BakeInput([a, b], []), BakeInput([bad_mesh], []),
+             BakeInput([], [scenes.point_light(3, (0, 1, 0), -1.0)])]
+    for inp in cases:
+        want = _errors(oracle.bake, inp, abi.default_options())
+        assert want is not None
+        assert _errors(ctx.bake, inp, abi.default_options()) == want
+    o = abi.default_options()
+    o.max_lightmap_size = 1
+    assert _errors(ctx.bake, BakeInput([], []), o) == _errors(oracle.bake, BakeInput([], []), o)
+
+
+def test_empty_scenes_match_oracle(ctx):
+    from dynlight_hip.abi import BakeInput
+    for inp in (BakeInput([], []), BakeInput([scenes.floor(1, (0, 0, 0), 1.0)], []),
+                BakeInput([], [scenes.point_light(1, (0, 0, 0), 1.0)])):
+        assert_scenes_equal(ctx.bake(inp, abi.default_options()), oracle.bake(inp, abi.default_options()))
