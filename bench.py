@@ -3,7 +3,7 @@
 One step is one full bake (dl_bake: BVH build, association, texels, burial, shadow rays,
 post-passes, serialized words, distance cubes) of the interior scene: an 8x8 grid of 6x3x6 m
 rooms with door openings, seeded boxes and pillars to 200k triangles, 256 point/spot lights of
-range 4-12 m, 32 texels/m, 1-bit shadows.  A sample is one (texel, light) pair for which the
+range 4-12 m, 32 texels/m, 1-bit shadows and one colored bounce from every light.  A sample is one (texel, light) pair for which the
 reference's texel_shadow returns Some (src/bake/mesh_bake.rs:241-247).
 
 Prints one JSON line.  `roofline` prices the dominant kernel (the shadow-ray kernel) at its
@@ -26,23 +26,42 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(inp, opts, modulus=64):
+def cpu_baseline(inp, opts, full_samples, modulus=256, cube_lights=2):
+    """The CPU oracle on a bounded sample, extrapolated to the whole bake.
+
+    Meshes: receivers with splitmix64(id) % modulus == 0 baked against every occluder and light
+    (photon cubes at 64^2 there: the gather's cost does not depend on the cube resolution).
+    Photon cubes: `cube_lights` lights at the full resolution.  Whole-bake CPU time =
+    mesh time x (full samples / sample samples) + cube time x (bounce lights / cube_lights).
+    """
     import numpy as np
 
     import oracle
     from dynlight_hip import abi, scenes
 
+    threads = len(os.sched_getaffinity(0))
     mask = scenes.mesh_sample_mask(inp.mesh_array["id"], modulus)
     o = abi.dl_bake_options.from_buffer_copy(opts)
     o.dynamic_geometry = abi.DYNAMIC_LIGHTING_ONLY
-    threads = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
+    o.photon_cube_resolution = 64
     s = oracle.bake_receivers(inp, o, mask, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": s.stats["samples"] / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle, receivers splitmix64(id) % {modulus} == 0 ({int(np.sum(mask))} of "
-                      f"{len(mask)} meshes) against all 200k occluders and 256 lights, no distance cubes; "
-                      f"{s.stats['samples']} samples in {dt:.2f} s"}
+    meshes_s = oracle.last_timings()["meshes_ms"] / 1000.0
+    few = abi.BakeInput.from_arrays(
+        ids=inp.mesh_array["id"], positions=inp._positions, indices=inp._indices, uv1=inp._uv1, normals=inp._normals,
+        vertex_offsets=inp.vertex_offsets, index_offsets=inp.index_offsets, albedo=inp.mesh_array["albedo"],
+        lights=inp.lights[:cube_lights])
+    o2 = abi.dl_bake_options.from_buffer_copy(opts)
+    o2.dynamic_geometry = abi.DYNAMIC_LIGHTING_ONLY
+    oracle.bake_receivers(few, o2, np.zeros(inp.mesh_count, bool), threads=threads)
+    cubes_s = oracle.last_timings()["photon_cubes_ms"] / 1000.0
+    bounce_lights = sum(1 for l in inp.lights if l.illumination == abi.ILLUMINATION_SINGLE_BOUNCE)
+    est = meshes_s * full_samples / max(1, s.stats["samples"]) + cubes_s * bounce_lights / cube_lights
+    return {"value": full_samples / est, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle on {threads} threads: receivers splitmix64(id) % {modulus} == 0 "
+                      f"({int(np.sum(mask))} of {len(mask)} meshes, {s.stats['samples']} samples, "
+                      f"{s.stats['bounce_rays']} bounce rays) against all occluders and lights in "
+                      f"{meshes_s:.1f} s, plus {cube_lights} photon cubes at full resolution in {cubes_s:.1f} s; "
+                      f"extrapolated whole bake {est:.0f} s"}
 
 
 def main():
@@ -61,7 +80,6 @@ def main():
     from dynlight_hip import scenes
 
     inp, opts = scenes.interior(2)
-    opts.skip_bounce = 1
     ctx = dynlight_hip.Context(0)
     for _ in range(args.warmup):
         ctx.bake(inp, opts)
@@ -99,20 +117,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "interior: 8x8 rooms 6x3x6 m with doors, 200k triangles, 256 point/spot "
-                               "lights r 4-12 m, 32 texels/m, 1-bit shadows, bounce skipped",
+                               "lights r 4-12 m, 32 texels/m, 1-bit shadows + one colored bounce (32 samples, "
+                               "512^2 photon cubes)",
                    "meshes": inp.mesh_count, "triangles": inp.triangle_count,
                    "texels": scene.stats["texels"], "samples_per_step": samples,
-                   "rays_per_step": scene.stats["rays"]},
+                   "rays_per_step": scene.stats["rays"], "bounce_rays_per_step": scene.stats["bounce_rays"]},
         "trace_kernel_ms": trace_ms / args.steps,
         "trace_samples_per_s": samples / kernel_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": b_alg, "node_fetches": nodes, "triangle_tests": tris},
-        "stages_ms": {k: counts[k] for k in ("upload_ms", "texels_ms", "trace_ms", "post_ms", "maps_ms",
-                                             "download_ms", "total_ms")},
+        "stages_ms": {k: tm[k] for k in ("upload_ms", "texels_ms", "trace_ms", "bounce_ms", "post_ms", "maps_ms",
+                                         "download_ms", "total_ms")},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(inp, opts)
+        out["cpu_baseline"] = cpu_baseline(inp, opts, samples)
+        out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -227,6 +227,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   sc.tris = reinterpret_cast<const float4*>(d_occ.p);
   sc.ntris = nocc;
   sc.stack = 3 * std::max(1u, bvh.depth) + 1;
+  sc.stack16 = bvh.nodes.size() < 65536 ? 1u : 0u;
   sc.overflow = d_overflow.p;
   sc.counters = nullptr;
   DevOcc occ{nullptr, nullptr, nullptr};

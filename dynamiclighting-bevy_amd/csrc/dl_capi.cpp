@@ -19,8 +19,9 @@ int set_err(dl_ctx* ctx, int code, const std::string& msg) {
 int upload_trig(dl_ctx* ctx) {
   using namespace dl;
   const TrigHost& t = ctx->trig_host;
-  // six arrays, each followed by one spare word
-  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() + 6;
+  // seven arrays, each followed by one spare word
+  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() +
+                 t.flags.size() + 7;
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
@@ -38,6 +39,7 @@ int upload_trig(dl_ctx* ctx) {
   ctx->trig.cos.bucket = put(t.cos_bucket);
   ctx->trig.cos.keys = put(t.cos_keys);
   ctx->trig.cos.vals = put(t.cos_vals);
+  ctx->trig.flags = put(t.flags);
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
 }
@@ -185,6 +187,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   sc.tris = reinterpret_cast<const float4*>(t.p);
   sc.ntris = tri_count;
   sc.stack = 3 * std::max(1u, bvh.depth) + 1;
+  sc.stack16 = bvh.nodes.size() < 65536 ? 1u : 0u;
   sc.overflow = ovf.p;
   sc.counters = nullptr;
   if ((e = launch_ray_queries(ctx->stream, sc, r.p, ray_count, closest, tout.p, idx.p))) return fail(e);
@@ -226,6 +229,7 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
   TrigTables t;
   t.sin = {host.sin_bucket.data(), host.sin_keys.data(), host.sin_vals.data()};
   t.cos = {host.cos_bucket.data(), host.cos_keys.data(), host.cos_vals.data()};
+  t.flags = host.flags.data();
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);
     cos_out[i] = lcosf(t, x[i]);

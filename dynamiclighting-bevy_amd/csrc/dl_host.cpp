@@ -66,6 +66,12 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
   };
   gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_bucket);
   gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_bucket);
+  out.flags.assign((kDomCount + 31) / 32, 0u);
+  for (const auto* keys : {&out.sin_keys, &out.cos_keys})
+    for (uint32_t k : *keys) {
+      uint32_t i = trig_domain_index(k);
+      if (i != ~0u) out.flags[i >> 5] |= 1u << (i & 31u);
+    }
 }
 
 // ---- light BVH: SAH split at light positions, leaves of at most two (light_bvh.rs:59-189) --------

@@ -39,6 +39,7 @@ struct DevScene {
   const float4* tris;
   uint32_t ntris;
   uint32_t stack;  // LDS stack entries per lane
+  uint32_t stack16;  // 1: 16-bit stack entries (fewer than 65536 nodes), half the LDS per wave
   uint32_t* overflow;
   unsigned long long* counters;  // null, or [node fetches, triangle tests] for instrumented runs
 };

@@ -47,6 +47,9 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
   uint32_t sp = 0;
   int32_t node = 0;
   uint32_t nvisit = 0, ntest = 0;
+  // `stack` is lds + threadIdx.x; in 16-bit mode the same LDS holds half-word entries.
+  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
+  const bool s16 = sc.stack16 != 0;
   for (;;) {
     nvisit++;
     const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
@@ -104,13 +107,15 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
       } else if (next < 0) {
         next = c;
         next_t = tn;
-      } else if (tn < next_t) {
-        stack[sp * stride] = (uint32_t)next;
-        sp++;
-        next = c;
-        next_t = tn;
       } else {
-        stack[sp * stride] = (uint32_t)c;
+        uint32_t push = (uint32_t)c;
+        if (tn < next_t) {
+          push = (uint32_t)next;
+          next = c;
+          next_t = tn;
+        }
+        if (s16) stack16[sp * stride] = (uint16_t)push;
+        else stack[sp * stride] = push;
         sp++;
       }
     }
@@ -121,7 +126,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
     }
     if (sp == 0) break;
     sp--;
-    node = (int32_t)stack[sp * stride];
+    node = s16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
   }
   if (sc.counters) {
     atomicAdd(&sc.counters[0], (unsigned long long)nvisit);
@@ -812,7 +817,9 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u <<
   if (g == 0) g = 1;
   return (uint32_t)(g < cap ? g : cap);
 }
-static inline size_t stack_bytes(const DevScene& sc, uint32_t block) { return (size_t)(sc.stack + 4) * block * 4; }
+static inline size_t stack_bytes(const DevScene& sc, uint32_t block) {
+  return (size_t)(sc.stack + 4) * block * (sc.stack16 ? 2 : 4);
+}
 
 hipError_t launch_setup(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, uint32_t nlights,
                         uint32_t local_lights, uint64_t* light_count, uint64_t* trace_count, uint64_t* touched_count) {

@@ -60,7 +60,21 @@ struct TrigTable {
 };
 struct TrigTables {
   TrigTable sin, cos;
+  // One bit per float of the table domain, set when sin or cos of it is an exception; inputs
+  // whose bit is clear skip the search.  Domain: bits [kDomPosLo, kDomPosHi] then
+  // [kDomNegLo, kDomNegHi].
+  const uint32_t* flags;
 };
+static constexpr uint32_t kDomPosLo = 0x37800000u, kDomPosHi = 0x41000000u;  // [2^-16, 8]
+static constexpr uint32_t kDomNegLo = 0xB7800000u, kDomNegHi = 0xBF4CCCCDu;  // [-0.8, -2^-16]
+static constexpr uint32_t kDomPosCount = kDomPosHi - kDomPosLo + 1;
+static constexpr uint32_t kDomCount = kDomPosCount + (kDomNegHi - kDomNegLo + 1);
+// Index of a float in the domain, or ~0u outside it.
+DLI uint32_t trig_domain_index(uint32_t bits) {
+  if (bits >= kDomPosLo && bits <= kDomPosHi) return bits - kDomPosLo;
+  if (bits >= kDomNegLo && bits <= kDomNegHi) return kDomPosCount + (bits - kDomNegLo);
+  return ~0u;
+}
 
 DLI uint32_t f2u(float f) {
   uint32_t u;
@@ -86,7 +100,18 @@ DLI float trig_lookup(const TrigTable& t, float x, float computed) {
   }
   return computed;
 }
-DLI float lsinf(const TrigTables& t, float x) { return trig_lookup(t.sin, x, trig_core(x, 0)); }
-DLI float lcosf(const TrigTables& t, float x) { return trig_lookup(t.cos, x, trig_core(x, 1)); }
+// Exceptions exist only inside the domain (outside it the tables are empty).
+DLI bool trig_flagged(const TrigTables& t, float x) {
+  uint32_t i = trig_domain_index(f2u(x));
+  return i != ~0u && ((t.flags[i >> 5] >> (i & 31u)) & 1u);
+}
+DLI float lsinf(const TrigTables& t, float x) {
+  float c = trig_core(x, 0);
+  return trig_flagged(t, x) ? trig_lookup(t.sin, x, c) : c;
+}
+DLI float lcosf(const TrigTables& t, float x) {
+  float c = trig_core(x, 1);
+  return trig_flagged(t, x) ? trig_lookup(t.cos, x, c) : c;
+}
 
 }  // namespace dl

@@ -89,6 +89,14 @@ def load() -> C.CDLL:
     return lib
 
 
+def last_timings() -> dict:
+    """Stage wall times (ms) of the last oracle bake on this thread."""
+    lib = load()
+    out = (C.c_double * 3)()
+    lib.dlo_last_timings(out)
+    return {"photon_cubes_ms": out[0], "meshes_ms": out[1], "distance_cubes_ms": out[2]}
+
+
 def bake_receivers(inp: "abi.BakeInput", options: "abi.dl_bake_options", receivers, threads: int = 0):
     """Bakes only the meshes where `receivers` is true; every mesh still occludes."""
     import numpy as np

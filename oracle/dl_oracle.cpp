@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1979,6 +1980,7 @@ struct dlo_scene {
 };
 
 static thread_local std::string g_err;
+static thread_local double g_timing[3];  // photon cubes, meshes, distance cubes (ms) of the last bake
 static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -2046,6 +2048,9 @@ static int validate_light(const dl_light& l) {
 extern "C" {
 
 const char* dlo_last_error(void) { return g_err.c_str(); }
+void dlo_last_timings(double* out3) {
+  for (int i = 0; i < 3; ++i) out3[i] = g_timing[i];
+}
 
 static int bake_impl(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads, uint32_t brute_force,
                      const uint8_t* receivers, dlo_scene** out);
@@ -2119,6 +2124,7 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
     }
   }
   if (!sc->occ.brute) sc->occ.build_bvh();
+  auto t_cubes = std::chrono::steady_clock::now();
   // photon cubes (bounce.rs:126-151)
   sc->cubes.resize(sc->lights.size());
   for (size_t l = 0; l < sc->lights.size(); ++l) {
@@ -2127,6 +2133,7 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
     sc->cubes[l].reset(new PhotonCube());
     build_photon_cube(*sc, L, opt.photon_cube_resolution, *sc->cubes[l]);
   }
+  auto t_meshes = std::chrono::steady_clock::now();
   std::vector<Prepared> prepared;
   for (auto* m : meshes)
     if (!receivers || receivers[m - input->meshes]) prepared.push_back(prepare(m, opt));
@@ -2137,7 +2144,14 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
   } else {
     for (size_t i = 0; i < prepared.size(); ++i) res->meshes[i] = bake_mesh(*sc, prepared[i]);
   }
+  auto t_maps = std::chrono::steady_clock::now();
   if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) res->distance_cubes = bake_distance_cubes(*sc);
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  g_timing[0] = ms(t_cubes, t_meshes);
+  g_timing[1] = ms(t_meshes, t_maps);
+  g_timing[2] = ms(t_maps, std::chrono::steady_clock::now());
   // settings, lights, stats (baker.rs:154-200)
   res->settings.bounce = sc->bounce_in_scene;
   res->settings.colored_bounce = sc->bounce_in_scene && opt.colored_bounce;

@@ -29,6 +29,8 @@ int dlo_bake(const dl_bake_input* input, const dl_bake_options* options, uint32_
 int dlo_bake_receivers(const dl_bake_input* input, const dl_bake_options* options, uint32_t threads,
                        const uint8_t* receivers, dlo_scene** out);
 const char* dlo_last_error(void);
+/* Wall times (ms) of the last bake on this thread: photon cubes, meshes, distance cubes. */
+void dlo_last_timings(double* out3);
 void dlo_scene_free(dlo_scene* scene);
 void dlo_scene_settings(const dlo_scene* scene, dl_baked_settings* out);
 void dlo_scene_stats(const dlo_scene* scene, dl_bake_stats* out);
