@@ -6,10 +6,14 @@ rooms with door openings, seeded boxes and pillars to 200k triangles, 256 point/
 range 4-12 m, 32 texels/m, 1-bit shadows and one colored bounce from every light.  A sample is one (texel, light) pair for which the
 reference's texel_shadow returns Some (src/bake/mesh_bake.rs:241-247).
 
-Prints one JSON line.  `roofline` prices the dominant kernel (the shadow-ray kernel) at its
-algorithmic bytes: per sample, 128 B per BVH node fetched plus 48 B per triangle tested plus the
-mask bits, with node and triangle counts from an instrumented (untimed) bake.  `cpu_baseline`
-times the CPU oracle on a 1/64 receiver sample of the same scene (every mesh still occludes).
+Prints one JSON line.  `roofline` prices the dominant kernel, the bounce gather (one launch per
+mesh batch), at its algorithmic bytes: per visibility ray, 128 B per 4-wide BVH node fetched plus
+48 B per triangle tested plus the 12-byte photon it samples, with node and triangle counts from an
+instrumented (untimed) bake, divided by the kernel's average launch time from HIP events on the
+bake stream.  `traffic` is the rocprofv3-measured HBM bytes per launch recorded in
+profiles/interior_bake_kernels.md.  `trace_roofline` does the same for the direct shadow-ray
+kernel.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample of the same scene (every
+mesh still occludes) and extrapolates to the whole bake.
 """
 import argparse
 import json
@@ -24,6 +28,24 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "interior_bake_kernels.md")):
+    """HBM bytes per dispatch of `kernel` as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by
+    a plain-copy calibration) and recorded in the profile summary, or None when absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(inp, opts, full_samples, modulus=256, cube_lights=2):
@@ -70,7 +92,19 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the bake shards receiver meshes across GPUs only by design, see DESIGN.md)", flush=True)
@@ -81,19 +115,24 @@ def main():
 
     inp, opts = scenes.interior(2)
     ctx = dynlight_hip.Context(0)
-    for _ in range(args.warmup):
+    log(f"interior scene: {inp.mesh_count} meshes, {inp.triangle_count} triangles, {len(inp.lights)} lights")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
         ctx.bake(inp, opts)
+        log(f"warmup bake {w + 1}/{args.warmup}: {time.perf_counter() - t0:.2f} s")
     # instrumented bake: node and triangle counts of the shadow-ray kernel (untimed)
     ctx.set_counting(True)
     scene = ctx.bake(inp, opts)
     counts = ctx.timings()
     ctx.set_counting(False)
+    log("instrumented bake done")
     samples = scene.stats["samples"]
     t_total, trace_ms = 0.0, 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for step in range(args.steps):
         s = ctx.bake(inp, opts)
         tm = ctx.timings()
+        log(f"timed bake {step + 1}/{args.steps}: {tm['total_ms'] / 1000.0:.2f} s")
         trace_ms += tm["trace_kernel_ms"]
         assert s.stats["samples"] == samples
     t_total = time.perf_counter() - t0
@@ -103,6 +142,14 @@ def main():
     b_alg = 128.0 * nodes + 48.0 * tris + samples / 8.0
     kernel_s = trace_ms / args.steps / 1000.0
     achieved = b_alg / kernel_s / 1e9
+    # the bounce-gather kernel dominates the bake: its algorithmic bytes per launch are the BVH
+    # nodes and triangles its visibility rays fetch plus one 12-byte photon per ray
+    bnodes, btris = counts["bounce_node_visits"], counts["bounce_tri_tests"]
+    bounce_rays = scene.stats["bounce_rays"]
+    launches = max(1, tm["bounce_launches"])
+    g_alg = (128.0 * bnodes + 48.0 * btris + 12.0 * bounce_rays) / launches
+    g_launch_s = tm["bounce_kernel_ms"] / launches / 1000.0
+    g_achieved = g_alg / g_launch_s / 1e9 if g_launch_s > 0 else 0.0
     out = {
         "metric": "texel-light visibility samples/s (1 GPU)",
         "value": value,
@@ -124,13 +171,19 @@ def main():
                    "rays_per_step": scene.stats["rays"], "bounce_rays_per_step": scene.stats["bounce_rays"]},
         "trace_kernel_ms": trace_ms / args.steps,
         "trace_samples_per_s": samples / kernel_s,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": b_alg, "node_fetches": nodes, "triangle_tests": tris},
+        "roofline": {"kernel": "k_bounce_gather", "bound": "hbm", "achieved": g_achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": g_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_bounce_gather"),
+                     "alg_bytes_per_launch": g_alg, "launch_ms": g_launch_s * 1000.0, "launches_per_step": launches,
+                     "node_fetches_per_step": bnodes, "triangle_tests_per_step": btris},
+        "trace_roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_step": b_alg,
+                           "traffic": measured_traffic("k_trace"),
+                           "node_fetches": nodes, "triangle_tests": tris},
         "stages_ms": {k: tm[k] for k in ("upload_ms", "texels_ms", "trace_ms", "bounce_ms", "post_ms", "maps_ms",
                                          "download_ms", "total_ms")},
     }
     if not args.no_cpu:
+        log("cpu baseline (oracle on a receiver sample)")
         out["cpu_baseline"] = cpu_baseline(inp, opts, samples)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)

@@ -415,7 +415,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   std::unique_ptr<dl_baked_scene> scene(new dl_baked_scene());
   scene->meshes.resize(nm);
   DBuf<unsigned long long> d_counters;
-  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 2));
+  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 4));
+  float bounce_kernel_ms = 0.0f;
   const BlurWeights blur = blur_weights();
   const uint32_t stride = bounce_in_scene ? 3 : 2;
   float trace_kernel_ms = 0.0f;
@@ -605,8 +606,17 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bp.cube_res = cres;
       bp.photons = d_photons.p;
       HIP_TRY(launch_bounce_owner(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back()));
-      HIP_TRY(launch_bounce_gather(s, sc, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
+      DevScene scb = sc;
+      scb.counters = ctx->counting ? d_counters.p + 2 : nullptr;
+      HIP_TRY(hipEventRecord(ev0, s));
+      HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
                                    ctx->trace_block));
+      HIP_TRY(hipEventRecord(ev1, s));
+      HIP_TRY(hipEventSynchronize(ev1));
+      float gms = 0.0f;
+      (void)hipEventElapsedTime(&gms, ev0, ev1);
+      bounce_kernel_ms += gms;
+      ctx->timings.bounce_launches++;
       HIP_TRY(launch_bounce_filter(s, bb, d_win_res.p, d_win_start.p, nbw, win_off[nbw], blur));
       HIP_TRY(d_has.alloc(nbe + 1));
       HIP_TRY(launch_bounce_has(s, tr, ent, bb, d_has.p));
@@ -749,11 +759,14 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   ctx->timings.trace_kernel_ms = trace_kernel_ms;
   ctx->timings.bounce_ms = bounce_ms;
   ctx->timings.post_ms = post_ms;
+  ctx->timings.bounce_kernel_ms = bounce_kernel_ms;
   if (ctx->counting) {
-    unsigned long long c[2] = {0, 0};
-    HIP_TRY(hipMemcpy(c, d_counters.p, 16, hipMemcpyDeviceToHost));
+    unsigned long long c[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpy(c, d_counters.p, 32, hipMemcpyDeviceToHost));
     ctx->timings.trace_node_visits = c[0];
     ctx->timings.trace_tri_tests = c[1];
+    ctx->timings.bounce_node_visits = c[2];
+    ctx->timings.bounce_tri_tests = c[3];
   }
 
   // ---- distance cubes (distance_cubes.rs:29-72) -----------------------------------------------

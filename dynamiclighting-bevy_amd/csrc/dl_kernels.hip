@@ -37,21 +37,26 @@ struct Hit {
 
 // One ray against the occluder BVH.  ANY: true when some occluder has 0 <= t <= tmax (t finite).
 // !ANY: the lowest (t, occluder index) hit with t <= tmax in `hit`.
+// Node fetches and triangle tests of one lane, summed in registers and flushed per wave when a
+// kernel runs instrumented (DevScene::counters).
+struct Tally {
+  uint32_t nodes = 0, tris = 0;
+};
+
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
-                                         float tmax, Hit* hit) {
+                                         float tmax, Hit* hit, Tally& tl) {
   if (sc.ntris == 0) return false;
   const float ix = inv_dir(d.x), iy = inv_dir(d.y), iz = inv_dir(d.z);
   bool have = false;
   float bt = tmax;
   uint32_t sp = 0;
   int32_t node = 0;
-  uint32_t nvisit = 0, ntest = 0;
   // `stack` is lds + threadIdx.x; in 16-bit mode the same LDS holds half-word entries.
   uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
   const bool s16 = sc.stack16 != 0;
   for (;;) {
-    nvisit++;
+    tl.nodes++;
     const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
     const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
     const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
@@ -83,16 +88,10 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
         for (uint32_t i = first; i < first + count; ++i) {
           const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
           float t, u, vv;
-          ntest++;
+          tl.tris++;
           if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
           if (!finite_(t)) continue;
-          if (ANY) {
-            if (sc.counters) {
-              atomicAdd(&sc.counters[0], (unsigned long long)nvisit);
-              atomicAdd(&sc.counters[1], (unsigned long long)ntest);
-            }
-            return true;
-          }
+          if (ANY) return true;
           const uint32_t occ = __float_as_uint(a.w);
           if (!have || t < bt || (t == bt && occ < hit->occ)) {
             have = true;
@@ -127,10 +126,6 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
     if (sp == 0) break;
     sp--;
     node = s16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
-  }
-  if (sc.counters) {
-    atomicAdd(&sc.counters[0], (unsigned long long)nvisit);
-    atomicAdd(&sc.counters[1], (unsigned long long)ntest);
   }
   return have;
 }
@@ -313,6 +308,7 @@ __global__ void k_burial(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, ui
                          TrigTables trig, uint8_t* buried, unsigned long long* kept) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntex; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t t = tx.tri[i];
     f3 normal = ld3(tr.face + 3 * (size_t)t);
@@ -333,7 +329,7 @@ __global__ void k_burial(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, ui
       float phi = (((float)k + turn) * kTau) / 8.0f;
       f3 dir = add(mul(add(mul(tangent, lcosf(trig, phi)), mul(bitangent, lsinf(trig, phi))), sin_theta), mul(normal, cos_theta));
       Hit h;
-      bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
+      bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
       if (bf) back++;
       else front++;
       if (back >= 5) { is_buried = true; break; }
@@ -398,7 +394,7 @@ __device__ __forceinline__ bool share_result(const Share& s, float* fraction) {
 
 // area.rs:61-90 with adaptive_share area.rs:98-126 and toward area.rs:130-146
 __device__ bool area_visibility(const DevScene& sc, uint32_t* stack, uint32_t stride, const DevLight& L, f3 origin, f3 pos,
-                                f3 normal, Pcg& rng, uint32_t samples, uint32_t minimum, float* fraction, uint32_t* rays) {
+                                f3 normal, Pcg& rng, uint32_t samples, uint32_t minimum, float* fraction, uint32_t* rays, Tally& tl) {
   f3 c = ld3(L.pos), ax = ld3(L.ax), ay = ld3(L.ay), fw = ld3(L.af);
   f3 local = sub(pos, c);
   float npx = clampf_(dot(local, ax), -L.hx, L.hx), npy = clampf_(dot(local, ay), -L.hy, L.hy);
@@ -421,7 +417,7 @@ __device__ bool area_visibility(const DevScene& sc, uint32_t* stack, uint32_t st
     float weight = (receiver * emitter) / dsq;
     nr++;
     Hit h;
-    bool lit = !traverse<true>(sc, stack, stride, origin, dir, maxf_(distance - kEndGap, 0.0f), &h);
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, maxf_(distance - kEndGap, 0.0f), &h, tl);
     if (!probe) {
       s.total += weight;
       s.lit += lit ? weight : 0.0f;
@@ -454,12 +450,12 @@ __device__ bool area_visibility(const DevScene& sc, uint32_t* stack, uint32_t st
 // sun.rs:21-56
 __device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t stride, const TrigTables& trig,
                                const DevLight& L, f3 origin, f3 normal, Pcg& rng, uint32_t samples, uint32_t minimum,
-                               float* fraction, uint32_t* rays) {
+                               float* fraction, uint32_t* rays, Tally& tl) {
   f3 toward = ld3(L.toward);
   Hit h;
   if (L.srad <= 0.0f) {
     if (dot(normal, toward) <= kNdotlTolerance) return false;
-    *fraction = traverse<true>(sc, stack, stride, origin, toward, INFINITY, &h) ? 0.0f : 1.0f;
+    *fraction = traverse<true>(sc, stack, stride, origin, toward, INFINITY, &h, tl) ? 0.0f : 1.0f;
     *rays = 1;
     return true;
   }
@@ -472,7 +468,7 @@ __device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t str
     float receiver = dot(normal, dir) - kNdotlTolerance;
     if (receiver <= 0.0f) return;
     nr++;
-    bool lit = !traverse<true>(sc, stack, stride, origin, dir, INFINITY, &h);
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, INFINITY, &h, tl);
     if (!probe) {
       s.total += receiver;
       s.lit += lit ? receiver : 0.0f;
@@ -538,6 +534,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
                         const uint64_t* __restrict__ unit_start, uint64_t units, TraceParams p, TrigTables trig) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = blockDim.x >> 6;
   for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
@@ -576,7 +573,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
             if (dot(nrm, dir) > kNdotlTolerance) {
               some = true;
               rays = 1;
-              level = traverse<true>(sc, stack, blockDim.x, origin, dir, distance, &h) ? 0u : 1u;
+              level = traverse<true>(sc, stack, blockDim.x, origin, dir, distance, &h, tl) ? 0u : 1u;
             }
           }
         }
@@ -584,9 +581,9 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
         Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
         float fraction = 0.0f;
         if (L.shape == 1 || L.shape == 2) {
-          some = area_visibility(sc, stack, blockDim.x, L, origin, pos, nrm, rng, p.area_samples, p.area_min, &fraction, &rays);
+          some = area_visibility(sc, stack, blockDim.x, L, origin, pos, nrm, rng, p.area_samples, p.area_min, &fraction, &rays, tl);
         } else if (L.shape == 3) {
-          some = sun_visibility(sc, stack, blockDim.x, trig, L, origin, nrm, rng, p.area_samples, p.area_min, &fraction, &rays);
+          some = sun_visibility(sc, stack, blockDim.x, trig, L, origin, nrm, rng, p.area_samples, p.area_min, &fraction, &rays, tl);
         } else {
           f2 offset;
           offset.x = pcg_f32(rng);
@@ -594,7 +591,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
           uint32_t open = 0;
           for (uint32_t taken = 0; taken < p.sky_samples; ++taken) {
             f3 dir = cosine_dir(trig, nrm, spread_pt(taken, offset, true));
-            open += traverse<true>(sc, stack, blockDim.x, origin, dir, INFINITY, &h) ? 0u : 1u;
+            open += traverse<true>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) ? 0u : 1u;
           }
           fraction = (float)open / (float)(p.sky_samples > 1 ? p.sky_samples : 1u);
           rays = p.sky_samples;
@@ -616,6 +613,11 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
     uint32_t m = tr.mesh[t];
     wave_add(&ms.rays[m], rays);
     wave_add(&ms.samples[m], some_n);
+    if (sc.counters) {
+      wave_add(&sc.counters[0], tl.nodes);
+      wave_add(&sc.counters[1], tl.tris);
+      tl = Tally{};
+    }
   }
 }
 
@@ -769,6 +771,7 @@ __global__ void k_serialize(DevTris tr, const uint64_t* __restrict__ mesh_base, 
 __global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ lights, uint32_t nlocal, uint32_t* out) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   const uint32_t res = 64, per_face = res * res / 2;
   uint64_t total = (uint64_t)nlocal * 6 * per_face;
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
@@ -782,7 +785,7 @@ __global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ light
       uint32_t texel = pair * 2 + k;
       f3 dir = cube_texel_dir(face, texel % res, texel / res, res);
       Hit h;
-      d[k] = traverse<false>(sc, stack, blockDim.x, o, dir, L.radius, &h) ? h.t : L.radius;
+      d[k] = traverse<false>(sc, stack, blockDim.x, o, dir, L.radius, &h, tl) ? h.t : L.radius;
     }
     out[c] = (f32_to_f16_bits(d[1]) << 16) | f32_to_f16_bits(d[0]);
   }
@@ -791,15 +794,16 @@ __global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ light
 __global__ void k_ray_queries(DevScene sc, const float* __restrict__ rays, uint64_t n, int closest, float* t_out, uint32_t* idx_out) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const float* r = rays + 8 * i;
     Hit h;
     if (closest) {
-      bool hit = traverse<false>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h);
+      bool hit = traverse<false>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h, tl);
       t_out[i] = hit ? h.t : -1.0f;
       idx_out[i] = hit ? h.occ : 0xffffffffu;
     } else {
-      idx_out[i] = traverse<true>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h) ? 1u : 0u;
+      idx_out[i] = traverse<true>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h, tl) ? 1u : 0u;
     }
   }
 }
@@ -954,6 +958,7 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
                           uint32_t ncubes, uint32_t res, uint32_t* photons) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   const uint64_t per = 6ull * res * res;
   const uint64_t total = per * ncubes;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -967,7 +972,7 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
     f3 dir = cube_texel_dir(face, x, y, res);
     Hit h;
     uint32_t w0, w1 = 0, w2 = 0;
-    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h)) {
+    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h, tl)) {
       w0 = __float_as_uint(L.radius);
     } else {
       f3 normal = surface_normal(sc, occ, h);
@@ -1081,6 +1086,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
                                 BounceParams p, TrigTables trig) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
   const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
   for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
     BounceUnit u = bounce_unit(b, ent, tex_start, unit, lane);
@@ -1123,7 +1129,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
             if (is_zero(pn) || distance <= 0.0f) continue;
             f3 d = divs(to_photon, distance);
             rays++;
-            if (!traverse<true>(sc, stack, blockDim.x, origin, d, distance, &h)) {
+            if (!traverse<true>(sc, stack, blockDim.x, origin, d, distance, &h, tl)) {
               float weight = maxf_(dot(normal, d), 0.0f) * maxf_(dot(pn, neg(d)), 0.0f);
               f3 alb = p.colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f,
                                        (float)((w2 >> 16) & 0xffu) / 255.0f)
@@ -1142,7 +1148,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
         for (uint32_t k = 0; k < samples; ++k) {
           f3 direction = cosine_dir(trig, normal, spread_pt(k, offset, true));
           rays++;
-          if (!traverse<false>(sc, stack, blockDim.x, origin, direction, INFINITY, &h)) continue;
+          if (!traverse<false>(sc, stack, blockDim.x, origin, direction, INFINITY, &h, tl)) continue;
           f3 face = tri_face_normal(sc, h.pos);
           if (dot(face, direction) >= 0.0f) continue;
           float lit = dot(surface_normal(sc, occ, h), toward);
@@ -1150,7 +1156,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
           f3 point = add(add(origin, mul(direction, h.t)), mul(face, kRayOffset));
           rays++;
           Hit h2;
-          if (!traverse<true>(sc, stack, blockDim.x, point, toward, INFINITY, &h2)) {
+          if (!traverse<true>(sc, stack, blockDim.x, point, toward, INFINITY, &h2, tl)) {
             f3 alb = mk3(1.0f, 1.0f, 1.0f);
             if (p.colored) {
               const float* a = occ.albedo + 3 * (size_t)h.occ;
@@ -1175,6 +1181,11 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
       }
     }
     wave_add(&b.rays[m], rays);
+    if (sc.counters) {
+      wave_add(&sc.counters[0], tl.nodes);
+      wave_add(&sc.counters[1], tl.tris);
+      tl = Tally{};
+    }
   }
 }
 

@@ -144,7 +144,9 @@ class dl_timings(C.Structure):
                 ("trace_kernel_ms", C.c_double), ("trace_launches", C.c_uint64),
                 ("trace_rays", C.c_uint64), ("bvh_nodes", C.c_uint64),
                 ("occluder_triangles", C.c_uint64), ("trace_node_visits", C.c_uint64),
-                ("trace_tri_tests", C.c_uint64)]
+                ("trace_tri_tests", C.c_uint64), ("bounce_kernel_ms", C.c_double),
+                ("bounce_node_visits", C.c_uint64), ("bounce_tri_tests", C.c_uint64),
+                ("bounce_launches", C.c_uint64)]
 
 
 def _np_dtype(struct) -> np.dtype:
