@@ -1,0 +1,85 @@
+"""The C ABI without a GPU: the library loads, exports every function the headers declare, and the
+ctypes mirror of each struct has the C layout (sizes and field offsets from a gcc-compiled probe).
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+from dynlight_hip import abi
+
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("dynlight.h", "dynlight_testing.h")]
+LIB = os.path.join(ROOT, "dynamiclighting-bevy_amd", "dynlight_hip", "libdynlight_hip.so")
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**\s*(dl_[a-z0-9_]+)\s*\(", re.M)
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names += DECL.findall(text)
+    return sorted(set(names))
+
+
+def test_headers_declare_the_bake_entry_points():
+    names = declared_functions()
+    for required in ("dl_ctx_create", "dl_ctx_destroy", "dl_bake", "dl_baked_scene_free", "dl_last_error",
+                     "dl_default_options", "dl_scene_mesh", "dl_scene_stats"):
+        assert required in names
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(LIB):
+        pytest.skip("libdynlight_hip.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_default_options_match_the_reference_defaults():
+    """BakeOptions::default (input.rs:166-171) over BakeSettings::default (settings.rs:67-81);
+    dl_default_options touches no device."""
+    if not os.path.exists(LIB):
+        pytest.skip("libdynlight_hip.so is not built")
+    lib = ctypes.CDLL(LIB)
+    o = abi.dl_bake_options()
+    lib.dl_default_options(ctypes.byref(o))
+    assert o.photon_cube_resolution == 512
+    assert o.seed == 0
+    assert o.skip_bounce == 0
+    assert o.max_lightmap_size == 2048
+    assert o.dynamic_geometry == abi.DYNAMIC_ANGULAR
+
+
+STRUCTS = ["dl_alpha_texture", "dl_mesh", "dl_light", "dl_bake_input", "dl_bake_options", "dl_progress",
+           "dl_bake_stats", "dl_light_bvh_node", "dl_baked_light", "dl_baked_mesh_view", "dl_sun_map_view",
+           "dl_sky_probes_view", "dl_baked_settings", "dl_timings"]
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dynlight.h"', 'int main(void) {']
+    for s in STRUCTS:
+        cls = getattr(abi, s)
+        lines.append(f'  printf("{s} #sizeof %zu\\n", sizeof({s}));')
+        for name, _ in cls._fields_:
+            lines.append(f'  printf("{s} {name} %zu\\n", offsetof({s}, {name}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        s, field, value = line.split()
+        cls = getattr(abi, s)
+        if field == "#sizeof":
+            assert ctypes.sizeof(cls) == int(value), s
+        else:
+            assert getattr(cls, field).offset == int(value), f"{s}.{field}"
+        seen += 1
+    assert seen > len(STRUCTS) * 2
