@@ -385,7 +385,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   }
 
   // ---- batches of meshes under the HBM budget ---------------------------------------------------
+  // Default: half of the device memory still free after the scene-level buffers.
   uint64_t budget = 48ull << 30;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > (2ull << 30)) budget = free_b / 2;
+  }
   if (const char* env = std::getenv("DL_BATCH_BYTES")) budget = std::max<uint64_t>(1ull << 20, strtoull(env, nullptr, 10));
   std::vector<Batch> batches;
   {
@@ -398,6 +403,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         const uint32_t* c = &h_crop[4 * (size_t)t];
         uint64_t padded = (uint64_t)(c[2] - c[0] + 5) * (c[3] - c[1] + 5);
         est += (h_ent_start[t + 1] - h_ent_start[t]) * (48 + padded * (bounce_in_scene ? 4 : 1) / 4);
+        if (bounce_in_scene) est += (h_ent_start[t + 1] - h_ent_start[t]) * h_tcc[t] * 8;  // gather jobs
       }
       for (uint32_t r = mesh_reg_first[i]; r < mesh_reg_first[i + 1]; ++r) est += reg_words[r] * 4;
       for (uint32_t w = mesh_win_first[i]; w < mesh_win_first[i + 1]; ++w) est += win_area[w] * 24;
@@ -572,15 +578,28 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       for (uint32_t w = 0; w < nbw; ++w) win_off[w + 1] = win_off[w] + win_area[w0 + w];
       std::vector<uint32_t> b_win_rect(win_rect.begin() + 4 * (size_t)w0, win_rect.begin() + 4 * (size_t)w1);
       std::vector<uint32_t> b_win_res(win_res.begin() + w0, win_res.begin() + w1);
+      // Bounce entries grouped by light slot (triangle order within a light): the gather's
+      // results do not depend on the order, and consecutive work then shares one light's photon
+      // cube and the BVH nodes around it, which keeps the working set in L2/MALL.
       std::vector<uint64_t> be, be_units(1, 0);
-      for (uint32_t t = 0; t < nbt; ++t) {
-        uint64_t chunks = (h_tex_start[t + 1] - h_tex_start[t] + 63) / 64;
-        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
-          if (hlights[b_ent_light[e]].bakes_bounce) {
-            be.push_back(e);
-            be_units.push_back(be_units.back() + chunks);
-          }
+      {
+        std::vector<uint64_t> per_light(nl + 1, 0);
+        for (uint32_t t = 0; t < nbt; ++t)
+          for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
+            if (hlights[b_ent_light[e]].bakes_bounce) per_light[b_ent_light[e] + 1]++;
+        for (uint32_t l = 0; l < nl; ++l) per_light[l + 1] += per_light[l];
+        be.resize(per_light[nl]);
+        for (uint32_t t = 0; t < nbt; ++t)
+          for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
+            if (hlights[b_ent_light[e]].bakes_bounce) be[per_light[b_ent_light[e]]++] = e;
+        be_units.resize(be.size() + 1);
+        for (size_t i = 0; i < be.size(); ++i) {
+          const uint32_t t = b_ent_tri[be[i]];
+          be_units[i + 1] = be_units[i] + (h_tex_start[t + 1] - h_tex_start[t] + 63) / 64;
+        }
       }
+      bool sun_bounce = false;
+      for (uint64_t e : be) sun_bounce |= hlights[b_ent_light[e]].shape == 3;
       HIP_TRY(upload(ctx, d_win_rect, b_win_rect));
       HIP_TRY(upload(ctx, d_win_res, b_win_res));
       HIP_TRY(upload(ctx, d_win_off, win_off));
@@ -605,12 +624,43 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bp.colored = opt.colored_bounce ? 1u : 0u;
       bp.cube_res = cres;
       bp.photons = d_photons.p;
+      bp.sun_only = 0;
       HIP_TRY(launch_bounce_owner(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back()));
       DevScene scb = sc;
       scb.counters = ctx->counting ? d_counters.p + 2 : nullptr;
+      DBuf<unsigned long long> d_jobc;
+      DBuf<uint2> d_tjobs, d_cjobs;
+      uint64_t ntjobs = 0;
+      if (ctx->gather_mode == 1) {
+        // job lists of the streamed point-light gather (texel and entry indices are u32)
+        if (ntex >= (1ull << 32) || nbe >= (1ull << 32)) {
+          ctx->err = "a mesh batch holds 2^32 or more texels or light entries";
+          return DL_E_DEVICE;
+        }
+        HIP_TRY(zeroed(ctx, d_jobc, 4));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), 0, d_jobc.p, nullptr, nullptr));
+        std::vector<unsigned long long> jc;
+        HIP_TRY(download(ctx, jc, d_jobc.p, 2));
+        ntjobs = jc[0];
+        HIP_TRY(d_tjobs.alloc(std::max<uint64_t>(1, jc[0])));
+        HIP_TRY(d_cjobs.alloc(std::max<uint64_t>(1, jc[1])));
+        HIP_TRY(hipMemsetAsync(d_jobc.p, 0, 4 * sizeof(unsigned long long), s));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), 1, d_jobc.p, d_tjobs.p,
+                                   d_cjobs.p));
+        HIP_TRY(launch_gather_count(s, tr, ms, d_lights.p, tx, ent, bb, d_cjobs.p, jc[1], bp, ctx->trig));
+        bp.sun_only = 1;
+      }
       HIP_TRY(hipEventRecord(ev0, s));
-      HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
-                                   ctx->trace_block));
+      if (ctx->gather_mode == 1) {
+        HIP_TRY(launch_gather_stream(s, scb, tr, ms, d_lights.p, tx, ent, bb, d_tjobs.p, ntjobs, d_jobc.p + 2, bp, ctx->trig,
+                                     ctx->gather));
+        if (sun_bounce)
+          HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
+                                       ctx->trace_block));
+      } else {
+        HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
+                                     ctx->trace_block));
+      }
       HIP_TRY(hipEventRecord(ev1, s));
       HIP_TRY(hipEventSynchronize(ev1));
       float gms = 0.0f;
@@ -777,6 +827,70 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     HIP_TRY(d_cubes.alloc((size_t)nlocal * 6 * 2048));
     HIP_TRY(launch_distance_cubes(s, sc, d_lights.p, nlocal, d_cubes.p));
     HIP_TRY(download(ctx, scene->distance_cubes, d_cubes.p, (size_t)nlocal * 6 * 2048));
+  }
+  // ---- sun maps (sun_map.rs:25-75) and sky probes (sky_probes.rs:146-191), baker.rs:104-152 --
+  if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) {
+    bool sky = false;
+    for (uint32_t l = nlocal; l < nl; ++l) {
+      if (hlights[l].shape == DL_SHAPE_SKY) sky = true;
+      if (hlights[l].shape != DL_SHAPE_SUN) continue;
+      dl_sun_map_out m{};
+      SunMapParams sp{};
+      if (!sun_map_setup(occ9.data(), (size_t)nocc * 3, hlights[l].toward, opt.sun_map_resolution, &sp, m.rows)) {
+        m.resolution = 1;
+        m.texels.push_back(pack_sun_depth(1.0f, 1.0f));
+      } else {
+        m.resolution = opt.sun_map_resolution;
+        const uint64_t words = ((uint64_t)m.resolution * m.resolution + 1) / 2;
+        DBuf<uint32_t> d_map;
+        HIP_TRY(d_map.alloc(words));
+        HIP_TRY(launch_sun_map(s, sc, sp, d_map.p, words));
+        HIP_TRY(download(ctx, m.texels, d_map.p, words));
+      }
+      scene->sun_maps.push_back(std::move(m));
+    }
+    if (sky && nocc > 0) {
+      f3 mn = mk3(occ9[0], occ9[1], occ9[2]), mx = mn;
+      for (size_t i = 0; i < (size_t)nocc * 3; ++i) {
+        const f3 v = mk3(occ9[3 * i], occ9[3 * i + 1], occ9[3 * i + 2]);
+        mn = vmin(mn, v);
+        mx = vmax(mx, v);
+      }
+      SkyProbeParams pp{};
+      f3 origin;
+      probe_grid(mn, mx, opt.sky_probe_spacing, &origin, &pp.spacing, pp.counts);
+      pp.origin[0] = origin.x;
+      pp.origin[1] = origin.y;
+      pp.origin[2] = origin.z;
+      pp.samples = opt.sky_samples;
+      pp.seed = opt.seed;
+      const size_t n = (size_t)pp.counts[0] * pp.counts[1] * pp.counts[2];
+      DBuf<float> d_open;
+      DBuf<uint32_t> d_back;
+      HIP_TRY(d_open.alloc(n * 6));
+      HIP_TRY(d_back.alloc(n * 6));
+      HIP_TRY(launch_sky_probes(s, sc, ctx->trig, pp, d_open.p, d_back.p));
+      std::vector<float> open;
+      std::vector<uint32_t> back;
+      HIP_TRY(download(ctx, open, d_open.p, n * 6));
+      HIP_TRY(download(ctx, back, d_back.p, n * 6));
+      // a probe is inside geometry when more than a quarter of its rays reach a back face
+      std::vector<uint8_t> valid(n);
+      const float rays = (float)(std::max(opt.sky_samples, 1u) * 6u);
+      for (size_t i = 0; i < n; ++i) {
+        uint32_t b = 0;
+        for (int a = 0; a < 6; ++a) b += back[i * 6 + a];
+        valid[i] = (float)b <= 0.25f * rays ? 1 : 0;
+      }
+      fill_inside(pp.counts, open, valid);
+      scene->probes = true;
+      std::memcpy(scene->probe_origin, pp.origin, sizeof pp.origin);
+      scene->probe_spacing = pp.spacing;
+      std::memcpy(scene->probe_counts, pp.counts, sizeof pp.counts);
+      scene->probe_words.resize(n * 2);
+      const float ones[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+      for (size_t i = 0; i < n; ++i) pack_probe(valid[i] ? &open[i * 6] : ones, &scene->probe_words[i * 2]);
+    }
   }
   ctx->timings.maps_ms = ms_since(t_maps);
 

@@ -152,9 +152,27 @@ uint64_t dl_scene_distance_cubes(const dl_baked_scene* s, const uint32_t** words
   *words = s->distance_cubes.data();
   return s->distance_cubes.size();
 }
-uint32_t dl_scene_sun_map_count(const dl_baked_scene*) { return 0; }
-int dl_scene_sun_map(const dl_baked_scene*, uint32_t, dl_sun_map_view*) { return DL_E_ARGUMENT; }
-void dl_scene_sky_probes(const dl_baked_scene*, dl_sky_probes_view* out) { std::memset(out, 0, sizeof *out); }
+uint32_t dl_scene_sun_map_count(const dl_baked_scene* s) { return s ? (uint32_t)s->sun_maps.size() : 0u; }
+int dl_scene_sun_map(const dl_baked_scene* s, uint32_t i, dl_sun_map_view* out) {
+  if (!s || !out || i >= s->sun_maps.size()) return DL_E_ARGUMENT;
+  const dl_sun_map_out& m = s->sun_maps[i];
+  std::memcpy(out->rows, m.rows, sizeof m.rows);
+  out->resolution = m.resolution;
+  out->texels = m.texels.data();
+  out->texel_word_count = m.texels.size();
+  return DL_OK;
+}
+void dl_scene_sky_probes(const dl_baked_scene* s, dl_sky_probes_view* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof *out);
+  if (!s || !s->probes) return;
+  out->present = 1;
+  std::memcpy(out->origin, s->probe_origin, sizeof s->probe_origin);
+  out->spacing = s->probe_spacing;
+  std::memcpy(out->counts, s->probe_counts, sizeof s->probe_counts);
+  out->words = s->probe_words.data();
+  out->word_count = s->probe_words.size();
+}
 
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out) {
   if (!ctx || !out) return DL_E_ARGUMENT;
@@ -256,6 +274,21 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (trace_block) ctx->trace_block = trace_block;
   if (burial_block) ctx->burial_block = burial_block;
+  return DL_OK;
+}
+
+int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint32_t refill, uint32_t blocks_per_cu) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  if (mode > 1) return set_err(ctx, DL_E_ARGUMENT, "gather mode must be 0 or 1");
+  if (block && (block % 64 != 0 || block > 1024)) return set_err(ctx, DL_E_ARGUMENT, "block must be a multiple of 64 up to 1024");
+  if (refill > 63) return set_err(ctx, DL_E_ARGUMENT, "refill must be below 64");
+  if (blocks_per_cu > 64) return set_err(ctx, DL_E_ARGUMENT, "at most 64 blocks per CU");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->gather_mode = mode;
+  if (block) ctx->gather.block = block;
+  if (chunk) ctx->gather.chunk = chunk;
+  if (refill) ctx->gather.refill = refill;
+  ctx->gather.blocks_per_cu = blocks_per_cu;
   return DL_OK;
 }
 

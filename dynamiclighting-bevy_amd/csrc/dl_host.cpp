@@ -278,6 +278,107 @@ DevLight make_dev_light(const dl_light& l, const dl_bake_options& o) {
   return d;
 }
 
+bool sun_map_setup(const float* corners, size_t ncorners, const float toward_in[3], uint32_t resolution, SunMapParams* p,
+                   float rows[12]) {
+  const f3 toward = mk3(toward_in[0], toward_in[1], toward_in[2]);
+  f3 side, up;
+  orthonormal_pair(toward, &side, &up);
+  const f3 along = neg(toward);
+  f3 mn = mk3(INFINITY, INFINITY, INFINITY), mx = mk3(-INFINITY, -INFINITY, -INFINITY);
+  for (size_t i = 0; i < ncorners; ++i) {
+    const f3 v = mk3(corners[3 * i], corners[3 * i + 1], corners[3 * i + 2]);
+    const f3 l = mk3(dot(v, side), dot(v, up), dot(v, along));
+    mn = vmin(mn, l);
+    mx = vmax(mx, l);
+  }
+  if (!(finite_(mn.x) && finite_(mn.y) && finite_(mn.z) && finite_(mx.x) && finite_(mx.y) && finite_(mx.z))) return false;
+  const f3 size = sub(mx, mn);
+  const float margin = size.z * 0.01f + 0.05f;  // DEPTH_MARGIN (sun_map.rs:22)
+  const float extent = maxf_(size.x, size.y) + 2.0f * margin;
+  const f3 center = mul(add(mn, mx), 0.5f);
+  const f3 bmin = mk3(center.x - extent * 0.5f, center.y - extent * 0.5f, mn.z - margin);
+  const float depth = size.z + 2.0f * margin;
+  const f3 axes[3] = {side, up, along};
+  const float lo[3] = {bmin.x, bmin.y, bmin.z}, length[3] = {extent, extent, depth};
+  for (int r = 0; r < 3; ++r) {
+    rows[4 * r] = axes[r].x / length[r];
+    rows[4 * r + 1] = axes[r].y / length[r];
+    rows[4 * r + 2] = axes[r].z / length[r];
+    rows[4 * r + 3] = -lo[r] / length[r];
+  }
+  for (int k = 0; k < 3; ++k) {
+    p->side[k] = k == 0 ? side.x : k == 1 ? side.y : side.z;
+    p->up[k] = k == 0 ? up.x : k == 1 ? up.y : up.z;
+    p->along[k] = k == 0 ? along.x : k == 1 ? along.y : along.z;
+    p->min[k] = lo[k];
+  }
+  p->texel = extent / (float)resolution;
+  p->depth = depth;
+  p->res = resolution;
+  return true;
+}
+
+uint32_t pack_sun_depth(float lower, float upper) {
+  auto unorm = [](float d) { return sat_u32(roundf(clampf_(d, 0.0f, 1.0f) * 65535.0f)); };
+  return unorm(upper) << 16 | unorm(lower);
+}
+
+void probe_grid(f3 mn, f3 mx, float spacing, f3* origin, float* spacing_out, uint32_t counts[3]) {
+  const uint64_t cap = 65536;  // MAX_SKY_PROBES (sky_probes.rs:18)
+  const f3 extent = add(vmax(sub(mx, mn), mk3(0.0f, 0.0f, 0.0f)), mk3(spacing, spacing, spacing));
+  auto counts_at = [&](float s) {
+    counts[0] = sat_u32(ceilf(extent.x / s)) + 1;
+    counts[1] = sat_u32(ceilf(extent.y / s)) + 1;
+    counts[2] = sat_u32(ceilf(extent.z / s)) + 1;
+  };
+  auto count = [&]() { return (uint64_t)counts[0] * counts[1] * counts[2]; };
+  counts_at(spacing);
+  while (count() > cap) {
+    const float ratio = (float)count() / (float)cap;
+    spacing *= maxf_(cbrtf(ratio), 1.01f);
+    counts_at(spacing);
+  }
+  const f3 span = mul(mk3((float)(counts[0] - 1), (float)(counts[1] - 1), (float)(counts[2] - 1)), spacing);
+  *origin = sub(mul(add(mn, mx), 0.5f), mul(span, 0.5f));
+  *spacing_out = spacing;
+}
+
+void fill_inside(const uint32_t counts[3], std::vector<float>& open, std::vector<uint8_t>& valid) {
+  static const int step[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  const int64_t cx = counts[0], cy = counts[1], cz = counts[2];
+  std::vector<float> before;
+  std::vector<uint8_t> before_valid;
+  for (bool changed = true; changed;) {
+    changed = false;
+    before = open;
+    before_valid = valid;
+    for (size_t i = 0; i < valid.size(); ++i) {
+      if (valid[i]) continue;
+      const int64_t x = (int64_t)i % cx, y = (int64_t)i % (cx * cy) / cx, z = (int64_t)i / (cx * cy);
+      float sum[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      float found = 0.0f;
+      for (const auto& st : step) {
+        const int64_t nx = x + st[0], ny = y + st[1], nz = z + st[2];
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= cx || ny >= cy || nz >= cz) continue;
+        const size_t j = (size_t)((nz * cy + ny) * cx + nx);
+        if (!before_valid[j]) continue;
+        for (int a = 0; a < 6; ++a) sum[a] += before[j * 6 + a];
+        found += 1.0f;
+      }
+      if (found > 0.0f) {
+        for (int a = 0; a < 6; ++a) open[i * 6 + a] = sum[a] / found;
+        valid[i] = 1;
+        changed = true;
+      }
+    }
+  }
+}
+
+void pack_probe(const float* open6, uint32_t* words2) {
+  words2[0] = words2[1] = 0;
+  for (int a = 0; a < 6; ++a) words2[a / 4] |= sat_u32(roundf(clampf_(open6[a], 0.0f, 1.0f) * 255.0f)) << (8 * (a % 4));
+}
+
 uint32_t lightmap_size(float area, float density, uint32_t max_size) {
   float size = ceilf(sqrtf(density * density * area));
   if (size != size) size = 0.0f;

@@ -66,6 +66,8 @@ struct dl_ctx {
   size_t scan_tmp_bytes = 0;
   uint32_t trace_block = 256;
   uint32_t burial_block = 128;
+  dl::GatherLaunch gather{};
+  uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
   unsigned host_threads = 1;
   bool counting = false;
 };
@@ -76,6 +78,12 @@ struct dl_baked_mesh_out {
   std::vector<uint32_t> words;
 };
 
+struct dl_sun_map_out {
+  float rows[12];
+  uint32_t resolution;
+  std::vector<uint32_t> texels;
+};
+
 struct dl_baked_scene {
   dl_baked_settings settings{};
   dl_bake_stats stats{};
@@ -83,6 +91,12 @@ struct dl_baked_scene {
   std::vector<dl_light_bvh_node> bvh;
   std::vector<dl_baked_mesh_out> meshes;
   std::vector<uint32_t> distance_cubes;
+  std::vector<dl_sun_map_out> sun_maps;
+  bool probes = false;
+  float probe_origin[3] = {0, 0, 0};
+  float probe_spacing = 0.0f;
+  uint32_t probe_counts[3] = {0, 0, 0};
+  std::vector<uint32_t> probe_words;
 };
 
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
@@ -98,6 +112,18 @@ int check_light(dl_ctx* ctx, const dl_light& l);
 DevLight make_dev_light(const dl_light& l, const dl_bake_options& o);
 // lightmap_size (geometry.rs:66-70)
 uint32_t lightmap_size(float area, float density, uint32_t max_size);
+
+// Sun map frame and bounds over the occluder corners (sun_map.rs:100-151).  Returns false when
+// there is no static geometry (the map is then one texel nothing is behind).
+bool sun_map_setup(const float* corners, size_t ncorners, const float toward[3], uint32_t resolution, SunMapParams* p,
+                   float rows[12]);
+uint32_t pack_sun_depth(float lower, float upper);
+// ProbeGrid::over (sky_probes.rs:46-62)
+void probe_grid(f3 mn, f3 mx, float spacing, f3* origin, float* spacing_out, uint32_t counts[3]);
+// fill_inside (sky_probes.rs:110-144): open holds 6 values per probe, valid one flag per probe.
+void fill_inside(const uint32_t counts[3], std::vector<float>& open, std::vector<uint8_t>& valid);
+// SkyProbes::pack (format/types.rs:105-112)
+void pack_probe(const float* open6, uint32_t* words2);
 
 inline double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

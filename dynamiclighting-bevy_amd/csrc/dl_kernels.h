@@ -137,6 +137,7 @@ struct BounceParams {
   uint32_t colored;
   uint32_t cube_res;
   const uint32_t* photons;  // 3 words per photon, 6 * res^2 per cube
+  uint32_t sun_only;        // k_bounce_gather: point lights are gathered by the streamed kernel
 };
 // Bounce work: entries (triangle-light pairs) whose light bakes bounce, each with its window.
 struct DevBounce {
@@ -158,6 +159,39 @@ hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* ligh
 hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,
                                 const DevLight* lights, DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevBounce b,
                                 uint64_t units, BounceParams p, TrigTables trig, uint32_t block);
+// Streamed point-light gather: job lists (pass 0 counts into counts[0..1], pass 1 fills using
+// counts[0..1] as zeroed cursors), count-only jobs, and the persistent traced-job kernel.
+struct GatherLaunch {
+  uint32_t block = 256;        // threads per block
+  uint32_t chunk = 256;        // jobs a wave takes from the queue at once
+  uint32_t refill = 48;        // lanes hand out new samples/jobs when at most this many rays are live
+  uint32_t blocks_per_cu = 0;  // 0: as many as fit
+};
+hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
+                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts, uint2* trace_jobs,
+                              uint2* count_jobs);
+hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* jobs, uint64_t njobs, BounceParams p, TrigTables trig);
+hipError_t launch_gather_stream(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx,
+                                DevEntries ent, DevBounce b, const uint2* jobs, uint64_t njobs, unsigned long long* queue,
+                                BounceParams p, TrigTables trig, const GatherLaunch& gl);
+// Sun map of one sun (sun_map.rs:25-75): frame axes, bounds minimum, texel size and depth.
+struct SunMapParams {
+  float side[3], up[3], along[3];
+  float min[3];
+  float texel, depth;
+  uint32_t res;
+};
+hipError_t launch_sun_map(hipStream_t s, DevScene scene, const SunMapParams& p, uint32_t* out, uint64_t words);
+// Sky probe grid (sky_probes.rs:46-106): per (probe, axis) openness and back-face hit count.
+struct SkyProbeParams {
+  float origin[3];
+  float spacing;
+  uint32_t counts[3];
+  uint32_t samples;
+  uint64_t seed;
+};
+hipError_t launch_sky_probes(hipStream_t s, DevScene scene, TrigTables trig, const SkyProbeParams& p, float* open, uint32_t* back);
 struct BlurWeights {
   float w[7];
   float sum;

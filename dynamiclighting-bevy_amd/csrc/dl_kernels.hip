@@ -791,6 +791,72 @@ __global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ light
   }
 }
 
+// ---- sun maps: sun_map.rs:25-75; one thread per word (two texels, row-major) ------------------
+__device__ __forceinline__ uint32_t unorm16(float d) {
+  float v = roundf(clampf_(d, 0.0f, 1.0f) * 65535.0f);
+  return v > 0.0f ? (uint32_t)v : 0u;  // NaN -> 0
+}
+__global__ void k_sun_map(DevScene sc, SunMapParams p, uint32_t* out, uint64_t words) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  const f3 side = mk3(p.side[0], p.side[1], p.side[2]), up = mk3(p.up[0], p.up[1], p.up[2]);
+  const f3 along = mk3(p.along[0], p.along[1], p.along[2]);
+  const uint64_t texels = (uint64_t)p.res * p.res;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x) {
+    float d[2];
+    for (int k = 0; k < 2; ++k) {
+      const uint64_t idx = 2 * w + k;
+      d[k] = 1.0f;
+      if (idx >= texels) continue;
+      const uint32_t x = (uint32_t)(idx % p.res), y = (uint32_t)(idx / p.res);
+      float across = p.min[0] + ((float)x + 0.5f) * p.texel;
+      float down = p.min[1] + ((float)y + 0.5f) * p.texel;
+      f3 origin = add(add(mul(side, across), mul(up, down)), mul(along, p.min[2]));
+      Hit h;
+      if (traverse<false>(sc, stack, blockDim.x, origin, along, p.depth, &h, tl)) d[k] = h.t / p.depth;
+    }
+    out[w] = unorm16(d[1]) << 16 | unorm16(d[0]);
+  }
+}
+
+// ---- sky probes: trace_probe (sky_probes.rs:80-106); one thread per (probe, axis) -------------
+__global__ void k_sky_probes(DevScene sc, TrigTables trig, SkyProbeParams p, float* open, uint32_t* back) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  const uint64_t n = (uint64_t)p.counts[0] * p.counts[1] * p.counts[2] * 6;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t index = i / 6;
+    const uint32_t axis = (uint32_t)(i % 6);
+    const uint32_t px = (uint32_t)(index % p.counts[0]);
+    const uint32_t py = (uint32_t)(index % ((uint64_t)p.counts[0] * p.counts[1]) / p.counts[0]);
+    const uint32_t pz = (uint32_t)(index / ((uint64_t)p.counts[0] * p.counts[1]));
+    f3 position = add(mk3(p.origin[0], p.origin[1], p.origin[2]), mul(mk3((float)px, (float)py, (float)pz), p.spacing));
+    f3 normal = mk3(0.0f, 0.0f, 0.0f);
+    if (axis == 0) normal.x = 1.0f;
+    else if (axis == 1) normal.x = -1.0f;
+    else if (axis == 2) normal.y = 1.0f;
+    else if (axis == 3) normal.y = -1.0f;
+    else if (axis == 4) normal.z = 1.0f;
+    else normal.z = -1.0f;
+    Pcg rng = stream6(p.seed, kProbeStream, px, py, pz, axis);
+    f2 offset;
+    offset.x = pcg_f32(rng);
+    offset.y = pcg_f32(rng);
+    uint32_t nopen = 0, nback = 0;
+    for (uint32_t taken = 0; taken < p.samples; ++taken) {
+      f3 dir = cosine_dir(trig, normal, spread_pt(taken, offset, true));
+      Hit h;
+      bool occluded = traverse<false>(sc, stack, blockDim.x, position, dir, INFINITY, &h, tl) && h.t > 0.0f;
+      if (occluded && dot(tri_face_normal(sc, h.pos), dir) > 0.0f) nback++;
+      if (!occluded) nopen++;
+    }
+    open[i] = (float)nopen / (float)(p.samples > 1 ? p.samples : 1u);
+    back[i] = nback;
+  }
+}
+
 __global__ void k_ray_queries(DevScene sc, const float* __restrict__ rays, uint64_t n, int closest, float* t_out, uint32_t* idx_out) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
@@ -1107,7 +1173,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
       if (L.shape != 3) {
         f3 lp = ld3(L.pos);
         f3 to_texel = sub(pos, lp);
-        if (len_sq(to_texel) > L.radius * L.radius) {
+        if (p.sun_only || len_sq(to_texel) > L.radius * L.radius) {
           write = false;
         } else {
           const uint32_t* cube = p.photons + (size_t)b.cube_of_slot[ent.light[u.e]] * 6 * p.cube_res * p.cube_res * 3;
@@ -1186,6 +1252,319 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
       wave_add(&sc.counters[1], tl.tris);
       tl = Tally{};
     }
+  }
+}
+
+// ---- streamed point-light gather ----------------------------------------------------------------
+// The same gather as k_bounce_gather's point-light branch (bounce.rs:235-292), reorganised so a
+// wave's lanes do not wait for one another: every lane owns one (texel, light) job and walks its
+// samples in order, one BVH node per loop iteration; a lane whose ray ends sets up its next sample
+// (or takes a new job from a global queue) while the other lanes keep traversing.  Each lane
+// still sums its own samples in sample order from its own stream, so results are bit-identical.
+//
+// Jobs are (texel, entry) pairs of point bounce lights within the light's radius; those whose
+// texel the triangle owns in the window map are traced, the rest (their result would be
+// overwritten by a later triangle, bounce.rs:202-209) only count their rays for BakeStats.
+
+// One BVH node of an any-hit traversal (the loop body of traverse<true>).  Returns 0 while the
+// traversal continues (node/sp updated), 1 on a hit, 2 when the ray is unoccluded.
+__device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
+                                              float ix, float iy, float iz, float tmax, int32_t& node, uint32_t& sp, Tally& tl) {
+  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
+  const bool s16 = sc.stack16 != 0;
+  tl.nodes++;
+  const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
+  const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
+  const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
+  const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
+  const float limit = tmax * (1.0f + kBoxEps);
+  int32_t next = -1;
+  float next_t = INFINITY;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= cnt) break;
+    const float lx = k == 0 ? lox.x : k == 1 ? lox.y : k == 2 ? lox.z : lox.w;
+    const float hx = k == 0 ? hix.x : k == 1 ? hix.y : k == 2 ? hix.z : hix.w;
+    const float ly = k == 0 ? loy.x : k == 1 ? loy.y : k == 2 ? loy.z : loy.w;
+    const float hy = k == 0 ? hiy.x : k == 1 ? hiy.y : k == 2 ? hiy.z : hiy.w;
+    const float lz = k == 0 ? loz.x : k == 1 ? loz.y : k == 2 ? loz.z : loz.w;
+    const float hz = k == 0 ? hiz.x : k == 1 ? hiz.y : k == 2 ? hiz.z : hiz.w;
+    const int32_t c = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
+    const float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
+    const float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
+    const float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    tn = tn - fabsf(tn) * kBoxEps;
+    tf = tf + fabsf(tf) * kBoxEps;
+    if (!(tn <= tf && tf >= 0.0f && tn <= limit)) continue;
+    if (c < 0) {
+      const uint32_t v = (uint32_t)(-1 - c);
+      const uint32_t first = v >> 2, count = (v & 3u) + 1u;
+      for (uint32_t i = first; i < first + count; ++i) {
+        const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+        float t, u, vv;
+        tl.tris++;
+        if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+        if (finite_(t)) return 1u;
+      }
+    } else if (next < 0) {
+      next = c;
+      next_t = tn;
+    } else {
+      uint32_t push = (uint32_t)c;
+      if (tn < next_t) {
+        push = (uint32_t)next;
+        next = c;
+        next_t = tn;
+      }
+      if (s16) stack16[sp * stride] = (uint16_t)push;
+      else stack[sp * stride] = push;
+      sp++;
+    }
+  }
+  if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+  if (next >= 0) {
+    node = next;
+    return 0u;
+  }
+  if (sp == 0) return 2u;
+  sp--;
+  node = s16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
+  return 0u;
+}
+
+// Per-job constants of a point-light gather (bounce.rs:247-262).
+struct GatherJob {
+  f3 origin, normal, dir0, lp;
+  float step;
+  uint32_t samples;
+  const uint32_t* cube;
+};
+
+__device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMeshes& ms, const DevLight* __restrict__ lights,
+                                                const DevTexels& tx, const DevEntries& ent, const DevBounce& b,
+                                                const BounceParams& p, uint32_t j, uint32_t e, Pcg* rng, uint32_t* mesh) {
+  const uint32_t t = ent.tri[e], slot = ent.light[e];
+  const DevLight& L = lights[slot];
+  GatherJob g;
+  float4 p4 = tx.pos[j];
+  f3 pos = mk3(p4.x, p4.y, p4.z);
+  g.normal = ld3(tr.face + 3 * (size_t)t);
+  g.lp = ld3(L.pos);
+  g.samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
+  g.step = 0.9f / (float)(g.samples - 1);
+  g.dir0 = normalize_or(sub(pos, g.lp), mk3(0.0f, 1.0f, 0.0f));
+  g.origin = add(pos, mul(g.normal, kRayOffset));
+  g.cube = p.photons + (size_t)b.cube_of_slot[slot] * 6 * p.cube_res * p.cube_res * 3;
+  const uint32_t m = tr.mesh[t];
+  const uint32_t xy = tx.xy[j];
+  *rng = stream5(p.seed, ms.id[m], L.id, xy & 0xffffu, xy >> 16);
+  *mesh = m;
+  return g;
+}
+
+// Sample k of a job: true when it casts a visibility ray (direction, length and the light it
+// brings if unoccluded), false when the photon is empty (bounce.rs:262-286).
+__device__ __forceinline__ bool gather_sample(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k, uint32_t res,
+                                              uint32_t colored, f3* d, float* distance, f3* contrib) {
+  float spread = 0.1f + (float)k * g.step;
+  f3 rnd = on_unit_sphere(trig, rng);
+  f3 sample = normalize_or(slerp3(trig, g.dir0, rnd, spread), g.dir0);
+  uint32_t ci = cube_texel_of(sample, res);
+  uint32_t w0 = g.cube[3 * ci], w1 = g.cube[3 * ci + 1], w2 = g.cube[3 * ci + 2];
+  f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
+                                (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
+  f3 photon = add(g.lp, mul(sample, __uint_as_float(w0)));
+  f3 to_photon = sub(photon, g.origin);
+  float dist = len(to_photon);
+  if (is_zero(pn) || dist <= 0.0f) return false;
+  f3 dir = divs(to_photon, dist);
+  float weight = maxf_(dot(g.normal, dir), 0.0f) * maxf_(dot(pn, neg(dir)), 0.0f);
+  f3 alb = colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f, (float)((w2 >> 16) & 0xffu) / 255.0f)
+                   : mk3(1.0f, 1.0f, 1.0f);
+  *d = dir;
+  *distance = dist;
+  *contrib = mul(alb, weight);
+  return true;
+}
+
+// Job lists: pass 0 counts, pass 1 writes.  counts[0..1]: traced / count-only jobs (pass 0),
+// cursors[0..1] (pass 1).
+__global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, DevTexels tx, const uint64_t* __restrict__ tex_start,
+                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts,
+                              uint2* trace_jobs, uint2* count_jobs) {
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
+    BounceUnit u = bounce_unit(b, ent, tex_start, unit, lane);
+    bool traced = false, counted = false;
+    uint64_t j = 0;
+    if (u.i < u.nt) {
+      const DevLight& L = lights[ent.light[u.e]];
+      j = u.t0 + u.i;
+      if (L.shape != 3) {
+        float4 p4 = tx.pos[j];
+        f3 to_texel = sub(mk3(p4.x, p4.y, p4.z), ld3(L.pos));
+        if (len_sq(to_texel) <= L.radius * L.radius) {
+          uint32_t w = b.ent_window[u.e];
+          const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+          uint32_t xy = tx.xy[j];
+          uint64_t k = (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
+          const unsigned int* owner = reinterpret_cast<const unsigned int*>(b.map_b) + b.win_off[w] * 3;
+          traced = owner[k] == u.t + 1;
+          counted = !traced;
+        }
+      }
+    }
+    const uint64_t mt = __ballot(traced), mc = __ballot(counted);
+    const uint32_t rt = __builtin_amdgcn_mbcnt_hi((uint32_t)(mt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mt, 0u));
+    const uint32_t rc = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
+    unsigned long long bt = 0, bc = 0;
+    if (lane == 0) {
+      if (mt) bt = atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
+      if (mc) bc = atomicAdd(&counts[1], (unsigned long long)__popcll(mc));
+    }
+    if (pass == 1) {
+      bt = ((unsigned long long)__shfl((uint32_t)(bt >> 32), 0, 64) << 32) | __shfl((uint32_t)bt, 0, 64);
+      bc = ((unsigned long long)__shfl((uint32_t)(bc >> 32), 0, 64) << 32) | __shfl((uint32_t)bc, 0, 64);
+      if (traced) trace_jobs[bt + rt] = make_uint2((uint32_t)j, (uint32_t)u.e);
+      if (counted) count_jobs[bc + rc] = make_uint2((uint32_t)j, (uint32_t)u.e);
+    }
+  }
+}
+
+// Rays of the jobs whose result a later triangle overwrites: the samples are set up (they decide
+// which samples cast a ray) but nothing is traced.
+__global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* __restrict__ jobs, uint64_t njobs, BounceParams p, TrigTables trig) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < njobs; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 jb = jobs[i];
+    Pcg rng;
+    uint32_t m;
+    GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
+    uint32_t rays = 0;
+    for (uint32_t k = 0; k < g.samples; ++k) {
+      f3 d, c;
+      float dist;
+      rays += gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c) ? 1u : 0u;
+    }
+    atomicAdd(&b.rays[m], (unsigned long long)rays);
+  }
+}
+
+// Persistent streamed gather over the traced jobs; `queue` (zeroed) hands out chunks of jobs.
+__global__ void k_gather_stream(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
+                                DevEntries ent, DevBounce b, const uint2* __restrict__ jobs, uint64_t njobs,
+                                unsigned long long* queue, BounceParams p, TrigTables trig, uint32_t chunk, uint32_t refill) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t cbase = 0, cend = 0;
+  bool drained = false;
+  // job state
+  bool job = false;
+  GatherJob g;
+  Pcg rng;
+  uint32_t k = 0, mesh = 0, jrays = 0;
+  uint64_t dst = 0;
+  float intensity = 0.0f;
+  f3 sum = mk3(0.0f, 0.0f, 0.0f);
+  uint32_t cur_mesh = 0xffffffffu;
+  uint32_t mesh_rays = 0;
+  // ray state
+  bool live = false;
+  f3 d = mk3(0.0f, 0.0f, 1.0f), contrib = mk3(0.0f, 0.0f, 0.0f);
+  float ix = 0.0f, iy = 0.0f, iz = 0.0f, tmax = 0.0f;
+  int32_t node = 0;
+  uint32_t sp = 0;
+  for (;;) {
+    if ((uint32_t)__popcll(__ballot(live)) <= refill) {
+      // hand out jobs to lanes without one
+      const uint64_t need = __ballot(!job);
+      if (need && !drained) {
+        if (cbase >= cend) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(queue, (unsigned long long)chunk);
+          base = ((unsigned long long)__shfl((uint32_t)(base >> 32), 0, 64) << 32) | __shfl((uint32_t)base, 0, 64);
+          cbase = base;
+          cend = base + chunk < njobs ? base + chunk : njobs;
+          if (cbase >= njobs) drained = true;
+        }
+        if (!drained) {
+          const uint64_t avail = cend - cbase;
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+          if (!job && rank < avail) {
+            const uint2 jb = jobs[cbase + rank];
+            g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &mesh);
+            const uint32_t w = b.ent_window[jb.y];
+            const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+            const uint32_t xy = tx.xy[jb.x];
+            dst = b.win_off[w] + (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
+            intensity = lights[ent.light[jb.y]].bounce_intensity;
+            sum = mk3(0.0f, 0.0f, 0.0f);
+            k = 0;
+            jrays = 0;
+            job = true;
+          }
+          const uint64_t took = avail < (uint64_t)__popcll(need) ? avail : (uint64_t)__popcll(need);
+          cbase += took;
+        }
+      }
+      // next sample (or the end of the job) for lanes with a job and no ray
+      if (job && !live) {
+        while (k < g.samples) {
+          float dist;
+          const bool ray = gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &contrib);
+          k++;
+          if (ray) {
+            jrays++;
+            if (sc.ntris == 0) {
+              sum = add(sum, contrib);
+              continue;
+            }
+            live = true;
+            tmax = dist;
+            ix = inv_dir(d.x);
+            iy = inv_dir(d.y);
+            iz = inv_dir(d.z);
+            node = 0;
+            sp = 0;
+            break;
+          }
+        }
+        if (!live) {
+          f3 v = divs(mul(sum, intensity), (float)g.samples);
+          float* out = b.map_a + dst * 3;
+          out[0] = v.x;
+          out[1] = v.y;
+          out[2] = v.z;
+          if (mesh != cur_mesh) {
+            if (cur_mesh != 0xffffffffu && mesh_rays) atomicAdd(&b.rays[cur_mesh], (unsigned long long)mesh_rays);
+            cur_mesh = mesh;
+            mesh_rays = 0;
+          }
+          mesh_rays += jrays;
+          job = false;
+        }
+      }
+      if (!__ballot(live)) {
+        if (drained && !__ballot(job)) break;
+        continue;
+      }
+    }
+    if (live) {
+      const uint32_t r = visit_any(sc, stack, blockDim.x, g.origin, d, ix, iy, iz, tmax, node, sp, tl);
+      if (r != 0u) {
+        live = false;
+        if (r == 2u) sum = add(sum, contrib);
+      }
+    }
+  }
+  if (cur_mesh != 0xffffffffu && mesh_rays) atomicAdd(&b.rays[cur_mesh], (unsigned long long)mesh_rays);
+  if (sc.counters) {
+    wave_add(&sc.counters[0], tl.nodes);
+    wave_add(&sc.counters[1], tl.tris);
   }
 }
 
@@ -1396,6 +1775,51 @@ hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTr
   if (units == 0) return hipSuccess;
   hipLaunchKernelGGL(k_bounce_gather, dim3(grid_for(units, block / 64)), dim3(block), stack_bytes(scene, block), s, scene, occ,
                      tris, meshes, lights, tx, tex_start, ent, b, units, p, trig);
+  return hipGetLastError();
+}
+hipError_t launch_sun_map(hipStream_t s, DevScene scene, const SunMapParams& p, uint32_t* out, uint64_t words) {
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sun_map, dim3(grid_for(words, kThreads)), dim3(kThreads), stack_bytes(scene, kThreads), s, scene, p, out,
+                     words);
+  return hipGetLastError();
+}
+hipError_t launch_sky_probes(hipStream_t s, DevScene scene, TrigTables trig, const SkyProbeParams& p, float* open, uint32_t* back) {
+  const uint64_t n = (uint64_t)p.counts[0] * p.counts[1] * p.counts[2] * 6;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sky_probes, dim3(grid_for(n, 64)), dim3(64), stack_bytes(scene, 64), s, scene, trig, p, open, back);
+  return hipGetLastError();
+}
+hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
+                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts, uint2* trace_jobs,
+                              uint2* count_jobs) {
+  if (units == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_jobs, dim3(grid_for(units, kThreads / 64)), dim3(kThreads), 0, s, tris, lights, tx, tex_start, ent, b,
+                     units, pass, counts, trace_jobs, count_jobs);
+  return hipGetLastError();
+}
+hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* jobs, uint64_t njobs, BounceParams p, TrigTables trig) {
+  if (njobs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_count, dim3(grid_for(njobs, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, tx, ent, b, jobs,
+                     njobs, p, trig);
+  return hipGetLastError();
+}
+hipError_t launch_gather_stream(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx,
+                                DevEntries ent, DevBounce b, const uint2* jobs, uint64_t njobs, unsigned long long* queue,
+                                BounceParams p, TrigTables trig, const GatherLaunch& gl) {
+  if (njobs == 0) return hipSuccess;
+  const size_t lds = stack_bytes(scene, gl.block);
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_stream, (int)gl.block, lds);
+  if (e != hipSuccess) return e;
+  if (gl.blocks_per_cu) per_cu = (int)gl.blocks_per_cu;
+  uint64_t grid = (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, per_cu);
+  const uint64_t useful = (njobs + gl.block - 1) / gl.block;
+  if (grid > useful) grid = useful;
+  hipLaunchKernelGGL(k_gather_stream, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, tris, meshes, lights, tx, ent, b, jobs,
+                     njobs, queue, p, trig, gl.chunk, gl.refill);
   return hipGetLastError();
 }
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,

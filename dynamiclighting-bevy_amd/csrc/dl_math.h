@@ -150,6 +150,7 @@ DLI Pcg stream4(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
   return pcg_from_key(s);
 }
 static constexpr uint64_t kBurialStream = 0x627572696564ull;  // mesh_texels.rs:28
+static constexpr uint64_t kProbeStream = 0x736b7970726f6265ull;  // sky_probes.rs:21
 
 // ---- texel rectangles: mesh_builder.rs:83-137 ------------------------------------------------
 struct Rect { uint32_t x0, y0, x1, y1; };

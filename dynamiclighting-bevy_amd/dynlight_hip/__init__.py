@@ -68,6 +68,8 @@ def load() -> C.CDLL:
     lib.dl_set_counting.argtypes = [P, C.c_int]
     lib.dl_set_launch.restype = C.c_int
     lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
+    lib.dl_set_gather.restype = C.c_int
+    lib.dl_set_gather.argtypes = [P] + [C.c_uint32] * 5
     _lib = lib
     return lib
 
@@ -119,6 +121,12 @@ class Context:
 
     def set_launch(self, trace_block: int = 0, burial_block: int = 0) -> None:
         st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)
+        if st != 0:
+            raise BakeError(st, self.error())
+
+    def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, refill: int = 0, blocks_per_cu: int = 0) -> None:
+        """Point-light bounce gather: 0 = one wave per (triangle, light, 64 texels), 1 = streamed jobs."""
+        st = self.lib.dl_set_gather(self.handle, mode, block, chunk, refill, blocks_per_cu)
         if st != 0:
             raise BakeError(st, self.error())
 

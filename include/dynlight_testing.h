@@ -39,6 +39,12 @@ int dl_set_counting(dl_ctx* ctx, int enable);
  * shadow-ray and burial kernels. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
 
+/* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels), mode 1 =
+ * streamed per-lane jobs (default).  For mode 1: threads per block, jobs per queue chunk,
+ * the live-ray count at or below which lanes take new samples, and blocks per CU (0 = as many
+ * as fit).  Zero keeps the current value for block, chunk and refill. */
+int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint32_t refill, uint32_t blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1965,6 +1965,189 @@ static std::vector<uint32_t> bake_distance_cubes(const Scene& sc) {
   return words;
 }
 
+// ---- sun maps: sun_map.rs:25-150 ---------------------------------------------------------------
+struct SunMapOut {
+  float rows[12];
+  uint32_t resolution;
+  std::vector<uint32_t> texels;
+};
+// pack_sun_depth (sun_map.rs:77-80): two unorm16 depths, `lower` in bits 0-15.
+static uint32_t pack_sun_depth(float lower, float upper) {
+  auto unorm = [](float d) { return as_u32(std::round(clampf(d, 0.0f, 1.0f) * 65535.0f)); };
+  return unorm(upper) << 16 | unorm(lower);
+}
+static SunMapOut bake_sun_map(const Scene& sc, const Light& L, uint32_t resolution) {
+  SunMapOut out{};
+  // SunFrame::new (sun_map.rs:100-109)
+  const V3 toward = L.sun_toward;
+  V3 side, up;
+  any_orthonormal_pair(toward, &side, &up);
+  const V3 along = -toward;
+  // SunFrame::bounds (sun_map.rs:119-142) over every occluder corner
+  V3 mn = v3(INFINITY, INFINITY, INFINITY), mx = v3(-INFINITY, -INFINITY, -INFINITY);
+  for (const OccTri& t : sc.occ.tris)
+    for (int k = 0; k < 3; ++k) {
+      V3 l = v3(dot(t.v[k], side), dot(t.v[k], up), dot(t.v[k], along));
+      mn = vmin(mn, l);
+      mx = vmax(mx, l);
+    }
+  bool finite = std::isfinite(mn.x) && std::isfinite(mn.y) && std::isfinite(mn.z) && std::isfinite(mx.x) && std::isfinite(mx.y) &&
+                std::isfinite(mx.z);
+  if (!finite) {
+    out.resolution = 1;
+    out.texels.push_back(pack_sun_depth(1.0f, 1.0f));
+    return out;
+  }
+  V3 size = mx - mn;
+  float margin = size.z * 0.01f + 0.05f;
+  float extent = fmax_(size.x, size.y) + 2.0f * margin;
+  V3 center = (mn + mx) * 0.5f;
+  V3 bmin = v3(center.x - extent * 0.5f, center.y - extent * 0.5f, mn.z - margin);
+  float depth = size.z + 2.0f * margin;
+  // SunFrame::rows (sun_map.rs:144-151)
+  auto row = [&](float* r, V3 axis, float lo, float length) {
+    V3 a = axis / length;
+    r[0] = a.x;
+    r[1] = a.y;
+    r[2] = a.z;
+    r[3] = -lo / length;
+  };
+  row(out.rows, side, bmin.x, extent);
+  row(out.rows + 4, up, bmin.y, extent);
+  row(out.rows + 8, along, bmin.z, depth);
+  out.resolution = resolution;
+  const float texel = extent / (float)resolution;
+  std::vector<float> depths((size_t)resolution * resolution);
+  par_for(resolution, [&](size_t y) {
+    for (uint32_t x = 0; x < resolution; ++x) {
+      float across = bmin.x + ((float)x + 0.5f) * texel;
+      float down = bmin.y + ((float)y + 0.5f) * texel;
+      V3 origin = (side * across + up * down) + along * bmin.z;
+      float dist;
+      uint32_t occ;
+      depths[y * resolution + x] = sc.occ.closest_hit(origin, along, depth, L.alpha_tested, &dist, &occ, nullptr) ? dist / depth : 1.0f;
+    }
+  });
+  for (size_t i = 0; i < depths.size(); i += 2)
+    out.texels.push_back(pack_sun_depth(depths[i], i + 1 < depths.size() ? depths[i + 1] : 1.0f));
+  return out;
+}
+
+// ---- sky probes: sky_probes.rs:46-191 ----------------------------------------------------------
+struct SkyProbesOut {
+  bool present = false;
+  V3 origin;
+  float spacing;
+  uint32_t counts[3];
+  std::vector<uint32_t> words;
+};
+static const uint64_t PROBE_STREAM = 0x736b7970726f6265ull;
+static const uint32_t MAX_SKY_PROBES = 65536;
+static const V3 PROBE_AXES[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+// ProbeGrid::over (sky_probes.rs:46-62)
+static void probe_grid(V3 mn, V3 mx, float spacing, V3* origin, float* spacing_out, uint32_t counts[3]) {
+  V3 extent = vmax(mx - mn, v3(0, 0, 0)) + v3(spacing, spacing, spacing);
+  auto counts_at = [&](float s, uint32_t* c) {
+    c[0] = as_u32(std::ceil(extent.x / s)) + 1;
+    c[1] = as_u32(std::ceil(extent.y / s)) + 1;
+    c[2] = as_u32(std::ceil(extent.z / s)) + 1;
+  };
+  auto count = [](const uint32_t* c) { return (uint64_t)c[0] * c[1] * c[2]; };
+  counts_at(spacing, counts);
+  while (count(counts) > MAX_SKY_PROBES) {
+    float ratio = (float)count(counts) / (float)MAX_SKY_PROBES;
+    spacing *= fmax_(std::cbrt(ratio), 1.01f);
+    counts_at(spacing, counts);
+  }
+  V3 span = v3((float)(counts[0] - 1), (float)(counts[1] - 1), (float)(counts[2] - 1)) * spacing;
+  *origin = (mn + mx) * 0.5f - span * 0.5f;
+  *spacing_out = spacing;
+}
+
+// SkyProbes::pack (format/types.rs:105-112)
+static void pack_probe(const float* open, uint32_t* words) {
+  words[0] = words[1] = 0;
+  for (int a = 0; a < 6; ++a) {
+    uint32_t byte = as_u32(std::round(clampf(open[a], 0.0f, 1.0f) * 255.0f));
+    words[a / 4] |= byte << (8 * (a % 4));
+  }
+}
+
+// fill_inside (sky_probes.rs:110-144): layer by layer from the previous pass.
+static void fill_inside(const uint32_t counts[3], std::vector<float>& open, std::vector<uint8_t>& valid) {
+  const int64_t cx = counts[0], cy = counts[1], cz = counts[2];
+  const size_t n = valid.size();
+  for (;;) {
+    std::vector<float> prev_open = open;
+    std::vector<uint8_t> prev_valid = valid;
+    bool changed = false;
+    for (size_t index = 0; index < n; ++index) {
+      if (valid[index]) continue;
+      int64_t at[3] = {(int64_t)(index % cx), (int64_t)(index % (cx * cy) / cx), (int64_t)(index / (cx * cy))};
+      float sum[6] = {0, 0, 0, 0, 0, 0};
+      float found = 0.0f;
+      for (int a = 0; a < 6; ++a) {
+        int64_t nx = at[0] + (int64_t)PROBE_AXES[a].x, ny = at[1] + (int64_t)PROBE_AXES[a].y, nz = at[2] + (int64_t)PROBE_AXES[a].z;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= cx || ny >= cy || nz >= cz) continue;
+        size_t ni = (size_t)((nz * cy + ny) * cx + nx);
+        if (!prev_valid[ni]) continue;
+        for (int k = 0; k < 6; ++k) sum[k] += prev_open[ni * 6 + k];
+        found += 1.0f;
+      }
+      if (found > 0.0f) {
+        for (int k = 0; k < 6; ++k) open[index * 6 + k] = sum[k] / found;
+        valid[index] = 1;
+        changed = true;
+      }
+    }
+    if (!changed) break;
+  }
+}
+
+// bake_sky_probes (sky_probes.rs:146-191) with trace_probe (sky_probes.rs:80-106)
+static SkyProbesOut bake_sky_probes(const Scene& sc, float spacing, uint32_t samples, bool alpha, uint64_t seed) {
+  SkyProbesOut out;
+  if (sc.occ.tris.empty()) return out;
+  V3 mn = sc.occ.tris[0].v[0], mx = mn;
+  for (const OccTri& t : sc.occ.tris)
+    for (int k = 0; k < 3; ++k) {
+      mn = vmin(mn, t.v[k]);
+      mx = vmax(mx, t.v[k]);
+    }
+  probe_grid(mn, mx, spacing, &out.origin, &out.spacing, out.counts);
+  const size_t n = (size_t)out.counts[0] * out.counts[1] * out.counts[2];
+  std::vector<float> open(n * 6);
+  std::vector<uint8_t> valid(n);
+  par_for(n, [&](size_t index) {
+    uint32_t px = (uint32_t)(index % out.counts[0]);
+    uint32_t py = (uint32_t)(index % ((size_t)out.counts[0] * out.counts[1]) / out.counts[0]);
+    uint32_t pz = (uint32_t)(index / ((size_t)out.counts[0] * out.counts[1]));
+    V3 position = out.origin + v3((float)px, (float)py, (float)pz) * out.spacing;
+    uint32_t back = 0;
+    for (int a = 0; a < 6; ++a) {
+      Rng rng = stream({seed, PROBE_STREAM, px, py, pz, (uint64_t)a});
+      auto occluded = [&](V3 direction) {
+        float dist;
+        uint32_t occ;
+        if (!sc.occ.closest_hit(position, direction, INFINITY, alpha, &dist, &occ, nullptr)) return false;
+        if (!(dist > 0.0f)) return false;
+        if (dot(sc.occ.face_normal(occ), direction) > 0.0f) back++;
+        return true;
+      };
+      open[index * 6 + a] = sky_visibility(PROBE_AXES[a], samples, rng, occluded);
+    }
+    float rays = (float)(std::max(samples, 1u) * 6u);
+    valid[index] = (float)back <= 0.25f * rays ? 1 : 0;
+  });
+  fill_inside(out.counts, open, valid);
+  out.words.resize(n * 2);
+  const float ones[6] = {1, 1, 1, 1, 1, 1};
+  for (size_t i = 0; i < n; ++i) pack_probe(valid[i] ? &open[i * 6] : ones, &out.words[i * 2]);
+  out.present = true;
+  return out;
+}
+
 }  // namespace o
 
 // ------------------------------------------------------------------------------------------
@@ -1977,6 +2160,8 @@ struct dlo_scene {
   std::vector<dl_light_bvh_node> bvh;
   std::vector<o::MeshOut> meshes;
   std::vector<uint32_t> distance_cubes;
+  std::vector<o::SunMapOut> sun_maps;
+  o::SkyProbesOut sky_probes;
 };
 
 static thread_local std::string g_err;
@@ -2145,7 +2330,20 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
     for (size_t i = 0; i < prepared.size(); ++i) res->meshes[i] = bake_mesh(*sc, prepared[i]);
   }
   auto t_maps = std::chrono::steady_clock::now();
-  if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) res->distance_cubes = bake_distance_cubes(*sc);
+  if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) {
+    // baker.rs:105-152: distance cubes, a sun map per sun (slot order), sky probes if a sky bakes
+    res->distance_cubes = bake_distance_cubes(*sc);
+    bool sky = false, sky_alpha = false;
+    for (size_t l = sc->local_lights; l < sc->lights.size(); ++l) {
+      const o::Light& L = sc->lights[l];
+      if (L.sun) res->sun_maps.push_back(o::bake_sun_map(*sc, L, opt.sun_map_resolution));
+      if (L.sky) {
+        sky = true;
+        sky_alpha |= L.alpha_tested;
+      }
+    }
+    if (sky) res->sky_probes = o::bake_sky_probes(*sc, opt.sky_probe_spacing, opt.sky_samples, sky_alpha, opt.seed);
+  }
   auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };
@@ -2217,9 +2415,29 @@ uint64_t dlo_scene_distance_cubes(const dlo_scene* s, const uint32_t** words) {
   *words = s->distance_cubes.data();
   return s->distance_cubes.size();
 }
-uint32_t dlo_scene_sun_map_count(const dlo_scene*) { return 0; }
-int dlo_scene_sun_map(const dlo_scene*, uint32_t, dl_sun_map_view*) { return DL_E_ARGUMENT; }
-void dlo_scene_sky_probes(const dlo_scene*, dl_sky_probes_view* out) { memset(out, 0, sizeof *out); }
+uint32_t dlo_scene_sun_map_count(const dlo_scene* s) { return (uint32_t)s->sun_maps.size(); }
+int dlo_scene_sun_map(const dlo_scene* s, uint32_t i, dl_sun_map_view* out) {
+  if (i >= s->sun_maps.size()) return DL_E_ARGUMENT;
+  const auto& m = s->sun_maps[i];
+  memcpy(out->rows, m.rows, sizeof m.rows);
+  out->resolution = m.resolution;
+  out->texels = m.texels.data();
+  out->texel_word_count = m.texels.size();
+  return DL_OK;
+}
+void dlo_scene_sky_probes(const dlo_scene* s, dl_sky_probes_view* out) {
+  memset(out, 0, sizeof *out);
+  const auto& p = s->sky_probes;
+  if (!p.present) return;
+  out->present = 1;
+  out->origin[0] = p.origin.x;
+  out->origin[1] = p.origin.y;
+  out->origin[2] = p.origin.z;
+  out->spacing = p.spacing;
+  memcpy(out->counts, p.counts, sizeof p.counts);
+  out->words = p.words.data();
+  out->word_count = p.words.size();
+}
 
 // ---- unit-level entry points -----------------------------------------------------------------
 uint64_t dlo_splitmix64(uint64_t v) { return o::splitmix64(v); }
@@ -2409,6 +2627,23 @@ uint32_t dlo_pack_bounce(const float* values, uint32_t count, uint32_t rgb, int3
   return (uint32_t)w.size();
 }
 uint32_t dlo_f16_pair(float lower, float upper) { return o::pack_f16_pair(lower, upper); }
+uint32_t dlo_sun_depth_pair(float lower, float upper) { return o::pack_sun_depth(lower, upper); }
+void dlo_probe_grid(const float* mn, const float* mx, float spacing, float* origin, float* spacing_out, uint32_t* counts) {
+  o::V3 org;
+  o::probe_grid(o::v3p(mn), o::v3p(mx), spacing, &org, spacing_out, counts);
+  origin[0] = org.x;
+  origin[1] = org.y;
+  origin[2] = org.z;
+}
+void dlo_fill_inside(const uint32_t* counts, float* open, uint8_t* valid) {
+  size_t n = (size_t)counts[0] * counts[1] * counts[2];
+  std::vector<float> o6(open, open + n * 6);
+  std::vector<uint8_t> v(valid, valid + n);
+  o::fill_inside(counts, o6, v);
+  memcpy(open, o6.data(), n * 6 * sizeof(float));
+  memcpy(valid, v.data(), n);
+}
+void dlo_pack_probe(const float* open6, uint32_t* words2) { o::pack_probe(open6, words2); }
 static o::Occluders soup(const float* tris, uint32_t n, uint32_t brute) {
   o::Occluders occ;
   for (uint32_t i = 0; i < n; ++i) {

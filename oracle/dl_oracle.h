@@ -99,6 +99,14 @@ uint32_t dlo_serialize(uint32_t triangles, const uint32_t* crops, const uint32_t
 uint32_t dlo_pack_bounce(const float* values, uint32_t count, uint32_t rgb, int32_t compression,
                          const uint64_t* key, uint32_t nkey, uint32_t* out, uint32_t cap);
 uint32_t dlo_f16_pair(float lower, float upper);
+/* pack_sun_depth (sun_map.rs:77-80) */
+uint32_t dlo_sun_depth_pair(float lower, float upper);
+/* ProbeGrid::over (sky_probes.rs:46-62) */
+void dlo_probe_grid(const float* min3, const float* max3, float spacing, float* origin3, float* spacing_out, uint32_t* counts3);
+/* fill_inside (sky_probes.rs:110-144): open is 6 floats per probe, valid 1 byte per probe */
+void dlo_fill_inside(const uint32_t* counts3, float* open, uint8_t* valid);
+/* SkyProbes::pack (format/types.rs:105-112) */
+void dlo_pack_probe(const float* open6, uint32_t* words2);
 /* Ray queries on an occluder soup (brute force or BVH). tris: n*9 floats. */
 int dlo_ray_any(const float* tris, uint32_t n, const float* origin, const float* dir, float max,
                 uint32_t brute);

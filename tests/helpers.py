@@ -27,5 +27,19 @@ def assert_scenes_equal(got, want, check_stats=True):
     assert not bad, "; ".join(bad[:8])
     d = first_diff(got.distance_cubes, want.distance_cubes)
     assert d is None, f"distance cubes: {d}"
+    assert len(got.sun_maps) == len(want.sun_maps)
+    for k, (g, w) in enumerate(zip(got.sun_maps, want.sun_maps)):
+        assert np.array_equal(np.float32(g["rows"]).view(np.uint32), np.float32(w["rows"]).view(np.uint32)), f"sun map {k} rows"
+        assert g["resolution"] == w["resolution"]
+        d = first_diff(g["texels"], w["texels"])
+        assert d is None, f"sun map {k}: {d}"
+    assert (got.sky_probes is None) == (want.sky_probes is None)
+    if got.sky_probes is not None:
+        g, w = got.sky_probes, want.sky_probes
+        assert np.array_equal(np.float32(g["origin"] + [g["spacing"]]).view(np.uint32),
+                              np.float32(w["origin"] + [w["spacing"]]).view(np.uint32))
+        assert g["counts"] == w["counts"]
+        d = first_diff(g["words"], w["words"])
+        assert d is None, f"sky probes: {d}"
     if check_stats:
         assert got.stats == want.stats

@@ -109,6 +109,17 @@ def bounce_white_dither_case():
     return inp, o
 
 
+def maps_case():
+    """Sun maps with an odd texel count (the last word's upper half is padding), two suns in slot
+    order, and sky probes at a spacing that leaves probes inside the boxes."""
+    meshes = [scenes.floor(1, (0, 0, 0), 4.0), scenes.cuboid(2, (-0.5, 0.0, -0.5), (0.5, 1.0, 0.5)),
+              scenes.cuboid(3, (1.0, 0.0, 1.0), (2.5, 2.0, 1.6))]
+    lights = [scenes.sun_light(51, (0.3, 1.0, 0.2), 0.53), scenes.sun_light(50, (-0.7, 0.5, 0.1), 0.0),
+              scenes.sky_light(52)]
+    return BakeInput(meshes, lights), opts(4.0, sun_map_resolution=33, sky_samples=20, sky_probe_spacing=0.35,
+                                           area_light_samples=4, area_light_min_samples=4)
+
+
 CASES = {"point": point_case, "panel": panel_case, "sun": sun_case, "sky": sky_case, "mixed": mixed_case,
          "odd_geometry": odd_geometry_case, "bounce_point": bounce_point_case, "bounce_sun": bounce_sun_case,
-         "bounce_dither": bounce_dither_case, "bounce_white_dither": bounce_white_dither_case}
+         "bounce_dither": bounce_dither_case, "bounce_white_dither": bounce_white_dither_case, "maps": maps_case}

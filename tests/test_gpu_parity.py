@@ -124,3 +124,22 @@ def test_empty_scenes_match_oracle(ctx):
     for inp in (BakeInput([], []), BakeInput([scenes.floor(1, (0, 0, 0), 1.0)], []),
                 BakeInput([], [scenes.point_light(1, (0, 0, 0), 1.0)])):
         assert_scenes_equal(ctx.bake(inp, abi.default_options()), oracle.bake(inp, abi.default_options()))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_bounce_gather_modes_match_oracle(ctx, mode):
+    """Both point-light gathers (one wave per triangle, light and 64 texels; streamed per-lane
+    jobs with light-major job order) against the oracle on a 2x2-room interior with 12 bounce
+    lights, and the bounce edge cases."""
+    try:
+        ctx.set_gather(mode)
+        inp, opts = scenes.interior(seed=9, target_triangles=3000, lights=12, grid=2, density=12.0)
+        got = ctx.bake(inp, opts)
+        assert got.stats["bounce_rays"] > 1_000_000
+        assert_scenes_equal(got, oracle.bake(inp, opts))
+        for name in sorted(scene_cases.CASES):
+            if name.startswith("bounce"):
+                inp, opts = scene_cases.CASES[name]()
+                assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
+    finally:
+        ctx.set_gather(1)
