@@ -119,19 +119,34 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       }
     }
   }
-  for (auto* m : meshes)
-    if (m->alpha && m->occludes)
-      for (auto* l : lights)
-        if (l->transparency != DL_TRANSPARENCY_DISABLED) {
-          ctx->err = "alpha-tested shadows are not supported by this build";
-          return DL_E_ARGUMENT;
-        }
+  bool alpha_lights = false;
+  for (auto* l : lights) alpha_lights |= l->transparency != DL_TRANSPARENCY_DISABLED;
 
   // ---- occluders (occluder.rs:80-115) with their normals and albedo --------------------------
   std::vector<float> occ9, occ_nrm, occ_alb;
   std::vector<uint32_t> occ_flags;
+  // alpha-tested occluders (occluder.rs:147-166): texture per occluder, corner UV0, opaque bits
+  std::vector<uint32_t> occ_tex, tex_info, tex_bits;
+  std::vector<float> occ_uv0;
+  bool any_alpha = false;
+  for (auto* m : meshes)
+    any_alpha |= alpha_lights && m->occludes && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0;
   for (auto* m : meshes) {
     if (!m->occludes) continue;
+    uint32_t tex = 0xffffffffu;
+    if (any_alpha && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0) {
+      const dl_alpha_texture* a = m->alpha;
+      tex = (uint32_t)(tex_info.size() / 4);
+      const uint64_t texels = (uint64_t)a->width * a->height;
+      tex_info.push_back(a->width);
+      tex_info.push_back(a->height);
+      tex_info.push_back((uint32_t)tex_bits.size());
+      tex_info.push_back(0);
+      const size_t first = tex_bits.size();
+      tex_bits.resize(first + (size_t)((texels + 31) / 32), 0u);
+      for (uint64_t i = 0; i < texels; ++i)
+        if ((float)a->rgba8[i * 4 + 3] / 255.0f >= a->cutoff) tex_bits[first + i / 32] |= 1u << (i % 32);
+    }
     bool has_n = m->normals && m->normal_count;
     for (uint32_t t = 0; t < m->index_count / 3; ++t) {
       f3 v[3];
@@ -147,6 +162,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       }
       occ_flags.push_back(has_n ? 1u : 0u);
       for (int a = 0; a < 3; ++a) occ_alb.push_back(m->albedo[a]);
+      if (any_alpha) {
+        occ_tex.push_back(tex);
+        for (int k = 0; k < 3; ++k)
+          for (int a = 0; a < 2; ++a) occ_uv0.push_back(tex != 0xffffffffu ? m->uv0[2 * (size_t)m->indices[t * 3 + k] + a] : 0.0f);
+      }
     }
   }
   const uint32_t nocc = (uint32_t)occ_flags.size();
@@ -230,6 +250,22 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   sc.stack16 = bvh.nodes.size() < 65536 ? 1u : 0u;
   sc.overflow = d_overflow.p;
   sc.counters = nullptr;
+  DBuf<uint32_t> d_occ_tex, d_tex_info, d_tex_bits;
+  DBuf<float> d_occ_uv0;
+  sc.occ_tex = nullptr;
+  sc.occ_uv0 = nullptr;
+  sc.tex = nullptr;
+  sc.tex_bits = nullptr;
+  if (any_alpha) {
+    HIP_TRY(upload(ctx, d_occ_tex, occ_tex));
+    HIP_TRY(upload(ctx, d_occ_uv0, occ_uv0));
+    HIP_TRY(upload(ctx, d_tex_info, tex_info));
+    HIP_TRY(upload(ctx, d_tex_bits, tex_bits));
+    sc.occ_tex = d_occ_tex.p;
+    sc.occ_uv0 = d_occ_uv0.p;
+    sc.tex = d_tex_info.p;
+    sc.tex_bits = d_tex_bits.p;
+  }
   DevOcc occ{nullptr, nullptr, nullptr};
   if (bounce_in_scene) {
     HIP_TRY(upload(ctx, d_occ_nrm, occ_nrm));
@@ -831,11 +867,16 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   // ---- sun maps (sun_map.rs:25-75) and sky probes (sky_probes.rs:146-191), baker.rs:104-152 --
   if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) {
     bool sky = false;
+    uint32_t sky_alpha = 0;
     for (uint32_t l = nlocal; l < nl; ++l) {
-      if (hlights[l].shape == DL_SHAPE_SKY) sky = true;
+      if (hlights[l].shape == DL_SHAPE_SKY) {
+        sky = true;
+        sky_alpha |= hlights[l].alpha;
+      }
       if (hlights[l].shape != DL_SHAPE_SUN) continue;
       dl_sun_map_out m{};
       SunMapParams sp{};
+      sp.alpha = hlights[l].alpha;
       if (!sun_map_setup(occ9.data(), (size_t)nocc * 3, hlights[l].toward, opt.sun_map_resolution, &sp, m.rows)) {
         m.resolution = 1;
         m.texels.push_back(pack_sun_depth(1.0f, 1.0f));
@@ -863,6 +904,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       pp.origin[1] = origin.y;
       pp.origin[2] = origin.z;
       pp.samples = opt.sky_samples;
+      pp.alpha = sky_alpha;
       pp.seed = opt.seed;
       const size_t n = (size_t)pp.counts[0] * pp.counts[1] * pp.counts[2];
       DBuf<float> d_open;

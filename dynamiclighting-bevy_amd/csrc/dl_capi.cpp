@@ -200,7 +200,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   if (!bvh.tris.empty()) (void)hipMemcpy(t.p, bvh.tris.data(), bvh.tris.size() * 4, hipMemcpyHostToDevice);
   if (ray_count) (void)hipMemcpy(r.p, rays8, ray_count * 32, hipMemcpyHostToDevice);
   (void)hipMemset(ovf.p, 0, 4);
-  DevScene sc;
+  DevScene sc{};
   sc.nodes = n.p;
   sc.tris = reinterpret_cast<const float4*>(t.p);
   sc.ntris = tri_count;

@@ -42,6 +42,13 @@ struct DevScene {
   uint32_t stack16;  // 1: 16-bit stack entries (fewer than 65536 nodes), half the LDS per wave
   uint32_t* overflow;
   unsigned long long* counters;  // null, or [node fetches, triangle tests] for instrumented runs
+  // Alpha-tested occluders (null when the scene has none): per occluder its texture or ~0u, the
+  // UV0 of its corners (6 floats), per texture {width, height, first word of its bits, 0}, and
+  // one bit per texel (alpha / 255 >= cutoff).
+  const uint32_t* occ_tex;
+  const float* occ_uv0;
+  const uint32_t* tex;
+  const uint32_t* tex_bits;
 };
 
 // Receiver triangles of the meshes being baked (SoA, one entry per triangle).
@@ -181,6 +188,7 @@ struct SunMapParams {
   float min[3];
   float texel, depth;
   uint32_t res;
+  uint32_t alpha;
 };
 hipError_t launch_sun_map(hipStream_t s, DevScene scene, const SunMapParams& p, uint32_t* out, uint64_t words);
 // Sky probe grid (sky_probes.rs:46-106): per (probe, axis) openness and back-face hit count.
@@ -189,6 +197,7 @@ struct SkyProbeParams {
   float spacing;
   uint32_t counts[3];
   uint32_t samples;
+  uint32_t alpha;
   uint64_t seed;
 };
 hipError_t launch_sky_probes(hipStream_t s, DevScene scene, TrigTables trig, const SkyProbeParams& p, float* open, uint32_t* back);

@@ -43,9 +43,34 @@ struct Tally {
   uint32_t nodes = 0, tris = 0;
 };
 
+// Texel index of a repeating texture coordinate (input.rs:90-97: rem_euclid, floor, clamp).
+__device__ __forceinline__ uint32_t wrap_texel(float c, uint32_t size) {
+  float r = fmodf(c, 1.0f);
+  if (r < 0.0f) r = r + 1.0f;
+  const float texel = floorf(r * (float)size);
+  if (!finite_(texel)) return 0u;
+  const uint32_t v = sat_u32(texel);
+  return v < size - 1 ? v : size - 1;
+}
+// Whether a hit with barycentrics ((1-u)-v, u, v) on occluder `occ` lands on an opaque texel of
+// its alpha texture (occluder.rs:147-160, AlphaTexture::is_opaque input.rs:79-88).
+__device__ __noinline__ bool alpha_blocks(const DevScene& sc, uint32_t occ, float u, float v) {
+  if (!sc.occ_tex) return true;
+  const uint32_t t = sc.occ_tex[occ];
+  if (t == 0xffffffffu) return true;
+  const float* uv = sc.occ_uv0 + 6 * (size_t)occ;
+  const float wx = (1.0f - u) - v;
+  const float ux = (uv[0] * wx + uv[2] * u) + uv[4] * v;
+  const float uy = (uv[1] * wx + uv[3] * u) + uv[5] * v;
+  const uint32_t* ti = sc.tex + 4 * (size_t)t;
+  const uint64_t bit = (uint64_t)wrap_texel(uy, ti[1]) * ti[0] + wrap_texel(ux, ti[0]);
+  return (sc.tex_bits[ti[2] + bit / 32] >> (bit % 32)) & 1u;
+}
+
+// `alpha`: the light's rays pass transparent texels of alpha-tested occluders.
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
-                                         float tmax, Hit* hit, Tally& tl) {
+                                         float tmax, Hit* hit, Tally& tl, uint32_t alpha = 0) {
   if (sc.ntris == 0) return false;
   const float ix = inv_dir(d.x), iy = inv_dir(d.y), iz = inv_dir(d.z);
   bool have = false;
@@ -91,6 +116,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
           tl.tris++;
           if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
           if (!finite_(t)) continue;
+          if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
           if (ANY) return true;
           const uint32_t occ = __float_as_uint(a.w);
           if (!have || t < bt || (t == bt && occ < hit->occ)) {
@@ -417,7 +443,7 @@ __device__ bool area_visibility(const DevScene& sc, uint32_t* stack, uint32_t st
     float weight = (receiver * emitter) / dsq;
     nr++;
     Hit h;
-    bool lit = !traverse<true>(sc, stack, stride, origin, dir, maxf_(distance - kEndGap, 0.0f), &h, tl);
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, maxf_(distance - kEndGap, 0.0f), &h, tl, L.alpha);
     if (!probe) {
       s.total += weight;
       s.lit += lit ? weight : 0.0f;
@@ -455,7 +481,7 @@ __device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t str
   Hit h;
   if (L.srad <= 0.0f) {
     if (dot(normal, toward) <= kNdotlTolerance) return false;
-    *fraction = traverse<true>(sc, stack, stride, origin, toward, INFINITY, &h, tl) ? 0.0f : 1.0f;
+    *fraction = traverse<true>(sc, stack, stride, origin, toward, INFINITY, &h, tl, L.alpha) ? 0.0f : 1.0f;
     *rays = 1;
     return true;
   }
@@ -468,7 +494,7 @@ __device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t str
     float receiver = dot(normal, dir) - kNdotlTolerance;
     if (receiver <= 0.0f) return;
     nr++;
-    bool lit = !traverse<true>(sc, stack, stride, origin, dir, INFINITY, &h, tl);
+    bool lit = !traverse<true>(sc, stack, stride, origin, dir, INFINITY, &h, tl, L.alpha);
     if (!probe) {
       s.total += receiver;
       s.lit += lit ? receiver : 0.0f;
@@ -573,7 +599,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
             if (dot(nrm, dir) > kNdotlTolerance) {
               some = true;
               rays = 1;
-              level = traverse<true>(sc, stack, blockDim.x, origin, dir, distance, &h, tl) ? 0u : 1u;
+              level = traverse<true>(sc, stack, blockDim.x, origin, dir, distance, &h, tl, L.alpha) ? 0u : 1u;
             }
           }
         }
@@ -591,7 +617,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
           uint32_t open = 0;
           for (uint32_t taken = 0; taken < p.sky_samples; ++taken) {
             f3 dir = cosine_dir(trig, nrm, spread_pt(taken, offset, true));
-            open += traverse<true>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) ? 0u : 1u;
+            open += traverse<true>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl, L.alpha) ? 0u : 1u;
           }
           fraction = (float)open / (float)(p.sky_samples > 1 ? p.sky_samples : 1u);
           rays = p.sky_samples;
@@ -785,7 +811,7 @@ __global__ void k_distance_cubes(DevScene sc, const DevLight* __restrict__ light
       uint32_t texel = pair * 2 + k;
       f3 dir = cube_texel_dir(face, texel % res, texel / res, res);
       Hit h;
-      d[k] = traverse<false>(sc, stack, blockDim.x, o, dir, L.radius, &h, tl) ? h.t : L.radius;
+      d[k] = traverse<false>(sc, stack, blockDim.x, o, dir, L.radius, &h, tl, L.alpha) ? h.t : L.radius;
     }
     out[c] = (f32_to_f16_bits(d[1]) << 16) | f32_to_f16_bits(d[0]);
   }
@@ -814,7 +840,7 @@ __global__ void k_sun_map(DevScene sc, SunMapParams p, uint32_t* out, uint64_t w
       float down = p.min[1] + ((float)y + 0.5f) * p.texel;
       f3 origin = add(add(mul(side, across), mul(up, down)), mul(along, p.min[2]));
       Hit h;
-      if (traverse<false>(sc, stack, blockDim.x, origin, along, p.depth, &h, tl)) d[k] = h.t / p.depth;
+      if (traverse<false>(sc, stack, blockDim.x, origin, along, p.depth, &h, tl, p.alpha)) d[k] = h.t / p.depth;
     }
     out[w] = unorm16(d[1]) << 16 | unorm16(d[0]);
   }
@@ -848,7 +874,7 @@ __global__ void k_sky_probes(DevScene sc, TrigTables trig, SkyProbeParams p, flo
     for (uint32_t taken = 0; taken < p.samples; ++taken) {
       f3 dir = cosine_dir(trig, normal, spread_pt(taken, offset, true));
       Hit h;
-      bool occluded = traverse<false>(sc, stack, blockDim.x, position, dir, INFINITY, &h, tl) && h.t > 0.0f;
+      bool occluded = traverse<false>(sc, stack, blockDim.x, position, dir, INFINITY, &h, tl, p.alpha) && h.t > 0.0f;
       if (occluded && dot(tri_face_normal(sc, h.pos), dir) > 0.0f) nback++;
       if (!occluded) nopen++;
     }
@@ -1038,7 +1064,7 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
     f3 dir = cube_texel_dir(face, x, y, res);
     Hit h;
     uint32_t w0, w1 = 0, w2 = 0;
-    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h, tl)) {
+    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h, tl, L.alpha)) {
       w0 = __float_as_uint(L.radius);
     } else {
       f3 normal = surface_normal(sc, occ, h);
@@ -1195,7 +1221,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
             if (is_zero(pn) || distance <= 0.0f) continue;
             f3 d = divs(to_photon, distance);
             rays++;
-            if (!traverse<true>(sc, stack, blockDim.x, origin, d, distance, &h, tl)) {
+            if (!traverse<true>(sc, stack, blockDim.x, origin, d, distance, &h, tl, L.alpha)) {
               float weight = maxf_(dot(normal, d), 0.0f) * maxf_(dot(pn, neg(d)), 0.0f);
               f3 alb = p.colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f,
                                        (float)((w2 >> 16) & 0xffu) / 255.0f)
@@ -1214,7 +1240,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
         for (uint32_t k = 0; k < samples; ++k) {
           f3 direction = cosine_dir(trig, normal, spread_pt(k, offset, true));
           rays++;
-          if (!traverse<false>(sc, stack, blockDim.x, origin, direction, INFINITY, &h, tl)) continue;
+          if (!traverse<false>(sc, stack, blockDim.x, origin, direction, INFINITY, &h, tl, L.alpha)) continue;
           f3 face = tri_face_normal(sc, h.pos);
           if (dot(face, direction) >= 0.0f) continue;
           float lit = dot(surface_normal(sc, occ, h), toward);
@@ -1222,7 +1248,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
           f3 point = add(add(origin, mul(direction, h.t)), mul(face, kRayOffset));
           rays++;
           Hit h2;
-          if (!traverse<true>(sc, stack, blockDim.x, point, toward, INFINITY, &h2, tl)) {
+          if (!traverse<true>(sc, stack, blockDim.x, point, toward, INFINITY, &h2, tl, L.alpha)) {
             f3 alb = mk3(1.0f, 1.0f, 1.0f);
             if (p.colored) {
               const float* a = occ.albedo + 3 * (size_t)h.occ;
@@ -1269,7 +1295,8 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
 // One BVH node of an any-hit traversal (the loop body of traverse<true>).  Returns 0 while the
 // traversal continues (node/sp updated), 1 on a hit, 2 when the ray is unoccluded.
 __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
-                                              float ix, float iy, float iz, float tmax, int32_t& node, uint32_t& sp, Tally& tl) {
+                                              float ix, float iy, float iz, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
+                                              uint32_t alpha) {
   uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
   const bool s16 = sc.stack16 != 0;
   tl.nodes++;
@@ -1306,7 +1333,9 @@ __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __re
         float t, u, vv;
         tl.tris++;
         if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
-        if (finite_(t)) return 1u;
+        if (!finite_(t)) continue;
+        if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
+        return 1u;
       }
     } else if (next < 0) {
       next = c;
@@ -1466,7 +1495,7 @@ __global__ void k_gather_stream(DevScene sc, DevTris tr, DevMeshes ms, const Dev
   bool job = false;
   GatherJob g;
   Pcg rng;
-  uint32_t k = 0, mesh = 0, jrays = 0;
+  uint32_t k = 0, mesh = 0, jrays = 0, jalpha = 0;
   uint64_t dst = 0;
   float intensity = 0.0f;
   f3 sum = mk3(0.0f, 0.0f, 0.0f);
@@ -1502,6 +1531,7 @@ __global__ void k_gather_stream(DevScene sc, DevTris tr, DevMeshes ms, const Dev
             const uint32_t xy = tx.xy[jb.x];
             dst = b.win_off[w] + (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
             intensity = lights[ent.light[jb.y]].bounce_intensity;
+            jalpha = lights[ent.light[jb.y]].alpha;
             sum = mk3(0.0f, 0.0f, 0.0f);
             k = 0;
             jrays = 0;
@@ -1554,7 +1584,7 @@ __global__ void k_gather_stream(DevScene sc, DevTris tr, DevMeshes ms, const Dev
       }
     }
     if (live) {
-      const uint32_t r = visit_any(sc, stack, blockDim.x, g.origin, d, ix, iy, iz, tmax, node, sp, tl);
+      const uint32_t r = visit_any(sc, stack, blockDim.x, g.origin, d, ix, iy, iz, tmax, node, sp, tl, jalpha);
       if (r != 0u) {
         live = false;
         if (r == 2u) sum = add(sum, contrib);

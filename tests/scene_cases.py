@@ -120,6 +120,37 @@ def maps_case():
                                            area_light_samples=4, area_light_min_samples=4)
 
 
+def half_transparent(mesh_id, center, half, rgba=(255, 255, 255, 0, 255, 255, 255, 255), width=2, height=1):
+    """A plane whose UV0 is its UV1 and whose alpha texture is transparent on one side
+    (src/bake/tests/occluder.rs:87-97)."""
+    plane = scenes.floor(mesh_id, center, half)
+    plane.uv0 = plane.uv1.copy()
+    plane.alpha = (width, height, np.array(rgba, np.uint8), 0.5)
+    return plane
+
+
+def alpha_case():
+    """A half-transparent screen over a floor: alpha-tested point, area, sun and sky lights pass
+    the transparent half; a light without transparency is blocked by both halves.  A tiled 3x5
+    texture with repeat wrap and UV0 outside the unit square exercises the wrap."""
+    screen = half_transparent(2, (0.0, 1.0, 0.0), 1.0)
+    tiles = half_transparent(3, (2.5, 0.8, 2.5), 0.6, width=3, height=5,
+                             rgba=[v for i in range(15) for v in (255, 255, 255, 255 if (i * 7) % 3 else 40)])
+    tiles.uv0 = tiles.uv1 * np.float32(2.7) - np.float32(0.35)
+    meshes = [scenes.floor(1, (0, 0, 0), 4.0), screen, tiles]
+    pa = scenes.point_light(60, (0.1, 2.5, 0.2), 6.0)
+    pa.transparency = abi.TRANSPARENCY_ENABLED
+    pb = scenes.point_light(61, (-0.2, 2.2, -0.1), 6.0)
+    area = scenes.area_light(62, (2.4, 2.0, 2.6), 6.0, scenes.quat_from_rotation_x(-math.pi / 2), (0.6, 0.4))
+    area.transparency = abi.TRANSPARENCY_ENABLED
+    sun = scenes.sun_light(63, (0.2, 1.0, 0.1), 1.0)
+    sun.transparency = abi.TRANSPARENCY_ENABLED_MAX
+    sky = scenes.sky_light(64)
+    sky.transparency = abi.TRANSPARENCY_ENABLED
+    return BakeInput(meshes, [pa, pb, area, sun, sky]), opts(10.0, area_light_samples=8, area_light_min_samples=4,
+                                                            sky_samples=16, sun_map_resolution=48, sky_probe_spacing=0.5)
+
+
 CASES = {"point": point_case, "panel": panel_case, "sun": sun_case, "sky": sky_case, "mixed": mixed_case,
          "odd_geometry": odd_geometry_case, "bounce_point": bounce_point_case, "bounce_sun": bounce_sun_case,
-         "bounce_dither": bounce_dither_case, "bounce_white_dither": bounce_white_dither_case, "maps": maps_case}
+         "bounce_dither": bounce_dither_case, "bounce_white_dither": bounce_white_dither_case, "maps": maps_case, "alpha": alpha_case}

@@ -142,3 +142,41 @@ def test_empty_scenes_bake():
     assert s.meshes == [] and s.stats["triangles"] == 0 and len(s.light_bvh) == 0
     s = oracle.bake(BakeInput([scenes.floor(1, (0, 0, 0), 1.0)], []), abi.default_options())
     assert len(s.meshes) == 1 and mirror.triangle_count(s.meshes[0].triangles) == 2
+
+
+def test_alpha_tested_lights_pass_transparent_texels():
+    """occluder.rs:99-121 at the level of a bake: under a screen whose x < 0 half is transparent,
+    the floor is lit on that side and shadowed on the other by an alpha-tested light straight
+    above, while a light without transparency is blocked by the whole screen."""
+    screen = scene_cases.half_transparent(2, (0.0, 1.0, 0.0), 1.0)
+    ol = oracle.load()
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    for transparency, lit_left in ((abi.TRANSPARENCY_ENABLED, True), (abi.TRANSPARENCY_DISABLED, False)):
+        light = scenes.point_light(10, (0.0, 3.0, 0.0), 6.0)
+        light.transparency = transparency
+        fl = scenes.floor(1, (0, 0, 0), 2.0)
+        o = scene_cases.opts(8.0)
+        o.dynamic_geometry = abi.DYNAMIC_LIGHTING_ONLY
+        baked = oracle.bake(BakeInput([fl, screen], [light]), o)
+        mesh = next(m for m in baked.meshes if m.id == 1)
+        left, right = [], []
+        for tri in range(2):
+            idx = fl.indices[tri * 3: tri * 3 + 3]
+            v = np.ascontiguousarray(fl.positions[idx].reshape(-1))
+            uv = np.ascontiguousarray(fl.uv1[idx].reshape(-1))
+            n = np.ascontiguousarray(np.tile(np.array([0, 1, 0], np.float32), 3))
+            out = np.zeros((8192, 8), np.float32)
+            cnt = ol.dlo_triangle_texels(fp(v), fp(n), fp(uv), fp(np.array([0, 1, 0], np.float32)), mesh.resolution,
+                                         fp(out), 8192)
+            view = mirror.TriangleView(mesh.triangles, tri, False)
+            entry = next((e for e in view.entries() if e.light_index == 0), None)
+            for r in out[:cnt]:
+                x, y, p = int(r[0]), int(r[1]), r[2:5]
+                lit = entry is not None and view.shadow_bit(entry, x, y) is True
+                if abs(p[2]) < 0.5 and -0.8 < p[0] < -0.2:
+                    left.append(lit)
+                if abs(p[2]) < 0.5 and 0.2 < p[0] < 0.8:
+                    right.append(lit)
+        assert len(left) > 20 and len(right) > 20
+        assert all(v == lit_left for v in left), transparency
+        assert not any(right)
