@@ -674,25 +674,71 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           return DL_E_DEVICE;
         }
         HIP_TRY(zeroed(ctx, d_jobc, 4));
-        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), 0, d_jobc.p, nullptr, nullptr));
-        std::vector<unsigned long long> jc;
-        HIP_TRY(download(ctx, jc, d_jobc.p, 2));
+        const uint64_t nb1 = bb.n + 1;
+        DBuf<uint64_t> d_jcnt, d_joff;
+        HIP_TRY(d_jcnt.alloc(2 * nb1));
+        HIP_TRY(d_joff.alloc(2 * nb1));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 0, d_jcnt.p, nullptr, nullptr, nullptr));
+        HIP_TRY(scan_u64(s, d_jcnt.p, d_joff.p, bb.n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+        HIP_TRY(scan_u64(s, d_jcnt.p + nb1, d_joff.p + nb1, bb.n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+        std::vector<uint64_t> jc(2);
+        HIP_TRY(hipMemcpyAsync(&jc[0], d_joff.p + bb.n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&jc[1], d_joff.p + nb1 + bb.n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
         ntjobs = jc[0];
         HIP_TRY(d_tjobs.alloc(std::max<uint64_t>(1, jc[0])));
         HIP_TRY(d_cjobs.alloc(std::max<uint64_t>(1, jc[1])));
-        HIP_TRY(hipMemsetAsync(d_jobc.p, 0, 4 * sizeof(unsigned long long), s));
-        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), 1, d_jobc.p, d_tjobs.p,
-                                   d_cjobs.p));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 1, nullptr, d_joff.p, d_tjobs.p, d_cjobs.p));
         HIP_TRY(launch_gather_count(s, tr, ms, d_lights.p, tx, ent, bb, d_cjobs.p, jc[1], bp, ctx->trig));
         bp.sun_only = 1;
       }
       HIP_TRY(hipEventRecord(ev0, s));
       if (ctx->gather_mode == 1) {
-        HIP_TRY(launch_gather_stream(s, scb, tr, ms, d_lights.p, tx, ent, bb, d_tjobs.p, ntjobs, d_jobc.p + 2, bp, ctx->trig,
-                                     ctx->gather));
+        // chunks of traced jobs: rays and contributions, any-hit visibility, ordered sums
+        uint32_t max_samples = 2;
+        for (uint64_t e : be) max_samples = std::max(max_samples, hlights[b_ent_light[e]].bounce_samples);
+        const uint64_t per_chunk = std::max<uint64_t>(1, ctx->gather.chunk_rays / max_samples);
+        const uint64_t cj = std::min<uint64_t>(per_chunk, std::max<uint64_t>(1, ntjobs));
+        DBuf<uint64_t> d_nr, d_rs;
+        DBuf<float4> d_rays;
+        DBuf<float> d_contrib;
+        HIP_TRY(d_nr.alloc(cj + 1));
+        HIP_TRY(d_rs.alloc(cj + 1));
+        HIP_TRY(d_rays.alloc(2 * cj * max_samples));
+        HIP_TRY(d_contrib.alloc(3 * cj * max_samples));
+        std::vector<hipEvent_t> tev;
+        struct EvList {
+          std::vector<hipEvent_t>& v;
+          ~EvList() { for (hipEvent_t e : v) (void)hipEventDestroy(e); }
+        } tev_guard{tev};
+        for (uint64_t j0 = 0; j0 < ntjobs; j0 += cj) {
+          const uint64_t n = std::min(cj, ntjobs - j0);
+          const uint2* jobs = d_tjobs.p + j0;
+          HIP_TRY(launch_gather_nrays(s, ent, d_lights.p, jobs, n, d_nr.p));
+          HIP_TRY(scan_u64(s, d_nr.p, d_rs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+          HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p));
+          HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, sizeof(unsigned long long), s));
+          hipEvent_t a, b2;
+          HIP_TRY(hipEventCreate(&a));
+          tev.push_back(a);
+          HIP_TRY(hipEventCreate(&b2));
+          tev.push_back(b2);
+          HIP_TRY(hipEventRecord(a, s));
+          HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_rs.p + n, n * max_samples, d_contrib.p,
+                                    reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather));
+          HIP_TRY(hipEventRecord(b2, s));
+          HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p));
+        }
         if (sun_bounce)
           HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
                                        ctx->trace_block));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (size_t k = 0; k + 1 < tev.size(); k += 2) {
+          float ms_k = 0.0f;
+          (void)hipEventElapsedTime(&ms_k, tev[k], tev[k + 1]);
+          ctx->timings.gather_trace_ms += ms_k;
+          ctx->timings.gather_trace_launches++;
+        }
       } else {
         HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
                                      ctx->trace_block));

@@ -1,13 +1,19 @@
-// Host build of the occluder BVH: binned SAH over centroids into a binary tree with leaves of at
-// most four triangles, then collapsed to 4-wide nodes by opening the largest children first.
-// Subtrees below a size threshold are built on worker threads.
+// Host build of the occluder BVH: large triangles are first split into several references, each
+// bounded by the part of the triangle inside half of its parent reference's box (clipped in
+// double precision, bounds rounded outward); then binned SAH over reference centroids into a
+// binary tree with leaves of at most four references, collapsed to 4-wide nodes by opening the
+// largest children first.  Subtrees below a size threshold are built on worker threads.  A
+// triangle split into several references appears in several leaves; any-hit and closest-hit
+// (lowest t, then lowest occluder index) results do not depend on that.
 #include "dl_bvh.h"
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <queue>
 #include <thread>
 
 namespace dl {
@@ -152,6 +158,93 @@ struct Builder {
   }
 };
 
+// Bounds of the part of triangle `t` (9 floats) inside the box [lo, hi], clipped in double;
+// false when the part is empty.  The result is rounded outward to float.
+bool clip_bounds(const float* t, const double lo[3], const double hi[3], Box* out) {
+  double poly[12][3], tmp[12][3];
+  int n = 3;
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) poly[k][a] = t[3 * k + a];
+  for (int plane = 0; plane < 6 && n > 0; ++plane) {
+    const int axis = plane >> 1;
+    const bool upper = plane & 1;
+    const double bound = upper ? hi[axis] : lo[axis];
+    auto inside = [&](const double* v) { return upper ? v[axis] <= bound : v[axis] >= bound; };
+    int m = 0;
+    for (int k = 0; k < n; ++k) {
+      const double* a = poly[k];
+      const double* b = poly[(k + 1) % n];
+      const bool ia = inside(a), ib = inside(b);
+      if (ia) {
+        for (int c = 0; c < 3; ++c) tmp[m][c] = a[c];
+        m++;
+      }
+      if (ia != ib) {
+        const double f = (bound - a[axis]) / (b[axis] - a[axis]);
+        for (int c = 0; c < 3; ++c) tmp[m][c] = a[c] + (b[c] - a[c]) * f;
+        tmp[m][axis] = bound;
+        m++;
+      }
+    }
+    n = m;
+    for (int k = 0; k < n; ++k)
+      for (int c = 0; c < 3; ++c) poly[k][c] = tmp[k][c];
+  }
+  if (n == 0) return false;
+  for (int a = 0; a < 3; ++a) {
+    double mn = poly[0][a], mx = poly[0][a];
+    for (int k = 1; k < n; ++k) {
+      mn = std::min(mn, poly[k][a]);
+      mx = std::max(mx, poly[k][a]);
+    }
+    float fl = (float)mn, fh = (float)mx;
+    if ((double)fl > mn) fl = std::nextafter(fl, -INFINITY);
+    if ((double)fh < mx) fh = std::nextafter(fh, INFINITY);
+    out->lo[a] = fl;
+    out->hi[a] = fh;
+  }
+  return true;
+}
+
+// Splits references of triangles whose box has a surface area above `factor` times the mean
+// triangle box area, largest first, into at most `budget` extra references.
+void split_refs(const float* tris9, uint32_t n, float factor, uint64_t budget, std::vector<uint32_t>& ref_tri, std::vector<Box>& ref_box) {
+  double mean = 0.0;
+  for (uint32_t i = 0; i < n; ++i) mean += ref_box[i].area();
+  mean /= std::max<uint32_t>(1, n);
+  const double limit = mean * factor;
+  if (!(limit > 0.0) || budget == 0) return;
+  using Item = std::pair<float, uint32_t>;  // area, reference
+  std::priority_queue<Item> heap;
+  for (uint32_t i = 0; i < n; ++i)
+    if (ref_box[i].area() > limit) heap.push({ref_box[i].area(), i});
+  while (!heap.empty() && budget > 0) {
+    const uint32_t r = heap.top().second;
+    heap.pop();
+    const Box b = ref_box[r];
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+      if (b.hi[a] - b.lo[a] > b.hi[axis] - b.lo[axis]) axis = a;
+    const double mid = 0.5 * ((double)b.lo[axis] + (double)b.hi[axis]);
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = b.lo[a]; hi[a] = b.hi[a]; }
+    const float* t = tris9 + (size_t)ref_tri[r] * 9;
+    Box left, right;
+    hi[axis] = mid;
+    const bool has_l = clip_bounds(t, lo, hi, &left);
+    hi[axis] = b.hi[axis];
+    lo[axis] = mid;
+    const bool has_r = clip_bounds(t, lo, hi, &right);
+    if (!has_l || !has_r) continue;  // a sliver: keep the reference whole
+    ref_box[r] = left;
+    ref_tri.push_back(ref_tri[r]);
+    ref_box.push_back(right);
+    budget--;
+    if (left.area() > limit) heap.push({left.area(), r});
+    if (right.area() > limit) heap.push({right.area(), (uint32_t)ref_box.size() - 1});
+  }
+}
+
 void pad(const Box& b, float* lo, float* hi) {
   for (int a = 0; a < 3; ++a) {
     float p = (std::fabs(b.lo[a]) + std::fabs(b.hi[a]) + (b.hi[a] - b.lo[a])) * (1.0f / 32768.0f) + 1e-30f;
@@ -170,26 +263,35 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
   if (n == 0) return;
   Builder b;
   b.tri = tris9;
+  std::vector<uint32_t> ref_tri(n);
   b.tb.resize(n);
-  b.cen.resize((size_t)n * 3);
-  b.idx.resize(n);
   for (uint32_t i = 0; i < n; ++i) {
     const float* t = tris9 + (size_t)i * 9;
     for (int a = 0; a < 3; ++a) {
       float v0 = t[a], v1 = t[3 + a], v2 = t[6 + a];
       b.tb[i].lo[a] = std::min(v0, std::min(v1, v2));
       b.tb[i].hi[a] = std::max(v0, std::max(v1, v2));
-      b.cen[(size_t)i * 3 + a] = 0.5f * (b.tb[i].lo[a] + b.tb[i].hi[a]);
     }
+    ref_tri[i] = i;
+  }
+  float factor = 4.0f, budget_share = 1.0f;
+  if (const char* e = std::getenv("DL_BVH_SPLIT")) factor = (float)std::atof(e);
+  if (const char* e = std::getenv("DL_BVH_SPLIT_BUDGET")) budget_share = (float)std::atof(e);
+  if (factor > 0.0f) split_refs(tris9, n, factor, (uint64_t)(budget_share * (float)n), ref_tri, b.tb);
+  const uint32_t nr = (uint32_t)ref_tri.size();
+  b.cen.resize((size_t)nr * 3);
+  b.idx.resize(nr);
+  for (uint32_t i = 0; i < nr; ++i) {
+    for (int a = 0; a < 3; ++a) b.cen[(size_t)i * 3 + a] = 0.5f * (b.tb[i].lo[a] + b.tb[i].hi[a]);
     b.idx[i] = i;
   }
   b.nodes.reserve((size_t)n * 2 / 3 + 16);
   b.nodes.emplace_back();
   // Top of the tree on this thread until subtrees are small, then the rest in parallel.
   if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
-  uint32_t spill_below = threads > 1 ? std::max<uint32_t>(4096, n / (threads * 8)) : 0;
+  uint32_t spill_below = threads > 1 ? std::max<uint32_t>(4096, nr / (threads * 8)) : 0;
   std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> spill;
-  b.build_subtree(0, 0, n, threads > 1 ? &spill : nullptr, spill_below);
+  b.build_subtree(0, 0, nr, threads > 1 ? &spill : nullptr, spill_below);
   if (!spill.empty()) {
     std::atomic<size_t> next{0};
     auto work = [&]() {
@@ -204,11 +306,12 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
     work();
     for (auto& t : ts) t.join();
   }
-  // Leaf-ordered triangles.
-  out.order = b.idx;
-  out.tris.resize((size_t)n * 12);
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t p = b.idx[k];
+  // Leaf-ordered triangles (one record per reference).
+  out.order.resize(nr);
+  for (uint32_t k = 0; k < nr; ++k) out.order[k] = ref_tri[b.idx[k]];
+  out.tris.resize((size_t)nr * 12);
+  for (uint32_t k = 0; k < nr; ++k) {
+    uint32_t p = out.order[k];
     const float* t = tris9 + (size_t)p * 9;
     float* o = out.tris.data() + (size_t)k * 12;
     o[0] = t[0]; o[1] = t[1]; o[2] = t[2];

@@ -24,8 +24,8 @@ static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
 
 struct Bvh4 {
   std::vector<Node4> nodes;
-  std::vector<float> tris;      // 12 floats per triangle, leaf order
-  std::vector<uint32_t> order;  // leaf position -> occluder index
+  std::vector<float> tris;      // 12 floats per reference, leaf order
+  std::vector<uint32_t> order;  // leaf position -> occluder index (a split triangle appears more than once)
   uint32_t depth = 0;           // node levels on the longest root-to-leaf path
 };
 

@@ -19,9 +19,8 @@ int set_err(dl_ctx* ctx, int code, const std::string& msg) {
 int upload_trig(dl_ctx* ctx) {
   using namespace dl;
   const TrigHost& t = ctx->trig_host;
-  // seven arrays, each followed by one spare word
-  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() +
-                 t.flags.size() + 7;
+  // six arrays, each followed by one spare word
+  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() + 6;
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
@@ -39,7 +38,7 @@ int upload_trig(dl_ctx* ctx) {
   ctx->trig.cos.bucket = put(t.cos_bucket);
   ctx->trig.cos.keys = put(t.cos_keys);
   ctx->trig.cos.vals = put(t.cos_vals);
-  ctx->trig.flags = put(t.flags);
+  ctx->trig.band = t.band;
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
 }
@@ -239,15 +238,53 @@ int dl_test_trig(dl_ctx* ctx, const float* x, uint64_t n, float* sin_out, float*
   return DL_OK;
 }
 
-int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out) {
-  using namespace dl;
+namespace {
+const dl::TrigHost& host_trig() {
   static std::once_flag once;
-  static TrigHost host;
-  std::call_once(once, [] { build_trig_tables(host, std::max(1u, std::thread::hardware_concurrency())); });
+  static dl::TrigHost host;
+  std::call_once(once, [] { dl::build_trig_tables(host, std::max(1u, std::thread::hardware_concurrency())); });
+  return host;
+}
+}  // namespace
+
+int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches) {
+  using namespace dl;
+  const TrigHost& host = host_trig();
   TrigTables t;
   t.sin = {host.sin_bucket.data(), host.sin_keys.data(), host.sin_vals.data()};
   t.cos = {host.cos_bucket.data(), host.cos_keys.data(), host.cos_vals.data()};
-  t.flags = host.flags.data();
+  t.band = host.band;
+  const unsigned n = std::max(1u, std::thread::hardware_concurrency());
+  std::vector<uint64_t> bad(n, 0);
+  std::vector<std::thread> ts;
+  for (unsigned k = 0; k < n; ++k)
+    ts.emplace_back([&, k] {
+      for (uint64_t b = (uint64_t)lo + k; b <= hi; b += n) {
+        const float x = u2f((uint32_t)b);
+        if (f2u(lsinf(t, x)) != f2u(sinf(x))) bad[k]++;
+        if (f2u(lcosf(t, x)) != f2u(cosf(x))) bad[k]++;
+      }
+    });
+  for (auto& th : ts) th.join();
+  *mismatches = 0;
+  for (uint64_t v : bad) *mismatches += v;
+  return DL_OK;
+}
+
+uint64_t dl_test_trig_keys(int which, uint32_t* out, uint64_t cap) {
+  const dl::TrigHost& host = host_trig();
+  const std::vector<uint32_t>& keys = which ? host.cos_keys : host.sin_keys;
+  if (out) std::memcpy(out, keys.data(), std::min<uint64_t>(cap, keys.size()) * 4);
+  return keys.size();
+}
+
+int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out) {
+  using namespace dl;
+  const TrigHost& host = host_trig();
+  TrigTables t;
+  t.sin = {host.sin_bucket.data(), host.sin_keys.data(), host.sin_vals.data()};
+  t.cos = {host.cos_bucket.data(), host.cos_keys.data(), host.cos_vals.data()};
+  t.band = host.band;
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);
     cos_out[i] = lcosf(t, x[i]);
@@ -277,18 +314,24 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
   return DL_OK;
 }
 
-int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint32_t refill, uint32_t blocks_per_cu) {
+int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint64_t chunk_rays, uint32_t blocks_per_cu,
+                  uint32_t min_waves, uint32_t refill) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   if (mode > 1) return set_err(ctx, DL_E_ARGUMENT, "gather mode must be 0 or 1");
   if (block && (block % 64 != 0 || block > 1024)) return set_err(ctx, DL_E_ARGUMENT, "block must be a multiple of 64 up to 1024");
-  if (refill > 63) return set_err(ctx, DL_E_ARGUMENT, "refill must be below 64");
+  if (chunk_rays && chunk_rays < 4096) return set_err(ctx, DL_E_ARGUMENT, "chunks need at least 4096 rays");
   if (blocks_per_cu > 64) return set_err(ctx, DL_E_ARGUMENT, "at most 64 blocks per CU");
+  if (chunk > (1u << 20)) return set_err(ctx, DL_E_ARGUMENT, "at most 2^20 rays per queue run");
+  if (min_waves != 0 && min_waves != 8) return set_err(ctx, DL_E_ARGUMENT, "min_waves must be 0 or 8");
+  if (refill > 64) return set_err(ctx, DL_E_ARGUMENT, "refill must be at most 64");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->gather_mode = mode;
   if (block) ctx->gather.block = block;
   if (chunk) ctx->gather.chunk = chunk;
-  if (refill) ctx->gather.refill = refill;
+  if (chunk_rays) ctx->gather.chunk_rays = chunk_rays;
   ctx->gather.blocks_per_cu = blocks_per_cu;
+  ctx->gather.min_waves = min_waves;
+  if (refill) ctx->gather.refill = refill;
   return DL_OK;
 }
 

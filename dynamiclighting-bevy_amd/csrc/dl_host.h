@@ -40,7 +40,7 @@ struct DBuf {
 struct TrigHost {
   std::vector<uint32_t> sin_keys, sin_vals, cos_keys, cos_vals;
   std::vector<uint32_t> sin_bucket, cos_bucket;
-  std::vector<uint32_t> flags;  // see TrigTables::flags
+  double band = 0.0;  // see TrigTables::band
 };
 void build_trig_tables(TrigHost& out, unsigned threads);
 

@@ -166,22 +166,32 @@ hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* ligh
 hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,
                                 const DevLight* lights, DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevBounce b,
                                 uint64_t units, BounceParams p, TrigTables trig, uint32_t block);
-// Streamed point-light gather: job lists (pass 0 counts into counts[0..1], pass 1 fills using
-// counts[0..1] as zeroed cursors), count-only jobs, and the persistent traced-job kernel.
+// Split point-light gather.  Jobs are (texel, entry) pairs within the light's radius: pass 0 of
+// launch_gather_jobs writes per bounce entry its traced and count-only job counts (2 (n + 1)
+// words), pass 1 fills both lists at the scanned offsets.  Traced jobs then run in chunks: ray offsets
+// (launch_gather_nrays + scan), rays and contributions (launch_gather_rays), any-hit visibility
+// (launch_any_stream, persistent) and the per-job sums (launch_gather_sum).
 struct GatherLaunch {
-  uint32_t block = 256;        // threads per block
-  uint32_t chunk = 256;        // jobs a wave takes from the queue at once
-  uint32_t refill = 48;        // lanes hand out new samples/jobs when at most this many rays are live
-  uint32_t blocks_per_cu = 0;  // 0: as many as fit
+  uint32_t block = 256;         // threads per block of the any-hit kernel
+  uint32_t chunk = 256;         // rays a wave takes from the queue at once
+  uint32_t blocks_per_cu = 0;   // 0: as many as fit
+  uint32_t min_waves = 0;       // 8: registers capped for 8 waves per SIMD; otherwise uncapped
+  uint32_t refill = 56;         // idle lanes before finished lanes take new rays (1-64)
+  uint64_t chunk_rays = 1ull << 28;  // rays per chunk of jobs (buffer size)
 };
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
-                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts, uint2* trace_jobs,
+                              DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,
                               uint2* count_jobs);
 hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                                DevBounce b, const uint2* jobs, uint64_t njobs, BounceParams p, TrigTables trig);
-hipError_t launch_gather_stream(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx,
-                                DevEntries ent, DevBounce b, const uint2* jobs, uint64_t njobs, unsigned long long* queue,
-                                BounceParams p, TrigTables trig, const GatherLaunch& gl);
+hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out);
+hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
+                              float4* rays, float* contrib);
+hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                             float* contrib, unsigned int* queue, const GatherLaunch& gl);
+hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
+                             uint64_t n, const uint64_t* rs, const float* contrib);
 // Sun map of one sun (sun_map.rs:25-75): frame axes, bounds minimum, texel size and depth.
 struct SunMapParams {
   float side[3], up[3], along[3];

@@ -24,9 +24,11 @@ __device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ start,
   return lo;
 }
 
+// Reciprocal of a direction component for the slab tests (hardware reciprocal, about 1 ulp: the
+// box tests only cull, and their padding covers it).
 __device__ __forceinline__ float inv_dir(float d) {
   float c = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
-  return 1.0f / c;
+  return __builtin_amdgcn_rcpf(c);
 }
 
 struct Hit {
@@ -54,7 +56,7 @@ __device__ __forceinline__ uint32_t wrap_texel(float c, uint32_t size) {
 }
 // Whether a hit with barycentrics ((1-u)-v, u, v) on occluder `occ` lands on an opaque texel of
 // its alpha texture (occluder.rs:147-160, AlphaTexture::is_opaque input.rs:79-88).
-__device__ __noinline__ bool alpha_blocks(const DevScene& sc, uint32_t occ, float u, float v) {
+__device__ __forceinline__ bool alpha_blocks(const DevScene& sc, uint32_t occ, float u, float v) {
   if (!sc.occ_tex) return true;
   const uint32_t t = sc.occ_tex[occ];
   if (t == 0xffffffffu) return true;
@@ -67,91 +69,148 @@ __device__ __noinline__ bool alpha_blocks(const DevScene& sc, uint32_t occ, floa
   return (sc.tex_bits[ti[2] + bit / 32] >> (bit % 32)) & 1u;
 }
 
-// `alpha`: the light's rays pass transparent texels of alpha-tested occluders.
+__device__ __forceinline__ int32_t sel4(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+__device__ __forceinline__ float sel4(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// A ray prepared for slab tests: origin, reciprocal direction and origin * reciprocal.
+struct SlabRay {
+  float ix, iy, iz, ox, oy, oz;
+};
+__device__ __forceinline__ SlabRay slab_ray(f3 o, f3 d) {
+  SlabRay r;
+  r.ix = inv_dir(d.x);
+  r.iy = inv_dir(d.y);
+  r.iz = inv_dir(d.z);
+  r.ox = o.x * r.ix;
+  r.oy = o.y * r.iy;
+  r.oz = o.z * r.iz;
+  return r;
+}
+
+// Box tests of the (up to four) children of `node`: bit k of the result is set when child k's
+// padded slab interval meets [0, limit], with tn[k] its entry distance.  Plane distances are
+// fused multiply-adds b * (1/d) - o * (1/d); their rounding is relative to the larger of |b| and
+// |o| (times 1/d), which the boxes' padding (2^-15 of the coordinate) and the interval's widening
+// (2^-16 of the distance) cover, so the test only culls boxes the ray misses.  The node record
+// is dead once this returns, which keeps it out of the registers of the leaf tests.
+__device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, const SlabRay& r, float limit, float tn[4],
+                                              int4* ch) {
+  const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
+  const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
+  *ch = *reinterpret_cast<const int4*>(nb + 6);
+  const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
+  uint32_t mask = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float t0x = __builtin_fmaf(sel4(lox, k), r.ix, -r.ox), t1x = __builtin_fmaf(sel4(hix, k), r.ix, -r.ox);
+    const float t0y = __builtin_fmaf(sel4(loy, k), r.iy, -r.oy), t1y = __builtin_fmaf(sel4(hiy, k), r.iy, -r.oy);
+    const float t0z = __builtin_fmaf(sel4(loz, k), r.iz, -r.oz), t1z = __builtin_fmaf(sel4(hiz, k), r.iz, -r.oz);
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    n = __builtin_fmaf(fabsf(n), -kBoxEps, n);
+    f = __builtin_fmaf(fabsf(f), kBoxEps, f);
+    tn[k] = n;
+    if (k < cnt && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
+  }
+  return mask;
+}
+
+// Bit k set when child k is a leaf (a negative child reference).
+__device__ __forceinline__ uint32_t leaf_bits(const int4& ch) {
+  return (ch.x < 0 ? 1u : 0u) | (ch.y < 0 ? 2u : 0u) | (ch.z < 0 ? 4u : 0u) | (ch.w < 0 ? 8u : 0u);
+}
+
+// Pushes the hit inner children of a node except the nearest, which it returns (-1 if none).
+__device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t mask, const int4& ch, const float tn[4],
+                                                 uint32_t* __restrict__ stack, uint32_t stride, uint32_t& sp) {
+  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
+  const bool s16 = sc.stack16 != 0;
+  int32_t next = -1;
+  float next_t = INFINITY;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int32_t c = sel4(ch, k);
+    if (!((mask >> k) & 1u) || c < 0) continue;
+    if (next < 0) {
+      next = c;
+      next_t = tn[k];
+      continue;
+    }
+    uint32_t push = (uint32_t)c;
+    if (tn[k] < next_t) {
+      push = (uint32_t)next;
+      next = c;
+      next_t = tn[k];
+    }
+    if (s16) stack16[sp * stride] = (uint16_t)push;
+    else stack[sp * stride] = push;
+    sp++;
+  }
+  if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+  return next;
+}
+
+__device__ __forceinline__ int32_t pop_node(const DevScene& sc, const uint32_t* __restrict__ stack, uint32_t stride, uint32_t sp) {
+  const uint16_t* const stack16 = reinterpret_cast<const uint16_t*>(stack - threadIdx.x) + threadIdx.x;
+  return sc.stack16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
+}
+
+// `alpha`: the light's rays pass transparent texels of alpha-tested occluders.  Children are
+// visited nearest first; leaves are tested before the inner children are pushed.  Neither order
+// changes a result: any-hit reports whether some hit exists and closest-hit keeps the lowest
+// (t, occluder index).
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
                                          float tmax, Hit* hit, Tally& tl, uint32_t alpha = 0) {
   if (sc.ntris == 0) return false;
-  const float ix = inv_dir(d.x), iy = inv_dir(d.y), iz = inv_dir(d.z);
+  const SlabRay sr = slab_ray(o, d);
   bool have = false;
   float bt = tmax;
   uint32_t sp = 0;
   int32_t node = 0;
-  // `stack` is lds + threadIdx.x; in 16-bit mode the same LDS holds half-word entries.
-  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
-  const bool s16 = sc.stack16 != 0;
   for (;;) {
     tl.nodes++;
-    const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
-    const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
-    const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
-    const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
+    float tn[4];
+    int4 ch;
     const float limit = (ANY || !have ? tmax : bt) * (1.0f + kBoxEps);
-    int32_t next = -1;
-    float next_t = INFINITY;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= cnt) break;
-      const float lx = k == 0 ? lox.x : k == 1 ? lox.y : k == 2 ? lox.z : lox.w;
-      const float hx = k == 0 ? hix.x : k == 1 ? hix.y : k == 2 ? hix.z : hix.w;
-      const float ly = k == 0 ? loy.x : k == 1 ? loy.y : k == 2 ? loy.z : loy.w;
-      const float hy = k == 0 ? hiy.x : k == 1 ? hiy.y : k == 2 ? hiy.z : hiy.w;
-      const float lz = k == 0 ? loz.x : k == 1 ? loz.y : k == 2 ? loz.z : loz.w;
-      const float hz = k == 0 ? hiz.x : k == 1 ? hiz.y : k == 2 ? hiz.z : hiz.w;
-      const int32_t c = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
-      const float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
-      const float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
-      const float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-      tn = tn - fabsf(tn) * kBoxEps;
-      tf = tf + fabsf(tf) * kBoxEps;
-      if (!(tn <= tf && tf >= 0.0f && tn <= limit)) continue;
-      if (c < 0) {
-        const uint32_t v = (uint32_t)(-1 - c);
-        const uint32_t first = v >> 2, count = (v & 3u) + 1u;
-        for (uint32_t i = first; i < first + count; ++i) {
-          const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
-          float t, u, vv;
-          tl.tris++;
-          if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
-          if (!finite_(t)) continue;
-          if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
-          if (ANY) return true;
-          const uint32_t occ = __float_as_uint(a.w);
-          if (!have || t < bt || (t == bt && occ < hit->occ)) {
-            have = true;
-            bt = t;
-            hit->t = t;
-            hit->u = u;
-            hit->v = vv;
-            hit->occ = occ;
-            hit->pos = i;
-          }
-        }
-      } else if (next < 0) {
-        next = c;
-        next_t = tn;
-      } else {
-        uint32_t push = (uint32_t)c;
-        if (tn < next_t) {
-          push = (uint32_t)next;
-          next = c;
-          next_t = tn;
-        }
-        if (s16) stack16[sp * stride] = (uint16_t)push;
-        else stack[sp * stride] = push;
-        sp++;
+    const uint32_t mask = node_hits(sc, node, sr, limit, tn, &ch);
+    // the triangles of every hit leaf child, one test per loop trip
+    uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
+    for (;;) {
+      if (i == end) {
+        if (!leaves) break;
+        const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
+        leaves &= leaves - 1;
+        i = v >> 2;
+        end = i + (v & 3u) + 1u;
+      }
+      const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+      const uint32_t pos = i++;
+      float t, u, vv;
+      tl.tris++;
+      if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+      if (!finite_(t)) continue;
+      if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
+      if (ANY) return true;
+      const uint32_t occ = __float_as_uint(a.w);
+      if (!have || t < bt || (t == bt && occ < hit->occ)) {
+        have = true;
+        bt = t;
+        hit->t = t;
+        hit->u = u;
+        hit->v = vv;
+        hit->occ = occ;
+        hit->pos = pos;
       }
     }
-    if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+    const int32_t next = push_children(sc, mask, ch, tn, stack, stride, sp);
     if (next >= 0) {
       node = next;
       continue;
     }
     if (sp == 0) break;
     sp--;
-    node = s16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
+    node = pop_node(sc, stack, stride, sp);
   }
   return have;
 }
@@ -353,7 +412,9 @@ __global__ void k_burial(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, ui
       float sin_theta = sqrtf(ring / 8.0f);
       float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
       float phi = (((float)k + turn) * kTau) / 8.0f;
-      f3 dir = add(mul(add(mul(tangent, lcosf(trig, phi)), mul(bitangent, lsinf(trig, phi))), sin_theta), mul(normal, cos_theta));
+      float sp, cp;
+      lsincosf(trig, phi, &sp, &cp);
+      f3 dir = add(mul(add(mul(tangent, cp), mul(bitangent, sp)), sin_theta), mul(normal, cos_theta));
       Hit h;
       bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
       if (bf) back++;
@@ -391,7 +452,9 @@ __device__ __forceinline__ f2 concentric_disk(const TrigTables& trig, f2 sq) {
     radius = py;
     angle = 2.0f * kFracPi4 - kFracPi4 * (px / py);
   }
-  return mk2(lcosf(trig, angle) * radius, lsinf(trig, angle) * radius);
+  float sa, ca;
+  lsincosf(trig, angle, &sa, &ca);
+  return mk2(ca * radius, sa * radius);
 }
 
 __device__ __forceinline__ f3 cosine_dir(const TrigTables& trig, f3 normal, f2 sq) {
@@ -1114,7 +1177,8 @@ __device__ f3 slerp3(const TrigTables& trig, f3 a, f3 b, float s) {
     f3 o = fabsf(a.x) > fabsf(a.y) ? mk3(-a.z, 0.0f, a.x) : mk3(0.0f, a.z, -a.y);
     f3 axis = normalize(o);
     float half = kPi * s * 0.5f;
-    float sn = lsinf(trig, half), cs = lcosf(trig, half);
+    float sn, cs;
+    lsincosf(trig, half, &sn, &cs);
     float q[4] = {axis.x * sn, axis.y * sn, axis.z * sn, cs};
     float rl = la + (lb - la) * s;
     return mul(quat_rotate(q, a), rl / la);
@@ -1126,7 +1190,9 @@ __device__ __forceinline__ f3 on_unit_sphere(const TrigTables& trig, Pcg& rng) {
   float z = pcg_f32(rng) * 2.0f - 1.0f;
   float angle = pcg_f32(rng) * kTau;
   float r = sqrtf(maxf_(1.0f - z * z, 0.0f));
-  return mk3(r * lcosf(trig, angle), r * lsinf(trig, angle), z);
+  float sa, ca;
+  lsincosf(trig, angle, &sa, &ca);
+  return mk3(r * ca, r * sa, z);
 }
 
 struct BounceUnit {
@@ -1295,71 +1361,38 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
 // One BVH node of an any-hit traversal (the loop body of traverse<true>).  Returns 0 while the
 // traversal continues (node/sp updated), 1 on a hit, 2 when the ray is unoccluded.
 __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
-                                              float ix, float iy, float iz, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
+                                              const SlabRay& sr, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
                                               uint32_t alpha) {
-  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
-  const bool s16 = sc.stack16 != 0;
   tl.nodes++;
-  const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
-  const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
-  const int4 ch = *reinterpret_cast<const int4*>(nb + 6);
-  const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
-  const float limit = tmax * (1.0f + kBoxEps);
-  int32_t next = -1;
-  float next_t = INFINITY;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (k >= cnt) break;
-    const float lx = k == 0 ? lox.x : k == 1 ? lox.y : k == 2 ? lox.z : lox.w;
-    const float hx = k == 0 ? hix.x : k == 1 ? hix.y : k == 2 ? hix.z : hix.w;
-    const float ly = k == 0 ? loy.x : k == 1 ? loy.y : k == 2 ? loy.z : loy.w;
-    const float hy = k == 0 ? hiy.x : k == 1 ? hiy.y : k == 2 ? hiy.z : hiy.w;
-    const float lz = k == 0 ? loz.x : k == 1 ? loz.y : k == 2 ? loz.z : loz.w;
-    const float hz = k == 0 ? hiz.x : k == 1 ? hiz.y : k == 2 ? hiz.z : hiz.w;
-    const int32_t c = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
-    const float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
-    const float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
-    const float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
-    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    tn = tn - fabsf(tn) * kBoxEps;
-    tf = tf + fabsf(tf) * kBoxEps;
-    if (!(tn <= tf && tf >= 0.0f && tn <= limit)) continue;
-    if (c < 0) {
-      const uint32_t v = (uint32_t)(-1 - c);
-      const uint32_t first = v >> 2, count = (v & 3u) + 1u;
-      for (uint32_t i = first; i < first + count; ++i) {
-        const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
-        float t, u, vv;
-        tl.tris++;
-        if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
-        if (!finite_(t)) continue;
-        if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
-        return 1u;
-      }
-    } else if (next < 0) {
-      next = c;
-      next_t = tn;
-    } else {
-      uint32_t push = (uint32_t)c;
-      if (tn < next_t) {
-        push = (uint32_t)next;
-        next = c;
-        next_t = tn;
-      }
-      if (s16) stack16[sp * stride] = (uint16_t)push;
-      else stack[sp * stride] = push;
-      sp++;
+  float tn[4];
+  int4 ch;
+  const uint32_t mask = node_hits(sc, node, sr, tmax * (1.0f + kBoxEps), tn, &ch);
+  uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
+  for (;;) {
+    if (i == end) {
+      if (!leaves) break;
+      const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
+      leaves &= leaves - 1;
+      i = v >> 2;
+      end = i + (v & 3u) + 1u;
     }
+    const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+    i++;
+    float t, u, vv;
+    tl.tris++;
+    if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+    if (!finite_(t)) continue;
+    if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
+    return 1u;
   }
-  if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+  const int32_t next = push_children(sc, mask, ch, tn, stack, stride, sp);
   if (next >= 0) {
     node = next;
     return 0u;
   }
   if (sp == 0) return 2u;
   sp--;
-  node = s16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
+  node = pop_node(sc, stack, stride, sp);
   return 0u;
 }
 
@@ -1420,45 +1453,62 @@ __device__ __forceinline__ bool gather_sample(const GatherJob& g, const TrigTabl
 
 // Job lists: pass 0 counts, pass 1 writes.  counts[0..1]: traced / count-only jobs (pass 0),
 // cursors[0..1] (pass 1).
+// Pass 0 writes each bounce entry's traced and count-only job counts (cnt[q], cnt[n + 1 + q]);
+// pass 1 writes the jobs at the scanned offsets (off[q], off[n + 1 + q]), in entry order and
+// texel order within an entry.
 __global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, DevTexels tx, const uint64_t* __restrict__ tex_start,
-                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts,
+                              DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* __restrict__ off,
                               uint2* trace_jobs, uint2* count_jobs) {
+  // one wave per bounce entry, 64 of its texels per trip
   const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
-  for (uint64_t unit = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); unit < units; unit += (uint64_t)gridDim.x * waves) {
-    BounceUnit u = bounce_unit(b, ent, tex_start, unit, lane);
-    bool traced = false, counted = false;
-    uint64_t j = 0;
-    if (u.i < u.nt) {
-      const DevLight& L = lights[ent.light[u.e]];
-      j = u.t0 + u.i;
-      if (L.shape != 3) {
-        float4 p4 = tx.pos[j];
-        f3 to_texel = sub(mk3(p4.x, p4.y, p4.z), ld3(L.pos));
-        if (len_sq(to_texel) <= L.radius * L.radius) {
-          uint32_t w = b.ent_window[u.e];
-          const uint32_t* wr = b.win_rect + 4 * (size_t)w;
-          uint32_t xy = tx.xy[j];
-          uint64_t k = (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
-          const unsigned int* owner = reinterpret_cast<const unsigned int*>(b.map_b) + b.win_off[w] * 3;
-          traced = owner[k] == u.t + 1;
-          counted = !traced;
+  const uint64_t n = b.n;
+  for (uint64_t q = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); q < n; q += (uint64_t)gridDim.x * waves) {
+    const uint64_t e = b.entry[q];
+    const uint32_t t = ent.tri[e];
+    const DevLight& L = lights[ent.light[e]];
+    uint64_t nt_traced = 0, nt_counted = 0;
+    uint64_t base_t = pass == 1 ? off[q] : 0, base_c = pass == 1 ? off[n + 1 + q] : 0;
+    if (L.shape != 3) {  // suns are gathered per unit
+      const uint64_t t0 = tex_start[t], nt = tex_start[t + 1] - t0;
+      const f3 lp = ld3(L.pos);
+      const float r2 = L.radius * L.radius;
+      const uint32_t w = b.ent_window[e];
+      const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+      const uint32_t wx0 = wr[0], wy0 = wr[1], ww = wr[2] - wr[0] + 1;
+      const unsigned int* owner = reinterpret_cast<const unsigned int*>(b.map_b) + b.win_off[w] * 3;
+      for (uint64_t c = 0; c < nt; c += 64) {
+        const uint64_t j = t0 + c + lane;
+        bool traced = false, counted = false;
+        if (c + lane < nt) {
+          const float4 p4 = tx.pos[j];
+          if (len_sq(sub(mk3(p4.x, p4.y, p4.z), lp)) <= r2) {
+            const uint32_t xy = tx.xy[j];
+            const uint64_t k = (uint64_t)((xy & 0xffffu) - wx0) + (uint64_t)((xy >> 16) - wy0) * ww;
+            traced = owner[k] == t + 1;
+            counted = !traced;
+          }
         }
+        const uint64_t mt = __ballot(traced), mc = __ballot(counted);
+        if (pass == 1) {
+          const uint32_t rt = __builtin_amdgcn_mbcnt_hi((uint32_t)(mt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mt, 0u));
+          const uint32_t rc = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
+          if (traced) trace_jobs[base_t + rt] = make_uint2((uint32_t)j, (uint32_t)e);
+          if (counted) count_jobs[base_c + rc] = make_uint2((uint32_t)j, (uint32_t)e);
+          base_t += (uint64_t)__popcll(mt);
+          base_c += (uint64_t)__popcll(mc);
+        }
+        nt_traced += (uint64_t)__popcll(mt);
+        nt_counted += (uint64_t)__popcll(mc);
       }
     }
-    const uint64_t mt = __ballot(traced), mc = __ballot(counted);
-    const uint32_t rt = __builtin_amdgcn_mbcnt_hi((uint32_t)(mt >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mt, 0u));
-    const uint32_t rc = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
-    unsigned long long bt = 0, bc = 0;
-    if (lane == 0) {
-      if (mt) bt = atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
-      if (mc) bc = atomicAdd(&counts[1], (unsigned long long)__popcll(mc));
+    if (pass == 0 && lane == 0) {
+      cnt[q] = nt_traced;
+      cnt[n + 1 + q] = nt_counted;
     }
-    if (pass == 1) {
-      bt = ((unsigned long long)__shfl((uint32_t)(bt >> 32), 0, 64) << 32) | __shfl((uint32_t)bt, 0, 64);
-      bc = ((unsigned long long)__shfl((uint32_t)(bc >> 32), 0, 64) << 32) | __shfl((uint32_t)bc, 0, 64);
-      if (traced) trace_jobs[bt + rt] = make_uint2((uint32_t)j, (uint32_t)u.e);
-      if (counted) count_jobs[bc + rc] = make_uint2((uint32_t)j, (uint32_t)u.e);
-    }
+  }
+  if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[n] = 0;
+    cnt[2 * n + 1] = 0;
   }
 }
 
@@ -1481,120 +1531,167 @@ __global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restr
   }
 }
 
-// Persistent streamed gather over the traced jobs; `queue` (zeroed) hands out chunks of jobs.
-__global__ void k_gather_stream(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
-                                DevEntries ent, DevBounce b, const uint2* __restrict__ jobs, uint64_t njobs,
-                                unsigned long long* queue, BounceParams p, TrigTables trig, uint32_t chunk, uint32_t refill) {
+// Ray offsets of a chunk of jobs: samples per job (n + 1 entries, the last one 0, for a scan).
+__global__ void k_gather_nrays(DevEntries ent, const DevLight* __restrict__ lights, const uint2* __restrict__ jobs, uint64_t n,
+                               uint64_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      out[i] = 0;
+      continue;
+    }
+    const uint32_t bs = lights[ent.light[jobs[i].y]].bounce_samples;
+    out[i] = bs > 2 ? bs : 2u;
+  }
+}
+
+// Phase 1 of the point-light gather: every sample of every job in order, written as a ray
+// (origin | length, direction | flags: bit 0 alpha test, bit 1 casts a ray) and the light it
+// brings if unoccluded (zero when it casts none).  Also counts each mesh's bounce rays.
+__global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
+                              BounceParams p, TrigTables trig, float4* rays, float* contrib) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    const uint64_t i = i0 + lane;
+    uint32_t m = 0xffffffffu, nr = 0;
+    if (i < n) {
+      const uint2 jb = jobs[i];
+      Pcg rng;
+      GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
+      const uint32_t alpha = lights[ent.light[jb.y]].alpha ? 1u : 0u;
+      const uint64_t base = rs[i];
+      for (uint32_t k = 0; k < g.samples; ++k) {
+        f3 d = mk3(0.0f, 0.0f, 0.0f), c = mk3(0.0f, 0.0f, 0.0f);
+        float dist = 0.0f;
+        const bool ray = gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c);
+        nr += ray ? 1u : 0u;
+        const uint64_t r = base + k;
+        rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist);
+        rays[2 * r + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(alpha | (ray ? 2u : 0u)));
+        contrib[3 * r] = c.x;
+        contrib[3 * r + 1] = c.y;
+        contrib[3 * r + 2] = c.z;
+      }
+    }
+    const uint32_t m0 = __shfl(m, 0, 64);
+    if (__ballot(i < n && m != m0) == 0ull) {
+      wave_add(&b.rays[m0], nr);
+    } else if (i < n && nr) {
+      atomicAdd(&b.rays[m], (unsigned long long)nr);
+    }
+  }
+}
+
+// Phase 2: persistent any-hit traversal of the chunk's rays.  Each lane keeps one ray in flight
+// and the next one prefetched in registers; a lane whose ray ends swaps in the prefetched ray at
+// once and requests another, so lanes never wait for the wave's longest ray.  `queue` (zeroed)
+// hands out runs of `chunk` rays per wave.  An occluded ray's contribution is set to zero.
+template <int MIN_WAVES>
+__global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(DevScene sc, const float4* __restrict__ rays, const uint64_t* __restrict__ total, float* contrib,
+                             unsigned int* queue, uint32_t chunk, uint32_t refill) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
   Tally tl;
   const uint32_t lane = threadIdx.x & 63u;
-  uint64_t cbase = 0, cend = 0;
+  const uint32_t nrays = (uint32_t)*total;  // a chunk holds fewer than 2^32 rays
+  uint32_t cbase = 0, cend = 0;
   bool drained = false;
-  // job state
-  bool job = false;
-  GatherJob g;
-  Pcg rng;
-  uint32_t k = 0, mesh = 0, jrays = 0, jalpha = 0;
-  uint64_t dst = 0;
-  float intensity = 0.0f;
-  f3 sum = mk3(0.0f, 0.0f, 0.0f);
-  uint32_t cur_mesh = 0xffffffffu;
-  uint32_t mesh_rays = 0;
-  // ray state
-  bool live = false;
-  f3 d = mk3(0.0f, 0.0f, 1.0f), contrib = mk3(0.0f, 0.0f, 0.0f);
-  float ix = 0.0f, iy = 0.0f, iz = 0.0f, tmax = 0.0f;
+  bool has_next = false, live = false;
+  uint32_t nidx = 0, cidx = 0;
+  float4 n0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), n1 = n0;
+  f3 o = mk3(0.0f, 0.0f, 0.0f), d = o;
+  SlabRay sr{};
+  float tmax = 0.0f;
+  uint32_t alpha = 0, sp = 0;
   int32_t node = 0;
-  uint32_t sp = 0;
   for (;;) {
-    if ((uint32_t)__popcll(__ballot(live)) <= refill) {
-      // hand out jobs to lanes without one
-      const uint64_t need = __ballot(!job);
-      if (need && !drained) {
-        if (cbase >= cend) {
-          unsigned long long base = 0;
-          if (lane == 0) base = atomicAdd(queue, (unsigned long long)chunk);
-          base = ((unsigned long long)__shfl((uint32_t)(base >> 32), 0, 64) << 32) | __shfl((uint32_t)base, 0, 64);
-          cbase = base;
-          cend = base + chunk < njobs ? base + chunk : njobs;
-          if (cbase >= njobs) drained = true;
-        }
-        if (!drained) {
-          const uint64_t avail = cend - cbase;
-          const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-          if (!job && rank < avail) {
-            const uint2 jb = jobs[cbase + rank];
-            g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &mesh);
-            const uint32_t w = b.ent_window[jb.y];
-            const uint32_t* wr = b.win_rect + 4 * (size_t)w;
-            const uint32_t xy = tx.xy[jb.x];
-            dst = b.win_off[w] + (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
-            intensity = lights[ent.light[jb.y]].bounce_intensity;
-            jalpha = lights[ent.light[jb.y]].alpha;
-            sum = mk3(0.0f, 0.0f, 0.0f);
-            k = 0;
-            jrays = 0;
-            job = true;
-          }
-          const uint64_t took = avail < (uint64_t)__popcll(need) ? avail : (uint64_t)__popcll(need);
-          cbase += took;
-        }
-      }
-      // next sample (or the end of the job) for lanes with a job and no ray
-      if (job && !live) {
-        while (k < g.samples) {
-          float dist;
-          const bool ray = gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &contrib);
-          k++;
-          if (ray) {
-            jrays++;
-            if (sc.ntris == 0) {
-              sum = add(sum, contrib);
-              continue;
-            }
-            live = true;
-            tmax = dist;
-            ix = inv_dir(d.x);
-            iy = inv_dir(d.y);
-            iz = inv_dir(d.z);
-            node = 0;
-            sp = 0;
-            break;
-          }
-        }
-        if (!live) {
-          f3 v = divs(mul(sum, intensity), (float)g.samples);
-          float* out = b.map_a + dst * 3;
-          out[0] = v.x;
-          out[1] = v.y;
-          out[2] = v.z;
-          if (mesh != cur_mesh) {
-            if (cur_mesh != 0xffffffffu && mesh_rays) atomicAdd(&b.rays[cur_mesh], (unsigned long long)mesh_rays);
-            cur_mesh = mesh;
-            mesh_rays = 0;
-          }
-          mesh_rays += jrays;
-          job = false;
-        }
-      }
-      if (!__ballot(live)) {
-        if (drained && !__ballot(job)) break;
-        continue;
+    // Finished lanes take their prefetched ray, and lanes without one request another, only
+    // once at least `refill` lanes are idle: these steps then run with many lanes at a time.
+    const uint32_t idle = (uint32_t)__popcll(__ballot(!live));
+    if (idle >= refill || idle == 64u) {
+    if (!live && has_next) {
+      has_next = false;
+      cidx = nidx;
+      const uint32_t flags = __float_as_uint(n1.w);
+      if ((flags & 2u) && sc.ntris != 0) {
+        o = mk3(n0.x, n0.y, n0.z);
+        tmax = n0.w;
+        d = mk3(n1.x, n1.y, n1.z);
+        alpha = flags & 1u;
+        sr = slab_ray(o, d);
+        node = 0;
+        sp = 0;
+        live = true;
       }
     }
+    const uint64_t need = __ballot(!has_next);
+    if (need && !drained) {
+      if (cbase >= cend) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(queue, chunk);
+        base = __builtin_amdgcn_readfirstlane(base);
+        cbase = base;
+        if (base >= nrays) drained = true;
+        else cend = nrays - base > chunk ? base + chunk : nrays;
+      }
+      if (!drained) {
+        const uint32_t avail = cend - cbase;
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        if (!has_next && rank < avail) {
+          nidx = cbase + rank;
+          n0 = rays[2 * (size_t)nidx];
+          n1 = rays[2 * (size_t)nidx + 1];
+          has_next = true;
+        }
+        const uint32_t want = (uint32_t)__popcll(need);
+        cbase += avail < want ? avail : want;
+      }
+    }
+    if (!__ballot(live || has_next)) {
+      if (drained) break;
+      continue;
+    }
+    }
     if (live) {
-      const uint32_t r = visit_any(sc, stack, blockDim.x, g.origin, d, ix, iy, iz, tmax, node, sp, tl, jalpha);
+      const uint32_t r = visit_any(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha);
       if (r != 0u) {
         live = false;
-        if (r == 2u) sum = add(sum, contrib);
+        if (r == 1u) {
+          contrib[3 * (size_t)cidx] = 0.0f;
+          contrib[3 * (size_t)cidx + 1] = 0.0f;
+          contrib[3 * (size_t)cidx + 2] = 0.0f;
+        }
       }
     }
   }
-  if (cur_mesh != 0xffffffffu && mesh_rays) atomicAdd(&b.rays[cur_mesh], (unsigned long long)mesh_rays);
   if (sc.counters) {
     wave_add(&sc.counters[0], tl.nodes);
     wave_add(&sc.counters[1], tl.tris);
+  }
+}
+
+// Phase 3: each job's samples summed in sample order (bounce.rs:287-290) into its window texel.
+// Samples without a ray and occluded samples hold +0, which leaves the sum unchanged.
+__global__ void k_gather_sum(DevEntries ent, const DevLight* __restrict__ lights, DevTexels tx, DevBounce b,
+                             const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
+                             const float* __restrict__ contrib) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 jb = jobs[i];
+    const DevLight& L = lights[ent.light[jb.y]];
+    const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
+    const float* c = contrib + 3 * rs[i];
+    f3 sum = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < samples; ++k) sum = add(sum, mk3(c[3 * k], c[3 * k + 1], c[3 * k + 2]));
+    const f3 v = divs(mul(sum, L.bounce_intensity), (float)samples);
+    const uint32_t w = b.ent_window[jb.y];
+    const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+    const uint32_t xy = tx.xy[jb.x];
+    const uint64_t dst = b.win_off[w] + (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
+    float* out = b.map_a + dst * 3;
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
   }
 }
 
@@ -1820,11 +1917,10 @@ hipError_t launch_sky_probes(hipStream_t s, DevScene scene, TrigTables trig, con
   return hipGetLastError();
 }
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
-                              DevEntries ent, DevBounce b, uint64_t units, int pass, unsigned long long* counts, uint2* trace_jobs,
+                              DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,
                               uint2* count_jobs) {
-  if (units == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_jobs, dim3(grid_for(units, kThreads / 64)), dim3(kThreads), 0, s, tris, lights, tx, tex_start, ent, b,
-                     units, pass, counts, trace_jobs, count_jobs);
+  hipLaunchKernelGGL(k_gather_jobs, dim3(grid_for(std::max<uint64_t>(b.n, 1), kThreads / 64)), dim3(kThreads), 0, s, tris, lights,
+                     tx, tex_start, ent, b, pass, cnt, off, trace_jobs, count_jobs);
   return hipGetLastError();
 }
 hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
@@ -1834,22 +1930,43 @@ hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, co
                      njobs, p, trig);
   return hipGetLastError();
 }
-hipError_t launch_gather_stream(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx,
-                                DevEntries ent, DevBounce b, const uint2* jobs, uint64_t njobs, unsigned long long* queue,
-                                BounceParams p, TrigTables trig, const GatherLaunch& gl) {
-  if (njobs == 0) return hipSuccess;
+hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out) {
+  hipLaunchKernelGGL(k_gather_nrays, dim3(grid_for(n + 1, kThreads)), dim3(kThreads), 0, s, ent, lights, jobs, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
+                              float4* rays, float* contrib) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rays, dim3(grid_for(n, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, b, jobs, n, rs, p, trig,
+                     rays, contrib);
+  return hipGetLastError();
+}
+hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                             float* contrib, unsigned int* queue, const GatherLaunch& gl) {
+  if (max_rays >= (1ull << 31) || gl.chunk == 0 || gl.chunk > (1u << 20)) return hipErrorInvalidValue;
+  if (max_rays == 0) return hipSuccess;
   const size_t lds = stack_bytes(scene, gl.block);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_stream, (int)gl.block, lds);
+  auto kernel = gl.min_waves >= 8 ? k_any_stream<8> : k_any_stream<1>;
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)gl.block, lds);
   if (e != hipSuccess) return e;
   if (gl.blocks_per_cu) per_cu = (int)gl.blocks_per_cu;
   uint64_t grid = (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, per_cu);
-  const uint64_t useful = (njobs + gl.block - 1) / gl.block;
+  const uint64_t useful = (max_rays + gl.block - 1) / gl.block;
   if (grid > useful) grid = useful;
-  hipLaunchKernelGGL(k_gather_stream, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, tris, meshes, lights, tx, ent, b, jobs,
-                     njobs, queue, p, trig, gl.chunk, gl.refill);
+  // the 32-bit queue may pass the ray count by one run per wave before every wave has drained
+  if (max_rays + (uint64_t)gl.chunk * grid * (gl.block / 64) >= (1ull << 32)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, rays, total, contrib, queue, gl.chunk,
+                     std::max(1u, std::min(64u, gl.refill)));
+  return hipGetLastError();
+}
+hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
+                             uint64_t n, const uint64_t* rs, const float* contrib) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, ent, lights, tx, b, jobs, n, rs, contrib);
   return hipGetLastError();
 }
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,

@@ -1,12 +1,14 @@
 // f32 sin/cos with the exact results of the host's libm.
 //
 // The reference computes f32::sin / f32::cos, which Rust std sends to glibc sinf/cosf; glibc's
-// results are not correctly rounded (about 1-2% of inputs in [-0.8, 8] differ from the nearest
+// results are not correctly rounded (about 1% of inputs in [-0.8, 8] differ from the nearest
 // float).  Kernels therefore evaluate sin/cos in double precision (argument reduction by a
-// three-part pi/2 split and the fdlibm polynomial kernels, fixed operation order, no FMA) and
+// two-part pi/2 split and the fdlibm polynomial kernels, fixed operation order, no FMA) and
 // round once to f32; the host compares that value with its own sinf/cosf for every float in the
-// domain the bake uses and stores the inputs where they differ.  A kernel looks its input up in
-// that sorted exception list (a bucket index on the top 16 bits plus a binary search).
+// domain the bake uses and stores the inputs where they differ.  Every such input lies close to
+// a rounding midpoint of the double value (glibc is off by less than 0.07 ulp), so a kernel only
+// searches the sorted exception list (a bucket index on the top 16 bits plus a binary search)
+// when the double value is within the host-measured band around a midpoint.
 #pragma once
 
 #include "dl_math.h"
@@ -36,21 +38,28 @@ DLI double poly_cos(double x) {
   double r = z * (kC1 + z * (kC2 + z * (kC3 + z * (kC4 + z * (kC5 + z * kC6)))));
   return 1.0 - (0.5 * z - z * r);
 }
-// which = 0: sin, 1: cos.  Rounded once to f32.
-DLI float trig_core(float xf, int which) {
-  double x = (double)xf;
-  double fn = floor(x * kInvPio2 + 0.5);
-  double r = (x - fn * kPio2_1) - fn * kPio2_1t;
-  int q = ((int)fn + which) & 3;
-  double v;
-  switch (q) {
-    case 0: v = poly_sin(r); break;
-    case 1: v = poly_cos(r); break;
-    case 2: v = -poly_sin(r); break;
-    default: v = -poly_cos(r); break;
-  }
-  return (float)v;
+// Argument reduction: x = fn * pi/2 + r with |r| <= pi/4 (for the domain's |x| <= 8).
+DLI double trig_reduce(float xf, int* quadrant) {
+  const double x = (double)xf;
+  const double fn = floor(x * kInvPio2 + 0.5);
+  *quadrant = (int)fn;
+  return (x - fn * kPio2_1) - fn * kPio2_1t;
 }
+// sin (which = 0) or cos (which = 1) of the reduced argument, before rounding: both kernels are
+// evaluated and the quadrant selects, so lanes do not diverge.
+DLI double trig_select(double r, int quadrant, int which) {
+  const double ps = poly_sin(r), pc = poly_cos(r);
+  const int q = (quadrant + which) & 3;
+  const double v = (q & 1) ? pc : ps;
+  return (q & 2) ? -v : v;
+}
+DLI double trig_value(float xf, int which) {
+  int q;
+  const double r = trig_reduce(xf, &q);
+  return trig_select(r, q, which);
+}
+// which = 0: sin, 1: cos.  Rounded once to f32.
+DLI float trig_core(float xf, int which) { return (float)trig_value(xf, which); }
 
 // Device view of one function's exception list.
 struct TrigTable {
@@ -60,20 +69,18 @@ struct TrigTable {
 };
 struct TrigTables {
   TrigTable sin, cos;
-  // One bit per float of the table domain, set when sin or cos of it is an exception; inputs
-  // whose bit is clear skip the search.  Domain: bits [kDomPosLo, kDomPosHi] then
+  // Inputs whose double value lies farther than band * (midpoint gap) from both rounding
+  // midpoints of its f32 rounding are never exceptions (checked on the host for every float of
+  // the domain); only the others are searched.  Domain: bits [kDomPosLo, kDomPosHi] then
   // [kDomNegLo, kDomNegHi].
-  const uint32_t* flags;
+  double band;
 };
 static constexpr uint32_t kDomPosLo = 0x37800000u, kDomPosHi = 0x41000000u;  // [2^-16, 8]
 static constexpr uint32_t kDomNegLo = 0xB7800000u, kDomNegHi = 0xBF4CCCCDu;  // [-0.8, -2^-16]
 static constexpr uint32_t kDomPosCount = kDomPosHi - kDomPosLo + 1;
 static constexpr uint32_t kDomCount = kDomPosCount + (kDomNegHi - kDomNegLo + 1);
-// Index of a float in the domain, or ~0u outside it.
-DLI uint32_t trig_domain_index(uint32_t bits) {
-  if (bits >= kDomPosLo && bits <= kDomPosHi) return bits - kDomPosLo;
-  if (bits >= kDomNegLo && bits <= kDomNegHi) return kDomPosCount + (bits - kDomNegLo);
-  return ~0u;
+DLI bool trig_in_domain(uint32_t bits) {
+  return (bits >= kDomPosLo && bits <= kDomPosHi) || (bits >= kDomNegLo && bits <= kDomNegHi);
 }
 
 DLI uint32_t f2u(float f) {
@@ -100,18 +107,30 @@ DLI float trig_lookup(const TrigTable& t, float x, float computed) {
   }
   return computed;
 }
-// Exceptions exist only inside the domain (outside it the tables are empty).
-DLI bool trig_flagged(const TrigTables& t, float x) {
-  uint32_t i = trig_domain_index(f2u(x));
-  return i != ~0u && ((t.flags[i >> 5] >> (i & 31u)) & 1u);
+// Whether the double value v (rounded to f) lies within band * gap of a rounding midpoint,
+// where gap is the distance between the midpoints below and above f.
+DLI bool trig_near_midpoint(double v, float f, double band) {
+  const uint32_t b = f2u(f);
+  if ((b & 0x7fffffffu) == 0u || (b & 0x7f800000u) == 0x7f800000u) return true;
+  const float away = u2f(b + 1u), toward = u2f(b - 1u);  // larger and smaller magnitude
+  const double m1 = ((double)f + (double)away) * 0.5, m0 = ((double)f + (double)toward) * 0.5;
+  const double lo = m0 < m1 ? m0 : m1, hi = m0 < m1 ? m1 : m0;
+  const double a = v - lo, c = hi - v;
+  return (a < c ? a : c) <= band * (hi - lo);
 }
-DLI float lsinf(const TrigTables& t, float x) {
-  float c = trig_core(x, 0);
-  return trig_flagged(t, x) ? trig_lookup(t.sin, x, c) : c;
+DLI float trig_finish(const TrigTables& t, const TrigTable& tab, float x, double v) {
+  const float f = (float)v;
+  if (!trig_in_domain(f2u(x)) || !trig_near_midpoint(v, f, t.band)) return f;
+  return trig_lookup(tab, x, f);
 }
-DLI float lcosf(const TrigTables& t, float x) {
-  float c = trig_core(x, 1);
-  return trig_flagged(t, x) ? trig_lookup(t.cos, x, c) : c;
+DLI float lsinf(const TrigTables& t, float x) { return trig_finish(t, t.sin, x, trig_value(x, 0)); }
+DLI float lcosf(const TrigTables& t, float x) { return trig_finish(t, t.cos, x, trig_value(x, 1)); }
+// sin and cos of one argument with one reduction.
+DLI void lsincosf(const TrigTables& t, float x, float* s, float* c) {
+  int q;
+  const double r = trig_reduce(x, &q);
+  *s = trig_finish(t, t.sin, x, trig_select(r, q, 0));
+  *c = trig_finish(t, t.cos, x, trig_select(r, q, 1));
 }
 
 }  // namespace dl

@@ -62,6 +62,10 @@ def load() -> C.CDLL:
     lib.dl_test_trig.argtypes = [P, f32p, C.c_uint64, f32p, f32p]
     lib.dl_test_trig_host.restype = C.c_int
     lib.dl_test_trig_host.argtypes = [f32p, C.c_uint64, f32p, f32p]
+    lib.dl_test_trig_exhaustive.restype = C.c_int
+    lib.dl_test_trig_exhaustive.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+    lib.dl_test_trig_keys.restype = C.c_uint64
+    lib.dl_test_trig_keys.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.c_uint64]
     lib.dl_test_trig_table_sizes.restype = None
     lib.dl_test_trig_table_sizes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.dl_set_counting.restype = C.c_int
@@ -69,7 +73,7 @@ def load() -> C.CDLL:
     lib.dl_set_launch.restype = C.c_int
     lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
     lib.dl_set_gather.restype = C.c_int
-    lib.dl_set_gather.argtypes = [P] + [C.c_uint32] * 5
+    lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
     _lib = lib
     return lib
 
@@ -124,9 +128,11 @@ class Context:
         if st != 0:
             raise BakeError(st, self.error())
 
-    def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, refill: int = 0, blocks_per_cu: int = 0) -> None:
-        """Point-light bounce gather: 0 = one wave per (triangle, light, 64 texels), 1 = streamed jobs."""
-        st = self.lib.dl_set_gather(self.handle, mode, block, chunk, refill, blocks_per_cu)
+    def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, chunk_rays: int = 0, blocks_per_cu: int = 0,
+                   min_waves: int = 0, refill: int = 0) -> None:
+        """Point-light bounce gather: 0 = one wave per (triangle, light, 64 texels), 1 = split
+        gather (rays, persistent any-hit traversal, ordered sums)."""
+        st = self.lib.dl_set_gather(self.handle, mode, block, chunk, chunk_rays, blocks_per_cu, min_waves, refill)
         if st != 0:
             raise BakeError(st, self.error())
 

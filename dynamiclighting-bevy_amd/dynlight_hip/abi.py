@@ -146,7 +146,8 @@ class dl_timings(C.Structure):
                 ("occluder_triangles", C.c_uint64), ("trace_node_visits", C.c_uint64),
                 ("trace_tri_tests", C.c_uint64), ("bounce_kernel_ms", C.c_double),
                 ("bounce_node_visits", C.c_uint64), ("bounce_tri_tests", C.c_uint64),
-                ("bounce_launches", C.c_uint64)]
+                ("bounce_launches", C.c_uint64), ("gather_trace_ms", C.c_double),
+                ("gather_trace_launches", C.c_uint64)]
 
 
 def _np_dtype(struct) -> np.dtype:

@@ -8,6 +8,9 @@ fill, `point_light`, `area_light`, `sun_light`, `sky_light`).  The procedural sc
 * ``interior``      -- 8x8 grid of 6x3x6 m rooms with 1x2 m door openings, seeded boxes and
                         pillars up to ~200k triangles, albedo U(0.1,0.9)^3, 256 point/spot
                         lights of range 4-12 m with one colored bounce, 32 texels/m.
+* ``outdoor``       -- 1024x512 fBm heightfield (~1M triangles) plus 2000 box buildings, a
+                        soft sun with one bounce, 64 rectangle/tube area lights, a baked sky,
+                        4 texels/m (terrain capped at 2048^2).
 
 Randomness is PCG32 seeded through SplitMix64, restated here so scenes are identical on every
 machine.
@@ -288,6 +291,97 @@ def interior(seed: int = 2, target_triangles: int = 200_000, lights: int = 256,
     opts = default_options()
     opts.pixel_density = density
     opts.colored_bounce = 1
+    opts.dynamic_geometry = DYNAMIC_ANGULAR
+    return inp, opts
+
+
+def _fbm(rng: Pcg32, nx: int, nz: int, octaves: int) -> np.ndarray:
+    """Value-noise fBm on an nx x nz vertex grid, scaled to [0, 1]: octave o has a lattice of
+    2^(o+2) cells across x, smoothstep-interpolated, amplitude 1/2^o."""
+    xs = np.arange(nx, dtype=np.float64) / max(1, nx - 1)
+    zs = np.arange(nz, dtype=np.float64) / max(1, nx - 1)
+    out = np.zeros((nz, nx))
+    for o in range(octaves):
+        cells = 2 ** (o + 2)
+        cz = int(math.ceil(zs[-1] * cells)) + 1 if nz > 1 else 1
+        lattice = np.array([rng.uniform() for _ in range((cells + 2) * (cz + 2))]).reshape(cz + 2, cells + 2)
+        fx, fz = xs * cells, zs * cells
+        ix, iz = np.floor(fx).astype(int), np.floor(fz).astype(int)
+        tx, tz = fx - ix, fz - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        a = lattice[iz][:, ix] * (1 - sx) + lattice[iz][:, ix + 1] * sx
+        b = lattice[iz + 1][:, ix] * (1 - sx) + lattice[iz + 1][:, ix + 1] * sx
+        out += (a * (1 - sz)[:, None] + b * sz[:, None]) / 2 ** o
+    lo, hi = out.min(), out.max()
+    return (out - lo) / (hi - lo) if hi > lo else out * 0.0
+
+
+def outdoor(seed: int = 3, nx: int = 1024, nz: int = 512, buildings: int = 2000, area_lights: int = 64,
+            density: float = 4.0, octaves: int = 6, height: float = 20.0):
+    """Returns (BakeInput, options) of the outdoor scene: an nx x nz heightfield at 1 m spacing
+    (fBm heights 0-20 m, planar UV1, one mesh), seeded box buildings (footprint 4-20 m, height
+    3-30 m), a soft sun (0.53 degrees, toward (0.45, 0.75, 0.5), one bounce of 32 samples),
+    rectangle and tube area lights (0.5-4 m, range 8-20 m, 16 samples) and a baked sky (64 rays)."""
+    rng = Pcg32.seeded(seed)
+    h = (_fbm(rng, nx, nz, octaves) * height).astype(np.float32)
+    gx, gz = np.meshgrid(np.arange(nx, dtype=np.float32), np.arange(nz, dtype=np.float32))
+    terrain_pos = np.stack([gx, h, gz], axis=-1).reshape(-1, 3).astype(np.float32)
+    du = np.gradient(h.astype(np.float64), axis=1)
+    dv = np.gradient(h.astype(np.float64), axis=0)
+    nrm = np.stack([-du, np.ones_like(du), -dv], axis=-1)
+    nrm = (nrm / np.linalg.norm(nrm, axis=-1, keepdims=True)).reshape(-1, 3).astype(np.float32)
+    uv = np.stack([gx / max(1, nx - 1), gz / max(1, nz - 1)], axis=-1).reshape(-1, 2).astype(np.float32)
+    q = (np.arange(nz - 1)[:, None] * nx + np.arange(nx - 1)[None, :]).reshape(-1)
+    a, b, c, d = q, q + 1, q + nx + 1, q + nx
+    idx = np.stack([a, c, b, a, d, c], axis=-1).reshape(-1).astype(np.uint32)  # counter-clockwise from +Y
+    positions, normals, indices, uvs, ids, albedo = [terrain_pos], [nrm], [idx], [uv], [1], [(0.35, 0.45, 0.3)]
+    vcount, icount = [0, len(terrain_pos)], [0, len(idx)]
+    local, tnrm, tidx, tuv = _TEMPLATE
+    for k in range(buildings):
+        ext = [rng.uniform(4.0, 20.0), rng.uniform(3.0, 30.0), rng.uniform(4.0, 20.0)]
+        cx = rng.uniform(ext[0] / 2, nx - 1 - ext[0] / 2)
+        cz = rng.uniform(ext[2] / 2, nz - 1 - ext[2] / 2)
+        base = float(h[min(nz - 1, int(cz)), min(nx - 1, int(cx))]) - 1.0
+        mn = np.array([cx - ext[0] / 2, base, cz - ext[2] / 2], np.float32)
+        mx = np.array([cx + ext[0] / 2, base + ext[1], cz + ext[2] / 2], np.float32)
+        pos = ((mn + mx) * np.float32(0.5) + local * (mx - mn) * np.float32(0.5)).astype(np.float32)
+        positions.append(pos)
+        normals.append(tnrm)
+        indices.append(tidx)
+        uvs.append(tuv)
+        ids.append(2 + k)
+        albedo.append((rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8)))
+        vcount.append(vcount[-1] + len(pos))
+        icount.append(icount[-1] + len(tidx))
+    sun = sun_light(100_000, (0.45, 0.75, 0.5), 0.53)
+    sun.illumination = ILLUMINATION_SINGLE_BOUNCE
+    sun.bounce_samples = 32
+    light_list = [sun, sky_light(100_001)]
+    for k in range(area_lights):
+        x, z = rng.uniform(0.0, nx - 1), rng.uniform(0.0, nz - 1)
+        y = float(h[min(nz - 1, int(z)), min(nx - 1, int(x))]) + rng.uniform(1.0, 6.0)
+        length = rng.uniform(0.5, 4.0)
+        radius = rng.uniform(8.0, 20.0)
+        if k % 2 == 0:
+            light_list.append(area_light(200_000 + k, (x, y, z), radius, quat_from_rotation_x(-math.pi / 2),
+                                         (length, rng.uniform(0.5, 4.0))))
+        else:
+            yaw = rng.uniform(0.0, math.pi)
+            light_list.append(area_light(200_000 + k, (x, y, z), radius, (0.0, math.sin(yaw / 2), 0.0, math.cos(yaw / 2)),
+                                         (length, 0.0)))
+    inp = BakeInput.from_arrays(
+        ids=np.array(ids, np.uint64), positions=np.concatenate(positions), indices=np.concatenate(indices),
+        uv1=np.concatenate(uvs), normals=np.concatenate(normals),
+        vertex_offsets=np.array(vcount, np.int64), index_offsets=np.array(icount, np.int64),
+        albedo=np.array(albedo, np.float32), content_hash=np.array([splitmix64(i) for i in ids], np.uint64),
+        lights=light_list)
+    opts = default_options()
+    opts.pixel_density = density
+    opts.area_light_samples = 16
+    opts.area_light_min_samples = 16
+    opts.sky_samples = 64
+    opts.sun_map_resolution = 1024
+    opts.sky_probe_spacing = 1.0
     opts.dynamic_geometry = DYNAMIC_ANGULAR
     return inp, opts
 

@@ -245,10 +245,12 @@ typedef struct dl_timings {
     uint64_t occluder_triangles;
     uint64_t trace_node_visits; /* BVH nodes fetched by the shadow-ray kernel (counting on)  */
     uint64_t trace_tri_tests;   /* triangles tested by the shadow-ray kernel (counting on)  */
-    double bounce_kernel_ms;    /* summed device time of the bounce-gather kernel launches   */
-    uint64_t bounce_node_visits;/* BVH nodes fetched by the bounce-gather kernel (counting)  */
-    uint64_t bounce_tri_tests;  /* triangles tested by the bounce-gather kernel (counting)   */
-    uint64_t bounce_launches;
+    double bounce_kernel_ms;    /* device time of the gather passes, one span per mesh batch */
+    uint64_t bounce_node_visits;/* BVH nodes fetched by the gather's visibility rays (counting) */
+    uint64_t bounce_tri_tests;  /* triangles tested by the gather's visibility rays (counting) */
+    uint64_t bounce_launches;   /* mesh batches with bounce work                              */
+    double gather_trace_ms;     /* summed device time of the any-hit gather kernel launches   */
+    uint64_t gather_trace_launches;
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 

@@ -31,6 +31,13 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
 /* Number of exception entries of the sin and cos tables built for this context. */
 void dl_test_trig_table_sizes(const dl_ctx* ctx, uint64_t* sin_entries, uint64_t* cos_entries);
 
+/* The host evaluation of the table path (no device) compared with the host libm for every float
+ * with bit pattern in [lo, hi]: the number of sin and cos results that differ. */
+int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches);
+
+/* The exception inputs of sin (which = 0) or cos (1): copies up to cap keys, returns the count. */
+uint64_t dl_test_trig_keys(int which, uint32_t* out, uint64_t cap);
+
 /* Counting mode: when enabled, later bakes on this context count the BVH nodes fetched and the
  * triangles tested by the shadow-ray kernel (dl_timings.trace_node_visits / trace_tri_tests). */
 int dl_set_counting(dl_ctx* ctx, int enable);
@@ -39,11 +46,15 @@ int dl_set_counting(dl_ctx* ctx, int enable);
  * shadow-ray and burial kernels. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
 
-/* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels), mode 1 =
- * streamed per-lane jobs (default).  For mode 1: threads per block, jobs per queue chunk,
- * the live-ray count at or below which lanes take new samples, and blocks per CU (0 = as many
- * as fit).  Zero keeps the current value for block, chunk and refill. */
-int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint32_t refill, uint32_t blocks_per_cu);
+/* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
+ * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
+ * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per
+ * chunk of jobs (the ray buffer's size), blocks per CU (0 = as many as fit) and the occupancy the
+ * any-hit kernel's registers are capped for (8 waves per SIMD, or 0 for no cap), and how many of a
+ * wave's lanes must be idle before they take new rays (1-64).  Zero keeps the current value for
+ * block, chunk, chunk_rays and refill. */
+int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint64_t chunk_rays, uint32_t blocks_per_cu,
+                  uint32_t min_waves, uint32_t refill);
 
 #ifdef __cplusplus
 }

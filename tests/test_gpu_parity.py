@@ -32,6 +32,17 @@ def test_trig_matches_host_libm(ctx):
     assert ctx.lib.dl_test_trig_host(_fp(x), x.size, _fp(hs), _fp(hc)) == 0
     assert np.array_equal(ds.view(np.uint32), hs.view(np.uint32))
     assert np.array_equal(dc.view(np.uint32), hc.view(np.uint32))
+    # every exception input (where the host libm is not the nearest float)
+    for which in (0, 1):
+        n = ctx.lib.dl_test_trig_keys(which, None, 0)
+        keys = np.zeros(n, np.uint32)
+        ctx.lib.dl_test_trig_keys(which, keys.ctypes.data_as(U32P), n)
+        xe = np.ascontiguousarray(keys.view(np.float32))
+        es, ec, hs2, hc2 = (np.zeros_like(xe) for _ in range(4))
+        assert ctx.lib.dl_test_trig(ctx.handle, _fp(xe), xe.size, _fp(es), _fp(ec)) == 0, ctx.error()
+        assert ctx.lib.dl_test_trig_host(_fp(xe), xe.size, _fp(hs2), _fp(hc2)) == 0
+        assert np.array_equal(es.view(np.uint32), hs2.view(np.uint32))
+        assert np.array_equal(ec.view(np.uint32), hc2.view(np.uint32))
     ol = oracle.load()
     pick = rng.integers(0, x.size, 20000)
     gs = np.array([ol.dlo_sinf(float(v)) for v in x[pick]], dtype=np.float32)
@@ -143,3 +154,15 @@ def test_bounce_gather_modes_match_oracle(ctx, mode):
                 assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
     finally:
         ctx.set_gather(1)
+
+
+def test_outdoor_matches_oracle(ctx):
+    """A 48x32 heightfield with 12 buildings: soft sun with one bounce, rectangle and tube area
+    lights (16 samples), a baked sky, sun map and sky probes."""
+    inp, opts = scenes.outdoor(nx=48, nz=32, buildings=12, area_lights=6, density=2.0)
+    opts.sun_map_resolution = 128
+    opts.sky_samples = 16
+    opts.sky_probe_spacing = 4.0
+    got = ctx.bake(inp, opts)
+    assert got.stats["bounce_rays"] > 100_000 and len(got.sun_maps) == 1 and got.sky_probes is not None
+    assert_scenes_equal(got, oracle.bake(inp, opts))
