@@ -328,7 +328,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   if (!cube_slot.empty()) {
     uint64_t photons = 6ull * cres * cres * cube_slot.size();
     HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
-    HIP_TRY(d_photons.alloc(photons * 3));
+    HIP_TRY(d_photons.alloc(photons * 4));  // one 16-byte record per photon (12 bytes used)
     HIP_TRY(launch_photons(s, sc, occ, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, 128));
     HIP_TRY(hipStreamSynchronize(s));
     advance(cube_slot.size());

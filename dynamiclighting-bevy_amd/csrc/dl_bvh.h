@@ -16,8 +16,8 @@ namespace dl {
 
 struct alignas(16) Node4 {
   float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
-  int32_t child[4];  // >= 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1))
-  int32_t count;     // children in use (1..4)
+  int32_t child[4];  // > 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1)); 0 unused slot
+  int32_t count;     // children in use (1..4); kernels read the child slots instead
   int32_t pad[3];
 };
 static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");

@@ -19,8 +19,11 @@ int set_err(dl_ctx* ctx, int code, const std::string& msg) {
 int upload_trig(dl_ctx* ctx) {
   using namespace dl;
   const TrigHost& t = ctx->trig_host;
-  // six arrays, each followed by one spare word
-  size_t total = t.sin_bucket.size() + t.cos_bucket.size() + 2 * t.sin_keys.size() + 2 * t.cos_keys.size() + 6;
+  if (t.sin_keys.size() >= (1u << 23) || t.cos_keys.size() >= (1u << 23))
+    return set_err(ctx, DL_E_DEVICE, "too many sin/cos exceptions for the bucket words");
+  // four arrays, each padded to an even word count
+  auto words = [](const std::vector<uint32_t>& v) { return (v.size() + 2) & ~(size_t)1; };
+  size_t total = words(t.sin_bucket) + words(t.cos_bucket) + words(t.sin_pairs) + words(t.cos_pairs);
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
@@ -29,18 +32,25 @@ int upload_trig(dl_ctx* ctx) {
   auto put = [&](const std::vector<uint32_t>& v) -> const uint32_t* {
     const uint32_t* p = base + off;
     if (!v.empty() && copy == hipSuccess) copy = hipMemcpy(base + off, v.data(), v.size() * 4, hipMemcpyHostToDevice);
-    off += v.size() + 1;
+    off += words(v);
     return p;
   };
   ctx->trig.sin.bucket = put(t.sin_bucket);
-  ctx->trig.sin.keys = put(t.sin_keys);
-  ctx->trig.sin.vals = put(t.sin_vals);
+  ctx->trig.sin.pairs = reinterpret_cast<const uint2*>(put(t.sin_pairs));
   ctx->trig.cos.bucket = put(t.cos_bucket);
-  ctx->trig.cos.keys = put(t.cos_keys);
-  ctx->trig.cos.vals = put(t.cos_vals);
+  ctx->trig.cos.pairs = reinterpret_cast<const uint2*>(put(t.cos_pairs));
   ctx->trig.band = t.band;
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
+}
+}  // namespace
+
+namespace {
+const dl::TrigHost& host_trig() {
+  static std::once_flag once;
+  static dl::TrigHost host;
+  std::call_once(once, [] { dl::build_trig_tables(host, std::max(1u, std::thread::hardware_concurrency())); });
+  return host;
 }
 }  // namespace
 
@@ -238,21 +248,13 @@ int dl_test_trig(dl_ctx* ctx, const float* x, uint64_t n, float* sin_out, float*
   return DL_OK;
 }
 
-namespace {
-const dl::TrigHost& host_trig() {
-  static std::once_flag once;
-  static dl::TrigHost host;
-  std::call_once(once, [] { dl::build_trig_tables(host, std::max(1u, std::thread::hardware_concurrency())); });
-  return host;
-}
-}  // namespace
 
 int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches) {
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {host.sin_bucket.data(), host.sin_keys.data(), host.sin_vals.data()};
-  t.cos = {host.cos_bucket.data(), host.cos_keys.data(), host.cos_vals.data()};
+  t.sin = {host.sin_bucket.data(), reinterpret_cast<const uint2*>(host.sin_pairs.data())};
+  t.cos = {host.cos_bucket.data(), reinterpret_cast<const uint2*>(host.cos_pairs.data())};
   t.band = host.band;
   const unsigned n = std::max(1u, std::thread::hardware_concurrency());
   std::vector<uint64_t> bad(n, 0);
@@ -282,8 +284,8 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {host.sin_bucket.data(), host.sin_keys.data(), host.sin_vals.data()};
-  t.cos = {host.cos_bucket.data(), host.cos_keys.data(), host.cos_vals.data()};
+  t.sin = {host.sin_bucket.data(), reinterpret_cast<const uint2*>(host.sin_pairs.data())};
+  t.cos = {host.cos_bucket.data(), reinterpret_cast<const uint2*>(host.cos_pairs.data())};
   t.band = host.band;
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);

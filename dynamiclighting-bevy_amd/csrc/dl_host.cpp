@@ -48,7 +48,7 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
   work();
   for (auto& t : ts) t.join();
   auto gather = [&](std::vector<std::vector<uint32_t>>& k, std::vector<std::vector<uint32_t>>& v, std::vector<uint32_t>& keys,
-                    std::vector<uint32_t>& vals, std::vector<uint32_t>& bucket) {
+                    std::vector<uint32_t>& vals, std::vector<uint32_t>& bucket, std::vector<uint32_t>& pairs) {
     keys.clear();
     vals.clear();
     // jobs are in ascending bit order within each range, and the positive range precedes the
@@ -57,15 +57,21 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
       keys.insert(keys.end(), k[j].begin(), k[j].end());
       vals.insert(vals.end(), v[j].begin(), v[j].end());
     }
-    bucket.assign(65537, 0);
+    bucket.assign(kTrigBuckets, 0);
     size_t i = 0;
-    for (uint32_t b = 0; b <= 65536; ++b) {
-      while (i < keys.size() && (keys[i] >> 16) < b) ++i;
-      bucket[b] = (uint32_t)i;
+    for (uint32_t b = 0; b < kTrigBuckets; ++b) {
+      const size_t first = i;
+      while (i < keys.size() && (trig_domain_index(keys[i]) >> kTrigBucketShift) == b) ++i;
+      bucket[b] = (uint32_t)(first << 9) | (uint32_t)(i - first);  // at most 256 pairs, first < 2^23
+    }
+    pairs.resize(2 * keys.size());
+    for (size_t j = 0; j < keys.size(); ++j) {
+      pairs[2 * j] = keys[j];
+      pairs[2 * j + 1] = vals[j];
     }
   };
-  gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_bucket);
-  gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_bucket);
+  gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_bucket, out.sin_pairs);
+  gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_bucket, out.cos_pairs);
   // The band: every exception's double value must lie within it of a rounding midpoint, so the
   // kernels search exactly when an exception is possible.  Start from the largest relative
   // distance among the exceptions and widen until the kernels' own test accepts every one.

@@ -143,7 +143,7 @@ struct BounceParams {
   uint64_t seed;
   uint32_t colored;
   uint32_t cube_res;
-  const uint32_t* photons;  // 3 words per photon, 6 * res^2 per cube
+  const uint32_t* photons;  // 4 words per photon (distance, normal, albedo, 0), 6 * res^2 per cube
   uint32_t sun_only;        // k_bounce_gather: point lights are gathered by the streamed kernel
 };
 // Bounce work: entries (triangle-light pairs) whose light bakes bounce, each with its window.

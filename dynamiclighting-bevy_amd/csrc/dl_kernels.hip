@@ -97,8 +97,7 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
                                               int4* ch) {
   const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
   const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
-  *ch = *reinterpret_cast<const int4*>(nb + 6);
-  const int cnt = reinterpret_cast<const int*>(nb + 7)[0];
+  *ch = *reinterpret_cast<const int4*>(nb + 6);  // 0 marks an unused child slot (the root is no child)
   uint32_t mask = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -110,7 +109,7 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
     n = __builtin_fmaf(fabsf(n), -kBoxEps, n);
     f = __builtin_fmaf(fabsf(f), kBoxEps, f);
     tn[k] = n;
-    if (k < cnt && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
+    if (sel4(*ch, k) != 0 && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
   }
   return mask;
 }
@@ -1143,9 +1142,7 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
       w2 = sat_u8(roundf(clampf_(a[0], 0.0f, 1.0f) * 255.0f)) | (sat_u8(roundf(clampf_(a[1], 0.0f, 1.0f) * 255.0f)) << 8) |
            (sat_u8(roundf(clampf_(a[2], 0.0f, 1.0f) * 255.0f)) << 16);
     }
-    photons[3 * i] = w0;
-    photons[3 * i + 1] = w1;
-    photons[3 * i + 2] = w2;
+    reinterpret_cast<uint4*>(photons)[i] = make_uint4(w0, w1, w2, 0u);
   }
 }
 
@@ -1268,7 +1265,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
         if (p.sun_only || len_sq(to_texel) > L.radius * L.radius) {
           write = false;
         } else {
-          const uint32_t* cube = p.photons + (size_t)b.cube_of_slot[ent.light[u.e]] * 6 * p.cube_res * p.cube_res * 3;
+          const uint32_t* cube = p.photons + (size_t)b.cube_of_slot[ent.light[u.e]] * 6 * p.cube_res * p.cube_res * 4;
           float step = 0.9f / (float)(samples - 1);
           f3 direction = normalize_or(to_texel, mk3(0.0f, 1.0f, 0.0f));
           f3 origin = add(pos, mul(normal, kRayOffset));
@@ -1278,7 +1275,8 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
             f3 rnd = on_unit_sphere(trig, rng);
             f3 sample = normalize_or(slerp3(trig, direction, rnd, spread), direction);
             uint32_t ci = cube_texel_of(sample, p.cube_res);
-            uint32_t w0 = cube[3 * ci], w1 = cube[3 * ci + 1], w2 = cube[3 * ci + 2];
+            const uint4 ph = reinterpret_cast<const uint4*>(cube)[ci];
+            const uint32_t w0 = ph.x, w1 = ph.y, w2 = ph.z;
             f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
                                           (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
             f3 photon = add(lp, mul(sample, __uint_as_float(w0)));
@@ -1418,7 +1416,7 @@ __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMesh
   g.step = 0.9f / (float)(g.samples - 1);
   g.dir0 = normalize_or(sub(pos, g.lp), mk3(0.0f, 1.0f, 0.0f));
   g.origin = add(pos, mul(g.normal, kRayOffset));
-  g.cube = p.photons + (size_t)b.cube_of_slot[slot] * 6 * p.cube_res * p.cube_res * 3;
+  g.cube = p.photons + (size_t)b.cube_of_slot[slot] * 6 * p.cube_res * p.cube_res * 4;
   const uint32_t m = tr.mesh[t];
   const uint32_t xy = tx.xy[j];
   *rng = stream5(p.seed, ms.id[m], L.id, xy & 0xffffu, xy >> 16);
@@ -1434,7 +1432,8 @@ __device__ __forceinline__ bool gather_sample(const GatherJob& g, const TrigTabl
   f3 rnd = on_unit_sphere(trig, rng);
   f3 sample = normalize_or(slerp3(trig, g.dir0, rnd, spread), g.dir0);
   uint32_t ci = cube_texel_of(sample, res);
-  uint32_t w0 = g.cube[3 * ci], w1 = g.cube[3 * ci + 1], w2 = g.cube[3 * ci + 2];
+  const uint4 ph = reinterpret_cast<const uint4*>(g.cube)[ci];
+  const uint32_t w0 = ph.x, w1 = ph.y, w2 = ph.z;
   f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
                                 (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
   f3 photon = add(g.lp, mul(sample, __uint_as_float(w0)));

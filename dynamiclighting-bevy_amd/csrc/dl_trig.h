@@ -7,7 +7,7 @@
 // round once to f32; the host compares that value with its own sinf/cosf for every float in the
 // domain the bake uses and stores the inputs where they differ.  Every such input lies close to
 // a rounding midpoint of the double value (glibc is off by less than 0.07 ulp), so a kernel only
-// searches the sorted exception list (a bucket index on the top 16 bits plus a binary search)
+// searches the sorted exception list (a bucket index over runs of 256 inputs, then a short scan)
 // when the double value is within the host-measured band around a midpoint.
 #pragma once
 
@@ -61,11 +61,12 @@ DLI double trig_value(float xf, int which) {
 // which = 0: sin, 1: cos.  Rounded once to f32.
 DLI float trig_core(float xf, int which) { return (float)trig_value(xf, which); }
 
-// Device view of one function's exception list.
+// Device view of one function's exception list: (input bits, host libm result bits) pairs in
+// ascending input order (which is also ascending domain index), and per run of 256 domain inputs
+// one word: first pair << 9 | pairs in the run (kTrigBuckets words).
 struct TrigTable {
-  const uint32_t* bucket;  // 65537 entries: first exception with (bits >> 16) >= b
-  const uint32_t* keys;    // input bit patterns, ascending
-  const uint32_t* vals;    // host libm result bits
+  const uint32_t* bucket;
+  const uint2* pairs;
 };
 struct TrigTables {
   TrigTable sin, cos;
@@ -79,8 +80,14 @@ static constexpr uint32_t kDomPosLo = 0x37800000u, kDomPosHi = 0x41000000u;  // 
 static constexpr uint32_t kDomNegLo = 0xB7800000u, kDomNegHi = 0xBF4CCCCDu;  // [-0.8, -2^-16]
 static constexpr uint32_t kDomPosCount = kDomPosHi - kDomPosLo + 1;
 static constexpr uint32_t kDomCount = kDomPosCount + (kDomNegHi - kDomNegLo + 1);
+static constexpr uint32_t kTrigBucketShift = 8;
+static constexpr uint32_t kTrigBuckets = (kDomCount >> kTrigBucketShift) + 1;
 DLI bool trig_in_domain(uint32_t bits) {
   return (bits >= kDomPosLo && bits <= kDomPosHi) || (bits >= kDomNegLo && bits <= kDomNegHi);
+}
+// Position of an in-domain input in the domain (positive range first).
+DLI uint32_t trig_domain_index(uint32_t bits) {
+  return bits <= kDomPosHi ? bits - kDomPosLo : kDomPosCount + (bits - kDomNegLo);
 }
 
 DLI uint32_t f2u(float f) {
@@ -94,16 +101,15 @@ DLI float u2f(uint32_t u) {
   return f;
 }
 
+// An in-domain input's libm result when it is an exception, else `computed`: the bucket of
+// 256 consecutive inputs holds about two exceptions, scanned in order.
 DLI float trig_lookup(const TrigTable& t, float x, float computed) {
-  uint32_t bits = f2u(x);
-  uint32_t b = bits >> 16;
-  uint32_t lo = t.bucket[b], hi = t.bucket[b + 1];
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    uint32_t k = t.keys[mid];
-    if (k == bits) return u2f(t.vals[mid]);
-    if (k < bits) lo = mid + 1;
-    else hi = mid;
+  const uint32_t bits = f2u(x);
+  const uint32_t w = t.bucket[trig_domain_index(bits) >> kTrigBucketShift];
+  const uint32_t lo = w >> 9, hi = lo + (w & 511u);
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint2 p = t.pairs[i];
+    if (p.x == bits) return u2f(p.y);
   }
   return computed;
 }
