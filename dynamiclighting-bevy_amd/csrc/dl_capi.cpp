@@ -19,11 +19,9 @@ int set_err(dl_ctx* ctx, int code, const std::string& msg) {
 int upload_trig(dl_ctx* ctx) {
   using namespace dl;
   const TrigHost& t = ctx->trig_host;
-  if (t.sin_keys.size() >= (1u << 23) || t.cos_keys.size() >= (1u << 23))
-    return set_err(ctx, DL_E_DEVICE, "too many sin/cos exceptions for the bucket words");
-  // four arrays, each padded to an even word count
+  // four arrays, each padded to an even word count (8-byte alignment of the run pairs)
   auto words = [](const std::vector<uint32_t>& v) { return (v.size() + 2) & ~(size_t)1; };
-  size_t total = words(t.sin_bucket) + words(t.cos_bucket) + words(t.sin_pairs) + words(t.cos_pairs);
+  size_t total = words(t.sin_runs) + words(t.cos_runs) + words(t.sin_vals) + words(t.cos_vals);
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
@@ -35,10 +33,10 @@ int upload_trig(dl_ctx* ctx) {
     off += words(v);
     return p;
   };
-  ctx->trig.sin.bucket = put(t.sin_bucket);
-  ctx->trig.sin.pairs = reinterpret_cast<const uint2*>(put(t.sin_pairs));
-  ctx->trig.cos.bucket = put(t.cos_bucket);
-  ctx->trig.cos.pairs = reinterpret_cast<const uint2*>(put(t.cos_pairs));
+  ctx->trig.sin.runs = reinterpret_cast<const uint2*>(put(t.sin_runs));
+  ctx->trig.cos.runs = reinterpret_cast<const uint2*>(put(t.cos_runs));
+  ctx->trig.sin.vals = put(t.sin_vals);
+  ctx->trig.cos.vals = put(t.cos_vals);
   ctx->trig.band = t.band;
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
@@ -253,8 +251,8 @@ int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches) {
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {host.sin_bucket.data(), reinterpret_cast<const uint2*>(host.sin_pairs.data())};
-  t.cos = {host.cos_bucket.data(), reinterpret_cast<const uint2*>(host.cos_pairs.data())};
+  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data()};
+  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data()};
   t.band = host.band;
   const unsigned n = std::max(1u, std::thread::hardware_concurrency());
   std::vector<uint64_t> bad(n, 0);
@@ -284,8 +282,8 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {host.sin_bucket.data(), reinterpret_cast<const uint2*>(host.sin_pairs.data())};
-  t.cos = {host.cos_bucket.data(), reinterpret_cast<const uint2*>(host.cos_pairs.data())};
+  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data()};
+  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data()};
   t.band = host.band;
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);

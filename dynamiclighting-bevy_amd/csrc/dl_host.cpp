@@ -48,30 +48,28 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
   work();
   for (auto& t : ts) t.join();
   auto gather = [&](std::vector<std::vector<uint32_t>>& k, std::vector<std::vector<uint32_t>>& v, std::vector<uint32_t>& keys,
-                    std::vector<uint32_t>& vals, std::vector<uint32_t>& bucket, std::vector<uint32_t>& pairs) {
+                    std::vector<uint32_t>& vals, std::vector<uint32_t>& runs) {
     keys.clear();
     vals.clear();
     // jobs are in ascending bit order within each range, and the positive range precedes the
-    // negative one, so concatenation is sorted by bit pattern.
+    // negative one, so concatenation is sorted by bit pattern and by domain index.
     for (size_t j = 0; j < k.size(); ++j) {
       keys.insert(keys.end(), k[j].begin(), k[j].end());
       vals.insert(vals.end(), v[j].begin(), v[j].end());
     }
-    bucket.assign(kTrigBuckets, 0);
-    size_t i = 0;
-    for (uint32_t b = 0; b < kTrigBuckets; ++b) {
-      const size_t first = i;
-      while (i < keys.size() && (trig_domain_index(keys[i]) >> kTrigBucketShift) == b) ++i;
-      bucket[b] = (uint32_t)(first << 9) | (uint32_t)(i - first);  // at most 256 pairs, first < 2^23
+    runs.assign(2 * (size_t)kTrigRuns, 0u);
+    for (uint32_t key : keys) {
+      const uint32_t di = trig_domain_index(key);
+      runs[2 * (size_t)(di >> 5)] |= 1u << (di & 31u);
     }
-    pairs.resize(2 * keys.size());
-    for (size_t j = 0; j < keys.size(); ++j) {
-      pairs[2 * j] = keys[j];
-      pairs[2 * j + 1] = vals[j];
+    uint32_t before = 0;
+    for (size_t r = 0; r < kTrigRuns; ++r) {
+      runs[2 * r + 1] = before;
+      before += (uint32_t)__builtin_popcount(runs[2 * r]);
     }
   };
-  gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_bucket, out.sin_pairs);
-  gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_bucket, out.cos_pairs);
+  gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_runs);
+  gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_runs);
   // The band: every exception's double value must lie within it of a rounding midpoint, so the
   // kernels search exactly when an exception is possible.  Start from the largest relative
   // distance among the exceptions and widen until the kernels' own test accepts every one.

@@ -38,9 +38,8 @@ struct DBuf {
 
 // The exception tables of f32 sin/cos (see dl_trig.h), built from the host's libm.
 struct TrigHost {
-  std::vector<uint32_t> sin_keys, sin_vals, cos_keys, cos_vals;
-  std::vector<uint32_t> sin_bucket, cos_bucket;  // see TrigTable::bucket
-  std::vector<uint32_t> sin_pairs, cos_pairs;    // key, value interleaved
+  std::vector<uint32_t> sin_keys, sin_vals, cos_keys, cos_vals;  // exceptions in ascending order
+  std::vector<uint32_t> sin_runs, cos_runs;  // see TrigTable::runs (bitmap, rank) word pairs
   double band = 0.0;  // see TrigTables::band
 };
 void build_trig_tables(TrigHost& out, unsigned threads);

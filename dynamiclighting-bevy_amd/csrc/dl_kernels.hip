@@ -12,6 +12,10 @@
 namespace dl {
 
 static constexpr float kBoxEps = 1.0f / 65536.0f;
+#ifndef DL_GATHER_GROUP
+#define DL_GATHER_GROUP 2
+#endif
+static constexpr int kGatherGroup = DL_GATHER_GROUP;  // samples generated together per job
 static constexpr uint32_t kThreads = 256;
 
 __device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ start, uint64_t n, uint64_t idx) {
@@ -1450,6 +1454,119 @@ __device__ __forceinline__ bool gather_sample(const GatherJob& g, const TrigTabl
   return true;
 }
 
+// ---- batched sample generation ------------------------------------------------------------
+// The exception lookups of N values of one function (see trig_finish), with the N lookups'
+// loads issued together: the run words first, then the results of the actual exceptions.
+template <int N>
+__device__ __forceinline__ void trig_finish_n(const TrigTables& t, const TrigTable& tab, const float* x, const double* v, float* out) {
+  bool need[N];
+  uint2 run[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    out[i] = (float)v[i];
+    need[i] = trig_in_domain(f2u(x[i])) && trig_near_midpoint(v[i], out[i], t.band);
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) run[i] = need[i] ? tab.runs[trig_domain_index(f2u(x[i])) >> 5] : make_uint2(0u, 0u);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t bit = trig_domain_index(f2u(x[i])) & 31u;
+    if (need[i] && ((run[i].x >> bit) & 1u))
+      out[i] = u2f(tab.vals[run[i].y + (uint32_t)__builtin_popcount(run[i].x & ((1u << bit) - 1u))]);
+  }
+}
+
+// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples), the same values as
+// G calls of gather_sample in order: random draws in sample order, then the sin/cos searches
+// and the photon fetches of the G samples issued together.  la = |g.dir0|.
+template <int G>
+__device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k0, float la,
+                                             uint32_t res, uint32_t colored, f3* dout, float* dist, f3* cout, bool* ray) {
+  const f3 a = g.dir0;
+  // on_unit_sphere (random.rs:22-28): the draws, then sin/cos of the azimuths
+  float z[G], ang[G];
+  double vs[G], vc[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    z[j] = 0.0f;
+    ang[j] = 0.0f;
+    if (k0 + j < g.samples) {
+      z[j] = pcg_f32(rng) * 2.0f - 1.0f;
+      ang[j] = pcg_f32(rng) * kTau;
+    }
+    int q;
+    const double r = trig_reduce(ang[j], &q);
+    vs[j] = trig_select(r, q, 0);
+    vc[j] = trig_select(r, q, 1);
+  }
+  float sa[G], ca[G];
+  trig_finish_n<G>(trig, trig.sin, ang, vs, sa);
+  trig_finish_n<G>(trig, trig.cos, ang, vc, ca);
+  // slerp (glam Vec3::slerp) up to its sines
+  f3 rnd[G];
+  float lb[G], sp[G], arg[3 * G];
+  bool common[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const float r = sqrtf(maxf_(1.0f - z[j] * z[j], 0.0f));
+    rnd[j] = mk3(r * ca[j], r * sa[j], z[j]);
+    sp[j] = 0.1f + (float)(k0 + j) * g.step;
+    lb[j] = len(rnd[j]);
+    const float dd = dot(a, rnd[j]) / (la * lb[j]);
+    common[j] = fabsf(dd) < 1.0f - 3.0e-7f;
+    const float theta = common[j] ? acos_approx(dd) : 0.0f;
+    arg[3 * j] = theta;
+    arg[3 * j + 1] = theta * (1.0f - sp[j]);
+    arg[3 * j + 2] = theta * sp[j];
+  }
+  double va[3 * G];
+#pragma unroll
+  for (int i = 0; i < 3 * G; ++i) va[i] = trig_value(arg[i], 0);
+  float sv[3 * G];
+  trig_finish_n<3 * G>(trig, trig.sin, arg, va, sv);
+  // sample direction, cube texel, photon
+  uint32_t ci[G];
+  f3 sample[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    f3 sl;
+    if (common[j]) {
+      const float rl = la + (lb[j] - la) * sp[j];
+      sl = mul(add(mul(mul(a, rl / la), sv[3 * j + 1]), mul(mul(rnd[j], rl / lb[j]), sv[3 * j + 2])), 1.0f / sv[3 * j]);
+    } else {
+      sl = slerp3(trig, a, rnd[j], sp[j]);
+    }
+    sample[j] = normalize_or(sl, a);
+    ci[j] = cube_texel_of(sample[j], res);
+  }
+  uint4 ph[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) ph[j] = k0 + j < g.samples ? reinterpret_cast<const uint4*>(g.cube)[ci[j]] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const uint32_t w1 = ph[j].y, w2 = ph[j].z;
+    const f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
+                                        (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
+    const f3 photon = add(g.lp, mul(sample[j], __uint_as_float(ph[j].x)));
+    const f3 to_photon = sub(photon, g.origin);
+    const float dl = len(to_photon);
+    ray[j] = false;
+    dout[j] = mk3(0.0f, 0.0f, 0.0f);
+    cout[j] = mk3(0.0f, 0.0f, 0.0f);
+    dist[j] = 0.0f;
+    if (k0 + j >= g.samples || is_zero(pn) || dl <= 0.0f) continue;
+    const f3 dir = divs(to_photon, dl);
+    const float weight = maxf_(dot(g.normal, dir), 0.0f) * maxf_(dot(pn, neg(dir)), 0.0f);
+    const f3 alb = colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f,
+                                 (float)((w2 >> 16) & 0xffu) / 255.0f)
+                           : mk3(1.0f, 1.0f, 1.0f);
+    ray[j] = true;
+    dout[j] = dir;
+    dist[j] = dl;
+    cout[j] = mul(alb, weight);
+  }
+}
+
 // Job lists: pass 0 counts, pass 1 writes.  counts[0..1]: traced / count-only jobs (pass 0),
 // cursors[0..1] (pass 1).
 // Pass 0 writes each bounce entry's traced and count-only job counts (cnt[q], cnt[n + 1 + q]);
@@ -1560,17 +1677,24 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
       GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
       const uint32_t alpha = lights[ent.light[jb.y]].alpha ? 1u : 0u;
       const uint64_t base = rs[i];
-      for (uint32_t k = 0; k < g.samples; ++k) {
-        f3 d = mk3(0.0f, 0.0f, 0.0f), c = mk3(0.0f, 0.0f, 0.0f);
-        float dist = 0.0f;
-        const bool ray = gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c);
-        nr += ray ? 1u : 0u;
-        const uint64_t r = base + k;
-        rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist);
-        rays[2 * r + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(alpha | (ray ? 2u : 0u)));
-        contrib[3 * r] = c.x;
-        contrib[3 * r + 1] = c.y;
-        contrib[3 * r + 2] = c.z;
+      const float la = len(g.dir0);
+      constexpr int G = kGatherGroup;
+      for (uint32_t k0 = 0; k0 < g.samples; k0 += G) {
+        f3 d[G], c[G];
+        float dist[G];
+        bool ray[G];
+        gather_group<G>(g, trig, rng, k0, la, p.cube_res, p.colored, d, dist, c, ray);
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          if (k0 + j >= g.samples) continue;
+          nr += ray[j] ? 1u : 0u;
+          const uint64_t r = base + k0 + j;
+          rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist[j]);
+          rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | (ray[j] ? 2u : 0u)));
+          contrib[3 * r] = c[j].x;
+          contrib[3 * r + 1] = c[j].y;
+          contrib[3 * r + 2] = c[j].z;
+        }
       }
     }
     const uint32_t m0 = __shfl(m, 0, 64);

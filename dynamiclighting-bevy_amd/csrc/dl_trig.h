@@ -7,7 +7,7 @@
 // round once to f32; the host compares that value with its own sinf/cosf for every float in the
 // domain the bake uses and stores the inputs where they differ.  Every such input lies close to
 // a rounding midpoint of the double value (glibc is off by less than 0.07 ulp), so a kernel only
-// searches the sorted exception list (a bucket index over runs of 256 inputs, then a short scan)
+// looks its input up in a bitmap of the exceptions (with per-32-input ranks into their results)
 // when the double value is within the host-measured band around a midpoint.
 #pragma once
 
@@ -61,13 +61,14 @@ DLI double trig_value(float xf, int which) {
 // which = 0: sin, 1: cos.  Rounded once to f32.
 DLI float trig_core(float xf, int which) { return (float)trig_value(xf, which); }
 
-// Device view of one function's exception list: (input bits, host libm result bits) pairs in
-// ascending input order (which is also ascending domain index), and per run of 256 domain inputs
-// one word: first pair << 9 | pairs in the run (kTrigBuckets words).
+// Device view of one function's exception list: per run of 32 consecutive domain inputs one
+// (bitmap of the exceptions in the run, exceptions in all earlier runs) pair, and the host libm
+// results of the exceptions in domain order.
 struct TrigTable {
-  const uint32_t* bucket;
-  const uint2* pairs;
+  const uint2* runs;
+  const uint32_t* vals;
 };
+
 struct TrigTables {
   TrigTable sin, cos;
   // Inputs whose double value lies farther than band * (midpoint gap) from both rounding
@@ -80,8 +81,7 @@ static constexpr uint32_t kDomPosLo = 0x37800000u, kDomPosHi = 0x41000000u;  // 
 static constexpr uint32_t kDomNegLo = 0xB7800000u, kDomNegHi = 0xBF4CCCCDu;  // [-0.8, -2^-16]
 static constexpr uint32_t kDomPosCount = kDomPosHi - kDomPosLo + 1;
 static constexpr uint32_t kDomCount = kDomPosCount + (kDomNegHi - kDomNegLo + 1);
-static constexpr uint32_t kTrigBucketShift = 8;
-static constexpr uint32_t kTrigBuckets = (kDomCount >> kTrigBucketShift) + 1;
+static constexpr uint32_t kTrigRuns = (kDomCount + 31) / 32;
 DLI bool trig_in_domain(uint32_t bits) {
   return (bits >= kDomPosLo && bits <= kDomPosHi) || (bits >= kDomNegLo && bits <= kDomNegHi);
 }
@@ -101,17 +101,14 @@ DLI float u2f(uint32_t u) {
   return f;
 }
 
-// An in-domain input's libm result when it is an exception, else `computed`: the bucket of
-// 256 consecutive inputs holds about two exceptions, scanned in order.
+// An in-domain input's libm result when it is an exception, else `computed`: one load tells
+// whether it is (and its rank among the exceptions), a second one fetches the result.
 DLI float trig_lookup(const TrigTable& t, float x, float computed) {
-  const uint32_t bits = f2u(x);
-  const uint32_t w = t.bucket[trig_domain_index(bits) >> kTrigBucketShift];
-  const uint32_t lo = w >> 9, hi = lo + (w & 511u);
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint2 p = t.pairs[i];
-    if (p.x == bits) return u2f(p.y);
-  }
-  return computed;
+  const uint32_t di = trig_domain_index(f2u(x));
+  const uint2 run = t.runs[di >> 5];
+  const uint32_t bit = di & 31u;
+  if (!((run.x >> bit) & 1u)) return computed;
+  return u2f(t.vals[run.y + (uint32_t)__builtin_popcount(run.x & ((1u << bit) - 1u))]);
 }
 // Whether the double value v (rounded to f) lies within band * gap of a rounding midpoint,
 // where gap is the distance between the midpoints below and above f.
