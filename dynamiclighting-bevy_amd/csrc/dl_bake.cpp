@@ -236,7 +236,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   // ---- scene-level device state --------------------------------------------------------------
   auto t_up = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
-  DBuf<Node4> d_nodes;
+  DBuf<CNode> d_nodes;
   DBuf<float> d_occ, d_occ_nrm, d_occ_alb;
   DBuf<uint32_t> d_overflow, d_occ_flags;
   HIP_TRY(upload(ctx, d_nodes, bvh.nodes));

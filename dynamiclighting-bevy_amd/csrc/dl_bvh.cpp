@@ -253,6 +253,72 @@ void pad(const Box& b, float* lo, float* hi) {
   }
 }
 
+// The step of one axis: a power of two with 254 steps covering the node's extent and no finer
+// than the float spacing at the node's coordinates.
+int step_exponent(float lo, float hi) {
+  const double ext = (double)hi - (double)lo;
+  int e = -126;
+  if (ext > 0.0) e = std::max(e, (int)std::ceil(std::log2(ext / 254.0)));
+  const float big = std::max(std::fabs(lo), std::fabs(hi));
+  if (big > 0.0f) {
+    int be;
+    std::frexp(big, &be);     // big in [2^(be-1), 2^be)
+    e = std::max(e, be - 24);  // the float spacing there is 2^(be-24)
+  }
+  while (std::ldexp(254.0, e) < ext) ++e;
+  return std::min(e, 127);
+}
+
+// Node4 (padded float child boxes) -> CNode; checks that every decoded box contains its child's.
+void quantize(const Node4& nd, CNode* out) {
+  std::memset(out, 0, sizeof *out);
+  const float* los[3] = {nd.lox, nd.loy, nd.loz};
+  const float* his[3] = {nd.hix, nd.hiy, nd.hiz};
+  float origin[3];
+  uint32_t lo_q[3] = {0, 0, 0}, hi_q[3] = {0, 0, 0};
+  int exps[3];
+  for (int a = 0; a < 3; ++a) {
+    float mn = INFINITY, mx = -INFINITY;
+    for (int k = 0; k < nd.count; ++k) {
+      mn = std::min(mn, los[a][k]);
+      mx = std::max(mx, his[a][k]);
+    }
+    if (nd.count == 0) mn = mx = 0.0f;
+    origin[a] = mn;
+    exps[a] = step_exponent(mn, mx);
+    const double step = std::ldexp(1.0, exps[a]);
+    for (int k = 0; k < 4; ++k) {
+      uint32_t ql = 255, qh = 0;
+      if (k < nd.count) {
+        const double l = std::floor(((double)los[a][k] - (double)mn) / step) - 1.0;
+        const double h = std::ceil(((double)his[a][k] - (double)mn) / step) + 1.0;
+        ql = (uint32_t)std::max(0.0, std::min(255.0, l));
+        qh = (uint32_t)std::max(0.0, std::min(255.0, h));
+        // the decode as the kernels do it: one float fma of the exact q * step and the origin
+        const float dl = std::fma((float)ql, (float)step, mn), dh = std::fma((float)qh, (float)step, mn);
+        if (!(dl <= los[a][k] && dh >= his[a][k])) {
+          // cannot happen with the step bounds above; fall back to the whole node range
+          ql = 0;
+          qh = 255;
+        }
+      }
+      lo_q[a] |= ql << (8 * k);
+      hi_q[a] |= qh << (8 * k);
+    }
+  }
+  out->ox = origin[0];
+  out->oy = origin[1];
+  out->oz = origin[2];
+  out->exps = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16);
+  out->lox = lo_q[0];
+  out->hix = hi_q[0];
+  out->loy = lo_q[1];
+  out->hiy = hi_q[1];
+  out->loz = lo_q[2];
+  out->hiz = hi_q[2];
+  for (int k = 0; k < 4; ++k) out->child[k] = nd.child[k];
+}
+
 }  // namespace
 
 void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
@@ -322,7 +388,8 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
   // Collapse to 4-wide nodes.
   const std::vector<Node2>& n2 = b.nodes;
   struct Pending { uint32_t n2, n4, depth; };
-  out.nodes.emplace_back();
+  std::vector<Node4> wide;
+  wide.emplace_back();
   std::vector<Pending> work{{0, 0, 1}};
   while (!work.empty()) {
     Pending p = work.back();
@@ -364,14 +431,16 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
         uint32_t first = n2[kids[k]].first, count = n2[kids[k]].count;
         nd.child[k] = -1 - (int32_t)((first << 2) | (count - 1));
       } else {
-        uint32_t idx = (uint32_t)out.nodes.size();
-        out.nodes.emplace_back();
+        uint32_t idx = (uint32_t)wide.size();
+        wide.emplace_back();
         nd.child[k] = (int32_t)idx;
         work.push_back({kids[k], idx, p.depth + 1});
       }
     }
-    out.nodes[p.n4] = nd;
+    wide[p.n4] = nd;
   }
+  out.nodes.resize(wide.size());
+  for (size_t i = 0; i < wide.size(); ++i) quantize(wide[i], &out.nodes[i]);
 }
 
 }  // namespace dl

@@ -14,6 +14,7 @@
 
 namespace dl {
 
+// A collapsed 4-wide node with float child boxes (builder intermediate, quantized to CNode).
 struct alignas(16) Node4 {
   float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
   int32_t child[4];  // > 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1)); 0 unused slot
@@ -22,8 +23,21 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
 
+// The 4-wide node the kernels read: 64 bytes.  Child bounds are 8-bit steps from the node's
+// origin (its padded minimum corner) with a power-of-two step per axis, rounded outward by one
+// extra step, so each decoded box (origin + q * step, one float rounding) contains the padded
+// box of the child it stands for.
+struct alignas(64) CNode {
+  float ox, oy, oz;
+  uint32_t exps;        // biased float exponents of the x, y, z steps in bits 0-7, 8-15, 16-23
+  uint32_t lox, hix, loy, hiy;  // byte k: child k
+  uint32_t loz, hiz, pad0, pad1;
+  int32_t child[4];     // as Node4::child; 0 marks an unused slot
+};
+static_assert(sizeof(CNode) == 64, "CNode is 64 bytes");
+
 struct Bvh4 {
-  std::vector<Node4> nodes;
+  std::vector<CNode> nodes;
   std::vector<float> tris;      // 12 floats per reference, leaf order
   std::vector<uint32_t> order;  // leaf position -> occluder index (a split triangle appears more than once)
   uint32_t depth = 0;           // node levels on the longest root-to-leaf path

@@ -195,7 +195,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   (void)hipSetDevice(ctx->device);
   Bvh4 bvh;
   build_bvh4(tris9, tri_count, ctx->host_threads, bvh);
-  DBuf<Node4> n;
+  DBuf<CNode> n;
   DBuf<float> t, r, tout;
   DBuf<uint32_t> idx, ovf;
   auto fail = [&](hipError_t e) { return set_err(ctx, e == hipErrorOutOfMemory ? DL_E_OOM : DL_E_DEVICE, hipGetErrorString(e)); };
@@ -203,7 +203,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   if ((e = n.alloc(bvh.nodes.size())) || (e = t.alloc(bvh.tris.size())) || (e = r.alloc(ray_count * 8 + 1)) ||
       (e = tout.alloc(ray_count + 1)) || (e = idx.alloc(ray_count + 1)) || (e = ovf.alloc(1)))
     return fail(e);
-  if (!bvh.nodes.empty()) (void)hipMemcpy(n.p, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice);
+  if (!bvh.nodes.empty()) (void)hipMemcpy(n.p, bvh.nodes.data(), bvh.nodes.size() * sizeof(CNode), hipMemcpyHostToDevice);
   if (!bvh.tris.empty()) (void)hipMemcpy(t.p, bvh.tris.data(), bvh.tris.size() * 4, hipMemcpyHostToDevice);
   if (ray_count) (void)hipMemcpy(r.p, rays8, ray_count * 32, hipMemcpyHostToDevice);
   (void)hipMemset(ovf.p, 0, 4);

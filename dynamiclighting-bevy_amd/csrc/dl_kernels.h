@@ -35,7 +35,7 @@ struct DevLight {
 
 // Occluder scene on the device.
 struct DevScene {
-  const Node4* nodes;
+  const CNode* nodes;
   const float4* tris;
   uint32_t ntris;
   uint32_t stack;  // LDS stack entries per lane

@@ -92,22 +92,32 @@ __device__ __forceinline__ SlabRay slab_ray(f3 o, f3 d) {
 }
 
 // Box tests of the (up to four) children of `node`: bit k of the result is set when child k's
-// padded slab interval meets [0, limit], with tn[k] its entry distance.  Plane distances are
-// fused multiply-adds b * (1/d) - o * (1/d); their rounding is relative to the larger of |b| and
-// |o| (times 1/d), which the boxes' padding (2^-15 of the coordinate) and the interval's widening
-// (2^-16 of the distance) cover, so the test only culls boxes the ray misses.  The node record
-// is dead once this returns, which keeps it out of the registers of the leaf tests.
+// decoded slab interval meets [0, limit], with tn[k] its entry distance.  A child bound is
+// origin + q * step (8-bit q, power-of-two step); its plane distance is evaluated as
+// q * (step / d) + (origin / d - o / d) with fused multiply-adds.  step / d is exact, and the
+// roundings of the other terms are relative to the node's coordinates, the ray origin and the
+// distance itself: the extra quantization step (at least the float spacing of the node's
+// coordinates), the boxes' padding (2^-15 of the coordinate) and the interval's widening
+// (2^-16 of the distance) cover them, so the test only culls boxes the ray misses.  The node
+// record is dead once this returns, which keeps it out of the registers of the leaf tests.
 __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, const SlabRay& r, float limit, float tn[4],
                                               int4* ch) {
-  const float4* nb = reinterpret_cast<const float4*>(sc.nodes + node);
-  const float4 lox = nb[0], hix = nb[1], loy = nb[2], hiy = nb[3], loz = nb[4], hiz = nb[5];
-  *ch = *reinterpret_cast<const int4*>(nb + 6);  // 0 marks an unused child slot (the root is no child)
+  const uint4* nb = reinterpret_cast<const uint4*>(sc.nodes + node);
+  const uint4 q0 = nb[0], q1 = nb[1], q2 = nb[2];
+  *ch = *reinterpret_cast<const int4*>(nb + 3);  // 0 marks an unused child slot (the root is no child)
+  const float ax = __uint_as_float((q0.w & 0xffu) << 23) * r.ix;
+  const float ay = __uint_as_float(((q0.w >> 8) & 0xffu) << 23) * r.iy;
+  const float az = __uint_as_float(((q0.w >> 16) & 0xffu) << 23) * r.iz;
+  const float bx = __builtin_fmaf(__uint_as_float(q0.x), r.ix, -r.ox);
+  const float by = __builtin_fmaf(__uint_as_float(q0.y), r.iy, -r.oy);
+  const float bz = __builtin_fmaf(__uint_as_float(q0.z), r.iz, -r.oz);
   uint32_t mask = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float t0x = __builtin_fmaf(sel4(lox, k), r.ix, -r.ox), t1x = __builtin_fmaf(sel4(hix, k), r.ix, -r.ox);
-    const float t0y = __builtin_fmaf(sel4(loy, k), r.iy, -r.oy), t1y = __builtin_fmaf(sel4(hiy, k), r.iy, -r.oy);
-    const float t0z = __builtin_fmaf(sel4(loz, k), r.iz, -r.oz), t1z = __builtin_fmaf(sel4(hiz, k), r.iz, -r.oz);
+    const int sh = 8 * k;
+    const float t0x = __builtin_fmaf((float)((q1.x >> sh) & 0xffu), ax, bx), t1x = __builtin_fmaf((float)((q1.y >> sh) & 0xffu), ax, bx);
+    const float t0y = __builtin_fmaf((float)((q1.z >> sh) & 0xffu), ay, by), t1y = __builtin_fmaf((float)((q1.w >> sh) & 0xffu), ay, by);
+    const float t0z = __builtin_fmaf((float)((q2.x >> sh) & 0xffu), az, bz), t1z = __builtin_fmaf((float)((q2.y >> sh) & 0xffu), az, bz);
     float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
     float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
     n = __builtin_fmaf(fabsf(n), -kBoxEps, n);
