@@ -50,6 +50,8 @@ uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOU
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
                  dl_baked_scene** out) {
   auto t_start = std::chrono::steady_clock::now();
+  ctx->cache.stream = ctx->stream;
+  const CacheScope cache_scope(&ctx->cache);  // before every device buffer of the bake
   ctx->timings = dl_timings{};
   if (!input || !options || !out) { ctx->err = "null argument"; return DL_E_ARGUMENT; }
   *out = nullptr;
@@ -421,11 +423,22 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   }
 
   // ---- batches of meshes under the HBM budget ---------------------------------------------------
-  // Default: half of the device memory still free after the scene-level buffers.
+  // Default: half of the device memory still available after the scene-level buffers (free in
+  // the driver, or held unused by the memory pool or this context's cache), over the number of
+  // bakes running side by side.
   uint64_t budget = 48ull << 30;
   {
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > (2ull << 30)) budget = free_b / 2;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+      uint64_t avail = free_b + ctx->cache.cached;
+      hipMemPool_t pool = nullptr;
+      uint64_t reserved = 0, used = 0;
+      if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool &&
+          hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+          hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used)
+        avail += reserved - used;
+      if (avail > (2ull << 30)) budget = avail / 2 / std::max(1u, ctx->share);
+    }
   }
   if (const char* env = std::getenv("DL_BATCH_BYTES")) budget = std::max<uint64_t>(1ull << 20, strtoull(env, nullptr, 10));
   std::vector<Batch> batches;

@@ -1,5 +1,11 @@
 // The C ABI of libdynlight_hip.so (include/dynlight.h, include/dynlight_testing.h).  Nothing
 // unwinds across it: every entry point catches, stores the message and returns a status.
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <chrono>
 #include <cstring>
 #include <new>
 #include <thread>
@@ -44,10 +50,56 @@ int upload_trig(dl_ctx* ctx) {
 }  // namespace
 
 namespace {
+// CPUs this process may run on (its affinity mask), which can be fewer than the machine has
+unsigned usable_cpus() {
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, CPU_COUNT(&set));
+  return std::max(1u, std::thread::hardware_concurrency());
+}
+
+// A rough relative cost of baking one job, to order a batch: texels estimated from the mesh
+// areas and the density, times the rays a texel casts (one per light, 16 per area light, 32 more
+// per bounce light), plus the photon rays of the bounce lights.
+double job_cost(const dl_bake_input& in, const dl_bake_options& o) {
+  double area = 0.0;
+  for (uint32_t m = 0; m < in.mesh_count; ++m) {
+    const dl_mesh& mesh = in.meshes[m];
+    if (!mesh.positions || !mesh.indices) continue;
+    for (uint32_t t = 0; t + 2 < mesh.index_count; t += 3) {
+      const uint32_t a = mesh.indices[t], b = mesh.indices[t + 1], c = mesh.indices[t + 2];
+      if (a >= mesh.vertex_count || b >= mesh.vertex_count || c >= mesh.vertex_count) continue;
+      const float* p = mesh.positions;
+      const double e1[3] = {p[3 * b] - (double)p[3 * a], p[3 * b + 1] - (double)p[3 * a + 1], p[3 * b + 2] - (double)p[3 * a + 2]};
+      const double e2[3] = {p[3 * c] - (double)p[3 * a], p[3 * c + 1] - (double)p[3 * a + 1], p[3 * c + 2] - (double)p[3 * a + 2]};
+      const double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
+      const double a2 = std::sqrt(x * x + y * y + z * z) * 0.5;
+      if (std::isfinite(a2)) area += a2;
+    }
+  }
+  double rays = 1.0, photons = 0.0;
+  for (uint32_t l = 0; l < in.light_count; ++l) {
+    const dl_light& L = in.lights[l];
+    rays += (L.shape == DL_SHAPE_RECTANGLE || L.shape == DL_SHAPE_TUBE) ? 16.0 : 1.0;
+    if (L.illumination == DL_ILLUMINATION_SINGLE_BOUNCE) {
+      rays += 32.0;
+      photons += 6.0 * o.photon_cube_resolution * (double)o.photon_cube_resolution;
+    }
+  }
+  const double density = std::isfinite(o.pixel_density) ? o.pixel_density : 0.0;
+  return area * density * density * rays + photons;
+}
+
+// Returns the device pool's unused memory beyond 4 GiB to the driver (after every stream that
+// used it is idle).
+void trim_pool(int device) {
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) (void)hipMemPoolTrimTo(pool, 4ull << 30);
+}
+
 const dl::TrigHost& host_trig() {
   static std::once_flag once;
   static dl::TrigHost host;
-  std::call_once(once, [] { dl::build_trig_tables(host, std::max(1u, std::thread::hardware_concurrency())); });
+  std::call_once(once, [] { dl::build_trig_tables(host, usable_cpus()); });
   return host;
 }
 }  // namespace
@@ -64,10 +116,22 @@ int dl_ctx_create(int device, dl_ctx** out) {
     if (device < 0 || device >= count) return set_err(nullptr, DL_E_ARGUMENT, "device index out of range");
     dl_ctx* ctx = new dl_ctx();
     ctx->device = device;
-    ctx->host_threads = std::max(1u, std::thread::hardware_concurrency());
+    ctx->host_threads = usable_cpus();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
       delete ctx;
       return set_err(nullptr, DL_E_DEVICE, "creating the HIP stream failed");
+    }
+    // Bakes allocate from the device's default pool.  It keeps freed memory instead of returning
+    // it to the driver at every synchronization (releasing can wait for the whole device, which
+    // stalls bakes running side by side), and it reuses memory freed on another stream only once
+    // that free has completed, so no stream is made to wait for another.  Contexts trim the pool
+    // after a batch and when they are destroyed.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      int no = 0;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no);
     }
     dl::build_trig_tables(ctx->trig_host, ctx->host_threads);
     int st = upload_trig(ctx);
@@ -87,7 +151,15 @@ int dl_ctx_create(int device, dl_ctx** out) {
 
 void dl_ctx_destroy(dl_ctx* ctx) {
   if (!ctx) return;
+  for (dl_ctx* w : ctx->workers) dl_ctx_destroy(w);
+  ctx->workers.clear();
   (void)hipSetDevice(ctx->device);
+  ctx->cache.stream = ctx->stream;
+  ctx->cache.clear();
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  trim_pool(ctx->device);
+  if (ctx->staging.host) (void)hipHostFree(ctx->staging.host);
+  ctx->staging = dl::Staging{};
   ctx->trig_buf.release();
   if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -128,6 +200,108 @@ int dl_bake(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* opti
   } catch (...) {
     return set_err(ctx, DL_E_DEVICE, "unexpected failure");
   }
+}
+
+int dl_bake_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_options* options, uint32_t count, uint32_t streams,
+                  dl_baked_scene** out, int* status) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  if (count && (!inputs || !options || !out || !status)) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  if (streams == 0 || streams > 16) return set_err(ctx, DL_E_ARGUMENT, "streams must be 1 to 16");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    for (uint32_t i = 0; i < count; ++i) {
+      out[i] = nullptr;
+      status[i] = DL_E_CANCELLED;
+    }
+    const unsigned hw = usable_cpus();
+    while (ctx->workers.size() < streams) {
+      dl_ctx* w = new dl_ctx();
+      w->device = ctx->device;
+      if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete w;
+        return set_err(ctx, DL_E_DEVICE, "creating a worker stream failed");
+      }
+      w->trig = ctx->trig;  // read-only tables owned by ctx (the worker's trig_buf stays empty)
+      ctx->workers.push_back(w);
+    }
+    for (uint32_t k = 0; k < streams; ++k) {
+      dl_ctx* w = ctx->workers[k];
+      w->host_threads = std::max(1u, hw / streams);
+      w->share = streams;
+      w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
+      w->trace_block = ctx->trace_block;
+      w->burial_block = ctx->burial_block;
+      w->gather = ctx->gather;
+      w->gather.share = streams;
+      w->gather_mode = ctx->gather_mode;
+      w->counting = false;
+    }
+    // the host hands out jobs largest first (by a rough cost estimate), so that the last jobs
+    // running alone are small ones; each worker bakes on its own stream
+    std::vector<uint32_t> order(count);
+    {
+      std::vector<double> cost(count);
+      for (uint32_t i = 0; i < count; ++i) cost[i] = job_cost(inputs[i], options[i]);
+      for (uint32_t i = 0; i < count; ++i) order[i] = i;
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::string> errs(count);
+    ctx->batch_timings.assign(count, dl_timings{});
+    ctx->batch_worker.assign(count, 0);
+    ctx->batch_span.assign(2 * (size_t)count, 0.0);
+    const auto t_batch = std::chrono::steady_clock::now();
+    auto run = [&](dl_ctx* w, uint32_t wi) {
+      (void)hipSetDevice(w->device);
+      for (;;) {
+        const uint32_t n = next.fetch_add(1);
+        if (n >= count) break;
+        const uint32_t i = order[n];
+        ctx->batch_span[2 * (size_t)i] = dl::ms_since(t_batch);
+        ctx->batch_worker[i] = wi;
+        try {
+          status[i] = dl_bake_impl(w, &inputs[i], &options[i], nullptr, &out[i]);
+        } catch (const std::bad_alloc&) {
+          w->err = "host allocation failed";
+          status[i] = DL_E_OOM;
+        } catch (...) {
+          w->err = "unexpected failure";
+          status[i] = DL_E_DEVICE;
+        }
+        if (status[i] != DL_OK) errs[i] = "job " + std::to_string(i) + ": " + w->err;
+        ctx->batch_timings[i] = w->timings;
+        ctx->batch_span[2 * (size_t)i + 1] = dl::ms_since(t_batch);
+      }
+    };
+    std::vector<std::thread> ts;
+    for (uint32_t k = 1; k < streams; ++k) ts.emplace_back(run, ctx->workers[k], k);
+    run(ctx->workers[0], 0);
+    for (auto& t : ts) t.join();
+    trim_pool(ctx->device);
+    for (uint32_t i = 0; i < count; ++i)
+      if (status[i] != DL_OK) return set_err(ctx, status[i], errs[i]);
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_err(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker, double* span, uint32_t cap) {
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const uint32_t n = (uint32_t)std::min<size_t>(cap, ctx->batch_worker.size());
+  for (uint32_t i = 0; i < n; ++i) {
+    if (timings) timings[i] = ctx->batch_timings[i];
+    if (worker) worker[i] = ctx->batch_worker[i];
+    if (span) {
+      span[2 * (size_t)i] = ctx->batch_span[2 * (size_t)i];
+      span[2 * (size_t)i + 1] = ctx->batch_span[2 * (size_t)i + 1];
+    }
+  }
+  return (uint32_t)ctx->batch_worker.size();
 }
 
 void dl_baked_scene_free(dl_baked_scene* s) { delete s; }

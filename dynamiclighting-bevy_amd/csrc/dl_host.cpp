@@ -404,6 +404,56 @@ void pack_probe(const float* open6, uint32_t* words2) {
   for (int a = 0; a < 6; ++a) words2[a / 4] |= sat_u32(roundf(clampf_(open6[a], 0.0f, 1.0f) * 255.0f)) << (8 * (a % 4));
 }
 
+namespace {
+// Room for `bytes` in the staging buffer (allocated on first use), waiting for the stream when the
+// buffer is full; null when staging is not possible (too large, or no pinned memory).
+char* staging_slot(dl_ctx* ctx, size_t bytes, hipError_t* err) {
+  Staging& st = ctx->staging;
+  *err = hipSuccess;
+  if (bytes > kStagedMax) return nullptr;
+  if (!st.host) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, kStagingBytes, hipHostMallocDefault) != hipSuccess || !h) return nullptr;
+    st.host = static_cast<char*>(h);
+    st.cap = kStagingBytes;
+    st.off = 0;
+  }
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (st.off + need > st.cap) {  // every earlier staged copy must be done before the space is reused
+    *err = hipStreamSynchronize(ctx->stream);
+    if (*err != hipSuccess) return nullptr;
+    st.off = 0;
+  }
+  char* slot = st.host + st.off;
+  st.off += need;
+  return slot;
+}
+}  // namespace
+
+hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipSuccess;
+  if (char* slot = staging_slot(ctx, bytes, &e)) {
+    std::memcpy(slot, src, bytes);
+    return hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
+hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipSuccess;
+  char* slot = staging_slot(ctx, bytes, &e);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(slot ? slot : dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return e;
+  if (slot) std::memcpy(dst, slot, bytes);
+  ctx->staging.off = 0;  // the stream is idle: every staged copy is done
+  return hipSuccess;
+}
+
 uint32_t lightmap_size(float area, float density, uint32_t max_size) {
   float size = ceilf(sqrtf(density * density * area));
   if (size != size) size = 0.0f;

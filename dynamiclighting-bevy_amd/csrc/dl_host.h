@@ -7,6 +7,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/dynlight.h"
@@ -14,27 +15,105 @@
 
 namespace dl {
 
+// Device memory of one context's bakes.  Sizes are rounded up to classes of eight steps per
+// power of two (at most 12.5% more; 256 bytes at least), and a freed buffer is kept, up to `cap`
+// bytes in all, for a later allocation of its class: every use is on the context's one stream,
+// so stream order alone makes the reuse safe.  New memory is a stream-ordered allocation from
+// the device's pool.  hipMalloc/hipFree, and the pool when it has to grow, can block the host for
+// a long time, which stalls the stream that waits for this host thread and every bake running
+// beside it.
+struct DevCache {
+  hipStream_t stream = nullptr;
+  std::unordered_map<size_t, std::vector<void*>> lists;
+  size_t cached = 0, cap = 32ull << 30;
+  static size_t class_of(size_t bytes) {
+    if (bytes <= 256) return 256;
+    const int k = 63 - __builtin_clzll((unsigned long long)(bytes - 1));  // 2^k < bytes <= 2^(k+1)
+    const size_t step = (size_t)1 << (k - 3);
+    return ((size_t)1 << k) + ((bytes - ((size_t)1 << k) + step - 1) / step) * step;
+  }
+  hipError_t get(size_t bytes, void** p, size_t* cls) {
+    *cls = class_of(bytes);
+    auto it = lists.find(*cls);
+    if (it != lists.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      cached -= *cls;
+      return hipSuccess;
+    }
+    return hipMallocAsync(p, *cls, stream);
+  }
+  void put(void* p, size_t cls) {
+    if (cached + cls > cap) {
+      (void)hipFreeAsync(p, stream);
+      return;
+    }
+    lists[cls].push_back(p);
+    cached += cls;
+  }
+  void clear() {
+    for (auto& l : lists)
+      for (void* q : l.second) (void)hipFreeAsync(q, stream);
+    lists.clear();
+    cached = 0;
+  }
+};
+
+// While a CacheScope is alive, DBuf allocations of this thread come from its cache.
+inline thread_local DevCache* tls_cache = nullptr;
+struct CacheScope {
+  DevCache* prev;
+  explicit CacheScope(DevCache* c) : prev(tls_cache) { tls_cache = c; }
+  ~CacheScope() { tls_cache = prev; }
+  CacheScope(const CacheScope&) = delete;
+  CacheScope& operator=(const CacheScope&) = delete;
+};
+
 // Device allocation owned by the caller scope.
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevCache* cache = nullptr;  // non-null: taken from (and returned to) this cache
+  size_t cls = 0;
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (cache) cache->put(p, cls);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    cache = nullptr;
   }
   hipError_t alloc(size_t count) {
     release();
     n = count;
     if (count == 0) return hipSuccess;
+    cache = tls_cache;
+    if (cache) {
+      hipError_t e = cache->get(count * sizeof(T), (void**)&p, &cls);
+      if (e != hipSuccess) {
+        p = nullptr;
+        cache = nullptr;
+      }
+      return e;
+    }
     return hipMalloc((void**)&p, count * sizeof(T));
   }
 };
+
+// Pinned host memory through which a context's small copies go: copies from pageable memory are
+// staged by the runtime and cost far more per call.
+struct Staging {
+  char* host = nullptr;
+  size_t cap = 0, off = 0;
+};
+constexpr size_t kStagingBytes = 64ull << 20;  // staging buffer of a context
+constexpr size_t kStagedMax = 8ull << 20;      // larger copies use the pageable path
 
 // The exception tables of f32 sin/cos (see dl_trig.h), built from the host's libm.
 struct TrigHost {
@@ -70,6 +149,17 @@ struct dl_ctx {
   uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
   unsigned host_threads = 1;
   bool counting = false;
+  // batched bakes: worker contexts with their own streams (sharing this context's sin/cos
+  // tables); a worker plans for 1/share of the device: of half the free memory and of the
+  // resident blocks of persistent kernels
+  std::vector<dl_ctx*> workers;
+  uint32_t share = 1;
+  dl::DevCache cache;
+  dl::Staging staging;
+  // per job of the last batch: its timings, worker, and start/end in ms from the batch start
+  std::vector<dl_timings> batch_timings;
+  std::vector<uint32_t> batch_worker;
+  std::vector<double> batch_span;
 };
 
 struct dl_baked_mesh_out {
@@ -129,19 +219,23 @@ inline double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Host-to-device and device-to-host copies on the context's stream, through its pinned staging
+// buffer when they are small.  copy_to_device returns once the source may be reused;
+// copy_to_host returns once the data has arrived.
+hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes);
+hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 template <class T>
 hipError_t upload(dl_ctx* ctx, DBuf<T>& b, const std::vector<T>& h) {
   hipError_t e = b.alloc(h.size());
   if (e != hipSuccess || h.empty()) return e;
-  return hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  return copy_to_device(ctx, b.p, h.data(), h.size() * sizeof(T));
 }
 template <class T>
 hipError_t download(dl_ctx* ctx, std::vector<T>& h, const T* p, size_t n) {
   h.resize(n);
   if (n == 0) return hipSuccess;
-  hipError_t e = hipMemcpyAsync(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
-  if (e != hipSuccess) return e;
-  return hipStreamSynchronize(ctx->stream);
+  return copy_to_host(ctx, h.data(), p, n * sizeof(T));
 }
 template <class T>
 hipError_t zeroed(dl_ctx* ctx, DBuf<T>& b, size_t n) {

@@ -178,6 +178,7 @@ struct GatherLaunch {
   uint32_t min_waves = 0;       // 8: registers capped for 8 waves per SIMD; otherwise uncapped
   uint32_t refill = 56;         // idle lanes before finished lanes take new rays (1-64)
   uint64_t chunk_rays = 1ull << 28;  // rays per chunk of jobs (buffer size)
+  uint32_t share = 1;           // the persistent grid fills 1/share of the device (bakes side by side)
 };
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                               DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,

@@ -2087,7 +2087,7 @@ hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, 
   if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)gl.block, lds);
   if (e != hipSuccess) return e;
   if (gl.blocks_per_cu) per_cu = (int)gl.blocks_per_cu;
-  uint64_t grid = (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, per_cu);
+  uint64_t grid = std::max<uint64_t>(1, (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, per_cu) / std::max(1u, gl.share));
   const uint64_t useful = (max_rays + gl.block - 1) / gl.block;
   if (grid > useful) grid = useful;
   // the 32-bit queue may pass the ray count by one run per wave before every wave has drained

@@ -50,6 +50,9 @@ def load() -> C.CDLL:
     lib.dl_bake.restype = C.c_int
     lib.dl_bake.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options),
                             C.POINTER(abi.dl_progress), C.POINTER(P)]
+    lib.dl_bake_batch.restype = C.c_int
+    lib.dl_bake_batch.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
+                                  C.c_uint32, C.POINTER(P), C.POINTER(C.c_int)]
     lib.dl_baked_scene_free.restype = None
     lib.dl_baked_scene_free.argtypes = [P]
     lib.dl_last_timings.restype = C.c_int
@@ -72,6 +75,9 @@ def load() -> C.CDLL:
     lib.dl_set_counting.argtypes = [P, C.c_int]
     lib.dl_set_launch.restype = C.c_int
     lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
+    lib.dl_test_batch_trace.restype = C.c_uint32
+    lib.dl_test_batch_trace.argtypes = [P, C.POINTER(abi.dl_timings), C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                        C.c_uint32]
     lib.dl_set_gather.restype = C.c_int
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
     _lib = lib
@@ -113,6 +119,45 @@ class Context:
             return abi.read_scene(self.lib, "dl_", h)
         finally:
             self.lib.dl_baked_scene_free(h)
+
+    def bake_batch(self, inputs: list[BakeInput], options: list[abi.dl_bake_options] | abi.dl_bake_options | None = None,
+                   streams: int = 4, read: bool | str = True) -> list | None:
+        """Bakes independent jobs on `streams` HIP streams at once (dl_bake_batch); each result
+        equals bake() of the same job alone.  read=True returns BakedScenes, read="stats" only
+        each job's stats, read=False nothing (the results are freed unread)."""
+        n = len(inputs)
+        if options is None or isinstance(options, abi.dl_bake_options):
+            options = [options if options is not None else default_options()] * n
+        ins = (abi.dl_bake_input * max(1, n))(*[i.struct for i in inputs])
+        ops = (abi.dl_bake_options * max(1, n))(*options)
+        outs = (C.c_void_p * max(1, n))()
+        status = (C.c_int * max(1, n))()
+        st = self.lib.dl_bake_batch(self.handle, ins, ops, n, streams, outs, status)
+        try:
+            if st != 0:
+                raise BakeError(st, self.error())
+            if read == "stats":
+                out = []
+                for i in range(n):
+                    st = abi.dl_bake_stats()
+                    self.lib.dl_scene_stats(outs[i], C.byref(st))
+                    out.append({name: getattr(st, name) for name, _ in st._fields_})
+                return out
+            return [abi.read_scene(self.lib, "dl_", outs[i]) for i in range(n)] if read else None
+        finally:
+            for i in range(n):
+                if outs[i]:
+                    self.lib.dl_baked_scene_free(outs[i])
+
+    def batch_trace(self) -> list[dict]:
+        """Per job of the last bake_batch: worker, start/end ms from the batch start, timings."""
+        n = self.lib.dl_test_batch_trace(self.handle, None, None, None, 0)
+        tm = (abi.dl_timings * max(1, n))()
+        wk = (C.c_uint32 * max(1, n))()
+        sp = (C.c_double * max(2, 2 * n))()
+        self.lib.dl_test_batch_trace(self.handle, tm, wk, sp, n)
+        return [{"worker": wk[i], "start_ms": sp[2 * i], "end_ms": sp[2 * i + 1],
+                 **{name: getattr(tm[i], name) for name, _ in tm[i]._fields_}} for i in range(n)]
 
     def timings(self) -> dict:
         t = abi.dl_timings()

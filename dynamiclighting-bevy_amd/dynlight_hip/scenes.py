@@ -11,6 +11,9 @@ fill, `point_light`, `area_light`, `sun_light`, `sky_light`).  The procedural sc
 * ``outdoor``       -- 1024x512 fBm heightfield (~1M triangles) plus 2000 box buildings, a
                         soft sun with one bounce, 64 rectangle/tube area lights, a baked sky,
                         4 texels/m (terrain capped at 2048^2).
+* ``batch``         -- 512 independent small interiors (grids of 6x3x6 m rooms with boxes):
+                        log-uniform 500-5e4 triangles, 1-64 point/spot/area lights, 1e3-2e6
+                        texels, bounce on odd jobs.
 
 Randomness is PCG32 seeded through SplitMix64, restated here so scenes are identical on every
 machine.
@@ -63,8 +66,33 @@ class Pcg32:
     def below(self, n: int) -> int:
         return self.next_u32() % n
 
+    def u32s(self, count: int) -> np.ndarray:
+        """The next `count` outputs of next_u32, computed at once: the state after k steps is an
+        affine map of the current one, whose coefficients come from a prefix composition."""
+        if count <= 0:
+            return np.zeros(0, np.uint32)
+        with np.errstate(over="ignore"):
+            a = np.full(count, 6364136223846793005, np.uint64)
+            c = np.full(count, self.inc, np.uint64)
+            shift = 1
+            while shift < count:  # (a[k], c[k]) becomes the map over steps 0..k
+                na, nc = a.copy(), c.copy()
+                na[shift:] = a[shift:] * a[:-shift]
+                nc[shift:] = a[shift:] * c[:-shift] + c[shift:]
+                a, c, shift = na, nc, shift * 2
+            s0 = np.uint64(self.state)
+            old = np.empty(count, np.uint64)
+            old[0] = s0
+            old[1:] = a[:-1] * s0 + c[:-1]
+            self.state = int(a[-1] * s0 + c[-1])
+        x = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)) & np.uint64(0xFFFFFFFF)
+        rot = old >> np.uint64(59)
+        out = (x >> rot) | (x << ((np.uint64(32) - rot) & np.uint64(31)))
+        return (out & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
     def uniforms(self, count: int, lo=0.0, hi=1.0) -> np.ndarray:
-        return np.array([self.uniform(lo, hi) for _ in range(count)], dtype=np.float64)
+        """`count` successive uniform(lo, hi) draws."""
+        return lo + (hi - lo) * ((self.u32s(count) >> 8).astype(np.float64) / 16777216.0)
 
 
 # ---- reference test helpers (src/bake/tests/mod.rs:23-137) -----------------------------------
@@ -384,6 +412,100 @@ def outdoor(seed: int = 3, nx: int = 1024, nz: int = 512, buildings: int = 2000,
     opts.sky_probe_spacing = 1.0
     opts.dynamic_geometry = DYNAMIC_ANGULAR
     return inp, opts
+
+
+# ---- batch: many independent small scenes of uneven size -------------------------------------
+def batch_job(j: int, seed: int = 4):
+    """Returns (BakeInput, options) of job j of the batch workload (job seed splitmix64(seed ^ j)).
+
+    A job is a small interior in the style of ``interior``: a row-major grid of R = max(1,
+    round(T / 3000)) rooms of 6x3x6 m with 1x2 m doors between neighbours, filled with boxes and
+    pillars (30%, full height) to a log-uniform triangle count T in [500, 5e4].  Lights: a
+    log-uniform count in [1, 64] placed in random rooms, 40% point, 30% spot (baked as points),
+    30% rectangle/tube area lights (0.5-2 m), range U(4, 12), single bounce for odd j.  The
+    density is solved from a log-uniform texel target in [1e3, 2e6] with a 0.64 atlas fill:
+    density = sqrt(target / (0.64 * total area))."""
+    rng = Pcg32.seeded(splitmix64((seed ^ j) & MASK64))
+    tris = int(round(math.exp(rng.uniform(math.log(500.0), math.log(5e4)))))
+    nlights = min(64, int(math.exp(rng.uniform(0.0, math.log(65.0)))))
+    texels = math.exp(rng.uniform(math.log(1e3), math.log(2e6)))
+    W, H, D = 6.0, 3.0, 6.0
+    rooms = max(1, int(round(tris / 3000.0)))
+    gx = int(math.ceil(math.sqrt(rooms)))
+    origin = [((r % gx) * W, (r // gx) * D) for r in range(rooms)]
+    occupied = set((r % gx, r // gx) for r in range(rooms))
+    positions, normals, indices, uvs = [], [], [], []
+    area = 0.0
+    for r in range(rooms):
+        cx, cz = r % gx, r // gx
+        doors = {"-x": (cx - 1, cz) in occupied, "+x": (cx + 1, cz) in occupied,
+                 "-z": (cx, cz - 1) in occupied, "+z": (cx, cz + 1) in occupied}
+        p, n, idx, uv = _room_shell(origin[r][0], origin[r][1], W, H, D, doors)
+        positions.append(p)
+        normals.append(n)
+        indices.append(idx)
+        uvs.append(uv)
+        t = p[idx.reshape(-1, 3)].astype(np.float64)
+        area += float(np.sum(np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)) / 2)
+    shell_tris = sum(len(x) for x in indices) // 3
+    boxes = max(1, (tris - shell_tris) // 12)
+    u = rng.uniforms(boxes * 9).reshape(boxes, 9)
+    room = np.minimum((u[:, 0] * rooms).astype(np.int64), rooms - 1)
+    pillar = u[:, 1] < 0.3
+    ext = np.where(pillar[:, None], np.stack([0.2 + 0.8 * u[:, 2], np.full(boxes, H), 0.2 + 0.8 * u[:, 4]], 1),
+                   0.2 + 2.8 * u[:, 2:5])
+    ox = np.array([o[0] for o in origin])[room]
+    oz = np.array([o[1] for o in origin])[room]
+    cxs = ox + ext[:, 0] / 2 + u[:, 5] * (W - ext[:, 0])
+    czs = oz + ext[:, 2] / 2 + u[:, 6] * (D - ext[:, 2])
+    mn = np.stack([cxs - ext[:, 0] / 2, np.zeros(boxes), czs - ext[:, 2] / 2], 1).astype(np.float32)
+    mx = np.stack([cxs + ext[:, 0] / 2, ext[:, 1], czs + ext[:, 2] / 2], 1).astype(np.float32)
+    albedo = np.concatenate([np.full((rooms, 3), 0.8, np.float32),
+                             (0.1 + 0.8 * rng.uniforms(boxes * 3).reshape(boxes, 3)).astype(np.float32)])
+    local, tnrm, tidx, tuv = _TEMPLATE
+    box_pos = ((mn + mx)[:, None, :] * np.float32(0.5) + local[None] * (mx - mn)[:, None, :] * np.float32(0.5))
+    area += float(np.sum(2 * (ext[:, 0] * ext[:, 1] + ext[:, 1] * ext[:, 2] + ext[:, 0] * ext[:, 2])))
+    vcount = np.concatenate([[0], np.cumsum([len(x) for x in positions]),
+                             sum(len(x) for x in positions) + len(local) * np.arange(1, boxes + 1)])
+    icount = np.concatenate([[0], np.cumsum([len(x) for x in indices]),
+                             sum(len(x) for x in indices) + len(tidx) * np.arange(1, boxes + 1)])
+    positions.append(box_pos.reshape(-1, 3).astype(np.float32))
+    normals.append(np.tile(tnrm, (boxes, 1)))
+    indices.append(np.tile(tidx, boxes))
+    uvs.append(np.tile(tuv, (boxes, 1)))
+    ids = np.arange(1, rooms + boxes + 1, dtype=np.uint64)
+    light_list = []
+    for k in range(nlights):
+        kind = rng.uniform()
+        r = rng.below(rooms)
+        p = (origin[r][0] + rng.uniform(0.1, W - 0.1), rng.uniform(0.1, H - 0.1), origin[r][1] + rng.uniform(0.1, D - 0.1))
+        radius = rng.uniform(4.0, 12.0)
+        if kind < 0.7:
+            light = point_light(1000 + k, p, radius)
+        else:
+            length = rng.uniform(0.5, 2.0)
+            if rng.uniform() < 0.5:
+                light = area_light(1000 + k, p, radius, quat_from_rotation_x(-math.pi / 2), (length, rng.uniform(0.5, 2.0)))
+            else:
+                yaw = rng.uniform(0.0, math.pi)
+                light = area_light(1000 + k, p, radius, (0.0, math.sin(yaw / 2), 0.0, math.cos(yaw / 2)), (length, 0.0))
+        if j % 2 == 1:
+            light.illumination = ILLUMINATION_SINGLE_BOUNCE
+        light_list.append(light)
+    inp = BakeInput.from_arrays(
+        ids=ids, positions=np.concatenate(positions), indices=np.concatenate(indices), uv1=np.concatenate(uvs),
+        normals=np.concatenate(normals), vertex_offsets=vcount.astype(np.int64), index_offsets=icount.astype(np.int64),
+        albedo=albedo, content_hash=np.array([splitmix64(int(i)) for i in ids], np.uint64), lights=light_list)
+    opts = default_options()
+    opts.pixel_density = float(math.sqrt(texels / (0.64 * area)))
+    opts.dynamic_geometry = DYNAMIC_ANGULAR
+    return inp, opts
+
+
+def batch(count: int = 512, seed: int = 4):
+    """The batch workload: `count` independent jobs (lists of BakeInput and options)."""
+    jobs = [batch_job(j, seed) for j in range(count)]
+    return [j[0] for j in jobs], [j[1] for j in jobs]
 
 
 def mesh_sample_mask(ids: np.ndarray, modulus: int) -> np.ndarray:

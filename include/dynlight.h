@@ -215,6 +215,14 @@ int dl_bake(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* opti
             dl_progress* progress, dl_baked_scene** out);
 void dl_baked_scene_free(dl_baked_scene* scene);
 
+/* Many independent bakes (no counterpart in the reference, which bakes one scene per call):
+ * job i bakes inputs[i] with options[i] into out[i] (status[i] is its dl_bake result).  Jobs are
+ * handed out in order to `streams` worker contexts (1-16), each baking on its own HIP stream, so
+ * small jobs overlap on the device; each result equals dl_bake of the same job alone.  Returns
+ * DL_OK, or the first failing job's code with its message in dl_last_error. */
+int dl_bake_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_options* options, uint32_t count,
+                  uint32_t streams, dl_baked_scene** out, int* status);
+
 /* ---- BakedScene accessors (src/bake/format/types.rs:13-37) --------------------------------- */
 void dl_scene_settings(const dl_baked_scene* scene, dl_baked_settings* out);
 void dl_scene_stats(const dl_baked_scene* scene, dl_bake_stats* out);

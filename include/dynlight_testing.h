@@ -45,6 +45,10 @@ int dl_set_counting(dl_ctx* ctx, int enable);
 /* Launch shape of the traversal kernels (0 keeps the current value): threads per block of the
  * shadow-ray and burial kernels. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
+/* Per job of the context's last dl_bake_batch (up to cap jobs): its dl_timings, the worker that
+ * baked it, and its start and end in ms from the start of the batch (two doubles per job).  Any
+ * output may be null.  Returns the job count of that batch. */
+uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker, double* span, uint32_t cap);
 
 /* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
  * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
