@@ -48,7 +48,7 @@ uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOU
 }  // namespace
 
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
-                 dl_baked_scene** out) {
+                 dl_baked_scene** out, Resident* resident) {
   auto t_start = std::chrono::steady_clock::now();
   ctx->cache.stream = ctx->stream;
   const CacheScope cache_scope(&ctx->cache);  // before every device buffer of the bake
@@ -64,16 +64,19 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     if (progress) progress->done = progress->done + n;
   };
   stage(0, 0);
+  // resident geometry: `use_res` bakes the resident scene, `fill_res` fills it from this bake
+  const bool use_res = resident && resident->ready, fill_res = resident && !resident->ready;
 
   // ---- validation and ordering (baker.rs:31-50, 205-233) -------------------------------------
   int st = check_options(ctx, opt);
   if (st) return st;
-  if (input->mesh_count && !input->meshes) { ctx->err = "null mesh array"; return DL_E_ARGUMENT; }
+  if (!use_res && input->mesh_count && !input->meshes) { ctx->err = "null mesh array"; return DL_E_ARGUMENT; }
   if (input->light_count && !input->lights) { ctx->err = "null light array"; return DL_E_ARGUMENT; }
   for (uint32_t i = 0; i < input->light_count; ++i)
     if (input->lights[i].shape < 0 || input->lights[i].shape > 4) { ctx->err = "unknown light shape"; return DL_E_ARGUMENT; }
   std::vector<const dl_mesh*> meshes;
-  for (uint32_t i = 0; i < input->mesh_count; ++i) meshes.push_back(&input->meshes[i]);
+  if (!use_res)
+    for (uint32_t i = 0; i < input->mesh_count; ++i) meshes.push_back(&input->meshes[i]);
   std::stable_sort(meshes.begin(), meshes.end(), [](const dl_mesh* a, const dl_mesh* b) { return a->id < b->id; });
   for (size_t i = 1; i < meshes.size(); ++i)
     if (meshes[i - 1]->id == meshes[i]->id) { ctx->err = "two meshes share BakeId " + hex_id(meshes[i]->id); return DL_E_DUPLICATE_ID; }
@@ -125,18 +128,21 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   for (auto* l : lights) alpha_lights |= l->transparency != DL_TRANSPARENCY_DISABLED;
 
   // ---- occluders (occluder.rs:80-115) with their normals and albedo --------------------------
-  std::vector<float> occ9, occ_nrm, occ_alb;
+  std::vector<float> occ9_local, occ_nrm, occ_alb;
+  std::vector<float>& occ9 = use_res ? resident->occ9 : occ9_local;
   std::vector<uint32_t> occ_flags;
   // alpha-tested occluders (occluder.rs:147-166): texture per occluder, corner UV0, opaque bits
   std::vector<uint32_t> occ_tex, tex_info, tex_bits;
   std::vector<float> occ_uv0;
-  bool any_alpha = false;
-  for (auto* m : meshes)
-    any_alpha |= alpha_lights && m->occludes && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0;
+  bool mesh_alpha = use_res && resident->has_alpha;
+  for (auto* m : meshes) mesh_alpha |= m->occludes && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0;
+  // alpha data is used when a light tests alpha; a resident scene keeps it for later lights
+  const bool any_alpha = alpha_lights && mesh_alpha;
+  const bool build_alpha = any_alpha || (fill_res && mesh_alpha);
   for (auto* m : meshes) {
     if (!m->occludes) continue;
     uint32_t tex = 0xffffffffu;
-    if (any_alpha && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0) {
+    if (build_alpha && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0) {
       const dl_alpha_texture* a = m->alpha;
       tex = (uint32_t)(tex_info.size() / 4);
       const uint64_t texels = (uint64_t)a->width * a->height;
@@ -164,25 +170,32 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       }
       occ_flags.push_back(has_n ? 1u : 0u);
       for (int a = 0; a < 3; ++a) occ_alb.push_back(m->albedo[a]);
-      if (any_alpha) {
+      if (build_alpha) {
         occ_tex.push_back(tex);
         for (int k = 0; k < 3; ++k)
           for (int a = 0; a < 2; ++a) occ_uv0.push_back(tex != 0xffffffffu ? m->uv0[2 * (size_t)m->indices[t * 3 + k] + a] : 0.0f);
       }
     }
   }
-  const uint32_t nocc = (uint32_t)occ_flags.size();
+  const uint32_t nocc = use_res ? resident->nocc : (uint32_t)occ_flags.size();
   Bvh4 bvh;
-  build_bvh4(occ9.data(), nocc, ctx->host_threads, bvh);
-  ctx->timings.bvh_nodes = bvh.nodes.size();
+  if (!use_res) build_bvh4(occ9.data(), nocc, ctx->host_threads, bvh);
+  ctx->timings.bvh_nodes = use_res ? resident->nodes : bvh.nodes.size();
   ctx->timings.occluder_triangles = nocc;
 
   // ---- prepared meshes and receiver triangles (mesh_builder.rs:21-80) -----------------------
-  const uint32_t nm = (uint32_t)meshes.size();
-  std::vector<uint32_t> res(nm), first_tri(nm + 1, 0);
-  std::vector<uint64_t> mid(nm), owned_off(nm + 1, 0);
-  uint64_t ntri = 0;
-  for (uint32_t i = 0; i < nm; ++i) {
+  std::vector<MeshMeta> meta_local;
+  std::vector<MeshMeta>& meta = use_res ? resident->meshes : meta_local;
+  for (auto* m : meshes) meta.push_back(MeshMeta{m->id, m->content_hash, m->occludes});
+  const uint32_t nm = (uint32_t)meta.size();
+  std::vector<uint32_t> res_local(use_res ? 0 : nm), first_local(use_res ? 0 : nm + 1, 0);
+  std::vector<uint64_t> mid_local(use_res ? 0 : nm), owned_local(use_res ? 0 : nm + 1, 0);
+  std::vector<uint32_t>& res = use_res ? resident->res : res_local;
+  std::vector<uint32_t>& first_tri = use_res ? resident->first_tri : first_local;
+  std::vector<uint64_t>& mid = use_res ? resident->mid : mid_local;
+  std::vector<uint64_t>& owned_off = use_res ? resident->owned_off : owned_local;
+  uint64_t ntri = use_res ? resident->ntri : 0;
+  for (uint32_t i = 0; i < (use_res ? 0u : nm); ++i) {
     const dl_mesh* m = meshes[i];
     float area = -0.0f;
     for (uint32_t t = 0; t < m->index_count / 3; ++t) {
@@ -198,12 +211,13 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     ntri += m->index_count / 3;
     owned_off[i + 1] = owned_off[i] + ((uint64_t)res[i] * res[i] + 31) / 32;
   }
-  first_tri[nm] = (uint32_t)ntri;
+  if (!use_res) first_tri[nm] = (uint32_t)ntri;
   if (ntri >= (1ull << 31)) { ctx->err = "too many triangles"; return DL_E_ARGUMENT; }
   const uint32_t nt = (uint32_t)ntri;
-  std::vector<float> hv((size_t)nt * 9), hn((size_t)nt * 9), huv((size_t)nt * 6);
-  std::vector<uint32_t> hmesh(nt), hlocal(nt);
-  for (uint32_t i = 0; i < nm; ++i) {
+  const size_t nth = use_res ? 0 : nt;  // the host copies of the receiver triangles
+  std::vector<float> hv(nth * 9), hn(nth * 9), huv(nth * 6);
+  std::vector<uint32_t> hmesh(nth), hlocal(nth);
+  for (uint32_t i = 0; i < (use_res ? 0u : nm); ++i) {
     const dl_mesh* m = meshes[i];
     bool flat = !m->normals || m->normal_count == 0;
     for (uint32_t t = 0; t < m->index_count / 3; ++t) {
@@ -236,85 +250,89 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
 
   // ---- scene-level device state --------------------------------------------------------------
+  // Geometry buffers live in the resident scene when there is one (filled here on its first
+  // bake), otherwise in this bake; per-bake buffers are always this bake's.
   auto t_up = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
-  DBuf<CNode> d_nodes;
-  DBuf<float> d_occ, d_occ_nrm, d_occ_alb;
-  DBuf<uint32_t> d_overflow, d_occ_flags;
-  HIP_TRY(upload(ctx, d_nodes, bvh.nodes));
-  HIP_TRY(upload(ctx, d_occ, bvh.tris));
+  Resident local_geo;
+  Resident& geo = resident ? *resident : local_geo;
+  DBuf<uint32_t> d_overflow;
+  if (!use_res) {
+    HIP_TRY(upload(ctx, geo.d_nodes, bvh.nodes));
+    HIP_TRY(upload(ctx, geo.d_occ, bvh.tris));
+    HIP_TRY(upload(ctx, geo.d_occ_nrm, occ_nrm));
+    HIP_TRY(upload(ctx, geo.d_occ_alb, occ_alb));
+    HIP_TRY(upload(ctx, geo.d_occ_flags, occ_flags));
+    if (build_alpha) {
+      HIP_TRY(upload(ctx, geo.d_occ_tex, occ_tex));
+      HIP_TRY(upload(ctx, geo.d_occ_uv0, occ_uv0));
+      HIP_TRY(upload(ctx, geo.d_tex_info, tex_info));
+      HIP_TRY(upload(ctx, geo.d_tex_bits, tex_bits));
+    }
+    HIP_TRY(upload(ctx, geo.d_v, hv));
+    HIP_TRY(upload(ctx, geo.d_n, hn));
+    HIP_TRY(upload(ctx, geo.d_uv, huv));
+    HIP_TRY(upload(ctx, geo.d_mesh, hmesh));
+    HIP_TRY(upload(ctx, geo.d_local, hlocal));
+    HIP_TRY(upload(ctx, geo.d_res, res));
+    HIP_TRY(upload(ctx, geo.d_mid, mid));
+    HIP_TRY(upload(ctx, geo.d_owned_off, owned_off));
+    geo.nocc = nocc;
+    geo.depth = bvh.depth;
+    geo.nodes = bvh.nodes.size();
+  }
   HIP_TRY(zeroed(ctx, d_overflow, 1));
   DevScene sc;
-  sc.nodes = d_nodes.p;
-  sc.tris = reinterpret_cast<const float4*>(d_occ.p);
+  sc.nodes = geo.d_nodes.p;
+  sc.tris = reinterpret_cast<const float4*>(geo.d_occ.p);
   sc.ntris = nocc;
-  sc.stack = 3 * std::max(1u, bvh.depth) + 1;
-  sc.stack16 = bvh.nodes.size() < 65536 ? 1u : 0u;
+  sc.stack = 3 * std::max(1u, geo.depth) + 1;
+  sc.stack16 = geo.nodes < 65536 ? 1u : 0u;
   sc.overflow = d_overflow.p;
   sc.counters = nullptr;
-  DBuf<uint32_t> d_occ_tex, d_tex_info, d_tex_bits;
-  DBuf<float> d_occ_uv0;
   sc.occ_tex = nullptr;
   sc.occ_uv0 = nullptr;
   sc.tex = nullptr;
   sc.tex_bits = nullptr;
   if (any_alpha) {
-    HIP_TRY(upload(ctx, d_occ_tex, occ_tex));
-    HIP_TRY(upload(ctx, d_occ_uv0, occ_uv0));
-    HIP_TRY(upload(ctx, d_tex_info, tex_info));
-    HIP_TRY(upload(ctx, d_tex_bits, tex_bits));
-    sc.occ_tex = d_occ_tex.p;
-    sc.occ_uv0 = d_occ_uv0.p;
-    sc.tex = d_tex_info.p;
-    sc.tex_bits = d_tex_bits.p;
+    sc.occ_tex = geo.d_occ_tex.p;
+    sc.occ_uv0 = geo.d_occ_uv0.p;
+    sc.tex = geo.d_tex_info.p;
+    sc.tex_bits = geo.d_tex_bits.p;
   }
   DevOcc occ{nullptr, nullptr, nullptr};
-  if (bounce_in_scene) {
-    HIP_TRY(upload(ctx, d_occ_nrm, occ_nrm));
-    HIP_TRY(upload(ctx, d_occ_alb, occ_alb));
-    HIP_TRY(upload(ctx, d_occ_flags, occ_flags));
-    occ = DevOcc{d_occ_nrm.p, d_occ_alb.p, d_occ_flags.p};
-  }
+  if (bounce_in_scene) occ = DevOcc{geo.d_occ_nrm.p, geo.d_occ_alb.p, geo.d_occ_flags.p};
   DBuf<DevLight> d_lights;
   HIP_TRY(upload(ctx, d_lights, hlights));
   DBuf<int32_t> d_comp, d_cube_of;
   HIP_TRY(upload(ctx, d_comp, comp_of_slot));
   HIP_TRY(upload(ctx, d_cube_of, cube_of_slot));
-  DBuf<float> d_v, d_n, d_uv, d_face;
-  DBuf<uint32_t> d_mesh, d_local, d_crop;
-  HIP_TRY(upload(ctx, d_v, hv));
-  HIP_TRY(upload(ctx, d_n, hn));
-  HIP_TRY(upload(ctx, d_uv, huv));
-  HIP_TRY(upload(ctx, d_mesh, hmesh));
-  HIP_TRY(upload(ctx, d_local, hlocal));
+  DBuf<float> d_face;
+  DBuf<uint32_t> d_crop;
   HIP_TRY(d_face.alloc((size_t)nt * 3 + 3));
   HIP_TRY(d_crop.alloc((size_t)nt * 4 + 4));
   DevTris trall;
   trall.n = nt;
-  trall.v = d_v.p;
-  trall.nrm = d_n.p;
-  trall.uv = d_uv.p;
-  trall.mesh = d_mesh.p;
-  trall.local = d_local.p;
+  trall.v = geo.d_v.p;
+  trall.nrm = geo.d_n.p;
+  trall.uv = geo.d_uv.p;
+  trall.mesh = geo.d_mesh.p;
+  trall.local = geo.d_local.p;
   trall.face = d_face.p;
   trall.crop = d_crop.p;
-  DBuf<uint32_t> d_res, d_owned, d_visited;
-  DBuf<uint64_t> d_mid, d_owned_off;
+  DBuf<uint32_t> d_owned, d_visited;
   DBuf<unsigned long long> d_mrays, d_msamples, d_mbrays;
-  HIP_TRY(upload(ctx, d_res, res));
-  HIP_TRY(upload(ctx, d_mid, mid));
-  HIP_TRY(upload(ctx, d_owned_off, owned_off));
-  HIP_TRY(zeroed(ctx, d_owned, owned_off[nm] + 1));
+  if (!use_res) HIP_TRY(zeroed(ctx, d_owned, owned_off[nm] + 1));  // owned texels: only to find the texels
   HIP_TRY(zeroed(ctx, d_visited, owned_off[nm] + 1));
   HIP_TRY(zeroed(ctx, d_mrays, nm + 1));
   HIP_TRY(zeroed(ctx, d_msamples, nm + 1));
   HIP_TRY(zeroed(ctx, d_mbrays, nm + 1));
   DevMeshes ms;
   ms.n = nm;
-  ms.res = d_res.p;
-  ms.id = d_mid.p;
-  ms.owned_off = d_owned_off.p;
-  ms.visited_off = d_owned_off.p;
+  ms.res = geo.d_res.p;
+  ms.id = geo.d_mid.p;
+  ms.owned_off = geo.d_owned_off.p;
+  ms.visited_off = geo.d_owned_off.p;
   ms.owned = d_owned.p;
   ms.visited = d_visited.p;
   ms.rays = d_mrays.p;
@@ -537,52 +555,77 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     DevEntries ent{nbe, d_bel.p, d_ber.p, d_bet.p, d_bes.p};
 
     // ---- texels: owned, traced, burial, compaction (mesh_texels.rs:41-108) -------------------
-    DBuf<unsigned long long> d_cnt;
-    HIP_TRY(zeroed(ctx, d_cnt, (size_t)nbt + 1));
-    HIP_TRY(launch_owned(s, tr, ms, d_bts.p, b_trace_start[nbt]));
-    DevTexels none{};
-    const uint64_t* tc_b = d_tc.p + t0;
-    HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 0, d_cnt.p, nullptr, none));
-    DBuf<uint64_t> d_tex0_start;
-    HIP_TRY(d_tex0_start.alloc((size_t)nbt + 1));
-    HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex0_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
-    uint64_t ntex0 = 0;
-    HIP_TRY(hipMemcpyAsync(&ntex0, d_tex0_start.p + nbt, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    DBuf<uint32_t> d_t0xy, d_t0tri;
-    DBuf<float4> d_t0pos, d_t0nrm;
-    HIP_TRY(d_t0xy.alloc(ntex0 + 1));
-    HIP_TRY(d_t0tri.alloc(ntex0 + 1));
-    HIP_TRY(d_t0pos.alloc(ntex0 + 1));
-    HIP_TRY(d_t0nrm.alloc(ntex0 + 1));
-    DevTexels tx0{d_t0xy.p, d_t0pos.p, d_t0nrm.p, d_t0tri.p};
-    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
-    HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 1, d_cnt.p, d_tex0_start.p, tx0));
-    DBuf<uint8_t> d_buried;
-    HIP_TRY(d_buried.alloc(ntex0 + 1));
-    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
-    HIP_TRY(launch_burial(s, sc, tr, ms, tx0, ntex0, opt.seed, ctx->trig, d_buried.p, d_cnt.p, ctx->burial_block));
-    DBuf<uint64_t> d_tex_start;
-    HIP_TRY(d_tex_start.alloc((size_t)nbt + 1));
-    HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+    // A resident scene holds every triangle's texels already: the batch's are a contiguous run.
     std::vector<uint64_t> h_tex_start;
-    HIP_TRY(download(ctx, h_tex_start, d_tex_start.p, (size_t)nbt + 1));
-    const uint64_t ntex = h_tex_start[nbt];
+    DBuf<uint64_t> d_tex_start;
     DBuf<uint32_t> d_txy, d_ttri;
     DBuf<float4> d_tpos, d_tnrm;
-    HIP_TRY(d_txy.alloc(ntex + 1));
-    HIP_TRY(d_ttri.alloc(ntex + 1));
-    HIP_TRY(d_tpos.alloc(ntex + 1));
-    HIP_TRY(d_tnrm.alloc(ntex + 1));
-    DevTexels tx{d_txy.p, d_tpos.p, d_tnrm.p, d_ttri.p};
-    HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
-    HIP_TRY(launch_compact(s, tx0, ntex0, d_buried.p, d_tex_start.p, d_cnt.p, tx));
-    HIP_TRY(hipStreamSynchronize(s));
-    d_t0xy.release();
-    d_t0tri.release();
-    d_t0pos.release();
-    d_t0nrm.release();
-    d_buried.release();
+    DevTexels tx{};
+    uint64_t ntex = 0;
+    if (use_res) {
+      const uint64_t base = resident->tex_start[t0];
+      h_tex_start.resize((size_t)nbt + 1);
+      for (uint32_t t = 0; t <= nbt; ++t) h_tex_start[t] = resident->tex_start[t0 + t] - base;
+      ntex = h_tex_start[nbt];
+      HIP_TRY(upload(ctx, d_tex_start, h_tex_start));
+      HIP_TRY(d_ttri.alloc(ntex + 1));
+      HIP_TRY(launch_add_u32(s, resident->t_tri.p + base, d_ttri.p, ntex, 0u - t0));  // scene-wide -> batch triangle
+      tx = DevTexels{resident->t_xy.p + base, resident->t_pos.p + base, resident->t_nrm.p + base, d_ttri.p};
+    } else {
+      DBuf<unsigned long long> d_cnt;
+      HIP_TRY(zeroed(ctx, d_cnt, (size_t)nbt + 1));
+      HIP_TRY(launch_owned(s, tr, ms, d_bts.p, b_trace_start[nbt]));
+      DevTexels none{};
+      const uint64_t* tc_b = d_tc.p + t0;
+      HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 0, d_cnt.p, nullptr, none));
+      DBuf<uint64_t> d_tex0_start;
+      HIP_TRY(d_tex0_start.alloc((size_t)nbt + 1));
+      HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex0_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+      uint64_t ntex0 = 0;
+      HIP_TRY(hipMemcpyAsync(&ntex0, d_tex0_start.p + nbt, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      DBuf<uint32_t> d_t0xy, d_t0tri;
+      DBuf<float4> d_t0pos, d_t0nrm;
+      HIP_TRY(d_t0xy.alloc(ntex0 + 1));
+      HIP_TRY(d_t0tri.alloc(ntex0 + 1));
+      HIP_TRY(d_t0pos.alloc(ntex0 + 1));
+      HIP_TRY(d_t0nrm.alloc(ntex0 + 1));
+      DevTexels tx0{d_t0xy.p, d_t0pos.p, d_t0nrm.p, d_t0tri.p};
+      HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+      HIP_TRY(launch_texels(s, tr, ms, tc_b, d_bcs.p, b_cand_start[nbt], 1, d_cnt.p, d_tex0_start.p, tx0));
+      DBuf<uint8_t> d_buried;
+      HIP_TRY(d_buried.alloc(ntex0 + 1));
+      HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+      HIP_TRY(launch_burial(s, sc, tr, ms, tx0, ntex0, opt.seed, ctx->trig, d_buried.p, d_cnt.p, ctx->burial_block));
+      HIP_TRY(d_tex_start.alloc((size_t)nbt + 1));
+      HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+      HIP_TRY(download(ctx, h_tex_start, d_tex_start.p, (size_t)nbt + 1));
+      ntex = h_tex_start[nbt];
+      HIP_TRY(d_txy.alloc(ntex + 1));
+      HIP_TRY(d_ttri.alloc(ntex + 1));
+      HIP_TRY(d_tpos.alloc(ntex + 1));
+      HIP_TRY(d_tnrm.alloc(ntex + 1));
+      tx = DevTexels{d_txy.p, d_tpos.p, d_tnrm.p, d_ttri.p};
+      HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
+      HIP_TRY(launch_compact(s, tx0, ntex0, d_buried.p, d_tex_start.p, d_cnt.p, tx));
+      HIP_TRY(hipStreamSynchronize(s));
+      d_t0xy.release();
+      d_t0tri.release();
+      d_t0pos.release();
+      d_t0nrm.release();
+      d_buried.release();
+      if (fill_res) {  // keep this batch's texels for the resident scene
+        auto piece = std::make_unique<Resident::Piece>();
+        piece->xy.take(d_txy);
+        piece->pos.take(d_tpos);
+        piece->nrm.take(d_tnrm);
+        HIP_TRY(piece->tri.alloc(ntex + 1));
+        HIP_TRY(launch_add_u32(s, d_ttri.p, piece->tri.p, ntex, t0));  // batch -> scene-wide triangle
+        piece->start = h_tex_start;
+        piece->n = ntex;
+        resident->pieces.push_back(std::move(piece));
+      }
+    }
     for (uint32_t t = 0; t < nbt; ++t) scene->stats.texels += h_tex_start[t + 1] - h_tex_start[t];
     texels_ms += ms_since(tb);
     if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
@@ -889,15 +932,41 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     HIP_TRY(download(ctx, words, d_out.p, bw));
     for (uint32_t i = B.m0; i < B.m1; ++i) {
       dl_baked_mesh_out& mo = scene->meshes[i];
-      mo.id = meshes[i]->id;
-      mo.content_hash = meshes[i]->content_hash;
+      mo.id = meta[i].id;
+      mo.content_hash = meta[i].content_hash;
       mo.resolution = res[i];
-      mo.occludes = meshes[i]->occludes;
+      mo.occludes = meta[i].occludes;
       mo.words.assign(words.begin() + (long)mesh_base[i], words.begin() + (long)mesh_base[i + 1]);
     }
     post_ms += ms_since(t_post);
     advance(B.m1 - B.m0);
     (void)dithered_bounce;
+  }
+  if (fill_res) {  // join the batches' texels into the resident scene's, in triangle order
+    uint64_t total = 0;
+    for (auto& piece : resident->pieces) total += piece->n;
+    HIP_TRY(resident->t_xy.alloc(total + 1));
+    HIP_TRY(resident->t_tri.alloc(total + 1));
+    HIP_TRY(resident->t_pos.alloc(total + 1));
+    HIP_TRY(resident->t_nrm.alloc(total + 1));
+    resident->tex_start.clear();
+    resident->tex_start.reserve((size_t)nt + 1);
+    uint64_t off = 0;
+    for (auto& piece : resident->pieces) {
+      const uint64_t n = piece->n;
+      if (n) {
+        HIP_TRY(hipMemcpyAsync(resident->t_xy.p + off, piece->xy.p, n * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(resident->t_tri.p + off, piece->tri.p, n * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(resident->t_pos.p + off, piece->pos.p, n * 16, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(resident->t_nrm.p + off, piece->nrm.p, n * 16, hipMemcpyDeviceToDevice, s));
+      }
+      for (size_t t = 0; t + 1 < piece->start.size(); ++t) resident->tex_start.push_back(off + piece->start[t]);
+      off += n;
+    }
+    resident->tex_start.push_back(off);
+    HIP_TRY(hipStreamSynchronize(s));
+    resident->pieces.clear();
+    if (resident->tex_start.size() != (size_t)nt + 1) { ctx->err = "resident texels do not cover the scene"; return DL_E_DEVICE; }
   }
   ctx->timings.texels_ms += texels_ms;
   ctx->timings.trace_ms = trace_ms;
@@ -1036,6 +1105,17 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     scene->stats.bounce_rays += h_brays[i];
   }
   ctx->timings.trace_rays = scene->stats.rays;
+  if (fill_res) {  // the geometry of this bake becomes the resident scene
+    resident->meshes = std::move(meta_local);
+    resident->res = std::move(res_local);
+    resident->first_tri = std::move(first_local);
+    resident->mid = std::move(mid_local);
+    resident->owned_off = std::move(owned_local);
+    resident->ntri = ntri;
+    resident->occ9 = std::move(occ9_local);
+    resident->has_alpha = mesh_alpha;
+    resident->ready = true;
+  }
   stage(4, 0);
   ctx->timings.total_ms = ms_since(t_start);
   *out = scene.release();

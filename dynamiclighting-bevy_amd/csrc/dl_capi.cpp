@@ -96,6 +96,35 @@ void trim_pool(int device) {
   if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) (void)hipMemPoolTrimTo(pool, 4ull << 30);
 }
 
+// Worker contexts 0..streams-1 of ctx, each with its own stream and the parent's sin/cos tables,
+// set up to share the device `streams` ways.  Returns false when a stream cannot be created.
+bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
+  const unsigned hw = usable_cpus();
+  while (ctx->workers.size() < streams) {
+    dl_ctx* w = new dl_ctx();
+    w->device = ctx->device;
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete w;
+      return false;
+    }
+    w->trig = ctx->trig;  // read-only tables owned by ctx (the worker's trig_buf stays empty)
+    ctx->workers.push_back(w);
+  }
+  for (uint32_t k = 0; k < streams; ++k) {
+    dl_ctx* w = ctx->workers[k];
+    w->host_threads = std::max(1u, hw / streams);
+    w->share = streams;
+    w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
+    w->trace_block = ctx->trace_block;
+    w->burial_block = ctx->burial_block;
+    w->gather = ctx->gather;
+    w->gather.share = streams;
+    w->gather_mode = ctx->gather_mode;
+    w->counting = false;
+  }
+  return true;
+}
+
 const dl::TrigHost& host_trig() {
   static std::once_flag once;
   static dl::TrigHost host;
@@ -214,29 +243,7 @@ int dl_bake_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_option
       out[i] = nullptr;
       status[i] = DL_E_CANCELLED;
     }
-    const unsigned hw = usable_cpus();
-    while (ctx->workers.size() < streams) {
-      dl_ctx* w = new dl_ctx();
-      w->device = ctx->device;
-      if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete w;
-        return set_err(ctx, DL_E_DEVICE, "creating a worker stream failed");
-      }
-      w->trig = ctx->trig;  // read-only tables owned by ctx (the worker's trig_buf stays empty)
-      ctx->workers.push_back(w);
-    }
-    for (uint32_t k = 0; k < streams; ++k) {
-      dl_ctx* w = ctx->workers[k];
-      w->host_threads = std::max(1u, hw / streams);
-      w->share = streams;
-      w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
-      w->trace_block = ctx->trace_block;
-      w->burial_block = ctx->burial_block;
-      w->gather = ctx->gather;
-      w->gather.share = streams;
-      w->gather_mode = ctx->gather_mode;
-      w->counting = false;
-    }
+    if (!prepare_workers(ctx, streams)) return set_err(ctx, DL_E_DEVICE, "creating a worker stream failed");
     // the host hands out jobs largest first (by a rough cost estimate), so that the last jobs
     // running alone are small ones; each worker bakes on its own stream
     std::vector<uint32_t> order(count);
@@ -287,6 +294,121 @@ int dl_bake_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_option
   } catch (...) {
     return set_err(ctx, DL_E_DEVICE, "unexpected failure");
   }
+}
+
+// ---- rebake service ----------------------------------------------------------------------
+int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_options* options, dl_service** out,
+                      dl_baked_scene** initial) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  if (!scene || !options || !out) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (initial) *initial = nullptr;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    std::unique_ptr<dl_service> svc(new dl_service());
+    svc->ctx = ctx;
+    svc->options = *options;
+    dl_baked_scene* first = nullptr;
+    const int st = dl_bake_impl(ctx, scene, options, nullptr, &first, &svc->res);
+    if (st != DL_OK) return st;
+    if (initial) *initial = first;
+    else delete first;
+    *out = svc.release();
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_err(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+int dl_service_bake(dl_service* svc, const dl_light* lights, uint32_t light_count, dl_baked_scene** out) {
+  if (!svc) return set_err(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (!out || (light_count && !lights)) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    const dl_bake_input in{nullptr, 0, lights, light_count};
+    return dl_bake_impl(ctx, &in, &svc->options, nullptr, out, &svc->res);
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_err(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint32_t count, uint32_t workers,
+                      double* latency_ms, int* status, dl_baked_scene** out) {
+  if (!svc) return set_err(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (count && (!requests || !latency_ms || !status)) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  if (workers == 0 || workers > 16) return set_err(ctx, DL_E_ARGUMENT, "workers must be 1 to 16");
+  for (uint32_t i = 0; i < count; ++i) {
+    if (requests[i].light_count && !requests[i].lights) return set_err(ctx, DL_E_ARGUMENT, "null light array in a request");
+    if (!(requests[i].arrival_s >= 0.0) || (i && requests[i].arrival_s < requests[i - 1].arrival_s))
+      return set_err(ctx, DL_E_ARGUMENT, "arrival times must be non-negative and in order");
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (!prepare_workers(ctx, workers)) return set_err(ctx, DL_E_DEVICE, "creating a worker stream failed");
+    for (uint32_t i = 0; i < count; ++i) {
+      status[i] = DL_E_CANCELLED;
+      latency_ms[i] = 0.0;
+      if (out) out[i] = nullptr;
+    }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::string> errs(count);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto run = [&](dl_ctx* w) {
+      (void)hipSetDevice(w->device);
+      for (;;) {
+        const uint32_t i = next.fetch_add(1);  // requests are admitted in arrival order
+        if (i >= count) break;
+        const auto arrival = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                      std::chrono::duration<double>(requests[i].arrival_s));
+        std::this_thread::sleep_until(arrival);
+        const dl_bake_input in{nullptr, 0, requests[i].lights, requests[i].light_count};
+        dl_baked_scene* scene = nullptr;
+        try {
+          status[i] = dl_bake_impl(w, &in, &svc->options, nullptr, &scene, &svc->res);
+        } catch (const std::bad_alloc&) {
+          w->err = "host allocation failed";
+          status[i] = DL_E_OOM;
+        } catch (...) {
+          w->err = "unexpected failure";
+          status[i] = DL_E_DEVICE;
+        }
+        latency_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - arrival).count();
+        if (status[i] != DL_OK) errs[i] = "request " + std::to_string(i) + ": " + w->err;
+        if (out) out[i] = scene;
+        else delete scene;
+      }
+    };
+    std::vector<std::thread> ts;
+    for (uint32_t k = 1; k < workers; ++k) ts.emplace_back(run, ctx->workers[k]);
+    run(ctx->workers[0]);
+    for (auto& t : ts) t.join();
+    for (uint32_t i = 0; i < count; ++i)
+      if (status[i] != DL_OK) return set_err(ctx, status[i], errs[i]);
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_err(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+void dl_service_destroy(dl_service* svc) {
+  if (!svc) return;
+  dl_ctx* ctx = svc->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();  // no worker still reads the resident buffers
+  delete svc;
 }
 
 uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker, double* span, uint32_t cap) {

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <chrono>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -88,6 +89,17 @@ struct DBuf {
     p = nullptr;
     n = 0;
     cache = nullptr;
+  }
+  void take(DBuf& o) {  // takes over o's allocation
+    if (&o == this) return;
+    release();
+    p = o.p;
+    n = o.n;
+    cache = o.cache;
+    cls = o.cls;
+    o.p = nullptr;
+    o.n = 0;
+    o.cache = nullptr;
   }
   hipError_t alloc(size_t count) {
     release();
@@ -189,8 +201,56 @@ struct dl_baked_scene {
   std::vector<uint32_t> probe_words;
 };
 
+namespace dl {
+// The geometry side of a bake kept on the device between bakes of one scene with different lights
+// (a rebake service): receiver meshes in BakeId order, the occluder BVH and triangle records, the
+// receiver triangles, and every triangle's unburied texels.  None of these depend on the lights.
+struct MeshMeta {
+  uint64_t id, content_hash;
+  uint32_t occludes;
+};
+struct Resident {
+  bool ready = false;  // filled by a first bake, then read by later bakes (by several at once)
+  std::vector<MeshMeta> meshes;
+  std::vector<uint32_t> res, first_tri;
+  std::vector<uint64_t> mid, owned_off;
+  uint64_t ntri = 0;
+  std::vector<float> occ9;  // occluder corners (sun map frames, sky probe bounds)
+  uint32_t nocc = 0, depth = 0;
+  uint64_t nodes = 0;
+  bool has_alpha = false;  // some occluder has an alpha texture (used when a light tests alpha)
+  DBuf<CNode> d_nodes;
+  DBuf<float> d_occ, d_occ_nrm, d_occ_alb, d_occ_uv0;
+  DBuf<uint32_t> d_occ_flags, d_occ_tex, d_tex_info, d_tex_bits;
+  DBuf<float> d_v, d_n, d_uv;
+  DBuf<uint32_t> d_mesh, d_local, d_res;
+  DBuf<uint64_t> d_mid, d_owned_off;
+  // unburied texels of every triangle in triangle order; tri holds the scene-wide triangle index
+  DBuf<uint32_t> t_xy, t_tri;
+  DBuf<float4> t_pos, t_nrm;
+  std::vector<uint64_t> tex_start;  // ntri + 1
+  // while the first bake runs: the texels of each of its mesh batches
+  struct Piece {
+    DBuf<uint32_t> xy, tri;
+    DBuf<float4> pos, nrm;
+    std::vector<uint64_t> start;  // per batch triangle, relative
+    uint64_t n = 0;
+  };
+  std::vector<std::unique_ptr<Piece>> pieces;
+};
+}  // namespace dl
+
+// A rebake service: one context, the bake options, and the resident scene.
+struct dl_service {
+  dl_ctx* ctx = nullptr;
+  dl_bake_options options{};
+  dl::Resident res;
+};
+
+// The bake.  With `res` not ready, a normal bake that also fills it; with `res` ready, the meshes
+// of `input` are ignored and the resident ones are baked with the lights of `input`.
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
-                 dl_baked_scene** out);
+                 dl_baked_scene** out, dl::Resident* res = nullptr);
 
 namespace dl {
 

@@ -106,6 +106,7 @@ hipError_t launch_owned(hipStream_t s, DevTris tris, DevMeshes meshes, const uin
 hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_count,
                          const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
                          const uint64_t* tex_start, DevTexels out);
+hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta);
 hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
                          uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept_count,
                          uint32_t block);

@@ -983,6 +983,12 @@ __global__ void k_trig_eval(TrigTables trig, const float* __restrict__ x, uint64
   }
 }
 
+// out[i] = in[i] + delta (mod 2^32): moves triangle indices between numberings.
+__global__ void k_add_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n, uint32_t delta) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] + delta;
+}
+
 // ---- launchers ------------------------------------------------------------------------------
 static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
   uint64_t g = (n + block - 1) / block;
@@ -1025,6 +1031,11 @@ hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes 
   if (ntex == 0) return hipSuccess;
   hipLaunchKernelGGL(k_burial, dim3(grid_for(ntex, block)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
                      tx, ntex, seed, trig, buried, kept_count);
+  return hipGetLastError();
+}
+hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_u32, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, in, out, n, delta);
   return hipGetLastError();
 }
 hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint8_t* buried, const uint64_t* start,

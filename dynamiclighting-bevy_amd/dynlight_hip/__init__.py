@@ -53,6 +53,16 @@ def load() -> C.CDLL:
     lib.dl_bake_batch.restype = C.c_int
     lib.dl_bake_batch.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
                                   C.c_uint32, C.POINTER(P), C.POINTER(C.c_int)]
+    lib.dl_service_create.restype = C.c_int
+    lib.dl_service_create.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.POINTER(P),
+                                      C.POINTER(P)]
+    lib.dl_service_bake.restype = C.c_int
+    lib.dl_service_bake.argtypes = [P, C.c_void_p, C.c_uint32, C.POINTER(P)]
+    lib.dl_service_replay.restype = C.c_int
+    lib.dl_service_replay.argtypes = [P, C.POINTER(abi.dl_service_request), C.c_uint32, C.c_uint32,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(P)]
+    lib.dl_service_destroy.restype = None
+    lib.dl_service_destroy.argtypes = [P]
     lib.dl_baked_scene_free.restype = None
     lib.dl_baked_scene_free.argtypes = [P]
     lib.dl_last_timings.restype = C.c_int
@@ -180,6 +190,78 @@ class Context:
         st = self.lib.dl_set_gather(self.handle, mode, block, chunk, chunk_rays, blocks_per_cu, min_waves, refill)
         if st != 0:
             raise BakeError(st, self.error())
+
+
+class Service:
+    """A rebake service (dl_service): the scene's geometry stays resident on the device and later
+    bakes of other lights reuse it.  `initial` is the first bake of the scene with its lights."""
+
+    def __init__(self, ctx: Context, inp: BakeInput, options: abi.dl_bake_options | None = None):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.options = options if options is not None else default_options()
+        h, first = C.c_void_p(), C.c_void_p()
+        st = self.lib.dl_service_create(ctx.handle, C.byref(inp.struct), C.byref(self.options), C.byref(h), C.byref(first))
+        if st != 0:
+            raise BakeError(st, ctx.error())
+        self.handle = h
+        try:
+            self.initial = abi.read_scene(self.lib, "dl_", first)
+        finally:
+            self.lib.dl_baked_scene_free(first)
+
+    def bake(self, lights) -> abi.BakedScene:
+        """The scene's meshes baked with `lights` (a list of LightData), reusing the resident geometry."""
+        packed = BakeInput([], list(lights))
+        h = C.c_void_p()
+        st = self.lib.dl_service_bake(self.handle, packed.struct.lights, packed.struct.light_count, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        try:
+            return abi.read_scene(self.lib, "dl_", h)
+        finally:
+            self.lib.dl_baked_scene_free(h)
+
+    def replay(self, requests, workers: int = 4, read: bool | str = False):
+        """Serves (lights, arrival_s) requests as they arrive on `workers` streams.  Returns the
+        latencies in ms (arrival to result on the host) and, with read=True, the results
+        (read="stats": only their stats)."""
+        n = len(requests)
+        packed = [BakeInput([], list(l)) for l, _ in requests]
+        reqs = (abi.dl_service_request * max(1, n))()
+        for i, (p, (_, t)) in enumerate(zip(packed, requests)):
+            reqs[i].lights = p.struct.lights
+            reqs[i].light_count = p.struct.light_count
+            reqs[i].arrival_s = float(t)
+        lat = (C.c_double * max(1, n))()
+        status = (C.c_int * max(1, n))()
+        outs = (C.c_void_p * max(1, n))()
+        st = self.lib.dl_service_replay(self.handle, reqs, n, workers, lat, status, outs if read else None)
+        try:
+            if st != 0:
+                raise BakeError(st, self.ctx.error())
+            results = None
+            if read == "stats":
+                results = []
+                for i in range(n):
+                    sst = abi.dl_bake_stats()
+                    self.lib.dl_scene_stats(outs[i], C.byref(sst))
+                    results.append({name: getattr(sst, name) for name, _ in sst._fields_})
+            elif read:
+                results = [abi.read_scene(self.lib, "dl_", outs[i]) for i in range(n)]
+            return list(lat[:n]), results
+        finally:
+            if read:
+                for i in range(n):
+                    if outs[i]:
+                        self.lib.dl_baked_scene_free(outs[i])
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dl_service_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 _default_ctx = None

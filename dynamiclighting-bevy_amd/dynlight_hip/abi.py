@@ -81,6 +81,10 @@ class dl_bake_input(C.Structure):
                 ("light_count", C.c_uint32)]
 
 
+class dl_service_request(C.Structure):
+    _fields_ = [("lights", C.c_void_p), ("light_count", C.c_uint32), ("arrival_s", C.c_double)]
+
+
 class dl_bake_options(C.Structure):
     _fields_ = [("pixel_density", C.c_float), ("max_lightmap_size", C.c_uint32),
                 ("bounce_compression", C.c_int32), ("dynamic_geometry", C.c_int32),
@@ -343,6 +347,18 @@ class BakeInput:
     @property
     def triangle_count(self) -> int:
         return int(self._mesh_arr["index_count"].sum() // 3)
+
+    def with_lights(self, lights: list[LightData]) -> "BakeInput":
+        """The same meshes (buffers are shared) with other lights."""
+        other = BakeInput.__new__(BakeInput)
+        other.meshes = self.meshes
+        other.lights = list(lights)
+        other._keep = [self]
+        other._alphas = []
+        other._mesh_arr = self._mesh_arr
+        other._pack_lights()
+        other._make_struct()
+        return other
 
     def subset(self, keep: np.ndarray) -> "BakeInput":
         """A view with only the meshes where `keep` is true (buffers are shared)."""

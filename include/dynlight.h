@@ -215,6 +215,33 @@ int dl_bake(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* opti
             dl_progress* progress, dl_baked_scene** out);
 void dl_baked_scene_free(dl_baked_scene* scene);
 
+/* ---- rebake service (no counterpart in the reference, which re-runs bake() on every change) ----
+ * A service keeps one scene's geometry resident on the device: receiver meshes, the occluder BVH
+ * and every triangle's unburied texels (none of which depend on the lights).  A rebake then bakes
+ * a set of lights (moved or added ones) against it, re-tracing only the (triangle, light) pairs of
+ * those lights: the result equals dl_bake of the scene's meshes with exactly those lights and the
+ * service's options.  Destroy services before their context. */
+typedef struct dl_service dl_service;
+typedef struct dl_service_request {
+    const dl_light* lights;  /* the lights to rebake                                   */
+    uint32_t light_count;
+    double arrival_s;        /* arrival time in seconds after the replay starts (sorted) */
+} dl_service_request;
+
+/* Bakes `scene` (meshes and lights) once and keeps its geometry resident; *initial (if not NULL)
+ * receives that bake's result. */
+int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_options* options, dl_service** out,
+                      dl_baked_scene** initial);
+/* Bakes `lights` against the resident scene. */
+int dl_service_bake(dl_service* svc, const dl_light* lights, uint32_t light_count, dl_baked_scene** out);
+/* Serves requests as they arrive: each is admitted, in arrival order, by the first of `workers`
+ * (1-16) worker streams that is free, so requests that arrive while others run overlap on the
+ * device.  latency_ms[i] runs from the request's arrival to its result being readable on the host;
+ * out (may be NULL) receives the results, which equal dl_service_bake of each request alone. */
+int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint32_t count, uint32_t workers,
+                      double* latency_ms, int* status, dl_baked_scene** out);
+void dl_service_destroy(dl_service* svc);
+
 /* Many independent bakes (no counterpart in the reference, which bakes one scene per call):
  * job i bakes inputs[i] with options[i] into out[i] (status[i] is its dl_bake result).  Jobs are
  * handed out in order to `streams` worker contexts (1-16), each baking on its own HIP stream, so
