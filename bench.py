@@ -168,6 +168,74 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_main(args):
+    """Serving: the interior stays resident (dl_service) while light requests arrive as a Poisson
+    process (20/s); each rebakes the moved or added lights (with bounce) against the resident
+    geometry.  Latency runs from arrival to the result being readable on the host."""
+    import numpy as np
+
+    import dynlight_hip
+    from dynlight_hip import scenes
+
+    inp, opts = scenes.interior(2)
+    ctx = dynlight_hip.Context(0)
+    t0 = time.perf_counter()
+    svc = dynlight_hip.Service(ctx, inp, opts)
+    create_s = time.perf_counter() - t0
+    log(f"service created in {create_s:.2f} s (initial bake of {len(inp.lights)} lights)")
+    reqs = scenes.serve_requests(inp, count=args.requests, rate=args.rate)
+    served = [(l, t) for kind, l, t in reqs if kind != "room"]
+    rooms = sum(1 for kind, _, _ in reqs if kind == "room")
+    # warm-up: a few requests alone
+    for l, _ in served[:4]:
+        svc.bake(l)
+    # service time of each request alone: all arrive at once and one worker takes them in turn
+    alone = []
+    svc.replay([(l, 0.0) for l, _ in served], workers=1, read=False, service_times=alone)
+    alone = np.array(alone)
+    log(f"alone: p50 {np.percentile(alone, 50):.1f} ms, p99 {np.percentile(alone, 99):.1f} ms, "
+        f"capacity {1000.0 * len(alone) / alone.sum():.1f} requests/s on one stream")
+    t0 = time.perf_counter()
+    svc_ms = []
+    lat, stats = svc.replay(served, workers=args.workers, read="stats", service_times=svc_ms)
+    wall = time.perf_counter() - t0
+    lat = np.array(lat)
+    samples = sum(x["samples"] for x in stats)
+    log(f"{len(served)} light requests in {wall:.1f} s; p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms")
+    out = {
+        "metric": f"p50/p99 rebake latency (1 GPU, resident interior, requests at {args.rate:g}/s)",
+        "value": float(np.percentile(lat, 99)),
+        "unit": "ms (p99)",
+        "n_gpus": 1,
+        "steps": len(served),
+        "warmup": 4,
+        "ms_per_step": float(np.mean(lat)),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serve: interior resident (200k triangles, 256 bounce lights, 32 texels/m); Poisson "
+                               f"arrivals {args.rate:g}/s; light requests move or add 1-4 point lights with one bounce",
+                   "requests": len(reqs), "light_requests": len(served), "room_requests_not_served": rooms,
+                   "workers": args.workers},
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p90_ms": float(np.percentile(lat, 90)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(lat.max()),
+        "alone_p50_ms": float(np.percentile(alone, 50)),
+        "alone_p99_ms": float(np.percentile(alone, 99)),
+        "in_mix_service_p50_ms": float(np.percentile(svc_ms, 50)),
+        "in_mix_service_p99_ms": float(np.percentile(svc_ms, 99)),
+        "arrival_rate_light_requests_per_s": len(served) / max(1e-9, served[-1][1]),
+        "requests_per_s": len(served) / wall,
+        "samples_per_s": samples / wall,
+        "service_create_s": create_s,
+    }
+    svc.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,8 +244,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("interior", "batch"), default="interior",
-                    help="interior: the 200k-triangle interior bake; batch: 512 independent small bakes")
+    ap.add_argument("--workload", choices=("interior", "batch", "serve"), default="interior",
+                    help="interior: the 200k-triangle interior bake; batch: 512 independent small bakes; "
+                         "serve: light rebakes of the resident interior")
+    ap.add_argument("--requests", type=int, default=400, help="serve workload: requests (80%% are light requests)")
+    ap.add_argument("--workers", type=int, default=4, help="serve workload: worker streams")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve workload: request arrivals per second")
     ap.add_argument("--jobs", type=int, default=512, help="batch workload: number of jobs")
     ap.add_argument("--streams", default="2,4,8", help="batch workload: worker stream counts to time")
     args = ap.parse_args()
@@ -198,6 +270,9 @@ def main():
 
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "serve":
+        serve_main(args)
         return
 
     import dynlight_hip

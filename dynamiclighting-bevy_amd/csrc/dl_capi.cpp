@@ -119,6 +119,8 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     w->burial_block = ctx->burial_block;
     w->gather = ctx->gather;
     w->gather.share = streams;
+    // smaller ray chunks, so that each worker's gather buffers stay within its cache
+    w->gather.chunk_rays = std::max<uint64_t>(1ull << 22, ctx->gather.chunk_rays / streams);
     w->gather_mode = ctx->gather_mode;
     w->counting = false;
   }
@@ -341,7 +343,7 @@ int dl_service_bake(dl_service* svc, const dl_light* lights, uint32_t light_coun
 }
 
 int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint32_t count, uint32_t workers,
-                      double* latency_ms, int* status, dl_baked_scene** out) {
+                      double* latency_ms, double* service_ms, int* status, dl_baked_scene** out) {
   if (!svc) return set_err(nullptr, DL_E_ARGUMENT, "null service");
   dl_ctx* ctx = svc->ctx;
   if (count && (!requests || !latency_ms || !status)) return set_err(ctx, DL_E_ARGUMENT, "null argument");
@@ -358,6 +360,7 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
     for (uint32_t i = 0; i < count; ++i) {
       status[i] = DL_E_CANCELLED;
       latency_ms[i] = 0.0;
+      if (service_ms) service_ms[i] = 0.0;
       if (out) out[i] = nullptr;
     }
     std::atomic<uint32_t> next{0};
@@ -371,6 +374,7 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
         const auto arrival = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
                                       std::chrono::duration<double>(requests[i].arrival_s));
         std::this_thread::sleep_until(arrival);
+        const auto started = std::chrono::steady_clock::now();
         const dl_bake_input in{nullptr, 0, requests[i].lights, requests[i].light_count};
         dl_baked_scene* scene = nullptr;
         try {
@@ -382,7 +386,9 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
           w->err = "unexpected failure";
           status[i] = DL_E_DEVICE;
         }
-        latency_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - arrival).count();
+        const auto done = std::chrono::steady_clock::now();
+        latency_ms[i] = std::chrono::duration<double, std::milli>(done - arrival).count();
+        if (service_ms) service_ms[i] = std::chrono::duration<double, std::milli>(done - started).count();
         if (status[i] != DL_OK) errs[i] = "request " + std::to_string(i) + ": " + w->err;
         if (out) out[i] = scene;
         else delete scene;

@@ -60,7 +60,7 @@ def load() -> C.CDLL:
     lib.dl_service_bake.argtypes = [P, C.c_void_p, C.c_uint32, C.POINTER(P)]
     lib.dl_service_replay.restype = C.c_int
     lib.dl_service_replay.argtypes = [P, C.POINTER(abi.dl_service_request), C.c_uint32, C.c_uint32,
-                                      C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(P)]
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(P)]
     lib.dl_service_destroy.restype = None
     lib.dl_service_destroy.argtypes = [P]
     lib.dl_baked_scene_free.restype = None
@@ -221,10 +221,11 @@ class Service:
         finally:
             self.lib.dl_baked_scene_free(h)
 
-    def replay(self, requests, workers: int = 4, read: bool | str = False):
+    def replay(self, requests, workers: int = 4, read: bool | str = False, service_times: list | None = None):
         """Serves (lights, arrival_s) requests as they arrive on `workers` streams.  Returns the
         latencies in ms (arrival to result on the host) and, with read=True, the results
-        (read="stats": only their stats)."""
+        (read="stats": only their stats).  service_times, if a list, receives each request's time
+        from admission to result."""
         n = len(requests)
         packed = [BakeInput([], list(l)) for l, _ in requests]
         reqs = (abi.dl_service_request * max(1, n))()
@@ -233,9 +234,12 @@ class Service:
             reqs[i].light_count = p.struct.light_count
             reqs[i].arrival_s = float(t)
         lat = (C.c_double * max(1, n))()
+        svc_ms = (C.c_double * max(1, n))()
         status = (C.c_int * max(1, n))()
         outs = (C.c_void_p * max(1, n))()
-        st = self.lib.dl_service_replay(self.handle, reqs, n, workers, lat, status, outs if read else None)
+        st = self.lib.dl_service_replay(self.handle, reqs, n, workers, lat, svc_ms, status, outs if read else None)
+        if service_times is not None:
+            service_times[:] = list(svc_ms[:n])
         try:
             if st != 0:
                 raise BakeError(st, self.ctx.error())

@@ -508,6 +508,50 @@ def batch(count: int = 512, seed: int = 4):
     return [j[0] for j in jobs], [j[1] for j in jobs]
 
 
+# ---- serve: rebake requests against the resident interior -------------------------------------
+def serve_requests(inp: BakeInput, count: int = 2000, seed: int = 5, rate: float = 20.0, grid: int = 8,
+                   room=(6.0, 3.0, 6.0), room_share: float = 0.2):
+    """Requests arriving as a Poisson process of `rate` per second against the interior scene.
+    A share 1 - room_share are light requests: with equal odds, move 1-4 existing lights (same ids,
+    displaced U(0.5, 3) m horizontally) or add 1-4 new lights from the interior's distribution
+    (a random room, range U(4, 12), one bounce).  The rest add a room of geometry; they are
+    returned with kind "room" and no lights.  Returns a list of (kind, lights, arrival_s)."""
+    rng = Pcg32.seeded(seed)
+    W, H, D = room
+    t = 0.0
+    out = []
+    next_id = 1_000_000
+    for r in range(count):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        if rng.uniform() < room_share:
+            out.append(("room", [], t))
+            continue
+        k = 1 + rng.below(4)
+        lights = []
+        if rng.uniform() < 0.5:
+            picked = []
+            while len(picked) < k:
+                i = rng.below(len(inp.lights))
+                if i not in picked:
+                    picked.append(i)
+            for i in sorted(picked):
+                base = inp.lights[i]
+                d, a = rng.uniform(0.5, 3.0), rng.uniform(0.0, 2.0 * math.pi)
+                x = min(grid * W - 0.1, max(0.1, base.position[0] + d * math.cos(a)))
+                z = min(grid * D - 0.1, max(0.1, base.position[2] + d * math.sin(a)))
+                lights.append(point_light(base.id, (x, base.position[1], z), base.radius, bounce=True))
+            out.append(("move", lights, t))
+        else:
+            for _ in range(k):
+                room_i = rng.below(grid * grid)
+                rx, rz = (room_i % grid) * W, (room_i // grid) * D
+                p = (rx + rng.uniform(0.1, W - 0.1), rng.uniform(0.1, H - 0.1), rz + rng.uniform(0.1, D - 0.1))
+                lights.append(point_light(next_id, p, rng.uniform(4.0, 12.0), bounce=True))
+                next_id += 1
+            out.append(("add", lights, t))
+    return out
+
+
 def mesh_sample_mask(ids: np.ndarray, modulus: int) -> np.ndarray:
     """Deterministic receiver subset: meshes with splitmix64(id) % modulus == 0."""
     return np.array([splitmix64(int(i)) % modulus == 0 for i in ids], dtype=bool)

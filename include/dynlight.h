@@ -236,10 +236,11 @@ int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_opt
 int dl_service_bake(dl_service* svc, const dl_light* lights, uint32_t light_count, dl_baked_scene** out);
 /* Serves requests as they arrive: each is admitted, in arrival order, by the first of `workers`
  * (1-16) worker streams that is free, so requests that arrive while others run overlap on the
- * device.  latency_ms[i] runs from the request's arrival to its result being readable on the host;
- * out (may be NULL) receives the results, which equal dl_service_bake of each request alone. */
+ * device.  latency_ms[i] runs from the request's arrival to its result being readable on the host,
+ * service_ms[i] (may be NULL) from its admission; out (may be NULL) receives the results, which
+ * equal dl_service_bake of each request alone. */
 int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint32_t count, uint32_t workers,
-                      double* latency_ms, int* status, dl_baked_scene** out);
+                      double* latency_ms, double* service_ms, int* status, dl_baked_scene** out);
 void dl_service_destroy(dl_service* svc);
 
 /* Many independent bakes (no counterpart in the reference, which bakes one scene per call):
