@@ -1088,9 +1088,17 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     std::memcpy(b.origin, l->position, 12);
     b.radius = l->radius;
     b.shape = l->shape;
-    std::memcpy(b.rotation, l->rotation, 16);
-    std::memcpy(b.size, l->size, 8);
-    b.angular_diameter = l->angular_diameter;
+    // BakedLight keeps only what its LightShape holds: an area light's rotation and size, a sun's
+    // rotation and angular diameter, nothing for points and skies
+    const float identity[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    std::memcpy(b.rotation, identity, 16);
+    if (l->shape == DL_SHAPE_RECTANGLE || l->shape == DL_SHAPE_TUBE) {
+      std::memcpy(b.rotation, l->rotation, 16);
+      std::memcpy(b.size, l->size, 8);
+    } else if (l->shape == DL_SHAPE_SUN) {
+      std::memcpy(b.rotation, l->rotation, 16);
+      b.angular_diameter = l->angular_diameter;
+    }
     b.shadow_bits_per_texel = hlights[i].bits;
     b.bounce_compression = hlights[i].bakes_bounce ? comp_of_slot[i] : -1;
     scene->lights.push_back(b);

@@ -135,6 +135,8 @@ const dl::TrigHost& host_trig() {
 }
 }  // namespace
 
+void dl_set_thread_error(const std::string& msg) { g_null_ctx_err = msg; }
+
 extern "C" {
 
 int dl_ctx_create(int device, dl_ctx** out) {

@@ -247,6 +247,9 @@ struct dl_service {
   dl::Resident res;
 };
 
+// Sets the error that dl_last_error(NULL) reports on this thread.
+void dl_set_thread_error(const std::string& msg);
+
 // The bake.  With `res` not ready, a normal bake that also fills it; with `res` ready, the meshes
 // of `input` are ignored and the resident ones are baked with the lights of `input`.
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,

@@ -63,6 +63,16 @@ def load() -> C.CDLL:
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(P)]
     lib.dl_service_destroy.restype = None
     lib.dl_service_destroy.argtypes = [P]
+    lib.dl_encode.restype = C.c_int
+    lib.dl_encode.argtypes = [P, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint64)]
+    lib.dl_bytes_free.restype = None
+    lib.dl_bytes_free.argtypes = [C.POINTER(C.c_uint8)]
+    lib.dl_decode.restype = C.c_int
+    lib.dl_decode.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(P)]
+    lib.dl_decode_stream.restype = C.c_int
+    lib.dl_decode_stream.argtypes = [READ_FN, C.c_void_p, C.POINTER(P)]
+    lib.dl_format_error_kind.restype = C.c_int
+    lib.dl_format_error_kind.argtypes = []
     lib.dl_baked_scene_free.restype = None
     lib.dl_baked_scene_free.argtypes = [P]
     lib.dl_last_timings.restype = C.c_int
@@ -92,6 +102,84 @@ def load() -> C.CDLL:
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
     _lib = lib
     return lib
+
+
+READ_FN = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.POINTER(C.c_uint8), C.c_uint64)
+
+
+def _format_error(lib) -> BakeError:
+    e = BakeError(abi.DL_E_FORMAT, lib.dl_last_error(None).decode())
+    e.format_kind = lib.dl_format_error_kind()
+    return e
+
+
+def _encode_handle(lib, h) -> bytes:
+    buf, n = C.POINTER(C.c_uint8)(), C.c_uint64()
+    st = lib.dl_encode(h, C.byref(buf), C.byref(n))
+    if st != 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    try:
+        return C.string_at(buf, n.value)
+    finally:
+        lib.dl_bytes_free(buf)
+
+
+def _decode_handle(lib, data: bytes):
+    h = C.c_void_p()
+    st = lib.dl_decode(data, len(data), C.byref(h))
+    if st == abi.DL_E_FORMAT:
+        raise _format_error(lib)
+    if st != 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    return h
+
+
+def decode(data: bytes) -> abi.BakedScene:
+    """decode (src/bake/format/mod.rs:152-169): a .dynlight file held in memory."""
+    lib = load()
+    h = _decode_handle(lib, bytes(data))
+    try:
+        return abi.read_scene(lib, "dl_", h)
+    finally:
+        lib.dl_baked_scene_free(h)
+
+
+def decode_stream(source) -> abi.BakedScene:
+    """decode_stream (src/bake/format/stream.rs:34-71): a .dynlight file read from `source`
+    (anything with read(n)) as it is parsed."""
+    lib = load()
+    failed = []
+
+    def read(_user, buf, cap):
+        try:
+            chunk = source.read(int(cap))
+        except Exception as exc:  # the source's own failure
+            failed.append(exc)
+            return -1
+        C.memmove(buf, chunk, len(chunk))
+        return len(chunk)
+
+    cb = READ_FN(read)
+    h = C.c_void_p()
+    st = lib.dl_decode_stream(cb, None, C.byref(h))
+    if st == abi.DL_E_FORMAT:
+        raise _format_error(lib)
+    if st != 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    try:
+        return abi.read_scene(lib, "dl_", h)
+    finally:
+        lib.dl_baked_scene_free(h)
+
+
+def reencode(data: bytes) -> bytes:
+    """decode followed by encode (src/bake/format/mod.rs:101-119) of the decoded scene."""
+    lib = load()
+    h = _decode_handle(lib, bytes(data))
+    try:
+        return _encode_handle(lib, h)
+    finally:
+        lib.dl_baked_scene_free(h)
 
 
 class Context:
@@ -168,6 +256,18 @@ class Context:
         self.lib.dl_test_batch_trace(self.handle, tm, wk, sp, n)
         return [{"worker": wk[i], "start_ms": sp[2 * i], "end_ms": sp[2 * i + 1],
                  **{name: getattr(tm[i], name) for name, _ in tm[i]._fields_}} for i in range(n)]
+
+    def bake_encoded(self, inp: BakeInput, options: abi.dl_bake_options | None = None) -> tuple[abi.BakedScene, bytes]:
+        """bake() and encode() of its result: the scene and its .dynlight bytes."""
+        options = options if options is not None else default_options()
+        h = C.c_void_p()
+        st = self.lib.dl_bake(self.handle, C.byref(inp.struct), C.byref(options), None, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.error())
+        try:
+            return abi.read_scene(self.lib, "dl_", h), _encode_handle(self.lib, h)
+        finally:
+            self.lib.dl_baked_scene_free(h)
 
     def timings(self) -> dict:
         t = abi.dl_timings()

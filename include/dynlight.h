@@ -215,6 +215,23 @@ int dl_bake(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* opti
             dl_progress* progress, dl_baked_scene** out);
 void dl_baked_scene_free(dl_baked_scene* scene);
 
+/* ---- the .dynlight file (src/bake/format/mod.rs:1-21, 101-301; format/stream.rs:34-71) ----------
+ * dl_encode writes the scene's file bytes (encode); the same scene always gives the same bytes.
+ * The buffer is the caller's to release with dl_bytes_free.  dl_decode parses a file held in
+ * memory (decode), checking magic, version, length and checksum before the payload;
+ * dl_decode_stream parses one as it is read (decode_stream): `read` fills up to cap bytes and
+ * returns how many (0 at the end, negative on failure).  Both return DL_E_FORMAT with the
+ * reference's FormatError message in dl_last_error(NULL) and its variant in
+ * dl_format_error_kind().  A decoded scene's stats have no sample count (the file holds none). */
+enum { DL_FORMAT_NOT_DYNLIGHT = 1, DL_FORMAT_UNSUPPORTED_VERSION = 2, DL_FORMAT_TRUNCATED = 3,
+       DL_FORMAT_DECOMPRESS = 4, DL_FORMAT_CHECKSUM = 5, DL_FORMAT_INVALID = 6 };
+typedef int64_t (*dl_read_fn)(void* user, uint8_t* buf, uint64_t cap);
+int dl_encode(const dl_baked_scene* scene, uint8_t** bytes, uint64_t* len);
+void dl_bytes_free(uint8_t* bytes);
+int dl_decode(const uint8_t* bytes, uint64_t len, dl_baked_scene** out);
+int dl_decode_stream(dl_read_fn read, void* user, dl_baked_scene** out);
+int dl_format_error_kind(void);
+
 /* ---- rebake service (no counterpart in the reference, which re-runs bake() on every change) ----
  * A service keeps one scene's geometry resident on the device: receiver meshes, the occluder BVH
  * and every triangle's unburied texels (none of which depend on the lights).  A rebake then bakes

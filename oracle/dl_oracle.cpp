@@ -2363,9 +2363,17 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
     memcpy(b.origin, L.src->position, 12);
     b.radius = L.radius;
     b.shape = L.shape;
-    memcpy(b.rotation, L.src->rotation, 16);
-    memcpy(b.size, L.src->size, 8);
-    b.angular_diameter = L.src->angular_diameter;
+    // BakedLight keeps only what its LightShape holds (format/types.rs, light/shape.rs): an area
+    // light's rotation and size, a sun's rotation and angular diameter, nothing for the rest
+    const float identity[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    memcpy(b.rotation, identity, 16);
+    if (L.shape == DL_SHAPE_RECTANGLE || L.shape == DL_SHAPE_TUBE) {
+      memcpy(b.rotation, L.src->rotation, 16);
+      memcpy(b.size, L.src->size, 8);
+    } else if (L.shape == DL_SHAPE_SUN) {
+      memcpy(b.rotation, L.src->rotation, 16);
+      b.angular_diameter = L.src->angular_diameter;
+    }
     b.shadow_bits_per_texel = L.bits;
     b.bounce_compression = L.bakes_bounce ? (L.src->bounce_compression >= 0 ? L.src->bounce_compression : opt.bounce_compression) : -1;
     res->lights.push_back(b);
