@@ -772,7 +772,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           const uint2* jobs = d_tjobs.p + j0;
           HIP_TRY(launch_gather_nrays(s, ent, d_lights.p, jobs, n, d_nr.p));
           HIP_TRY(scan_u64(s, d_nr.p, d_rs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
-          HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p));
+          HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
+                                     ctx->gather.order));
           HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, sizeof(unsigned long long), s));
           hipEvent_t a, b2;
           HIP_TRY(hipEventCreate(&a));
@@ -780,10 +781,10 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(hipEventCreate(&b2));
           tev.push_back(b2);
           HIP_TRY(hipEventRecord(a, s));
-          HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_rs.p + n, n * max_samples, d_contrib.p,
+          HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_rs.p + n, n * max_samples, d_rs.p, n, d_contrib.p,
                                     reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather));
           HIP_TRY(hipEventRecord(b2, s));
-          HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p));
+          HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p, ctx->gather.order));
         }
         if (sun_bounce)
           HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,

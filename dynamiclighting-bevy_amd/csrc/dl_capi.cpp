@@ -619,7 +619,7 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
 }
 
 int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint64_t chunk_rays, uint32_t blocks_per_cu,
-                  uint32_t min_waves, uint32_t refill) {
+                  uint32_t min_waves, uint32_t refill, uint32_t order) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   if (mode > 1) return set_err(ctx, DL_E_ARGUMENT, "gather mode must be 0 or 1");
   if (block && (block % 64 != 0 || block > 1024)) return set_err(ctx, DL_E_ARGUMENT, "block must be a multiple of 64 up to 1024");
@@ -628,8 +628,10 @@ int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, ui
   if (chunk > (1u << 20)) return set_err(ctx, DL_E_ARGUMENT, "at most 2^20 rays per queue run");
   if (min_waves != 0 && min_waves != 8) return set_err(ctx, DL_E_ARGUMENT, "min_waves must be 0 or 8");
   if (refill > 64) return set_err(ctx, DL_E_ARGUMENT, "refill must be at most 64");
+  if (order > 2) return set_err(ctx, DL_E_ARGUMENT, "order must be 0 (keep), 1 (job-major) or 2 (sample-major)");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->gather_mode = mode;
+  if (order) ctx->gather.order = order - 1;
   if (block) ctx->gather.block = block;
   if (chunk) ctx->gather.chunk = chunk;
   if (chunk_rays) ctx->gather.chunk_rays = chunk_rays;

@@ -180,6 +180,7 @@ struct GatherLaunch {
   uint32_t refill = 56;         // idle lanes before finished lanes take new rays (1-64)
   uint64_t chunk_rays = 1ull << 28;  // rays per chunk of jobs (buffer size)
   uint32_t share = 1;           // the persistent grid fills 1/share of the device (bakes side by side)
+  uint32_t order = 1;           // ray slots: 0 job-major, 1 sample-major
 };
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                               DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,
@@ -189,11 +190,13 @@ hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, co
 hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out);
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
-                              float4* rays, float* contrib);
+                              float4* rays, float* contrib, uint32_t order);
+// max_rays: the chunk's ray slots (jobs x the most samples of any of its lights); rs, njobs: the
+// jobs' ray offsets and count, which sample-major order needs to skip the slots past a job's samples
 hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                             float* contrib, unsigned int* queue, const GatherLaunch& gl);
+                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl);
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
-                             uint64_t n, const uint64_t* rs, const float* contrib);
+                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order);
 // Sun map of one sun (sun_map.rs:25-75): frame axes, bounds minimum, texel size and depth.
 struct SunMapParams {
   float side[3], up[3], along[3];

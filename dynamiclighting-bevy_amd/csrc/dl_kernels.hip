@@ -1668,6 +1668,15 @@ __global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restr
   }
 }
 
+// Where sample k of job i of a chunk of n jobs keeps its ray and contribution: job-major (order 0:
+// the job's samples together, from rs[i]) or sample-major (order 1: sample k of every job
+// together, so a wave of jobs writes and reads each sample's slots coalesced).  The any-hit kernel
+// hands out a job's samples together in either order: they share an origin, so a wave's lanes
+// walk the same part of the BVH.
+__device__ __forceinline__ uint64_t ray_slot(uint32_t order, uint64_t job_base, uint64_t i, uint64_t n, uint32_t k) {
+  return order ? (uint64_t)k * n + i : job_base + k;
+}
+
 // Ray offsets of a chunk of jobs: samples per job (n + 1 entries, the last one 0, for a scan).
 __global__ void k_gather_nrays(DevEntries ent, const DevLight* __restrict__ lights, const uint2* __restrict__ jobs, uint64_t n,
                                uint64_t* out) {
@@ -1686,7 +1695,7 @@ __global__ void k_gather_nrays(DevEntries ent, const DevLight* __restrict__ ligh
 // brings if unoccluded (zero when it casts none).  Also counts each mesh's bounce rays.
 __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
-                              BounceParams p, TrigTables trig, float4* rays, float* contrib) {
+                              BounceParams p, TrigTables trig, float4* rays, float* contrib, uint32_t order) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
@@ -1709,7 +1718,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
         for (int j = 0; j < G; ++j) {
           if (k0 + j >= g.samples) continue;
           nr += ray[j] ? 1u : 0u;
-          const uint64_t r = base + k0 + j;
+          const uint64_t r = ray_slot(order, base, i, n, k0 + j);
           rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist[j]);
           rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | (ray[j] ? 2u : 0u)));
           contrib[3 * r] = c[j].x;
@@ -1733,12 +1742,16 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
 // hands out runs of `chunk` rays per wave.  An occluded ray's contribution is set to zero.
 template <int MIN_WAVES>
 __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(DevScene sc, const float4* __restrict__ rays, const uint64_t* __restrict__ total, float* contrib,
-                             unsigned int* queue, uint32_t chunk, uint32_t refill) {
+                             unsigned int* queue, uint32_t chunk, uint32_t refill, uint32_t order,
+                             const uint64_t* __restrict__ rs, uint32_t njobs, uint32_t positions) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
   Tally tl;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t nrays = (uint32_t)*total;  // a chunk holds fewer than 2^32 rays
+  // slots to hand out: the rays of the chunk (job-major), or every (sample, job) slot (sample-major,
+  // where slots past a job's sample count hold no ray)
+  const uint32_t nrays = order ? positions : (uint32_t)*total;  // a chunk holds fewer than 2^32 slots
+  const uint32_t per_job = order ? positions / njobs : 0;       // slots per job (the most samples)
   uint32_t cbase = 0, cend = 0;
   bool drained = false;
   bool has_next = false, live = false;
@@ -1784,9 +1797,17 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (!has_next && rank < avail) {
           nidx = cbase + rank;
-          n0 = rays[2 * (size_t)nidx];
-          n1 = rays[2 * (size_t)nidx + 1];
-          has_next = true;
+          bool slot = true;
+          if (order) {  // hand out a job's samples together, from their sample-major slots
+            const uint32_t job = nidx / per_job, k = nidx - job * per_job;
+            slot = (uint64_t)k < rs[job + 1] - rs[job];
+            nidx = k * njobs + job;
+          }
+          if (slot) {
+            n0 = rays[2 * (size_t)nidx];
+            n1 = rays[2 * (size_t)nidx + 1];
+            has_next = true;
+          }
         }
         const uint32_t want = (uint32_t)__popcll(need);
         cbase += avail < want ? avail : want;
@@ -1819,14 +1840,17 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
 // Samples without a ray and occluded samples hold +0, which leaves the sum unchanged.
 __global__ void k_gather_sum(DevEntries ent, const DevLight* __restrict__ lights, DevTexels tx, DevBounce b,
                              const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
-                             const float* __restrict__ contrib) {
+                             const float* __restrict__ contrib, uint32_t order) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint2 jb = jobs[i];
     const DevLight& L = lights[ent.light[jb.y]];
     const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
-    const float* c = contrib + 3 * rs[i];
+    const uint64_t base = rs[i];
     f3 sum = mk3(0.0f, 0.0f, 0.0f);
-    for (uint32_t k = 0; k < samples; ++k) sum = add(sum, mk3(c[3 * k], c[3 * k + 1], c[3 * k + 2]));
+    for (uint32_t k = 0; k < samples; ++k) {
+      const float* c = contrib + 3 * ray_slot(order, base, i, n, k);
+      sum = add(sum, mk3(c[0], c[1], c[2]));
+    }
     const f3 v = divs(mul(sum, L.bounce_intensity), (float)samples);
     const uint32_t w = b.ent_window[jb.y];
     const uint32_t* wr = b.win_rect + 4 * (size_t)w;
@@ -2080,14 +2104,14 @@ hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* li
 }
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
-                              float4* rays, float* contrib) {
+                              float4* rays, float* contrib, uint32_t order) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather_rays, dim3(grid_for(n, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, b, jobs, n, rs, p, trig,
-                     rays, contrib);
+                     rays, contrib, order);
   return hipGetLastError();
 }
 hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                             float* contrib, unsigned int* queue, const GatherLaunch& gl) {
+                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
   if (max_rays >= (1ull << 31) || gl.chunk == 0 || gl.chunk > (1u << 20)) return hipErrorInvalidValue;
   if (max_rays == 0) return hipSuccess;
   const size_t lds = stack_bytes(scene, gl.block);
@@ -2103,14 +2127,15 @@ hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, 
   if (grid > useful) grid = useful;
   // the 32-bit queue may pass the ray count by one run per wave before every wave has drained
   if (max_rays + (uint64_t)gl.chunk * grid * (gl.block / 64) >= (1ull << 32)) return hipErrorInvalidValue;
+  if (gl.order && (njobs == 0 || njobs >= (1ull << 32))) return hipErrorInvalidValue;
   hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, rays, total, contrib, queue, gl.chunk,
-                     std::max(1u, std::min(64u, gl.refill)));
+                     std::max(1u, std::min(64u, gl.refill)), gl.order ? 1u : 0u, rs, (uint32_t)njobs, (uint32_t)max_rays);
   return hipGetLastError();
 }
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
-                             uint64_t n, const uint64_t* rs, const float* contrib) {
+                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, ent, lights, tx, b, jobs, n, rs, contrib);
+  hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, ent, lights, tx, b, jobs, n, rs, contrib, order);
   return hipGetLastError();
 }
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,

@@ -99,7 +99,8 @@ def load() -> C.CDLL:
     lib.dl_test_batch_trace.argtypes = [P, C.POINTER(abi.dl_timings), C.POINTER(C.c_uint32), C.POINTER(C.c_double),
                                         C.c_uint32]
     lib.dl_set_gather.restype = C.c_int
-    lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32]
     _lib = lib
     return lib
 
@@ -284,10 +285,11 @@ class Context:
             raise BakeError(st, self.error())
 
     def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, chunk_rays: int = 0, blocks_per_cu: int = 0,
-                   min_waves: int = 0, refill: int = 0) -> None:
+                   min_waves: int = 0, refill: int = 0, order: int = 0) -> None:
         """Point-light bounce gather: 0 = one wave per (triangle, light, 64 texels), 1 = split
-        gather (rays, persistent any-hit traversal, ordered sums)."""
-        st = self.lib.dl_set_gather(self.handle, mode, block, chunk, chunk_rays, blocks_per_cu, min_waves, refill)
+        gather (rays, persistent any-hit traversal, ordered sums).  order: 1 job-major ray slots,
+        2 sample-major, 0 keep."""
+        st = self.lib.dl_set_gather(self.handle, mode, block, chunk, chunk_rays, blocks_per_cu, min_waves, refill, order)
         if st != 0:
             raise BakeError(st, self.error())
 

@@ -55,10 +55,11 @@ uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker,
  * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per
  * chunk of jobs (the ray buffer's size), blocks per CU (0 = as many as fit) and the occupancy the
  * any-hit kernel's registers are capped for (8 waves per SIMD, or 0 for no cap), and how many of a
- * wave's lanes must be idle before they take new rays (1-64).  Zero keeps the current value for
- * block, chunk, chunk_rays and refill. */
+ * wave's lanes must be idle before they take new rays (1-64), and the order of the ray slots
+ * (1 job-major: a job's samples together; 2 sample-major: one sample of every job together, the
+ * default).  Zero keeps the current value for block, chunk, chunk_rays, refill and order. */
 int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint64_t chunk_rays, uint32_t blocks_per_cu,
-                  uint32_t min_waves, uint32_t refill);
+                  uint32_t min_waves, uint32_t refill, uint32_t order);
 
 #ifdef __cplusplus
 }

@@ -137,13 +137,13 @@ def test_empty_scenes_match_oracle(ctx):
         assert_scenes_equal(ctx.bake(inp, abi.default_options()), oracle.bake(inp, abi.default_options()))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_bounce_gather_modes_match_oracle(ctx, mode):
+@pytest.mark.parametrize("mode,order", [(0, 0), (1, 1), (1, 2)])
+def test_bounce_gather_modes_match_oracle(ctx, mode, order):
     """Both point-light gathers (one wave per triangle, light and 64 texels; streamed per-lane
-    jobs with light-major job order) against the oracle on a 2x2-room interior with 12 bounce
-    lights, and the bounce edge cases."""
+    jobs with light-major job order, with job-major and sample-major ray slots) against the
+    oracle on a 2x2-room interior with 12 bounce lights, and the bounce edge cases."""
     try:
-        ctx.set_gather(mode)
+        ctx.set_gather(mode, order=order)
         inp, opts = scenes.interior(seed=9, target_triangles=3000, lights=12, grid=2, density=12.0)
         got = ctx.bake(inp, opts)
         assert got.stats["bounce_rays"] > 1_000_000
@@ -153,7 +153,7 @@ def test_bounce_gather_modes_match_oracle(ctx, mode):
                 inp, opts = scene_cases.CASES[name]()
                 assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
     finally:
-        ctx.set_gather(1)
+        ctx.set_gather(1, order=2)
 
 
 def test_outdoor_matches_oracle(ctx):
