@@ -12,6 +12,8 @@ device is missing.
 from __future__ import annotations
 
 import ctypes as C
+
+import numpy as np
 import os
 
 from . import abi
@@ -73,6 +75,17 @@ def load() -> C.CDLL:
     lib.dl_decode_stream.argtypes = [READ_FN, C.c_void_p, C.POINTER(P)]
     lib.dl_format_error_kind.restype = C.c_int
     lib.dl_format_error_kind.argtypes = []
+    lib.dl_scene_buffers.restype = C.c_int
+    lib.dl_scene_buffers.argtypes = [P, C.POINTER(P)]
+    for name in ("dl_buffers_triangles", "dl_buffers_aux"):
+        getattr(lib, name).restype = C.c_uint64
+        getattr(lib, name).argtypes = [P, C.POINTER(C.POINTER(C.c_uint32))]
+    lib.dl_buffers_receiver_count.restype = C.c_uint32
+    lib.dl_buffers_receiver_count.argtypes = [P]
+    lib.dl_buffers_receiver.restype = C.c_int
+    lib.dl_buffers_receiver.argtypes = [P, C.c_uint32, C.POINTER(abi.dl_receiver)]
+    lib.dl_baked_buffers_free.restype = None
+    lib.dl_baked_buffers_free.argtypes = [P]
     lib.dl_baked_scene_free.restype = None
     lib.dl_baked_scene_free.argtypes = [P]
     lib.dl_last_timings.restype = C.c_int
@@ -173,6 +186,41 @@ def decode_stream(source) -> abi.BakedScene:
         lib.dl_baked_scene_free(h)
 
 
+def _buffers_of_handle(lib, h) -> dict:
+    """scene_buffers (src/baked_lighting/apply.rs:27-38) of a library scene: the triangles and aux
+    word arrays and the receivers in mesh order (tag = index + 1)."""
+    b = C.c_void_p()
+    st = lib.dl_scene_buffers(h, C.byref(b))
+    if st != 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    try:
+        out = {}
+        for name in ("triangles", "aux"):
+            p = C.POINTER(C.c_uint32)()
+            n = getattr(lib, f"dl_buffers_{name}")(b, C.byref(p))
+            out[name] = np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+        recs = []
+        for i in range(lib.dl_buffers_receiver_count(b)):
+            r = abi.dl_receiver()
+            lib.dl_buffers_receiver(b, i, C.byref(r))
+            recs.append({"id": r.id, "content_hash": r.content_hash, "tag": r.tag, "triangle_count": r.triangle_count,
+                         "occludes": bool(r.occludes)})
+        out["receivers"] = recs
+        return out
+    finally:
+        lib.dl_baked_buffers_free(b)
+
+
+def scene_buffers_of_file(data: bytes) -> dict:
+    """The BakedBuffers of a decoded .dynlight file (decode_stream then take_scene_buffers)."""
+    lib = load()
+    h = _decode_handle(lib, bytes(data))
+    try:
+        return _buffers_of_handle(lib, h)
+    finally:
+        lib.dl_baked_scene_free(h)
+
+
 def reencode(data: bytes) -> bytes:
     """decode followed by encode (src/bake/format/mod.rs:101-119) of the decoded scene."""
     lib = load()
@@ -267,6 +315,18 @@ class Context:
             raise BakeError(st, self.error())
         try:
             return abi.read_scene(self.lib, "dl_", h), _encode_handle(self.lib, h)
+        finally:
+            self.lib.dl_baked_scene_free(h)
+
+    def bake_buffers(self, inp: BakeInput, options: abi.dl_bake_options | None = None) -> tuple[abi.BakedScene, dict]:
+        """bake() and scene_buffers() of its result."""
+        options = options if options is not None else default_options()
+        h = C.c_void_p()
+        st = self.lib.dl_bake(self.handle, C.byref(inp.struct), C.byref(options), None, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.error())
+        try:
+            return abi.read_scene(self.lib, "dl_", h), _buffers_of_handle(self.lib, h)
         finally:
             self.lib.dl_baked_scene_free(h)
 

@@ -81,6 +81,11 @@ class dl_bake_input(C.Structure):
                 ("light_count", C.c_uint32)]
 
 
+class dl_receiver(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("content_hash", C.c_uint64), ("tag", C.c_uint32), ("triangle_count", C.c_uint32),
+                ("occludes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class dl_service_request(C.Structure):
     _fields_ = [("lights", C.c_void_p), ("light_count", C.c_uint32), ("arrival_s", C.c_double)]
 

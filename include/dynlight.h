@@ -232,6 +232,26 @@ int dl_decode(const uint8_t* bytes, uint64_t len, dl_baked_scene** out);
 int dl_decode_stream(dl_read_fn read, void* user, dl_baked_scene** out);
 int dl_format_error_kind(void);
 
+/* ---- BakedBuffers (src/baked_lighting/layout.rs:1-215, apply.rs:27-98) ----------------------
+ * The `triangles` and `aux` word buffers the runtime's shaders read, and the receiver lookup
+ * (BakeId -> Receiver), built from a baked scene as scene_buffers does.  Receivers are listed in
+ * the scene's mesh order: receiver i has tag i + 1 (MeshTag 0 means "not baked"). */
+typedef struct dl_receiver {
+    uint64_t id;
+    uint64_t content_hash;
+    uint32_t tag;
+    uint32_t triangle_count;
+    uint32_t occludes;
+    uint32_t reserved;
+} dl_receiver;
+typedef struct dl_baked_buffers dl_baked_buffers;
+int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out);
+uint64_t dl_buffers_triangles(const dl_baked_buffers* buffers, const uint32_t** words);
+uint64_t dl_buffers_aux(const dl_baked_buffers* buffers, const uint32_t** words);
+uint32_t dl_buffers_receiver_count(const dl_baked_buffers* buffers);
+int dl_buffers_receiver(const dl_baked_buffers* buffers, uint32_t index, dl_receiver* out);
+void dl_baked_buffers_free(dl_baked_buffers* buffers);
+
 /* ---- rebake service (no counterpart in the reference, which re-runs bake() on every change) ----
  * A service keeps one scene's geometry resident on the device: receiver meshes, the occluder BVH
  * and every triangle's unburied texels (none of which depend on the lights).  A rebake then bakes

@@ -82,3 +82,59 @@ def test_baked_scenes_round_trip_through_the_file(ctx, name):
     assert dynlight_hip.reencode(data) == data, "decode then encode gives the same bytes"
     # a file written by another gzip implementation reads the same
     same_scene(dynlight_hip.decode(file_of(unpacked(data))), scene)
+
+
+def py_buffers(s):
+    """BufferBuilder (src/baked_lighting/layout.rs:148-214) over a BakedScene, written here."""
+    tris, recs = [], []
+    for m in s.meshes:
+        words = [int(w) for w in m.triangles]
+        recs += [len(tris), m.resolution, words[0] // 4 if words else 0, 0]
+        tris += words
+    tris = tris or [0]
+    aux = [0] * 12
+    aux[0], aux[1] = len(aux), len(recs) // 4
+    aux += recs
+    bvh = [int(w) for w in np.asarray(s.light_bvh).reshape(-1)]
+    aux[2], aux[3] = len(aux), len(bvh) // 8
+    aux += bvh
+    local = len(s.lights)
+    while local and s.lights[local - 1]["shape"] in (3, 4):
+        local -= 1
+    aux[4], aux[5] = len(aux), (local if len(s.distance_cubes) else 0)
+    aux += [int(w) for w in s.distance_cubes]
+    st = s.settings
+    aux[6] = (1 if st["bounce"] else 0) | {abi.DYNAMIC_LIGHTING_ONLY: 0, abi.DYNAMIC_DISTANCE_CUBES: 2,
+                                            abi.DYNAMIC_ANGULAR: 4}[st["dynamic_geometry"]]
+    aux[7] = len(s.lights)
+    aux[8], aux[9] = len(aux), len(s.sun_maps)
+    texels = len(aux) + 16 * len(s.sun_maps)
+    for m in s.sun_maps:
+        aux += list(np.asarray(m["rows"], np.float32).view(np.uint32)) + [m["resolution"], texels, 0, 0]
+        texels += len(m["texels"])
+    for m in s.sun_maps:
+        aux += [int(w) for w in m["texels"]]
+    if s.sky_probes is not None:
+        p = s.sky_probes
+        aux[10] = len(aux)
+        aux += list(np.asarray(list(p["origin"]) + [p["spacing"]], np.float32).view(np.uint32)) + list(p["counts"])
+        aux.append(len(aux) + 1)
+        aux += [int(w) for w in p["words"]]
+    return tris, aux
+
+
+@pytest.mark.parametrize("name", ["box_room", "outdoor"])
+def test_scene_buffers_of_a_bake(ctx, name):
+    if name == "box_room":
+        inp, opts = scenes.box_room(1)
+    else:
+        inp, opts = scenes.outdoor(nx=48, nz=32, buildings=12, area_lights=6, density=2.0)
+        opts.sun_map_resolution = 128
+        opts.sky_samples = 16
+        opts.sky_probe_spacing = 4.0
+    scene, b = ctx.bake_buffers(inp, opts)
+    tris, aux = py_buffers(scene)
+    assert [int(w) for w in b["triangles"]] == tris
+    assert [int(w) for w in b["aux"]] == aux
+    assert [r["id"] for r in b["receivers"]] == [m.id for m in scene.meshes]
+    assert [r["tag"] for r in b["receivers"]] == list(range(1, len(scene.meshes) + 1))
