@@ -280,6 +280,58 @@ def outdoor_main(args):
     print(json.dumps(out), flush=True)
 
 
+def box_main(args):
+    """The CPU-runnable case: a closed 10x4x10 m room (12 triangles) with 2 seeded boxes, 4 point
+    lights of range 6-12 m, 16 texels/m, 1-bit shadows, no bounce.  The oracle runs the whole bake
+    too, on every host thread and on one (best of 3 each)."""
+    import oracle
+    import dynlight_hip
+    from dynlight_hip import scenes
+
+    inp, opts = scenes.box_room(1)
+    ctx = dynlight_hip.Context(0)
+    for _ in range(max(1, args.warmup)):
+        ctx.bake(inp, opts)
+    steps = max(args.steps, 20)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = ctx.bake(inp, opts)
+    wall = time.perf_counter() - t0
+    samples = s.stats["samples"]
+    threads = len(os.sched_getaffinity(0))
+    cpu = {}
+    for n in (threads, 1):
+        best = None
+        for _ in range(3):
+            t1 = time.perf_counter()
+            oracle.bake(inp, opts, threads=n)
+            dt = time.perf_counter() - t1
+            best = dt if best is None else min(best, dt)
+        cpu[n] = best
+    out = {
+        "metric": "texel-light visibility samples/s (1 GPU, box room)",
+        "value": samples * steps / wall,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": 1000.0 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "box room: 10x4x10 m, 2 boxes, 4 point lights r 6-12 m, 16 texels/m, 1-bit, no bounce",
+                   **s.stats},
+        "cpu_baseline": {"value": samples / cpu[threads], "unit": "samples/s", "cores": threads, "kind": "port",
+                         "sample": f"the whole bake on the CPU oracle, best of 3: {cpu[threads] * 1000:.1f} ms on {threads} "
+                                   f"threads, {cpu[1] * 1000:.1f} ms on 1 thread"},
+        "cpu_one_thread_samples_per_s": samples / cpu[1],
+    }
+    out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,9 +340,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=("interior", "batch", "serve", "outdoor"), default="interior",
+    ap.add_argument("--workload", choices=("interior", "batch", "serve", "outdoor", "box"), default="interior",
                     help="interior: the 200k-triangle interior bake; batch: 512 independent small bakes; "
-                         "serve: light rebakes of the resident interior; outdoor: the 1M-triangle outdoor bake")
+                         "serve: light rebakes of the resident interior; outdoor: the 1M-triangle outdoor bake; "
+                         "box: the CPU-runnable box room")
     ap.add_argument("--requests", type=int, default=400, help="serve workload: requests (80%% are light requests)")
     ap.add_argument("--workers", type=int, default=4, help="serve workload: worker streams")
     ap.add_argument("--rate", type=float, default=20.0, help="serve workload: request arrivals per second")
@@ -320,6 +373,9 @@ def main():
         return
     if args.workload == "outdoor":
         outdoor_main(args)
+        return
+    if args.workload == "box":
+        box_main(args)
         return
 
     import dynlight_hip

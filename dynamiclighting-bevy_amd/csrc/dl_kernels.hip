@@ -18,14 +18,25 @@ static constexpr float kBoxEps = 1.0f / 65536.0f;
 static constexpr int kGatherGroup = DL_GATHER_GROUP;  // samples generated together per job
 static constexpr uint32_t kThreads = 256;
 
-__device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ start, uint64_t n, uint64_t idx) {
-  uint64_t lo = 0, hi = n;
+__device__ __forceinline__ uint64_t owner_in(const uint64_t* __restrict__ start, uint64_t lo, uint64_t hi, uint64_t idx) {
   while (hi - lo > 1) {
     uint64_t mid = (lo + hi) >> 1;
     if (start[mid] <= idx) lo = mid;
     else hi = mid;
   }
   return lo;
+}
+// The segment s < n with start[s] <= idx < start[s + 1] (the last such s when segments are empty).
+__device__ __forceinline__ uint64_t owner_of(const uint64_t* __restrict__ start, uint64_t n, uint64_t idx) {
+  return owner_in(start, 0, n, idx);
+}
+// owner_of for grid-stride loops whose lanes hold increasing indices: one search per wave, for its
+// first active lane, and a search of its own only for a lane past that segment.
+__device__ __forceinline__ uint64_t owner_of_wave(const uint64_t* __restrict__ start, uint64_t n, uint64_t idx) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)idx), hi = __builtin_amdgcn_readfirstlane((uint32_t)(idx >> 32));
+  const uint64_t g = owner_in(start, 0, n, ((uint64_t)hi << 32) | lo);
+  if (g + 1 >= n || idx < start[g + 1]) return g;
+  return owner_in(start, g + 1, n, idx);
 }
 
 // Reciprocal of a direction component for the slab tests (hardware reciprocal, about 1 ulp: the
@@ -333,7 +344,7 @@ __global__ void k_assoc_fill(DevTris tr, DevMeshes ms, const DevLight* __restric
 // ---- owned texels: mesh_texels.rs:47-55 over triangle_texels (texels.rs:48-72) ------------------
 __global__ void k_owned(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ start, uint64_t total) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t t = (uint32_t)owner_of(start, tr.n, c);
+    uint32_t t = (uint32_t)owner_of_wave(start, tr.n, c);
     uint64_t local = c - start[t];
     uint32_t m = tr.mesh[t], res = ms.res[m];
     f2 uv[3];
@@ -354,7 +365,7 @@ __global__ void k_texels(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ 
                          const uint64_t* __restrict__ start, uint64_t total, int fill, unsigned long long* counts,
                          const uint64_t* __restrict__ tex_start, DevTexels out) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t t = (uint32_t)owner_of(start, tr.n, c);
+    uint32_t t = (uint32_t)owner_of_wave(start, tr.n, c);
     uint64_t local = c - start[t];
     uint32_t m = tr.mesh[t], res = ms.res[m];
     f3 v[3], n[3];
@@ -759,7 +770,7 @@ __device__ __forceinline__ bool visited_at(const DevMeshes& ms, uint32_t m, uint
 __global__ void k_pad_seams(DevMeshes ms, DevRegions rg, const uint32_t* __restrict__ used, const uint64_t* __restrict__ start,
                             uint32_t nused, uint64_t total) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t u = (uint32_t)owner_of(start, nused, c);
+    uint32_t u = (uint32_t)owner_of_wave(start, nused, c);
     uint32_t r = used[u];
     uint64_t local = c - start[u];
     const uint32_t* rc = rg.rect + 4 * (size_t)r;
@@ -816,7 +827,7 @@ __global__ void k_crop(DevTris tr, DevEntries ent, DevRegions rg, const uint64_t
                        const uint64_t* __restrict__ word_start, uint64_t nkept, uint64_t total,
                        const uint64_t* __restrict__ kept_out, uint32_t* out) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t j = owner_of(word_start, nkept, c);
+    uint64_t j = owner_of_wave(word_start, nkept, c);
     uint64_t wi = c - word_start[j];
     uint64_t e = kept_entry[j];
     uint32_t t = ent.tri[e], r = ent.region[e];
@@ -1879,7 +1890,7 @@ __global__ void k_bounce_filter(DevBounce b, const uint32_t* __restrict__ win_re
   const float* src = pass == 1 ? b.map_b : b.map_a;
   float* dst = pass == 1 ? b.map_a : b.map_b;
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t w = (uint32_t)owner_of(win_start, nwin, c);
+    uint32_t w = (uint32_t)owner_of_wave(win_start, nwin, c);
     uint64_t local = c - win_start[w];
     const uint32_t* wr = b.win_rect + 4 * (size_t)w;
     uint32_t ww = wr[2] - wr[0] + 1;
@@ -1985,7 +1996,7 @@ __global__ void k_bounce_pack_words(DevTris tr, DevEntries ent, DevBounce b, con
                                     const uint64_t* __restrict__ kept_out, const int32_t* __restrict__ comp_of_slot,
                                     uint32_t colored, uint32_t* out) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t j = owner_of(word_start, nkept, c);
+    uint64_t j = owner_of_wave(word_start, nkept, c);
     uint64_t wi = c - word_start[j];
     uint64_t e = kept_entry[j];
     PackView pv = pack_view(tr, ent, b, e);
