@@ -646,9 +646,88 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     tp.sky_samples = opt.sky_samples;
     DevScene sct = sc;
     sct.counters = ctx->counting ? d_counters.p : nullptr;
+    // Lights with several rays per texel (area lights, soft suns, skies) are streamed: their
+    // (texel, entry) jobs, light by light, go through the persistent any-hit kernel.
+    std::vector<uint32_t> vis_me;
+    std::vector<uint64_t> vis_js(1, 0);
+    uint32_t vis_slots_a = 0;
+    if (ctx->trace_stream) {
+      std::vector<std::pair<uint32_t, uint32_t>> multi;  // (light slot, entry) in entry order
+      for (uint32_t t = 0; t < nbt; ++t) {
+        if (h_tex_start[t + 1] == h_tex_start[t]) continue;
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) {
+          const DevLight& L = hlights[b_ent_light[e]];
+          if (L.shape == 1 || L.shape == 2 || L.shape == 4 || (L.shape == 3 && L.srad > 0.0f))
+            multi.push_back({b_ent_light[e], (uint32_t)e});
+        }
+      }
+      std::stable_sort(multi.begin(), multi.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+      for (const auto& le : multi) {
+        const uint32_t t = b_ent_tri[le.second];
+        vis_me.push_back(le.second);
+        vis_js.push_back(vis_js.back() + (h_tex_start[t + 1] - h_tex_start[t]));
+        const DevLight& L = hlights[le.first];
+        const uint32_t probes = L.shape == 4 ? 0u : (L.shape == 2 ? 3u : 5u);
+        const uint32_t slots = L.shape == 4 ? tp.sky_samples : probes + std::min(tp.area_min, tp.area_samples);
+        vis_slots_a = std::max(vis_slots_a, slots);
+      }
+      if (vis_js.back() > 0) tp.stream_multi = 1;
+    }
     HIP_TRY(hipEventRecord(ev0, s));
     HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nbt], tp, ctx->trig,
                          ctx->trace_block));
+    if (tp.stream_multi) {
+      const uint64_t njobs = vis_js.back();
+      const uint32_t nme = (uint32_t)vis_me.size();
+      const uint32_t slots_b = tp.area_samples > tp.area_min ? tp.area_samples - tp.area_min : 0u;
+      const uint64_t slots = std::max<uint64_t>(1, std::max(vis_slots_a, slots_b));
+      const uint64_t cn = std::min<uint64_t>(njobs, std::max<uint64_t>(1, ctx->gather.chunk_rays / slots));
+      DBuf<uint64_t> d_vjs, d_vcnt, d_vrs, d_vneed, d_vbscan;
+      DBuf<uint32_t> d_vme, d_vblist;
+      DBuf<unsigned int> d_vq;
+      DBuf<float4> d_vrays;
+      DBuf<float> d_vcontrib, d_vwts;
+      DBuf<VisState> d_vstate;
+      HIP_TRY(upload(ctx, d_vjs, vis_js));
+      HIP_TRY(upload(ctx, d_vme, vis_me));
+      HIP_TRY(d_vcnt.alloc(cn + 1));
+      HIP_TRY(d_vrs.alloc(cn + 1));
+      HIP_TRY(d_vneed.alloc(cn + 1));
+      HIP_TRY(d_vbscan.alloc(cn + 1));
+      HIP_TRY(d_vblist.alloc(cn + 1));
+      HIP_TRY(d_vq.alloc(1));
+      HIP_TRY(d_vrays.alloc(2 * cn * slots));
+      HIP_TRY(d_vcontrib.alloc(3 * cn * slots));
+      HIP_TRY(d_vwts.alloc(cn * slots));
+      HIP_TRY(d_vstate.alloc(cn + 1));
+      GatherLaunch vgl = ctx->gather;
+      vgl.order = 1;
+      for (uint64_t q0 = 0; q0 < njobs; q0 += cn) {
+        const uint64_t n = std::min(cn, njobs - q0);
+        HIP_TRY(launch_vis_count(s, ent, d_lights.p, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, d_vcnt.p));
+        HIP_TRY(scan_u64(s, d_vcnt.p, d_vrs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+        HIP_TRY(launch_vis_rays_a(s, tr, ms, d_lights.p, tx, ent, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, ctx->trig,
+                                  d_vrays.p, d_vcontrib.p, d_vwts.p, d_vstate.p));
+        HIP_TRY(hipMemsetAsync(d_vq.p, 0, sizeof(unsigned int), s));
+        HIP_TRY(launch_any_stream(s, sct, d_vrays.p, nullptr, n * vis_slots_a, d_vrs.p, n, d_vcontrib.p, d_vq.p, vgl));
+        HIP_TRY(launch_vis_mid(s, ent, d_lights.p, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, d_vrays.p, d_vcontrib.p,
+                               d_vwts.p, d_vstate.p, d_vneed.p));
+        HIP_TRY(scan_u64(s, d_vneed.p, d_vbscan.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+        uint64_t nb = 0;
+        HIP_TRY(hipMemcpyAsync(&nb, d_vbscan.p + n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (nb > n) { ctx->err = "streamed visibility: bad phase B count"; return DL_E_DEVICE; }
+        if (nb) {
+          HIP_TRY(launch_vis_list(s, d_vneed.p, d_vbscan.p, n, d_vblist.p));
+          HIP_TRY(launch_vis_rays_b(s, tr, ms, d_lights.p, tx, ent, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, d_vblist.p, nb, tp,
+                                    ctx->trig, d_vrays.p, d_vcontrib.p, d_vwts.p));
+          HIP_TRY(hipMemsetAsync(d_vq.p, 0, sizeof(unsigned int), s));
+          HIP_TRY(launch_any_stream(s, sct, d_vrays.p, nullptr, nb * slots_b, nullptr, nb, d_vcontrib.p, d_vq.p, vgl));
+        }
+        HIP_TRY(launch_vis_finish(s, tr, ms, d_lights.p, tx, ent, rg, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, d_vstate.p,
+                                  d_vbscan.p, nb, d_vrays.p, d_vcontrib.p, d_vwts.p));
+      }
+    }
     HIP_TRY(hipEventRecord(ev1, s));
     HIP_TRY(hipStreamSynchronize(s));
     float kms = 0.0f;

@@ -122,6 +122,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     // smaller ray chunks, so that each worker's gather buffers stay within its cache
     w->gather.chunk_rays = std::max<uint64_t>(1ull << 22, ctx->gather.chunk_rays / streams);
     w->gather_mode = ctx->gather_mode;
+    w->trace_stream = ctx->trace_stream;
     w->counting = false;
   }
   return true;
@@ -615,6 +616,13 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (trace_block) ctx->trace_block = trace_block;
   if (burial_block) ctx->burial_block = burial_block;
+  return DL_OK;
+}
+
+int dl_set_trace_stream(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->trace_stream = enable ? 1u : 0u;
   return DL_OK;
 }
 

@@ -159,6 +159,7 @@ struct dl_ctx {
   uint32_t burial_block = 128;
   dl::GatherLaunch gather{};
   uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
+  uint32_t trace_stream = 1;  // 1: area lights, soft suns and skies through the streamed any-hit path
   unsigned host_threads = 1;
   bool counting = false;
   // batched bakes: worker contexts with their own streams (sharing this context's sin/cos

@@ -115,7 +115,33 @@ hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint
 struct TraceParams {
   uint64_t seed;
   uint32_t area_samples, area_min, sky_samples;
+  uint32_t stream_multi = 0;  // 1: k_trace leaves area lights, soft suns and skies to the k_vis_* kernels
 };
+// Streamed multi-ray visibility (area lights, soft suns, skies).  A job is one (texel, entry) of
+// such a light: job q of the batch is texel q - js[k] of entry me[k] (js: nme + 1 prefix counts of
+// the entries' texels).  A chunk holds jobs q0 .. q0 + n - 1.  The running share of a job:
+struct VisState {
+  float total, lit;
+  uint32_t nr, flags;  // bit 0 any ray lit, 1 any ray blocked, 2 texel in reach, 3 needs phase B
+};
+hipError_t launch_vis_count(hipStream_t s, DevEntries ent, const DevLight* lights, const uint64_t* js, const uint32_t* me,
+                            uint32_t nme, const uint64_t* tex_start, uint64_t q0, uint64_t n, TraceParams p, uint64_t* out);
+hipError_t launch_vis_rays_a(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start, uint64_t q0,
+                             uint64_t n, TraceParams p, TrigTables trig, float4* rays, float* contrib, float* wts,
+                             VisState* state);
+hipError_t launch_vis_mid(hipStream_t s, DevEntries ent, const DevLight* lights, const uint64_t* js, const uint32_t* me,
+                          uint32_t nme, const uint64_t* tex_start, uint64_t q0, uint64_t n, TraceParams p, const float4* rays,
+                          const float* contrib, const float* wts, VisState* state, uint64_t* need);
+hipError_t launch_vis_list(hipStream_t s, const uint64_t* need, const uint64_t* scan, uint64_t n, uint32_t* blist);
+hipError_t launch_vis_rays_b(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start, uint64_t q0,
+                             const uint32_t* blist, uint64_t nb, TraceParams p, TrigTables trig, float4* rays, float* contrib,
+                             float* wts);
+hipError_t launch_vis_finish(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             DevRegions rg, const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start,
+                             uint64_t q0, uint64_t n, TraceParams p, const VisState* state, const uint64_t* bscan, uint64_t nb,
+                             const float4* rays, const float* contrib, const float* wts);
 hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights,
                         DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevRegions reg,
                         const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block);

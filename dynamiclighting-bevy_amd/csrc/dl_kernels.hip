@@ -642,6 +642,11 @@ __device__ __forceinline__ uint32_t region_level(const DevRegions& rg, uint32_t 
 
 // The (triangle x light x texel) loop of mesh_bake.rs:203-267; one wave per (triangle, light,
 // 64-texel chunk).
+// Lights that cast several rays per texel: area lights, soft suns, skies.
+__device__ __forceinline__ bool vis_multi(const DevLight& L) {
+  return L.shape == 1 || L.shape == 2 || L.shape == 4 || (L.shape == 3 && L.srad > 0.0f);
+}
+
 __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
                         const uint64_t* __restrict__ tex_start, DevEntries ent, DevRegions rg,
                         const uint64_t* __restrict__ unit_start, uint64_t units, TraceParams p, TrigTables trig) {
@@ -690,7 +695,7 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
             }
           }
         }
-      } else {
+      } else if (!(p.stream_multi && vis_multi(L))) {  // streamed lights are traced by the k_vis_* kernels
         Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
         float fraction = 0.0f;
         if (L.shape == 1 || L.shape == 2) {
@@ -730,6 +735,318 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
       wave_add(&sc.counters[0], tl.nodes);
       wave_add(&sc.counters[1], tl.tris);
       tl = Tally{};
+    }
+  }
+}
+
+// ---- streamed multi-ray visibility: area lights, soft suns, skies -----------------------------
+// The same results as k_trace's area_visibility, sun_visibility and sky branches, reorganised so
+// the traversal runs in the persistent any-hit kernel.  A job is one (texel, light entry) pair of
+// a light that casts several rays per texel.  Phase A writes every job's probe rays and its first
+// min(minimum, samples) sample rays (sample-major slots; a slot whose ray fails the receiver or
+// emitter test holds no ray), k_any_stream clears the "lit" word of every occluded ray, and
+// k_vis_mid folds them in order into the job's running share.  Jobs still undecided after
+// `minimum` samples (area.rs:82-88, sun.rs:47-53) get phase B: the remaining samples, traced the
+// same way; k_vis_finish folds those in and writes the level, the visited bit and the stats.
+// Sums run over the samples in their order, so every f32 result equals the per-lane loop's.
+
+struct VisJob {
+  uint32_t e, t;
+  uint64_t j;
+};
+// Job q: entry me[k] for the k with js[k] <= q < js[k + 1], texel q - js[k] of its triangle.
+__device__ __forceinline__ VisJob vis_job(const uint64_t* __restrict__ js, const uint32_t* __restrict__ me, uint32_t nme,
+                                          const DevEntries& ent, const uint64_t* __restrict__ tex_start, uint64_t q) {
+  const uint32_t k = (uint32_t)owner_of_wave(js, nme, q);
+  VisJob v;
+  v.e = me[k];
+  v.t = ent.tri[v.e];
+  v.j = tex_start[v.t] + (q - js[k]);
+  return v;
+}
+
+// Probe rays before the samples: 5 for rectangles and soft suns, 3 for tubes, none for skies.
+__device__ __forceinline__ uint32_t vis_probes(const DevLight& L) {
+  return L.shape == 4 ? 0u : (L.shape == 2 ? 3u : 5u);
+}
+// Phase A slots of a job: its probes and min(minimum, samples) samples; a sky's samples.
+__device__ __forceinline__ uint32_t vis_slots_a(const DevLight& L, const TraceParams& p) {
+  if (L.shape == 4) return p.sky_samples;
+  return vis_probes(L) + (p.area_min < p.area_samples ? p.area_min : p.area_samples);
+}
+
+struct VisRay {
+  bool ray;
+  f3 dir;
+  float tmax, weight;
+};
+// Ray number k of a job (probes first, then samples k - probes), as area_visibility, sun_visibility
+// and the sky loop build it.  `offset` is the job's spread offset.
+__device__ __forceinline__ VisRay vis_ray(const DevLight& L, const TrigTables& trig, f3 pos, f3 nrm, f2 offset, uint32_t k) {
+  VisRay r{false, mk3(0.0f, 0.0f, 0.0f), 0.0f, 0.0f};
+  if (L.shape == 4) {
+    r.dir = cosine_dir(trig, nrm, spread_pt(k, offset, true));
+    r.ray = true;
+    r.tmax = INFINITY;
+    r.weight = 1.0f;
+    return r;
+  }
+  const uint32_t probes = vis_probes(L);
+  if (L.shape == 3) {
+    f2 disk;
+    if (k < probes) {
+      const float px[5] = {0.0f, 1.0f, 0.0f, -1.0f, 0.0f}, py[5] = {0.0f, 0.0f, 1.0f, 0.0f, -1.0f};
+      disk = mk2(px[k], py[k]);
+    } else {
+      disk = concentric_disk(trig, spread_pt(k - probes, offset, true));
+    }
+    const f3 toward = ld3(L.toward);
+    f3 side, up;
+    orthonormal_pair(toward, &side, &up);
+    const f3 dir = normalize(add(toward, mul(add(mul(side, disk.x), mul(up, disk.y)), L.stan)));
+    const float receiver = dot(nrm, dir) - kNdotlTolerance;
+    if (receiver <= 0.0f) return r;
+    r.ray = true;
+    r.dir = dir;
+    r.tmax = INFINITY;
+    r.weight = receiver;
+    return r;
+  }
+  f2 uv;
+  if (k < probes) {
+    const float rx[5] = {0.0f, 1.0f, 1.0f, 0.0f, 0.5f}, ry[5] = {0.0f, 0.0f, 1.0f, 1.0f, 0.5f};
+    const float tx_[3] = {0.0f, 1.0f, 0.5f};
+    uv = L.rect ? mk2(rx[k], ry[k]) : mk2(tx_[k], 0.5f);
+  } else {
+    uv = spread_pt(k - probes, offset, L.rect != 0);
+  }
+  const f3 c = ld3(L.pos), ax = ld3(L.ax), ay = ld3(L.ay), fw = ld3(L.af);
+  const float ox = (uv.x * 2.0f - 1.0f) * L.hx, oy = (uv.y * 2.0f - 1.0f) * L.hy;
+  const f3 point = add(add(c, mul(ax, ox)), mul(ay, oy));
+  const f3 to_point = sub(point, pos);
+  const float dsq = len_sq(to_point);
+  if (dsq < 1.0e-12f) return r;
+  const float distance = sqrtf(dsq);
+  const f3 dir = divs(to_point, distance);
+  const float receiver = dot(nrm, dir) - kNdotlTolerance;
+  const float emitter = L.rect ? -dot(fw, dir) : 1.0f;
+  if (!(receiver > 0.0f && emitter > 0.0f)) return r;
+  r.ray = true;
+  r.dir = dir;
+  r.tmax = maxf_(distance - kEndGap, 0.0f);
+  r.weight = (receiver * emitter) / dsq;
+  return r;
+}
+
+// area_visibility's reach test (area.rs:61-70): false when the texel is out of the light's reach.
+__device__ __forceinline__ bool vis_reach(const DevLight& L, f3 pos) {
+  if (L.shape != 1 && L.shape != 2) return true;
+  const f3 c = ld3(L.pos), ax = ld3(L.ax), ay = ld3(L.ay), fw = ld3(L.af);
+  const f3 local = sub(pos, c);
+  const float npx = clampf_(dot(local, ax), -L.hx, L.hx), npy = clampf_(dot(local, ay), -L.hy, L.hy);
+  const f3 nearest = add(add(c, mul(ax, npx)), mul(ay, npy));
+  if (L.reach <= 0.0f || len_sq(sub(pos, nearest)) > L.reach * L.reach) return false;
+  if (L.rect && dot(fw, sub(pos, c)) <= 0.0f) return false;
+  return true;
+}
+
+struct VisTexel {
+  f3 pos, nrm, origin;
+  uint32_t x, y, m;
+  f2 offset;
+};
+__device__ __forceinline__ VisTexel vis_texel(const DevTris& tr, const DevMeshes& ms, const DevTexels& tx, const DevLight& L,
+                                              uint64_t seed, uint32_t t, uint64_t j) {
+  VisTexel v;
+  const float4 p4 = tx.pos[j], n4 = tx.nrm[j];
+  v.pos = mk3(p4.x, p4.y, p4.z);
+  v.nrm = mk3(n4.x, n4.y, n4.z);
+  v.origin = add(v.pos, mul(ld3(tr.face + 3 * (size_t)t), kRayOffset));
+  const uint32_t xy = tx.xy[j];
+  v.x = xy & 0xffffu;
+  v.y = xy >> 16;
+  v.m = tr.mesh[t];
+  Pcg rng = stream5(seed, ms.id[v.m], L.id, v.x, v.y);
+  v.offset.x = pcg_f32(rng);  // drawn after the probes, which take no random numbers
+  v.offset.y = pcg_f32(rng);
+  return v;
+}
+
+
+__global__ void k_vis_count(DevEntries ent, const DevLight* __restrict__ lights, const uint64_t* __restrict__ js,
+                            const uint32_t* __restrict__ me, uint32_t nme, const uint64_t* __restrict__ tex_start,
+                            uint64_t q0, uint64_t n, TraceParams p, uint64_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      out[i] = 0;
+      continue;
+    }
+    const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + i);
+    out[i] = vis_slots_a(lights[ent.light[v.e]], p);
+  }
+}
+
+// Phase A rays.  Slot of ray k of job i: k * n + i.  contrib[3 * slot] is 1 for a traced ray (the
+// any-hit kernel clears it when the ray is blocked); wts[slot] is its sample weight.
+__global__ void k_vis_rays_a(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* __restrict__ js, const uint32_t* __restrict__ me, uint32_t nme,
+                             const uint64_t* __restrict__ tex_start, uint64_t q0, uint64_t n, TraceParams p, TrigTables trig,
+                             float4* rays, float* contrib, float* wts, VisState* state) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + i);
+    const DevLight& L = lights[ent.light[v.e]];
+    const VisTexel vt = vis_texel(tr, ms, tx, L, p.seed, v.t, v.j);
+    const bool reach = vis_reach(L, vt.pos);
+    const uint32_t slots = vis_slots_a(L, p);
+    const uint32_t alpha = L.alpha ? 1u : 0u;
+    for (uint32_t k = 0; k < slots; ++k) {
+      const uint64_t slot = (uint64_t)k * n + i;
+      VisRay r{false, mk3(0.0f, 0.0f, 0.0f), 0.0f, 0.0f};
+      if (reach) r = vis_ray(L, trig, vt.pos, vt.nrm, vt.offset, k);
+      rays[2 * slot] = make_float4(vt.origin.x, vt.origin.y, vt.origin.z, r.tmax);
+      rays[2 * slot + 1] = make_float4(r.dir.x, r.dir.y, r.dir.z, __uint_as_float(alpha | (r.ray ? 2u : 0u)));
+      contrib[3 * slot] = r.ray ? 1.0f : 0.0f;
+      contrib[3 * slot + 1] = 0.0f;
+      contrib[3 * slot + 2] = 0.0f;
+      wts[slot] = r.weight;
+    }
+    state[i] = VisState{0.0f, 0.0f, 0u, reach ? 4u : 0u};
+  }
+}
+
+// Folds ray k of a job into its share (the `trace` closure of area_visibility / sun_visibility).
+__device__ __forceinline__ void vis_fold(VisState& s, bool ray, bool lit, bool probe, float w) {
+  if (!ray) return;
+  s.nr++;
+  if (!probe) {
+    s.total += w;
+    s.lit += lit ? w : 0.0f;
+  }
+  s.flags |= lit ? 1u : 2u;
+}
+
+__global__ void k_vis_mid(DevEntries ent, const DevLight* __restrict__ lights, const uint64_t* __restrict__ js,
+                          const uint32_t* __restrict__ me, uint32_t nme, const uint64_t* __restrict__ tex_start, uint64_t q0,
+                          uint64_t n, TraceParams p, const float4* __restrict__ rays, const float* __restrict__ contrib,
+                          const float* __restrict__ wts, VisState* state, uint64_t* need) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      need[i] = 0;
+      continue;
+    }
+    const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + i);
+    const DevLight& L = lights[ent.light[v.e]];
+    VisState s = state[i];
+    const uint32_t slots = vis_slots_a(L, p), probes = vis_probes(L);
+    for (uint32_t k = 0; k < slots; ++k) {
+      const uint64_t slot = (uint64_t)k * n + i;
+      const bool ray = (__float_as_uint(rays[2 * slot + 1].w) & 2u) != 0u;
+      vis_fold(s, ray, contrib[3 * slot] > 0.0f, k < probes, wts[slot]);
+    }
+    // area.rs:82-88: after `minimum` samples, stop once the rays agree
+    bool more = false;
+    if (L.shape != 4 && (s.flags & 4u) && p.area_min < p.area_samples) {
+      const bool decided = ((s.flags & 1u) != 0u) != ((s.flags & 2u) != 0u);
+      more = !decided;
+    }
+    if (more) s.flags |= 8u;
+    state[i] = s;
+    need[i] = more ? 1u : 0u;
+  }
+}
+
+// Phase B rays of the jobs listed in `blist` (nb of them): samples minimum .. samples - 1, sample
+// k in slot (k - minimum) * nb + b.
+__global__ void k_vis_rays_b(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* __restrict__ js, const uint32_t* __restrict__ me, uint32_t nme,
+                             const uint64_t* __restrict__ tex_start, uint64_t q0, const uint32_t* __restrict__ blist, uint64_t nb,
+                             TraceParams p, TrigTables trig, float4* rays, float* contrib, float* wts) {
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x) {
+    const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + blist[b]);
+    const DevLight& L = lights[ent.light[v.e]];
+    const VisTexel vt = vis_texel(tr, ms, tx, L, p.seed, v.t, v.j);
+    const uint32_t probes = vis_probes(L), alpha = L.alpha ? 1u : 0u;
+    for (uint32_t k = p.area_min; k < p.area_samples; ++k) {
+      const uint64_t slot = (uint64_t)(k - p.area_min) * nb + b;
+      const VisRay r = vis_ray(L, trig, vt.pos, vt.nrm, vt.offset, probes + k);
+      rays[2 * slot] = make_float4(vt.origin.x, vt.origin.y, vt.origin.z, r.tmax);
+      rays[2 * slot + 1] = make_float4(r.dir.x, r.dir.y, r.dir.z, __uint_as_float(alpha | (r.ray ? 2u : 0u)));
+      contrib[3 * slot] = r.ray ? 1.0f : 0.0f;
+      contrib[3 * slot + 1] = 0.0f;
+      contrib[3 * slot + 2] = 0.0f;
+      wts[slot] = r.weight;
+    }
+  }
+}
+
+__global__ void k_vis_list(const uint64_t* __restrict__ need, const uint64_t* __restrict__ scan, uint64_t n, uint32_t* blist) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (need[i]) blist[scan[i]] = (uint32_t)i;
+}
+
+// Phase B folded in, then the texel's level, visited bit and stats, as k_trace writes them.
+__global__ void k_vis_finish(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                             DevRegions rg, const uint64_t* __restrict__ js, const uint32_t* __restrict__ me, uint32_t nme,
+                             const uint64_t* __restrict__ tex_start, uint64_t q0, uint64_t n, TraceParams p,
+                             const VisState* __restrict__ state, const uint64_t* __restrict__ bscan, uint64_t nb,
+                             const float4* __restrict__ rays, const float* __restrict__ contrib, const float* __restrict__ wts) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    const uint64_t i = i0 + lane;
+    uint32_t m = 0xffffffffu, rays_n = 0, some_n = 0;
+    if (i < n) {
+      const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + i);
+      const DevLight& L = lights[ent.light[v.e]];
+      VisState s = state[i];
+      if (s.flags & 8u) {
+        const uint64_t b = bscan[i];
+        for (uint32_t k = p.area_min; k < p.area_samples; ++k) {
+          const uint64_t slot = (uint64_t)(k - p.area_min) * nb + b;
+          const bool ray = (__float_as_uint(rays[2 * slot + 1].w) & 2u) != 0u;
+          vis_fold(s, ray, contrib[3 * slot] > 0.0f, false, wts[slot]);
+        }
+      }
+      bool some = false;
+      float fraction = 0.0f;
+      if (L.shape == 4) {
+        // sky.rs:30-48: every sample traced; the fraction is the open share
+        some = true;
+        fraction = s.lit / (float)(p.sky_samples > 1 ? p.sky_samples : 1u);
+        rays_n = p.sky_samples;
+      } else if (s.flags & 4u) {
+        Share sh;
+        sh.any_lit = (s.flags & 1u) != 0u;
+        sh.any_blocked = (s.flags & 2u) != 0u;
+        sh.lit = s.lit;
+        sh.total = s.total;
+        some = share_result(sh, &fraction);
+        rays_n = some ? s.nr : 0u;
+      }
+      const uint32_t t = v.t;
+      m = tr.mesh[t];
+      if (some) {
+        some_n = 1;
+        const uint32_t xy = tx.xy[v.j];
+        const uint32_t x = xy & 0xffffu, y = xy >> 16;
+        const uint32_t res = ms.res[m];
+        const uint64_t vi = (uint64_t)x + (uint64_t)y * res;
+        atomicOr(&ms.visited[ms.visited_off[m] + vi / 32], 1u << (vi % 32));
+        const uint32_t r = ent.region[v.e];
+        const uint32_t* rc = rg.rect + 4 * (size_t)r;
+        const uint64_t ri = (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * (rc[2] - rc[0] + 1);
+        raise_level(rg.pool, rg.off[r], rg.bits[r], ri, level_of(fraction, L.bits));
+      }
+    }
+    const uint32_t m0 = __shfl(m, 0, 64);
+    if (__ballot(i < n && m != m0) == 0ull) {
+      if (m0 != 0xffffffffu) {
+        wave_add(&ms.rays[m0], rays_n);
+        wave_add(&ms.samples[m0], some_n);
+      }
+    } else if (i < n) {
+      if (rays_n) atomicAdd(&ms.rays[m], (unsigned long long)rays_n);
+      if (some_n) atomicAdd(&ms.samples[m], 1ull);
     }
   }
 }
@@ -1062,6 +1379,51 @@ hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes m
   uint32_t waves = block / 64;
   hipLaunchKernelGGL(k_trace, dim3(grid_for(units, waves)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
                      lights, tx, tex_start, ent, reg, unit_start, units, p, trig);
+  return hipGetLastError();
+}
+hipError_t launch_vis_count(hipStream_t s, DevEntries ent, const DevLight* lights, const uint64_t* js, const uint32_t* me,
+                            uint32_t nme, const uint64_t* tex_start, uint64_t q0, uint64_t n, TraceParams p, uint64_t* out) {
+  hipLaunchKernelGGL(k_vis_count, dim3(grid_for(n + 1, kThreads)), dim3(kThreads), 0, s, ent, lights, js, me, nme, tex_start, q0,
+                     n, p, out);
+  return hipGetLastError();
+}
+hipError_t launch_vis_rays_a(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start, uint64_t q0,
+                             uint64_t n, TraceParams p, TrigTables trig, float4* rays, float* contrib, float* wts,
+                             VisState* state) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vis_rays_a, dim3(grid_for(n, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, js, me, nme, tex_start,
+                     q0, n, p, trig, rays, contrib, wts, state);
+  return hipGetLastError();
+}
+hipError_t launch_vis_mid(hipStream_t s, DevEntries ent, const DevLight* lights, const uint64_t* js, const uint32_t* me,
+                          uint32_t nme, const uint64_t* tex_start, uint64_t q0, uint64_t n, TraceParams p, const float4* rays,
+                          const float* contrib, const float* wts, VisState* state, uint64_t* need) {
+  hipLaunchKernelGGL(k_vis_mid, dim3(grid_for(n + 1, kThreads)), dim3(kThreads), 0, s, ent, lights, js, me, nme, tex_start, q0,
+                     n, p, rays, contrib, wts, state, need);
+  return hipGetLastError();
+}
+hipError_t launch_vis_list(hipStream_t s, const uint64_t* need, const uint64_t* scan, uint64_t n, uint32_t* blist) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vis_list, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, need, scan, n, blist);
+  return hipGetLastError();
+}
+hipError_t launch_vis_rays_b(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start, uint64_t q0,
+                             const uint32_t* blist, uint64_t nb, TraceParams p, TrigTables trig, float4* rays, float* contrib,
+                             float* wts) {
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vis_rays_b, dim3(grid_for(nb, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, js, me, nme,
+                     tex_start, q0, blist, nb, p, trig, rays, contrib, wts);
+  return hipGetLastError();
+}
+hipError_t launch_vis_finish(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                             DevRegions rg, const uint64_t* js, const uint32_t* me, uint32_t nme, const uint64_t* tex_start,
+                             uint64_t q0, uint64_t n, TraceParams p, const VisState* state, const uint64_t* bscan, uint64_t nb,
+                             const float4* rays, const float* contrib, const float* wts) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vis_finish, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, tx, ent, rg, js, me,
+                     nme, tex_start, q0, n, p, state, bscan, nb, rays, contrib, wts);
   return hipGetLastError();
 }
 hipError_t launch_optimize(hipStream_t s, DevTris tris, DevMeshes meshes, DevEntries ent, DevRegions reg,
@@ -1811,7 +2173,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
           bool slot = true;
           if (order) {  // hand out a job's samples together, from their sample-major slots
             const uint32_t job = nidx / per_job, k = nidx - job * per_job;
-            slot = (uint64_t)k < rs[job + 1] - rs[job];
+            slot = rs == nullptr || (uint64_t)k < rs[job + 1] - rs[job];
             nidx = k * njobs + job;
           }
           if (slot) {

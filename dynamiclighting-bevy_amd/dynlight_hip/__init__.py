@@ -111,6 +111,8 @@ def load() -> C.CDLL:
     lib.dl_test_batch_trace.restype = C.c_uint32
     lib.dl_test_batch_trace.argtypes = [P, C.POINTER(abi.dl_timings), C.POINTER(C.c_uint32), C.POINTER(C.c_double),
                                         C.c_uint32]
+    lib.dl_set_trace_stream.restype = C.c_int
+    lib.dl_set_trace_stream.argtypes = [P, C.c_int]
     lib.dl_set_gather.restype = C.c_int
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32]
@@ -343,6 +345,10 @@ class Context:
         st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)
         if st != 0:
             raise BakeError(st, self.error())
+
+    def set_trace_stream(self, enable: bool) -> None:
+        """Area lights, soft suns and skies through the streamed any-hit path (default) or per lane."""
+        self.lib.dl_set_trace_stream(self.handle, 1 if enable else 0)
 
     def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, chunk_rays: int = 0, blocks_per_cu: int = 0,
                    min_waves: int = 0, refill: int = 0, order: int = 0) -> None:

@@ -50,6 +50,10 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
  * output may be null.  Returns the job count of that batch. */
 uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker, double* span, uint32_t cap);
 
+/* 1 (default): area lights, soft suns and skies are traced as streamed jobs through the persistent
+ * any-hit kernel; 0: each lane of the shadow-ray kernel casts its texel's rays in turn. */
+int dl_set_trace_stream(dl_ctx* ctx, int enable);
+
 /* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
  * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
  * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per

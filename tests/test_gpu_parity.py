@@ -166,3 +166,20 @@ def test_outdoor_matches_oracle(ctx):
     got = ctx.bake(inp, opts)
     assert got.stats["bounce_rays"] > 100_000 and len(got.sun_maps) == 1 and got.sky_probes is not None
     assert_scenes_equal(got, oracle.bake(inp, opts))
+
+
+def test_per_lane_multi_ray_path_matches_oracle(ctx):
+    """Area lights, soft suns and skies traced per lane in the shadow-ray kernel (the streamed
+    any-hit path switched off) against the oracle."""
+    try:
+        ctx.set_trace_stream(False)
+        for name in ("panel", "sun", "sky", "mixed", "maps"):
+            inp, opts = scene_cases.CASES[name]()
+            assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
+        inp, opts = scenes.outdoor(nx=48, nz=32, buildings=12, area_lights=6, density=2.0)
+        opts.sun_map_resolution = 128
+        opts.sky_samples = 16
+        opts.sky_probe_spacing = 4.0
+        assert_scenes_equal(ctx.bake(inp, opts), oracle.bake(inp, opts))
+    finally:
+        ctx.set_trace_stream(True)
