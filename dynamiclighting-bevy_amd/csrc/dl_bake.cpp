@@ -800,8 +800,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       DevScene scb = sc;
       scb.counters = ctx->counting ? d_counters.p + 2 : nullptr;
       DBuf<unsigned long long> d_jobc;
-      DBuf<uint2> d_tjobs, d_cjobs;
-      uint64_t ntjobs = 0;
+      DBuf<uint2> d_tjobs, d_cjobs, d_sjobs;
+      uint64_t ntjobs = 0, nsjobs = 0;
       if (ctx->gather_mode == 1) {
         // job lists of the streamed point-light gather (texel and entry indices are u32)
         if (ntex >= (1ull << 32) || nbe >= (1ull << 32)) {
@@ -811,19 +811,22 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         HIP_TRY(zeroed(ctx, d_jobc, 4));
         const uint64_t nb1 = bb.n + 1;
         DBuf<uint64_t> d_jcnt, d_joff;
-        HIP_TRY(d_jcnt.alloc(2 * nb1));
-        HIP_TRY(d_joff.alloc(2 * nb1));
-        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 0, d_jcnt.p, nullptr, nullptr, nullptr));
-        HIP_TRY(scan_u64(s, d_jcnt.p, d_joff.p, bb.n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
-        HIP_TRY(scan_u64(s, d_jcnt.p + nb1, d_joff.p + nb1, bb.n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
-        std::vector<uint64_t> jc(2);
-        HIP_TRY(hipMemcpyAsync(&jc[0], d_joff.p + bb.n, 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&jc[1], d_joff.p + nb1 + bb.n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(d_jcnt.alloc(3 * nb1));
+        HIP_TRY(d_joff.alloc(3 * nb1));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 0, d_jcnt.p, nullptr, nullptr, nullptr, nullptr));
+        for (int l = 0; l < 3; ++l)
+          HIP_TRY(scan_u64(s, d_jcnt.p + l * nb1, d_joff.p + l * nb1, bb.n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+        std::vector<uint64_t> jc(3);
+        for (int l = 0; l < 3; ++l)
+          HIP_TRY(hipMemcpyAsync(&jc[l], d_joff.p + l * nb1 + bb.n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         ntjobs = jc[0];
+        nsjobs = jc[2];
         HIP_TRY(d_tjobs.alloc(std::max<uint64_t>(1, jc[0])));
         HIP_TRY(d_cjobs.alloc(std::max<uint64_t>(1, jc[1])));
-        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 1, nullptr, d_joff.p, d_tjobs.p, d_cjobs.p));
+        HIP_TRY(d_sjobs.alloc(std::max<uint64_t>(1, jc[2])));
+        HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 1, nullptr, d_joff.p, d_tjobs.p, d_cjobs.p,
+                                   d_sjobs.p));
         HIP_TRY(launch_gather_count(s, tr, ms, d_lights.p, tx, ent, bb, d_cjobs.p, jc[1], bp, ctx->trig));
         bp.sun_only = 1;
       }
@@ -865,9 +868,35 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(hipEventRecord(b2, s));
           HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p, ctx->gather.order));
         }
-        if (sun_bounce)
-          HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,
-                                       ctx->trace_block));
+        if (sun_bounce && nsjobs) {
+          // sun bounce in chunks of jobs: cosine rays, closest hits, sun rays, their visibility, sums
+          uint32_t sun_slots = 2;
+          for (uint64_t e : be)
+            if (hlights[b_ent_light[e]].shape == 3) sun_slots = std::max(sun_slots, hlights[b_ent_light[e]].bounce_samples);
+          const uint64_t sj = std::min<uint64_t>(nsjobs, std::max<uint64_t>(1, ctx->gather.chunk_rays / 2 / sun_slots));
+          DBuf<float4> d_srays, d_shits;
+          DBuf<float> d_scontrib;
+          DBuf<uint64_t> d_stotal;
+          HIP_TRY(d_srays.alloc(2 * sj * sun_slots));
+          HIP_TRY(d_shits.alloc(sj * sun_slots));
+          HIP_TRY(d_scontrib.alloc(3 * sj * sun_slots));
+          HIP_TRY(d_stotal.alloc(1));
+          for (uint64_t j0 = 0; j0 < nsjobs; j0 += sj) {
+            const uint64_t n = std::min(sj, nsjobs - j0);
+            const uint2* jobs = d_sjobs.p + j0;
+            HIP_TRY(launch_sun_rays(s, tr, ms, d_lights.p, tx, ent, jobs, n, sun_slots, ctx->gather.order, bp, ctx->trig, d_srays.p,
+                                    d_shits.p, d_stotal.p));
+            HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, 2 * sizeof(unsigned long long), s));
+            HIP_TRY(launch_closest_stream(s, scb, d_srays.p, d_stotal.p, n * sun_slots, n, d_shits.p,
+                                          reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather));
+            HIP_TRY(launch_sun_mid(s, scb, occ, d_lights.p, ent, tx, bb, jobs, n, sun_slots, ctx->gather.order, bp.colored,
+                                   d_srays.p, d_shits.p, d_scontrib.p));
+            HIP_TRY(launch_any_stream(s, scb, d_srays.p, d_stotal.p, n * sun_slots, nullptr, n, d_scontrib.p,
+                                      reinterpret_cast<unsigned int*>(d_jobc.p + 3), ctx->gather));
+            HIP_TRY(launch_sun_sum(s, tr, ent, d_lights.p, tx, bb, jobs, n, sun_slots, ctx->gather.order, d_srays.p,
+                                   d_scontrib.p));
+          }
+        }
         HIP_TRY(hipStreamSynchronize(s));
         for (size_t k = 0; k + 1 < tev.size(); k += 2) {
           float ms_k = 0.0f;

@@ -194,8 +194,9 @@ hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTr
                                 const DevLight* lights, DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevBounce b,
                                 uint64_t units, BounceParams p, TrigTables trig, uint32_t block);
 // Split point-light gather.  Jobs are (texel, entry) pairs within the light's radius: pass 0 of
-// launch_gather_jobs writes per bounce entry its traced and count-only job counts (2 (n + 1)
-// words), pass 1 fills both lists at the scanned offsets.  Traced jobs then run in chunks: ray offsets
+// launch_gather_jobs writes per bounce entry its traced and count-only job counts, and its sun
+// job count (every texel of a sun bounce entry), 3 (n + 1) words; pass 1 fills the three lists at
+// the scanned offsets.  Traced jobs then run in chunks: ray offsets
 // (launch_gather_nrays + scan), rays and contributions (launch_gather_rays), any-hit visibility
 // (launch_any_stream, persistent) and the per-job sums (launch_gather_sum).
 struct GatherLaunch {
@@ -210,7 +211,7 @@ struct GatherLaunch {
 };
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                               DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,
-                              uint2* count_jobs);
+                              uint2* count_jobs, uint2* sun_jobs);
 hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                                DevBounce b, const uint2* jobs, uint64_t njobs, BounceParams p, TrigTables trig);
 hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out);
@@ -223,6 +224,23 @@ hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, 
                              const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl);
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
                              uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order);
+// Closest-hit form of launch_any_stream: every traced ray that hits writes (t, u, v, leaf position)
+// to its slot of `hits`; slots of rays that miss keep what they held.  rs is implied uniform.
+hipError_t launch_closest_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                                 uint64_t njobs, float4* hits, unsigned int* queue, const GatherLaunch& gl);
+// Streamed sun bounce over a chunk of n sun jobs with `slots` ray slots each (the most bounce
+// samples of the chunk's suns): cosine rays and cleared hits (launch_sun_rays, which also stores
+// n * slots to *total), closest hits (launch_closest_stream), sun rays and contributions
+// (launch_sun_mid), their visibility (launch_any_stream), ordered sums and ray counts
+// (launch_sun_sum).
+hipError_t launch_sun_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                           const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, BounceParams p, TrigTables trig,
+                           float4* rays, float4* hits, uint64_t* total);
+hipError_t launch_sun_mid(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, DevEntries ent, DevTexels tx,
+                          DevBounce b, const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, uint32_t colored,
+                          float4* rays, const float4* hits, float* contrib);
+hipError_t launch_sun_sum(hipStream_t s, DevTris tris, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b,
+                          const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, const float4* rays, const float* contrib);
 // Sun map of one sun (sun_map.rs:25-75): frame axes, bounds minimum, texel size and depth.
 struct SunMapParams {
   float side[3], up[3], along[3];

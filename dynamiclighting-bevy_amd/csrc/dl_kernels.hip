@@ -1792,6 +1792,53 @@ __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __re
   return 0u;
 }
 
+// One BVH node of a closest-hit traversal (the loop body of traverse<false>): keeps the lowest
+// (t, occluder index) hit in `hit` (`have` once there is one).  Returns 0 while the traversal
+// continues, 2 once it has ended.
+__device__ __forceinline__ uint32_t visit_closest(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
+                                                  const SlabRay& sr, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
+                                                  uint32_t alpha, bool& have, Hit& hit) {
+  tl.nodes++;
+  float tn[4];
+  int4 ch;
+  const uint32_t mask = node_hits(sc, node, sr, (have ? hit.t : tmax) * (1.0f + kBoxEps), tn, &ch);
+  uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
+  for (;;) {
+    if (i == end) {
+      if (!leaves) break;
+      const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
+      leaves &= leaves - 1;
+      i = v >> 2;
+      end = i + (v & 3u) + 1u;
+    }
+    const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+    const uint32_t pos = i++;
+    float t, u, vv;
+    tl.tris++;
+    if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+    if (!finite_(t)) continue;
+    if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
+    const uint32_t occ = __float_as_uint(a.w);
+    if (!have || t < hit.t || (t == hit.t && occ < hit.occ)) {
+      have = true;
+      hit.t = t;
+      hit.u = u;
+      hit.v = vv;
+      hit.occ = occ;
+      hit.pos = pos;
+    }
+  }
+  const int32_t next = push_children(sc, mask, ch, tn, stack, stride, sp);
+  if (next >= 0) {
+    node = next;
+    return 0u;
+  }
+  if (sp == 0) return 2u;
+  sp--;
+  node = pop_node(sc, stack, stride, sp);
+  return 0u;
+}
+
 // Per-job constants of a point-light gather (bounce.rs:247-262).
 struct GatherJob {
   f3 origin, normal, dir0, lp;
@@ -1968,7 +2015,7 @@ __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTable
 // texel order within an entry.
 __global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, DevTexels tx, const uint64_t* __restrict__ tex_start,
                               DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* __restrict__ off,
-                              uint2* trace_jobs, uint2* count_jobs) {
+                              uint2* trace_jobs, uint2* count_jobs, uint2* sun_jobs) {
   // one wave per bounce entry, 64 of its texels per trip
   const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
   const uint64_t n = b.n;
@@ -1976,10 +2023,14 @@ __global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, D
     const uint64_t e = b.entry[q];
     const uint32_t t = ent.tri[e];
     const DevLight& L = lights[ent.light[e]];
-    uint64_t nt_traced = 0, nt_counted = 0;
+    uint64_t nt_traced = 0, nt_counted = 0, nt_sun = 0;
     uint64_t base_t = pass == 1 ? off[q] : 0, base_c = pass == 1 ? off[n + 1 + q] : 0;
-    if (L.shape != 3) {  // suns are gathered per unit
-      const uint64_t t0 = tex_start[t], nt = tex_start[t + 1] - t0;
+    const uint64_t t0 = tex_start[t], nt = tex_start[t + 1] - t0;
+    if (L.shape == 3) {  // a sun gathers at every texel of the triangle
+      nt_sun = nt;
+      if (pass == 1)
+        for (uint64_t c = lane; c < nt; c += 64) sun_jobs[off[2 * (n + 1) + q] + c] = make_uint2((uint32_t)(t0 + c), (uint32_t)e);
+    } else {
       const f3 lp = ld3(L.pos);
       const float r2 = L.radius * L.radius;
       const uint32_t w = b.ent_window[e];
@@ -2014,11 +2065,13 @@ __global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, D
     if (pass == 0 && lane == 0) {
       cnt[q] = nt_traced;
       cnt[n + 1 + q] = nt_counted;
+      cnt[2 * (n + 1) + q] = nt_sun;
     }
   }
   if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[n] = 0;
     cnt[2 * n + 1] = 0;
+    cnt[3 * n + 2] = 0;
   }
 }
 
@@ -2113,7 +2166,10 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
 // and the next one prefetched in registers; a lane whose ray ends swaps in the prefetched ray at
 // once and requests another, so lanes never wait for the wave's longest ray.  `queue` (zeroed)
 // hands out runs of `chunk` rays per wave.  An occluded ray's contribution is set to zero.
-template <int MIN_WAVES>
+// CLOSEST: closest-hit traversal instead; `contrib` is then a float4 per ray slot that receives
+// (t, u, v, leaf position) of the hit of every traced ray that hits (slots of rays that miss, or
+// hold no ray, keep what they held).
+template <int MIN_WAVES, bool CLOSEST>
 __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(DevScene sc, const float4* __restrict__ rays, const uint64_t* __restrict__ total, float* contrib,
                              unsigned int* queue, uint32_t chunk, uint32_t refill, uint32_t order,
                              const uint64_t* __restrict__ rs, uint32_t njobs, uint32_t positions) {
@@ -2135,6 +2191,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
   float tmax = 0.0f;
   uint32_t alpha = 0, sp = 0;
   int32_t node = 0;
+  bool have = false;
+  Hit hit{};
   for (;;) {
     // Finished lanes take their prefetched ray, and lanes without one request another, only
     // once at least `refill` lanes are idle: these steps then run with many lanes at a time.
@@ -2152,6 +2210,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
         sr = slab_ray(o, d);
         node = 0;
         sp = 0;
+        have = false;
         live = true;
       }
     }
@@ -2192,13 +2251,20 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
     }
     }
     if (live) {
-      const uint32_t r = visit_any(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha);
-      if (r != 0u) {
-        live = false;
-        if (r == 1u) {
-          contrib[3 * (size_t)cidx] = 0.0f;
-          contrib[3 * (size_t)cidx + 1] = 0.0f;
-          contrib[3 * (size_t)cidx + 2] = 0.0f;
+      if constexpr (CLOSEST) {
+        if (visit_closest(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha, have, hit) != 0u) {
+          live = false;
+          if (have) reinterpret_cast<float4*>(contrib)[cidx] = make_float4(hit.t, hit.u, hit.v, __uint_as_float(hit.pos));
+        }
+      } else {
+        const uint32_t r = visit_any(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha);
+        if (r != 0u) {
+          live = false;
+          if (r == 1u) {
+            contrib[3 * (size_t)cidx] = 0.0f;
+            contrib[3 * (size_t)cidx + 1] = 0.0f;
+            contrib[3 * (size_t)cidx + 2] = 0.0f;
+          }
         }
       }
     }
@@ -2233,6 +2299,148 @@ __global__ void k_gather_sum(DevEntries ent, const DevLight* __restrict__ lights
     out[0] = v.x;
     out[1] = v.y;
     out[2] = v.z;
+  }
+}
+
+// ---- streamed sun bounce -----------------------------------------------------------------------
+// gather_sun (sun_bounce.rs:24-80) split like the point-light gather: a job is a (texel, sun
+// bounce entry) pair; its `samples` cosine-weighted rays from the texel look for the closest hit
+// (k_sun_rays, then the closest-hit ray stream), a hit on a surface facing the ray and lit by the
+// sun casts one ray from the hit point toward the sun (k_sun_mid, then the any-hit stream), and
+// the job's texel receives the mean of albedo x cosine over the samples whose sun ray is
+// unoccluded, summed in sample order (k_sun_sum).  A chunk holds n jobs of S ray slots each, at
+// ray_slot(order, i * S, i, n, k).  Ray flags: 1 alpha-tested occluders let light through
+// transparent texels, 2 the ray is traced, 4 the sun ray counts as cast (texels a later triangle
+// overwrites, bounce.rs:202-209, count their sun rays without tracing them).
+
+// Whether the job's triangle owns its texel in the window map.
+__device__ __forceinline__ bool owns_texel(const DevBounce& b, const DevEntries& ent, const DevTexels& tx, uint2 jb) {
+  const uint32_t w = b.ent_window[jb.y];
+  const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+  const uint32_t xy = tx.xy[jb.x];
+  const uint64_t k = (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1);
+  const unsigned int* owner = reinterpret_cast<const unsigned int*>(b.map_b) + b.win_off[w] * 3;
+  return owner[k] == ent.tri[jb.y] + 1;
+}
+
+__global__ void k_sun_rays(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                           const uint2* __restrict__ jobs, uint64_t n, uint32_t S, uint32_t order, BounceParams p, TrigTables trig,
+                           float4* rays, float4* hits, uint64_t* total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = n * S;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 jb = jobs[i];
+    const uint32_t t = ent.tri[jb.y];
+    const DevLight& L = lights[ent.light[jb.y]];
+    const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
+    const float4 p4 = tx.pos[jb.x];
+    const f3 normal = ld3(tr.face + 3 * (size_t)t);
+    const f3 origin = add(mk3(p4.x, p4.y, p4.z), mul(normal, kRayOffset));
+    const uint32_t xy = tx.xy[jb.x];
+    Pcg rng = stream5(p.seed, ms.id[tr.mesh[t]], L.id, xy & 0xffffu, xy >> 16);
+    f2 offset;
+    offset.x = pcg_f32(rng);
+    offset.y = pcg_f32(rng);
+    const float flags = __uint_as_float(2u | (L.alpha ? 1u : 0u));
+    for (uint32_t k = 0; k < S; ++k) {
+      const uint64_t r = ray_slot(order, i * S, i, n, k);
+      hits[r] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(~0u));
+      if (k < samples) {
+        const f3 d = cosine_dir(trig, normal, spread_pt(k, offset, true));
+        rays[2 * r] = make_float4(origin.x, origin.y, origin.z, INFINITY);
+        rays[2 * r + 1] = make_float4(d.x, d.y, d.z, flags);
+      } else {
+        rays[2 * r + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+    }
+  }
+}
+
+// One thread per ray slot, in storage order: the closest hit of the slot's ray (leaf position
+// ~0u: none) becomes the sun ray of the slot and its contribution if unoccluded.
+__global__ void k_sun_mid(DevScene sc, DevOcc occ, const DevLight* __restrict__ lights, DevEntries ent, DevTexels tx, DevBounce b,
+                          const uint2* __restrict__ jobs, uint64_t n, uint32_t S, uint32_t order, uint32_t colored, float4* rays,
+                          const float4* __restrict__ hits, float* contrib) {
+  const uint64_t slots = n * S;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < slots; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = order ? r % n : r / S;
+    const float4 n1 = rays[2 * r + 1];
+    const float4 h4 = hits[r];
+    const uint32_t pos = __float_as_uint(h4.w);
+    f3 c = mk3(0.0f, 0.0f, 0.0f);
+    float4 out1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if ((__float_as_uint(n1.w) & 2u) && pos != ~0u) {
+      const f3 dir = mk3(n1.x, n1.y, n1.z);
+      const f3 face = tri_face_normal(sc, pos);
+      if (!(dot(face, dir) >= 0.0f)) {
+        const uint2 jb = jobs[i];
+        const DevLight& L = lights[ent.light[jb.y]];
+        const f3 toward = ld3(L.toward);
+        Hit h;
+        h.t = h4.x;
+        h.u = h4.y;
+        h.v = h4.z;
+        h.pos = pos;
+        h.occ = __float_as_uint(sc.tris[3 * (size_t)pos].w);
+        const float lit = dot(surface_normal(sc, occ, h), toward);
+        if (!(lit <= 0.0f || dot(face, toward) <= 0.0f)) {
+          const float4 n0 = rays[2 * r];
+          const f3 point = add(add(mk3(n0.x, n0.y, n0.z), mul(dir, h.t)), mul(face, kRayOffset));
+          const uint32_t traced = owns_texel(b, ent, tx, jb) ? 2u | (__float_as_uint(n1.w) & 1u) : 0u;
+          rays[2 * r] = make_float4(point.x, point.y, point.z, INFINITY);
+          out1 = make_float4(toward.x, toward.y, toward.z, __uint_as_float(4u | traced));
+          f3 alb = mk3(1.0f, 1.0f, 1.0f);
+          if (colored) alb = ld3(occ.albedo + 3 * (size_t)h.occ);
+          c = mul(alb, lit);
+        }
+      }
+    }
+    rays[2 * r + 1] = out1;
+    contrib[3 * r] = c.x;
+    contrib[3 * r + 1] = c.y;
+    contrib[3 * r + 2] = c.z;
+  }
+}
+
+// One thread per job: contributions summed in sample order (an occluded or absent sun ray adds
+// +0, which leaves the sum unchanged), the mean written to the owned window texel, and the job's
+// rays (every sample's, plus each sun ray cast) added to its mesh.
+__global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict__ lights, DevTexels tx, DevBounce b,
+                          const uint2* __restrict__ jobs, uint64_t n, uint32_t S, uint32_t order, const float4* __restrict__ rays,
+                          const float* __restrict__ contrib) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    const uint64_t i = i0 + lane;
+    uint32_t m = 0xffffffffu, nr = 0;
+    if (i < n) {
+      const uint2 jb = jobs[i];
+      const DevLight& L = lights[ent.light[jb.y]];
+      const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
+      f3 sum = mk3(0.0f, 0.0f, 0.0f);
+      nr = samples;
+      for (uint32_t k = 0; k < samples; ++k) {
+        const uint64_t r = ray_slot(order, i * S, i, n, k);
+        sum = add(sum, mk3(contrib[3 * r], contrib[3 * r + 1], contrib[3 * r + 2]));
+        nr += (__float_as_uint(rays[2 * r + 1].w) >> 2) & 1u;
+      }
+      m = tr.mesh[ent.tri[jb.y]];
+      if (owns_texel(b, ent, tx, jb)) {
+        const f3 v = divs(mul(sum, L.bounce_intensity), (float)samples);
+        const uint32_t w = b.ent_window[jb.y];
+        const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+        const uint32_t xy = tx.xy[jb.x];
+        float* out = b.map_a + 3 * (b.win_off[w] + (uint64_t)((xy & 0xffffu) - wr[0]) + (uint64_t)((xy >> 16) - wr[1]) * (wr[2] - wr[0] + 1));
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+      }
+    }
+    const uint32_t m0 = __shfl(m, 0, 64);
+    if (__ballot(i < n && m != m0) == 0ull) {
+      wave_add(&b.rays[m0], nr);
+    } else if (i < n && nr) {
+      atomicAdd(&b.rays[m], (unsigned long long)nr);
+    }
   }
 }
 
@@ -2459,9 +2667,32 @@ hipError_t launch_sky_probes(hipStream_t s, DevScene scene, TrigTables trig, con
 }
 hipError_t launch_gather_jobs(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                               DevEntries ent, DevBounce b, int pass, uint64_t* cnt, const uint64_t* off, uint2* trace_jobs,
-                              uint2* count_jobs) {
+                              uint2* count_jobs, uint2* sun_jobs) {
   hipLaunchKernelGGL(k_gather_jobs, dim3(grid_for(std::max<uint64_t>(b.n, 1), kThreads / 64)), dim3(kThreads), 0, s, tris, lights,
-                     tx, tex_start, ent, b, pass, cnt, off, trace_jobs, count_jobs);
+                     tx, tex_start, ent, b, pass, cnt, off, trace_jobs, count_jobs, sun_jobs);
+  return hipGetLastError();
+}
+hipError_t launch_sun_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                           const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, BounceParams p, TrigTables trig,
+                           float4* rays, float4* hits, uint64_t* total) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sun_rays, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, tx, ent, jobs, n, slots,
+                     order, p, trig, rays, hits, total);
+  return hipGetLastError();
+}
+hipError_t launch_sun_mid(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, DevEntries ent, DevTexels tx,
+                          DevBounce b, const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, uint32_t colored,
+                          float4* rays, const float4* hits, float* contrib) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sun_mid, dim3(grid_for(n * slots, kThreads)), dim3(kThreads), 0, s, scene, occ, lights, ent, tx, b, jobs, n,
+                     slots, order, colored, rays, hits, contrib);
+  return hipGetLastError();
+}
+hipError_t launch_sun_sum(hipStream_t s, DevTris tris, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b,
+                          const uint2* jobs, uint64_t n, uint32_t slots, uint32_t order, const float4* rays, const float* contrib) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sun_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, tris, ent, lights, tx, b, jobs, n, slots, order,
+                     rays, contrib);
   return hipGetLastError();
 }
 hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
@@ -2483,15 +2714,16 @@ hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, con
                      rays, contrib, order);
   return hipGetLastError();
 }
-hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
+template <bool CLOSEST>
+static hipError_t launch_ray_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                                    const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
   if (max_rays >= (1ull << 31) || gl.chunk == 0 || gl.chunk > (1u << 20)) return hipErrorInvalidValue;
   if (max_rays == 0) return hipSuccess;
   const size_t lds = stack_bytes(scene, gl.block);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  auto kernel = gl.min_waves >= 8 ? k_any_stream<8> : k_any_stream<1>;
+  auto kernel = gl.min_waves >= 8 ? k_any_stream<8, CLOSEST> : k_any_stream<1, CLOSEST>;
   if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)gl.block, lds);
   if (e != hipSuccess) return e;
   if (gl.blocks_per_cu) per_cu = (int)gl.blocks_per_cu;
@@ -2504,6 +2736,14 @@ hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, 
   hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, rays, total, contrib, queue, gl.chunk,
                      std::max(1u, std::min(64u, gl.refill)), gl.order ? 1u : 0u, rs, (uint32_t)njobs, (uint32_t)max_rays);
   return hipGetLastError();
+}
+hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
+  return launch_ray_stream<false>(s, scene, rays, total, max_rays, rs, njobs, contrib, queue, gl);
+}
+hipError_t launch_closest_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
+                                 uint64_t njobs, float4* hits, unsigned int* queue, const GatherLaunch& gl) {
+  return launch_ray_stream<true>(s, scene, rays, total, max_rays, nullptr, njobs, reinterpret_cast<float*>(hits), queue, gl);
 }
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
                              uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order) {
