@@ -179,7 +179,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   }
   const uint32_t nocc = use_res ? resident->nocc : (uint32_t)occ_flags.size();
   Bvh4 bvh;
+  const auto t_bvh = std::chrono::steady_clock::now();
   if (!use_res) build_bvh4(occ9.data(), nocc, ctx->host_threads, bvh);
+  ctx->timings.bvh_ms = ms_since(t_bvh);
   ctx->timings.bvh_nodes = use_res ? resident->nodes : bvh.nodes.size();
   ctx->timings.occluder_triangles = nocc;
 
@@ -252,6 +254,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   // ---- scene-level device state --------------------------------------------------------------
   // Geometry buffers live in the resident scene when there is one (filled here on its first
   // bake), otherwise in this bake; per-bake buffers are always this bake's.
+  ctx->timings.prepare_ms = ms_since(t_start);
   auto t_up = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   Resident local_geo;

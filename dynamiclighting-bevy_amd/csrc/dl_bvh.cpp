@@ -2,9 +2,10 @@
 // bounded by the part of the triangle inside half of its parent reference's box (clipped in
 // double precision, bounds rounded outward); then binned SAH over reference centroids into a
 // binary tree with leaves of at most four references, collapsed to 4-wide nodes by opening the
-// largest children first.  Subtrees below a size threshold are built on worker threads.  A
-// triangle split into several references appears in several leaves; any-hit and closest-hit
-// (lowest t, then lowest occluder index) results do not depend on that.
+// largest children first.  Reference cutting, subtrees above a size threshold, leaf records and
+// quantization run on worker threads.  A triangle split into several references appears in
+// several leaves; any-hit and closest-hit (lowest t, then lowest occluder index) results do not
+// depend on that.
 #include "dl_bvh.h"
 
 #include <algorithm>
@@ -12,6 +13,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <queue>
 #include <thread>
@@ -34,126 +37,244 @@ struct Box {
   }
 };
 
-struct Node2 {
+// One reference: the bounds of a triangle or of a part of it, their centre and the triangle.
+struct Ref {
   Box box;
-  int32_t left = -1, right = -1;  // children (inner); -1 for leaves
-  uint32_t first = 0, count = 0;  // leaf range in `idx`
+  float c[3];
+  uint32_t tri;
 };
 
-struct Builder {
-  const float* tri;
-  std::vector<Box> tb;
-  std::vector<float> cen;
-  std::vector<uint32_t> idx;
-  std::vector<Node2> nodes;
-  std::mutex mu;
+// Binary build node.  Plain data: the node array is allocated once, uninitialized, for the
+// largest possible tree (2 references - 1 nodes) and handed out by an atomic counter, so threads
+// building different subtrees never wait for one another.
+struct Node2 {
+  Box box;
+  int32_t left, right;    // children (inner); -1 for leaves
+  uint32_t first, count;  // leaf range in the reference array
+};
 
-  uint32_t alloc2() {
-    std::lock_guard<std::mutex> g(mu);
-    nodes.emplace_back();
-    nodes.emplace_back();
-    return (uint32_t)nodes.size() - 2;
+// fn(i) for i in [0, n), in contiguous blocks on up to `threads` threads.
+template <class F>
+void parallel_for(uint64_t n, unsigned threads, const F& fn) {
+  const unsigned th = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(threads, (n + 8191) / 8192));
+  if (th <= 1) {
+    for (uint64_t i = 0; i < n; ++i) fn(i);
+    return;
   }
+  const uint64_t per = (n + th - 1) / th;
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < th; ++t)
+    ts.emplace_back([&fn, t, per, n]() {
+      for (uint64_t i = t * per, e = std::min(n, (t + 1) * per); i < e; ++i) fn(i);
+    });
+  for (uint64_t i = 0; i < std::min(n, per); ++i) fn(i);
+  for (auto& t : ts) t.join();
+}
 
-  // Splits [first, first+count) of idx; returns the split point or `first` for a leaf.
-  uint32_t split(uint32_t first, uint32_t count, Box& box) {
-    Box cb;
+// fn(k) for blocks k in [0, blocks), handed out one at a time to up to `threads` threads (for
+// blocks of uneven cost).
+template <class F>
+void parallel_blocks(uint64_t blocks, unsigned threads, const F& fn) {
+  const unsigned th = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(threads, blocks));
+  std::atomic<uint64_t> next{0};
+  auto work = [&]() {
+    for (uint64_t k; (k = next.fetch_add(1, std::memory_order_relaxed)) < blocks;) fn(k);
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < th; ++t) ts.emplace_back(work);
+  work();
+  for (auto& t : ts) t.join();
+}
+
+struct Builder {
+  Ref* refs = nullptr;
+  Node2* nodes = nullptr;
+  std::atomic<uint32_t> used{1};  // node 0 is the root
+
+  // Work shared between threads: subtrees of at least `share` references.
+  struct Item {
+    uint32_t node, first, count;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<Item> shared;
+  uint32_t pending = 0;  // shared items not yet finished (under mu)
+  uint32_t share = 0;
+
+  // Bounds of [first, first + count) in `box`; returns the split point of a binned SAH split
+  // over reference centres (32 bins per axis), or `first` for a leaf of at most four.  The
+  // references are partitioned in place.
+  static constexpr int B = 32;
+  struct Bins {
+    Box box[3][B];
+    uint32_t cnt[3][B];
+  };
+  unsigned threads = 1;
+  static constexpr uint32_t kWide = 1u << 17;  // ranges split by several threads at once
+
+  static void bounds(const Ref* r, uint32_t count, Box& box_out, Box& cb_out) {
+    Box box, cb;  // locals: the caller's boxes may share cache lines with other threads'
     box.reset();
     cb.reset();
-    for (uint32_t k = first; k < first + count; ++k) {
-      uint32_t p = idx[k];
-      box.grow(tb[p]);
+    for (uint32_t k = 0; k < count; ++k) {
+      box.grow(r[k].box);
       for (int a = 0; a < 3; ++a) {
-        cb.lo[a] = std::min(cb.lo[a], cen[p * 3 + a]);
-        cb.hi[a] = std::max(cb.hi[a], cen[p * 3 + a]);
+        cb.lo[a] = std::min(cb.lo[a], r[k].c[a]);
+        cb.hi[a] = std::max(cb.hi[a], r[k].c[a]);
       }
     }
+    box_out = box;
+    cb_out = cb;
+  }
+  static void bin(const Ref* r, uint32_t count, const Box& cb, const float* scale, Bins& out) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < B; ++b) {
+        out.box[a][b].reset();
+        out.cnt[a][b] = 0;
+      }
+    for (uint32_t k = 0; k < count; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const int b = std::min(B - 1, std::max(0, (int)((r[k].c[a] - cb.lo[a]) * scale[a])));
+        out.cnt[a][b]++;
+        out.box[a][b].grow(r[k].box);
+      }
+  }
+  // fn(part, begin, count) over `parts` slices of [0, count) on their own threads
+  template <class F>
+  static void slices(uint32_t count, unsigned parts, const F& fn) {
+    const uint32_t per = (count + parts - 1) / parts;
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < parts; ++t) {
+      const uint32_t a = std::min(count, t * per);
+      ts.emplace_back([&fn, t, a, per, count]() { fn(t, a, std::min(per, count - a)); });
+    }
+    fn(0, 0, std::min(per, count));
+    for (auto& t : ts) t.join();
+  }
+
+  uint32_t split(uint32_t first, uint32_t count, Box& box) {
+    Ref* const r = refs + first;
+    const unsigned parts = count >= kWide ? std::min<unsigned>(threads, count / (kWide / 2)) : 1u;
+    Box cb;
+    if (parts > 1) {
+      std::vector<Box> pb(parts), pc(parts);
+      slices(count, parts, [&](unsigned t, uint32_t a, uint32_t c) { bounds(r + a, c, pb[t], pc[t]); });
+      box = pb[0];
+      cb = pc[0];
+      for (unsigned t = 1; t < parts; ++t) {
+        box.grow(pb[t]);
+        cb.grow(pc[t]);
+      }
+    } else {
+      bounds(r, count, box, cb);
+    }
     if (count <= 4) return first;
-    const int B = 32;
+    float scale[3];
+    for (int a = 0; a < 3; ++a) {
+      const float ext = cb.hi[a] - cb.lo[a];
+      scale[a] = ext > 0.0f ? (float)B / ext : 0.0f;
+    }
+    Bins bins;
+    if (parts > 1) {
+      std::vector<Bins> pbins(parts);
+      slices(count, parts, [&](unsigned t, uint32_t a, uint32_t c) { bin(r + a, c, cb, scale, pbins[t]); });
+      bins = pbins[0];
+      for (unsigned t = 1; t < parts; ++t)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < B; ++b) {
+            bins.box[a][b].grow(pbins[t].box[a][b]);
+            bins.cnt[a][b] += pbins[t].cnt[a][b];
+          }
+    } else {
+      bin(r, count, cb, scale, bins);
+    }
     float best = INFINITY;
     int best_axis = -1, best_bin = -1;
     for (int axis = 0; axis < 3; ++axis) {
-      float ext = cb.hi[axis] - cb.lo[axis];
-      if (!(ext > 0.0f)) continue;
-      float scale = (float)B / ext;
-      Box bins[B];
-      uint32_t cnt[B] = {};
-      for (int b = 0; b < B; ++b) bins[b].reset();
-      for (uint32_t k = first; k < first + count; ++k) {
-        uint32_t p = idx[k];
-        int b = std::min(B - 1, std::max(0, (int)((cen[p * 3 + axis] - cb.lo[axis]) * scale)));
-        cnt[b]++;
-        bins[b].grow(tb[p]);
-      }
+      if (!(cb.hi[axis] - cb.lo[axis] > 0.0f)) continue;
       float right_area[B];
       uint32_t right_cnt[B];
       Box acc;
       acc.reset();
       uint32_t c = 0;
       for (int b = B - 1; b > 0; --b) {
-        acc.grow(bins[b]);
-        c += cnt[b];
+        acc.grow(bins.box[axis][b]);
+        c += bins.cnt[axis][b];
         right_area[b] = acc.area();
         right_cnt[b] = c;
       }
       acc.reset();
       c = 0;
       for (int b = 0; b < B - 1; ++b) {
-        acc.grow(bins[b]);
-        c += cnt[b];
+        acc.grow(bins.box[axis][b]);
+        c += bins.cnt[axis][b];
         if (c == 0 || right_cnt[b + 1] == 0) continue;
-        float cost = (float)c * acc.area() + (float)right_cnt[b + 1] * right_area[b + 1];
-        if (cost < best) { best = cost; best_axis = axis; best_bin = b + 1; }
+        const float cost = (float)c * acc.area() + (float)right_cnt[b + 1] * right_area[b + 1];
+        if (cost < best) {
+          best = cost;
+          best_axis = axis;
+          best_bin = b + 1;
+        }
       }
     }
-    uint32_t mid;
-    if (best_axis < 0) {
-      mid = first + count / 2;  // coincident centroids: split the list
-    } else {
-      float ext = cb.hi[best_axis] - cb.lo[best_axis];
-      float scale = (float)B / ext;
-      int axis = best_axis, bin = best_bin;
-      auto it = std::partition(idx.begin() + first, idx.begin() + first + count, [&](uint32_t p) {
-        int b = std::min(B - 1, std::max(0, (int)((cen[p * 3 + axis] - cb.lo[axis]) * scale)));
-        return b < bin;
-      });
-      mid = (uint32_t)(it - idx.begin());
-      if (mid == first || mid == first + count) mid = first + count / 2;
-    }
-    return mid;
+    if (best_axis < 0) return first + count / 2;  // coincident centres: split the list
+    const int axis = best_axis, bin = best_bin;
+    const float lo = cb.lo[axis], sc = scale[axis];
+    Ref* const mid = std::partition(r, r + count, [&](const Ref& x) {
+      return std::min(B - 1, std::max(0, (int)((x.c[axis] - lo) * sc))) < bin;
+    });
+    const uint32_t m = first + (uint32_t)(mid - r);
+    return m == first || m == first + count ? first + count / 2 : m;
   }
 
-  void build_subtree(uint32_t node, uint32_t first, uint32_t count, std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>>* spill,
-                     uint32_t spill_below) {
-    struct Item { uint32_t node, first, count; };
-    std::vector<Item> work{{node, first, count}};
-    while (!work.empty()) {
-      Item it = work.back();
-      work.pop_back();
-      if (spill && it.count < spill_below && it.count > 4) {
-        spill->push_back({it.node, {it.first, it.count}});
+  // The subtree of `it`, depth first; children of at least `share` references go to the shared
+  // list while the thread carries on with their sibling.
+  void run(Item it) {
+    std::vector<Item> local{it};
+    while (!local.empty()) {
+      const Item w = local.back();
+      local.pop_back();
+      Node2& n = nodes[w.node];
+      const uint32_t mid = split(w.first, w.count, n.box);
+      if (mid == w.first) {
+        n.left = n.right = -1;
+        n.first = w.first;
+        n.count = w.count;
         continue;
       }
-      Box box;
-      uint32_t mid = split(it.first, it.count, box);
-      if (mid == it.first) {
-        std::lock_guard<std::mutex> g(mu);
-        Node2& n = nodes[it.node];
-        n.box = box;
-        n.first = it.first;
-        n.count = it.count;
-        continue;
+      const uint32_t l = used.fetch_add(2, std::memory_order_relaxed);
+      n.left = (int32_t)l;
+      n.right = (int32_t)l + 1;
+      n.first = w.first;  // inner nodes keep their range too: the collapse uses its size
+      n.count = w.count;
+      const Item a{l, w.first, mid - w.first}, b{l + 1, mid, w.first + w.count - mid};
+      if (share && a.count >= share) {
+        {
+          std::lock_guard<std::mutex> g(mu);
+          shared.push_back(a);
+          pending++;
+        }
+        cv.notify_one();
+      } else {
+        local.push_back(a);
       }
-      uint32_t l = alloc2();
+      local.push_back(b);
+    }
+  }
+
+  void worker() {
+    for (;;) {
+      Item it;
       {
-        std::lock_guard<std::mutex> g(mu);
-        Node2& n = nodes[it.node];
-        n.box = box;
-        n.left = (int32_t)l;
-        n.right = (int32_t)l + 1;
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&]() { return !shared.empty() || pending == 0; });
+        if (shared.empty()) return;
+        it = shared.back();
+        shared.pop_back();
       }
-      work.push_back({l, it.first, mid - it.first});
-      work.push_back({l + 1, mid, it.first + it.count - mid});
+      run(it);
+      std::lock_guard<std::mutex> g(mu);
+      if (--pending == 0) cv.notify_all();
     }
   }
 };
@@ -206,14 +327,34 @@ bool clip_bounds(const float* t, const double lo[3], const double hi[3], Box* ou
   return true;
 }
 
+// Cuts the part of triangle `t` inside `b` at the middle of the box's longest axis: the bounds
+// of the triangle's part on either side.  False when one side holds none of it.
+bool cut(const float* t, const Box& b, Box* left, Box* right) {
+  int axis = 0;
+  for (int a = 1; a < 3; ++a)
+    if (b.hi[a] - b.lo[a] > b.hi[axis] - b.lo[axis]) axis = a;
+  const double mid = 0.5 * ((double)b.lo[axis] + (double)b.hi[axis]);
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = b.lo[a];
+    hi[a] = b.hi[a];
+  }
+  hi[axis] = mid;
+  const bool has_l = clip_bounds(t, lo, hi, left);
+  hi[axis] = b.hi[axis];
+  lo[axis] = mid;
+  const bool has_r = clip_bounds(t, lo, hi, right);
+  return has_l && has_r;
+}
+
 // Splits references of triangles whose box has a surface area above `factor` times the mean
-// triangle box area, largest first, into at most `budget` extra references.
-void split_refs(const float* tris9, uint32_t n, float factor, uint64_t budget, std::vector<uint32_t>& ref_tri, std::vector<Box>& ref_box) {
-  double mean = 0.0;
-  for (uint32_t i = 0; i < n; ++i) mean += ref_box[i].area();
-  mean /= std::max<uint32_t>(1, n);
-  const double limit = mean * factor;
-  if (!(limit > 0.0) || budget == 0) return;
+// triangle box area into parts: a reference above that area is cut at the middle of its box's
+// longest axis into the bounds of the triangle's part on either side, and both halves are cut
+// again while above it; a cut that leaves one side empty keeps the reference whole.  At most
+// `budget` extra references: when the triangles would need more, the largest references are
+// cut first until the budget is spent.  Per triangle, its references in cut order.
+void split_refs_heap(const float* tris9, uint32_t n, double limit, uint64_t budget, std::vector<uint32_t>& ref_tri,
+                     std::vector<Box>& ref_box) {
   using Item = std::pair<float, uint32_t>;  // area, reference
   std::priority_queue<Item> heap;
   for (uint32_t i = 0; i < n; ++i)
@@ -221,21 +362,8 @@ void split_refs(const float* tris9, uint32_t n, float factor, uint64_t budget, s
   while (!heap.empty() && budget > 0) {
     const uint32_t r = heap.top().second;
     heap.pop();
-    const Box b = ref_box[r];
-    int axis = 0;
-    for (int a = 1; a < 3; ++a)
-      if (b.hi[a] - b.lo[a] > b.hi[axis] - b.lo[axis]) axis = a;
-    const double mid = 0.5 * ((double)b.lo[axis] + (double)b.hi[axis]);
-    double lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) { lo[a] = b.lo[a]; hi[a] = b.hi[a]; }
-    const float* t = tris9 + (size_t)ref_tri[r] * 9;
     Box left, right;
-    hi[axis] = mid;
-    const bool has_l = clip_bounds(t, lo, hi, &left);
-    hi[axis] = b.hi[axis];
-    lo[axis] = mid;
-    const bool has_r = clip_bounds(t, lo, hi, &right);
-    if (!has_l || !has_r) continue;  // a sliver: keep the reference whole
+    if (!cut(tris9 + (size_t)ref_tri[r] * 9, ref_box[r], &left, &right)) continue;  // a sliver: keep it whole
     ref_box[r] = left;
     ref_tri.push_back(ref_tri[r]);
     ref_box.push_back(right);
@@ -243,6 +371,83 @@ void split_refs(const float* tris9, uint32_t n, float factor, uint64_t budget, s
     if (left.area() > limit) heap.push({left.area(), r});
     if (right.area() > limit) heap.push({right.area(), (uint32_t)ref_box.size() - 1});
   }
+}
+
+void make_refs(const float* tris9, uint32_t n, float factor, uint64_t budget, unsigned threads, std::vector<Ref>& refs) {
+  std::vector<Box> tb(n);
+  parallel_for(n, threads, [&](uint64_t i) {
+    const float* t = tris9 + i * 9;
+    for (int a = 0; a < 3; ++a) {
+      const float v0 = t[a], v1 = t[3 + a], v2 = t[6 + a];
+      tb[i].lo[a] = std::min(v0, std::min(v1, v2));
+      tb[i].hi[a] = std::max(v0, std::max(v1, v2));
+    }
+  });
+  // mean box area, summed in fixed blocks so the value does not depend on the thread count
+  constexpr uint64_t kBlock = 1 << 16;
+  const uint64_t blocks = (n + kBlock - 1) / kBlock;
+  std::vector<double> part(blocks, 0.0);
+  parallel_for(blocks, threads, [&](uint64_t k) {
+    double sum = 0.0;
+    for (uint64_t i = k * kBlock, e = std::min<uint64_t>(n, (k + 1) * kBlock); i < e; ++i) sum += tb[i].area();
+    part[k] = sum;
+  });
+  double mean = 0.0;
+  for (double v : part) mean += v;
+  mean /= std::max<uint32_t>(1, n);
+  const double limit = mean * factor;
+  auto ref_of = [&](const Box& b, uint32_t tri) {
+    Ref r;
+    r.box = b;
+    for (int a = 0; a < 3; ++a) r.c[a] = 0.5f * (b.lo[a] + b.hi[a]);
+    r.tri = tri;
+    return r;
+  };
+  if (!(limit > 0.0) || budget == 0) {
+    refs.resize(n);
+    parallel_for(n, threads, [&](uint64_t i) { refs[i] = ref_of(tb[i], (uint32_t)i); });
+    return;
+  }
+  // every triangle cut as far as the limit asks, small blocks of triangles in parallel (the
+  // triangles that need cuts tend to sit together)
+  constexpr uint64_t kCutBlock = 1 << 11;
+  std::vector<std::vector<Ref>> out((n + kCutBlock - 1) / kCutBlock);
+  parallel_blocks(out.size(), threads, [&](uint64_t k) {
+    std::vector<Ref>& o = out[k];
+    std::vector<Box> stack;
+    for (uint64_t i = k * kCutBlock, e = std::min<uint64_t>(n, (k + 1) * kCutBlock); i < e; ++i) {
+      stack.assign(1, tb[i]);
+      while (!stack.empty()) {
+        const Box b = stack.back();
+        stack.pop_back();
+        Box left, right;
+        if (b.area() > limit && cut(tris9 + i * 9, b, &left, &right)) {
+          stack.push_back(right);
+          stack.push_back(left);
+        } else {
+          o.push_back(ref_of(b, (uint32_t)i));
+        }
+      }
+    }
+  });
+  uint64_t total = 0;
+  for (const auto& o : out) total += o.size();
+  if (total - n <= budget) {
+    refs.resize(total);
+    uint64_t at = 0;
+    for (auto& o : out) {
+      std::copy(o.begin(), o.end(), refs.begin() + at);
+      at += o.size();
+      std::vector<Ref>().swap(o);
+    }
+    return;
+  }
+  // over budget: the largest references first
+  std::vector<uint32_t> ref_tri(n);
+  for (uint32_t i = 0; i < n; ++i) ref_tri[i] = i;
+  split_refs_heap(tris9, n, limit, budget, ref_tri, tb);
+  refs.resize(tb.size());
+  for (size_t i = 0; i < tb.size(); ++i) refs[i] = ref_of(tb[i], ref_tri[i]);
 }
 
 void pad(const Box& b, float* lo, float* hi) {
@@ -319,6 +524,60 @@ void quantize(const Node4& nd, CNode* out) {
   for (int k = 0; k < 4; ++k) out->child[k] = nd.child[k];
 }
 
+struct Pending {
+  uint32_t n2, n4, depth;
+};
+
+// The 4-wide node `p.n4` from binary node `p.n2`: its two children, then the inner child with the
+// largest box opened into its two, up to four children; inner children get the next indices of
+// `wide` and are queued in `work`.
+void collapse_node(const Node2* n2, const Pending& p, std::vector<Node4>& wide, std::vector<Pending>& work) {
+  uint32_t kids[4];
+  int nk = 0;
+  if (n2[p.n2].left < 0) {
+    kids[nk++] = p.n2;  // whole tree is one leaf
+  } else {
+    kids[nk++] = (uint32_t)n2[p.n2].left;
+    kids[nk++] = (uint32_t)n2[p.n2].right;
+    while (nk < 4) {
+      int open = -1;
+      float best = -1.0f;
+      for (int k = 0; k < nk; ++k)
+        if (n2[kids[k]].left >= 0 && n2[kids[k]].box.area() > best) {
+          best = n2[kids[k]].box.area();
+          open = k;
+        }
+      if (open < 0) break;
+      const uint32_t o = kids[open];
+      kids[open] = (uint32_t)n2[o].left;
+      kids[nk++] = (uint32_t)n2[o].right;
+    }
+  }
+  Node4 nd;
+  std::memset(&nd, 0, sizeof nd);
+  nd.count = nk;
+  for (int k = 0; k < nk; ++k) {
+    float lo[3], hi[3];
+    pad(n2[kids[k]].box, lo, hi);
+    nd.lox[k] = lo[0];
+    nd.loy[k] = lo[1];
+    nd.loz[k] = lo[2];
+    nd.hix[k] = hi[0];
+    nd.hiy[k] = hi[1];
+    nd.hiz[k] = hi[2];
+    if (n2[kids[k]].left < 0) {
+      const uint32_t first = n2[kids[k]].first, count = n2[kids[k]].count;
+      nd.child[k] = -1 - (int32_t)((first << 2) | (count - 1));
+    } else {
+      const uint32_t idx = (uint32_t)wide.size();
+      wide.emplace_back();
+      nd.child[k] = (int32_t)idx;
+      work.push_back({kids[k], idx, p.depth + 1});
+    }
+  }
+  wide[p.n4] = nd;
+}
+
 }  // namespace
 
 void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
@@ -327,120 +586,92 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
   out.order.clear();
   out.depth = 0;
   if (n == 0) return;
-  Builder b;
-  b.tri = tris9;
-  std::vector<uint32_t> ref_tri(n);
-  b.tb.resize(n);
-  for (uint32_t i = 0; i < n; ++i) {
-    const float* t = tris9 + (size_t)i * 9;
-    for (int a = 0; a < 3; ++a) {
-      float v0 = t[a], v1 = t[3 + a], v2 = t[6 + a];
-      b.tb[i].lo[a] = std::min(v0, std::min(v1, v2));
-      b.tb[i].hi[a] = std::max(v0, std::max(v1, v2));
-    }
-    ref_tri[i] = i;
-  }
+  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  threads = std::min(threads, 16u);
   float factor = 4.0f, budget_share = 1.0f;
   if (const char* e = std::getenv("DL_BVH_SPLIT")) factor = (float)std::atof(e);
   if (const char* e = std::getenv("DL_BVH_SPLIT_BUDGET")) budget_share = (float)std::atof(e);
-  if (factor > 0.0f) split_refs(tris9, n, factor, (uint64_t)(budget_share * (float)n), ref_tri, b.tb);
-  const uint32_t nr = (uint32_t)ref_tri.size();
-  b.cen.resize((size_t)nr * 3);
-  b.idx.resize(nr);
-  for (uint32_t i = 0; i < nr; ++i) {
-    for (int a = 0; a < 3; ++a) b.cen[(size_t)i * 3 + a] = 0.5f * (b.tb[i].lo[a] + b.tb[i].hi[a]);
-    b.idx[i] = i;
-  }
-  b.nodes.reserve((size_t)n * 2 / 3 + 16);
-  b.nodes.emplace_back();
-  // Top of the tree on this thread until subtrees are small, then the rest in parallel.
-  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
-  uint32_t spill_below = threads > 1 ? std::max<uint32_t>(4096, nr / (threads * 8)) : 0;
-  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> spill;
-  b.build_subtree(0, 0, nr, threads > 1 ? &spill : nullptr, spill_below);
-  if (!spill.empty()) {
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-      for (;;) {
-        size_t i = next.fetch_add(1);
-        if (i >= spill.size()) break;
-        b.build_subtree(spill[i].first, spill[i].second.first, spill[i].second.second, nullptr, 0);
-      }
-    };
+  std::vector<Ref> refs;
+  make_refs(tris9, n, factor, factor > 0.0f ? (uint64_t)(budget_share * (float)n) : 0, threads, refs);
+  const uint32_t nr = (uint32_t)refs.size();
+
+  Builder b;
+  b.refs = refs.data();
+  std::unique_ptr<Node2[]> node_store(new Node2[2 * (size_t)nr]);
+  b.nodes = node_store.get();
+  b.threads = threads;
+  b.share = threads > 1 ? std::max<uint32_t>(2048, nr / (threads * 16)) : 0;
+  if (b.share == 0) {
+    b.run({0, 0, nr});
+  } else {
+    b.shared.push_back({0, 0, nr});
+    b.pending = 1;
     std::vector<std::thread> ts;
-    for (unsigned t = 1; t < threads; ++t) ts.emplace_back(work);
-    work();
+    for (unsigned t = 1; t < threads; ++t) ts.emplace_back([&b]() { b.worker(); });
+    b.worker();
     for (auto& t : ts) t.join();
   }
+  const Node2* n2 = b.nodes;
+
   // Leaf-ordered triangles (one record per reference).
   out.order.resize(nr);
-  for (uint32_t k = 0; k < nr; ++k) out.order[k] = ref_tri[b.idx[k]];
   out.tris.resize((size_t)nr * 12);
-  for (uint32_t k = 0; k < nr; ++k) {
-    uint32_t p = out.order[k];
+  parallel_for(nr, threads, [&](uint64_t k) {
+    const uint32_t p = refs[k].tri;
+    out.order[k] = p;
     const float* t = tris9 + (size_t)p * 9;
-    float* o = out.tris.data() + (size_t)k * 12;
+    float* o = out.tris.data() + k * 12;
     o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
     std::memcpy(&o[3], &p, 4);
     o[4] = t[3] - t[0]; o[5] = t[4] - t[1]; o[6] = t[5] - t[2]; o[7] = 0.0f;
     o[8] = t[6] - t[0]; o[9] = t[7] - t[1]; o[10] = t[8] - t[2]; o[11] = 0.0f;
-  }
-  // Collapse to 4-wide nodes.
-  const std::vector<Node2>& n2 = b.nodes;
-  struct Pending { uint32_t n2, n4, depth; };
-  std::vector<Node4> wide;
-  wide.emplace_back();
-  std::vector<Pending> work{{0, 0, 1}};
+  });
+  // Collapse to 4-wide nodes: the top of the tree here, subtrees below `cut` references on
+  // worker threads; the nodes of each subtree follow the top's nodes in one block.
+  const uint32_t cut = threads > 1 ? std::max<uint32_t>(4096, nr / (threads * 8)) : 0u;
+  std::vector<Node4> wide(1);
+  std::vector<Pending> work{{0, 0, 1}}, deferred;
   while (!work.empty()) {
-    Pending p = work.back();
+    const Pending p = work.back();
     work.pop_back();
+    if (p.n2 != 0 && n2[p.n2].left >= 0 && n2[p.n2].count < cut) {
+      deferred.push_back(p);
+      continue;
+    }
     out.depth = std::max(out.depth, p.depth);
-    uint32_t kids[4];
-    int nk = 0;
-    if (n2[p.n2].left < 0) {
-      kids[nk++] = p.n2;  // whole tree is one leaf
-    } else {
-      kids[nk++] = (uint32_t)n2[p.n2].left;
-      kids[nk++] = (uint32_t)n2[p.n2].right;
-      while (nk < 4) {
-        int open = -1;
-        float best = -1.0f;
-        for (int k = 0; k < nk; ++k)
-          if (n2[kids[k]].left >= 0 && n2[kids[k]].box.area() > best) { best = n2[kids[k]].box.area(); open = k; }
-        if (open < 0) break;
-        uint32_t o = kids[open];
-        kids[open] = (uint32_t)n2[o].left;
-        kids[nk++] = (uint32_t)n2[o].right;
-      }
-    }
-    Node4 nd;
-    std::memset(&nd, 0, sizeof nd);
-    nd.count = nk;
-    for (int k = 0; k < 4; ++k) {
-      if (k >= nk) {
-        nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.0f;
-        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.0f;
-        nd.child[k] = 0;
-        continue;
-      }
-      float lo[3], hi[3];
-      pad(n2[kids[k]].box, lo, hi);
-      nd.lox[k] = lo[0]; nd.loy[k] = lo[1]; nd.loz[k] = lo[2];
-      nd.hix[k] = hi[0]; nd.hiy[k] = hi[1]; nd.hiz[k] = hi[2];
-      if (n2[kids[k]].left < 0) {
-        uint32_t first = n2[kids[k]].first, count = n2[kids[k]].count;
-        nd.child[k] = -1 - (int32_t)((first << 2) | (count - 1));
-      } else {
-        uint32_t idx = (uint32_t)wide.size();
-        wide.emplace_back();
-        nd.child[k] = (int32_t)idx;
-        work.push_back({kids[k], idx, p.depth + 1});
-      }
-    }
-    wide[p.n4] = nd;
+    collapse_node(n2, p, wide, work);
   }
-  out.nodes.resize(wide.size());
-  for (size_t i = 0; i < wide.size(); ++i) quantize(wide[i], &out.nodes[i]);
+  std::vector<std::vector<Node4>> sub(deferred.size());
+  std::vector<uint32_t> sub_depth(deferred.size(), 0);
+  parallel_blocks(deferred.size(), threads, [&](uint64_t i) {
+    std::vector<Node4>& local = sub[i];
+    local.resize(1);
+    std::vector<Pending> w{{deferred[i].n2, 0, deferred[i].depth}};
+    while (!w.empty()) {
+      const Pending p = w.back();
+      w.pop_back();
+      sub_depth[i] = std::max(sub_depth[i], p.depth);
+      collapse_node(n2, p, local, w);
+    }
+  });
+  std::vector<size_t> base(deferred.size());
+  size_t total = wide.size();
+  for (size_t i = 0; i < sub.size(); ++i) {
+    base[i] = total;
+    total += sub[i].size() - 1;
+    out.depth = std::max(out.depth, sub_depth[i]);
+  }
+  out.nodes.resize(total);
+  parallel_for(wide.size(), threads, [&](uint64_t i) { quantize(wide[i], &out.nodes[i]); });
+  // a subtree's root takes the slot the top reserved for it; its other nodes follow at base[i]
+  parallel_blocks(sub.size(), threads, [&](uint64_t i) {
+    for (size_t j = 0; j < sub[i].size(); ++j) {
+      Node4 nd = sub[i][j];
+      for (int k = 0; k < 4; ++k)
+        if (nd.child[k] > 0) nd.child[k] = (int32_t)(base[i] + (size_t)nd.child[k] - 1);
+      quantize(nd, &out.nodes[j == 0 ? deferred[i].n4 : base[i] + j - 1]);
+    }
+  });
 }
 
 }  // namespace dl

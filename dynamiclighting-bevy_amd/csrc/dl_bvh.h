@@ -10,6 +10,8 @@
 
 #include <stdint.h>
 
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace dl {
@@ -36,10 +38,31 @@ struct alignas(64) CNode {
 };
 static_assert(sizeof(CNode) == 64, "CNode is 64 bytes");
 
+// An allocator that leaves value-initialized elements uninitialized: for large buffers that are
+// written in full right after they are sized.
+template <class T>
+struct NoInit : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <class U>
+  NoInit(const NoInit<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+
 struct Bvh4 {
   std::vector<CNode> nodes;
-  std::vector<float> tris;      // 12 floats per reference, leaf order
-  std::vector<uint32_t> order;  // leaf position -> occluder index (a split triangle appears more than once)
+  std::vector<float, NoInit<float>> tris;      // 12 floats per reference, leaf order
+  std::vector<uint32_t, NoInit<uint32_t>> order;  // leaf position -> occluder index (a split triangle appears more than once)
   uint32_t depth = 0;           // node levels on the longest root-to-leaf path
 };
 

@@ -492,6 +492,64 @@ int dl_last_timings(const dl_ctx* ctx, dl_timings* out) {
   return DL_OK;
 }
 
+int dl_test_bvh_structure(const float* tris9, uint32_t tri_count, uint32_t threads, uint64_t out[4]) {
+  using namespace dl;
+  if ((!tris9 && tri_count) || !out) return set_err(nullptr, DL_E_ARGUMENT, "null argument");
+  Bvh4 bvh;
+  build_bvh4(tris9, tri_count, threads, bvh);
+  const size_t nn = bvh.nodes.size(), nr = bvh.order.size();
+  auto bad = [](const char* what) { return set_err(nullptr, DL_E_ARGUMENT, std::string("BVH structure: ") + what); };
+  if (tri_count == 0) {
+    if (nn || nr) return bad("nodes without triangles");
+  } else {
+    std::vector<uint8_t> node_seen(nn, 0), ref_seen(nr, 0);
+    std::vector<uint32_t> depth(nn, 0), stack{0};
+    node_seen[0] = 1;
+    depth[0] = 1;
+    uint32_t deepest = 0;
+    while (!stack.empty()) {
+      const uint32_t n = stack.back();
+      stack.pop_back();
+      deepest = std::max(deepest, depth[n]);
+      for (int k = 0; k < 4; ++k) {
+        const int32_t c = bvh.nodes[n].child[k];
+        if (c > 0) {
+          if ((size_t)c >= nn || node_seen[c]++) return bad("an inner child out of range or reached twice");
+          depth[c] = depth[n] + 1;
+          stack.push_back((uint32_t)c);
+        } else if (c < 0) {
+          const uint32_t e = (uint32_t)(-1 - c), first = e >> 2, count = (e & 3u) + 1u;
+          for (uint32_t i = first; i < first + count; ++i)
+            if (i >= nr || ref_seen[i]++) return bad("a reference out of range or in two leaves");
+        }
+      }
+    }
+    for (size_t i = 0; i < nn; ++i)
+      if (!node_seen[i]) return bad("a node not reached from the root");
+    for (size_t i = 0; i < nr; ++i)
+      if (!ref_seen[i]) return bad("a reference in no leaf");
+    if (deepest != bvh.depth) return bad("depth differs from the longest path");
+    for (size_t k = 0; k < nr; ++k) {
+      const uint32_t p = bvh.order[k];
+      if (p >= tri_count) return bad("a reference of no triangle");
+      const float* t = tris9 + (size_t)p * 9;
+      const float* o = bvh.tris.data() + k * 12;
+      uint32_t q;
+      std::memcpy(&q, &o[3], 4);
+      if (q != p || o[0] != t[0] || o[1] != t[1] || o[2] != t[2] || o[4] != t[3] - t[0] || o[5] != t[4] - t[1] ||
+          o[6] != t[5] - t[2] || o[8] != t[6] - t[0] || o[9] != t[7] - t[1] || o[10] != t[8] - t[2])
+        return bad("a leaf record differs from its triangle");
+    }
+  }
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the reference order
+  for (size_t k = 0; k < nr; ++k) h = (h ^ bvh.order[k]) * 1099511628211ull;
+  out[0] = nn;
+  out[1] = nr;
+  out[2] = bvh.depth;
+  out[3] = h;
+  return DL_OK;
+}
+
 int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, const float* rays8, uint64_t ray_count,
                         int closest, float* t_out, uint32_t* hit_or_index) {
   using namespace dl;

@@ -289,8 +289,8 @@ inline double ms_since(std::chrono::steady_clock::time_point t0) {
 hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes);
 hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes);
 
-template <class T>
-hipError_t upload(dl_ctx* ctx, DBuf<T>& b, const std::vector<T>& h) {
+template <class T, class A>
+hipError_t upload(dl_ctx* ctx, DBuf<T>& b, const std::vector<T, A>& h) {
   hipError_t e = b.alloc(h.size());
   if (e != hipSuccess || h.empty()) return e;
   return copy_to_device(ctx, b.p, h.data(), h.size() * sizeof(T));

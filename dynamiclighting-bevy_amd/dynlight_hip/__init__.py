@@ -100,6 +100,8 @@ def load() -> C.CDLL:
     lib.dl_test_trig_host.argtypes = [f32p, C.c_uint64, f32p, f32p]
     lib.dl_test_trig_exhaustive.restype = C.c_int
     lib.dl_test_trig_exhaustive.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+    lib.dl_test_bvh_structure.restype = C.c_int
+    lib.dl_test_bvh_structure.argtypes = [f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
     lib.dl_test_trig_keys.restype = C.c_uint64
     lib.dl_test_trig_keys.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.c_uint64]
     lib.dl_test_trig_table_sizes.restype = None

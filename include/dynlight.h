@@ -324,6 +324,8 @@ typedef struct dl_timings {
     uint64_t bounce_launches;   /* mesh batches with bounce work                              */
     double gather_trace_ms;     /* summed device time of the any-hit gather kernel launches   */
     uint64_t gather_trace_launches;
+    double prepare_ms;          /* host: validation, occluders, BVH build, receiver triangles */
+    double bvh_ms;              /* host: the occluder BVH build alone                         */
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 

@@ -21,6 +21,13 @@ extern "C" {
 int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, const float* rays8,
                         uint64_t ray_count, int closest, float* t_out, uint32_t* hit_or_index);
 
+/* Host only (no device): builds the occluder BVH over `tri_count` triangles (9 floats each) on
+ * `threads` threads and checks its structure: every node is reached once from the root, the
+ * leaves' reference ranges partition the references, and each leaf record holds its triangle
+ * (v0, occluder index, edges).  out: node count, reference count, depth, and a hash of the
+ * references' triangle order (equal for every thread count).  DL_E_ARGUMENT when a check fails. */
+int dl_test_bvh_structure(const float* tris9, uint32_t tri_count, uint32_t threads, uint64_t out[4]);
+
 /* Evaluates the device f32 sin and cos at `n` inputs. */
 int dl_test_trig(dl_ctx* ctx, const float* x, uint64_t n, float* sin_out, float* cos_out);
 
