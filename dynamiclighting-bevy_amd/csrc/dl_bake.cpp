@@ -1040,15 +1040,16 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(launch_bounce_pack(s, tr, ms, d_lights.p, ent, bb, d_kde.p, d_kdws.p, kd_entry.size(), kd_flat, d_kdout.p, d_comp.p,
                                  opt.colored_bounce ? 1u : 0u, opt.seed, 1, d_out.p));
     }
-    std::vector<uint32_t> words;
-    HIP_TRY(download(ctx, words, d_out.p, bw));
+    // the batch's words in one buffer, each mesh holding its slice
+    auto words = std::make_shared<std::vector<uint32_t, NoInit<uint32_t>>>(bw);
+    HIP_TRY(copy_to_host(ctx, words->data(), d_out.p, bw * sizeof(uint32_t)));
     for (uint32_t i = B.m0; i < B.m1; ++i) {
       dl_baked_mesh_out& mo = scene->meshes[i];
       mo.id = meta[i].id;
       mo.content_hash = meta[i].content_hash;
       mo.resolution = res[i];
       mo.occludes = meta[i].occludes;
-      mo.words.assign(words.begin() + (long)mesh_base[i], words.begin() + (long)mesh_base[i + 1]);
+      mo.words.share(words, words->data() + mesh_base[i], mesh_base[i + 1] - mesh_base[i]);
     }
     post_ms += ms_since(t_post);
     advance(B.m1 - B.m0);

@@ -350,7 +350,7 @@ dl_baked_scene* parse_payload(Reader& r) {
     const uint32_t occ = r.u32();
     if (occ > 1) fail(DL_FORMAT_INVALID, "mesh shadow flag");
     m.occludes = occ;
-    m.words = r.words();
+    m.words.own(r.words());
     s->meshes.push_back(std::move(m));
   }
 

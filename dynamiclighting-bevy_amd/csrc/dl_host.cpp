@@ -444,6 +444,41 @@ hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes)
 hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return hipSuccess;
   hipError_t e = hipSuccess;
+  if (bytes > kStagedMax) {
+    // through the staging buffer in two alternating halves: the device copy of one piece runs
+    // while the host copies the previous piece out
+    if (!staging_slot(ctx, 1, &e)) {  // no pinned memory: one copy into pageable memory
+      if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+      return e == hipSuccess ? hipStreamSynchronize(ctx->stream) : e;
+    }
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;  // earlier staged copies
+    Staging& st = ctx->staging;
+    const size_t half = st.cap / 2;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& v : ev)
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    size_t prev_off = 0, prev_len = 0;
+    for (size_t off = 0, k = 0; e == hipSuccess && off < bytes; off += half, ++k) {
+      const size_t len = std::min(half, bytes - off);
+      char* slot = st.host + (k & 1u) * half;
+      e = hipMemcpyAsync(slot, from + off, len, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ev[k & 1u], ctx->stream);
+      if (e == hipSuccess && k > 0) {
+        e = hipEventSynchronize(ev[(k - 1) & 1u]);
+        if (e == hipSuccess) std::memcpy(to + prev_off, st.host + ((k - 1) & 1u) * half, prev_len);
+      }
+      prev_off = off;
+      prev_len = len;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) std::memcpy(to + prev_off, st.host + ((((bytes + half - 1) / half) - 1) & 1u) * half, prev_len);
+    for (auto& v : ev)
+      if (v) (void)hipEventDestroy(v);
+    st.off = 0;
+    return e;
+  }
   char* slot = staging_slot(ctx, bytes, &e);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(slot ? slot : dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);

@@ -175,10 +175,33 @@ struct dl_ctx {
   std::vector<double> batch_span;
 };
 
+// A run of words that shares ownership of the buffer holding it: a bake downloads the triangle
+// words of a batch of meshes into one buffer and each mesh keeps its slice of it.
+struct dl_word_slice {
+  std::shared_ptr<const uint32_t> first;  // aliases the owner of the whole buffer
+  size_t n = 0;
+  const uint32_t* data() const { return first.get(); }
+  size_t size() const { return n; }
+  bool empty() const { return n == 0; }
+  const uint32_t& operator[](size_t i) const { return first.get()[i]; }
+  const uint32_t* begin() const { return first.get(); }
+  const uint32_t* end() const { return first.get() + n; }
+  // a slice of [p, p + count) kept alive by `owner`
+  template <class Owner>
+  void share(const std::shared_ptr<Owner>& owner, const uint32_t* p, size_t count) {
+    first = std::shared_ptr<const uint32_t>(owner, p);
+    n = count;
+  }
+  void own(std::vector<uint32_t>&& v) {
+    auto h = std::make_shared<std::vector<uint32_t>>(std::move(v));
+    share(h, h->data(), h->size());
+  }
+};
+
 struct dl_baked_mesh_out {
   uint64_t id, content_hash;
   uint32_t resolution, occludes;
-  std::vector<uint32_t> words;
+  dl_word_slice words;
 };
 
 struct dl_sun_map_out {
