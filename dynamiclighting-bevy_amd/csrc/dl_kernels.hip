@@ -2144,9 +2144,12 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
         for (int j = 0; j < G; ++j) {
           if (k0 + j >= g.samples) continue;
           nr += ray[j] ? 1u : 0u;
+          // a ray whose contribution is exactly zero adds +0 to the sum whether or not it is
+          // occluded: it counts as cast but is not traced (NaN contributions are traced)
+          const bool trace = ray[j] && !(c[j].x == 0.0f && c[j].y == 0.0f && c[j].z == 0.0f);
           const uint64_t r = ray_slot(order, base, i, n, k0 + j);
           rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist[j]);
-          rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | (ray[j] ? 2u : 0u)));
+          rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | (trace ? 2u : 0u)));
           contrib[3 * r] = c[j].x;
           contrib[3 * r + 1] = c[j].y;
           contrib[3 * r + 2] = c[j].z;
