@@ -840,11 +840,17 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         for (uint64_t e : be) max_samples = std::max(max_samples, hlights[b_ent_light[e]].bounce_samples);
         const uint64_t per_chunk = std::max<uint64_t>(1, ctx->gather.chunk_rays / max_samples);
         const uint64_t cj = std::min<uint64_t>(per_chunk, std::max<uint64_t>(1, ntjobs));
-        DBuf<uint64_t> d_nr, d_rs;
+        const uint32_t mask_words = (max_samples + 31) / 32;
+        DBuf<uint64_t> d_nr, d_rs, d_tcount, d_toff;
         DBuf<float4> d_rays;
         DBuf<float> d_contrib;
+        DBuf<uint32_t> d_tmask, d_list;
         HIP_TRY(d_nr.alloc(cj + 1));
         HIP_TRY(d_rs.alloc(cj + 1));
+        HIP_TRY(d_tcount.alloc(cj + 1));
+        HIP_TRY(d_toff.alloc(cj + 1));
+        HIP_TRY(d_tmask.alloc(cj * mask_words));
+        HIP_TRY(d_list.alloc(cj * max_samples));
         HIP_TRY(d_rays.alloc(2 * cj * max_samples));
         HIP_TRY(d_contrib.alloc(3 * cj * max_samples));
         std::vector<hipEvent_t> tev;
@@ -858,7 +864,10 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(launch_gather_nrays(s, ent, d_lights.p, jobs, n, d_nr.p));
           HIP_TRY(scan_u64(s, d_nr.p, d_rs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
           HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
-                                     ctx->gather.order));
+                                     ctx->gather.order, d_tmask.p, mask_words, d_tcount.p));
+          // the traced rays only, job by job
+          HIP_TRY(scan_u64(s, d_tcount.p, d_toff.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+          HIP_TRY(launch_gather_list(s, d_tmask.p, mask_words, d_toff.p, d_rs.p, n, ctx->gather.order, d_list.p));
           HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, sizeof(unsigned long long), s));
           hipEvent_t a, b2;
           HIP_TRY(hipEventCreate(&a));
@@ -866,8 +875,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(hipEventCreate(&b2));
           tev.push_back(b2);
           HIP_TRY(hipEventRecord(a, s));
-          HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_rs.p + n, n * max_samples, d_rs.p, n, d_contrib.p,
-                                    reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather));
+          HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_toff.p + n, n * max_samples, d_rs.p, n, d_contrib.p,
+                                    reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather, d_list.p));
           HIP_TRY(hipEventRecord(b2, s));
           HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p, ctx->gather.order));
         }

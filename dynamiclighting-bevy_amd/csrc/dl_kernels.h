@@ -217,11 +217,17 @@ hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, co
 hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out);
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
-                              float4* rays, float* contrib, uint32_t order);
+                              float4* rays, float* contrib, uint32_t order, uint32_t* tmask, uint32_t words, uint64_t* tcount);
+// The slots of the traced rays of a chunk (a job's in sample order, jobs in order) from the
+// per-job bits `tmask` (words per job) and the scanned traced counts `toff`.
+hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t words, const uint64_t* toff, const uint64_t* rs,
+                              uint64_t n, uint32_t order, uint32_t* list);
 // max_rays: the chunk's ray slots (jobs x the most samples of any of its lights); rs, njobs: the
-// jobs' ray offsets and count, which sample-major order needs to skip the slots past a job's samples
+// jobs' ray offsets and count, which sample-major order needs to skip the slots past a job's samples.
+// list (optional): the slots to trace, *total of them, in this order.
 hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl);
+                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl,
+                             const uint32_t* list = nullptr);
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
                              uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order);
 // Closest-hit form of launch_any_stream: every traced ray that hits writes (t, u, v, leaf position)

@@ -2121,9 +2121,11 @@ __global__ void k_gather_nrays(DevEntries ent, const DevLight* __restrict__ ligh
 // brings if unoccluded (zero when it casts none).  Also counts each mesh's bounce rays.
 __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
-                              BounceParams p, TrigTables trig, float4* rays, float* contrib, uint32_t order) {
+                              BounceParams p, TrigTables trig, float4* rays, float* contrib, uint32_t order, uint32_t* tmask,
+                              uint32_t words, uint64_t* tcount) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) tcount[n] = 0;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
     const uint64_t i = i0 + lane;
     uint32_t m = 0xffffffffu, nr = 0;
@@ -2135,6 +2137,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
       const uint64_t base = rs[i];
       const float la = len(g.dir0);
       constexpr int G = kGatherGroup;
+      uint32_t word = 0, traced = 0;  // bits of the traced samples, 32 per word of tmask
       for (uint32_t k0 = 0; k0 < g.samples; k0 += G) {
         f3 d[G], c[G];
         float dist[G];
@@ -2153,8 +2156,19 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
           contrib[3 * r] = c[j].x;
           contrib[3 * r + 1] = c[j].y;
           contrib[3 * r + 2] = c[j].z;
+          const uint32_t k = k0 + j;
+          if (trace) {
+            word |= 1u << (k & 31u);
+            traced++;
+          }
+          if ((k & 31u) == 31u) {
+            tmask[i * words + (k >> 5)] = word;
+            word = 0;
+          }
         }
       }
+      for (uint32_t w = g.samples >> 5; w < words; ++w) tmask[i * words + w] = w == (g.samples >> 5) ? word : 0u;
+      tcount[i] = traced;
     }
     const uint32_t m0 = __shfl(m, 0, 64);
     if (__ballot(i < n && m != m0) == 0ull) {
@@ -2162,6 +2176,19 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
     } else if (i < n && nr) {
       atomicAdd(&b.rays[m], (unsigned long long)nr);
     }
+  }
+}
+
+// The ray slots of the traced samples of a chunk's jobs, job by job in sample order, at the
+// scanned offsets of the jobs' traced counts: the any-hit stream then walks traced rays only.
+__global__ void k_gather_list(const uint32_t* __restrict__ tmask, uint32_t words, const uint64_t* __restrict__ toff,
+                              const uint64_t* __restrict__ rs, uint64_t n, uint32_t order, uint32_t* list) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t at = toff[i];
+    const uint64_t base = rs[i];
+    for (uint32_t w = 0; w < words; ++w)
+      for (uint32_t m = tmask[i * words + w]; m; m &= m - 1u)
+        list[at++] = (uint32_t)ray_slot(order, base, i, n, w * 32u + (uint32_t)__builtin_ctz(m));
   }
 }
 
@@ -2175,14 +2202,16 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
 template <int MIN_WAVES, bool CLOSEST>
 __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(DevScene sc, const float4* __restrict__ rays, const uint64_t* __restrict__ total, float* contrib,
                              unsigned int* queue, uint32_t chunk, uint32_t refill, uint32_t order,
-                             const uint64_t* __restrict__ rs, uint32_t njobs, uint32_t positions) {
+                             const uint64_t* __restrict__ rs, uint32_t njobs, uint32_t positions,
+                             const uint32_t* __restrict__ list) {
   extern __shared__ uint32_t lds[];
   uint32_t* stack = lds + threadIdx.x;
   Tally tl;
   const uint32_t lane = threadIdx.x & 63u;
   // slots to hand out: the rays of the chunk (job-major), or every (sample, job) slot (sample-major,
   // where slots past a job's sample count hold no ray)
-  const uint32_t nrays = order ? positions : (uint32_t)*total;  // a chunk holds fewer than 2^32 slots
+  // with a list: the *total listed ray slots, in list order
+  const uint32_t nrays = order && !list ? positions : (uint32_t)*total;  // a chunk holds fewer than 2^32 slots
   const uint32_t per_job = order ? positions / njobs : 0;       // slots per job (the most samples)
   uint32_t cbase = 0, cend = 0;
   bool drained = false;
@@ -2233,7 +2262,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
         if (!has_next && rank < avail) {
           nidx = cbase + rank;
           bool slot = true;
-          if (order) {  // hand out a job's samples together, from their sample-major slots
+          if (list) {
+            nidx = list[nidx];
+          } else if (order) {  // hand out a job's samples together, from their sample-major slots
             const uint32_t job = nidx / per_job, k = nidx - job * per_job;
             slot = rs == nullptr || (uint64_t)k < rs[job + 1] - rs[job];
             nidx = k * njobs + job;
@@ -2711,15 +2742,16 @@ hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* li
 }
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
-                              float4* rays, float* contrib, uint32_t order) {
+                              float4* rays, float* contrib, uint32_t order, uint32_t* tmask, uint32_t words, uint64_t* tcount) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather_rays, dim3(grid_for(n, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, b, jobs, n, rs, p, trig,
-                     rays, contrib, order);
+                     rays, contrib, order, tmask, words, tcount);
   return hipGetLastError();
 }
 template <bool CLOSEST>
 static hipError_t launch_ray_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                                    const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
+                                    const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl,
+                                    const uint32_t* list) {
   if (max_rays >= (1ull << 31) || gl.chunk == 0 || gl.chunk > (1u << 20)) return hipErrorInvalidValue;
   if (max_rays == 0) return hipSuccess;
   const size_t lds = stack_bytes(scene, gl.block);
@@ -2735,18 +2767,26 @@ static hipError_t launch_ray_stream(hipStream_t s, DevScene scene, const float4*
   if (grid > useful) grid = useful;
   // the 32-bit queue may pass the ray count by one run per wave before every wave has drained
   if (max_rays + (uint64_t)gl.chunk * grid * (gl.block / 64) >= (1ull << 32)) return hipErrorInvalidValue;
-  if (gl.order && (njobs == 0 || njobs >= (1ull << 32))) return hipErrorInvalidValue;
+  if (!list && gl.order && (njobs == 0 || njobs >= (1ull << 32))) return hipErrorInvalidValue;
+  if (list && !total) return hipErrorInvalidValue;
   hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(gl.block), lds, s, scene, rays, total, contrib, queue, gl.chunk,
-                     std::max(1u, std::min(64u, gl.refill)), gl.order ? 1u : 0u, rs, (uint32_t)njobs, (uint32_t)max_rays);
+                     std::max(1u, std::min(64u, gl.refill)), gl.order ? 1u : 0u, rs, (uint32_t)njobs, (uint32_t)max_rays, list);
   return hipGetLastError();
 }
 hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
-                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl) {
-  return launch_ray_stream<false>(s, scene, rays, total, max_rays, rs, njobs, contrib, queue, gl);
+                             const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl,
+                             const uint32_t* list) {
+  return launch_ray_stream<false>(s, scene, rays, total, max_rays, rs, njobs, contrib, queue, gl, list);
 }
 hipError_t launch_closest_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,
                                  uint64_t njobs, float4* hits, unsigned int* queue, const GatherLaunch& gl) {
-  return launch_ray_stream<true>(s, scene, rays, total, max_rays, nullptr, njobs, reinterpret_cast<float*>(hits), queue, gl);
+  return launch_ray_stream<true>(s, scene, rays, total, max_rays, nullptr, njobs, reinterpret_cast<float*>(hits), queue, gl, nullptr);
+}
+hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t words, const uint64_t* toff, const uint64_t* rs,
+                              uint64_t n, uint32_t order, uint32_t* list) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_list, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, tmask, words, toff, rs, n, order, list);
+  return hipGetLastError();
 }
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
                              uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order) {
