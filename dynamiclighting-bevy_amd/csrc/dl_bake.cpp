@@ -703,6 +703,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(d_vcontrib.alloc(3 * cn * slots));
       HIP_TRY(d_vwts.alloc(cn * slots));
       HIP_TRY(d_vstate.alloc(cn + 1));
+      DBuf<uint64_t> d_lcount, d_loff;  // traced slots of a phase, job by job
+      DBuf<uint32_t> d_llist;
+      HIP_TRY(d_lcount.alloc(cn + 1));
+      HIP_TRY(d_loff.alloc(cn + 1));
+      HIP_TRY(d_llist.alloc(cn * slots));
       GatherLaunch vgl = ctx->gather;
       vgl.order = 1;
       for (uint64_t q0 = 0; q0 < njobs; q0 += cn) {
@@ -711,8 +716,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         HIP_TRY(scan_u64(s, d_vcnt.p, d_vrs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
         HIP_TRY(launch_vis_rays_a(s, tr, ms, d_lights.p, tx, ent, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, ctx->trig,
                                   d_vrays.p, d_vcontrib.p, d_vwts.p, d_vstate.p));
+        HIP_TRY(launch_ray_list(s, d_vrays.p, n, vis_slots_a, d_vrs.p, 1, d_lcount.p, d_loff.p, d_llist.p, &ctx->scan_tmp,
+                                &ctx->scan_tmp_bytes));
         HIP_TRY(hipMemsetAsync(d_vq.p, 0, sizeof(unsigned int), s));
-        HIP_TRY(launch_any_stream(s, sct, d_vrays.p, nullptr, n * vis_slots_a, d_vrs.p, n, d_vcontrib.p, d_vq.p, vgl));
+        HIP_TRY(launch_any_stream(s, sct, d_vrays.p, d_loff.p + n, n * vis_slots_a, d_vrs.p, n, d_vcontrib.p, d_vq.p, vgl,
+                                  d_llist.p));
         HIP_TRY(launch_vis_mid(s, ent, d_lights.p, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, d_vrays.p, d_vcontrib.p,
                                d_vwts.p, d_vstate.p, d_vneed.p));
         HIP_TRY(scan_u64(s, d_vneed.p, d_vbscan.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
@@ -724,8 +732,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(launch_vis_list(s, d_vneed.p, d_vbscan.p, n, d_vblist.p));
           HIP_TRY(launch_vis_rays_b(s, tr, ms, d_lights.p, tx, ent, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, d_vblist.p, nb, tp,
                                     ctx->trig, d_vrays.p, d_vcontrib.p, d_vwts.p));
+          HIP_TRY(launch_ray_list(s, d_vrays.p, nb, slots_b, nullptr, 1, d_lcount.p, d_loff.p, d_llist.p, &ctx->scan_tmp,
+                                  &ctx->scan_tmp_bytes));
           HIP_TRY(hipMemsetAsync(d_vq.p, 0, sizeof(unsigned int), s));
-          HIP_TRY(launch_any_stream(s, sct, d_vrays.p, nullptr, nb * slots_b, nullptr, nb, d_vcontrib.p, d_vq.p, vgl));
+          HIP_TRY(launch_any_stream(s, sct, d_vrays.p, d_loff.p + nb, nb * slots_b, nullptr, nb, d_vcontrib.p, d_vq.p, vgl,
+                                    d_llist.p));
         }
         HIP_TRY(launch_vis_finish(s, tr, ms, d_lights.p, tx, ent, rg, d_vjs.p, d_vme.p, nme, d_tex_start.p, q0, n, tp, d_vstate.p,
                                   d_vbscan.p, nb, d_vrays.p, d_vcontrib.p, d_vwts.p));
@@ -878,7 +889,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(launch_any_stream(s, scb, d_rays.p, d_toff.p + n, n * max_samples, d_rs.p, n, d_contrib.p,
                                     reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather, d_list.p));
           HIP_TRY(hipEventRecord(b2, s));
-          HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p, ctx->gather.order));
+          HIP_TRY(launch_gather_sum(s, ent, d_lights.p, tx, bb, jobs, n, d_rs.p, d_contrib.p, ctx->gather.order, d_tmask.p,
+                                    mask_words));
         }
         if (sun_bounce && nsjobs) {
           // sun bounce in chunks of jobs: cosine rays, closest hits, sun rays, their visibility, sums
@@ -888,7 +900,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           const uint64_t sj = std::min<uint64_t>(nsjobs, std::max<uint64_t>(1, ctx->gather.chunk_rays / 2 / sun_slots));
           DBuf<float4> d_srays, d_shits;
           DBuf<float> d_scontrib;
-          DBuf<uint64_t> d_stotal;
+          DBuf<uint64_t> d_stotal, d_scount, d_soff;
+          DBuf<uint32_t> d_slist;
+          HIP_TRY(d_scount.alloc(sj + 1));
+          HIP_TRY(d_soff.alloc(sj + 1));
+          HIP_TRY(d_slist.alloc(sj * sun_slots));
           HIP_TRY(d_srays.alloc(2 * sj * sun_slots));
           HIP_TRY(d_shits.alloc(sj * sun_slots));
           HIP_TRY(d_scontrib.alloc(3 * sj * sun_slots));
@@ -903,8 +919,10 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
                                           reinterpret_cast<unsigned int*>(d_jobc.p + 2), ctx->gather));
             HIP_TRY(launch_sun_mid(s, scb, occ, d_lights.p, ent, tx, bb, jobs, n, sun_slots, ctx->gather.order, bp.colored,
                                    d_srays.p, d_shits.p, d_scontrib.p));
-            HIP_TRY(launch_any_stream(s, scb, d_srays.p, d_stotal.p, n * sun_slots, nullptr, n, d_scontrib.p,
-                                      reinterpret_cast<unsigned int*>(d_jobc.p + 3), ctx->gather));
+            HIP_TRY(launch_ray_list(s, d_srays.p, n, sun_slots, nullptr, ctx->gather.order, d_scount.p, d_soff.p, d_slist.p,
+                                    &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+            HIP_TRY(launch_any_stream(s, scb, d_srays.p, d_soff.p + n, n * sun_slots, nullptr, n, d_scontrib.p,
+                                      reinterpret_cast<unsigned int*>(d_jobc.p + 3), ctx->gather, d_slist.p));
             HIP_TRY(launch_sun_sum(s, tr, ent, d_lights.p, tx, bb, jobs, n, sun_slots, ctx->gather.order, d_srays.p,
                                    d_scontrib.p));
           }

@@ -222,6 +222,11 @@ hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, con
 // per-job bits `tmask` (words per job) and the scanned traced counts `toff`.
 hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t words, const uint64_t* toff, const uint64_t* rs,
                               uint64_t n, uint32_t order, uint32_t* list);
+// The traced slots (flag 2) of n jobs of `slots` ray slots each (the first rs[i + 1] - rs[i] of
+// job i when rs is given) stored at ray_slot(order, ...), job by job: count (n + 1), off (n + 1;
+// off[n] is the total) and list (up to n * slots).
+hipError_t launch_ray_list(hipStream_t s, const float4* rays, uint64_t n, uint32_t slots, const uint64_t* rs, uint32_t order,
+                           uint64_t* count, uint64_t* off, uint32_t* list, void** tmp, size_t* tmp_bytes);
 // max_rays: the chunk's ray slots (jobs x the most samples of any of its lights); rs, njobs: the
 // jobs' ray offsets and count, which sample-major order needs to skip the slots past a job's samples.
 // list (optional): the slots to trace, *total of them, in this order.
@@ -229,7 +234,8 @@ hipError_t launch_any_stream(hipStream_t s, DevScene scene, const float4* rays, 
                              const uint64_t* rs, uint64_t njobs, float* contrib, unsigned int* queue, const GatherLaunch& gl,
                              const uint32_t* list = nullptr);
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
-                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order);
+                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order, const uint32_t* tmask,
+                             uint32_t words);
 // Closest-hit form of launch_any_stream: every traced ray that hits writes (t, u, v, leaf position)
 // to its slot of `hits`; slots of rays that miss keep what they held.  rs is implied uniform.
 hipError_t launch_closest_stream(hipStream_t s, DevScene scene, const float4* rays, const uint64_t* total, uint64_t max_rays,

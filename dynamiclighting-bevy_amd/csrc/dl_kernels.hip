@@ -2149,15 +2149,17 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
           nr += ray[j] ? 1u : 0u;
           // a ray whose contribution is exactly zero adds +0 to the sum whether or not it is
           // occluded: it counts as cast but is not traced (NaN contributions are traced)
+          // Only traced samples keep a ray and a contribution: the others add +0 to the sum,
+          // which k_gather_sum skips by the same bits.
           const bool trace = ray[j] && !(c[j].x == 0.0f && c[j].y == 0.0f && c[j].z == 0.0f);
-          const uint64_t r = ray_slot(order, base, i, n, k0 + j);
-          rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist[j]);
-          rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | (trace ? 2u : 0u)));
-          contrib[3 * r] = c[j].x;
-          contrib[3 * r + 1] = c[j].y;
-          contrib[3 * r + 2] = c[j].z;
           const uint32_t k = k0 + j;
           if (trace) {
+            const uint64_t r = ray_slot(order, base, i, n, k);
+            rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist[j]);
+            rays[2 * r + 1] = make_float4(d[j].x, d[j].y, d[j].z, __uint_as_float(alpha | 2u));
+            contrib[3 * r] = c[j].x;
+            contrib[3 * r + 1] = c[j].y;
+            contrib[3 * r + 2] = c[j].z;
             word |= 1u << (k & 31u);
             traced++;
           }
@@ -2189,6 +2191,27 @@ __global__ void k_gather_list(const uint32_t* __restrict__ tmask, uint32_t words
     for (uint32_t w = 0; w < words; ++w)
       for (uint32_t m = tmask[i * words + w]; m; m &= m - 1u)
         list[at++] = (uint32_t)ray_slot(order, base, i, n, w * 32u + (uint32_t)__builtin_ctz(m));
+  }
+}
+
+// The traced ray slots (flag 2) of n jobs, job by job in sample order.  Job i owns its first
+// rs[i + 1] - rs[i] samples (all `slots` without rs), stored at ray_slot(order, base, i, n, k)
+// with base rs[i] (i * slots without rs).  Pass 0 counts each job's traced rays into count
+// (count[n] = 0, for the scan); pass 1 writes them at the scanned offsets.
+__global__ void k_ray_list(const float4* __restrict__ rays, uint64_t n, uint32_t slots, const uint64_t* __restrict__ rs,
+                           uint32_t order, int pass, uint64_t* count, const uint64_t* __restrict__ off, uint32_t* list) {
+  if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) count[n] = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t mine = rs ? (uint32_t)(rs[i + 1] - rs[i]) : slots;
+    const uint64_t base = rs ? rs[i] : i * slots;
+    uint64_t at = pass == 1 ? off[i] : 0;
+    for (uint32_t k = 0; k < mine; ++k) {
+      const uint64_t slot = ray_slot(order, base, i, n, k);
+      if (!(__float_as_uint(rays[2 * slot + 1].w) & 2u)) continue;
+      if (pass == 1) list[at] = (uint32_t)slot;
+      at++;
+    }
+    if (pass == 0) count[i] = at;
   }
 }
 
@@ -2310,20 +2333,23 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
 }
 
 // Phase 3: each job's samples summed in sample order (bounce.rs:287-290) into its window texel.
-// Samples without a ray and occluded samples hold +0, which leaves the sum unchanged.
+// Only the traced samples (the bits of tmask) are read: the others contribute +0, occluded ones
+// hold +0, and adding +0 leaves the sum (never -0) unchanged.
 __global__ void k_gather_sum(DevEntries ent, const DevLight* __restrict__ lights, DevTexels tx, DevBounce b,
                              const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
-                             const float* __restrict__ contrib, uint32_t order) {
+                             const float* __restrict__ contrib, uint32_t order, const uint32_t* __restrict__ tmask,
+                             uint32_t words) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint2 jb = jobs[i];
     const DevLight& L = lights[ent.light[jb.y]];
     const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
     const uint64_t base = rs[i];
     f3 sum = mk3(0.0f, 0.0f, 0.0f);
-    for (uint32_t k = 0; k < samples; ++k) {
-      const float* c = contrib + 3 * ray_slot(order, base, i, n, k);
-      sum = add(sum, mk3(c[0], c[1], c[2]));
-    }
+    for (uint32_t w = 0; w < words; ++w)
+      for (uint32_t m = tmask[i * words + w]; m; m &= m - 1u) {
+        const float* c = contrib + 3 * ray_slot(order, base, i, n, w * 32u + (uint32_t)__builtin_ctz(m));
+        sum = add(sum, mk3(c[0], c[1], c[2]));
+      }
     const f3 v = divs(mul(sum, L.bounce_intensity), (float)samples);
     const uint32_t w = b.ent_window[jb.y];
     const uint32_t* wr = b.win_rect + 4 * (size_t)w;
@@ -2782,6 +2808,18 @@ hipError_t launch_closest_stream(hipStream_t s, DevScene scene, const float4* ra
                                  uint64_t njobs, float4* hits, unsigned int* queue, const GatherLaunch& gl) {
   return launch_ray_stream<true>(s, scene, rays, total, max_rays, nullptr, njobs, reinterpret_cast<float*>(hits), queue, gl, nullptr);
 }
+hipError_t launch_ray_list(hipStream_t s, const float4* rays, uint64_t n, uint32_t slots, const uint64_t* rs, uint32_t order,
+                           uint64_t* count, uint64_t* off, uint32_t* list, void** tmp, size_t* tmp_bytes) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ray_list, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, rays, n, slots, rs, order, 0, count,
+                     (const uint64_t*)nullptr, (uint32_t*)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = scan_u64(s, count, off, n, tmp, tmp_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ray_list, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, rays, n, slots, rs, order, 1, count,
+                     (const uint64_t*)off, list);
+  return hipGetLastError();
+}
 hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t words, const uint64_t* toff, const uint64_t* rs,
                               uint64_t n, uint32_t order, uint32_t* list) {
   if (n == 0) return hipSuccess;
@@ -2789,9 +2827,11 @@ hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t wor
   return hipGetLastError();
 }
 hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* lights, DevTexels tx, DevBounce b, const uint2* jobs,
-                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order) {
+                             uint64_t n, const uint64_t* rs, const float* contrib, uint32_t order, const uint32_t* tmask,
+                             uint32_t words) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, ent, lights, tx, b, jobs, n, rs, contrib, order);
+  hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, ent, lights, tx, b, jobs, n, rs, contrib, order,
+                     tmask, words);
   return hipGetLastError();
 }
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,
