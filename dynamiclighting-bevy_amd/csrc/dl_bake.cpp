@@ -756,7 +756,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     DBuf<uint8_t> d_has;
     DBuf<float> d_map_a, d_map_b;
     DBuf<uint32_t> d_win_rect, d_win_res, d_ent_window;
-    DBuf<uint64_t> d_win_off, d_win_start, d_be, d_be_units;
+    DBuf<uint64_t> d_win_off, d_win_start, d_be, d_be_units;  // d_win_start: filter tiles per window
     DevBounce bb{};
     if (bounce_in_scene && nbw > 0) {
       std::vector<uint64_t> win_off(nbw + 1, 0);
@@ -788,7 +788,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(upload(ctx, d_win_rect, b_win_rect));
       HIP_TRY(upload(ctx, d_win_res, b_win_res));
       HIP_TRY(upload(ctx, d_win_off, win_off));
-      HIP_TRY(upload(ctx, d_win_start, win_off));
+      std::vector<uint64_t> win_tiles(nbw + 1, 0);  // 16 x 16 tiles of the filter per window
+      for (uint32_t w = 0; w < nbw; ++w) {
+        const uint32_t* q = &b_win_rect[4 * (size_t)w];
+        win_tiles[w + 1] = win_tiles[w] + (uint64_t)((q[2] - q[0] + 16) / 16) * ((q[3] - q[1] + 16) / 16);
+      }
+      HIP_TRY(upload(ctx, d_win_start, win_tiles));
       HIP_TRY(upload(ctx, d_ent_window, b_ent_window));
       HIP_TRY(upload(ctx, d_be, be));
       HIP_TRY(upload(ctx, d_be_units, be_units));
@@ -944,7 +949,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       (void)hipEventElapsedTime(&gms, ev0, ev1);
       bounce_kernel_ms += gms;
       ctx->timings.bounce_launches++;
-      HIP_TRY(launch_bounce_filter(s, bb, d_win_res.p, d_win_start.p, nbw, win_off[nbw], blur));
+      HIP_TRY(launch_bounce_filter(s, bb, d_win_res.p, d_win_start.p, nbw, win_tiles[nbw], blur));
       HIP_TRY(d_has.alloc(nbe + 1));
       HIP_TRY(launch_bounce_has(s, tr, ent, bb, d_has.p));
       HIP_TRY(hipStreamSynchronize(s));
@@ -1116,7 +1121,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   ctx->timings.bounce_kernel_ms = bounce_kernel_ms;
   if (ctx->counting) {
     unsigned long long c[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpy(c, d_counters.p, 32, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(ctx, c, d_counters.p, 32));
     ctx->timings.trace_node_visits = c[0];
     ctx->timings.trace_tri_tests = c[1];
     ctx->timings.bounce_node_visits = c[2];
@@ -1211,7 +1216,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   HIP_TRY(download(ctx, h_samples, d_msamples.p, nm));
   HIP_TRY(download(ctx, h_brays, d_mbrays.p, nm));
   uint32_t overflow = 0;
-  HIP_TRY(hipMemcpy(&overflow, d_overflow.p, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(copy_to_host(ctx, &overflow, d_overflow.p, 4));
   if (overflow) { ctx->err = "BVH traversal stack overflow"; return DL_E_DEVICE; }
   ctx->timings.download_ms = ms_since(t_down);
 

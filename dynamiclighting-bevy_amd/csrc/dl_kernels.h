@@ -276,8 +276,10 @@ struct BlurWeights {
   float w[7];
   float sum;
 };
-hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,
-                                uint32_t nwin, uint64_t total, BlurWeights weights);
+// Dilate and blur of every window from map_a into map_b; tile_start (nwin + 1): the first 16 x 16
+// tile of each window (ceil(width / 16) x ceil(height / 16) tiles, row-major).
+hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* tile_start,
+                                uint32_t nwin, uint64_t ntiles, BlurWeights weights);
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
 // Packs the bounce of kept entries into `out` (bounce_pack.rs:18-86): per-word when no dither is
 // drawn, one thread per entry otherwise.

@@ -2514,73 +2514,121 @@ struct WinView {
   }
 };
 
-// BounceMap::dilate (pass 0) and blur (passes 1, 2) (bounce_map.rs:60-125)
-__global__ void k_bounce_filter(DevBounce b, const uint32_t* __restrict__ win_res, const uint64_t* __restrict__ win_start,
-                                uint32_t nwin, uint64_t total, int pass, BlurWeights bw) {
-  const float* src = pass == 1 ? b.map_b : b.map_a;
-  float* dst = pass == 1 ? b.map_a : b.map_b;
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t w = (uint32_t)owner_of_wave(win_start, nwin, c);
-    uint64_t local = c - win_start[w];
+// BounceMap::dilate, then blur along x, then along y (bounce_map.rs:60-125), fused: a block takes
+// a 16 x 16 tile of one window, loads the gather's values (map_a) with a halo of 4 into LDS,
+// dilates the tile and a halo of 3, blurs the tile's columns and 3 rows either side along x, then
+// along y into map_b.  Texels outside the window read as zero at every stage, as the three
+// separate passes read them; the sums run in the same order, so every value is the same.
+constexpr int kFilterTile = 16;
+__global__ __launch_bounds__(256) void k_bounce_filter(DevBounce b, const uint32_t* __restrict__ win_res,
+                                                       const uint64_t* __restrict__ tile_start, uint32_t nwin, uint64_t ntiles,
+                                                       BlurWeights bw) {
+  constexpr int T = kFilterTile, TI = T + 8, TD = T + 6;
+  __shared__ float s_in[TI * TI * 3];
+  __shared__ float s_d[TD * TD * 3];
+  __shared__ float s_h[T * TD * 3];
+  const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1}, oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t w = (uint32_t)owner_of(tile_start, nwin, tile);
     const uint32_t* wr = b.win_rect + 4 * (size_t)w;
-    uint32_t ww = wr[2] - wr[0] + 1;
-    int64_t x = wr[0] + (int64_t)(local % ww), y = wr[1] + (int64_t)(local / ww);
-    WinView v{src + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
-    f3 r = mk3(0.0f, 0.0f, 0.0f);
-    if (pass == 0) {
-      const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1}, oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
-      for (int k = 0; k < 9; ++k) {
-        f3 s = v.get(x + ox[k], y + oy[k]);
-        if (max_elem(s) > 0.0f) {
-          r = s;
-          break;
+    const int64_t x0 = wr[0], y0 = wr[1], x1 = wr[2], y1 = wr[3];
+    const uint64_t ww = (uint64_t)(x1 - x0 + 1), tiles_x = (ww + T - 1) / T;
+    const uint64_t local = tile - tile_start[w];
+    const int64_t X0 = x0 + (int64_t)(local % tiles_x) * T, Y0 = y0 + (int64_t)(local / tiles_x) * T;
+    const int64_t last = (int64_t)win_res[w] - 1;
+    const float* in = b.map_a + 3 * b.win_off[w];
+    auto inside = [&](int64_t x, int64_t y) { return x >= x0 && y >= y0 && x <= x1 && y <= y1; };
+    for (int q = threadIdx.x; q < TI * TI; q += blockDim.x) {
+      const int64_t x = X0 - 4 + q % TI, y = Y0 - 4 + q / TI;
+      f3 v = mk3(0.0f, 0.0f, 0.0f);
+      if (inside(x, y)) {
+        const float* p = in + 3 * ((uint64_t)(x - x0) + (uint64_t)(y - y0) * ww);
+        v = mk3(p[0], p[1], p[2]);
+      }
+      s_in[3 * q] = v.x;
+      s_in[3 * q + 1] = v.y;
+      s_in[3 * q + 2] = v.z;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < TD * TD; q += blockDim.x) {  // dilate: the first lit of 9 neighbours
+      const int qx = q % TD, qy = q / TD;
+      f3 r = mk3(0.0f, 0.0f, 0.0f);
+      if (inside(X0 - 3 + qx, Y0 - 3 + qy)) {
+        for (int k = 0; k < 9; ++k) {
+          const int c = 3 * ((qx + 1 + ox[k]) + (qy + 1 + oy[k]) * TI);
+          const f3 sv = mk3(s_in[c], s_in[c + 1], s_in[c + 2]);
+          if (max_elem(sv) > 0.0f) {
+            r = sv;
+            break;
+          }
         }
       }
-    } else {
-      int64_t last = (int64_t)win_res[w] - 1;
-      f3 sum = mk3(0.0f, 0.0f, 0.0f);
-      for (int k = -3; k <= 3; ++k) {
-        f3 s;
-        if (pass == 2) {
-          int64_t yy = y + k;
-          yy = yy < 0 ? 0 : (yy > last ? last : yy);
-          s = v.get(x, yy);
-        } else {
+      s_d[3 * q] = r.x;
+      s_d[3 * q + 1] = r.y;
+      s_d[3 * q + 2] = r.z;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < T * TD; q += blockDim.x) {  // blur along x, clamped to the map
+      const int qx = q % T, qy = q / T;
+      const int64_t x = X0 + qx, y = Y0 - 3 + qy;
+      f3 r = mk3(0.0f, 0.0f, 0.0f);
+      if (inside(x, y)) {
+        f3 sum = mk3(0.0f, 0.0f, 0.0f);
+        for (int k = -3; k <= 3; ++k) {
           int64_t xx = x + k;
           xx = xx < 0 ? 0 : (xx > last ? last : xx);
-          s = v.get(xx, y);
+          const int c = 3 * ((int)(xx - (X0 - 3)) + qy * TD);
+          sum = add(sum, mul(mk3(s_d[c], s_d[c + 1], s_d[c + 2]), bw.w[k + 3]));
         }
-        sum = add(sum, mul(s, bw.w[k + 3]));
+        r = divs(sum, bw.sum);
       }
-      r = divs(sum, bw.sum);
+      s_h[3 * q] = r.x;
+      s_h[3 * q + 1] = r.y;
+      s_h[3 * q + 2] = r.z;
     }
-    float* d = dst + 3 * (b.win_off[w] + local);
-    d[0] = r.x;
-    d[1] = r.y;
-    d[2] = r.z;
+    __syncthreads();
+    for (int q = threadIdx.x; q < T * T; q += blockDim.x) {  // blur along y into map_b
+      const int qx = q % T, qy = q / T;
+      const int64_t x = X0 + qx, y = Y0 + qy;
+      if (!inside(x, y)) continue;
+      f3 sum = mk3(0.0f, 0.0f, 0.0f);
+      for (int k = -3; k <= 3; ++k) {
+        int64_t yy = y + k;
+        yy = yy < 0 ? 0 : (yy > last ? last : yy);
+        const int c = 3 * (qx + (int)(yy - (Y0 - 3)) * T);
+        sum = add(sum, mul(mk3(s_h[c], s_h[c + 1], s_h[c + 2]), bw.w[k + 3]));
+      }
+      const f3 r = divs(sum, bw.sum);
+      float* d = b.map_b + 3 * (b.win_off[w] + (uint64_t)(x - x0) + (uint64_t)(y - y0) * ww);
+      d[0] = r.x;
+      d[1] = r.y;
+      d[2] = r.z;
+    }
+    __syncthreads();  // the next tile reuses the LDS
   }
 }
 
 // BounceMap::crop's lit test (bounce_map.rs:128-137): any positive texel inside the crop.
 __global__ void k_bounce_has(DevTris tr, DevEntries ent, DevBounce b, uint8_t* has) {
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ent.n; e += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t w = b.ent_window[e];
-    if (w == 0xffffffffu) {
-      has[e] = 0;
-      continue;
-    }
-    uint32_t t = ent.tri[e];
-    const uint32_t* c = tr.crop + 4 * (size_t)t;
-    const uint32_t* wr = b.win_rect + 4 * (size_t)w;
-    WinView v{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+  // one wave per entry: 64 consecutive texels of the crop (row-major) per step, until one is lit
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t e = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); e < ent.n; e += (uint64_t)gridDim.x * waves) {
+    const uint32_t w = b.ent_window[e];
     bool any = false;
-    for (uint32_t y = c[1]; y <= c[3] && !any; ++y)
-      for (uint32_t x = c[0]; x <= c[2]; ++x)
-        if (max_elem(v.get(x, y)) > 0.0f) {
-          any = true;
-          break;
-        }
-    has[e] = any ? 1 : 0;
+    if (w != 0xffffffffu) {
+      const uint32_t* c = tr.crop + 4 * (size_t)ent.tri[e];
+      const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+      const WinView v{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+      const uint32_t cw = c[2] >= c[0] ? c[2] - c[0] + 1u : 0u, ch = c[3] >= c[1] ? c[3] - c[1] + 1u : 0u;
+      const uint32_t count = cw * ch;  // crop sides are at most 65535
+      for (uint32_t base = 0; base < count && !any; base += 64u) {
+        const uint32_t q = base + lane;
+        bool lit = false;
+        if (q < count) lit = max_elem(v.get((int64_t)c[0] + q % cw, (int64_t)c[1] + q / cw)) > 0.0f;
+        any = __ballot(lit) != 0ull;
+      }
+    }
+    if (lane == 0) has[e] = any ? 1 : 0;
   }
 }
 
@@ -2834,20 +2882,16 @@ hipError_t launch_gather_sum(hipStream_t s, DevEntries ent, const DevLight* ligh
                      tmask, words);
   return hipGetLastError();
 }
-hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* win_start,
-                                uint32_t nwin, uint64_t total, BlurWeights weights) {
-  if (total == 0) return hipSuccess;
-  for (int pass = 0; pass < 3; ++pass) {
-    hipLaunchKernelGGL(k_bounce_filter, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, b, win_mesh_res, win_start, nwin,
-                       total, pass, weights);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* tile_start,
+                                uint32_t nwin, uint64_t ntiles, BlurWeights weights) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bounce_filter, dim3((uint32_t)std::min<uint64_t>(ntiles, 1u << 20)), dim3(256), 0, s, b, win_mesh_res,
+                     tile_start, nwin, ntiles, weights);
+  return hipGetLastError();
 }
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce) {
   if (ent.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bounce_has, dim3(grid_for(ent.n, kThreads)), dim3(kThreads), 0, s, tris, ent, b, has_bounce);
+  hipLaunchKernelGGL(k_bounce_has, dim3(grid_for(ent.n, kThreads / 64)), dim3(kThreads), 0, s, tris, ent, b, has_bounce);
   return hipGetLastError();
 }
 hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
