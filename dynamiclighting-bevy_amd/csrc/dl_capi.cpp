@@ -27,7 +27,8 @@ int upload_trig(dl_ctx* ctx) {
   const TrigHost& t = ctx->trig_host;
   // four arrays, each padded to an even word count (8-byte alignment of the run pairs)
   auto words = [](const std::vector<uint32_t>& v) { return (v.size() + 2) & ~(size_t)1; };
-  size_t total = words(t.sin_runs) + words(t.cos_runs) + words(t.sin_vals) + words(t.cos_vals);
+  size_t total = words(t.sin_runs) + words(t.cos_runs) + words(t.sin_vals) + words(t.cos_vals) + words(t.sin_any) +
+                 words(t.cos_any);
   hipError_t e = ctx->trig_buf.alloc(total);
   if (e != hipSuccess) return set_err(ctx, DL_E_OOM, "allocating the sin/cos tables failed");
   uint32_t* base = ctx->trig_buf.p;
@@ -43,6 +44,8 @@ int upload_trig(dl_ctx* ctx) {
   ctx->trig.cos.runs = reinterpret_cast<const uint2*>(put(t.cos_runs));
   ctx->trig.sin.vals = put(t.sin_vals);
   ctx->trig.cos.vals = put(t.cos_vals);
+  ctx->trig.sin.any = put(t.sin_any);
+  ctx->trig.cos.any = put(t.cos_any);
   ctx->trig.band = t.band;
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
@@ -614,8 +617,8 @@ int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches) {
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data()};
-  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data()};
+  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data(), host.sin_any.data()};
+  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data(), host.cos_any.data()};
   t.band = host.band;
   const unsigned n = std::max(1u, std::thread::hardware_concurrency());
   std::vector<uint64_t> bad(n, 0);
@@ -645,8 +648,8 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
   using namespace dl;
   const TrigHost& host = host_trig();
   TrigTables t;
-  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data()};
-  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data()};
+  t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data(), host.sin_any.data()};
+  t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data(), host.cos_any.data()};
   t.band = host.band;
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);

@@ -70,6 +70,13 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
   };
   gather(sk, sv, out.sin_keys, out.sin_vals, out.sin_runs);
   gather(ck, cv, out.cos_keys, out.cos_vals, out.cos_runs);
+  for (int w = 0; w < 2; ++w) {  // one bit per run: the run holds an exception
+    const std::vector<uint32_t>& runs = w ? out.cos_runs : out.sin_runs;
+    std::vector<uint32_t>& any = w ? out.cos_any : out.sin_any;
+    any.assign(((size_t)kTrigRuns + 31) / 32, 0u);
+    for (size_t r = 0; r < kTrigRuns; ++r)
+      if (runs[2 * r]) any[r / 32] |= 1u << (r % 32);
+  }
   // The band: every exception's double value must lie within it of a rounding midpoint, so the
   // kernels search exactly when an exception is possible.  Start from the largest relative
   // distance among the exceptions and widen until the kernels' own test accepts every one.

@@ -131,6 +131,7 @@ constexpr size_t kStagedMax = 8ull << 20;      // larger copies use the pageable
 struct TrigHost {
   std::vector<uint32_t> sin_keys, sin_vals, cos_keys, cos_vals;  // exceptions in ascending order
   std::vector<uint32_t> sin_runs, cos_runs;  // see TrigTable::runs (bitmap, rank) word pairs
+  std::vector<uint32_t> sin_any, cos_any;    // see TrigTable::any
   double band = 0.0;  // see TrigTables::band
 };
 void build_trig_tables(TrigHost& out, unsigned threads);

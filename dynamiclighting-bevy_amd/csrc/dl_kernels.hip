@@ -1908,6 +1908,13 @@ __device__ __forceinline__ void trig_finish_n(const TrigTables& t, const TrigTab
     need[i] = trig_in_domain(f2u(x[i])) && trig_near_midpoint(v[i], out[i], t.band);
   }
 #pragma unroll
+  for (int i = 0; i < N; ++i) {  // runs without an exception are cleared by the cached bitmap
+    if (need[i]) {
+      const uint32_t di = trig_domain_index(f2u(x[i]));
+      need[i] = ((tab.any[di >> 10] >> ((di >> 5) & 31u)) & 1u) != 0u;
+    }
+  }
+#pragma unroll
   for (int i = 0; i < N; ++i) run[i] = need[i] ? tab.runs[trig_domain_index(f2u(x[i])) >> 5] : make_uint2(0u, 0u);
 #pragma unroll
   for (int i = 0; i < N; ++i) {

@@ -67,6 +67,9 @@ DLI float trig_core(float xf, int which) { return (float)trig_value(xf, which); 
 struct TrigTable {
   const uint2* runs;
   const uint32_t* vals;
+  // bit r: run r of `runs` holds an exception (1.1 MB, small enough to stay in cache, so most
+  // inputs near a midpoint are cleared without reading their run)
+  const uint32_t* any;
 };
 
 struct TrigTables {
@@ -105,6 +108,7 @@ DLI float u2f(uint32_t u) {
 // whether it is (and its rank among the exceptions), a second one fetches the result.
 DLI float trig_lookup(const TrigTable& t, float x, float computed) {
   const uint32_t di = trig_domain_index(f2u(x));
+  if (!((t.any[di >> 10] >> ((di >> 5) & 31u)) & 1u)) return computed;
   const uint2 run = t.runs[di >> 5];
   const uint32_t bit = di & 31u;
   if (!((run.x >> bit) & 1u)) return computed;
