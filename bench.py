@@ -6,13 +6,16 @@ rooms with door openings, seeded boxes and pillars to 200k triangles, 256 point/
 range 4-12 m, 32 texels/m, 1-bit shadows and one colored bounce from every light.  A sample is one (texel, light) pair for which the
 reference's texel_shadow returns Some (src/bake/mesh_bake.rs:241-247).
 
-Prints one JSON line.  `roofline` prices the dominant kernel, the any-hit traversal of the bounce
-gather (k_any_stream, one launch per chunk of gather rays), at its algorithmic bytes: per
-visibility ray, 112 B per 4-wide BVH node read plus 48 B per triangle tested plus its 32-byte ray
-record, with node and triangle counts from an instrumented (untimed) bake, divided by the kernel's
-average launch time from HIP events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes per launch recorded in
-profiles/interior_bake_kernels.md.  `trace_roofline` does the same for the direct shadow-ray
-kernel.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample of the same scene (every
+Prints one JSON line.  `roofline` prices the dominant kernel, the bounce gather's sample setup
+(k_gather_rays, one launch per chunk of gather jobs), at its algorithmic bytes: 48 B per job (job
+record, texel position and coordinates, ray offset, traced count and bits), 16 B per sample (its
+photon record) and 44 B per traced sample (ray and contribution written), with job, sample and
+traced counts from an instrumented (untimed) bake, divided by the kernel's average launch time
+from HIP events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes per launch
+recorded in profiles/interior_bake_kernels.md.  `stream_roofline` does the same for the any-hit
+traversal of the gather rays (k_any_stream: per ray, 112 B per 4-wide BVH node read plus 48 B per
+triangle tested plus its 32-byte ray record and 4-byte list entry) and `trace_roofline` for the
+direct shadow-ray kernel.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample of the same scene (every
 mesh still occludes) and extrapolates to the whole bake.
 """
 import argparse
@@ -31,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 112.0  # a 4-wide node as the kernels read it: six float4 of child bounds, one int4 of child slots
 TRI_BYTES = 48.0    # v0 | occluder index, e1, e2 as three float4
 RAY_BYTES = 32.0    # origin | length, direction | flags
+JOB_BYTES = 48.0    # gather job: (texel, entry) pair, texel position and coordinates, ray offset, traced count and bits
+PHOTON_BYTES = 16.0  # one photon record per gather sample
+TRACED_BYTES = 44.0  # ray record and RGB contribution written per traced sample
 
 
 def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "interior_bake_kernels.md")):
@@ -410,15 +416,20 @@ def main():
     b_alg = NODE_BYTES * nodes + TRI_BYTES * tris + samples / 8.0
     kernel_s = trace_ms / args.steps / 1000.0
     achieved = b_alg / kernel_s / 1e9
-    # The any-hit kernel of the bounce gather dominates the bake.  Its algorithmic bytes per launch
-    # are the BVH node records and triangle records its visibility rays fetch plus the 32-byte ray
-    # record each ray reads; node and triangle counts come from the instrumented bake.
+    # The any-hit kernel of the bounce gather: per launch, the BVH node records and triangle records
+    # its visibility rays fetch plus each traced ray's 32-byte record and 4-byte list entry; counts
+    # come from the instrumented bake.
     bnodes, btris = counts["bounce_node_visits"], counts["bounce_tri_tests"]
-    bounce_rays = scene.stats["bounce_rays"]
     launches = max(1, tm["gather_trace_launches"])
-    g_alg = (NODE_BYTES * bnodes + TRI_BYTES * btris + RAY_BYTES * bounce_rays) / launches
+    g_alg = (NODE_BYTES * bnodes + TRI_BYTES * btris + (RAY_BYTES + 4.0) * counts["gather_traced"]) / launches
     g_launch_s = tm["gather_trace_ms"] / launches / 1000.0
     g_achieved = g_alg / g_launch_s / 1e9 if g_launch_s > 0 else 0.0
+    # The sample setup of the gather (the largest kernel): per job, per sample and per traced sample.
+    r_launches = max(1, tm["gather_rays_launches"])
+    r_alg = (JOB_BYTES * counts["gather_jobs"] + PHOTON_BYTES * counts["gather_samples"]
+             + TRACED_BYTES * counts["gather_traced"]) / r_launches
+    r_launch_s = tm["gather_rays_ms"] / r_launches / 1000.0
+    r_achieved = r_alg / r_launch_s / 1e9 if r_launch_s > 0 else 0.0
     out = {
         "metric": "texel-light visibility samples/s (1 GPU)",
         "value": value,
@@ -440,18 +451,26 @@ def main():
                    "rays_per_step": scene.stats["rays"], "bounce_rays_per_step": scene.stats["bounce_rays"]},
         "trace_kernel_ms": trace_ms / args.steps,
         "trace_samples_per_s": samples / kernel_s,
-        "roofline": {"kernel": "k_any_stream", "bound": "hbm", "achieved": g_achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": g_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_any_stream"),
-                     "alg_bytes_per_launch": g_alg, "launch_ms": g_launch_s * 1000.0, "launches_per_step": launches,
-                     "node_fetches_per_step": bnodes, "triangle_tests_per_step": btris,
-                     "bytes": f"{NODE_BYTES:.0f} B per node record, {TRI_BYTES:.0f} B per triangle record, "
-                              f"{RAY_BYTES:.0f} B per ray record"},
+        "roofline": {"kernel": "k_gather_rays", "bound": "hbm", "achieved": r_achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": r_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_gather_rays"),
+                     "alg_bytes_per_launch": r_alg, "launch_ms": r_launch_s * 1000.0, "launches_per_step": r_launches,
+                     "jobs_per_step": counts["gather_jobs"], "samples_per_step": counts["gather_samples"],
+                     "traced_per_step": counts["gather_traced"],
+                     "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per sample (photon record), "
+                              f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written)"},
+        "stream_roofline": {"kernel": "k_any_stream", "bound": "hbm", "achieved": g_achieved, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": g_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_any_stream"),
+                            "alg_bytes_per_launch": g_alg, "launch_ms": g_launch_s * 1000.0, "launches_per_step": launches,
+                            "node_fetches_per_step": bnodes, "triangle_tests_per_step": btris,
+                            "bytes": f"{NODE_BYTES:.0f} B per node record, {TRI_BYTES:.0f} B per triangle record, "
+                                     f"{RAY_BYTES:.0f} B per ray record, 4 B per list entry"},
         "trace_roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_step": b_alg,
                            "traffic": measured_traffic("k_trace"),
                            "node_fetches": nodes, "triangle_tests": tris},
         "stages_ms": {k: tm[k] for k in ("upload_ms", "texels_ms", "trace_ms", "bounce_ms", "post_ms", "maps_ms",
-                                         "download_ms", "total_ms", "bounce_kernel_ms", "gather_trace_ms")},
+                                         "download_ms", "total_ms", "bounce_kernel_ms", "gather_trace_ms",
+                                         "gather_rays_ms")},
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on a receiver sample)")

@@ -869,20 +869,36 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         HIP_TRY(d_list.alloc(cj * max_samples));
         HIP_TRY(d_rays.alloc(2 * cj * max_samples));
         HIP_TRY(d_contrib.alloc(3 * cj * max_samples));
-        std::vector<hipEvent_t> tev;
+        std::vector<hipEvent_t> tev, gev;  // spans of the any-hit and sample-setup launches
         struct EvList {
           std::vector<hipEvent_t>& v;
           ~EvList() { for (hipEvent_t e : v) (void)hipEventDestroy(e); }
-        } tev_guard{tev};
+        } tev_guard{tev}, gev_guard{gev};
         for (uint64_t j0 = 0; j0 < ntjobs; j0 += cj) {
           const uint64_t n = std::min(cj, ntjobs - j0);
           const uint2* jobs = d_tjobs.p + j0;
           HIP_TRY(launch_gather_nrays(s, ent, d_lights.p, jobs, n, d_nr.p));
           HIP_TRY(scan_u64(s, d_nr.p, d_rs.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+          hipEvent_t g0, g1;
+          HIP_TRY(hipEventCreate(&g0));
+          gev.push_back(g0);
+          HIP_TRY(hipEventCreate(&g1));
+          gev.push_back(g1);
+          HIP_TRY(hipEventRecord(g0, s));
           HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
                                      ctx->gather.order, d_tmask.p, mask_words, d_tcount.p));
+          HIP_TRY(hipEventRecord(g1, s));
           // the traced rays only, job by job
           HIP_TRY(scan_u64(s, d_tcount.p, d_toff.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+          if (ctx->counting) {  // the chunk's samples and traced samples, for the instrumented figures
+            uint64_t both[2] = {0, 0};
+            HIP_TRY(hipMemcpyAsync(&both[0], d_rs.p + n, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(&both[1], d_toff.p + n, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            ctx->timings.gather_jobs += n;
+            ctx->timings.gather_samples += both[0];
+            ctx->timings.gather_traced += both[1];
+          }
           HIP_TRY(launch_gather_list(s, d_tmask.p, mask_words, d_toff.p, d_rs.p, n, ctx->gather.order, d_list.p));
           HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, sizeof(unsigned long long), s));
           hipEvent_t a, b2;
@@ -938,6 +954,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           (void)hipEventElapsedTime(&ms_k, tev[k], tev[k + 1]);
           ctx->timings.gather_trace_ms += ms_k;
           ctx->timings.gather_trace_launches++;
+        }
+        for (size_t k = 0; k + 1 < gev.size(); k += 2) {
+          float ms_k = 0.0f;
+          (void)hipEventElapsedTime(&ms_k, gev[k], gev[k + 1]);
+          ctx->timings.gather_rays_ms += ms_k;
+          ctx->timings.gather_rays_launches++;
         }
       } else {
         HIP_TRY(launch_bounce_gather(s, scb, occ, tr, ms, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back(), bp, ctx->trig,

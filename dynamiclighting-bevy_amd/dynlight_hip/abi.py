@@ -156,7 +156,9 @@ class dl_timings(C.Structure):
                 ("trace_tri_tests", C.c_uint64), ("bounce_kernel_ms", C.c_double),
                 ("bounce_node_visits", C.c_uint64), ("bounce_tri_tests", C.c_uint64),
                 ("bounce_launches", C.c_uint64), ("gather_trace_ms", C.c_double),
-                ("gather_trace_launches", C.c_uint64), ("prepare_ms", C.c_double), ("bvh_ms", C.c_double)]
+                ("gather_trace_launches", C.c_uint64), ("prepare_ms", C.c_double), ("bvh_ms", C.c_double),
+                ("gather_rays_ms", C.c_double), ("gather_rays_launches", C.c_uint64), ("gather_jobs", C.c_uint64),
+                ("gather_samples", C.c_uint64), ("gather_traced", C.c_uint64)]
 
 
 def _np_dtype(struct) -> np.dtype:

@@ -326,6 +326,11 @@ typedef struct dl_timings {
     uint64_t gather_trace_launches;
     double prepare_ms;          /* host: validation, occluders, BVH build, receiver triangles */
     double bvh_ms;              /* host: the occluder BVH build alone                         */
+    double gather_rays_ms;      /* summed device time of the gather's sample-setup launches     */
+    uint64_t gather_rays_launches;
+    uint64_t gather_jobs;       /* traced (texel, light) gather jobs (counting on)            */
+    uint64_t gather_samples;    /* samples those jobs set up (counting on)                    */
+    uint64_t gather_traced;     /* samples with a ray to trace (counting on)                  */
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 
