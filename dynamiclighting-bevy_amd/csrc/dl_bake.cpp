@@ -677,8 +677,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       if (vis_js.back() > 0) tp.stream_multi = 1;
     }
     HIP_TRY(hipEventRecord(ev0, s));
+    // with streaming, the per-lane kernel only meets point lights and hard suns
     HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nbt], tp, ctx->trig,
-                         ctx->trace_block));
+                         ctx->trace_block, !ctx->trace_stream));
     if (tp.stream_multi) {
       const uint64_t njobs = vis_js.back();
       const uint32_t nme = (uint32_t)vis_me.size();

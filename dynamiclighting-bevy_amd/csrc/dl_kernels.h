@@ -144,7 +144,8 @@ hipError_t launch_vis_finish(hipStream_t s, DevTris tris, DevMeshes meshes, cons
                              const float4* rays, const float* contrib, const float* wts);
 hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights,
                         DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevRegions reg,
-                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block);
+                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block,
+                        bool multi = true);
 hipError_t launch_optimize(hipStream_t s, DevTris tris, DevMeshes meshes, DevEntries ent, DevRegions reg,
                            const uint8_t* has_bounce, uint8_t* keep);
 hipError_t launch_pad_seams(hipStream_t s, DevMeshes meshes, DevRegions reg, const uint32_t* used_regions,

@@ -647,6 +647,9 @@ __device__ __forceinline__ bool vis_multi(const DevLight& L) {
   return L.shape == 1 || L.shape == 2 || L.shape == 4 || (L.shape == 3 && L.srad > 0.0f);
 }
 
+// MULTI: the per-lane area, soft-sun and sky branches are compiled in; without them (when those
+// lights are streamed) the kernel holds far fewer registers.
+template <bool MULTI>
 __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx,
                         const uint64_t* __restrict__ tex_start, DevEntries ent, DevRegions rg,
                         const uint64_t* __restrict__ unit_start, uint64_t units, TraceParams p, TrigTables trig) {
@@ -695,6 +698,16 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
             }
           }
         }
+      } else if (!MULTI) {
+        if (L.shape == 3 && !(L.srad > 0.0f)) {  // a sun of zero angular size (sun.rs:21-30)
+          const f3 toward = ld3(L.toward);
+          if (!(dot(nrm, toward) <= kNdotlTolerance)) {
+            const float fraction = traverse<true>(sc, stack, blockDim.x, origin, toward, INFINITY, &h, tl, L.alpha) ? 0.0f : 1.0f;
+            some = true;
+            rays = 1;
+            level = level_of(fraction, L.bits);
+          }
+        }  // the other lights are traced by the k_vis_* kernels
       } else if (!(p.stream_multi && vis_multi(L))) {  // streamed lights are traced by the k_vis_* kernels
         Pcg rng = stream5(p.seed, ms.id[m], L.id, x, y);
         float fraction = 0.0f;
@@ -1374,11 +1387,11 @@ hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint
 }
 hipError_t launch_trace(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, const DevLight* lights,
                         DevTexels tx, const uint64_t* tex_start, DevEntries ent, DevRegions reg,
-                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block) {
+                        const uint64_t* unit_start, uint64_t units, TraceParams p, TrigTables trig, uint32_t block, bool multi) {
   if (units == 0) return hipSuccess;
   uint32_t waves = block / 64;
-  hipLaunchKernelGGL(k_trace, dim3(grid_for(units, waves)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
-                     lights, tx, tex_start, ent, reg, unit_start, units, p, trig);
+  hipLaunchKernelGGL(multi ? k_trace<true> : k_trace<false>, dim3(grid_for(units, waves)), dim3(block), stack_bytes(scene, block), s,
+                     scene, tris, meshes, lights, tx, tex_start, ent, reg, unit_start, units, p, trig);
   return hipGetLastError();
 }
 hipError_t launch_vis_count(hipStream_t s, DevEntries ent, const DevLight* lights, const uint64_t* js, const uint32_t* me,
