@@ -448,6 +448,28 @@ hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes)
   return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
 }
 
+// memcpy on up to `threads` threads in 4 MiB parts: a fresh destination's pages fault in parallel.
+static void parallel_copy(char* dst, const char* src, size_t bytes, unsigned threads) {
+  constexpr size_t kPart = 4u << 20;
+  const size_t parts = (bytes + kPart - 1) / kPart;
+  const unsigned th = (unsigned)std::min<size_t>(std::min(threads, 8u), parts);
+  if (th <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t k; (k = next.fetch_add(1)) < parts;) {
+      const size_t a = k * kPart;
+      std::memcpy(dst + a, src + a, std::min(kPart, bytes - a));
+    }
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < th; ++t) ts.emplace_back(work);
+  work();
+  for (auto& t : ts) t.join();
+}
+
 hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return hipSuccess;
   hipError_t e = hipSuccess;
@@ -474,13 +496,14 @@ hipError_t copy_to_host(dl_ctx* ctx, void* dst, const void* src, size_t bytes) {
       if (e == hipSuccess) e = hipEventRecord(ev[k & 1u], ctx->stream);
       if (e == hipSuccess && k > 0) {
         e = hipEventSynchronize(ev[(k - 1) & 1u]);
-        if (e == hipSuccess) std::memcpy(to + prev_off, st.host + ((k - 1) & 1u) * half, prev_len);
+        if (e == hipSuccess) parallel_copy(to + prev_off, st.host + ((k - 1) & 1u) * half, prev_len, ctx->host_threads);
       }
       prev_off = off;
       prev_len = len;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess) std::memcpy(to + prev_off, st.host + ((((bytes + half - 1) / half) - 1) & 1u) * half, prev_len);
+    if (e == hipSuccess)
+      parallel_copy(to + prev_off, st.host + ((((bytes + half - 1) / half) - 1) & 1u) * half, prev_len, ctx->host_threads);
     for (auto& v : ev)
       if (v) (void)hipEventDestroy(v);
     st.off = 0;
