@@ -139,12 +139,15 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   // alpha data is used when a light tests alpha; a resident scene keeps it for later lights
   const bool any_alpha = alpha_lights && mesh_alpha;
   const bool build_alpha = any_alpha || (fill_res && mesh_alpha);
-  for (auto* m : meshes) {
+  // alpha textures first (per mesh, in order), then the occluder records in mesh and triangle
+  // order: blocks of triangles count their valid triangles, then fill at the scanned offsets
+  std::vector<uint32_t> mesh_tex(meshes.size(), 0xffffffffu);
+  for (size_t mi = 0; mi < meshes.size(); ++mi) {
+    const dl_mesh* m = meshes[mi];
     if (!m->occludes) continue;
-    uint32_t tex = 0xffffffffu;
     if (build_alpha && m->alpha && m->uv0 && m->alpha->width > 0 && m->alpha->height > 0) {
       const dl_alpha_texture* a = m->alpha;
-      tex = (uint32_t)(tex_info.size() / 4);
+      mesh_tex[mi] = (uint32_t)(tex_info.size() / 4);
       const uint64_t texels = (uint64_t)a->width * a->height;
       tex_info.push_back(a->width);
       tex_info.push_back(a->height);
@@ -155,28 +158,75 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       for (uint64_t i = 0; i < texels; ++i)
         if ((float)a->rgba8[i * 4 + 3] / 255.0f >= a->cutoff) tex_bits[first + i / 32] |= 1u << (i % 32);
     }
-    bool has_n = m->normals && m->normal_count;
-    for (uint32_t t = 0; t < m->index_count / 3; ++t) {
-      f3 v[3];
-      for (int k = 0; k < 3; ++k) {
-        const float* p = m->positions + 3 * (size_t)m->indices[t * 3 + k];
-        v[k] = mk3(p[0], p[1], p[2]);
-      }
-      f3 n = cross(sub(v[1], v[0]), sub(v[2], v[0]));
-      if (!(len_sq(n) > 0.0f && finite_(n.x) && finite_(n.y) && finite_(n.z))) continue;
-      for (int k = 0; k < 3; ++k) {
-        occ9.push_back(v[k].x); occ9.push_back(v[k].y); occ9.push_back(v[k].z);
-        for (int a = 0; a < 3; ++a) occ_nrm.push_back(has_n ? m->normals[3 * (size_t)m->indices[t * 3 + k] + a] : 0.0f);
-      }
-      occ_flags.push_back(has_n ? 1u : 0u);
-      for (int a = 0; a < 3; ++a) occ_alb.push_back(m->albedo[a]);
-      if (build_alpha) {
-        occ_tex.push_back(tex);
-        for (int k = 0; k < 3; ++k)
-          for (int a = 0; a < 2; ++a) occ_uv0.push_back(tex != 0xffffffffu ? m->uv0[2 * (size_t)m->indices[t * 3 + k] + a] : 0.0f);
-      }
+  }
+  struct OccBlock {
+    uint32_t mesh, t0, t1;
+    uint64_t at;  // valid triangles of the block, then its first occluder
+  };
+  constexpr uint32_t kOccBlock = 1u << 14;
+  std::vector<OccBlock> oblocks;
+  for (size_t mi = 0; mi < meshes.size(); ++mi) {
+    if (!meshes[mi]->occludes) continue;
+    const uint32_t n = meshes[mi]->index_count / 3;
+    for (uint32_t t0 = 0; t0 < n; t0 += kOccBlock) oblocks.push_back({(uint32_t)mi, t0, std::min(n, t0 + kOccBlock), 0});
+  }
+  auto occ_corners = [](const dl_mesh* m, uint32_t t, f3* v) {
+    for (int k = 0; k < 3; ++k) {
+      const float* p = m->positions + 3 * (size_t)m->indices[t * 3 + k];
+      v[k] = mk3(p[0], p[1], p[2]);
+    }
+    const f3 n = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+    return len_sq(n) > 0.0f && finite_(n.x) && finite_(n.y) && finite_(n.z);
+  };
+  run_parallel(oblocks.size(), ctx->host_threads, [&](uint64_t k) {
+    OccBlock& b = oblocks[k];
+    const dl_mesh* m = meshes[b.mesh];
+    f3 v[3];
+    for (uint32_t t = b.t0; t < b.t1; ++t) b.at += occ_corners(m, t, v) ? 1u : 0u;
+  });
+  uint64_t nvalid = 0;
+  for (OccBlock& b : oblocks) {
+    const uint64_t c = b.at;
+    b.at = nvalid;
+    nvalid += c;
+  }
+  if (!use_res) {
+    occ9.resize(9 * nvalid);
+    occ_nrm.resize(9 * nvalid);
+    occ_flags.resize(nvalid);
+    occ_alb.resize(3 * nvalid);
+    if (build_alpha) {
+      occ_tex.resize(nvalid);
+      occ_uv0.resize(6 * nvalid);
     }
   }
+  run_parallel(oblocks.size(), ctx->host_threads, [&](uint64_t k) {
+    const OccBlock& b = oblocks[k];
+    const dl_mesh* m = meshes[b.mesh];
+    const uint32_t tex = mesh_tex[b.mesh];
+    const bool has_n = m->normals && m->normal_count;
+    uint64_t o = b.at;
+    f3 v[3];
+    for (uint32_t t = b.t0; t < b.t1; ++t) {
+      if (!occ_corners(m, t, v)) continue;
+      for (int k3 = 0; k3 < 3; ++k3) {
+        occ9[9 * o + 3 * k3] = v[k3].x;
+        occ9[9 * o + 3 * k3 + 1] = v[k3].y;
+        occ9[9 * o + 3 * k3 + 2] = v[k3].z;
+        for (int a = 0; a < 3; ++a)
+          occ_nrm[9 * o + 3 * k3 + a] = has_n ? m->normals[3 * (size_t)m->indices[t * 3 + k3] + a] : 0.0f;
+      }
+      occ_flags[o] = has_n ? 1u : 0u;
+      for (int a = 0; a < 3; ++a) occ_alb[3 * o + a] = m->albedo[a];
+      if (build_alpha) {
+        occ_tex[o] = tex;
+        for (int k3 = 0; k3 < 3; ++k3)
+          for (int a = 0; a < 2; ++a)
+            occ_uv0[6 * o + 2 * k3 + a] = tex != 0xffffffffu ? m->uv0[2 * (size_t)m->indices[t * 3 + k3] + a] : 0.0f;
+      }
+      o++;
+    }
+  });
   const uint32_t nocc = use_res ? resident->nocc : (uint32_t)occ_flags.size();
   Bvh4 bvh;
   const auto t_bvh = std::chrono::steady_clock::now();
@@ -219,10 +269,19 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   const size_t nth = use_res ? 0 : nt;  // the host copies of the receiver triangles
   std::vector<float> hv(nth * 9), hn(nth * 9), huv(nth * 6);
   std::vector<uint32_t> hmesh(nth), hlocal(nth);
+  struct RecvBlock {
+    uint32_t mesh, t0, t1;
+  };
+  std::vector<RecvBlock> rblocks;
   for (uint32_t i = 0; i < (use_res ? 0u : nm); ++i) {
+    const uint32_t n = meshes[i]->index_count / 3;
+    for (uint32_t t0 = 0; t0 < n; t0 += kOccBlock) rblocks.push_back({i, t0, std::min(n, t0 + kOccBlock)});
+  }
+  run_parallel(rblocks.size(), ctx->host_threads, [&](uint64_t bk) {
+    const uint32_t i = rblocks[bk].mesh;
     const dl_mesh* m = meshes[i];
     bool flat = !m->normals || m->normal_count == 0;
-    for (uint32_t t = 0; t < m->index_count / 3; ++t) {
+    for (uint32_t t = rblocks[bk].t0; t < rblocks[bk].t1; ++t) {
       uint32_t g = first_tri[i] + t;
       hmesh[g] = i;
       hlocal[g] = t;
@@ -248,7 +307,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         }
       }
     }
-  }
+  });
   if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
 
   // ---- scene-level device state --------------------------------------------------------------

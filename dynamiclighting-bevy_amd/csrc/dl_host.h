@@ -4,10 +4,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -302,6 +304,21 @@ void probe_grid(f3 mn, f3 mx, float spacing, f3* origin, float* spacing_out, uin
 void fill_inside(const uint32_t counts[3], std::vector<float>& open, std::vector<uint8_t>& valid);
 // SkyProbes::pack (format/types.rs:105-112)
 void pack_probe(const float* open6, uint32_t* words2);
+
+// fn(k) for k in [0, n), handed out one at a time to up to `threads` threads (this one included,
+// at most 16).
+template <class F>
+void run_parallel(uint64_t n, unsigned threads, const F& fn) {
+  const unsigned th = (unsigned)std::min<uint64_t>(std::min(std::max(1u, threads), 16u), n);
+  std::atomic<uint64_t> next{0};
+  auto work = [&]() {
+    for (uint64_t k; (k = next.fetch_add(1, std::memory_order_relaxed)) < n;) fn(k);
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < th; ++t) ts.emplace_back(work);
+  work();
+  for (auto& t : ts) t.join();
+}
 
 inline double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
