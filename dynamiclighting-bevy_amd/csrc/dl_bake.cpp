@@ -848,10 +848,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(upload(ctx, d_win_rect, b_win_rect));
       HIP_TRY(upload(ctx, d_win_res, b_win_res));
       HIP_TRY(upload(ctx, d_win_off, win_off));
-      std::vector<uint64_t> win_tiles(nbw + 1, 0);  // 16 x 16 tiles of the filter per window
+      std::vector<uint64_t> win_tiles(nbw + 1, 0);  // square tiles of the filter per window
       for (uint32_t w = 0; w < nbw; ++w) {
         const uint32_t* q = &b_win_rect[4 * (size_t)w];
-        win_tiles[w + 1] = win_tiles[w] + (uint64_t)((q[2] - q[0] + 16) / 16) * ((q[3] - q[1] + 16) / 16);
+        win_tiles[w + 1] = win_tiles[w] + (uint64_t)((q[2] - q[0] + kFilterTile) / kFilterTile) *
+                                              ((q[3] - q[1] + kFilterTile) / kFilterTile);
       }
       HIP_TRY(upload(ctx, d_win_start, win_tiles));
       HIP_TRY(upload(ctx, d_ent_window, b_ent_window));

@@ -277,8 +277,9 @@ struct BlurWeights {
   float w[7];
   float sum;
 };
-// Dilate and blur of every window from map_a into map_b; tile_start (nwin + 1): the first 16 x 16
-// tile of each window (ceil(width / 16) x ceil(height / 16) tiles, row-major).
+// Dilate and blur of every window from map_a into map_b; tile_start (nwin + 1): the first tile of
+// each window (ceil(width / kFilterTile) x ceil(height / kFilterTile) square tiles, row-major).
+constexpr int kFilterTile = 16;
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* tile_start,
                                 uint32_t nwin, uint64_t ntiles, BlurWeights weights);
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);

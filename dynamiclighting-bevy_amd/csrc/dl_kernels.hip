@@ -2535,11 +2535,10 @@ struct WinView {
 };
 
 // BounceMap::dilate, then blur along x, then along y (bounce_map.rs:60-125), fused: a block takes
-// a 16 x 16 tile of one window, loads the gather's values (map_a) with a halo of 4 into LDS,
+// a kFilterTile-square tile of one window, loads the gather's values (map_a) with a halo of 4 into LDS,
 // dilates the tile and a halo of 3, blurs the tile's columns and 3 rows either side along x, then
 // along y into map_b.  Texels outside the window read as zero at every stage, as the three
 // separate passes read them; the sums run in the same order, so every value is the same.
-constexpr int kFilterTile = 16;
 __global__ __launch_bounds__(256) void k_bounce_filter(DevBounce b, const uint32_t* __restrict__ win_res,
                                                        const uint64_t* __restrict__ tile_start, uint32_t nwin, uint64_t ntiles,
                                                        BlurWeights bw) {
