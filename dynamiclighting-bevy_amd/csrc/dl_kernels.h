@@ -202,10 +202,10 @@ hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTr
 // (launch_any_stream, persistent) and the per-job sums (launch_gather_sum).
 struct GatherLaunch {
   uint32_t block = 256;         // threads per block of the any-hit kernel
-  uint32_t chunk = 256;         // rays a wave takes from the queue at once
+  uint32_t chunk = 128;         // rays a wave takes from the queue at once
   uint32_t blocks_per_cu = 0;   // 0: as many as fit
   uint32_t min_waves = 0;       // 8: registers capped for 8 waves per SIMD; otherwise uncapped
-  uint32_t refill = 56;         // idle lanes before finished lanes take new rays (1-64)
+  uint32_t refill = 64;         // idle lanes before finished lanes take new rays (1-64)
   uint64_t chunk_rays = 1ull << 28;  // rays per chunk of jobs (buffer size)
   uint32_t share = 1;           // the persistent grid fills 1/share of the device (bakes side by side)
   uint32_t order = 1;           // ray slots: 0 job-major, 1 sample-major
