@@ -713,14 +713,18 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     std::vector<uint32_t> vis_me;
     std::vector<uint64_t> vis_js(1, 0);
     uint32_t vis_slots_a = 0;
+    bool per_lane = true;  // entries left to k_trace (with streaming: point lights and hard suns)
     if (ctx->trace_stream) {
       std::vector<std::pair<uint32_t, uint32_t>> multi;  // (light slot, entry) in entry order
+      per_lane = false;
       for (uint32_t t = 0; t < nbt; ++t) {
         if (h_tex_start[t + 1] == h_tex_start[t]) continue;
         for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) {
           const DevLight& L = hlights[b_ent_light[e]];
           if (L.shape == 1 || L.shape == 2 || L.shape == 4 || (L.shape == 3 && L.srad > 0.0f))
             multi.push_back({b_ent_light[e], (uint32_t)e});
+          else
+            per_lane = true;
         }
       }
       std::stable_sort(multi.begin(), multi.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
@@ -737,8 +741,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     }
     HIP_TRY(hipEventRecord(ev0, s));
     // with streaming, the per-lane kernel only meets point lights and hard suns
-    HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nbt], tp, ctx->trig,
-                         ctx->trace_block, !ctx->trace_stream));
+    if (per_lane)
+      HIP_TRY(launch_trace(s, sct, tr, ms, d_lights.p, tx, d_tex_start.p, ent, rg, d_unit_start.p, h_unit_start[nbt], tp,
+                           ctx->trig, ctx->trace_block, !ctx->trace_stream));
     if (tp.stream_multi) {
       const uint64_t njobs = vis_js.back();
       const uint32_t nme = (uint32_t)vis_me.size();
