@@ -24,6 +24,11 @@ import os
 import sys
 import time
 
+# HIP spreads a process's streams over GPU_MAX_HW_QUEUES hardware queues (4 unless set), so the
+# batch and serve workloads' worker streams (up to 16) would share 4 queues and serialize.  Set
+# before anything initializes HIP.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "dynamiclighting-bevy_amd")
 for _p in (ROOT, PKG):
