@@ -822,6 +822,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     DBuf<float> d_map_a, d_map_b;
     DBuf<uint32_t> d_win_rect, d_win_res, d_ent_window;
     DBuf<uint64_t> d_win_off, d_win_start, d_be, d_be_units;  // d_win_start: filter tiles per window
+    DBuf<uint8_t> d_tile_lit;
     DevBounce bb{};
     if (bounce_in_scene && nbw > 0) {
       std::vector<uint64_t> win_off(nbw + 1, 0);
@@ -860,6 +861,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
                                               ((q[3] - q[1] + kFilterTile) / kFilterTile);
       }
       HIP_TRY(upload(ctx, d_win_start, win_tiles));
+      HIP_TRY(zeroed(ctx, d_tile_lit, win_tiles[nbw] + 1));
       HIP_TRY(upload(ctx, d_ent_window, b_ent_window));
       HIP_TRY(upload(ctx, d_be, be));
       HIP_TRY(upload(ctx, d_be_units, be_units));
@@ -875,6 +877,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bb.map_a = d_map_a.p;
       bb.map_b = d_map_b.p;
       bb.rays = d_mbrays.p;
+      bb.tile_start = d_win_start.p;
+      bb.tile_lit = d_tile_lit.p;
       BounceParams bp;
       bp.seed = opt.seed;
       bp.colored = opt.colored_bounce ? 1u : 0u;

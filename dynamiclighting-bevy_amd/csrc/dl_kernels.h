@@ -186,6 +186,8 @@ struct DevBounce {
   float* map_a;               // 3 floats per window texel
   float* map_b;
   unsigned long long* rays;   // per mesh
+  const uint64_t* tile_start; // filter tiles (kFilterTile squares) before each window, windows + 1
+  uint8_t* tile_lit;          // per filter tile, zeroed: 1 once a gathered value in it has a positive component
 };
 hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
                           uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block);

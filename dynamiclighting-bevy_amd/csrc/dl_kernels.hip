@@ -1622,6 +1622,16 @@ __device__ __forceinline__ BounceUnit bounce_unit(const DevBounce& b, const DevE
   return u;
 }
 
+// Marks the filter tile of window w that holds window texel (lx, ly) when the value v written there
+// has a positive component: only such values survive the dilation, so the blur of a tile whose
+// 3x3 tile neighbourhood holds none is zero everywhere.
+__device__ __forceinline__ void mark_tile(const DevBounce& b, uint32_t w, uint64_t lx, uint64_t ly, f3 v) {
+  if (!(max_elem(v) > 0.0f)) return;
+  const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+  const uint64_t tiles_x = ((uint64_t)(wr[2] - wr[0]) + kFilterTile) / kFilterTile;
+  b.tile_lit[b.tile_start[w] + (ly / kFilterTile) * tiles_x + lx / kFilterTile] = 1;
+}
+
 // "Later triangles overwrite shared texels" (bounce.rs:202-209): the highest triangle that
 // gathers a texel owns it.
 __global__ void k_bounce_owner(DevTris tr, const DevLight* __restrict__ lights, DevTexels tx, const uint64_t* __restrict__ tex_start,
@@ -1744,6 +1754,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
           dst[0] = v.x;
           dst[1] = v.y;
           dst[2] = v.z;
+          mark_tile(b, w, (uint64_t)(x - wr[0]), (uint64_t)(y - wr[1]), v);
         }
       }
     }
@@ -2379,6 +2390,7 @@ __global__ void k_gather_sum(DevEntries ent, const DevLight* __restrict__ lights
     out[0] = v.x;
     out[1] = v.y;
     out[2] = v.z;
+    mark_tile(b, w, (uint64_t)((xy & 0xffffu) - wr[0]), (uint64_t)((xy >> 16) - wr[1]), v);
   }
 }
 
@@ -2513,6 +2525,7 @@ __global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict
         out[0] = v.x;
         out[1] = v.y;
         out[2] = v.z;
+        mark_tile(b, w, (uint64_t)((xy & 0xffffu) - wr[0]), (uint64_t)((xy >> 16) - wr[1]), v);
       }
     }
     const uint32_t m0 = __shfl(m, 0, 64);
@@ -2557,6 +2570,27 @@ __global__ __launch_bounds__(256) void k_bounce_filter(DevBounce b, const uint32
     const int64_t last = (int64_t)win_res[w] - 1;
     const float* in = b.map_a + 3 * b.win_off[w];
     auto inside = [&](int64_t x, int64_t y) { return x >= x0 && y >= y0 && x <= x1 && y <= y1; };
+    {
+      // no positive value within the tile's 3x3 tile neighbourhood (its halo of 4 lies inside it):
+      // every output is zero (the same for every thread of the block)
+      const int64_t tx = (int64_t)(local % tiles_x), ty = (int64_t)(local / tiles_x);
+      const int64_t tiles_y = (y1 - y0 + T) / T;
+      bool lit = false;
+      for (int64_t v = ty - 1; v <= ty + 1; ++v)
+        for (int64_t u = tx - 1; u <= tx + 1; ++u)
+          if (u >= 0 && v >= 0 && u < (int64_t)tiles_x && v < tiles_y) lit |= b.tile_lit[tile_start[w] + (uint64_t)(v * (int64_t)tiles_x + u)] != 0;
+      if (!lit) {
+        for (int q = threadIdx.x; q < T * T; q += blockDim.x) {
+          const int64_t x = X0 + q % T, y = Y0 + q / T;
+          if (!inside(x, y)) continue;
+          float* d = b.map_b + 3 * (b.win_off[w] + (uint64_t)(x - x0) + (uint64_t)(y - y0) * ww);
+          d[0] = 0.0f;
+          d[1] = 0.0f;
+          d[2] = 0.0f;
+        }
+        continue;
+      }
+    }
     for (int q = threadIdx.x; q < TI * TI; q += blockDim.x) {
       const int64_t x = X0 - 4 + q % TI, y = Y0 - 4 + q / TI;
       f3 v = mk3(0.0f, 0.0f, 0.0f);
