@@ -2673,7 +2673,27 @@ __global__ void k_bounce_has(DevTris tr, DevEntries ent, DevBounce b, uint8_t* h
       const uint32_t* wr = b.win_rect + 4 * (size_t)w;
       const WinView v{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
       const uint32_t cw = c[2] >= c[0] ? c[2] - c[0] + 1u : 0u, ch = c[3] >= c[1] ? c[3] - c[1] + 1u : 0u;
-      const uint32_t count = cw * ch;  // crop sides are at most 65535
+      uint32_t count = cw * ch;  // crop sides are at most 65535
+      // A filtered texel is positive only in a tile with a marked tile among its 3x3 neighbours:
+      // without a mark on the window tiles the crop overlaps, grown by one, nothing is lit.
+      const int64_t ix0 = c[0] > wr[0] ? c[0] : wr[0], ix1 = c[2] < wr[2] ? c[2] : wr[2];
+      const int64_t iy0 = c[1] > wr[1] ? c[1] : wr[1], iy1 = c[3] < wr[3] ? c[3] : wr[3];
+      bool marked = false;
+      if (count && ix0 <= ix1 && iy0 <= iy1) {
+        const int64_t T = kFilterTile, tiles_x = ((int64_t)wr[2] - wr[0] + T) / T, tiles_y = ((int64_t)wr[3] - wr[1] + T) / T;
+        const int64_t tx0 = (ix0 - wr[0]) / T > 0 ? (ix0 - wr[0]) / T - 1 : 0;
+        const int64_t ty0 = (iy0 - wr[1]) / T > 0 ? (iy0 - wr[1]) / T - 1 : 0;
+        const int64_t tx1 = (ix1 - wr[0]) / T + 1 < tiles_x ? (ix1 - wr[0]) / T + 1 : tiles_x - 1;
+        const int64_t ty1 = (iy1 - wr[1]) / T + 1 < tiles_y ? (iy1 - wr[1]) / T + 1 : tiles_y - 1;
+        const uint64_t nx = (uint64_t)(tx1 - tx0 + 1), nt = nx * (uint64_t)(ty1 - ty0 + 1);
+        for (uint64_t base = 0; base < nt && !marked; base += 64u) {
+          const uint64_t q = base + lane;
+          const bool f = q < nt && b.tile_lit[b.tile_start[w] + (uint64_t)(ty0 + (int64_t)(q / nx)) * (uint64_t)tiles_x +
+                                              (uint64_t)(tx0 + (int64_t)(q % nx))] != 0;
+          marked = __ballot(f) != 0ull;
+        }
+      }
+      if (!marked) count = 0;
       for (uint32_t base = 0; base < count && !any; base += 64u) {
         const uint32_t q = base + lane;
         bool lit = false;
