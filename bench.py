@@ -25,9 +25,14 @@ import sys
 import time
 
 # HIP spreads a process's streams over GPU_MAX_HW_QUEUES hardware queues (4 unless set), so the
-# batch and serve workloads' worker streams (up to 16) would share 4 queues and serialize.  Set
-# before anything initializes HIP.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# batch and serve workloads' worker streams (up to 16) would share 4 queues and serialize.  Raised
+# to 16 (an environment may already set 4) before anything initializes HIP.
+try:
+    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _queues = 0
+if _queues < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "dynamiclighting-bevy_amd")
@@ -356,7 +361,7 @@ def main():
                          "serve: light rebakes of the resident interior; outdoor: the 1M-triangle outdoor bake; "
                          "box: the CPU-runnable box room")
     ap.add_argument("--requests", type=int, default=400, help="serve workload: requests (80%% are light requests)")
-    ap.add_argument("--workers", type=int, default=4, help="serve workload: worker streams")
+    ap.add_argument("--workers", type=int, default=2, help="serve workload: worker streams")
     ap.add_argument("--rate", type=float, default=20.0, help="serve workload: request arrivals per second")
     ap.add_argument("--jobs", type=int, default=512, help="batch workload: number of jobs")
     ap.add_argument("--streams", default="2,4,8,16", help="batch workload: worker stream counts to time (the fastest is reported)")
