@@ -53,6 +53,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   ctx->cache.stream = ctx->stream;
   const CacheScope cache_scope(&ctx->cache);  // before every device buffer of the bake
   ctx->timings = dl_timings{};
+  ctx->captured.clear();
+  ctx->captured_values.clear();
   if (!input || !options || !out) { ctx->err = "null argument"; return DL_E_ARGUMENT; }
   *out = nullptr;
   const dl_bake_options& opt = *options;
@@ -1163,6 +1165,33 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(upload(ctx, d_kdout, kd_out));
       HIP_TRY(launch_bounce_pack(s, tr, ms, d_lights.p, ent, bb, d_kde.p, d_kdws.p, kd_entry.size(), kd_flat, d_kdout.p, d_comp.p,
                                  opt.colored_bounce ? 1u : 0u, opt.seed, 1, d_out.p));
+    }
+    if (ctx->capture_bounce && !(kb_entry.empty() && kd_entry.empty())) {
+      // test hook: the f32 crops the packing above read, entry by entry
+      std::vector<uint64_t> cap_entry(kb_entry), vstart(1, 0);
+      cap_entry.insert(cap_entry.end(), kd_entry.begin(), kd_entry.end());
+      for (uint64_t e : cap_entry) {
+        const uint32_t* c = &h_crop[4 * (size_t)(t0 + b_ent_tri[e])];
+        vstart.push_back(vstart.back() + 3ull * (c[2] - c[0] + 5) * (c[3] - c[1] + 5));
+      }
+      DBuf<uint64_t> d_ce, d_vs;
+      DBuf<float> d_vals;
+      HIP_TRY(upload(ctx, d_ce, cap_entry));
+      HIP_TRY(upload(ctx, d_vs, vstart));
+      HIP_TRY(d_vals.alloc(vstart.back() + 1));
+      HIP_TRY(launch_bounce_dump(s, tr, ent, bb, d_ce.p, cap_entry.size(), d_vs.p, d_vals.p));
+      std::vector<float> vals;
+      HIP_TRY(download(ctx, vals, d_vals.p, vstart.back()));
+      for (size_t k = 0; k < cap_entry.size(); ++k) {
+        const uint64_t e = cap_entry[k];
+        const uint32_t tg = t0 + b_ent_tri[e];
+        const uint32_t mi = (uint32_t)(std::upper_bound(first_tri.begin(), first_tri.begin() + nm + 1, tg) - first_tri.begin()) - 1;
+        const uint32_t* c = &h_crop[4 * (size_t)tg];
+        dl_ctx::CapturedCrop cc{meta[mi].id, lights[b_ent_light[e]]->id, tg - first_tri[mi], c[2] - c[0] + 5, c[3] - c[1] + 5,
+                                ctx->captured_values.size()};
+        ctx->captured.push_back(cc);
+        ctx->captured_values.insert(ctx->captured_values.end(), vals.begin() + (long)vstart[k], vals.begin() + (long)vstart[k + 1]);
+      }
     }
     // the batch's words in one buffer, each mesh holding its slice
     auto words = std::make_shared<std::vector<uint32_t, NoInit<uint32_t>>>(bw);

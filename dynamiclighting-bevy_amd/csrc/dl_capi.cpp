@@ -670,6 +670,37 @@ int dl_set_counting(dl_ctx* ctx, int enable) {
   return DL_OK;
 }
 
+int dl_set_bounce_capture(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->capture_bounce = enable != 0;
+  ctx->captured.clear();
+  ctx->captured_values.clear();
+  return DL_OK;
+}
+
+uint64_t dl_test_bounce_capture(dl_ctx* ctx, uint64_t index, uint64_t meta[5], const float** values) {
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto& c = ctx->captured;
+  std::sort(c.begin(), c.end(), [](const dl_ctx::CapturedCrop& a, const dl_ctx::CapturedCrop& b) {
+    if (a.mesh_id != b.mesh_id) return a.mesh_id < b.mesh_id;
+    if (a.triangle != b.triangle) return a.triangle < b.triangle;
+    return a.light_id < b.light_id;
+  });
+  if (index < c.size()) {
+    if (meta) {
+      meta[0] = c[index].mesh_id;
+      meta[1] = c[index].triangle;
+      meta[2] = c[index].light_id;
+      meta[3] = c[index].width;
+      meta[4] = c[index].height;
+    }
+    if (values) *values = ctx->captured_values.data() + c[index].offset;
+  }
+  return c.size();
+}
+
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   auto ok = [](uint32_t b) { return b == 0 || (b % 64 == 0 && b >= 64 && b <= 1024); };

@@ -165,6 +165,15 @@ struct dl_ctx {
   uint32_t trace_stream = 1;  // 1: area lights, soft suns and skies through the streamed any-hit path
   unsigned host_threads = 1;
   bool counting = false;
+  // capture mode (test hook): later bakes keep each kept bounce entry's pre-pack f32 crop
+  bool capture_bounce = false;
+  struct CapturedCrop {
+    uint64_t mesh_id, light_id;
+    uint32_t triangle, width, height;
+    uint64_t offset;  // into captured_values (floats)
+  };
+  std::vector<CapturedCrop> captured;
+  std::vector<float> captured_values;
   // batched bakes: worker contexts with their own streams (sharing this context's sin/cos
   // tables); a worker plans for 1/share of the device: of half the free memory and of the
   // resident blocks of persistent kernels

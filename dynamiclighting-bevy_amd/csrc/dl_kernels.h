@@ -287,6 +287,10 @@ hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
 // Packs the bounce of kept entries into `out` (bounce_pack.rs:18-86): per-word when no dither is
 // drawn, one thread per entry otherwise.
+// Copies the f32 RGB crop (with 2 texels of zero padding) each kept bounce entry's packing reads
+// to out + value_start[j] (3 floats per texel).
+hipError_t launch_bounce_dump(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, const uint64_t* kept_entry, uint64_t nkept,
+                              const uint64_t* value_start, float* out);
 hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
                               DevBounce b, const uint64_t* kept_entry, const uint64_t* word_start, uint64_t nkept,
                               uint64_t total_words, const uint64_t* kept_out, const int32_t* comp_of_slot,

@@ -2775,6 +2775,23 @@ __global__ void k_bounce_pack_words(DevTris tr, DevEntries ent, DevBounce b, con
   }
 }
 
+// Capture (test hook): the f32 RGB values every kept bounce entry's packing reads, its crop
+// with 2 texels of zero padding, at the entry's offset (3 floats per texel) of `out`.
+__global__ void k_bounce_dump(DevTris tr, DevEntries ent, DevBounce b, const uint64_t* __restrict__ kept_entry, uint64_t nkept,
+                              const uint64_t* __restrict__ value_start, float* out) {
+  for (uint64_t j = blockIdx.x; j < nkept; j += gridDim.x) {
+    const PackView pv = pack_view(tr, ent, b, kept_entry[j]);
+    const uint64_t n = (uint64_t)pv.W * pv.H;
+    float* o = out + value_start[j];
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const f3 v = pv.pixel(i);
+      o[3 * i] = v.x;
+      o[3 * i + 1] = v.y;
+      o[3 * i + 2] = v.z;
+    }
+  }
+}
+
 // Dithered precisions draw from one stream per (mesh, light, triangle) in pixel order.
 __global__ void k_bounce_pack_seq(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevEntries ent, DevBounce b,
                                   const uint64_t* __restrict__ kept_entry, uint64_t nkept, const uint64_t* __restrict__ kept_out,
@@ -2967,6 +2984,14 @@ hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBou
   hipLaunchKernelGGL(k_bounce_has, dim3(grid_for(ent.n, kThreads / 64)), dim3(kThreads), 0, s, tris, ent, b, has_bounce);
   return hipGetLastError();
 }
+hipError_t launch_bounce_dump(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, const uint64_t* kept_entry, uint64_t nkept,
+                              const uint64_t* value_start, float* out) {
+  if (nkept == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bounce_dump, dim3((uint32_t)std::min<uint64_t>(nkept, 1u << 20)), dim3(256), 0, s, tris, ent, b, kept_entry,
+                     nkept, value_start, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
                               DevBounce b, const uint64_t* kept_entry, const uint64_t* word_start, uint64_t nkept,
                               uint64_t total_words, const uint64_t* kept_out, const int32_t* comp_of_slot,

@@ -108,6 +108,10 @@ def load() -> C.CDLL:
     lib.dl_test_trig_table_sizes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.dl_set_counting.restype = C.c_int
     lib.dl_set_counting.argtypes = [P, C.c_int]
+    lib.dl_set_bounce_capture.restype = C.c_int
+    lib.dl_set_bounce_capture.argtypes = [P, C.c_int]
+    lib.dl_test_bounce_capture.restype = C.c_uint64
+    lib.dl_test_bounce_capture.argtypes = [P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(f32p)]
     lib.dl_set_launch.restype = C.c_int
     lib.dl_set_launch.argtypes = [P, C.c_uint32, C.c_uint32]
     lib.dl_test_batch_trace.restype = C.c_uint32
@@ -342,6 +346,25 @@ class Context:
     def set_counting(self, enable: bool) -> None:
         """Counts BVH node fetches and triangle tests of the shadow-ray kernel in later bakes."""
         self.lib.dl_set_counting(self.handle, 1 if enable else 0)
+
+    def bake_bounce_crops(self, inp: BakeInput, options: abi.dl_bake_options | None = None):
+        """bake() with the bounce capture on: (scene, {(mesh id, triangle, light id): (h, w, 3)
+        float32}), the pre-pack f32 crop of every kept bounce entry."""
+        self.lib.dl_set_bounce_capture(self.handle, 1)
+        try:
+            scene = self.bake(inp, options)
+            n = self.lib.dl_test_bounce_capture(self.handle, 0, None, None)
+            meta = (C.c_uint64 * 5)()
+            vals = C.POINTER(C.c_float)()
+            out = {}
+            for i in range(n):
+                self.lib.dl_test_bounce_capture(self.handle, i, meta, C.byref(vals))
+                w, h = int(meta[3]), int(meta[4])
+                out[(int(meta[0]), int(meta[1]), int(meta[2]))] = \
+                    np.ctypeslib.as_array(vals, shape=(h * w * 3,)).copy().reshape(h, w, 3)
+        finally:
+            self.lib.dl_set_bounce_capture(self.handle, 0)
+        return scene, out
 
     def set_launch(self, trace_block: int = 0, burial_block: int = 0) -> None:
         st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)

@@ -57,6 +57,14 @@ class TriangleView:
         v = self.shadow_value(e, x, y, 1)
         return None if v is None else v > 0.5
 
+    def bounce_texel(self, e: LightEntry, x: int, y: int, bits: int) -> float:
+        """bounce_texel (mirror.rs:96-104): the stored (square-rooted) white bounce value."""
+        i = self.grid_index(x, y)
+        per = 32 // bits
+        mask = (1 << bits) - 1
+        word = int(self.words[e.bounce_offset + i // per])
+        return ((word >> ((i % per) * bits)) & mask) / mask
+
     def bounce_texel_rgb(self, e: LightEntry, x: int, y: int, bits: int):
         i = self.grid_index(x, y)
         pixel_bits = 3 * bits

@@ -49,6 +49,16 @@ uint64_t dl_test_trig_keys(int which, uint32_t* out, uint64_t cap);
  * triangles tested by the shadow-ray kernel (dl_timings.trace_node_visits / trace_tri_tests). */
 int dl_set_counting(dl_ctx* ctx, int enable);
 
+/* Bounce capture: when enabled, each later bake on this context keeps, for every kept (triangle,
+ * light) entry with bounce, the f32 RGB values its bounce packing reads: the filtered bounce map
+ * cropped to the triangle with 2 texels of zero padding (bounce_map.rs:123-137), before the square
+ * root and quantization of bounce_pack.rs.  Enabling or disabling clears what was kept. */
+int dl_set_bounce_capture(dl_ctx* ctx, int enable);
+/* The last bake's captured entries, sorted by (mesh id, triangle, light id): returns their count;
+ * for index < count fills meta = {mesh id, triangle (mesh-local), light id, width, height} and
+ * *values = width * height * 3 floats (row-major, RGB), valid until the context's next bake. */
+uint64_t dl_test_bounce_capture(dl_ctx* ctx, uint64_t index, uint64_t meta[5], const float** values);
+
 /* Launch shape of the traversal kernels (0 keeps the current value): threads per block of the
  * shadow-ray and burial kernels. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);

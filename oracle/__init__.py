@@ -80,6 +80,23 @@ def load() -> C.CDLL:
         "dlo_rays_batch": (None, [f32p, C.c_uint32, f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, u32p]),
         "dlo_sinf": (C.c_float, [C.c_float]),
         "dlo_cosf": (C.c_float, [C.c_float]),
+        "dlo_cube_face_uv": (None, [f32p, u32p, f32p]),
+        "dlo_cube_direction": (None, [C.c_uint32, f32p, f32p]),
+        "dlo_cube_texel_direction": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, f32p]),
+        "dlo_cube_texel": (None, [f32p, C.c_uint32, u32p]),
+        "dlo_cube_texel_index": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "dlo_gaussian_weights": (None, [f32p, f32p]),
+        "dlo_bounce_map": (None, [C.c_uint32, u32p, f32p, C.c_uint32, C.c_uint32, f32p]),
+        "dlo_bounce_crop": (C.c_int, [C.c_uint32, u32p, f32p, C.c_uint32, u32p, f32p]),
+        "dlo_slerp": (None, [f32p, f32p, C.c_float, f32p]),
+        "dlo_acos_approx": (C.c_float, [C.c_float]),
+        "dlo_on_unit_sphere": (None, [u64p, C.c_uint32, C.c_uint32, f32p]),
+        "dlo_set_bounce_capture": (None, [C.c_uint32]),
+        "dlo_set_window": (None, [C.c_uint64, u32p]),
+        "dlo_bounce_capture_count": (C.c_uint64, []),
+        "dlo_bounce_capture_get": (C.c_int, [C.c_uint64, u64p, C.POINTER(C.c_double), C.c_uint64]),
+        "dlo_photon_cube": (C.c_int, [C.c_uint64, u32p, C.POINTER(f32p), C.POINTER(C.POINTER(C.c_int8)),
+                                      C.POINTER(C.POINTER(C.c_uint8))]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -125,3 +142,55 @@ def bake(inp: "abi.BakeInput", options: "abi.dl_bake_options", threads: int = 0,
         return abi.read_scene(lib, "dlo_", h)
     finally:
         lib.dlo_scene_free(h)
+
+
+def bounce_crops(inp: "abi.BakeInput", options: "abi.dl_bake_options", fp64: bool = False, threads: int = 0,
+                 photon_cubes: bool = False):
+    """Bakes with the bounce capture on and returns {(mesh id, triangle, light id): (h, w, 3) float64}
+    of every kept bounce entry: the f32 crops the bounce packing reads, or (fp64=True) the same
+    entries gathered and filtered in double.  With photon_cubes=True also returns the bake's photon
+    cubes {light id: (dist (6, r, r), normal (6, r, r, 3) int8, albedo (6, r, r, 3) uint8)}."""
+    import numpy as np
+    lib = load()
+    lib.dlo_set_bounce_capture(2 if fp64 else 1)
+    try:
+        scene = bake(inp, options, threads=threads)
+        n = lib.dlo_bounce_capture_count()
+        out = {}
+        meta = (C.c_uint64 * 5)()
+        for i in range(n):
+            lib.dlo_bounce_capture_get(i, meta, None, 0)
+            w, h = int(meta[3]), int(meta[4])
+            buf = np.zeros(w * h * 3, np.float64)
+            lib.dlo_bounce_capture_get(i, meta, buf.ctypes.data_as(C.POINTER(C.c_double)), buf.size)
+            out[(int(meta[0]), int(meta[1]), int(meta[2]))] = buf.reshape(h, w, 3)
+        cubes = {}
+        if photon_cubes:
+            for light in inp.lights:
+                res = C.c_uint32()
+                d, nr, al = C.POINTER(C.c_float)(), C.POINTER(C.c_int8)(), C.POINTER(C.c_uint8)()
+                if lib.dlo_photon_cube(light.id, C.byref(res), C.byref(d), C.byref(nr), C.byref(al)) != 0:
+                    continue
+                r = res.value
+                k = 6 * r * r
+                cubes[light.id] = (np.ctypeslib.as_array(d, (k,)).copy().reshape(6, r, r),
+                                   np.ctypeslib.as_array(nr, (k * 3,)).copy().reshape(6, r, r, 3),
+                                   np.ctypeslib.as_array(al, (k * 3,)).copy().reshape(6, r, r, 3))
+    finally:
+        lib.dlo_set_bounce_capture(0)
+    return (scene, out, cubes) if photon_cubes else (scene, out)
+
+
+def bake_window(inp: "abi.BakeInput", options: "abi.dl_bake_options", mesh_id: int, rect, threads: int = 0):
+    """Bakes only mesh `mesh_id`, and in it only the triangles whose crop meets rect = (min_x,
+    min_y, max_x, max_y); every mesh still occludes.  Triangles whose crop lies 8 texels inside
+    rect get the words of the whole bake."""
+    import numpy as np
+    lib = load()
+    r = np.array(rect, np.uint32)
+    lib.dlo_set_window(mesh_id, r.ctypes.data_as(C.POINTER(C.c_uint32)))
+    try:
+        ids = inp.mesh_array["id"]
+        return bake_receivers(inp, options, ids == mesh_id, threads=threads)
+    finally:
+        lib.dlo_set_window(0, None)

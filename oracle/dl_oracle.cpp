@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <string>
 #include <thread>
@@ -1579,45 +1580,48 @@ static V3 slerp(V3 a, V3 b, float s) {
   return lerp3(a, b, s);
 }
 
-struct BounceMap {
+// BounceMap (bounce_map.rs:22-137) over a value type T: V3 for the bake, D3 for the FP64 checker.
+template <class T>
+struct BounceMapT {
+  using S = decltype(T{}.x);
   uint32_t size;
   Rect rect;
-  std::vector<V3> values;
-  BounceMap(uint32_t size_, Rect crops) : size(size_) {
+  std::vector<T> values;
+  BounceMapT(uint32_t size_, Rect crops) : size(size_) {
     uint32_t last = size - 1, m = 4;
     rect = {crops.min_x >= m ? crops.min_x - m : 0, crops.min_y >= m ? crops.min_y - m : 0,
             std::min(crops.max_x + m, last), std::min(crops.max_y + m, last)};
-    values.assign((size_t)rect.width() * rect.height(), v3(0, 0, 0));
+    values.assign((size_t)rect.width() * rect.height(), T{});
   }
   size_t index(uint32_t x, uint32_t y) const { return (size_t)(y - rect.min_y) * rect.width() + (x - rect.min_x); }
-  V3 get(int64_t x, int64_t y) const { return rect.contains(x, y) ? values[index((uint32_t)x, (uint32_t)y)] : v3(0, 0, 0); }
-  void set(uint32_t x, uint32_t y, V3 v) { values[index(x, y)] = v; }
+  T get(int64_t x, int64_t y) const { return rect.contains(x, y) ? values[index((uint32_t)x, (uint32_t)y)] : T{}; }
+  void set(uint32_t x, uint32_t y, T v) { values[index(x, y)] = v; }
   void dilate() {
     static const int ORD[9][2] = {{0, 0}, {-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
-    std::vector<V3> out(values.size());
+    std::vector<T> out(values.size());
     size_t k = 0;
     for (uint32_t y = rect.min_y; y <= rect.max_y; ++y)
       for (uint32_t x = rect.min_x; x <= rect.max_x; ++x) {
-        V3 r = v3(0, 0, 0);
+        T r{};
         for (auto& d : ORD) {
-          V3 v = get((int64_t)x + d[0], (int64_t)y + d[1]);
-          if (max_element(v) > 0.0f) { r = v; break; }
+          T v = get((int64_t)x + d[0], (int64_t)y + d[1]);
+          if (max_element(v) > 0) { r = v; break; }
         }
         out[k++] = r;
       }
     values.swap(out);
   }
-  void blur(const float* weights, float weight_sum) {
+  void blur(const S* weights, S weight_sum) {
     int64_t last = (int64_t)size - 1;
     for (int vertical = 0; vertical < 2; ++vertical) {
-      std::vector<V3> out(values.size());
+      std::vector<T> out(values.size());
       size_t k = 0;
       for (uint32_t y = rect.min_y; y <= rect.max_y; ++y)
         for (uint32_t x = rect.min_x; x <= rect.max_x; ++x) {
-          V3 sum = v3(0, 0, 0);
+          T sum{};
           for (int kk = -3; kk <= 3; ++kk) {
-            V3 s = vertical ? get(x, std::min(std::max((int64_t)y + kk, (int64_t)0), last))
-                            : get(std::min(std::max((int64_t)x + kk, (int64_t)0), last), y);
+            T s = vertical ? get(x, std::min(std::max((int64_t)y + kk, (int64_t)0), last))
+                           : get(std::min(std::max((int64_t)x + kk, (int64_t)0), last), y);
             sum = sum + s * weights[kk + 3];
           }
           out[k++] = sum / weight_sum;
@@ -1625,19 +1629,20 @@ struct BounceMap {
       values.swap(out);
     }
   }
-  bool crop(Rect c, std::vector<V3>& grid) const {
+  bool crop(Rect c, std::vector<T>& grid) const {
     uint32_t width = c.width() + 4;
-    grid.assign((size_t)width * (c.height() + 4), v3(0, 0, 0));
+    grid.assign((size_t)width * (c.height() + 4), T{});
     bool lit = false;
     for (uint32_t y = c.min_y; y <= c.max_y; ++y)
       for (uint32_t x = c.min_x; x <= c.max_x; ++x) {
-        V3 v = get(x, y);
-        lit |= max_element(v) > 0.0f;
+        T v = get(x, y);
+        lit |= max_element(v) > 0;
         grid[(size_t)(y - c.min_y + 2) * width + (x - c.min_x + 2)] = v;
       }
     return lit;
   }
 };
+using BounceMap = BounceMapT<V3>;
 static void gaussian_weights(float* w, float* sum_out) {
   float sigma2 = 5.0f * 5.0f;
   float sum = 0.0f;
@@ -1727,6 +1732,257 @@ static Gathered gather_sun(const Scene& sc, const std::vector<SurfaceTexel>& tex
   return g;
 }
 
+// ------------------------------------------------------------------------------------------
+// FP64 bounce: the same per-texel samples as gather_triangle / gather_sun (same random streams,
+// same photon cubes, same sample counts), with every later operation in double: directions,
+// slerp by its closed form, cube lookup, visibility by a double Moller-Trumbore scan of every
+// occluder, weights, sums, and the window filter with double Gaussian weights.  It is the
+// reference the f32 bounce's rounding error is measured against; nothing else reads it.
+// ------------------------------------------------------------------------------------------
+struct D3 { double x, y, z; };
+static inline D3 d3(double x, double y, double z) { return {x, y, z}; }
+static inline D3 d3(V3 v) { return {v.x, v.y, v.z}; }
+static inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline D3 operator-(D3 a) { return {-a.x, -a.y, -a.z}; }
+static inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+static inline double length(D3 a) { return std::sqrt(dot(a, a)); }
+static inline double max_element(D3 a) { return std::fmax(a.x, std::fmax(a.y, a.z)); }
+static inline D3 normalize_or(D3 a, D3 fallback) {
+  double l = length(a);
+  return (l > 0.0 && std::isfinite(l)) ? a / l : fallback;
+}
+static const double TAU_D = 6.28318530717958647692;
+static const double PI_D = 3.14159265358979323846;
+
+// on_unit_sphere (random.rs:22-28) from the same two f32 draws
+static inline D3 on_unit_sphere_d(Rng& rng) {
+  double z = (double)rng.next_f32() * 2.0 - 1.0;
+  double angle = (double)rng.next_f32() * TAU_D;
+  double r = std::sqrt(std::fmax(1.0 - z * z, 0.0));
+  return d3(r * std::cos(angle), r * std::sin(angle), z);
+}
+// spherical interpolation of directions with lengths interpolated linearly:
+// (sin((1-s)t) a/|a| + sin(s t) b/|b|) / sin t * lerp(|a|, |b|, s)
+static D3 slerp_d(D3 a, D3 b, double s) {
+  double la = length(a), lb = length(b);
+  double c = dot(a, b) / (la * lb);
+  double rl = la + (lb - la) * s;
+  if (std::fabs(c) < 1.0 - 1e-12) {
+    double th = std::acos(c), sn = std::sin(th);
+    return (a * (std::sin((1.0 - s) * th) / la) + b * (std::sin(s * th) / lb)) * (rl / sn);
+  }
+  if (c < 0.0) {  // opposite directions: a half turn about an axis orthogonal to a
+    D3 o = std::fabs(a.x) > std::fabs(a.y) ? d3(-a.z, 0.0, a.x) : d3(0.0, a.z, -a.y);
+    D3 k = o / length(o), u = a / la;
+    double ang = PI_D * s;
+    D3 r = u * std::cos(ang) + cross(k, u) * std::sin(ang) + k * (dot(k, u) * (1.0 - std::cos(ang)));
+    return r * rl;
+  }
+  return a + (b - a) * s;
+}
+static void direction_to_texel_d(D3 d, uint32_t res, uint32_t* face, uint32_t* x, uint32_t* y) {
+  double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z), sc, tc, ma;
+  if (ax > ay && ax > az) {
+    if (d.x > 0) { *face = 0; sc = -d.z; tc = -d.y; } else { *face = 1; sc = d.z; tc = -d.y; }
+    ma = ax;
+  } else if (ay > az) {
+    if (d.y > 0) { *face = 2; sc = d.x; tc = d.z; } else { *face = 3; sc = d.x; tc = -d.z; }
+    ma = ay;
+  } else {
+    if (d.z > 0) { *face = 4; sc = d.x; tc = -d.y; } else { *face = 5; sc = -d.x; tc = -d.y; }
+    ma = az;
+  }
+  double u = 0.5, v = 0.5;
+  if (ma != 0.0) { u = (sc / ma) * 0.5 + 0.5; v = (tc / ma) * 0.5 + 0.5; }
+  double mx = (double)(res == 0 ? 0 : res - 1);
+  *x = (uint32_t)std::fmin(std::fmax(std::floor(u * res), 0.0), mx);
+  *y = (uint32_t)std::fmin(std::fmax(std::floor(v * res), 0.0), mx);
+}
+static void cube_sample_d(const PhotonCube& c, D3 light, D3 direction, D3* pos, D3* nrm, D3* alb) {
+  uint32_t face, x, y;
+  direction_to_texel_d(direction, c.res, &face, &x, &y);
+  size_t index = (size_t)face * c.res * c.res + (size_t)y * c.res + x;
+  D3 n = d3(c.nrm[index * 3] / 127.0, c.nrm[index * 3 + 1] / 127.0, c.nrm[index * 3 + 2] / 127.0);
+  *pos = light + direction * (double)c.dist[index];
+  *nrm = normalize_or(n, d3(0, 0, 0));
+  *alb = d3(c.alb[index * 3] / 255.0, c.alb[index * 3 + 1] / 255.0, c.alb[index * 3 + 2] / 255.0);
+}
+// two-sided Moller-Trumbore in double (occluder.rs:225-251)
+static bool intersect_d(const OccTri& o, D3 origin, D3 dir, double max, double* t_out, D3* bary) {
+  D3 v0 = d3(o.v[0]), e1 = d3(o.v[1]) - v0, e2 = d3(o.v[2]) - v0;
+  D3 p = cross(dir, e2);
+  double det = dot(e1, p);
+  if (det == 0.0 || !std::isfinite(det)) return false;
+  double inv = 1.0 / det;
+  D3 s = origin - v0;
+  double u = dot(s, p) * inv;
+  if (!(u >= 0.0 && u <= 1.0)) return false;
+  D3 q = cross(s, e1);
+  double v = dot(dir, q) * inv;
+  if (v < 0.0 || u + v > 1.0) return false;
+  double t = dot(e2, q) * inv;
+  if (!(t >= 0.0 && t <= max)) return false;
+  *t_out = t;
+  if (bary) *bary = d3((1.0 - u) - v, u, v);
+  return true;
+}
+static bool alpha_blocks_d(const Occluders& oc, const OccTri& o, D3 w, bool alpha) {
+  return !alpha || oc.blocks(o, v3((float)w.x, (float)w.y, (float)w.z));
+}
+static bool occluded_d(const Occluders& oc, D3 origin, D3 dir, double max, bool alpha) {
+  for (const OccTri& o : oc.tris) {
+    double t;
+    D3 w;
+    if (intersect_d(o, origin, dir, max, &t, &w) && alpha_blocks_d(oc, o, w, alpha)) return true;
+  }
+  return false;
+}
+static bool closest_hit_d(const Occluders& oc, D3 origin, D3 dir, bool alpha, double* dist, uint32_t* occ, D3* bary) {
+  bool hit = false;
+  for (uint32_t i = 0; i < oc.tris.size(); ++i) {
+    double t;
+    D3 w;
+    if (intersect_d(oc.tris[i], origin, dir, INFINITY, &t, &w) && alpha_blocks_d(oc, oc.tris[i], w, alpha) && (!hit || t < *dist)) {
+      hit = true;
+      *dist = t;
+      *occ = i;
+      *bary = w;
+    }
+  }
+  return hit;
+}
+static D3 face_normal_d(const Occluders& oc, uint32_t i) {
+  const OccTri& o = oc.tris[i];
+  return normalize_or(cross(d3(o.v[1]) - d3(o.v[0]), d3(o.v[2]) - d3(o.v[0])), d3(0, 0, 0));
+}
+static D3 surface_normal_d(const Occluders& oc, uint32_t i, D3 w) {
+  const OccTri& o = oc.tris[i];
+  D3 face = face_normal_d(oc, i);
+  const dl_mesh* m = oc.meshes[o.mesh];
+  if (!m->normals || m->normal_count == 0) return face;
+  const uint32_t* c = m->indices + o.triangle * 3;
+  D3 n = d3(v3p(m->normals + c[0] * 3)) * w.x + d3(v3p(m->normals + c[1] * 3)) * w.y + d3(v3p(m->normals + c[2] * 3)) * w.z;
+  return normalize_or(n, face);
+}
+static D3 cosine_direction_d(D3 normal, V2 square) {
+  double px = square.x * 2.0 - 1.0, py = square.y * 2.0 - 1.0, dx = 0.0, dy = 0.0;
+  if (px != 0.0 || py != 0.0) {
+    double radius, angle;
+    if (std::fabs(px) > std::fabs(py)) { radius = px; angle = (PI_D / 4) * (py / px); }
+    else { radius = py; angle = PI_D / 2 - (PI_D / 4) * (px / py); }
+    dx = std::cos(angle) * radius;
+    dy = std::sin(angle) * radius;
+  }
+  double up = std::sqrt(std::fmax(1.0 - (dx * dx + dy * dy), 0.0));
+  double sign = std::signbit(normal.z) ? -1.0 : 1.0;
+  double a = -1.0 / (sign + normal.z), b = normal.x * normal.y * a;
+  D3 side = d3(1.0 + sign * normal.x * normal.x * a, sign * b, -sign * normal.x);
+  D3 across = d3(b, sign + normal.y * normal.y * a, -normal.y);
+  D3 d = side * dx + across * dy + normal * up;
+  return d / length(d);
+}
+struct GatheredD {
+  std::vector<std::pair<std::pair<uint32_t, uint32_t>, D3>> texels;
+};
+static inline D3 reflected_d(const Scene& sc, D3 albedo) { return sc.opt->colored_bounce ? albedo : d3(1, 1, 1); }
+static GatheredD gather_triangle_d(const Scene& sc, const std::vector<SurfaceTexel>& texels, const Setup& setup,
+                                   const Prepared& pm, const Light& L, const PhotonCube& cube) {
+  GatheredD g;
+  if (setup.face == v3(0, 0, 0)) return g;
+  uint32_t samples = std::max(L.src->bounce_samples, 2u);
+  double step = 0.9 / (double)(samples - 1);
+  D3 normal = d3(setup.face), light = d3(L.position);
+  for (const auto& tx : texels) {
+    D3 pos = d3(tx.position), to_texel = pos - light;
+    if (dot(to_texel, to_texel) > (double)L.radius * (double)L.radius) continue;
+    D3 direction = normalize_or(to_texel, d3(0, 1, 0));
+    D3 origin = pos + normal * (double)RAY_OFFSET;
+    Rng rng = stream({sc.opt->seed, pm.mesh->id, L.id, (uint64_t)tx.x, (uint64_t)tx.y});
+    D3 sum = d3(0, 0, 0);
+    for (uint32_t i = 0; i < samples; ++i) {
+      double sp = 0.1 + (double)i * step;
+      D3 rnd = on_unit_sphere_d(rng);
+      D3 sample = normalize_or(slerp_d(direction, rnd, sp), direction);
+      D3 photon, pn, alb;
+      cube_sample_d(cube, light, sample, &photon, &pn, &alb);
+      D3 to_photon = photon - origin;
+      double distance = length(to_photon);
+      if ((pn.x == 0 && pn.y == 0 && pn.z == 0) || distance <= 0.0) continue;
+      D3 d = to_photon / distance;
+      if (!occluded_d(sc.occ, origin, d, distance, L.alpha_tested))
+        sum = sum + reflected_d(sc, alb) * (std::fmax(dot(normal, d), 0.0) * std::fmax(dot(pn, -d), 0.0));
+    }
+    g.texels.push_back({{tx.x, tx.y}, sum * (double)L.src->bounce_intensity / (double)samples});
+  }
+  return g;
+}
+static GatheredD gather_sun_d(const Scene& sc, const std::vector<SurfaceTexel>& texels, const Setup& setup,
+                              const Prepared& pm, const Light& L) {
+  GatheredD g;
+  if (setup.face == v3(0, 0, 0)) return g;
+  uint32_t samples = std::max(L.src->bounce_samples, 2u);
+  D3 normal = d3(setup.face), toward = d3(L.sun_toward);
+  for (const auto& tx : texels) {
+    D3 origin = d3(tx.position) + normal * (double)RAY_OFFSET;
+    Rng rng = stream({sc.opt->seed, pm.mesh->id, L.id, (uint64_t)tx.x, (uint64_t)tx.y});
+    V2 offset;
+    offset.x = rng.next_f32();
+    offset.y = rng.next_f32();
+    D3 sum = d3(0, 0, 0);
+    for (uint32_t i = 0; i < samples; ++i) {
+      D3 direction = cosine_direction_d(normal, spread(i, offset, true));
+      double dist;
+      uint32_t occ;
+      D3 w;
+      if (!closest_hit_d(sc.occ, origin, direction, L.alpha_tested, &dist, &occ, &w)) continue;
+      D3 face = face_normal_d(sc.occ, occ);
+      if (dot(face, direction) >= 0.0) continue;
+      double lit = dot(surface_normal_d(sc.occ, occ, w), toward);
+      if (lit <= 0.0 || dot(face, toward) <= 0.0) continue;
+      D3 point = origin + direction * dist + face * (double)RAY_OFFSET;
+      if (!occluded_d(sc.occ, point, toward, INFINITY, L.alpha_tested)) sum = sum + reflected_d(sc, d3(sc.occ.albedo(occ))) * lit;
+    }
+    g.texels.push_back({{tx.x, tx.y}, sum * (double)L.src->bounce_intensity / (double)samples});
+  }
+  return g;
+}
+static void gaussian_weights_d(double* w, double* sum_out) {
+  double sigma2 = 25.0, sum = 0.0;
+  for (int i = -3; i <= 3; ++i) {
+    w[i + 3] = (1.0 / std::sqrt(2.0 * PI_D * sigma2)) * std::exp(-(double)(i * i) / (2.0 * sigma2));
+    sum += w[i + 3];
+  }
+  double s2 = 0.0;
+  for (int i = 0; i < 7; ++i) { w[i] /= sum; s2 += w[i]; }
+  *sum_out = s2;
+}
+
+// Bounce capture (test hook): the crops the bounce packing reads, per (mesh, triangle, light).
+// Mode 1 keeps the f32 crops of the bake, mode 2 the FP64 crops of the same entries.
+struct CapturedCrop {
+  uint64_t mesh_id, light_id;
+  uint32_t triangle, width, height;
+  std::vector<double> rgb;
+};
+static uint32_t g_capture_mode = 0;
+static std::mutex g_capture_mu;
+static std::vector<CapturedCrop> g_captured;
+static std::vector<std::pair<uint64_t, PhotonCube>> g_cubes;  // the last bake's photon cubes, by light id
+// Texel window (test hook): in mesh `g_window_mesh` only triangles whose crop meets the rectangle
+// [x0, x1] x [y0, y1] are baked; the others keep neither texels nor lights.
+static bool g_window_on = false;
+static uint64_t g_window_mesh = 0;
+static Rect g_window{0, 0, 0, 0};
+static void capture_crop(uint64_t mesh_id, uint64_t light_id, uint32_t triangle, Rect crop, const double* rgb, size_t n) {
+  CapturedCrop c{mesh_id, light_id, triangle, crop.width() + 4, crop.height() + 4, std::vector<double>(rgb, rgb + n)};
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  g_captured.push_back(std::move(c));
+}
+
 struct Entry {
   uint32_t light;
   bool has_bounce = false;
@@ -1797,6 +2053,13 @@ static MeshOut bake_mesh(const Scene& sc, const Prepared& pm) {
   size_t nl = sc.lights.size();
   std::vector<Setup> setups(nt);
   par_for(nt, [&](size_t t) { setups[t] = associate(sc, pm, (uint32_t)t); });
+  if (g_window_on && pm.mesh->id == g_window_mesh)
+    for (auto& st : setups)
+      if (st.crop.max_x < g_window.min_x || st.crop.min_x > g_window.max_x || st.crop.max_y < g_window.min_y ||
+          st.crop.min_y > g_window.max_y) {
+        st.face = v3(0, 0, 0);
+        st.lights.clear();
+      }
   // light_regions mesh_bake.rs:175-192
   std::vector<int> have(nl, 0);
   std::vector<Rect> rects(nl);
@@ -1891,6 +2154,40 @@ static MeshOut bake_mesh(const Scene& sc, const Prepared& pm) {
         Entry& e = entries[r.first][r.second];
         e.has_bounce = map.crop(setups[r.first].crop, e.bounce);
         if (!e.has_bounce) e.bounce.clear();
+      }
+      if (g_capture_mode == 1) {
+        for (auto& r : receivers) {
+          const Entry& e = entries[r.first][r.second];
+          if (!e.has_bounce) continue;
+          std::vector<double> rgb(e.bounce.size() * 3);
+          for (size_t k = 0; k < e.bounce.size(); ++k) {
+            rgb[3 * k] = e.bounce[k].x;
+            rgb[3 * k + 1] = e.bounce[k].y;
+            rgb[3 * k + 2] = e.bounce[k].z;
+          }
+          capture_crop(pm.mesh->id, L.id, r.first, setups[r.first].crop, rgb.data(), rgb.size());
+        }
+      } else if (g_capture_mode == 2) {
+        std::vector<GatheredD> gd(receivers.size());
+        par_for(receivers.size(), [&](size_t i) {
+          uint32_t t = receivers[i].first;
+          std::vector<SurfaceTexel> tx;
+          texels_of(mt, pm, t, setups[t].face, tx);
+          gd[i] = is_cube ? gather_triangle_d(sc, tx, setups[t], pm, L, *sc.cubes[l]) : gather_sun_d(sc, tx, setups[t], pm, L);
+        });
+        BounceMapT<D3> md(size, crops);
+        for (auto& g : gd)
+          for (auto& e : g.texels) md.set(e.first.first, e.first.second, e.second);
+        md.dilate();
+        double gwd[7], gsumd;
+        gaussian_weights_d(gwd, &gsumd);
+        md.blur(gwd, gsumd);
+        for (auto& r : receivers) {
+          if (!entries[r.first][r.second].has_bounce) continue;
+          std::vector<D3> grid;
+          md.crop(setups[r.first].crop, grid);
+          capture_crop(pm.mesh->id, L.id, r.first, setups[r.first].crop, &grid[0].x, grid.size() * 3);
+        }
       }
     }
   }
@@ -2312,11 +2609,35 @@ static int bake_impl(const dl_bake_input* input, const dl_bake_options* options,
   auto t_cubes = std::chrono::steady_clock::now();
   // photon cubes (bounce.rs:126-151)
   sc->cubes.resize(sc->lights.size());
+  // With a receiver sample only the lights whose sphere meets a sampled mesh's bounds can be
+  // gathered from; the other cubes would never be read.
+  std::vector<Aabb> sampled;
+  if (receivers)
+    for (uint32_t i = 0; i < input->mesh_count; ++i) {
+      if (!receivers[i]) continue;
+      const dl_mesh& m = input->meshes[i];
+      Aabb b = Aabb::empty();
+      for (uint32_t v = 0; v < m.vertex_count; ++v) b = b.grow(v3p(m.positions + 3 * (size_t)v));
+      sampled.push_back(b);
+    }
   for (size_t l = 0; l < sc->lights.size(); ++l) {
     const Light& L = sc->lights[l];
     if (!L.bakes_bounce || L.sun) continue;
+    if (receivers) {
+      const V3 r = v3(L.radius, L.radius, L.radius);
+      const Aabb sphere = Aabb::empty().grow(L.position - r).grow(L.position + r);
+      bool near = false;
+      for (const Aabb& b : sampled) near |= b.intersects(sphere);
+      if (!near) continue;
+    }
     sc->cubes[l].reset(new PhotonCube());
     build_photon_cube(*sc, L, opt.photon_cube_resolution, *sc->cubes[l]);
+  }
+  if (g_capture_mode) {
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    g_cubes.clear();
+    for (size_t l = 0; l < sc->lights.size(); ++l)
+      if (sc->cubes[l]) g_cubes.push_back({sc->lights[l].id, *sc->cubes[l]});
   }
   auto t_meshes = std::chrono::steady_clock::now();
   std::vector<Prepared> prepared;
@@ -2691,5 +3012,112 @@ void dlo_rays_batch(const float* tris, uint32_t n, const float* rays, uint32_t c
 }
 float dlo_sinf(float x) { return sinf(x); }
 float dlo_cosf(float x) { return cosf(x); }
+
+// ---- bounce-side hooks: cube conventions, bounce map, slerp, photons, captured crops ----------
+void dlo_cube_face_uv(const float* d3, uint32_t* face, float* uv2) {
+  o::V2 uv;
+  o::direction_to_face_uv(o::v3p(d3), face, &uv);
+  uv2[0] = uv.x;
+  uv2[1] = uv.y;
+}
+void dlo_cube_direction(uint32_t face, const float* uv2, float* out3) {
+  o::V3 d = o::face_uv_to_direction(face, o::v2(uv2[0], uv2[1]));
+  out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+}
+void dlo_cube_texel_direction(uint32_t face, uint32_t x, uint32_t y, uint32_t res, float* out3) {
+  o::V3 d = o::texel_direction(face, x, y, res);
+  out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+}
+void dlo_cube_texel(const float* d3, uint32_t res, uint32_t* out3) { o::direction_to_texel(o::v3p(d3), res, &out3[0], &out3[1], &out3[2]); }
+uint32_t dlo_cube_texel_index(uint32_t face, uint32_t x, uint32_t y, uint32_t res) { return face * res * res + y * res + x; }
+void dlo_gaussian_weights(float* w7, float* sum) { o::gaussian_weights(w7, sum); }
+static o::BounceMap bounce_map_of(uint32_t size, const uint32_t* crops4, const float* texels5, uint32_t n, uint32_t ops) {
+  o::BounceMap map(size, o::Rect{crops4[0], crops4[1], crops4[2], crops4[3]});
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* t = texels5 + 5 * (size_t)i;
+    map.set((uint32_t)t[0], (uint32_t)t[1], o::v3(t[2], t[3], t[4]));
+  }
+  if (ops & 1) map.dilate();
+  if (ops & 2) {
+    float w[7], sum;
+    o::gaussian_weights(w, &sum);
+    map.blur(w, sum);
+  }
+  return map;
+}
+void dlo_bounce_map(uint32_t size, const uint32_t* crops4, const float* texels5, uint32_t n, uint32_t ops, float* out) {
+  o::BounceMap map = bounce_map_of(size, crops4, texels5, n, ops);
+  for (uint32_t y = 0; y < size; ++y)
+    for (uint32_t x = 0; x < size; ++x) {
+      o::V3 v = map.get(x, y);
+      float* q = out + 3 * ((size_t)y * size + x);
+      q[0] = v.x; q[1] = v.y; q[2] = v.z;
+    }
+}
+int dlo_bounce_crop(uint32_t size, const uint32_t* crops4, const float* texels5, uint32_t n, const uint32_t* crop4, float* out) {
+  o::BounceMap map = bounce_map_of(size, crops4, texels5, n, 0);
+  std::vector<o::V3> grid;
+  bool lit = map.crop(o::Rect{crop4[0], crop4[1], crop4[2], crop4[3]}, grid);
+  for (size_t k = 0; k < grid.size(); ++k) {
+    out[3 * k] = grid[k].x; out[3 * k + 1] = grid[k].y; out[3 * k + 2] = grid[k].z;
+  }
+  return lit ? 1 : 0;
+}
+void dlo_slerp(const float* a3, const float* b3, float s, float* out3) {
+  o::V3 r = o::slerp(o::v3p(a3), o::v3p(b3), s);
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+float dlo_acos_approx(float v) { return o::acos_approx(v); }
+void dlo_on_unit_sphere(const uint64_t* key, uint32_t nkey, uint32_t n, float* out3n) {
+  o::Rng rng = o::stream_n(key, nkey);
+  for (uint32_t i = 0; i < n; ++i) {
+    o::V3 v = o::on_unit_sphere(rng);
+    out3n[3 * i] = v.x; out3n[3 * i + 1] = v.y; out3n[3 * i + 2] = v.z;
+  }
+}
+void dlo_set_window(uint64_t mesh_id, const uint32_t* rect4) {
+  o::g_window_on = rect4 != nullptr;
+  o::g_window_mesh = mesh_id;
+  if (rect4) o::g_window = o::Rect{rect4[0], rect4[1], rect4[2], rect4[3]};
+}
+void dlo_set_bounce_capture(uint32_t mode) {
+  std::lock_guard<std::mutex> lock(o::g_capture_mu);
+  o::g_capture_mode = mode;
+  o::g_captured.clear();
+  o::g_cubes.clear();
+}
+uint64_t dlo_bounce_capture_count(void) {
+  std::lock_guard<std::mutex> lock(o::g_capture_mu);
+  std::sort(o::g_captured.begin(), o::g_captured.end(), [](const o::CapturedCrop& a, const o::CapturedCrop& b) {
+    if (a.mesh_id != b.mesh_id) return a.mesh_id < b.mesh_id;
+    if (a.triangle != b.triangle) return a.triangle < b.triangle;
+    return a.light_id < b.light_id;
+  });
+  return o::g_captured.size();
+}
+int dlo_bounce_capture_get(uint64_t i, uint64_t* meta5, double* rgb, uint64_t cap) {
+  std::lock_guard<std::mutex> lock(o::g_capture_mu);
+  if (i >= o::g_captured.size()) return DL_E_ARGUMENT;
+  const o::CapturedCrop& c = o::g_captured[i];
+  meta5[0] = c.mesh_id;
+  meta5[1] = c.triangle;
+  meta5[2] = c.light_id;
+  meta5[3] = c.width;
+  meta5[4] = c.height;
+  if (rgb) std::memcpy(rgb, c.rgb.data(), std::min<uint64_t>(cap, c.rgb.size()) * sizeof(double));
+  return DL_OK;
+}
+int dlo_photon_cube(uint64_t light_id, uint32_t* res, const float** dist, const int8_t** nrm, const uint8_t** alb) {
+  std::lock_guard<std::mutex> lock(o::g_capture_mu);
+  for (auto& c : o::g_cubes)
+    if (c.first == light_id) {
+      *res = c.second.res;
+      *dist = c.second.dist.data();
+      *nrm = c.second.nrm.data();
+      *alb = c.second.alb.data();
+      return DL_OK;
+    }
+  return DL_E_ARGUMENT;
+}
 
 }  // extern "C"

@@ -119,6 +119,39 @@ void dlo_rays_batch(const float* tris, uint32_t n, const float* rays, uint32_t c
 float dlo_sinf(float x);
 float dlo_cosf(float x);
 
+/* ---- bounce side (shared/cube.rs, bounce_map.rs, bounce.rs, random.rs) -------------------- */
+void dlo_cube_face_uv(const float* d3, uint32_t* face, float* uv2);              /* direction_to_face_uv */
+void dlo_cube_direction(uint32_t face, const float* uv2, float* out3);            /* face_uv_to_direction */
+void dlo_cube_texel_direction(uint32_t face, uint32_t x, uint32_t y, uint32_t res, float* out3);
+void dlo_cube_texel(const float* d3, uint32_t res, uint32_t* out3);               /* direction_to_texel */
+uint32_t dlo_cube_texel_index(uint32_t face, uint32_t x, uint32_t y, uint32_t res);
+void dlo_gaussian_weights(float* w7, float* sum);
+/* BounceMap of side `size` windowed on crops4 with n texels (x, y, r, g, b); ops bit 0 dilates,
+ * bit 1 blurs; out receives get(x, y) for every texel of the map (size*size*3 floats). */
+void dlo_bounce_map(uint32_t size, const uint32_t* crops4, const float* texels5, uint32_t n, uint32_t ops, float* out);
+/* BounceMap::crop of crop4 into out ((w+4)*(h+4)*3 floats); returns 1 when lit. */
+int dlo_bounce_crop(uint32_t size, const uint32_t* crops4, const float* texels5, uint32_t n, const uint32_t* crop4, float* out);
+void dlo_slerp(const float* a3, const float* b3, float s, float* out3);
+float dlo_acos_approx(float v);
+void dlo_on_unit_sphere(const uint64_t* key, uint32_t nkey, uint32_t n, float* out3n);
+/* Texel window of later bakes (rect4 NULL ends it): in mesh `mesh_id` only the triangles whose
+ * crop rectangle meets rect4 = (min_x, min_y, max_x, max_y) are baked; the others keep neither
+ * texels nor lights.  Words of triangles whose crop lies 8 or more texels inside the rectangle
+ * are then those of the whole bake (padding, ring dilation and the bounce filter reach at most
+ * 4 texels). */
+void dlo_set_window(uint64_t mesh_id, const uint32_t* rect4);
+/* Capture of later bakes: mode 1 keeps each kept bounce entry's f32 crop (the values the bounce
+ * packing reads) and the photon cubes; mode 2 the FP64 crops of the same entries; 0 stops.  The
+ * setter clears what was captured. */
+void dlo_set_bounce_capture(uint32_t mode);
+/* Sorts the captured crops by (mesh id, triangle, light id) and returns their count. */
+uint64_t dlo_bounce_capture_count(void);
+/* meta5 = mesh id, triangle (mesh-local), light id, width, height; rgb: width*height*3 doubles. */
+int dlo_bounce_capture_get(uint64_t i, uint64_t* meta5, double* rgb, uint64_t cap);
+/* The captured photon cube of light `light_id`: 6*res*res distances, 3 normal bytes and 3 albedo
+ * bytes per photon. */
+int dlo_photon_cube(uint64_t light_id, uint32_t* res, const float** dist, const int8_t** nrm, const uint8_t** alb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,3 +43,44 @@ def assert_scenes_equal(got, want, check_stats=True):
         assert d is None, f"sky probes: {d}"
     if check_stats:
         assert got.stats == want.stats
+
+
+def triangle_payloads(words: np.ndarray, bounce_layout: bool, triangles):
+    """Per triangle of `triangles`: its header bounds and, per light entry, (light index, shadow
+    words, bounce words), read through the runtime's TriangleView (mirror.rs:38-50).  Payloads
+    are contiguous in serialization order, so one ends where the next begins."""
+    import bisect
+    from dynlight_hip.mirror import TriangleView, triangle_count
+    n = triangle_count(words)
+    offs = set()
+    views = [TriangleView(words, t, bounce_layout) for t in range(n)]
+    for v in views:
+        for e in v.entries():
+            offs.update(o for o in (e.shadow_offset, e.bounce_offset) if o)
+    offs = sorted(offs) + [len(words)]
+
+    def part(o):
+        if o == 0:
+            return None
+        return tuple(int(w) for w in words[o:offs[bisect.bisect_right(offs, o)]])
+
+    return {t: (views[t].bounds, [(e.light_index, part(e.shadow_offset), part(e.bounce_offset)) for e in views[t].entries()])
+            for t in triangles}
+
+
+def window_triangles(indices, uv1, resolution: int, rect, inset: int = 8):
+    """Triangles (of a mesh's indices and UV1) whose crop rectangle (mesh_builder.rs:103-113)
+    lies `inset` texels inside rect = (min_x, min_y, max_x, max_y)."""
+    idx = np.asarray(indices).reshape(-1, 3)
+    uv = np.asarray(uv1, np.float32)[idx]  # (T, 3, 2)
+    lo = np.floor(uv.min(axis=1) * np.float32(resolution)) - 2
+    hi = np.ceil(uv.max(axis=1) * np.float32(resolution)) + 2
+    ok = (lo[:, 0] >= rect[0] + inset) & (lo[:, 1] >= rect[1] + inset) & (hi[:, 0] <= rect[2] - inset) & \
+         (hi[:, 1] <= rect[3] - inset)
+    return [int(t) for t in np.nonzero(ok)[0]]
+
+
+def terrain_arrays(inp):
+    """Indices (mesh-local) and UV1 of the first mesh of a BakeInput built from arrays."""
+    vo, io = inp.vertex_offsets, inp.index_offsets
+    return inp._indices[io[0]:io[1]], inp._uv1[vo[0]:vo[1]]

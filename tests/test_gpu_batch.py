@@ -22,10 +22,20 @@ def _cheap_jobs(count, pool=96):
     return pick, [jobs[j][0] for j in pick], [jobs[j][1] for j in pick]
 
 
+def _spanning_jobs(count=32, pool=128):
+    """`count` jobs spread over the cost order of the first `pool` batch jobs (every pool/count-th
+    by triangles x lights x density^2): small and large, with and without bounce."""
+    jobs = [scenes.batch_job(j) for j in range(pool)]
+    cost = [i.triangle_count * len(i.lights) * o.pixel_density ** 2 for i, o in jobs]
+    pick = sorted(int(j) for j in np.argsort(cost)[::pool // count])
+    return pick, [jobs[j][0] for j in pick], [jobs[j][1] for j in pick]
+
+
 def test_batched_jobs_equal_single_bakes(ctx):
-    pick, ins, ops = _cheap_jobs(12)
+    pick, ins, ops = _spanning_jobs(32)
     print(f"batch jobs {pick}", flush=True)
-    got = ctx.bake_batch(ins, ops, streams=3)
+    assert sum(j % 2 for j in pick) >= 8 and sum(1 - j % 2 for j in pick) >= 8, "with and without bounce"
+    got = ctx.bake_batch(ins, ops, streams=4)
     assert len(got) == len(ins)
     for j, inp, o, g in zip(pick, ins, ops, got):
         alone = ctx.bake(inp, o)
@@ -38,11 +48,28 @@ def test_batched_jobs_equal_single_bakes(ctx):
 
 
 def test_cheapest_batched_jobs_equal_the_oracle(ctx):
-    pick, ins, ops = _cheap_jobs(4)
+    pick, ins, ops = _cheap_jobs(6)
     got = ctx.bake_batch(ins, ops, streams=2)
     for j, inp, o, g in zip(pick, ins, ops, got):
         print(f"oracle job {j}", flush=True)
         assert_scenes_equal(g, oracle.bake(inp, o, threads=8))
+
+
+def test_spanning_batched_jobs_equal_the_oracle_on_a_mesh_sample(ctx):
+    """Jobs across the size distribution against the oracle on an eighth of their meshes."""
+    pick, ins, ops = _spanning_jobs(32)
+    pick, ins, ops = pick[::4], ins[::4], ops[::4]
+    got = ctx.bake_batch(ins, ops, streams=4)
+    for j, inp, o, g in zip(pick, ins, ops, got):
+        mask = scenes.mesh_sample_mask(inp.mesh_array["id"], 8)
+        mask[0] = True  # the first room shell
+        print(f"oracle job {j}: {int(mask.sum())} of {len(mask)} meshes", flush=True)
+        want = oracle.bake_receivers(inp, o, mask, threads=8)
+        by_id = {m.id: m for m in g.meshes}
+        assert g.lights == want.lights and np.array_equal(g.light_bvh, want.light_bvh)
+        for w in want.meshes:
+            assert np.array_equal(by_id[w.id].triangles, w.triangles), f"job {j} mesh {w.id:#x}"
+        assert np.array_equal(g.distance_cubes, want.distance_cubes)
 
 
 def test_batch_errors(ctx):

@@ -1,0 +1,72 @@
+"""Bounce RGB before packing: the f32 crops the device's bounce packing reads (its filtered bounce
+maps cropped to each triangle) against the oracle's, entry by entry.  The path's tolerance is 1e-4
+relative to the reference's f32 result; the device follows the reference's operation order, so the
+crops are also required to be bit-identical.  Photon cubes are compared at 48^2/64^2 and at the
+default 512^2."""
+import numpy as np
+import pytest
+
+import oracle
+import scene_cases
+from dynlight_hip import abi, scenes
+from helpers import assert_scenes_equal
+from test_oracle_bounce import colored_options, photon_color_input, sun_input, RED
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4  # bounce RGB within 1e-4 relative of the reference's f32 result
+
+
+def _bounce_512():
+    inp, o = scene_cases.bounce_point_case()
+    o.photon_cube_resolution = 512
+    return inp, o
+
+
+CASES = {
+    "point": scene_cases.bounce_point_case,
+    "sun": scene_cases.bounce_sun_case,
+    "dither": scene_cases.bounce_dither_case,
+    "white_dither": scene_cases.bounce_white_dither_case,
+    "colored_sun_floor": lambda: (sun_input(RED), colored_options(True)),
+    "photon_color": lambda: (photon_color_input(), colored_options(True)),
+    "point_512": _bounce_512,
+}
+
+
+def compare_crops(got, want):
+    assert got.keys() == want.keys(), (sorted(set(got) ^ set(want)))[:8]
+    worst = 0.0
+    for k in want:
+        g = got[k].astype(np.float64)
+        w = want[k]
+        assert g.shape == w.shape, k
+        scale = np.maximum(np.abs(w), np.float64(np.finfo(np.float32).tiny))
+        worst = max(worst, float(np.max(np.abs(g - w) / scale)))
+    assert worst <= REL_TOL, worst
+    for k in want:
+        assert np.array_equal(got[k].astype(np.float64), want[k]), f"entry {k} is within tolerance but not bit-identical"
+    return worst
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pre_pack_bounce_equals_the_oracle(ctx, name):
+    inp, o = CASES[name]()
+    print(f"bounce case {name}: cubes {o.photon_cube_resolution}", flush=True)
+    scene, got = ctx.bake_bounce_crops(inp, o)
+    want_scene, want = oracle.bounce_crops(inp, o)
+    assert len(want) > 0
+    compare_crops(got, want)
+    assert_scenes_equal(scene, want_scene)
+    # the error of the f32 bounce against the same samples in double (stated, not a parity gate)
+    _, d = oracle.bounce_crops(inp, o, fp64=True)
+    rel = np.concatenate([np.abs(got[k].ravel() - d[k].ravel()) / np.maximum(np.abs(d[k].ravel()), 1e-30)
+                          for k in d if np.any(d[k] != 0)])
+    print(f"  f32 vs fp64: median {np.median(rel):.2e}, p99 {np.percentile(rel, 99):.2e}, "
+          f"share above 1e-4 {np.mean(rel > 1e-4):.4f}", flush=True)
+
+
+def test_capture_is_off_unless_asked(ctx):
+    inp, o = scene_cases.bounce_point_case()
+    ctx.bake(inp, o)
+    assert ctx.lib.dl_test_bounce_capture(ctx.handle, 0, None, None) == 0
