@@ -414,6 +414,31 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
   }
 }
 
+int dl_service_ray_queries_noalloc(dl_service* svc, const float* rays8, uint64_t ray_count, int closest, float* t_out,
+                                   uint32_t* hit_or_index, uint64_t* hits) {
+  using namespace dl;
+  if (!svc) return set_err(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (ray_count > DL_NOALLOC_MAX_RAYS)
+    return set_err(ctx, DL_E_ARGUMENT, "at most " + std::to_string((unsigned long long)DL_NOALLOC_MAX_RAYS) + " rays per call");
+  if (ray_count && (!rays8 || !hit_or_index || (closest && !t_out))) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+  const Resident& r = svc->res;
+  DevScene sc{};
+  sc.nodes = r.d_nodes.p;
+  sc.tris = reinterpret_cast<const float4*>(r.d_occ.p);
+  sc.ntris = r.nocc;
+  sc.stack = 3 * std::max(1u, r.depth) + 1;
+  sc.stack16 = r.nodes < 65536 ? 1u : 0u;
+  uint32_t overflow = 0;
+  const hipError_t e = launch_ray_queries_noalloc(ctx->stream, ctx->device, sc, rays8, ray_count, closest ? 1 : 0, t_out,
+                                                  hit_or_index, hits, &overflow);
+  if (e != hipSuccess) return set_err(ctx, DL_E_DEVICE, hipGetErrorString(e));
+  if (overflow) return set_err(ctx, DL_E_DEVICE, "BVH traversal stack overflow");
+  return DL_OK;
+}
+
 void dl_service_destroy(dl_service* svc) {
   if (!svc) return;
   dl_ctx* ctx = svc->ctx;

@@ -285,12 +285,17 @@ constexpr int kFilterTile = 16;
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* tile_start,
                                 uint32_t nwin, uint64_t ntiles, BlurWeights weights);
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
-// Packs the bounce of kept entries into `out` (bounce_pack.rs:18-86): per-word when no dither is
-// drawn, one thread per entry otherwise.
+// Ray queries through a fixed per-GPU device scratch (queue head, hit count, overflow flag): no
+// device memory is allocated.  Rays, t_out and idx_out are device memory; *hits and *overflow are
+// read back when the queries are done.
+hipError_t launch_ray_queries_noalloc(hipStream_t s, int device, DevScene scene, const float* rays, uint64_t n, int closest,
+                                      float* t_out, uint32_t* idx_out, uint64_t* hits, uint32_t* overflow);
 // Copies the f32 RGB crop (with 2 texels of zero padding) each kept bounce entry's packing reads
 // to out + value_start[j] (3 floats per texel).
 hipError_t launch_bounce_dump(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, const uint64_t* kept_entry, uint64_t nkept,
                               const uint64_t* value_start, float* out);
+// Packs the bounce of kept entries into `out` (bounce_pack.rs:18-86): per-word when no dither is
+// drawn, one thread per entry otherwise.
 hipError_t launch_bounce_pack(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevEntries ent,
                               DevBounce b, const uint64_t* kept_entry, const uint64_t* word_start, uint64_t nkept,
                               uint64_t total_words, const uint64_t* kept_out, const int32_t* comp_of_slot,

@@ -7,6 +7,8 @@
 // trace neighbouring texels toward the same light.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "dl_kernels.h"
 
 namespace dl {
@@ -1317,6 +1319,49 @@ __global__ void k_ray_queries(DevScene sc, const float* __restrict__ rays, uint6
   }
 }
 
+// ---- ray queries without device allocation ------------------------------------------------
+// One fixed scratch per GPU for the process lifetime: the persistent query kernel's queue head
+// (u64, words 0-1), its hit count (u64, words 2-3) and the traversal's stack-overflow flag (word
+// 4).  Every call on a GPU shares it, so calls that overlap on one GPU race.
+constexpr uint32_t kNoAllocScratchWords = 64;  // 256 bytes
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+__global__ void k_scratch_clear(uint32_t* s, uint32_t words) {
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) s[i] = 0u;
+}
+
+// Persistent any-hit / closest-hit queries: each wave takes the next 64 rays from the queue head
+// until the head passes n (every wave reaches that exit, the head only grows).
+__global__ void k_ray_queries_queue(DevScene sc, const float* __restrict__ rays, uint64_t n, int closest, float* t_out,
+                                    uint32_t* idx_out, unsigned long long* head, unsigned long long* hits) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(head, 64ull);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint64_t i = base + lane;
+    bool hit = false;
+    if (i < n) {
+      const float* r = rays + 8 * i;
+      Hit h;
+      if (closest) {
+        hit = traverse<false>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h, tl);
+        t_out[i] = hit ? h.t : -1.0f;
+        idx_out[i] = hit ? h.occ : 0xffffffffu;
+      } else {
+        hit = traverse<true>(sc, stack, blockDim.x, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], &h, tl);
+        idx_out[i] = hit ? 1u : 0u;
+      }
+    }
+    const unsigned long long mask = __ballot(hit);
+    if (lane == 0 && mask) atomicAdd(hits, (unsigned long long)__popcll(mask));
+  }
+}
+
 __global__ void k_trig_eval(TrigTables trig, const float* __restrict__ x, uint64_t n, float* s, float* c) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     s[i] = lsinf(trig, x[i]);
@@ -1482,6 +1527,49 @@ hipError_t launch_ray_queries(hipStream_t s, DevScene scene, const float* rays, 
   hipLaunchKernelGGL(k_ray_queries, dim3(grid_for(n, 128)), dim3(128), stack_bytes(scene, 128), s, scene, rays, n, closest,
                      t_out, idx_out);
   return hipGetLastError();
+}
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[64] = {};  // the scratch's device address, per device
+int g_scratch_cus[64] = {};
+}  // namespace
+
+hipError_t launch_ray_queries_noalloc(hipStream_t s, int device, DevScene scene, const float* rays, uint64_t n, int closest,
+                                      float* t_out, uint32_t* idx_out, uint64_t* hits, uint32_t* overflow) {
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  uint32_t* ptr = nullptr;
+  int cus = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {  // once per device, with the device current
+      void* p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return e;
+      e = hipDeviceGetAttribute(&g_scratch_cus[device], hipDeviceAttributeMultiprocessorCount, device);
+      if (e != hipSuccess) return e;
+      g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    ptr = g_scratch_ptr[device];
+    cus = g_scratch_cus[device];
+  }
+  scene.overflow = ptr + 4;
+  scene.counters = nullptr;
+  hipLaunchKernelGGL(k_scratch_clear, dim3(1), dim3(64), 0, s, ptr, kNoAllocScratchWords);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (n) {
+    const uint32_t block = 128;
+    const uint32_t grid = grid_for(n, block, (uint32_t)std::max(1, cus) * 8u);
+    hipLaunchKernelGGL(k_ray_queries_queue, dim3(grid), dim3(block), stack_bytes(scene, block), s, scene, rays, n, closest, t_out,
+                       idx_out, reinterpret_cast<unsigned long long*>(ptr), reinterpret_cast<unsigned long long*>(ptr + 2));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  uint32_t back[5] = {0, 0, 0, 0, 0};
+  if ((e = hipMemcpyAsync(back, ptr, sizeof back, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (hits) *hits = (uint64_t)back[2] | ((uint64_t)back[3] << 32);
+  if (overflow) *overflow = back[4];
+  return hipSuccess;
 }
 hipError_t launch_trig_eval(hipStream_t s, TrigTables trig, const float* x, uint64_t n, float* sin_out, float* cos_out) {
   if (n == 0) return hipSuccess;

@@ -63,6 +63,9 @@ def load() -> C.CDLL:
     lib.dl_service_replay.restype = C.c_int
     lib.dl_service_replay.argtypes = [P, C.POINTER(abi.dl_service_request), C.c_uint32, C.c_uint32,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(P)]
+    lib.dl_service_ray_queries_noalloc.restype = C.c_int
+    lib.dl_service_ray_queries_noalloc.argtypes = [P, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
+                                                   C.POINTER(C.c_uint64)]
     lib.dl_service_destroy.restype = None
     lib.dl_service_destroy.argtypes = [P]
     lib.dl_encode.restype = C.c_int

@@ -23,6 +23,7 @@ DL_E_DEVICE = -16
 DL_E_OOM = -17
 DL_E_ARGUMENT = -18
 DL_E_FORMAT = -19
+DL_NOALLOC_MAX_RAYS = 1 << 30  # include/dynlight.h
 
 ILLUMINATION_DIRECT, ILLUMINATION_SINGLE_BOUNCE = 0, 1
 TRANSPARENCY_DISABLED, TRANSPARENCY_ENABLED, TRANSPARENCY_ENABLED_MAX = 0, 1, 2

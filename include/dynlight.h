@@ -280,6 +280,21 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
                       double* latency_ms, double* service_ms, int* status, dl_baked_scene** out);
 void dl_service_destroy(dl_service* svc);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Answers `ray_count`
+ * rays against the occluder BVH resident in `svc`, like dl_test_ray_queries against the same
+ * triangles: rays8 holds 8 floats per ray (origin, direction, max distance, unused); closest = 0
+ * sets hit_or_index[i] to 1 for a hit, closest = 1 sets t_out[i] to the nearest distance (-1 on a
+ * miss) and hit_or_index[i] to its occluder index (lowest on ties).  rays8, t_out and
+ * hit_or_index are device memory of the service's GPU; *hits (may be NULL) receives the number
+ * of rays that hit.  Allocates nothing: the kernels' queue head, hit count and overflow flag live
+ * in a fixed 256-byte device scratch per GPU, kept until the process ends and shared by every
+ * call on that GPU, so calls on one GPU from different contexts must not overlap (they race).
+ * At most DL_NOALLOC_MAX_RAYS rays per call. */
+#define DL_NOALLOC_MAX_RAYS (1ull << 30)
+int dl_service_ray_queries_noalloc(dl_service* svc, const float* rays8, uint64_t ray_count, int closest, float* t_out,
+                                   uint32_t* hit_or_index, uint64_t* hits);
+
 /* Many independent bakes (no counterpart in the reference, which bakes one scene per call):
  * job i bakes inputs[i] with options[i] into out[i] (status[i] is its dl_bake result).  Jobs are
  * handed out in order to `streams` worker contexts (1-16), each baking on its own HIP stream, so
