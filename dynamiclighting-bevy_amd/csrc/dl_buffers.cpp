@@ -43,16 +43,23 @@ uint32_t local_lights(const dl_baked_scene& s) {
 }
 }  // namespace
 
-extern "C" {
-
-int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out) {
+namespace {
+// scene_buffers / take_scene_buffers (apply.rs:27-61): with `take`, each mesh's triangle words
+// are released as soon as they are copied, so the scene and the buffers are never both whole.
+int build_buffers(dl_baked_scene* scene, bool take, dl_baked_buffers** out) {
   if (!scene || !out) {
     dl_set_thread_error("null argument");
     return DL_E_ARGUMENT;
   }
   *out = nullptr;
+  if (scene->triangles_taken) {
+    dl_set_thread_error(take ? "the scene's triangle words were already taken"
+                             : "scene_buffers needs a full scene, but this one's triangle words were taken; use the "
+                               "buffers taken from it");
+    return DL_E_TRIANGLES_TAKEN;
+  }
   try {
-    const dl_baked_scene& s = *scene;
+    dl_baked_scene& s = *scene;
     std::unique_ptr<dl_baked_buffers> b(new dl_baked_buffers());
     // BufferBuilder::push (layout.rs:148-169)
     size_t words = 0;
@@ -60,7 +67,7 @@ int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out) {
     b->triangles.reserve(words ? words : 1);
     std::vector<uint32_t> records;
     records.reserve(s.meshes.size() * 4);
-    for (const auto& m : s.meshes) {
+    for (auto& m : s.meshes) {
       const uint32_t triangle_count = m.words.empty() ? 0u : m.words[0] / 4;
       dl_receiver r{};
       r.id = m.id;
@@ -71,7 +78,9 @@ int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out) {
       b->receivers.push_back(r);
       records.insert(records.end(), {(uint32_t)b->triangles.size(), m.resolution, triangle_count, 0u});
       b->triangles.insert(b->triangles.end(), m.words.begin(), m.words.end());
+      if (take) m.words = dl_word_slice{};
     }
+    if (take) s.triangles_taken = true;
     // BufferBuilder::finish (layout.rs:171-214) with scene_input (apply.rs:73-97)
     if (b->triangles.empty()) b->triangles.push_back(0);
     std::vector<uint32_t>& aux = b->aux;
@@ -115,6 +124,17 @@ int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out) {
     return DL_E_OOM;
   }
 }
+}  // namespace
+
+extern "C" {
+
+int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out) {
+  return build_buffers(const_cast<dl_baked_scene*>(scene), false, out);
+}
+
+int dl_take_scene_buffers(dl_baked_scene* scene, dl_baked_buffers** out) { return build_buffers(scene, true, out); }
+
+int dl_scene_triangles_taken(const dl_baked_scene* scene) { return scene && scene->triangles_taken ? 1 : 0; }
 
 uint64_t dl_buffers_triangles(const dl_baked_buffers* b, const uint32_t** words) {
   if (!b) return 0;

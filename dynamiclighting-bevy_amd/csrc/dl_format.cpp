@@ -444,6 +444,10 @@ int dl_encode(const dl_baked_scene* scene, uint8_t** bytes, uint64_t* len) {
   }
   *bytes = nullptr;
   *len = 0;
+  if (scene->triangles_taken) {  // TrianglesTaken (format/mod.rs:84-104)
+    dl_set_thread_error("the scene's triangle words were moved into its GPU buffers, so it can't be written");
+    return DL_E_TRIANGLES_TAKEN;
+  }
   try {
     // encode (format/mod.rs:101-119): header, then the payload as one gzip member
     const std::vector<uint8_t> payload = encode_payload(*scene);

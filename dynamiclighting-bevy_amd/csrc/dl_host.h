@@ -235,6 +235,7 @@ struct dl_baked_scene {
   float probe_spacing = 0.0f;
   uint32_t probe_counts[3] = {0, 0, 0};
   std::vector<uint32_t> probe_words;
+  bool triangles_taken = false;  // the meshes' words were moved into buffers (take_scene_buffers)
 };
 
 namespace dl {

@@ -80,6 +80,20 @@ def load() -> C.CDLL:
     lib.dl_format_error_kind.argtypes = []
     lib.dl_scene_buffers.restype = C.c_int
     lib.dl_scene_buffers.argtypes = [P, C.POINTER(P)]
+    f32p_, u32p_ = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+    lib.dl_mesh_content_hash.restype = C.c_uint64
+    lib.dl_mesh_content_hash.argtypes = [f32p_, C.c_uint32, u32p_, C.c_uint64, f32p_, C.c_uint32]
+    lib.dl_bake_id_from_path.restype = C.c_uint64
+    lib.dl_bake_id_from_path.argtypes = [C.POINTER(C.c_char_p), C.POINTER(C.c_uint64), C.c_uint32]
+    lib.dl_assign_ids.restype = C.c_int
+    lib.dl_assign_ids.argtypes = [C.POINTER(abi.dl_id_candidate), C.c_uint32, C.POINTER(C.c_uint64)]
+    lib.dl_mesh_to_world.restype = C.c_int
+    lib.dl_mesh_to_world.argtypes = [f32p_, f32p_, C.c_uint32, f32p_, u32p_, C.c_uint64, f32p_, C.c_uint32, f32p_, f32p_,
+                                     u32p_, C.POINTER(C.c_uint64)]
+    lib.dl_take_scene_buffers.restype = C.c_int
+    lib.dl_take_scene_buffers.argtypes = [P, C.POINTER(P)]
+    lib.dl_scene_triangles_taken.restype = C.c_int
+    lib.dl_scene_triangles_taken.argtypes = [P]
     for name in ("dl_buffers_triangles", "dl_buffers_aux"):
         getattr(lib, name).restype = C.c_uint64
         getattr(lib, name).argtypes = [P, C.POINTER(C.POINTER(C.c_uint32))]
@@ -169,10 +183,7 @@ def decode(data: bytes) -> abi.BakedScene:
         lib.dl_baked_scene_free(h)
 
 
-def decode_stream(source) -> abi.BakedScene:
-    """decode_stream (src/bake/format/stream.rs:34-71): a .dynlight file read from `source`
-    (anything with read(n)) as it is parsed."""
-    lib = load()
+def _decode_stream_handle(lib, source):
     failed = []
 
     def read(_user, buf, cap):
@@ -191,17 +202,43 @@ def decode_stream(source) -> abi.BakedScene:
         raise _format_error(lib)
     if st != 0:
         raise BakeError(st, lib.dl_last_error(None).decode())
+    return h
+
+
+def decode_stream(source) -> abi.BakedScene:
+    """decode_stream (src/bake/format/stream.rs:34-71): a .dynlight file read from `source`
+    (anything with read(n)) as it is parsed."""
+    lib = load()
+    h = _decode_stream_handle(lib, source)
     try:
         return abi.read_scene(lib, "dl_", h)
     finally:
         lib.dl_baked_scene_free(h)
 
 
-def _buffers_of_handle(lib, h) -> dict:
-    """scene_buffers (src/baked_lighting/apply.rs:27-38) of a library scene: the triangles and aux
-    word arrays and the receivers in mesh order (tag = index + 1)."""
+def load_bake(source, copy: bool = True):
+    """The loader path (baked_lighting/loader.rs:47-68): decode_stream of `source`, then
+    take_scene_buffers, which releases each mesh's words as it copies them into the buffers.
+    Returns (scene, buffers): the scene's meshes keep their ids, hashes and resolutions but no
+    words (its `triangles_taken` is set).  copy=False returns only the buffer sizes (no Python
+    copies of the words), for memory measurements."""
+    lib = load()
+    h = _decode_stream_handle(lib, source)
+    try:
+        buffers = _buffers_of_handle(lib, h, take=True, copy=copy)
+        scene = abi.read_scene(lib, "dl_", h)
+        scene.triangles_taken = bool(lib.dl_scene_triangles_taken(h))
+        return scene, buffers
+    finally:
+        lib.dl_baked_scene_free(h)
+
+
+def _buffers_of_handle(lib, h, take: bool = False, copy: bool = True) -> dict:
+    """scene_buffers (src/baked_lighting/apply.rs:27-38), or with take=True take_scene_buffers
+    (apply.rs:45-61), of a library scene: the triangles and aux word arrays and the receivers in
+    mesh order (tag = index + 1)."""
     b = C.c_void_p()
-    st = lib.dl_scene_buffers(h, C.byref(b))
+    st = (lib.dl_take_scene_buffers if take else lib.dl_scene_buffers)(h, C.byref(b))
     if st != 0:
         raise BakeError(st, lib.dl_last_error(None).decode())
     try:
@@ -209,6 +246,9 @@ def _buffers_of_handle(lib, h) -> dict:
         for name in ("triangles", "aux"):
             p = C.POINTER(C.c_uint32)()
             n = getattr(lib, f"dl_buffers_{name}")(b, C.byref(p))
+            if not copy:
+                out[name] = int(n)
+                continue
             out[name] = np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
         recs = []
         for i in range(lib.dl_buffers_receiver_count(b)):
@@ -223,7 +263,7 @@ def _buffers_of_handle(lib, h) -> dict:
 
 
 def scene_buffers_of_file(data: bytes) -> dict:
-    """The BakedBuffers of a decoded .dynlight file (decode_stream then take_scene_buffers)."""
+    """The BakedBuffers of a decoded .dynlight file (decode, then scene_buffers)."""
     lib = load()
     h = _decode_handle(lib, bytes(data))
     try:
@@ -474,3 +514,73 @@ def bake(inp: BakeInput, options: abi.dl_bake_options | None = None,
     if _default_ctx is None:
         _default_ctx = Context(0)
     return _default_ctx.bake(inp, options, progress)
+
+
+# ---- the ECS caller's conversions (src/bake/snapshot.rs, identity.rs, shared/bake_id.rs) -------
+def _f32(a, cols):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, cols))
+
+
+def mesh_content_hash(positions, indices, uv) -> int:
+    """mesh_content_hash (src/shared/mesh_hash.rs:16-33)."""
+    lib = load()
+    p, u = _f32(positions, 3), _f32(uv, 2)
+    i = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
+    return int(lib.dl_mesh_content_hash(p.ctypes.data_as(C.POINTER(C.c_float)), len(p), i.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        len(i), u.ctypes.data_as(C.POINTER(C.c_float)), len(u)))
+
+
+def _segments(segments):
+    raw = [s.encode() if isinstance(s, str) else bytes(s) for s in segments]
+    ptrs = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_uint64 * max(1, len(raw)))(*[len(r) for r in raw])
+    return raw, ptrs, lens
+
+
+def bake_id_from_path(segments) -> int:
+    """BakeId::from_path (src/shared/bake_id.rs:15-23)."""
+    raw, ptrs, lens = _segments(segments)
+    return int(load().dl_bake_id_from_path(ptrs, lens, len(raw)))
+
+
+def assign_ids(candidates) -> list[int]:
+    """assign_ids (src/bake/identity.rs:64-101) of (explicit id or None, name path, tiebreak hash,
+    position) candidates."""
+    lib = load()
+    keep, arr = [], (abi.dl_id_candidate * max(1, len(candidates)))()
+    for k, (explicit, segments, tiebreak, position) in enumerate(candidates):
+        raw, ptrs, lens = _segments(segments)
+        keep.append((raw, ptrs, lens))
+        c = arr[k]
+        c.has_explicit = 0 if explicit is None else 1
+        c.explicit_id = 0 if explicit is None else explicit
+        c.segment_count = len(raw)
+        c.segments = ptrs
+        c.segment_lengths = lens
+        c.tiebreak_hash = tiebreak
+        c.position = tuple(position)
+    out = (C.c_uint64 * max(1, len(candidates)))()
+    st = lib.dl_assign_ids(arr, len(candidates), out)
+    if st != 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    return [int(out[k]) for k in range(len(candidates))]
+
+
+def mesh_to_world(mesh_id: int, affine, positions, indices, uv1, normals=None) -> MeshData:
+    """MeshData::to_world (src/bake/snapshot.rs:229-272): a BakeMesh in world space from local
+    data and a 3x4 affine transform (columns of the linear part, then the translation)."""
+    lib = load()
+    fp = C.POINTER(C.c_float)
+    a = np.ascontiguousarray(np.asarray(affine, np.float32).reshape(12))
+    p, u = _f32(positions, 3), _f32(uv1, 2)
+    n = None if normals is None else _f32(normals, 3)
+    i = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
+    po, no, io_ = np.zeros_like(p), (None if n is None else np.zeros_like(n)), np.zeros_like(i)
+    h = C.c_uint64()
+    st = lib.dl_mesh_to_world(a.ctypes.data_as(fp), p.ctypes.data_as(fp), len(p), None if n is None else n.ctypes.data_as(fp),
+                              i.ctypes.data_as(C.POINTER(C.c_uint32)), len(i), u.ctypes.data_as(fp), len(u),
+                              po.ctypes.data_as(fp), None if no is None else no.ctypes.data_as(fp),
+                              io_.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h))
+    if st < 0:
+        raise BakeError(st, lib.dl_last_error(None).decode())
+    return MeshData(mesh_id, po, io_, u.copy(), normals=no if st == 1 else None, content_hash=int(h.value))

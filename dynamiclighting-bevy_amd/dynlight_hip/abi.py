@@ -23,6 +23,7 @@ DL_E_DEVICE = -16
 DL_E_OOM = -17
 DL_E_ARGUMENT = -18
 DL_E_FORMAT = -19
+DL_E_TRIANGLES_TAKEN = -20
 DL_NOALLOC_MAX_RAYS = 1 << 30  # include/dynlight.h
 
 ILLUMINATION_DIRECT, ILLUMINATION_SINGLE_BOUNCE = 0, 1
@@ -41,6 +42,7 @@ STATUS_NAMES = {
     DL_E_OOM: "OutOfMemory",
     DL_E_ARGUMENT: "Argument",
     DL_E_FORMAT: "Format",
+    DL_E_TRIANGLES_TAKEN: "TrianglesTaken",
 }
 
 
@@ -85,6 +87,12 @@ class dl_bake_input(C.Structure):
 class dl_receiver(C.Structure):
     _fields_ = [("id", C.c_uint64), ("content_hash", C.c_uint64), ("tag", C.c_uint32), ("triangle_count", C.c_uint32),
                 ("occludes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class dl_id_candidate(C.Structure):
+    _fields_ = [("has_explicit", C.c_uint32), ("segment_count", C.c_uint32), ("explicit_id", C.c_uint64),
+                ("segments", C.POINTER(C.c_char_p)), ("segment_lengths", C.POINTER(C.c_uint64)),
+                ("tiebreak_hash", C.c_uint64), ("position", C.c_float * 3), ("reserved", C.c_uint32)]
 
 
 class dl_service_request(C.Structure):
@@ -402,6 +410,7 @@ class BakedScene:
     distance_cubes: np.ndarray | None = None
     sun_maps: list = field(default_factory=list)
     sky_probes: dict | None = None
+    triangles_taken: bool = False
 
 
 def _struct_dict(s) -> dict:

@@ -32,7 +32,8 @@ typedef enum dl_status {
     DL_E_DEVICE = -16,         /* a HIP runtime call failed                                   */
     DL_E_OOM = -17,            /* device or host allocation failed                            */
     DL_E_ARGUMENT = -18,       /* null pointer or out-of-range argument at the ABI itself     */
-    DL_E_FORMAT = -19          /* FormatError of decode (src/bake/format/mod.rs:53-77)        */
+    DL_E_FORMAT = -19,         /* FormatError of decode (src/bake/format/mod.rs:53-77)        */
+    DL_E_TRIANGLES_TAKEN = -20 /* TrianglesTaken (src/bake/format/mod.rs:84-95)               */
 } dl_status;
 
 /* ---- enums mirroring src/light/kinds.rs:97-158 and src/shared/settings.rs:229-237 -------- */
@@ -246,11 +247,53 @@ typedef struct dl_receiver {
 } dl_receiver;
 typedef struct dl_baked_buffers dl_baked_buffers;
 int dl_scene_buffers(const dl_baked_scene* scene, dl_baked_buffers** out);
+/* take_scene_buffers (apply.rs:45-61): the same buffers, releasing each mesh's triangle words as
+ * they are copied so the scene and the buffers are never both whole in memory (the loader path,
+ * baked_lighting/loader.rs:47-68).  Afterwards the scene is marked taken: dl_encode and
+ * dl_scene_buffers refuse it with DL_E_TRIANGLES_TAKEN, as encode and scene_buffers do. */
+int dl_take_scene_buffers(dl_baked_scene* scene, dl_baked_buffers** out);
+/* BakedScene::triangles_taken */
+int dl_scene_triangles_taken(const dl_baked_scene* scene);
 uint64_t dl_buffers_triangles(const dl_baked_buffers* buffers, const uint32_t** words);
 uint64_t dl_buffers_aux(const dl_baked_buffers* buffers, const uint32_t** words);
 uint32_t dl_buffers_receiver_count(const dl_baked_buffers* buffers);
 int dl_buffers_receiver(const dl_baked_buffers* buffers, uint32_t index, dl_receiver* out);
 void dl_baked_buffers_free(dl_baked_buffers* buffers);
+
+/* ---- the ECS caller's conversions (src/bake/snapshot.rs:83-158, 229-272; identity.rs:64-101)
+ * For callers outside Bevy that assemble a dl_bake_input the way SceneSnapshot::take does. */
+/* mesh_content_hash (src/shared/mesh_hash.rs:16-33): positions (3 floats) and UV (2 floats) of
+ * every triangle corner in draw order by their bit patterns; indices past a buffer hash as
+ * missing. */
+uint64_t dl_mesh_content_hash(const float* positions, uint32_t vertex_count, const uint32_t* indices, uint64_t index_count,
+                              const float* uv, uint32_t uv_count);
+/* BakeId::from_path (src/shared/bake_id.rs:15-23): FNV-1a 64 over length-prefixed segments. */
+uint64_t dl_bake_id_from_path(const char* const* segments, const uint64_t* lengths, uint32_t count);
+/* A mesh or light awaiting its id (identity.rs:17-22): an explicit BakeId, or its Name path;
+ * tiebreak_hash and position order candidates whose derived ids collide (meshes pass their
+ * content hash, lights 0). */
+typedef struct dl_id_candidate {
+    uint32_t has_explicit;
+    uint32_t segment_count;
+    uint64_t explicit_id;
+    const char* const* segments;
+    const uint64_t* segment_lengths;
+    uint64_t tiebreak_hash;
+    float position[3];
+    uint32_t reserved;
+} dl_id_candidate;
+/* assign_ids (identity.rs:64-101): explicit ids are kept; derived ids that collide keep the plain
+ * id for the first in tiebreak order and get a "#k" segment for the k-th after it. */
+int dl_assign_ids(const dl_id_candidate* candidates, uint32_t count, uint64_t* out);
+/* MeshData::to_world (snapshot.rs:229-272): `affine` is 12 floats, the columns of the linear
+ * part then the translation.  Positions are transformed as points; normals (may be NULL) by the
+ * inverse transpose and normalized, or not written when the determinant is not normal (the
+ * reference then bakes flat normals); a negative determinant swaps corners 1 and 2 of every
+ * triangle so faces keep their front; *content_hash (may be NULL) hashes the local data.
+ * Returns 1 when normals were written, 0 when not, or a negative error. */
+int dl_mesh_to_world(const float* affine, const float* positions, uint32_t vertex_count, const float* normals,
+                     const uint32_t* indices, uint64_t index_count, const float* uv1, uint32_t uv_count, float* positions_out,
+                     float* normals_out, uint32_t* indices_out, uint64_t* content_hash);
 
 /* ---- rebake service (no counterpart in the reference, which re-runs bake() on every change) ----
  * A service keeps one scene's geometry resident on the device: receiver meshes, the occluder BVH

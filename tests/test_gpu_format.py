@@ -1,6 +1,8 @@
 """Baked scenes through the .dynlight codec on the GPU path: the file of a bake holds exactly the
 payload the format documents (src/bake/format/mod.rs:1-21, written independently in
 test_format.py), and decoding it gives the bake back (decode(encode(s)) == s)."""
+import io
+
 import numpy as np
 import pytest
 
@@ -138,3 +140,27 @@ def test_scene_buffers_of_a_bake(ctx, name):
     assert [int(w) for w in b["aux"]] == aux
     assert [r["id"] for r in b["receivers"]] == [m.id for m in scene.meshes]
     assert [r["tag"] for r in b["receivers"]] == list(range(1, len(scene.meshes) + 1))
+
+
+def test_a_full_outdoor_bake_loads_with_one_copy_of_its_words(ctx, tmp_path):
+    """The loader path on a real bake: the full outdoor scene's file (hundreds of MB of words) is
+    stream-decoded in a fresh process and its words are taken into the buffers mesh by mesh; the
+    peak host memory grows by about one copy of the words, and the buffers equal scene_buffers of
+    the baked scene."""
+    from test_loader_memory import _peak
+    inp, opts = scenes.outdoor(3)
+    scene, data = ctx.bake_encoded(inp, opts)
+    path = str(tmp_path / "outdoor.dynlight")
+    with open(path, "wb") as f:
+        f.write(data)
+    words = sum(len(m.triangles) for m in scene.meshes)
+    take = _peak(path, "take")
+    assert take["words"] == words
+    grow = take["peak"] - take["base"]
+    print(f"outdoor file: {len(data) / 2**20:.0f} MiB packed, {words * 4 / 2**20:.0f} MiB of words; load peak growth "
+          f"{grow / 2**20:.0f} MiB", flush=True)
+    assert grow < 1.15 * words * 4 + (64 << 20)
+    loaded, buffers = dynlight_hip.load_bake(io.BytesIO(data))
+    assert loaded.triangles_taken
+    tris = np.concatenate([m.triangles for m in scene.meshes])
+    assert np.array_equal(buffers["triangles"], tris)
