@@ -445,7 +445,10 @@ hipError_t copy_to_device(dl_ctx* ctx, void* dst, const void* src, size_t bytes)
     return hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream);
   }
   if (e != hipSuccess) return e;
-  return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  // A copy from pageable memory may still read `src` after hipMemcpyAsync returns; the caller's
+  // buffer is only borrowed for this call, so wait for the copy before giving it back.
+  e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  return e == hipSuccess ? hipStreamSynchronize(ctx->stream) : e;
 }
 
 // memcpy on up to `threads` threads in 4 MiB parts: a fresh destination's pages fault in parallel.
