@@ -12,11 +12,14 @@ record, texel position and coordinates, ray offset, traced count and bits), 16 B
 photon record) and 44 B per traced sample (ray and contribution written), with job, sample and
 traced counts from an instrumented (untimed) bake, divided by the kernel's average launch time
 from HIP events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes per launch
-recorded in profiles/interior_bake_kernels.md.  `stream_roofline` does the same for the any-hit
-traversal of the gather rays (k_any_stream: per ray, 112 B per 4-wide BVH node read plus 48 B per
-triangle tested plus its 32-byte ray record and 4-byte list entry) and `trace_roofline` for the
-direct shadow-ray kernel.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample of the same scene (every
-mesh still occludes) and extrapolates to the whole bake.
+recorded in profiles/interior_bake_kernels.md.  `other_kernels` gives the any-hit traversal of the
+gather rays (k_any_stream: per ray, 112 B per 4-wide BVH node read plus 48 B per triangle tested
+plus its 32-byte ray record and 4-byte list entry) and the direct shadow-ray kernel (k_trace) as
+work rates counted per use of each record (mostly L2-served, so not HBM figures), beside the HBM
+bytes rocprofv3 measured for them.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample
+of the same scene (every mesh still occludes) and extrapolates to the whole bake, and records the
+host CPU.  `batch` runs the 512-job batch workload in child processes at 4 (HIP's default) and 16
+hardware queues, and `serve` the rebake service at 20 requests/s (see --workload).
 """
 import argparse
 import json
@@ -24,15 +27,10 @@ import os
 import sys
 import time
 
-# HIP spreads a process's streams over GPU_MAX_HW_QUEUES hardware queues (4 unless set), so the
-# batch and serve workloads' worker streams (up to 16) would share 4 queues and serialize.  Raised
-# to 16 (an environment may already set 4) before anything initializes HIP.
-try:
-    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
-except ValueError:
-    _queues = 0
-if _queues < 16:
-    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# HIP spreads a process's streams over GPU_MAX_HW_QUEUES hardware queues (4 unless the
+# environment sets it before HIP starts).  The batch and serve workloads report the count they ran
+# with; the default interior run measures the batch at 4 and at 16 queues in child processes.
+HW_QUEUES = os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "dynamiclighting-bevy_amd")
@@ -63,6 +61,56 @@ def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "interior_bake_
     return None
 
 
+def host_cpu():
+    """The CPU the baseline ran on: model, sockets, cores and threads (lscpu), and the threads this
+    process may use."""
+    import subprocess
+    info = {"threads_used": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and keys[k.strip()] not in info:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def other_kernel(name, alg_bytes, launch_s, launches, extra):
+    """A traversal kernel's algorithmic bytes per launch over its HIP-event launch time, next to the
+    HBM bytes per launch rocprofv3 measured for it."""
+    traffic = measured_traffic(name)
+    alg = alg_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
+    hbm = traffic / launch_s / 1e9 if (traffic and launch_s > 0) else None
+    return {"kernel": name, "work_rate_GBps": alg, "work_rate_note": "bytes counted per use; mostly served from L2",
+            "alg_bytes_per_launch": alg_bytes, "launch_ms": launch_s * 1000.0, "launches_per_step": launches,
+            "hbm_bytes_per_launch": traffic, "hbm_GBps": hbm, "hbm_frac": hbm / HBM_PEAK_GBS if hbm else None, **extra}
+
+
+def child_run(argv, env_extra, timeout):
+    """Runs bench.py with `argv` in a child process (its progress goes to our stderr) and returns the
+    JSON line it prints, or None when it fails."""
+    import subprocess
+    env = dict(os.environ)
+    env.update(env_extra)
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__)] + argv, env=env, stdout=subprocess.PIPE, text=True,
+                           timeout=timeout)
+    except subprocess.TimeoutExpired:
+        log(f"child {' '.join(argv)} timed out")
+        return None
+    if r.returncode != 0:
+        log(f"child {' '.join(argv)} exited with {r.returncode}")
+        return None
+    for line in reversed(r.stdout.strip().splitlines()):
+        if line.startswith("{"):
+            return json.loads(line)
+    return None
+
+
 def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
@@ -87,17 +135,24 @@ def cpu_baseline(inp, opts, full_samples, modulus=256, cube_lights=2):
     o.photon_cube_resolution = 64
     s = oracle.bake_receivers(inp, o, mask, threads=threads)
     meshes_s = oracle.last_timings()["meshes_ms"] / 1000.0
+    # the cubes of the `cube_lights` lights nearest the first room shell, at full resolution (the
+    # oracle builds the cubes of lights that reach a sampled mesh: the shell is the sample)
+    shell = inp._positions[inp.vertex_offsets[0]:inp.vertex_offsets[1]]
+    centre = (shell.min(axis=0) + shell.max(axis=0)) * 0.5
+    near = sorted(inp.lights, key=lambda l: float(np.sum((np.asarray(l.position) - centre) ** 2)))[:cube_lights]
     few = abi.BakeInput.from_arrays(
         ids=inp.mesh_array["id"], positions=inp._positions, indices=inp._indices, uv1=inp._uv1, normals=inp._normals,
         vertex_offsets=inp.vertex_offsets, index_offsets=inp.index_offsets, albedo=inp.mesh_array["albedo"],
-        lights=inp.lights[:cube_lights])
+        lights=near)
     o2 = abi.dl_bake_options.from_buffer_copy(opts)
     o2.dynamic_geometry = abi.DYNAMIC_LIGHTING_ONLY
-    oracle.bake_receivers(few, o2, np.zeros(inp.mesh_count, bool), threads=threads)
+    first = np.zeros(inp.mesh_count, bool)
+    first[0] = True
+    oracle.bake_receivers(few, o2, first, threads=threads)
     cubes_s = oracle.last_timings()["photon_cubes_ms"] / 1000.0
     bounce_lights = sum(1 for l in inp.lights if l.illumination == abi.ILLUMINATION_SINGLE_BOUNCE)
     est = meshes_s * full_samples / max(1, s.stats["samples"]) + cubes_s * bounce_lights / cube_lights
-    return {"value": full_samples / est, "unit": "samples/s", "cores": threads, "kind": "port",
+    return {"value": full_samples / est, "unit": "samples/s", "cores": threads, "kind": "port", "cpu": host_cpu(),
             "sample": f"CPU oracle on {threads} threads: receivers splitmix64(id) % {modulus} == 0 "
                       f"({int(np.sum(mask))} of {len(mask)} meshes, {s.stats['samples']} samples, "
                       f"{s.stats['bounce_rays']} bounce rays) against all occluders and lights in "
@@ -120,7 +175,7 @@ def batch_cpu_baseline(ins, ops, stride=65, budget_s=25.0):
         done.append(j)
         if spent > budget_s:
             break
-    return {"value": samples / spent, "unit": "samples/s", "cores": threads, "kind": "port",
+    return {"value": samples / spent, "unit": "samples/s", "cores": threads, "kind": "port", "cpu": host_cpu(),
             "sample": f"CPU oracle on {threads} threads, whole bakes of jobs {done} ({samples} samples in {spent:.1f} s)"}
 
 
@@ -171,7 +226,7 @@ def batch_main(args):
                                "point/spot/area lights, 1e3-2e6 texels, bounce on odd jobs",
                    "jobs": len(ins), "samples_per_step": samples,
                    "bounce_rays_per_step": sum(x["bounce_rays"] for x in ref),
-                   "texels": sum(x["texels"] for x in ref), "streams": best_s},
+                   "texels": sum(x["texels"] for x in ref), "streams": best_s, "hw_queues": HW_QUEUES},
         "plain_value": samples / runs[1],
         "plain_ms": 1000.0 * runs[1],
         "batched_ms_by_streams": {str(k): 1000.0 * runs[k] for k in streams},
@@ -340,6 +395,7 @@ def box_main(args):
         "config": {"workload": "box room: 10x4x10 m, 2 boxes, 4 point lights r 6-12 m, 16 texels/m, 1-bit, no bounce",
                    **s.stats},
         "cpu_baseline": {"value": samples / cpu[threads], "unit": "samples/s", "cores": threads, "kind": "port",
+                         "cpu": host_cpu(),
                          "sample": f"the whole bake on the CPU oracle, best of 3: {cpu[threads] * 1000:.1f} ms on {threads} "
                                    f"threads, {cpu[1] * 1000:.1f} ms on 1 thread"},
         "cpu_one_thread_samples_per_s": samples / cpu[1],
@@ -354,6 +410,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="interior run: skip the batch workload's child runs")
+    ap.add_argument("--no-serve", action="store_true", help="interior run: skip the serve workload's child run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--workload", choices=("interior", "batch", "serve", "outdoor", "box"), default="interior",
@@ -468,16 +526,18 @@ def main():
                      "traced_per_step": counts["gather_traced"],
                      "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per sample (photon record), "
                               f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written)"},
-        "stream_roofline": {"kernel": "k_any_stream", "bound": "hbm", "achieved": g_achieved, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": g_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_any_stream"),
-                            "alg_bytes_per_launch": g_alg, "launch_ms": g_launch_s * 1000.0, "launches_per_step": launches,
-                            "node_fetches_per_step": bnodes, "triangle_tests_per_step": btris,
-                            "bytes": f"{NODE_BYTES:.0f} B per node record, {TRI_BYTES:.0f} B per triangle record, "
-                                     f"{RAY_BYTES:.0f} B per ray record, 4 B per list entry"},
-        "trace_roofline": {"kernel": "k_trace", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_step": b_alg,
-                           "traffic": measured_traffic("k_trace"),
-                           "node_fetches": nodes, "triangle_tests": tris},
+        # The other two traversal kernels, priced per use of each node and triangle record: most of
+        # those reads are served from L2, so this is a work rate, not an HBM bandwidth; the HBM
+        # bytes they move come from rocprofv3 (profiles/interior_bake_kernels.md).
+        "other_kernels": [other_kernel("k_any_stream", g_alg, g_launch_s, launches,
+                                       {"node_fetches_per_step": bnodes, "triangle_tests_per_step": btris,
+                                        "bytes": f"{NODE_BYTES:.0f} B per node record, {TRI_BYTES:.0f} B per triangle "
+                                                 f"record, {RAY_BYTES:.0f} B per ray record, 4 B per list entry"}),
+                          other_kernel("k_trace", b_alg / max(1, tm["trace_launches"]), kernel_s / max(1, tm["trace_launches"]),
+                                       tm["trace_launches"],
+                                       {"node_fetches_per_step": nodes, "triangle_tests_per_step": tris,
+                                        "bytes": f"{NODE_BYTES:.0f} B per node record, {TRI_BYTES:.0f} B per triangle "
+                                                 "record, the level bits of each sample"})],
         "stages_ms": {k: tm[k] for k in ("upload_ms", "texels_ms", "trace_ms", "bounce_ms", "post_ms", "maps_ms",
                                          "download_ms", "total_ms", "bounce_kernel_ms", "gather_trace_ms",
                                          "gather_rays_ms")},
@@ -486,6 +546,26 @@ def main():
         log("cpu baseline (oracle on a receiver sample)")
         out["cpu_baseline"] = cpu_baseline(inp, opts, samples)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+    del scene, s
+    ctx.close()
+    if not args.no_batch:
+        # the batched form against a loop of plain bakes (512 jobs), at HIP's default of 4 hardware
+        # queues and at 16
+        out["batch"] = {}
+        for q in ("4", "16"):
+            log(f"batch workload with GPU_MAX_HW_QUEUES={q} (child process)")
+            b = child_run(["--workload", "batch", "--steps", "1", "--no-cpu", "--streams", "4,8,16"],
+                          {"GPU_MAX_HW_QUEUES": q}, 900)
+            if b:
+                out["batch"][f"hw_queues_{q}"] = {k: b[k] for k in ("value", "plain_ms", "ms_per_step", "speedup_vs_plain",
+                                                                   "batched_ms_by_streams")}
+                out["batch"][f"hw_queues_{q}"]["streams"] = b["config"]["streams"]
+    if not args.no_serve:
+        log("serve workload (child process)")
+        v = child_run(["--workload", "serve"], {}, 900)
+        if v:
+            out["serve"] = {k: v[k] for k in v if k.endswith("_ms") or k.endswith("_per_s") or k in ("steps",)}
+            out["serve"]["config"] = v["config"]
     print(json.dumps(out), flush=True)
 
 

@@ -48,7 +48,7 @@ uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOU
 }  // namespace
 
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
-                 dl_baked_scene** out, Resident* resident) {
+                 dl_baked_scene** out, Resident* resident, Capture* cap) {
   auto t_start = std::chrono::steady_clock::now();
   ctx->cache.stream = ctx->stream;
   const CacheScope cache_scope(&ctx->cache);  // before every device buffer of the bake
@@ -443,6 +443,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   HIP_TRY(download(ctx, h_ent_light, d_ent_light.p, nent));
   HIP_TRY(download(ctx, h_crop, d_crop.p, (size_t)nt * 4));
   d_ent_tri.release();
+  if (cap) {
+    cap->pairs.clear();
+    cap->cubes.clear();
+    cap->crops = h_crop;
+    cap->texels.assign(nm, 0);
+  }
 
   // ---- regions (mesh_bake.rs:175-192) and bounce windows (bounce_map.rs:30-43) -----------------
   std::vector<uint32_t> reg_rect, reg_bits, reg_mesh, reg_light, h_ent_region(nent);
@@ -617,6 +623,25 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     HIP_TRY(zeroed(ctx, d_pool, b_reg_off[nbr] + 1));
     DevRegions rg{d_reg_rect.p, d_reg_bits.p, d_reg_off.p, d_reg_mesh.p, d_pool.p};
     DevEntries ent{nbe, d_bel.p, d_ber.p, d_bet.p, d_bes.p};
+    // capture: each region's own visited bits and counters
+    std::vector<uint64_t> b_vis_off(nbr + 1, 0);
+    DBuf<uint64_t> d_vis_off;
+    DBuf<uint32_t> d_rvis;
+    DBuf<unsigned long long> d_rcount, d_wcount;
+    if (cap) {
+      for (uint32_t r = 0; r < nbr; ++r) {
+        const uint32_t* q = &b_reg_rect[4 * (size_t)r];
+        b_vis_off[r + 1] = b_vis_off[r] + ((uint64_t)(q[2] - q[0] + 1) * (q[3] - q[1] + 1) + 31) / 32;
+      }
+      HIP_TRY(upload(ctx, d_vis_off, b_vis_off));
+      HIP_TRY(zeroed(ctx, d_rvis, b_vis_off[nbr] + 1));
+      HIP_TRY(zeroed(ctx, d_rcount, 2 * (size_t)nbr + 2));
+      HIP_TRY(zeroed(ctx, d_wcount, (size_t)nbw + 1));
+      rg.vis = d_rvis.p;
+      rg.vis_off = d_vis_off.p;
+      rg.rays = d_rcount.p;
+      rg.samples = d_rcount.p + nbr + 1;
+    }
 
     // ---- texels: owned, traced, burial, compaction (mesh_texels.rs:41-108) -------------------
     // A resident scene holds every triangle's texels already: the batch's are a contiguous run.
@@ -691,6 +716,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       }
     }
     for (uint32_t t = 0; t < nbt; ++t) scene->stats.texels += h_tex_start[t + 1] - h_tex_start[t];
+    if (cap)
+      for (uint32_t i = B.m0; i < B.m1; ++i) cap->texels[i] = h_tex_start[first_tri[i + 1] - t0] - h_tex_start[first_tri[i] - t0];
     texels_ms += ms_since(tb);
     if (cancelled()) { ctx->err = "the bake was cancelled"; return DL_E_CANCELLED; }
 
@@ -881,6 +908,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bb.rays = d_mbrays.p;
       bb.tile_start = d_win_start.p;
       bb.tile_lit = d_tile_lit.p;
+      bb.wrays = cap ? d_wcount.p : nullptr;
       BounceParams bp;
       bp.seed = opt.seed;
       bp.colored = opt.colored_bounce ? 1u : 0u;
@@ -1131,6 +1159,17 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         const uint32_t* q = &b_reg_rect[4 * (size_t)r];
         used_area.push_back(used_area.back() + (uint64_t)(q[2] - q[0] + 1) * (q[3] - q[1] + 1));
       }
+    std::shared_ptr<DevBlock> cap_block;
+    std::vector<unsigned long long> h_rcount, h_wcount;
+    if (cap) {  // the regions' levels before seam padding, and their visited bits
+      const uint64_t pool_words = b_reg_off[nbr], vis_words = b_vis_off[nbr];
+      cap_block = std::make_shared<DevBlock>();
+      HIP_TRY(cap_block->alloc(pool_words + vis_words, s));
+      if (pool_words) HIP_TRY(hipMemcpyAsync(cap_block->p, d_pool.p, pool_words * 4, hipMemcpyDeviceToDevice, s));
+      if (vis_words) HIP_TRY(hipMemcpyAsync(cap_block->p + pool_words, d_rvis.p, vis_words * 4, hipMemcpyDeviceToDevice, s));
+      HIP_TRY(download(ctx, h_rcount, d_rcount.p, 2 * (size_t)nbr + 2));
+      HIP_TRY(download(ctx, h_wcount, d_wcount.p, (size_t)nbw + 1));
+    }
     DBuf<uint32_t> d_used;
     DBuf<uint64_t> d_used_area;
     HIP_TRY(upload(ctx, d_used, used_list));
@@ -1192,6 +1231,49 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         ctx->captured.push_back(cc);
         ctx->captured_values.insert(ctx->captured_values.end(), vals.begin() + (long)vstart[k], vals.begin() + (long)vstart[k + 1]);
       }
+    }
+    if (cap) {
+      // the batch's words stay on the device: the kept entries' bounce payloads are read from them
+      auto wblock = std::make_shared<DevBlock>();
+      HIP_TRY(wblock->alloc(bw, s));
+      if (bw) HIP_TRY(hipMemcpyAsync(wblock->p, d_out.p, bw * 4, hipMemcpyDeviceToDevice, s));
+      std::vector<uint64_t> boff(nbe, 0), bwords(nbe, 0);  // bounce payload of each kept entry
+      for (size_t k = 0; k < kb_entry.size(); ++k) {
+        boff[kb_entry[k]] = kb_out[k];
+        bwords[kb_entry[k]] = kb_word_start[k + 1] - kb_word_start[k];
+      }
+      for (size_t k = 0; k < kd_entry.size(); ++k) {
+        boff[kd_entry[k]] = kd_out[k];
+        bwords[kd_entry[k]] = kd_word_start[k + 1] - kd_word_start[k];
+      }
+      const size_t first = cap->pairs.size();
+      for (uint32_t r = 0; r < nbr; ++r) {
+        CapturedPair c;
+        c.mesh = b_reg_mesh[r];
+        c.light_id = lights[reg_light[r0 + r]]->id;
+        std::memcpy(c.rect, &b_reg_rect[4 * (size_t)r], 16);
+        c.bits = b_reg_bits[r];
+        c.grid_off = b_reg_off[r];
+        c.grid_words = b_reg_off[r + 1] - b_reg_off[r];
+        c.vis_off = b_reg_off[nbr] + b_vis_off[r];
+        c.vis_words = b_vis_off[r + 1] - b_vis_off[r];
+        c.rays = h_rcount[r];
+        c.samples = h_rcount[nbr + 1 + r];
+        c.bounce_rays = win_of_reg[r0 + r] >= 0 ? h_wcount[(size_t)(win_of_reg[r0 + r] - w0)] : 0;
+        c.entries = 0;
+        c.block = cap_block;
+        c.words = wblock;
+        cap->pairs.push_back(std::move(c));
+      }
+      for (uint32_t t = 0; t < nbt; ++t)
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e) {
+          CapturedPair& c = cap->pairs[first + b_ent_region[e]];
+          c.entries++;
+          if (!h_keep[e]) continue;
+          c.kept.push_back(t0 + t - first_tri[c.mesh]);
+          c.bounce_off.push_back(boff[e]);
+          c.bounce_words.push_back(bwords[e]);
+        }
     }
     // the batch's words in one buffer, each mesh holding its slice
     auto words = std::make_shared<std::vector<uint32_t, NoInit<uint32_t>>>(bw);
@@ -1257,6 +1339,10 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     HIP_TRY(d_cubes.alloc((size_t)nlocal * 6 * 2048));
     HIP_TRY(launch_distance_cubes(s, sc, d_lights.p, nlocal, d_cubes.p));
     HIP_TRY(download(ctx, scene->distance_cubes, d_cubes.p, (size_t)nlocal * 6 * 2048));
+    if (cap)
+      for (uint32_t i = 0; i < nlocal; ++i)
+        cap->cubes.push_back({lights[i]->id, std::vector<uint32_t>(scene->distance_cubes.begin() + (long)i * 6 * 2048,
+                                                                   scene->distance_cubes.begin() + (long)(i + 1) * 6 * 2048)});
   }
   // ---- sun maps (sun_map.rs:25-75) and sky probes (sky_probes.rs:146-191), baker.rs:104-152 --
   if (opt.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY) {

@@ -141,6 +141,11 @@ const dl::TrigHost& host_trig() {
 
 void dl_set_thread_error(const std::string& msg) { g_null_ctx_err = msg; }
 
+namespace dl {
+int set_error(dl_ctx* ctx, int code, const std::string& msg) { return set_err(ctx, code, msg); }
+bool make_workers(dl_ctx* ctx, uint32_t streams) { return prepare_workers(ctx, streams); }
+}  // namespace dl
+
 extern "C" {
 
 int dl_ctx_create(int device, dl_ctx** out) {
@@ -318,10 +323,12 @@ int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_opt
     svc->ctx = ctx;
     svc->options = *options;
     dl_baked_scene* first = nullptr;
-    const int st = dl_bake_impl(ctx, scene, options, nullptr, &first, &svc->res);
+    dl::Capture cap;
+    int st = dl_bake_impl(ctx, scene, options, nullptr, &first, &svc->res, &cap);
     if (st != DL_OK) return st;
-    if (initial) *initial = first;
-    else delete first;
+    std::unique_ptr<dl_baked_scene> hold(first);
+    if ((st = dl::service_init_live(svc.get(), scene, cap, *first)) != DL_OK) return st;
+    if (initial) *initial = hold.release();
     *out = svc.release();
     return DL_OK;
   } catch (const std::bad_alloc&) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <memory>
@@ -277,20 +278,105 @@ struct Resident {
 };
 }  // namespace dl
 
-// A rebake service: one context, the bake options, and the resident scene.
+namespace dl {
+// Device memory outside any context's cache, for data that outlives a bake (a rebake service's
+// stores): allocated from the device pool on `stream`, freed on the stream it was made on.
+struct DevBlock {
+  uint32_t* p = nullptr;
+  size_t words = 0;
+  hipStream_t stream = nullptr;
+  DevBlock() = default;
+  DevBlock(const DevBlock&) = delete;
+  DevBlock& operator=(const DevBlock&) = delete;
+  ~DevBlock() {
+    if (p) (void)hipFreeAsync(p, stream);
+  }
+  hipError_t alloc(size_t n, hipStream_t s) {
+    stream = s;
+    words = n;
+    return hipMallocAsync((void**)&p, std::max<size_t>(n, 1) * 4, s);
+  }
+};
+
+// Per-(mesh, light) results of a bake, kept for a rebake service.  A "pair" is one region of a
+// mesh batch: a mesh and a light that reaches some of its triangles.
+struct CapturedPair {
+  uint32_t mesh;          // mesh index of the bake (sorted order)
+  uint64_t light_id;
+  uint32_t rect[4], bits;
+  uint64_t grid_off, grid_words;  // pre-padding level words, in `block`
+  uint64_t vis_off, vis_words;    // region-local visited bits, in `block`
+  uint64_t rays, samples, bounce_rays;
+  uint64_t entries;               // (triangle, light) entries of the pair; the kept ones follow
+  std::vector<uint32_t> kept;     // mesh-local triangles whose entry survived optimize, ascending
+  std::vector<uint64_t> bounce_off, bounce_words;  // per kept triangle: its bounce payload in `words`
+  std::shared_ptr<DevBlock> block, words;
+};
+struct Capture {
+  std::vector<CapturedPair> pairs;
+  std::vector<std::pair<uint64_t, std::vector<uint32_t>>> cubes;  // distance cube words per local light id
+  std::vector<uint32_t> crops;    // 4 per receiver triangle of the bake (scene order)
+  std::vector<uint64_t> texels;   // per mesh
+};
+}  // namespace dl
+
+namespace dl {
+// One (mesh, light) of a service's live scene: the light's region of the mesh (levels before seam
+// padding and its own visited bits, in the service's level store or, while new, in a capture
+// block), its counters, and its kept triangles with their payloads in the live words.
+struct LivePair {
+  uint64_t light_id = 0;
+  uint32_t rect[4] = {0, 0, 0, 0}, bits = 1;
+  uint64_t grid_off = 0, grid_words = 0, vis_off = 0, vis_words = 0;
+  uint64_t rays = 0, samples = 0, bounce_rays = 0, entries = 0;
+  std::vector<uint32_t> kept;          // mesh-local triangles, ascending
+  std::vector<uint64_t> shadow_off;    // per kept triangle: its shadow payload in the live words
+  std::vector<uint64_t> bounce_off, bounce_words;  // per kept triangle: its bounce payload (0 words: none)
+  std::shared_ptr<DevBlock> src_block, src_words;  // set while the pair's data is still in its capture
+};
+struct LiveMesh {
+  std::vector<LivePair> pairs;         // sorted by light id
+  uint64_t w_off = 0, w_len = 0;       // the mesh's words in the live words
+  uint64_t texels = 0;
+};
+}  // namespace dl
+
+// A rebake service: one context, the bake options, the resident scene, and its live scene (the
+// resident meshes with the current light set; every edit is applied to it).
 struct dl_service {
   dl_ctx* ctx = nullptr;
   dl_bake_options options{};
   dl::Resident res;
+  std::vector<dl_light> lights;                 // the live light set, sorted by id
+  std::vector<dl::LiveMesh> mesh;               // resident mesh order
+  std::vector<uint32_t> crops;                  // 4 per receiver triangle
+  std::unique_ptr<dl::DevBlock> W, G;           // live words; level store (levels, then visited bits)
+  std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
+  std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words
+  std::unique_ptr<dl_baked_scene> base;         // settings, sun maps and sky probes of the first bake
+  std::vector<dl_baked_light> baked;            // the live scene's lights in slot order
+  std::vector<dl_light_bvh_node> bvh;
+  dl_bake_stats stats{};
+  uint32_t stride = 2;
+  uint64_t version = 0;
 };
+
+namespace dl {
+// The C ABI's error bookkeeping and worker contexts, for the service's entry points.
+int set_error(dl_ctx* ctx, int code, const std::string& msg);
+bool make_workers(dl_ctx* ctx, uint32_t streams);
+// Builds the live scene of a service from the capture of its first bake.
+int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap, const dl_baked_scene& first);
+}  // namespace dl
 
 // Sets the error that dl_last_error(NULL) reports on this thread.
 void dl_set_thread_error(const std::string& msg);
 
 // The bake.  With `res` not ready, a normal bake that also fills it; with `res` ready, the meshes
 // of `input` are ignored and the resident ones are baked with the lights of `input`.
+
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
-                 dl_baked_scene** out, dl::Resident* res = nullptr);
+                 dl_baked_scene** out, dl::Resident* res = nullptr, dl::Capture* cap = nullptr);
 
 namespace dl {
 

@@ -88,6 +88,12 @@ struct DevRegions {
   const uint64_t* off;    // word offset in pool
   const uint32_t* mesh;
   uint32_t* pool;
+  // Capture of per-(mesh, light) data (null when not wanted): a region-local visited bit grid
+  // (1 bit per texel of the rect, LSB first, at vis[vis_off[r]]) and the region's rays and samples.
+  uint32_t* vis = nullptr;
+  const uint64_t* vis_off = nullptr;
+  unsigned long long* rays = nullptr;
+  unsigned long long* samples = nullptr;
 };
 
 struct DevEntries {
@@ -188,6 +194,7 @@ struct DevBounce {
   unsigned long long* rays;   // per mesh
   const uint64_t* tile_start; // filter tiles (kFilterTile squares) before each window, windows + 1
   uint8_t* tile_lit;          // per filter tile, zeroed: 1 once a gathered value in it has a positive component
+  unsigned long long* wrays = nullptr;  // per window: bounce rays of its (mesh, light) (capture; may be null)
 };
 hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
                           uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block);
@@ -285,6 +292,13 @@ constexpr int kFilterTile = 16;
 hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_mesh_res, const uint64_t* tile_start,
                                 uint32_t nwin, uint64_t ntiles, BlurWeights weights);
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
+// ORs every region's region-local visited bits into its mesh's visited grid.
+hipError_t launch_region_visited_or(hipStream_t s, DevMeshes meshes, DevRegions rg, uint32_t nregions, const uint64_t* vis_words_start,
+                                    uint64_t total_words);
+// Copies n word ranges: dst[r.y .. r.y + r.z) = src[r.x .. r.x + r.z) for r = ranges[i] (x, y, z as
+// u64 triples), from a scan of the lengths (range_start, n + 1 entries).
+hipError_t launch_copy_ranges(hipStream_t s, const uint32_t* src, uint32_t* dst, const uint64_t* ranges,
+                              const uint64_t* range_start, uint64_t n, uint64_t total_words);
 // Ray queries through a fixed per-GPU device scratch (queue head, hit count, overflow flag): no
 // device memory is allocated.  Rays, t_out and idx_out are device memory; *hits and *overflow are
 // read back when the queries are done.

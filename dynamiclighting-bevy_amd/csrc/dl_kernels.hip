@@ -741,11 +741,16 @@ __global__ void k_trace(DevScene sc, DevTris tr, DevMeshes ms, const DevLight* _
         const uint32_t* rc = rg.rect + 4 * (size_t)r;
         uint64_t ri = (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * (rc[2] - rc[0] + 1);
         raise_level(rg.pool, rg.off[r], rg.bits[r], ri, level);
+        if (rg.vis) atomicOr(&rg.vis[rg.vis_off[r] + ri / 32], 1u << (ri % 32));
       }
     }
     uint32_t m = tr.mesh[t];
     wave_add(&ms.rays[m], rays);
     wave_add(&ms.samples[m], some_n);
+    if (rg.rays) {  // one entry per wave: its region
+      wave_add(&rg.rays[r], rays);
+      wave_add(&rg.samples[r], some_n);
+    }
     if (sc.counters) {
       wave_add(&sc.counters[0], tl.nodes);
       wave_add(&sc.counters[1], tl.tris);
@@ -1051,6 +1056,12 @@ __global__ void k_vis_finish(DevTris tr, DevMeshes ms, const DevLight* __restric
         const uint32_t* rc = rg.rect + 4 * (size_t)r;
         const uint64_t ri = (uint64_t)(x - rc[0]) + (uint64_t)(y - rc[1]) * (rc[2] - rc[0] + 1);
         raise_level(rg.pool, rg.off[r], rg.bits[r], ri, level_of(fraction, L.bits));
+        if (rg.vis) atomicOr(&rg.vis[rg.vis_off[r] + ri / 32], 1u << (ri % 32));
+      }
+      if (rg.rays) {
+        const uint32_t rr = ent.region[v.e];
+        if (rays_n) atomicAdd(&rg.rays[rr], (unsigned long long)rays_n);
+        if (some_n) atomicAdd(&rg.samples[rr], 1ull);
       }
     }
     const uint32_t m0 = __shfl(m, 0, 64);
@@ -1370,6 +1381,34 @@ __global__ void k_trig_eval(TrigTables trig, const float* __restrict__ x, uint64
 }
 
 // out[i] = in[i] + delta (mod 2^32): moves triangle indices between numberings.
+// Region-local visited bits into the mesh visited grids: one thread per visited word.
+__global__ void k_region_visited_or(DevMeshes ms, DevRegions rg, uint32_t nreg, const uint64_t* __restrict__ start, uint64_t total) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)owner_of(start, nreg, w);
+    const uint32_t bits = rg.vis[rg.vis_off[r] + (w - start[r])];
+    if (!bits) continue;
+    const uint32_t* rc = rg.rect + 4 * (size_t)r;
+    const uint64_t rw = (uint64_t)(rc[2] - rc[0] + 1), texels = rw * (rc[3] - rc[1] + 1);
+    const uint32_t m = rg.mesh[r];
+    const uint64_t res = ms.res[m];
+    for (uint32_t b = 0; b < 32; ++b) {
+      const uint64_t ri = (w - start[r]) * 32 + b;
+      if (ri >= texels || !((bits >> b) & 1u)) continue;
+      const uint64_t vi = (rc[0] + ri % rw) + (rc[1] + ri / rw) * res;
+      atomicOr(&ms.visited[ms.visited_off[m] + vi / 32], 1u << (vi % 32));
+    }
+  }
+}
+
+__global__ void k_copy_ranges(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, const uint64_t* __restrict__ ranges,
+                              const uint64_t* __restrict__ start, uint64_t n, uint64_t total) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = owner_of(start, n, w);
+    const uint64_t off = w - start[k];
+    dst[ranges[3 * k + 1] + off] = src[ranges[3 * k] + off];
+  }
+}
+
 __global__ void k_add_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n, uint32_t delta) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = in[i] + delta;
@@ -1570,6 +1609,20 @@ hipError_t launch_ray_queries_noalloc(hipStream_t s, int device, DevScene scene,
   if (hits) *hits = (uint64_t)back[2] | ((uint64_t)back[3] << 32);
   if (overflow) *overflow = back[4];
   return hipSuccess;
+}
+hipError_t launch_region_visited_or(hipStream_t s, DevMeshes meshes, DevRegions rg, uint32_t nregions, const uint64_t* vis_words_start,
+                                    uint64_t total_words) {
+  if (total_words == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_region_visited_or, dim3(grid_for(total_words, kThreads)), dim3(kThreads), 0, s, meshes, rg, nregions,
+                     vis_words_start, total_words);
+  return hipGetLastError();
+}
+hipError_t launch_copy_ranges(hipStream_t s, const uint32_t* src, uint32_t* dst, const uint64_t* ranges, const uint64_t* range_start,
+                              uint64_t n, uint64_t total_words) {
+  if (total_words == 0 || n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_ranges, dim3(grid_for(total_words, kThreads)), dim3(kThreads), 0, s, src, dst, ranges, range_start, n,
+                     total_words);
+  return hipGetLastError();
 }
 hipError_t launch_trig_eval(hipStream_t s, TrigTables trig, const float* x, uint64_t n, float* sin_out, float* cos_out) {
   if (n == 0) return hipSuccess;
@@ -1847,6 +1900,7 @@ __global__ void k_bounce_gather(DevScene sc, DevOcc occ, DevTris tr, DevMeshes m
       }
     }
     wave_add(&b.rays[m], rays);
+    if (b.wrays) wave_add(&b.wrays[b.ent_window[u.e]], rays);  // one entry per wave
     if (sc.counters) {
       wave_add(&sc.counters[0], tl.nodes);
       wave_add(&sc.counters[1], tl.tris);
@@ -2210,6 +2264,7 @@ __global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restr
       rays += gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c) ? 1u : 0u;
     }
     atomicAdd(&b.rays[m], (unsigned long long)rays);
+    if (b.wrays && rays) atomicAdd(&b.wrays[b.ent_window[jb.y]], (unsigned long long)rays);
   }
 }
 
@@ -2247,9 +2302,10 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) tcount[n] = 0;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
     const uint64_t i = i0 + lane;
-    uint32_t m = 0xffffffffu, nr = 0;
+    uint32_t m = 0xffffffffu, nr = 0, win = 0;
     if (i < n) {
       const uint2 jb = jobs[i];
+      win = b.ent_window[jb.y];
       Pcg rng;
       GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
       const uint32_t alpha = lights[ent.light[jb.y]].alpha ? 1u : 0u;
@@ -2297,6 +2353,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
     } else if (i < n && nr) {
       atomicAdd(&b.rays[m], (unsigned long long)nr);
     }
+    if (b.wrays && i < n && nr) atomicAdd(&b.wrays[win], (unsigned long long)nr);
   }
 }
 
@@ -2591,9 +2648,10 @@ __global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
     const uint64_t i = i0 + lane;
-    uint32_t m = 0xffffffffu, nr = 0;
+    uint32_t m = 0xffffffffu, nr = 0, win = 0;
     if (i < n) {
       const uint2 jb = jobs[i];
+      win = b.ent_window[jb.y];
       const DevLight& L = lights[ent.light[jb.y]];
       const uint32_t samples = L.bounce_samples > 2 ? L.bounce_samples : 2u;
       f3 sum = mk3(0.0f, 0.0f, 0.0f);
@@ -2622,6 +2680,7 @@ __global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict
     } else if (i < n && nr) {
       atomicAdd(&b.rays[m], (unsigned long long)nr);
     }
+    if (b.wrays && i < n && nr) atomicAdd(&b.wrays[win], (unsigned long long)nr);
   }
 }
 

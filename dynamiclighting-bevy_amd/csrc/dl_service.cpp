@@ -1,0 +1,651 @@
+// The live scene of a rebake service: the resident meshes baked with the current light set, kept
+// on the device and updated edit by edit.
+//
+// bake() (baker.rs:27-203) couples a mesh's lights in two places only: the light slots (the
+// light BVH's leaf order, baker.rs:35-50) that every triangle's light data and entry order use,
+// and the mesh-wide `visited` grid that seam padding reads (mesh_bake.rs:246, seams.rs:15-39).
+// Everything else of a (mesh, light) pair depends on that light and the geometry alone: its
+// region's levels before padding, its own visited texels, which triangle entries optimize keeps
+// (shadows.rs:54-74, on unpadded levels), its bounce payloads (the dither streams are keyed by
+// light id, bounce_pack.rs:82-84) and its ray counts.  So the service keeps those per pair and an
+// edit
+//
+//   1. traces only the changed lights against the resident geometry (an ordinary resident bake
+//      of those lights, whose per-pair results are captured),
+//   2. replaces the changed lights' pairs, and for every mesh whose pairs changed ORs the pairs'
+//      visited bits into the mesh's grid, pads and crops every region of the mesh again,
+//   3. writes every mesh's words with the new slots: changed meshes from their new payloads,
+//      the others by copying their payloads (in the new entry order),
+//
+// which gives exactly the words, stats and lights of bake() of the edited light set.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <unordered_set>
+
+#include "../../include/dynlight.h"
+#include "dl_host.h"
+
+using namespace dl;
+
+namespace {
+
+struct Slots {
+  std::vector<const dl_light*> order;  // slot order: local lights in light-BVH order, then suns, then skies
+  std::vector<dl_light_bvh_node> bvh;
+};
+
+// baker.rs:35-50 on a light set sorted by id.
+Slots slots_of(const std::vector<dl_light>& lights) {
+  std::vector<const dl_light*> local, suns, skies;
+  for (const dl_light& l : lights) {
+    if (l.shape == DL_SHAPE_SUN) suns.push_back(&l);
+    else if (l.shape == DL_SHAPE_SKY) skies.push_back(&l);
+    else local.push_back(&l);
+  }
+  std::vector<f3> pos;
+  std::vector<float> rad;
+  for (const dl_light* l : local) {
+    pos.push_back(mk3(l->position[0], l->position[1], l->position[2]));
+    rad.push_back(l->radius);
+  }
+  LightBvhOut lb;
+  build_light_bvh(pos, rad, lb);
+  Slots s;
+  for (uint32_t i : lb.order) s.order.push_back(local[i]);
+  for (const dl_light* l : suns) s.order.push_back(l);
+  for (const dl_light* l : skies) s.order.push_back(l);
+  s.bvh = lb.nodes;
+  return s;
+}
+
+// BakedLight of a light (baker.rs:154-165): what its shape keeps, its shadow bits and bounce
+// compression.
+dl_baked_light baked_of(const dl_light& l, const dl_bake_options& opt) {
+  const DevLight d = make_dev_light(l, opt);
+  dl_baked_light b{};
+  b.id = l.id;
+  std::memcpy(b.origin, l.position, 12);
+  b.radius = l.radius;
+  b.shape = l.shape;
+  const float identity[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+  std::memcpy(b.rotation, identity, 16);
+  if (l.shape == DL_SHAPE_RECTANGLE || l.shape == DL_SHAPE_TUBE) {
+    std::memcpy(b.rotation, l.rotation, 16);
+    std::memcpy(b.size, l.size, 8);
+  } else if (l.shape == DL_SHAPE_SUN) {
+    std::memcpy(b.rotation, l.rotation, 16);
+    b.angular_diameter = l.angular_diameter;
+  }
+  b.shadow_bits_per_texel = d.bits;
+  b.bounce_compression = d.bakes_bounce ? (l.bounce_compression >= 0 ? l.bounce_compression : opt.bounce_compression) : -1;
+  return b;
+}
+
+uint64_t shadow_words(const uint32_t* crop, uint32_t bits) {
+  const uint64_t texels = (uint64_t)(crop[2] - crop[0] + 5) * (crop[3] - crop[1] + 5);
+  const uint64_t per = 32 / bits;
+  return (texels + per - 1) / per;
+}
+
+// Word ranges to copy from one source buffer: (src, dst, len) triples, merged when contiguous.
+struct Ranges {
+  std::vector<uint64_t> r;
+  void add(uint64_t src, uint64_t dst, uint64_t len) {
+    if (!len) return;
+    const size_t n = r.size();
+    if (n && r[n - 3] + r[n - 1] == src && r[n - 2] + r[n - 1] == dst) {
+      r[n - 1] += len;
+      return;
+    }
+    r.insert(r.end(), {src, dst, len});
+  }
+  void append(const Ranges& o) {
+    for (size_t i = 0; i < o.r.size(); i += 3) add(o.r[i], o.r[i + 1], o.r[i + 2]);
+  }
+};
+
+// Runs word copies grouped by source buffer on the context's stream.
+struct CopyPlan {
+  std::unordered_map<const uint32_t*, Ranges> by_src;
+  void add(const uint32_t* src, uint64_t s, uint64_t d, uint64_t len) { by_src[src].add(s, d, len); }
+  hipError_t run(dl_ctx* ctx, uint32_t* dst) {
+    for (auto& kv : by_src) {
+      const std::vector<uint64_t>& r = kv.second.r;
+      const uint64_t n = r.size() / 3;
+      if (!n) continue;
+      std::vector<uint64_t> start(n + 1, 0);
+      for (uint64_t k = 0; k < n; ++k) start[k + 1] = start[k] + r[3 * k + 2];
+      DBuf<uint64_t> d_r, d_s;
+      hipError_t e;
+      if ((e = upload(ctx, d_r, r)) || (e = upload(ctx, d_s, start))) return e;
+      if ((e = launch_copy_ranges(ctx->stream, kv.first, dst, d_r.p, d_s.p, n, start[n]))) return e;
+    }
+    return hipSuccess;
+  }
+};
+
+struct Entry {
+  uint32_t tri, slot, pair, k;
+};
+
+// Rewrites the live scene after its pairs changed: `dirty` meshes are padded and cropped again;
+// every mesh gets its words in the new slot order.  Replaces svc->W and svc->G.
+int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
+  dl_ctx* ctx = svc->ctx;
+  hipStream_t s = ctx->stream;
+  const dl_bake_options& opt = svc->options;
+  const Resident& res = svc->res;
+  const uint32_t nm = (uint32_t)svc->mesh.size();
+  const Slots sl = slots_of(svc->lights);
+  std::unordered_map<uint64_t, uint32_t> slot;
+  bool bounce = false;
+  std::vector<dl_baked_light> baked;
+  for (uint32_t i = 0; i < sl.order.size(); ++i) {
+    slot[sl.order[i]->id] = i;
+    baked.push_back(baked_of(*sl.order[i], opt));
+    bounce |= baked.back().bounce_compression >= 0;
+  }
+  const uint32_t stride = bounce ? 3u : 2u;
+  const bool restride = stride != svc->stride;
+
+  // ---- per mesh, on the host: entry order, word layout, light data, copies -----------------
+  struct Plan {
+    bool whole = false;            // words unchanged: one copy of the mesh
+    std::vector<uint32_t> ld;      // headers and light data (the mesh's first words)
+    uint64_t len = 0;
+    std::vector<Entry> ent;        // kept entries in word order
+    std::vector<uint64_t> sh, bo;  // per entry: shadow and bounce payload offset in the mesh
+  };
+  std::vector<Plan> plan(nm);
+  run_parallel(nm, ctx->host_threads, [&](uint64_t mi) {
+    const LiveMesh& m = svc->mesh[mi];
+    Plan& p = plan[mi];
+    bool same = !dirty[mi] && !restride && svc->W;
+    for (const LivePair& q : m.pairs) same = same && svc->slot.count(q.light_id) && svc->slot.at(q.light_id) == slot.at(q.light_id);
+    const uint32_t ft = res.first_tri[mi], nt = res.first_tri[mi + 1] - ft;
+    if (same) {
+      p.whole = true;
+      p.len = m.w_len;
+      return;
+    }
+    for (uint32_t pi = 0; pi < m.pairs.size(); ++pi) {
+      const LivePair& q = m.pairs[pi];
+      const uint32_t sl_q = slot.at(q.light_id);
+      for (uint32_t k = 0; k < q.kept.size(); ++k) p.ent.push_back({q.kept[k], sl_q, pi, k});
+    }
+    std::sort(p.ent.begin(), p.ent.end(), [](const Entry& a, const Entry& b) { return a.tri != b.tri ? a.tri < b.tri : a.slot < b.slot; });
+    std::vector<uint32_t> count(nt, 0);
+    for (const Entry& e : p.ent) count[e.tri]++;
+    std::vector<uint32_t> ld_off(nt);
+    uint64_t w = 4ull * nt;
+    for (uint32_t t = 0; t < nt; ++t) {
+      ld_off[t] = (uint32_t)w;
+      w += 1 + (uint64_t)stride * count[t];
+    }
+    p.ld.assign(w, 0u);
+    for (uint32_t t = 0; t < nt; ++t) {  // triangle_words.rs:47-89
+      const uint32_t* c = &svc->crops[4 * (size_t)(ft + t)];
+      p.ld[4 * t] = ld_off[t];
+      p.ld[4 * t + 1] = c[0] - 2u;
+      p.ld[4 * t + 2] = c[1] - 2u;
+      p.ld[4 * t + 3] = c[2] - c[0] + 1 + 4;
+      p.ld[ld_off[t]] = count[t];
+    }
+    p.sh.resize(p.ent.size());
+    p.bo.resize(p.ent.size());
+    std::vector<uint32_t> filled(nt, 0);
+    for (size_t j = 0; j < p.ent.size(); ++j) {
+      const Entry& e = p.ent[j];
+      const LivePair& q = m.pairs[e.pair];
+      p.sh[j] = w;
+      w += shadow_words(&svc->crops[4 * (size_t)(ft + e.tri)], q.bits);
+      const uint64_t bw = q.bounce_words[e.k];
+      p.bo[j] = bw ? w : 0;
+      w += bw;
+      uint32_t* le = &p.ld[ld_off[e.tri] + 1 + (uint64_t)stride * filled[e.tri]++];
+      le[0] = e.slot;
+      le[1] = (uint32_t)p.sh[j];
+      if (stride == 3) le[2] = (uint32_t)p.bo[j];
+    }
+    p.len = w;
+  });
+  std::vector<uint64_t> base(nm + 1, 0);
+  for (uint32_t i = 0; i < nm; ++i) base[i + 1] = base[i] + plan[i].len;
+  const uint64_t total = base[nm];
+
+  // ---- the copies into the new words ------------------------------------------------------
+  std::unique_ptr<DevBlock> W2(new DevBlock());
+  HIP_TRY(W2->alloc(total + 1, s));
+  std::vector<uint32_t> staged;  // light data of the rewritten meshes, uploaded once
+  CopyPlan words;
+  for (uint32_t i = 0; i < nm; ++i) {
+    const Plan& p = plan[i];
+    const LiveMesh& m = svc->mesh[i];
+    if (p.whole) {
+      words.add(svc->W->p, m.w_off, base[i], m.w_len);
+      continue;
+    }
+    words.add(nullptr, staged.size(), base[i], p.ld.size());  // resolved to the staging buffer below
+    staged.insert(staged.end(), p.ld.begin(), p.ld.end());
+    for (size_t j = 0; j < p.ent.size(); ++j) {
+      const Entry& e = p.ent[j];
+      const LivePair& q = m.pairs[e.pair];
+      if (!dirty[i]) {
+        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
+        words.add(svc->W->p, q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits));
+      }
+      if (q.bounce_words[e.k]) {
+        const uint32_t* src = q.src_words ? q.src_words->p : svc->W->p;
+        words.add(src, q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
+      }
+    }
+  }
+  DBuf<uint32_t> d_staged;
+  HIP_TRY(upload(ctx, d_staged, staged));
+  if (words.by_src.count(nullptr)) {
+    words.by_src[d_staged.p] = words.by_src[nullptr];
+    words.by_src.erase(nullptr);
+  }
+  HIP_TRY(words.run(ctx, W2->p));
+
+  // ---- the new level store: every pair's levels and visited bits, contiguous ---------------
+  uint64_t gtotal = 0;
+  for (const LiveMesh& m : svc->mesh)
+    for (const LivePair& q : m.pairs) gtotal += q.grid_words + q.vis_words;
+  std::unique_ptr<DevBlock> G2(new DevBlock());
+  HIP_TRY(G2->alloc(gtotal + 1, s));
+  std::vector<uint64_t> g_new;  // new grid offset of every pair, mesh by mesh
+  {
+    CopyPlan levels;
+    uint64_t at = 0;
+    for (const LiveMesh& m : svc->mesh)
+      for (const LivePair& q : m.pairs) {
+        const uint32_t* src = q.src_block ? q.src_block->p : svc->G->p;
+        levels.add(src, q.grid_off, at, q.grid_words);
+        levels.add(src, q.vis_off, at + q.grid_words, q.vis_words);
+        g_new.push_back(at);
+        at += q.grid_words + q.vis_words;
+      }
+    HIP_TRY(levels.run(ctx, G2->p));
+  }
+
+  // ---- dirty meshes: visited from their pairs, seam padding, crop and dilation -------------
+  {
+    std::vector<uint32_t> dm;
+    for (uint32_t i = 0; i < nm; ++i)
+      if (dirty[i] && !plan[i].whole) dm.push_back(i);
+    std::vector<uint32_t> r_rect, r_bits, r_mesh, used;
+    std::vector<uint64_t> r_off, r_vis, vis_start(1, 0), used_area(1, 0), m_res, m_voff(1, 0);
+    std::vector<uint32_t> kc_crop;  // 4 per kept entry
+    std::vector<uint32_t> k_region;
+    std::vector<uint64_t> k_start(1, 0), k_out;
+    size_t gi = 0;
+    std::vector<size_t> pair_g(nm + 1, 0);  // first g_new index of each mesh
+    for (uint32_t i = 0; i < nm; ++i) {
+      pair_g[i] = gi;
+      gi += svc->mesh[i].pairs.size();
+    }
+    for (uint32_t d = 0; d < dm.size(); ++d) {
+      const uint32_t i = dm[d];
+      const LiveMesh& m = svc->mesh[i];
+      const uint32_t r0 = (uint32_t)r_bits.size();
+      const uint32_t rs = res.res[i];
+      m_res.push_back(rs);
+      m_voff.push_back(m_voff.back() + ((uint64_t)rs * rs + 31) / 32);
+      for (size_t pi = 0; pi < m.pairs.size(); ++pi) {
+        const LivePair& q = m.pairs[pi];
+        r_rect.insert(r_rect.end(), q.rect, q.rect + 4);
+        r_bits.push_back(q.bits);
+        r_mesh.push_back(d);
+        const uint64_t at = g_new[pair_g[i] + pi];
+        r_off.push_back(at);  // levels are padded in a copy of the new store, below
+        r_vis.push_back(at + q.grid_words);
+        vis_start.push_back(vis_start.back() + q.vis_words);
+        if (!q.kept.empty()) {
+          used.push_back(r0 + (uint32_t)pi);
+          used_area.push_back(used_area.back() + (uint64_t)(q.rect[2] - q.rect[0] + 1) * (q.rect[3] - q.rect[1] + 1));
+        }
+      }
+      const Plan& p = plan[i];
+      for (size_t j = 0; j < p.ent.size(); ++j) {
+        const Entry& e = p.ent[j];
+        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
+        kc_crop.insert(kc_crop.end(), c, c + 4);
+        k_region.push_back(r0 + e.pair);
+        k_start.push_back(k_start.back() + shadow_words(c, m.pairs[e.pair].bits));
+        k_out.push_back(base[i] + p.sh[j]);
+      }
+    }
+    const uint32_t nreg = (uint32_t)r_bits.size();
+    const uint64_t nkept = k_out.size();
+    if (nreg) {
+      // seam padding changes levels in place: pad a copy of the store, keep the store unpadded
+      std::unique_ptr<DevBlock> work(new DevBlock());
+      HIP_TRY(work->alloc(gtotal + 1, s));
+      HIP_TRY(hipMemcpyAsync(work->p, G2->p, gtotal * 4, hipMemcpyDeviceToDevice, s));
+      DBuf<uint32_t> d_rect, d_bits, d_mesh, d_used, d_visited, d_kcrop, d_kregion;
+      DBuf<uint64_t> d_off, d_vis, d_vstart, d_uarea, d_mvoff, d_kstart, d_kout, d_kid;
+      DBuf<uint32_t> d_mres32;
+      std::vector<uint32_t> mres32(m_res.begin(), m_res.end());
+      HIP_TRY(upload(ctx, d_rect, r_rect));
+      HIP_TRY(upload(ctx, d_bits, r_bits));
+      HIP_TRY(upload(ctx, d_mesh, r_mesh));
+      HIP_TRY(upload(ctx, d_off, r_off));
+      HIP_TRY(upload(ctx, d_vis, r_vis));
+      HIP_TRY(upload(ctx, d_vstart, vis_start));
+      HIP_TRY(upload(ctx, d_mres32, mres32));
+      HIP_TRY(upload(ctx, d_mvoff, m_voff));
+      HIP_TRY(zeroed(ctx, d_visited, m_voff.back() + 1));
+      DevRegions rg{d_rect.p, d_bits.p, d_off.p, d_mesh.p, work->p};
+      rg.vis = work->p;
+      rg.vis_off = d_vis.p;
+      DevMeshes ms{};
+      ms.n = (uint32_t)m_res.size();
+      ms.res = d_mres32.p;
+      ms.visited_off = d_mvoff.p;
+      ms.owned_off = d_mvoff.p;
+      ms.visited = d_visited.p;
+      HIP_TRY(launch_region_visited_or(s, ms, rg, nreg, d_vstart.p, vis_start.back()));
+      HIP_TRY(upload(ctx, d_used, used));
+      HIP_TRY(upload(ctx, d_uarea, used_area));
+      HIP_TRY(launch_pad_seams(s, ms, rg, d_used.p, d_uarea.p, (uint32_t)used.size(), used_area.back()));
+      if (nkept) {
+        std::vector<uint64_t> kid(nkept);
+        std::vector<uint32_t> ktri(nkept);
+        for (uint64_t k = 0; k < nkept; ++k) {
+          kid[k] = k;
+          ktri[k] = (uint32_t)k;
+        }
+        DBuf<uint32_t> d_ktri;
+        HIP_TRY(upload(ctx, d_kcrop, kc_crop));
+        HIP_TRY(upload(ctx, d_kregion, k_region));
+        HIP_TRY(upload(ctx, d_ktri, ktri));
+        HIP_TRY(upload(ctx, d_kstart, k_start));
+        HIP_TRY(upload(ctx, d_kout, k_out));
+        HIP_TRY(upload(ctx, d_kid, kid));
+        DevTris tr{};
+        tr.n = (uint32_t)nkept;
+        tr.crop = d_kcrop.p;
+        DevEntries ent{nkept, nullptr, d_kregion.p, d_ktri.p, nullptr};
+        HIP_TRY(launch_crop(s, tr, ent, rg, d_kid.p, d_kstart.p, nkept, k_start.back(), d_kout.p, W2->p));
+      }
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+
+  // ---- the new state ------------------------------------------------------------------------
+  size_t gi = 0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    LiveMesh& m = svc->mesh[i];
+    const Plan& p = plan[i];
+    if (p.whole) {
+      for (LivePair& q : m.pairs) {
+        for (auto& o : q.shadow_off) o = o - m.w_off + base[i];
+        for (size_t k = 0; k < q.bounce_off.size(); ++k)
+          if (q.bounce_words[k]) q.bounce_off[k] = q.bounce_off[k] - m.w_off + base[i];
+      }
+    } else {
+      for (LivePair& q : m.pairs) q.shadow_off.assign(q.kept.size(), 0);
+      for (size_t j = 0; j < p.ent.size(); ++j) {
+        LivePair& q = m.pairs[p.ent[j].pair];
+        q.shadow_off[p.ent[j].k] = base[i] + p.sh[j];
+        if (q.bounce_words[p.ent[j].k]) q.bounce_off[p.ent[j].k] = base[i] + p.bo[j];
+      }
+    }
+    for (LivePair& q : m.pairs) {
+      q.grid_off = g_new[gi++];
+      q.vis_off = q.grid_off + q.grid_words;
+      q.src_block.reset();
+      q.src_words.reset();
+    }
+    m.w_off = base[i];
+    m.w_len = p.len;
+  }
+  svc->W = std::move(W2);
+  svc->G = std::move(G2);
+  svc->slot = std::move(slot);
+  svc->baked = std::move(baked);
+  svc->bvh = sl.bvh;
+  svc->stride = stride;
+  dl_bake_stats st{};
+  st.meshes = nm;
+  st.triangles = res.ntri;
+  for (const LiveMesh& m : svc->mesh) {
+    st.texels += m.texels;
+    for (const LivePair& q : m.pairs) {
+      st.rays += q.rays;
+      st.samples += q.samples;
+      st.bounce_rays += q.bounce_rays;
+      st.lights_removed += q.entries - q.kept.size();
+    }
+  }
+  st.triangle_words = total;
+  svc->stats = st;
+  return DL_OK;
+}
+
+// The captured pairs of a bake of `lights` replace the live pairs of those lights (and of
+// `removed`); marks the meshes whose pairs changed.
+void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& replaced, std::vector<uint8_t>& dirty) {
+  for (uint32_t i = 0; i < svc->mesh.size(); ++i) {
+    auto& pairs = svc->mesh[i].pairs;
+    const size_t before = pairs.size();
+    pairs.erase(std::remove_if(pairs.begin(), pairs.end(), [&](const LivePair& q) { return replaced.count(q.light_id) > 0; }),
+                pairs.end());
+    if (pairs.size() != before) dirty[i] = 1;
+  }
+  for (CapturedPair& c : cap.pairs) {
+    LivePair q;
+    q.light_id = c.light_id;
+    std::memcpy(q.rect, c.rect, 16);
+    q.bits = c.bits;
+    q.grid_off = c.grid_off;
+    q.grid_words = c.grid_words;
+    q.vis_off = c.vis_off;
+    q.vis_words = c.vis_words;
+    q.rays = c.rays;
+    q.samples = c.samples;
+    q.bounce_rays = c.bounce_rays;
+    q.entries = c.entries;
+    q.kept = std::move(c.kept);
+    q.bounce_off = std::move(c.bounce_off);
+    q.bounce_words = std::move(c.bounce_words);
+    q.shadow_off.assign(q.kept.size(), 0);
+    q.src_block = c.block;
+    q.src_words = c.words;
+    auto& pairs = svc->mesh[c.mesh].pairs;
+    pairs.insert(std::upper_bound(pairs.begin(), pairs.end(), q.light_id,
+                                  [](uint64_t id, const LivePair& p) { return id < p.light_id; }),
+                 std::move(q));
+    dirty[c.mesh] = 1;
+  }
+  for (auto& kv : cap.cubes) svc->cubes[kv.first] = std::move(kv.second);
+}
+
+// Bakes `lights` (the changed ones) against the resident geometry and captures their pairs.
+int trace_lights(dl_ctx* w, dl_service* svc, const std::vector<dl_light>& lights, Capture& cap) {
+  if (lights.empty()) return DL_OK;
+  const dl_bake_input in{nullptr, 0, lights.data(), (uint32_t)lights.size()};
+  dl_baked_scene* sc = nullptr;
+  const int st = dl_bake_impl(w, &in, &svc->options, nullptr, &sc, &svc->res, &cap);
+  delete sc;
+  return st;
+}
+
+// Checks an edit and returns the edited light set and the ids whose pairs it replaces.
+int edited_lights(dl_service* svc, const dl_service_edit& e, std::vector<dl_light>& out, std::unordered_set<uint64_t>& replaced) {
+  dl_ctx* ctx = svc->ctx;
+  if ((e.light_count && !e.lights) || (e.remove_count && !e.remove)) return set_error(ctx, DL_E_ARGUMENT, "null array in an edit");
+  std::unordered_set<uint64_t> current;
+  for (const dl_light& l : svc->lights) current.insert(l.id);
+  for (uint32_t k = 0; k < e.light_count; ++k) {
+    const dl_light& l = e.lights[k];
+    if (l.shape == DL_SHAPE_SUN || l.shape == DL_SHAPE_SKY)
+      return set_error(ctx, DL_E_ARGUMENT, "edits move or add local lights; light " + hex_id(l.id) + " is a sun or sky");
+    if (!replaced.insert(l.id).second) return set_error(ctx, DL_E_DUPLICATE_ID, "two lights share BakeId " + hex_id(l.id));
+    int st = check_light(ctx, l);
+    if (st) return st;
+  }
+  for (uint32_t k = 0; k < e.remove_count; ++k) {
+    const uint64_t id = e.remove[k];
+    if (!current.count(id)) return set_error(ctx, DL_E_ARGUMENT, "no light " + hex_id(id) + " to remove");
+    if (!replaced.insert(id).second) return set_error(ctx, DL_E_ARGUMENT, "light " + hex_id(id) + " is edited twice");
+    for (const dl_light& l : svc->lights)
+      if (l.id == id && (l.shape == DL_SHAPE_SUN || l.shape == DL_SHAPE_SKY))
+        return set_error(ctx, DL_E_ARGUMENT, "edits remove local lights; light " + hex_id(id) + " is a sun or sky");
+  }
+  for (const dl_light& l : svc->lights)
+    if (!replaced.count(l.id)) out.push_back(l);
+  for (uint32_t k = 0; k < e.light_count; ++k) out.push_back(e.lights[k]);
+  std::stable_sort(out.begin(), out.end(), [](const dl_light& a, const dl_light& b) { return a.id < b.id; });
+  return DL_OK;
+}
+
+double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+namespace dl {
+
+int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap, const dl_baked_scene& first) {
+  dl_ctx* ctx = svc->ctx;
+  svc->lights.assign(scene->lights, scene->lights + scene->light_count);
+  std::stable_sort(svc->lights.begin(), svc->lights.end(), [](const dl_light& a, const dl_light& b) { return a.id < b.id; });
+  svc->mesh.assign(svc->res.meshes.size(), LiveMesh{});
+  for (size_t i = 0; i < svc->mesh.size(); ++i) svc->mesh[i].texels = cap.texels[i];
+  svc->crops = cap.crops;
+  svc->W.reset();
+  svc->G.reset();
+  svc->slot.clear();
+  svc->stride = 0;  // forces every mesh to be written
+  std::vector<uint8_t> dirty(svc->mesh.size(), 1);
+  merge(svc, cap, {}, dirty);
+  svc->base.reset(new dl_baked_scene());
+  svc->base->settings = first.settings;
+  svc->base->sun_maps = first.sun_maps;
+  svc->base->probes = first.probes;
+  std::memcpy(svc->base->probe_origin, first.probe_origin, sizeof first.probe_origin);
+  svc->base->probe_spacing = first.probe_spacing;
+  std::memcpy(svc->base->probe_counts, first.probe_counts, sizeof first.probe_counts);
+  svc->base->probe_words = first.probe_words;
+  int st = assemble(svc, dirty);
+  if (st != DL_OK) return st;
+  // the assembled words must be the bake's own, mesh by mesh
+  std::vector<uint32_t> live;
+  HIP_TRY(download(ctx, live, svc->W->p, svc->stats.triangle_words));
+  for (size_t i = 0; i < svc->mesh.size(); ++i) {
+    const dl_word_slice& w = first.meshes[i].words;
+    if (w.size() != svc->mesh[i].w_len || !std::equal(w.begin(), w.end(), live.begin() + (long)svc->mesh[i].w_off))
+      return set_error(ctx, DL_E_DEVICE, "the live scene's words differ from its first bake (mesh " + hex_id(first.meshes[i].id) + ")");
+  }
+  svc->version = 0;
+  return DL_OK;
+}
+
+}  // namespace dl
+
+extern "C" {
+
+int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_update* info) {
+  if (!svc) return set_error(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (!edit) return set_error(ctx, DL_E_ARGUMENT, "null edit");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<dl_light> next;
+    std::unordered_set<uint64_t> replaced;
+    int st = edited_lights(svc, *edit, next, replaced);
+    if (st != DL_OK) return st;
+    std::vector<dl_light> changed(edit->lights, edit->lights + edit->light_count);
+    Capture cap;
+    if ((st = trace_lights(ctx, svc, changed, cap)) != DL_OK) return st;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<uint8_t> dirty(svc->mesh.size(), 0);
+    merge(svc, cap, replaced, dirty);
+    for (uint64_t id : replaced)
+      if (!std::any_of(changed.begin(), changed.end(), [&](const dl_light& l) { return l.id == id; })) svc->cubes.erase(id);
+    svc->lights = std::move(next);
+    if ((st = assemble(svc, dirty)) != DL_OK) return st;
+    svc->version++;
+    if (info) {
+      *info = dl_service_update{};
+      info->version = svc->version;
+      for (uint8_t d : dirty) info->dirty_meshes += d;
+      info->traced_pairs = cap.pairs.size();
+      info->trace_ms = ms_between(t0, t1);
+      info->assemble_ms = ms_between(t1, std::chrono::steady_clock::now());
+      info->total_ms = ms_between(t0, std::chrono::steady_clock::now());
+    }
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_error(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_error(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+int dl_service_scene(dl_service* svc, dl_baked_scene** out) {
+  if (!svc) return set_error(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (!out) return set_error(ctx, DL_E_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (!svc->W) return set_error(ctx, DL_E_ARGUMENT, "the service has no live scene");
+    std::unique_ptr<dl_baked_scene> sc(new dl_baked_scene());
+    const dl_baked_scene& b = *svc->base;
+    sc->settings = b.settings;
+    sc->settings.bounce = svc->stride == 3;
+    sc->settings.colored_bounce = svc->stride == 3 && svc->options.colored_bounce;
+    sc->stats = svc->stats;
+    sc->lights = svc->baked;
+    sc->bvh = svc->bvh;
+    auto words = std::make_shared<std::vector<uint32_t, NoInit<uint32_t>>>(svc->stats.triangle_words);
+    HIP_TRY(copy_to_host(ctx, words->data(), svc->W->p, svc->stats.triangle_words * 4));
+    sc->meshes.resize(svc->mesh.size());
+    for (size_t i = 0; i < svc->mesh.size(); ++i) {
+      dl_baked_mesh_out& mo = sc->meshes[i];
+      mo.id = svc->res.meshes[i].id;
+      mo.content_hash = svc->res.meshes[i].content_hash;
+      mo.resolution = svc->res.res[i];
+      mo.occludes = svc->res.meshes[i].occludes;
+      mo.words.share(words, words->data() + svc->mesh[i].w_off, svc->mesh[i].w_len);
+    }
+    if (svc->options.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY)
+      for (const dl_baked_light& l : svc->baked) {
+        if (l.shape == DL_SHAPE_SUN || l.shape == DL_SHAPE_SKY) continue;
+        const std::vector<uint32_t>& c = svc->cubes.at(l.id);
+        sc->distance_cubes.insert(sc->distance_cubes.end(), c.begin(), c.end());
+      }
+    sc->sun_maps = b.sun_maps;
+    sc->probes = b.probes;
+    std::memcpy(sc->probe_origin, b.probe_origin, sizeof b.probe_origin);
+    sc->probe_spacing = b.probe_spacing;
+    std::memcpy(sc->probe_counts, b.probe_counts, sizeof b.probe_counts);
+    sc->probe_words = b.probe_words;
+    *out = sc.release();
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_error(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_error(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}
+
+int dl_service_words(dl_service* svc, const uint32_t** device_words, uint64_t* count) {
+  if (!svc) return set_error(nullptr, DL_E_ARGUMENT, "null service");
+  if (!device_words || !count) return set_error(svc->ctx, DL_E_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lock(svc->ctx->mu);
+  *device_words = svc->W ? svc->W->p : nullptr;
+  *count = svc->W ? svc->stats.triangle_words : 0;
+  return DL_OK;
+}
+
+}  // extern "C"

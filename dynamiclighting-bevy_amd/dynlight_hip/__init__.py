@@ -66,6 +66,12 @@ def load() -> C.CDLL:
     lib.dl_service_ray_queries_noalloc.restype = C.c_int
     lib.dl_service_ray_queries_noalloc.argtypes = [P, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
                                                    C.POINTER(C.c_uint64)]
+    lib.dl_service_apply.restype = C.c_int
+    lib.dl_service_apply.argtypes = [P, C.POINTER(abi.dl_service_edit), C.POINTER(abi.dl_service_update)]
+    lib.dl_service_scene.restype = C.c_int
+    lib.dl_service_scene.argtypes = [P, C.POINTER(P)]
+    lib.dl_service_words.restype = C.c_int
+    lib.dl_service_words.argtypes = [P, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
     lib.dl_service_destroy.restype = None
     lib.dl_service_destroy.argtypes = [P]
     lib.dl_encode.restype = C.c_int
@@ -456,6 +462,37 @@ class Service:
             return abi.read_scene(self.lib, "dl_", h)
         finally:
             self.lib.dl_baked_scene_free(h)
+
+    def apply(self, lights=(), remove=()) -> dict:
+        """Edits the live scene (dl_service_apply): `lights` are added or replace the lights of the same
+        id, `remove` lists ids to drop.  Returns the update record."""
+        packed = BakeInput([], list(lights))
+        rm = (C.c_uint64 * max(1, len(remove)))(*remove)
+        e = abi.dl_service_edit(packed.struct.lights, packed.struct.light_count, rm, len(remove), 0.0)
+        u = abi.dl_service_update()
+        st = self.lib.dl_service_apply(self.handle, C.byref(e), C.byref(u))
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        return {name: getattr(u, name) for name, _ in u._fields_}
+
+    def scene(self) -> abi.BakedScene:
+        """The live scene (dl_service_scene), read back to the host."""
+        h = C.c_void_p()
+        st = self.lib.dl_service_scene(self.handle, C.byref(h))
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        try:
+            return abi.read_scene(self.lib, "dl_", h)
+        finally:
+            self.lib.dl_baked_scene_free(h)
+
+    def words(self) -> tuple[int, int]:
+        """(device address, word count) of the live scene's triangle words (dl_service_words)."""
+        p, n = C.c_void_p(), C.c_uint64()
+        st = self.lib.dl_service_words(self.handle, C.byref(p), C.byref(n))
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        return p.value or 0, n.value
 
     def replay(self, requests, workers: int = 4, read: bool | str = False, service_times: list | None = None):
         """Serves (lights, arrival_s) requests as they arrive on `workers` streams.  Returns the

@@ -95,6 +95,17 @@ class dl_id_candidate(C.Structure):
                 ("tiebreak_hash", C.c_uint64), ("position", C.c_float * 3), ("reserved", C.c_uint32)]
 
 
+class dl_service_edit(C.Structure):
+    _fields_ = [("lights", C.c_void_p), ("light_count", C.c_uint32), ("remove", C.POINTER(C.c_uint64)),
+                ("remove_count", C.c_uint32), ("arrival_s", C.c_double)]
+
+
+class dl_service_update(C.Structure):
+    _fields_ = [("version", C.c_uint64), ("dirty_meshes", C.c_uint32), ("reserved", C.c_uint32),
+                ("traced_pairs", C.c_uint64), ("trace_ms", C.c_double), ("assemble_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
 class dl_service_request(C.Structure):
     _fields_ = [("lights", C.c_void_p), ("light_count", C.c_uint32), ("arrival_s", C.c_double)]
 

@@ -323,6 +323,36 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
                       double* latency_ms, double* service_ms, int* status, dl_baked_scene** out);
 void dl_service_destroy(dl_service* svc);
 
+/* The live scene of a service: the resident meshes baked with its current light set (the lights
+ * it was created with, then every edit applied).  An edit adds lights, replaces lights by id (a
+ * moved or changed light) and removes lights; only local lights (point, rectangle, tube) are
+ * edited.  Applying it traces only the changed lights' (triangle, light) pairs against the
+ * resident geometry, pads and crops again the meshes whose pairs changed (seam padding reads the
+ * mesh-wide visited grid every light sets, mesh_bake.rs:246, seams.rs:15-39), and writes every
+ * mesh's words with the new light slots (baker.rs:35-50); afterwards the live scene equals dl_bake
+ * of the resident meshes with the edited light set (words, lights, light BVH, distance cubes,
+ * stats).  No counterpart in the reference, which re-runs bake() for every change. */
+typedef struct dl_service_edit {
+    const dl_light* lights;   /* lights to add, or to replace (same id)                        */
+    uint32_t light_count;
+    const uint64_t* remove;   /* ids of lights to remove                                       */
+    uint32_t remove_count;
+    double arrival_s;         /* for dl_service_replay_edits: arrival in seconds (sorted)      */
+} dl_service_edit;
+typedef struct dl_service_update {
+    uint64_t version;         /* edits applied to the live scene so far                        */
+    uint32_t dirty_meshes;    /* meshes padded and cropped again                               */
+    uint32_t reserved;
+    uint64_t traced_pairs;    /* (mesh, light) pairs traced for the edit                       */
+    double trace_ms, assemble_ms, total_ms;
+} dl_service_update;
+int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_update* info);
+/* A copy of the live scene on the host (downloads its words). */
+int dl_service_scene(dl_service* svc, dl_baked_scene** out);
+/* The live scene's triangle words on the device, every mesh's words back to back in mesh order
+ * (the `triangles` buffer of BakedBuffers, layout.rs:148-169); valid until the next edit. */
+int dl_service_words(dl_service* svc, const uint32_t** device_words, uint64_t* count);
+
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Answers `ray_count`
  * rays against the occluder BVH resident in `svc`, like dl_test_ray_queries against the same

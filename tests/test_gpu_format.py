@@ -145,8 +145,8 @@ def test_scene_buffers_of_a_bake(ctx, name):
 def test_a_full_outdoor_bake_loads_with_one_copy_of_its_words(ctx, tmp_path):
     """The loader path on a real bake: the full outdoor scene's file (hundreds of MB of words) is
     stream-decoded in a fresh process and its words are taken into the buffers mesh by mesh; the
-    peak host memory grows by about one copy of the words, and the buffers equal scene_buffers of
-    the baked scene."""
+    peak host memory grows by one copy of the words plus the mesh being taken, and the buffers
+    equal scene_buffers of the baked scene."""
     from test_loader_memory import _peak
     inp, opts = scenes.outdoor(3)
     scene, data = ctx.bake_encoded(inp, opts)
@@ -154,12 +154,15 @@ def test_a_full_outdoor_bake_loads_with_one_copy_of_its_words(ctx, tmp_path):
     with open(path, "wb") as f:
         f.write(data)
     words = sum(len(m.triangles) for m in scene.meshes)
+    largest = max(len(m.triangles) for m in scene.meshes)
     take = _peak(path, "take")
     assert take["words"] == words
     grow = take["peak"] - take["base"]
-    print(f"outdoor file: {len(data) / 2**20:.0f} MiB packed, {words * 4 / 2**20:.0f} MiB of words; load peak growth "
-          f"{grow / 2**20:.0f} MiB", flush=True)
-    assert grow < 1.15 * words * 4 + (64 << 20)
+    print(f"outdoor file: {len(data) / 2**20:.0f} MiB packed, {words * 4 / 2**20:.0f} MiB of words (largest mesh "
+          f"{largest * 4 / 2**20:.0f} MiB); load peak growth {grow / 2**20:.0f} MiB", flush=True)
+    # meshes are taken one at a time: the buffers plus the mesh being copied (the terrain is one
+    # mesh holding most of the words)
+    assert grow < 1.05 * (words + largest) * 4 + (64 << 20)
     loaded, buffers = dynlight_hip.load_bake(io.BytesIO(data))
     assert loaded.triangles_taken
     tris = np.concatenate([m.triangles for m in scene.meshes])

@@ -1,0 +1,112 @@
+"""The live scene of a rebake service (dl_service_apply): the service starts from a bake of its
+scene and applies light edits (moved, added, changed, removed lights) by tracing only the changed
+lights; after every edit the live scene must equal dl_bake of the resident meshes with the edited
+light set (and, on the small scene, the oracle's bake): words, lights, light BVH, distance cubes
+and stats."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from dynlight_hip import BakeError, Service, abi, scenes
+from helpers import assert_scenes_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def moved(light, dx, dy=0.0, dz=0.0, **kw):
+    x, y, z = light.position
+    return dataclasses.replace(light, position=(x + dx, y + dy, z + dz), **kw)
+
+
+def small():
+    return scenes.interior(seed=9, target_triangles=3000, lights=12, grid=2, density=12.0)
+
+
+def test_the_live_scene_starts_as_the_bake(ctx):
+    inp, o = small()
+    svc = Service(ctx, inp, o)
+    try:
+        want = ctx.bake(inp, o)
+        assert_scenes_equal(svc.initial, want)
+        assert_scenes_equal(svc.scene(), want)
+        ptr, n = svc.words()
+        assert ptr != 0 and n == sum(len(m.triangles) for m in want.meshes)
+    finally:
+        svc.close()
+
+
+def test_edits_equal_a_bake_of_the_edited_light_set(ctx):
+    inp, o = small()
+    lights = list(inp.lights)
+    svc = Service(ctx, inp, o)
+    try:
+        a, b, c = lights[0], lights[3], lights[7]
+        new = scenes.point_light(90_001, (4.0, 1.5, 2.0), 6.0, bounce=True)
+        edits = [
+            ("move one light", [moved(a, 1.0, 0.0, 0.5)], []),
+            ("add a light", [new], []),
+            ("move two, one across rooms", [moved(b, 0.5, 0.2, -0.3), moved(c, -4.0, 0.0, 0.0)], []),
+            ("change a radius and a bounce setting", [dataclasses.replace(lights[5], radius=3.0, bounce_samples=8)], []),
+            ("remove a light", [], [lights[2].id]),
+            ("move the added light and remove another", [moved(new, -1.5, 0.0, 1.0)], [lights[9].id]),
+        ]
+        current = {l.id: l for l in lights}
+        for step, (name, changed, removed) in enumerate(edits):
+            for l in changed:
+                current[l.id] = l
+            for i in removed:
+                del current[i]
+            info = svc.apply(changed, removed)
+            print(f"edit {step} ({name}): {info['dirty_meshes']} dirty meshes, {info['traced_pairs']} traced pairs, "
+                  f"{info['total_ms']:.1f} ms", flush=True)
+            assert info["version"] == step + 1
+            got = svc.scene()
+            now = inp.with_lights(sorted(current.values(), key=lambda l: l.id))
+            assert_scenes_equal(got, ctx.bake(now, o))
+            if step in (0, len(edits) - 1):
+                assert_scenes_equal(got, oracle.bake(now, o, threads=8))
+    finally:
+        svc.close()
+
+
+def test_edit_errors(ctx):
+    inp, o = small()
+    svc = Service(ctx, inp, o)
+    try:
+        with pytest.raises(BakeError) as e:
+            svc.apply([scenes.sun_light(5, (0.0, 1.0, 0.0), 0.5)])
+        assert e.value.code == abi.DL_E_ARGUMENT
+        with pytest.raises(BakeError) as e:
+            svc.apply([], [123456789])
+        assert e.value.code == abi.DL_E_ARGUMENT
+        with pytest.raises(BakeError) as e:
+            svc.apply([inp.lights[0], inp.lights[0]])
+        assert e.value.code == abi.DL_E_DUPLICATE_ID
+        bad = dataclasses.replace(inp.lights[1], radius=-1.0)
+        with pytest.raises(BakeError) as e:
+            svc.apply([bad])
+        assert e.value.code == abi.DL_E_INVALID_LIGHT
+        # a failed edit leaves the live scene as it was
+        assert_scenes_equal(svc.scene(), ctx.bake(inp, o))
+    finally:
+        svc.close()
+
+
+def test_an_edit_of_the_full_interior_equals_its_bake(ctx):
+    inp, o = scenes.interior(2)
+    svc = Service(ctx, inp, o)
+    try:
+        lights = list(inp.lights)
+        rng = np.random.default_rng(5)
+        changed = [moved(lights[int(k)], *rng.uniform(-2.0, 2.0, 3) * np.array([1, 0.2, 1])) for k in rng.choice(256, 4, replace=False)]
+        info = svc.apply(changed)
+        print(f"interior edit: {info['dirty_meshes']} dirty meshes, {info['traced_pairs']} traced pairs, "
+              f"trace {info['trace_ms']:.1f} ms, assemble {info['assemble_ms']:.1f} ms", flush=True)
+        current = {l.id: l for l in lights}
+        for l in changed:
+            current[l.id] = l
+        assert_scenes_equal(svc.scene(), ctx.bake(inp.with_lights(sorted(current.values(), key=lambda l: l.id)), o))
+    finally:
+        svc.close()
