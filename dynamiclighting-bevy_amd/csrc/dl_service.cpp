@@ -19,8 +19,11 @@
 //
 // which gives exactly the words, stats and lights of bake() of the edited light set.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <thread>
 #include <unordered_set>
 
 #include "../../include/dynlight.h"
@@ -649,3 +652,180 @@ int dl_service_words(dl_service* svc, const uint32_t** device_words, uint64_t* c
 }
 
 }  // extern "C"
+
+namespace {
+// FNV-1a over the live words and stats (a test's fingerprint of the live scene after an edit).
+uint64_t live_checksum(dl_service* svc, int* st) {
+  dl_ctx* ctx = svc->ctx;
+  std::vector<uint32_t> w;
+  *st = DL_OK;
+  if (svc->W && svc->stats.triangle_words) {
+    hipError_t e = download(ctx, w, svc->W->p, svc->stats.triangle_words);
+    if (e != hipSuccess) {
+      *st = set_error(ctx, DL_E_DEVICE, hipGetErrorString(e));
+      return 0;
+    }
+  }
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto put = [&](const void* p, size_t n) {
+    const uint8_t* b = static_cast<const uint8_t*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  };
+  put(w.data(), w.size() * 4);
+  put(&svc->stats, sizeof svc->stats);
+  for (const dl_baked_light& l : svc->baked) put(&l.id, 8);
+  return h;
+}
+
+// Device memory an edit's tracing may hold: photon cubes of its bounce lights and a share for
+// its rays and regions.
+uint64_t edit_bytes(const dl_service* svc, const dl_service_edit& e) {
+  const uint64_t r = svc->options.photon_cube_resolution;
+  uint64_t b = 0;
+  for (uint32_t k = 0; k < e.light_count; ++k) {
+    b += 256ull << 20;
+    if (e.lights[k].illumination == DL_ILLUMINATION_SINGLE_BOUNCE && !svc->options.skip_bounce) b += 6 * r * r * 16;
+  }
+  return b;
+}
+}  // namespace
+
+extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* edits, uint32_t count, uint32_t workers,
+                                       uint64_t budget_bytes, double* latency_ms, uint64_t* checksum, dl_service_update* info) {
+  if (!svc) return set_error(nullptr, DL_E_ARGUMENT, "null service");
+  dl_ctx* ctx = svc->ctx;
+  if (count && (!edits || !latency_ms)) return set_error(ctx, DL_E_ARGUMENT, "null argument");
+  if (workers == 0 || workers > 16) return set_error(ctx, DL_E_ARGUMENT, "workers must be 1 to 16");
+  for (uint32_t i = 0; i < count; ++i)
+    if (!(edits[i].arrival_s >= 0.0) || (i && edits[i].arrival_s < edits[i - 1].arrival_s))
+      return set_error(ctx, DL_E_ARGUMENT, "arrival times must be non-negative and in order");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (!make_workers(ctx, workers)) return set_error(ctx, DL_E_DEVICE, "creating a worker stream failed");
+    if (budget_bytes == 0) {
+      size_t free_b = 0, total_b = 0;
+      budget_bytes = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? free_b / 2 : (16ull << 30);
+    }
+    struct Slot {
+      Capture cap;
+      int status = DL_E_CANCELLED;
+      std::string err;
+      bool done = false;
+      uint64_t bytes = 0;
+      double trace_ms = 0;
+    };
+    std::vector<Slot> slots(count);
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t inflight = 0;
+    bool stop = false;
+    std::atomic<uint32_t> next{0};
+    uint32_t admit = 0;  // the next edit to admit: admission follows arrival order
+    const auto t0 = std::chrono::steady_clock::now();
+    auto arrival = [&](uint32_t i) {
+      return t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(std::chrono::duration<double>(edits[i].arrival_s));
+    };
+    auto work = [&](dl_ctx* w) {
+      (void)hipSetDevice(w->device);
+      for (;;) {
+        const uint32_t i = next.fetch_add(1);
+        if (i >= count) break;
+        Slot& sl = slots[i];
+        sl.bytes = edit_bytes(svc, edits[i]);
+        {
+          // Admission: edit i waits for its arrival, for every earlier edit to be admitted, and
+          // for the tracing in flight to leave room for it (an edit alone is always admitted).
+          // Bytes are returned when tracing ends, so an admitted edit never waits on another.
+          std::unique_lock<std::mutex> l(mu);
+          if (cv.wait_until(l, arrival(i), [&] { return stop; })) break;
+          cv.wait(l, [&] { return stop || (admit == i && (inflight == 0 || inflight + sl.bytes <= budget_bytes)); });
+          if (stop) break;
+          inflight += sl.bytes;
+          admit++;
+          cv.notify_all();
+        }
+        const auto a = std::chrono::steady_clock::now();
+        std::vector<dl_light> changed(edits[i].lights, edits[i].lights + edits[i].light_count);
+        int st;
+        try {
+          st = trace_lights(w, svc, changed, sl.cap);
+          if (st == DL_OK && hipStreamSynchronize(w->stream) != hipSuccess) st = set_error(w, DL_E_DEVICE, "tracing an edit failed");
+        } catch (const std::bad_alloc&) {
+          w->err = "host allocation failed";
+          st = DL_E_OOM;
+        } catch (...) {
+          w->err = "unexpected failure";
+          st = DL_E_DEVICE;
+        }
+        std::lock_guard<std::mutex> l(mu);
+        inflight -= sl.bytes;
+        sl.trace_ms = ms_between(a, std::chrono::steady_clock::now());
+        sl.status = st;
+        if (st != DL_OK) sl.err = "edit " + std::to_string(i) + ": " + w->err;
+        sl.done = true;
+        cv.notify_all();
+      }
+    };
+    std::vector<std::thread> ts;
+    for (uint32_t k = 0; k < workers; ++k) ts.emplace_back(work, ctx->workers[k]);
+    int result = DL_OK;
+    for (uint32_t i = 0; i < count && result == DL_OK; ++i) {  // edits join the live scene in order
+      Slot& sl = slots[i];
+      {
+        std::unique_lock<std::mutex> l(mu);
+        cv.wait(l, [&] { return sl.done; });
+      }
+      if (sl.status != DL_OK) {
+        result = set_error(ctx, sl.status, sl.err);
+        break;
+      }
+      const auto a = std::chrono::steady_clock::now();
+      std::vector<dl_light> next_lights;
+      std::unordered_set<uint64_t> replaced;
+      int st = edited_lights(svc, edits[i], next_lights, replaced);
+      std::vector<uint8_t> dirty(svc->mesh.size(), 0);
+      if (st == DL_OK) {
+        merge(svc, sl.cap, replaced, dirty);
+        for (uint64_t id : replaced)
+          if (!std::any_of(edits[i].lights, edits[i].lights + edits[i].light_count, [&](const dl_light& l) { return l.id == id; }))
+            svc->cubes.erase(id);
+        svc->lights = std::move(next_lights);
+        st = assemble(svc, dirty);
+      }
+      if (st != DL_OK) {
+        result = st;
+        break;
+      }
+      svc->version++;
+      const auto done = std::chrono::steady_clock::now();
+      latency_ms[i] = std::chrono::duration<double, std::milli>(done - arrival(i)).count();
+      if (info) {
+        info[i] = dl_service_update{};
+        info[i].version = svc->version;
+        for (uint8_t d : dirty) info[i].dirty_meshes += d;
+        info[i].traced_pairs = sl.cap.pairs.size();
+        info[i].trace_ms = sl.trace_ms;
+        info[i].assemble_ms = ms_between(a, done);
+        info[i].total_ms = info[i].trace_ms + info[i].assemble_ms;
+      }
+      sl.cap = Capture{};
+      if (checksum) {
+        checksum[i] = live_checksum(svc, &st);
+        if (st != DL_OK) result = st;
+      }
+    }
+    {
+      std::lock_guard<std::mutex> l(mu);
+      stop = true;
+      cv.notify_all();
+    }
+    next.store(count);
+    for (auto& t : ts) t.join();
+    return result;
+  } catch (const std::bad_alloc&) {
+    return set_error(ctx, DL_E_OOM, "host allocation failed");
+  } catch (...) {
+    return set_error(ctx, DL_E_DEVICE, "unexpected failure");
+  }
+}

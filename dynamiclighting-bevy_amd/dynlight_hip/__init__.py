@@ -68,6 +68,9 @@ def load() -> C.CDLL:
                                                    C.POINTER(C.c_uint64)]
     lib.dl_service_apply.restype = C.c_int
     lib.dl_service_apply.argtypes = [P, C.POINTER(abi.dl_service_edit), C.POINTER(abi.dl_service_update)]
+    lib.dl_service_replay_edits.restype = C.c_int
+    lib.dl_service_replay_edits.argtypes = [P, C.POINTER(abi.dl_service_edit), C.c_uint32, C.c_uint32, C.c_uint64,
+                                            C.c_void_p, C.c_void_p, C.POINTER(abi.dl_service_update)]
     lib.dl_service_scene.restype = C.c_int
     lib.dl_service_scene.argtypes = [P, C.POINTER(P)]
     lib.dl_service_words.restype = C.c_int
@@ -474,6 +477,25 @@ class Service:
         if st != 0:
             raise BakeError(st, self.ctx.error())
         return {name: getattr(u, name) for name, _ in u._fields_}
+
+    def replay_edits(self, edits, workers: int = 4, budget_bytes: int = 0, checksums: bool = False):
+        """Serves edits (lights, remove, arrival_s) as they arrive (dl_service_replay_edits): tracing
+        on `workers` streams under a device-memory budget, joining the live scene in arrival order.
+        Returns (latencies in ms, update records, checksums or None)."""
+        n = len(edits)
+        packed = [BakeInput([], list(l)) for l, _, _ in edits]
+        rms = [(C.c_uint64 * max(1, len(r)))(*r) for _, r, _ in edits]
+        arr = (abi.dl_service_edit * max(1, n))()
+        for i, (p, rm, (_, r, t)) in enumerate(zip(packed, rms, edits)):
+            arr[i] = abi.dl_service_edit(p.struct.lights, p.struct.light_count, rm, len(r), float(t))
+        lat = (C.c_double * max(1, n))()
+        sums = (C.c_uint64 * max(1, n))()
+        info = (abi.dl_service_update * max(1, n))()
+        st = self.lib.dl_service_replay_edits(self.handle, arr, n, workers, budget_bytes, lat, sums if checksums else None, info)
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        ups = [{name: getattr(info[i], name) for name, _ in info[i]._fields_} for i in range(n)]
+        return list(lat[:n]), ups, (list(sums[:n]) if checksums else None)
 
     def scene(self) -> abi.BakedScene:
         """The live scene (dl_service_scene), read back to the host."""

@@ -347,6 +347,15 @@ typedef struct dl_service_update {
     double trace_ms, assemble_ms, total_ms;
 } dl_service_update;
 int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_update* info);
+/* Serves edits as they arrive, with continuous batching: each edit's changed lights are traced on
+ * the first of `workers` (1-16) worker streams that is free, admitted while the device memory the
+ * edits being traced may hold fits `budget_bytes` (0: half the free device memory), so edits
+ * that arrive while others run overlap on the device; edits join the live scene in arrival order.
+ * latency_ms[i] runs from edit i's arrival to the live scene holding it; checksum (may be NULL)
+ * receives a fingerprint of the live words, stats and light order after each edit (the same
+ * whether the edits ran alone or in the mix); info (may be NULL) each edit's update record. */
+int dl_service_replay_edits(dl_service* svc, const dl_service_edit* edits, uint32_t count, uint32_t workers,
+                            uint64_t budget_bytes, double* latency_ms, uint64_t* checksum, dl_service_update* info);
 /* A copy of the live scene on the host (downloads its words). */
 int dl_service_scene(dl_service* svc, dl_baked_scene** out);
 /* The live scene's triangle words on the device, every mesh's words back to back in mesh order

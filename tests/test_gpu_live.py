@@ -110,3 +110,98 @@ def test_an_edit_of_the_full_interior_equals_its_bake(ctx):
         assert_scenes_equal(svc.scene(), ctx.bake(inp.with_lights(sorted(current.values(), key=lambda l: l.id)), o))
     finally:
         svc.close()
+
+
+def _edit_stream(lights, n, seed):
+    """n edits of a light set: moves of one or two lights, a bounce change, removals and additions,
+    each (changed lights, removed ids); returns the edits and the light set after each."""
+    rng = np.random.default_rng(seed)
+    current = {l.id: l for l in lights}
+    edits, after = [], []
+    for k in range(n):
+        ids = sorted(current)
+        kind = k % 5
+        if kind == 3 and len(ids) > 4:
+            gone = int(ids[int(rng.integers(len(ids)))])
+            if current[gone].shape in (abi.SHAPE_SUN, abi.SHAPE_SKY):
+                kind = 0
+            else:
+                del current[gone]
+                edits.append(([], [gone]))
+                after.append(dict(current))
+                continue
+        if kind == 4:
+            new = scenes.point_light(80_000 + k, tuple(rng.uniform([0.5, 0.5, 0.5], [7.5, 2.5, 7.5])), 5.0, bounce=bool(k % 2))
+            current[new.id] = new
+            edits.append(([new], []))
+            after.append(dict(current))
+            continue
+        local = [i for i in ids if current[i].shape not in (abi.SHAPE_SUN, abi.SHAPE_SKY)]
+        pick = rng.choice(local, 1 + (kind == 1), replace=False)
+        changed = []
+        for i in pick:
+            l = moved(current[int(i)], *rng.uniform(-1.0, 1.0, 3) * np.array([1, 0.2, 1]))
+            if kind == 2:
+                l = dataclasses.replace(l, bounce_samples=int(rng.integers(4, 16)))
+            current[l.id] = l
+            changed.append(l)
+        edits.append((changed, []))
+        after.append(dict(current))
+    return edits, after
+
+
+def test_replayed_edits_overlap_and_match_edits_one_at_a_time(ctx):
+    """Continuous batching of edits (dl_service_replay_edits): edits arriving together trace on
+    several streams at once, yet after each edit the live scene is the same as when the edits run
+    one at a time, under a budget that admits one edit at a time, and the last one equals a bake."""
+    inp, o = small()
+    edits, after = _edit_stream(list(inp.lights), 10, seed=3)
+    stream = [(c, r, 0.002 * i) for i, (c, r) in enumerate(edits)]
+    one = Service(ctx, inp, o)
+    try:
+        lat1, up1, sum1 = one.replay_edits(stream, workers=1, budget_bytes=1, checksums=True)
+    finally:
+        one.close()
+    many = Service(ctx, inp, o)
+    try:
+        lat4, up4, sum4 = many.replay_edits(stream, workers=4, checksums=True)
+        print("latency ms, one at a time:", [round(x, 1) for x in lat1], flush=True)
+        print("latency ms, four streams:", [round(x, 1) for x in lat4], flush=True)
+        assert sum1 == sum4
+        assert [u["version"] for u in up4] == list(range(1, len(edits) + 1))
+        assert [u["dirty_meshes"] for u in up1] == [u["dirty_meshes"] for u in up4]
+        assert all(x > 0 for x in lat4)
+        final = inp.with_lights(sorted(after[-1].values(), key=lambda l: l.id))
+        assert_scenes_equal(many.scene(), ctx.bake(final, o))
+    finally:
+        many.close()
+    # one edit applied alone gives the checksum the replay recorded after that edit
+    alone = Service(ctx, inp, o)
+    try:
+        c, r = edits[0]
+        _, _, s = alone.replay_edits([(c, r, 0.0)], workers=1, checksums=True)
+        assert s[0] == sum1[0]
+    finally:
+        alone.close()
+
+
+def test_replay_errors_keep_the_edits_before_them(ctx):
+    inp, o = small()
+    lights = list(inp.lights)
+    svc = Service(ctx, inp, o)
+    try:
+        with pytest.raises(BakeError) as e:
+            svc.replay_edits([([moved(lights[0], 0.5)], [], 1.0), ([moved(lights[1], 0.5)], [], 0.5)])
+        assert e.value.code == abi.DL_E_ARGUMENT
+        with pytest.raises(BakeError) as e:
+            svc.replay_edits([([moved(lights[0], 0.5)], [], 0.0)], workers=0)
+        assert e.value.code == abi.DL_E_ARGUMENT
+        first = moved(lights[0], 0.5)
+        with pytest.raises(BakeError) as e:
+            svc.replay_edits([([first], [], 0.0), ([], [987654321], 0.0), ([moved(lights[1], 0.5)], [], 0.0)], workers=3)
+        assert e.value.code == abi.DL_E_ARGUMENT
+        current = {l.id: l for l in lights}
+        current[first.id] = first
+        assert_scenes_equal(svc.scene(), ctx.bake(inp.with_lights(sorted(current.values(), key=lambda l: l.id)), o))
+    finally:
+        svc.close()
