@@ -45,6 +45,62 @@ BlurWeights blur_weights() {
 
 uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOUNCE_BITS6 ? 6u : c == DL_BOUNCE_BITS5 ? 5u : 4u; }
 
+// Burial of a batch's ntex texels (mesh_texels.rs:57-87) as ray streams, in chunks of texels:
+// rays 0-3 of every texel of the chunk through the closest-hit stream, then rays 4-7 of the
+// texels those leave undecided.  Writes buried[i] and counts the kept texels per triangle.
+// With `counters` (6 words, counting on), counters[4..5] receive the node fetches and triangle
+// tests of the burial rays; *rays counts the rays cast and *ms the device time.
+hipError_t burial_streamed(dl_ctx* ctx, DevScene sc, const DevTris& tr, const DevMeshes& ms, const DevTexels& tx, uint64_t ntex,
+                           uint64_t seed, uint8_t* buried, unsigned long long* kept, unsigned long long* counters,
+                           uint64_t* rays_cast, double* ms_spent) {
+  if (ntex == 0) return hipSuccess;
+  if (ntex >= (1ull << 32)) return hipErrorInvalidValue;  // texels are listed by 32-bit index
+  hipStream_t s = ctx->stream;
+  const uint64_t chunk = std::min<uint64_t>(ntex, std::max<uint64_t>(1ull << 20, ctx->gather.chunk_rays / 8));
+  GatherLaunch gl = ctx->gather;
+  gl.order = 0;  // slots go out in storage order
+  if (ctx->burial_block) gl.block = ctx->burial_block;
+  DBuf<float4> rays, hits;
+  DBuf<uint64_t> total;
+  DBuf<uint8_t> back;
+  DBuf<uint32_t> list;
+  DBuf<unsigned long long> ctr;  // queue of pass 1, queue of pass 2, undecided count
+  hipError_t e;
+  if ((e = rays.alloc(8 * chunk)) || (e = hits.alloc(4 * chunk)) || (e = total.alloc(1)) || (e = back.alloc(ntex)) ||
+      (e = list.alloc(chunk)) || (e = ctr.alloc(3)))
+    return e;
+  unsigned int* q1 = reinterpret_cast<unsigned int*>(ctr.p);
+  unsigned int* q2 = reinterpret_cast<unsigned int*>(ctr.p + 1);
+  sc.counters = counters ? counters + 4 : nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess) (void)hipEventRecord(ev[0], s);
+  for (uint64_t c0 = 0; c0 < ntex; c0 += chunk) {
+    const uint64_t n = std::min(chunk, ntex - c0);
+    if ((e = hipMemsetAsync(ctr.p, 0, 3 * sizeof(unsigned long long), s)) ||
+        (e = launch_burial_rays(s, tr, ms, tx, c0, nullptr, nullptr, n, 0, 4, seed, ctx->trig, rays.p, hits.p, total.p)) ||
+        (e = launch_closest_stream(s, sc, rays.p, total.p, 4 * n, n, hits.p, q1, gl)) ||
+        (e = launch_burial_decide(s, sc, tx, c0, nullptr, nullptr, n, 0, 4, rays.p, hits.p, back.p, buried, kept, list.p, ctr.p + 2)) ||
+        (e = launch_burial_rays(s, tr, ms, tx, 0, list.p, ctr.p + 2, n, 4, 4, seed, ctx->trig, rays.p, hits.p, total.p)) ||
+        (e = launch_closest_stream(s, sc, rays.p, total.p, 4 * n, n, hits.p, q2, gl)) ||
+        (e = launch_burial_decide(s, sc, tx, 0, list.p, ctr.p + 2, n, 4, 4, rays.p, hits.p, back.p, buried, kept, nullptr, nullptr)))
+      break;
+    if (counters) {
+      unsigned long long undecided = 0;
+      if ((e = hipMemcpyAsync(&undecided, ctr.p + 2, 8, hipMemcpyDeviceToHost, s)) || (e = hipStreamSynchronize(s))) break;
+      *rays_cast += 4 * (n + undecided);
+    }
+  }
+  if (ev[1]) {
+    float t = 0.0f;
+    if (e == hipSuccess && hipEventRecord(ev[1], s) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&t, ev[0], ev[1]) == hipSuccess)
+      *ms_spent += t;
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return e;
+}
+
 }  // namespace
 
 int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options* options, dl_progress* progress,
@@ -558,7 +614,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   std::unique_ptr<dl_baked_scene> scene(new dl_baked_scene());
   scene->meshes.resize(nm);
   DBuf<unsigned long long> d_counters;
-  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 4));
+  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 6));
+  uint64_t burial_rays = 0;
+  double burial_ms = 0.0;
   float bounce_kernel_ms = 0.0f;
   const BlurWeights blur = blur_weights();
   const uint32_t stride = bounce_in_scene ? 3 : 2;
@@ -685,7 +743,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       DBuf<uint8_t> d_buried;
       HIP_TRY(d_buried.alloc(ntex0 + 1));
       HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
-      HIP_TRY(launch_burial(s, sc, tr, ms, tx0, ntex0, opt.seed, ctx->trig, d_buried.p, d_cnt.p, ctx->burial_block));
+      HIP_TRY(burial_streamed(ctx, sc, tr, ms, tx0, ntex0, opt.seed, d_buried.p, d_cnt.p, ctx->counting ? d_counters.p : nullptr,
+                              &burial_rays, &burial_ms));
       HIP_TRY(d_tex_start.alloc((size_t)nbt + 1));
       HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
       HIP_TRY(download(ctx, h_tex_start, d_tex_start.p, (size_t)nbt + 1));
@@ -1322,9 +1381,13 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   ctx->timings.bounce_ms = bounce_ms;
   ctx->timings.post_ms = post_ms;
   ctx->timings.bounce_kernel_ms = bounce_kernel_ms;
+  ctx->timings.burial_ms = burial_ms;
   if (ctx->counting) {
-    unsigned long long c[4] = {0, 0, 0, 0};
-    HIP_TRY(copy_to_host(ctx, c, d_counters.p, 32));
+    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+    HIP_TRY(copy_to_host(ctx, c, d_counters.p, 48));
+    ctx->timings.burial_rays = burial_rays;
+    ctx->timings.burial_node_visits = c[4];
+    ctx->timings.burial_tri_tests = c[5];
     ctx->timings.trace_node_visits = c[0];
     ctx->timings.trace_tri_tests = c[1];
     ctx->timings.bounce_node_visits = c[2];

@@ -113,9 +113,15 @@ hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const ui
                          const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
                          const uint64_t* tex_start, DevTexels out);
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta);
-hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
-                         uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept_count,
-                         uint32_t block);
+// The burial test streamed (burial rays, the closest-hit stream, the tally): rays k0..k0+nk-1
+// (k0 + nk <= 8) of max_n texels, tx index first + j or list[j] (then *count of them).
+hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t first, const uint32_t* list,
+                              const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, uint64_t seed,
+                              TrigTables trig, float4* rays, float4* hits, uint64_t* total);
+hipError_t launch_burial_decide(hipStream_t s, DevScene scene, DevTexels tx, uint64_t first, const uint32_t* list,
+                                const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, const float4* rays,
+                                const float4* hits, uint8_t* back_count, uint8_t* buried, unsigned long long* kept, uint32_t* next,
+                                unsigned long long* next_n);
 hipError_t launch_compact(hipStream_t s, DevTexels in, uint64_t ntex, const uint8_t* buried, const uint64_t* start,
                           unsigned long long* cursor, DevTexels out);
 struct TraceParams {

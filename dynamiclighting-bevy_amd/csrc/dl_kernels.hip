@@ -176,6 +176,83 @@ __device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t ma
   return next;
 }
 
+// Closest-hit order: pushes the hit inner children of a node except the nearest, farthest first
+// (so that pops come nearest first), and returns the nearest (-1 if none).
+__device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint32_t mask, const int4& ch, const float tn[4],
+                                                        uint32_t* __restrict__ stack, uint32_t stride, uint32_t& sp) {
+  uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
+  const bool s16 = sc.stack16 != 0;
+  float key[4];
+  int32_t val[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int32_t c = sel4(ch, k);
+    const bool push = ((mask >> k) & 1u) && c > 0;
+    val[k] = push ? c : 0;  // 0: nothing to push (the order is a heuristic; no child is dropped)
+    key[k] = push ? tn[k] : INFINITY;
+  }
+  // sorting network on (key, val), ascending
+#define DL_CSWAP(a, b)                                   \
+  if (key[b] < key[a]) {                                 \
+    const float tk = key[a];                             \
+    key[a] = key[b];                                     \
+    key[b] = tk;                                         \
+    const int32_t tv = val[a];                           \
+    val[a] = val[b];                                     \
+    val[b] = tv;                                         \
+  }
+  DL_CSWAP(0, 1) DL_CSWAP(2, 3) DL_CSWAP(0, 2) DL_CSWAP(1, 3) DL_CSWAP(1, 2)
+#undef DL_CSWAP
+#pragma unroll
+  for (int k = 3; k >= 1; --k) {
+    if (val[k] == 0) continue;
+    if (s16) stack16[sp * stride] = (uint16_t)val[k];
+    else stack[sp * stride] = (uint32_t)val[k];
+    sp++;
+  }
+  if (sp > sc.stack) atomicOr(sc.overflow, 1u);
+  return val[0] == 0 ? -1 : val[0];
+}
+
+// The triangles of the hit leaf children of a closest-hit node visit, nearest box first; a leaf
+// whose box starts beyond the best hit so far (the bound of node_hits with the hit's distance)
+// holds no hit at or below it and is skipped.  Keeps the lowest (t, occluder index) hit.
+__device__ __forceinline__ void closest_leaves(const DevScene& sc, uint32_t leaves, const int4& ch, const float tn[4], f3 o, f3 d,
+                                               float tmax, uint32_t alpha, bool& have, Hit& hit, Tally& tl) {
+  while (leaves) {
+    int kb = 0;
+    float tb = INFINITY;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (((leaves >> k) & 1u) && (!any || tn[k] < tb)) {
+        kb = k;
+        tb = tn[k];
+        any = true;
+      }
+    leaves &= ~(1u << kb);
+    if (have && tb > hit.t * (1.0f + kBoxEps)) continue;
+    const uint32_t v = (uint32_t)(-1 - sel4(ch, kb));
+    for (uint32_t i = v >> 2, end = (v >> 2) + (v & 3u) + 1u; i < end; ++i) {
+      const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+      float t, u, vv;
+      tl.tris++;
+      if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
+      if (!finite_(t)) continue;
+      if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
+      const uint32_t occ = __float_as_uint(a.w);
+      if (!have || t < hit.t || (t == hit.t && occ < hit.occ)) {
+        have = true;
+        hit.t = t;
+        hit.u = u;
+        hit.v = vv;
+        hit.occ = occ;
+        hit.pos = i;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ int32_t pop_node(const DevScene& sc, const uint32_t* __restrict__ stack, uint32_t stride, uint32_t sp) {
   const uint16_t* const stack16 = reinterpret_cast<const uint16_t*>(stack - threadIdx.x) + threadIdx.x;
   return sc.stack16 ? (int32_t)stack16[sp * stride] : (int32_t)stack[sp * stride];
@@ -200,6 +277,19 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
     int4 ch;
     const float limit = (ANY || !have ? tmax : bt) * (1.0f + kBoxEps);
     const uint32_t mask = node_hits(sc, node, sr, limit, tn, &ch);
+    if constexpr (!ANY) {
+      closest_leaves(sc, mask & leaf_bits(ch), ch, tn, o, d, tmax, alpha, have, *hit, tl);
+      if (have) bt = hit->t;
+      const int32_t next = push_children_sorted(sc, mask, ch, tn, stack, stride, sp);
+      if (next >= 0) {
+        node = next;
+        continue;
+      }
+      if (sp == 0) break;
+      sp--;
+      node = pop_node(sc, stack, stride, sp);
+      continue;
+    }
     // the triangles of every hit leaf child, one test per loop trip
     uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
     for (;;) {
@@ -415,41 +505,84 @@ __global__ void k_texels(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ 
 }
 
 // ---- burial: mesh_texels.rs:57-87, is_buried mesh_texels.rs:111-135 ----------------------------
-__global__ void k_burial(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, uint64_t ntex, uint64_t seed,
-                         TrigTables trig, uint8_t* buried, unsigned long long* kept) {
-  extern __shared__ uint32_t lds[];
-  uint32_t* stack = lds + threadIdx.x;
-  Tally tl;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntex; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t t = tx.tri[i];
-    f3 normal = ld3(tr.face + 3 * (size_t)t);
-    float4 p4 = tx.pos[i];
-    uint32_t xy = tx.xy[i];
-    f3 origin = add(mk3(p4.x, p4.y, p4.z), mul(normal, kRayOffset));
-    uint32_t m = tr.mesh[t];
-    Pcg rng = stream6(seed, ms.id[m], kBurialStream, tr.local[t], xy & 0xffffu, xy >> 16);
-    float turn = pcg_f32(rng);
+// is_buried casts up to eight closest-hit rays from a texel (offset along the face normal) over
+// the hemisphere of the face: ray k at sin^2(theta) = ((5k mod 8) + 0.5) / 8 and azimuth
+// (k + turn) * tau / 8, turn drawn once from the texel's burial stream.  A hit on a face that
+// faces along the ray is a back face; five back faces bury the texel, four front faces (or
+// misses) keep it, whichever comes first.
+// Here the rays are streamed: k_burial_rays writes rays k0 .. k0+nk-1 as closest-hit rays, the
+// closest-hit ray stream traces them, and k_burial_decide tallies them in ray order.  Ray q of
+// texel j of a pass of n texels sits at slot q * n + j, so that the stream's waves take the same
+// ray of neighbouring texels together (near-parallel rays from nearby origins).  The first
+// pass casts rays 0-3 of every texel, which decide every texel whose four rays all reach front
+// faces (the common case); the second casts rays 4-7 of the rest.  The verdict is the one the
+// early exits give: rays after the deciding one are cast but not counted.
+// Texel j of a pass is tx index first + j, or list[j] with a list (whose length is *count).
+__global__ void k_burial_rays(DevTris tr, DevMeshes ms, DevTexels tx, uint64_t first, const uint32_t* __restrict__ list,
+                              const unsigned long long* __restrict__ count, uint64_t n, uint32_t k0, uint32_t nk,
+                              uint64_t seed, TrigTables trig, float4* rays, float4* hits, uint64_t* total) {
+  if (count) n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = n * nk;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = list ? (uint64_t)list[j] : first + j;
+    const uint32_t t = tx.tri[i];
+    const f3 normal = ld3(tr.face + 3 * (size_t)t);
+    const float4 p4 = tx.pos[i];
+    const uint32_t xy = tx.xy[i];
+    const f3 origin = add(mk3(p4.x, p4.y, p4.z), mul(normal, kRayOffset));
+    Pcg rng = stream6(seed, ms.id[tr.mesh[t]], kBurialStream, tr.local[t], xy & 0xffffu, xy >> 16);
+    const float turn = pcg_f32(rng);
     f3 tangent, bitangent;
     orthonormal_pair(normal, &tangent, &bitangent);
-    uint32_t back = 0, front = 0;
-    bool is_buried = false;
-    for (uint32_t k = 0; k < 8; ++k) {
-      float ring = (float)((k * 5u) % 8u) + 0.5f;
-      float sin_theta = sqrtf(ring / 8.0f);
-      float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
-      float phi = (((float)k + turn) * kTau) / 8.0f;
+    for (uint32_t q = 0; q < nk; ++q) {
+      const uint32_t k = k0 + q;
+      const float ring = (float)((k * 5u) % 8u) + 0.5f;
+      const float sin_theta = sqrtf(ring / 8.0f);
+      const float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
+      const float phi = (((float)k + turn) * kTau) / 8.0f;
       float sp, cp;
       lsincosf(trig, phi, &sp, &cp);
-      f3 dir = add(mul(add(mul(tangent, cp), mul(bitangent, sp)), sin_theta), mul(normal, cos_theta));
-      Hit h;
-      bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
+      const f3 dir = add(mul(add(mul(tangent, cp), mul(bitangent, sp)), sin_theta), mul(normal, cos_theta));
+      const uint64_t r = q * n + j;
+      rays[2 * r] = make_float4(origin.x, origin.y, origin.z, INFINITY);
+      rays[2 * r + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(2u));
+      hits[r] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xffffffffu));
+    }
+  }
+}
+
+// Tallies rays k0 .. k0+nk-1 of each texel of a pass on the streamed hits: a hit whose triangle faces along the ray is a back face.  back[i] carries the back-face
+// count of the earlier rays (their front faces are the rest).  Decided texels get buried[i] and,
+// when kept, a count in kept[triangle]; undecided ones are appended to `next` (*next_n).
+__global__ void k_burial_decide(DevScene sc, DevTexels tx, uint64_t first, const uint32_t* __restrict__ list,
+                                const unsigned long long* __restrict__ count, uint64_t n, uint32_t k0, uint32_t nk,
+                                const float4* __restrict__ rays, const float4* __restrict__ hits, uint8_t* back_count,
+                                uint8_t* buried, unsigned long long* kept, uint32_t* next, unsigned long long* next_n) {
+  if (count) n = *count;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = list ? (uint64_t)list[j] : first + j;
+    uint32_t back = k0 ? back_count[i] : 0u, front = k0 - back;
+    int verdict = -1;  // 1 buried, 0 kept
+    for (uint32_t q = 0; q < nk && verdict < 0; ++q) {
+      const uint64_t r = q * n + j;
+      const uint32_t pos = __float_as_uint(hits[r].w);
+      bool bf = false;
+      if (pos != 0xffffffffu) {
+        const float4 d4 = rays[2 * r + 1];
+        bf = dot(tri_face_normal(sc, pos), mk3(d4.x, d4.y, d4.z)) > 0.0f;
+      }
       if (bf) back++;
       else front++;
-      if (back >= 5) { is_buried = true; break; }
-      if (front > 3) break;
+      if (back >= 5) verdict = 1;
+      else if (front > 3) verdict = 0;
     }
-    buried[i] = is_buried ? 1 : 0;
-    if (!is_buried) atomicAdd(&kept[t], 1ull);
+    if (verdict < 0) {
+      back_count[i] = (uint8_t)back;
+      next[atomicAdd(next_n, 1ull)] = (uint32_t)i;
+      continue;
+    }
+    buried[i] = (uint8_t)verdict;
+    if (!verdict) atomicAdd(&kept[tx.tri[i]], 1ull);
   }
 }
 
@@ -1451,11 +1584,21 @@ hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const ui
                      total, fill, counts, tex_start, out);
   return hipGetLastError();
 }
-hipError_t launch_burial(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
-                         uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept_count, uint32_t block) {
-  if (ntex == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_burial, dim3(grid_for(ntex, block)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
-                     tx, ntex, seed, trig, buried, kept_count);
+hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t first, const uint32_t* list,
+                              const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, uint64_t seed,
+                              TrigTables trig, float4* rays, float4* hits, uint64_t* total) {
+  if (max_n == 0 || nk == 0 || k0 + nk > 8) return max_n == 0 ? hipSuccess : hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_burial_rays, dim3(grid_for(max_n, kThreads)), dim3(kThreads), 0, s, tris, meshes, tx, first, list, count,
+                     max_n, k0, nk, seed, trig, rays, hits, total);
+  return hipGetLastError();
+}
+hipError_t launch_burial_decide(hipStream_t s, DevScene scene, DevTexels tx, uint64_t first, const uint32_t* list,
+                                const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, const float4* rays,
+                                const float4* hits, uint8_t* back_count, uint8_t* buried, unsigned long long* kept, uint32_t* next,
+                                unsigned long long* next_n) {
+  if (max_n == 0 || nk == 0 || k0 + nk > 8) return max_n == 0 ? hipSuccess : hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_burial_decide, dim3(grid_for(max_n, kThreads)), dim3(kThreads), 0, s, scene, tx, first, list, count, max_n,
+                     k0, nk, rays, hits, back_count, buried, kept, next, next_n);
   return hipGetLastError();
 }
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta) {
@@ -1968,33 +2111,8 @@ __device__ __forceinline__ uint32_t visit_closest(const DevScene& sc, uint32_t* 
   float tn[4];
   int4 ch;
   const uint32_t mask = node_hits(sc, node, sr, (have ? hit.t : tmax) * (1.0f + kBoxEps), tn, &ch);
-  uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
-  for (;;) {
-    if (i == end) {
-      if (!leaves) break;
-      const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
-      leaves &= leaves - 1;
-      i = v >> 2;
-      end = i + (v & 3u) + 1u;
-    }
-    const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
-    const uint32_t pos = i++;
-    float t, u, vv;
-    tl.tris++;
-    if (!mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv)) continue;
-    if (!finite_(t)) continue;
-    if (alpha && !alpha_blocks(sc, __float_as_uint(a.w), u, vv)) continue;
-    const uint32_t occ = __float_as_uint(a.w);
-    if (!have || t < hit.t || (t == hit.t && occ < hit.occ)) {
-      have = true;
-      hit.t = t;
-      hit.u = u;
-      hit.v = vv;
-      hit.occ = occ;
-      hit.pos = pos;
-    }
-  }
-  const int32_t next = push_children(sc, mask, ch, tn, stack, stride, sp);
+  closest_leaves(sc, mask & leaf_bits(ch), ch, tn, o, d, tmax, alpha, have, hit, tl);
+  const int32_t next = push_children_sorted(sc, mask, ch, tn, stack, stride, sp);
   if (next >= 0) {
     node = next;
     return 0u;

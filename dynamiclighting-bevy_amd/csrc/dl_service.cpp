@@ -22,7 +22,10 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <unordered_set>
 
@@ -132,6 +135,23 @@ struct Entry {
   uint32_t tri, slot, pair, k;
 };
 
+// Phase times of an edit on stderr when DL_SERVICE_TIMING is set (a profiling aid).
+struct PhaseClock {
+  const bool on = std::getenv("DL_SERVICE_TIMING") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  std::string line;
+  void mark(const char* name, hipStream_t s = nullptr) {
+    if (!on) return;
+    if (s) (void)hipStreamSynchronize(s);
+    const auto now = std::chrono::steady_clock::now();
+    line += std::string(" ") + name + "=" + std::to_string(std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+  ~PhaseClock() {
+    if (on) std::fprintf(stderr, "[assemble]%s\n", line.c_str());
+  }
+};
+
 // Rewrites the live scene after its pairs changed: `dirty` meshes are padded and cropped again;
 // every mesh gets its words in the new slot order.  Replaces svc->W and svc->G.
 int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
@@ -140,6 +160,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   const dl_bake_options& opt = svc->options;
   const Resident& res = svc->res;
   const uint32_t nm = (uint32_t)svc->mesh.size();
+  PhaseClock clock;
   const Slots sl = slots_of(svc->lights);
   std::unordered_map<uint64_t, uint32_t> slot;
   bool bounce = false;
@@ -213,6 +234,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     }
     p.len = w;
   });
+  clock.mark("plan");
   std::vector<uint64_t> base(nm + 1, 0);
   for (uint32_t i = 0; i < nm; ++i) base[i + 1] = base[i] + plan[i].len;
   const uint64_t total = base[nm];
@@ -250,7 +272,9 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     words.by_src[d_staged.p] = words.by_src[nullptr];
     words.by_src.erase(nullptr);
   }
+  clock.mark("word_ranges");
   HIP_TRY(words.run(ctx, W2->p));
+  clock.mark("word_copies", s);
 
   // ---- the new level store: every pair's levels and visited bits, contiguous ---------------
   uint64_t gtotal = 0;
@@ -272,6 +296,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       }
     HIP_TRY(levels.run(ctx, G2->p));
   }
+  clock.mark("levels", s);
 
   // ---- dirty meshes: visited from their pairs, seam padding, crop and dilation -------------
   {
@@ -377,6 +402,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     }
   }
   HIP_TRY(hipStreamSynchronize(s));
+  clock.mark("dirty");
 
   // ---- the new state ------------------------------------------------------------------------
   size_t gi = 0;
@@ -426,6 +452,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   }
   st.triangle_words = total;
   svc->stats = st;
+  clock.mark("state");
   return DL_OK;
 }
 

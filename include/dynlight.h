@@ -428,6 +428,10 @@ typedef struct dl_timings {
     uint64_t gather_jobs;       /* traced (texel, light) gather jobs (counting on)            */
     uint64_t gather_samples;    /* samples those jobs set up (counting on)                    */
     uint64_t gather_traced;     /* samples with a ray to trace (counting on)                  */
+    double burial_ms;           /* device time of the burial test (rays, traversal, tally)     */
+    uint64_t burial_rays;       /* closest-hit rays cast by the burial test (counting on)      */
+    uint64_t burial_node_visits;/* BVH nodes fetched by those rays (counting on)               */
+    uint64_t burial_tri_tests;  /* triangles tested by those rays (counting on)                */
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 
