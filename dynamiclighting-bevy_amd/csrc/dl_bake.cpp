@@ -56,6 +56,19 @@ hipError_t burial_streamed(dl_ctx* ctx, DevScene sc, const DevTris& tr, const De
   if (ntex == 0) return hipSuccess;
   if (ntex >= (1ull << 32)) return hipErrorInvalidValue;  // texels are listed by 32-bit index
   hipStream_t s = ctx->stream;
+  if (ctx->burial_lanes) {  // one lane per texel instead (counters: node fetches and triangle tests only)
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess) (void)hipEventRecord(ev[0], s);
+    sc.counters = counters ? counters + 4 : nullptr;
+    hipError_t e = launch_burial_lanes(s, sc, tr, ms, tx, ntex, seed, ctx->trig, buried, kept, ctx->burial_lanes);
+    float t = 0.0f;
+    if (e == hipSuccess && ev[1] && hipEventRecord(ev[1], s) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&t, ev[0], ev[1]) == hipSuccess)
+      *ms_spent += t;
+    for (hipEvent_t x : ev)
+      if (x) (void)hipEventDestroy(x);
+    return e;
+  }
   const uint64_t chunk = std::min<uint64_t>(ntex, std::max<uint64_t>(1ull << 20, ctx->gather.chunk_rays / 8));
   GatherLaunch gl = ctx->gather;
   gl.order = 0;  // slots go out in storage order

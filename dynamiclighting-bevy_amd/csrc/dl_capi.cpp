@@ -120,6 +120,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
     w->trace_block = ctx->trace_block;
     w->burial_block = ctx->burial_block;
+    w->burial_lanes = ctx->burial_lanes;
     w->gather = ctx->gather;
     w->gather.share = streams;
     // smaller ray chunks, so that each worker's gather buffers stay within its cache
@@ -740,6 +741,15 @@ int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (trace_block) ctx->trace_block = trace_block;
   if (burial_block) ctx->burial_block = burial_block;
+  return DL_OK;
+}
+
+int dl_set_burial(dl_ctx* ctx, uint32_t lanes_block) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  if (lanes_block != 0 && (lanes_block % 64 != 0 || lanes_block > 1024))
+    return set_err(ctx, DL_E_ARGUMENT, "block sizes must be multiples of 64 in [64, 1024]");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->burial_lanes = lanes_block;
   return DL_OK;
 }
 

@@ -161,6 +161,7 @@ struct dl_ctx {
   size_t scan_tmp_bytes = 0;
   uint32_t trace_block = 128;
   uint32_t burial_block = 0;  // threads per block of the burial ray stream (0: the gather's)
+  uint32_t burial_lanes = 0;  // nonzero: the burial test with one lane per texel, this many threads per block
   dl::GatherLaunch gather{};
   uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
   uint32_t trace_stream = 1;  // 1: area lights, soft suns and skies through the streamed any-hit path

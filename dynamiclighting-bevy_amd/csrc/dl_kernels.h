@@ -113,6 +113,9 @@ hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const ui
                          const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
                          const uint64_t* tex_start, DevTexels out);
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta);
+// The burial test with one lane per texel (block threads per block).
+hipError_t launch_burial_lanes(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
+                               uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept, uint32_t block);
 // The burial test streamed (burial rays, the closest-hit stream, the tally): rays k0..k0+nk-1
 // (k0 + nk <= 8) of max_n texels, tx index first + j or list[j] (then *count of them).
 hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t first, const uint32_t* list,

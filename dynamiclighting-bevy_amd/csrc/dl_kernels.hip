@@ -141,6 +141,13 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
   return mask;
 }
 
+// Sum of v over the wave, added to *dst by its first lane (all 64 lanes must call it).
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
 // Bit k set when child k is a leaf (a negative child reference).
 __device__ __forceinline__ uint32_t leaf_bits(const int4& ch) {
   return (ch.x < 0 ? 1u : 0u) | (ch.y < 0 ? 2u : 0u) | (ch.z < 0 ? 4u : 0u) | (ch.w < 0 ? 8u : 0u);
@@ -517,6 +524,57 @@ __global__ void k_texels(DevTris tr, DevMeshes ms, const uint64_t* __restrict__ 
 // pass casts rays 0-3 of every texel, which decide every texel whose four rays all reach front
 // faces (the common case); the second casts rays 4-7 of the rest.  The verdict is the one the
 // early exits give: rays after the deciding one are cast but not counted.
+// Direction of burial ray k of a texel.
+__device__ __forceinline__ f3 burial_dir(const TrigTables& trig, f3 normal, f3 tangent, f3 bitangent, float turn, uint32_t k) {
+  const float ring = (float)((k * 5u) % 8u) + 0.5f;
+  const float sin_theta = sqrtf(ring / 8.0f);
+  const float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
+  const float phi = (((float)k + turn) * kTau) / 8.0f;
+  float sp, cp;
+  lsincosf(trig, phi, &sp, &cp);
+  return add(mul(add(mul(tangent, cp), mul(bitangent, sp)), sin_theta), mul(normal, cos_theta));
+}
+
+// The burial test with one lane per texel casting its rays in order (the early exits cut the
+// rays cast; lanes of a wave end at different rays).
+__global__ void k_burial_lanes(DevScene sc, DevTris tr, DevMeshes ms, DevTexels tx, uint64_t ntex, uint64_t seed,
+                               TrigTables trig, uint8_t* buried, unsigned long long* kept) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntex; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = tx.tri[i];
+    const f3 normal = ld3(tr.face + 3 * (size_t)t);
+    const float4 p4 = tx.pos[i];
+    const uint32_t xy = tx.xy[i];
+    const f3 origin = add(mk3(p4.x, p4.y, p4.z), mul(normal, kRayOffset));
+    Pcg rng = stream6(seed, ms.id[tr.mesh[t]], kBurialStream, tr.local[t], xy & 0xffffu, xy >> 16);
+    const float turn = pcg_f32(rng);
+    f3 tangent, bitangent;
+    orthonormal_pair(normal, &tangent, &bitangent);
+    uint32_t back = 0, front = 0;
+    bool is_buried = false;
+    for (uint32_t k = 0; k < 8; ++k) {
+      const f3 dir = burial_dir(trig, normal, tangent, bitangent, turn, k);
+      Hit h;
+      const bool bf = traverse<false>(sc, stack, blockDim.x, origin, dir, INFINITY, &h, tl) && dot(tri_face_normal(sc, h.pos), dir) > 0.0f;
+      if (bf) back++;
+      else front++;
+      if (back >= 5) {
+        is_buried = true;
+        break;
+      }
+      if (front > 3) break;
+    }
+    buried[i] = is_buried ? 1 : 0;
+    if (!is_buried) atomicAdd(&kept[t], 1ull);
+  }
+  if (sc.counters) {
+    wave_add(&sc.counters[0], tl.nodes);
+    wave_add(&sc.counters[1], tl.tris);
+  }
+}
+
 // Texel j of a pass is tx index first + j, or list[j] with a list (whose length is *count).
 __global__ void k_burial_rays(DevTris tr, DevMeshes ms, DevTexels tx, uint64_t first, const uint32_t* __restrict__ list,
                               const unsigned long long* __restrict__ count, uint64_t n, uint32_t k0, uint32_t nk,
@@ -535,14 +593,7 @@ __global__ void k_burial_rays(DevTris tr, DevMeshes ms, DevTexels tx, uint64_t f
     f3 tangent, bitangent;
     orthonormal_pair(normal, &tangent, &bitangent);
     for (uint32_t q = 0; q < nk; ++q) {
-      const uint32_t k = k0 + q;
-      const float ring = (float)((k * 5u) % 8u) + 0.5f;
-      const float sin_theta = sqrtf(ring / 8.0f);
-      const float cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
-      const float phi = (((float)k + turn) * kTau) / 8.0f;
-      float sp, cp;
-      lsincosf(trig, phi, &sp, &cp);
-      const f3 dir = add(mul(add(mul(tangent, cp), mul(bitangent, sp)), sin_theta), mul(normal, cos_theta));
+      const f3 dir = burial_dir(trig, normal, tangent, bitangent, turn, k0 + q);
       const uint64_t r = q * n + j;
       rays[2 * r] = make_float4(origin.x, origin.y, origin.z, INFINITY);
       rays[2 * r + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(2u));
@@ -738,12 +789,6 @@ __device__ bool sun_visibility(const DevScene& sc, uint32_t* stack, uint32_t str
   }
   *rays = nr;
   return share_result(s, fraction);
-}
-
-__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
 __device__ __forceinline__ void raise_level(uint32_t* pool, uint64_t off, uint32_t bits, uint64_t i, uint32_t level) {
@@ -1582,6 +1627,13 @@ hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const ui
   if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_texels, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, tris, meshes, trace_count, cand_start,
                      total, fill, counts, tex_start, out);
+  return hipGetLastError();
+}
+hipError_t launch_burial_lanes(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
+                               uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept, uint32_t block) {
+  if (ntex == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_burial_lanes, dim3(grid_for(ntex, block)), dim3(block), stack_bytes(scene, block), s, scene, tris, meshes,
+                     tx, ntex, seed, trig, buried, kept);
   return hipGetLastError();
 }
 hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t first, const uint32_t* list,

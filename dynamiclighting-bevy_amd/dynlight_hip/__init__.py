@@ -132,6 +132,8 @@ def load() -> C.CDLL:
     lib.dl_test_trig_keys.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.c_uint64]
     lib.dl_test_trig_table_sizes.restype = None
     lib.dl_test_trig_table_sizes.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.dl_set_burial.restype = C.c_int
+    lib.dl_set_burial.argtypes = [P, C.c_uint32]
     lib.dl_set_counting.restype = C.c_int
     lib.dl_set_counting.argtypes = [P, C.c_int]
     lib.dl_set_bounce_capture.restype = C.c_int
@@ -417,6 +419,13 @@ class Context:
         finally:
             self.lib.dl_set_bounce_capture(self.handle, 0)
         return scene, out
+
+    def set_burial(self, lanes_block: int = 0) -> None:
+        """Burial test form: 0 streams its rays (closest-hit ray stream); otherwise one lane per
+        texel, lanes_block threads per block."""
+        st = self.lib.dl_set_burial(self.handle, lanes_block)
+        if st != 0:
+            raise BakeError(st, self.error())
 
     def set_launch(self, trace_block: int = 0, burial_block: int = 0) -> None:
         st = self.lib.dl_set_launch(self.handle, trace_block, burial_block)

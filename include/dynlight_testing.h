@@ -62,6 +62,9 @@ uint64_t dl_test_bounce_capture(dl_ctx* ctx, uint64_t index, uint64_t meta[5], c
 /* Launch shape of the traversal kernels (0 keeps the current value): threads per block of the
  * shadow-ray kernel and of the burial test's closest-hit ray stream. */
 int dl_set_launch(dl_ctx* ctx, uint32_t trace_block, uint32_t burial_block);
+/* The burial test's form: 0 (default) streams its rays through the closest-hit ray stream;
+ * otherwise one lane per texel casts its rays in order, lanes_block threads per block. */
+int dl_set_burial(dl_ctx* ctx, uint32_t lanes_block);
 /* Per job of the context's last dl_bake_batch (up to cap jobs): its dl_timings, the worker that
  * baked it, and its start and end in ms from the start of the batch (two doubles per job).  Any
  * output may be null.  Returns the job count of that batch. */
