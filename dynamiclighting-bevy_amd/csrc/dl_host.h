@@ -331,9 +331,12 @@ struct LivePair {
   uint64_t grid_off = 0, grid_words = 0, vis_off = 0, vis_words = 0;
   uint64_t rays = 0, samples = 0, bounce_rays = 0, entries = 0;
   std::vector<uint32_t> kept;          // mesh-local triangles, ascending
-  std::vector<uint64_t> shadow_off;    // per kept triangle: its shadow payload in the live words
-  std::vector<uint64_t> bounce_off, bounce_words;  // per kept triangle: its bounce payload (0 words: none)
-  std::shared_ptr<DevBlock> src_block, src_words;  // set while the pair's data is still in its capture
+  std::vector<uint64_t> shadow_off;    // per kept triangle: its shadow payload, from its mesh's first word
+  // per kept triangle: its bounce payload (0 words: none), from its mesh's first word, or in
+  // src_words while that is set (the words of the bake that captured the pair)
+  std::vector<uint64_t> bounce_off, bounce_words;
+  std::shared_ptr<DevBlock> levels;     // holds the pair's unpadded levels and visited bits
+  std::shared_ptr<DevBlock> src_words;
 };
 struct LiveMesh {
   std::vector<LivePair> pairs;         // sorted by light id
@@ -351,7 +354,7 @@ struct dl_service {
   std::vector<dl_light> lights;                 // the live light set, sorted by id
   std::vector<dl::LiveMesh> mesh;               // resident mesh order
   std::vector<uint32_t> crops;                  // 4 per receiver triangle
-  std::unique_ptr<dl::DevBlock> W, G;           // live words; level store (levels, then visited bits)
+  std::unique_ptr<dl::DevBlock> W;              // live words
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
   std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words
   std::unique_ptr<dl_baked_scene> base;         // settings, sun maps and sky probes of the first bake

@@ -148,6 +148,16 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
   if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
+// base[key] += v over the active lanes of a wave: one atomic when they share a key, one per lane
+// otherwise (all 64 lanes must call it).
+__device__ __forceinline__ void wave_add_at(unsigned long long* base, uint32_t key, uint32_t v, bool active) {
+  const uint64_t act = __ballot(active);
+  if (!act) return;
+  const uint32_t k0 = __shfl(key, (int)__builtin_ctzll(act), 64);
+  if (__ballot(active && key != k0) == 0ull) wave_add(&base[k0], active ? v : 0u);
+  else if (active && v) atomicAdd(&base[key], (unsigned long long)v);
+}
+
 // Bit k set when child k is a leaf (a negative child reference).
 __device__ __forceinline__ uint32_t leaf_bits(const int4& ch) {
   return (ch.x < 0 ? 1u : 0u) | (ch.y < 0 ? 2u : 0u) | (ch.z < 0 ? 4u : 0u) | (ch.w < 0 ? 8u : 0u);
@@ -1192,7 +1202,7 @@ __global__ void k_vis_finish(DevTris tr, DevMeshes ms, const DevLight* __restric
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
     const uint64_t i = i0 + lane;
-    uint32_t m = 0xffffffffu, rays_n = 0, some_n = 0;
+    uint32_t m = 0xffffffffu, rays_n = 0, some_n = 0, rr = 0;
     if (i < n) {
       const VisJob v = vis_job(js, me, nme, ent, tex_start, q0 + i);
       const DevLight& L = lights[ent.light[v.e]];
@@ -1236,11 +1246,11 @@ __global__ void k_vis_finish(DevTris tr, DevMeshes ms, const DevLight* __restric
         raise_level(rg.pool, rg.off[r], rg.bits[r], ri, level_of(fraction, L.bits));
         if (rg.vis) atomicOr(&rg.vis[rg.vis_off[r] + ri / 32], 1u << (ri % 32));
       }
-      if (rg.rays) {
-        const uint32_t rr = ent.region[v.e];
-        if (rays_n) atomicAdd(&rg.rays[rr], (unsigned long long)rays_n);
-        if (some_n) atomicAdd(&rg.samples[rr], 1ull);
-      }
+      if (rg.rays) rr = ent.region[v.e];
+    }
+    if (rg.rays) {
+      wave_add_at(rg.rays, rr, rays_n, i < n);
+      wave_add_at(rg.samples, rr, some_n, i < n);
     }
     const uint32_t m0 = __shfl(m, 0, 64);
     if (__ballot(i < n && m != m0) == 0ull) {
@@ -2422,19 +2432,24 @@ __global__ void k_gather_jobs(DevTris tr, const DevLight* __restrict__ lights, D
 // which samples cast a ray) but nothing is traced.
 __global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
                                DevBounce b, const uint2* __restrict__ jobs, uint64_t njobs, BounceParams p, TrigTables trig) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < njobs; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint2 jb = jobs[i];
-    Pcg rng;
-    uint32_t m;
-    GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
-    uint32_t rays = 0;
-    for (uint32_t k = 0; k < g.samples; ++k) {
-      f3 d, c;
-      float dist;
-      rays += gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c) ? 1u : 0u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < njobs; i0 += stride) {
+    const uint64_t i = i0 + lane;
+    uint32_t m = 0, rays = 0, win = 0;
+    if (i < njobs) {
+      const uint2 jb = jobs[i];
+      Pcg rng;
+      GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
+      for (uint32_t k = 0; k < g.samples; ++k) {
+        f3 d, c;
+        float dist;
+        rays += gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c) ? 1u : 0u;
+      }
+      win = b.ent_window[jb.y];
     }
-    atomicAdd(&b.rays[m], (unsigned long long)rays);
-    if (b.wrays && rays) atomicAdd(&b.wrays[b.ent_window[jb.y]], (unsigned long long)rays);
+    wave_add_at(b.rays, m, rays, i < njobs);
+    if (b.wrays) wave_add_at(b.wrays, win, rays, i < njobs);
   }
 }
 
@@ -2517,13 +2532,8 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
       for (uint32_t w = g.samples >> 5; w < words; ++w) tmask[i * words + w] = w == (g.samples >> 5) ? word : 0u;
       tcount[i] = traced;
     }
-    const uint32_t m0 = __shfl(m, 0, 64);
-    if (__ballot(i < n && m != m0) == 0ull) {
-      wave_add(&b.rays[m0], nr);
-    } else if (i < n && nr) {
-      atomicAdd(&b.rays[m], (unsigned long long)nr);
-    }
-    if (b.wrays && i < n && nr) atomicAdd(&b.wrays[win], (unsigned long long)nr);
+    wave_add_at(b.rays, m, nr, i < n);
+    if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
   }
 }
 
@@ -2844,13 +2854,8 @@ __global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict
         mark_tile(b, w, (uint64_t)((xy & 0xffffu) - wr[0]), (uint64_t)((xy >> 16) - wr[1]), v);
       }
     }
-    const uint32_t m0 = __shfl(m, 0, 64);
-    if (__ballot(i < n && m != m0) == 0ull) {
-      wave_add(&b.rays[m0], nr);
-    } else if (i < n && nr) {
-      atomicAdd(&b.rays[m], (unsigned long long)nr);
-    }
-    if (b.wrays && i < n && nr) atomicAdd(&b.wrays[win], (unsigned long long)nr);
+    wave_add_at(b.rays, m, nr, i < n);
+    if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
   }
 }
 

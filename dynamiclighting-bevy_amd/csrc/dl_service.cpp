@@ -153,7 +153,10 @@ struct PhaseClock {
 };
 
 // Rewrites the live scene after its pairs changed: `dirty` meshes are padded and cropped again;
-// every mesh gets its words in the new slot order.  Replaces svc->W and svc->G.
+// every mesh gets its words in the new slot order (a mesh whose slots and pairs are unchanged is
+// one copy).  The work on the host and the device grows with the meshes that change, not with
+// the scene: pair levels stay in the blocks they were captured in, payload offsets are kept
+// relative to their mesh, and the stats are kept up to date by merge.  Replaces svc->W.
 int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   dl_ctx* ctx = svc->ctx;
   hipStream_t s = ctx->stream;
@@ -173,7 +176,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   const uint32_t stride = bounce ? 3u : 2u;
   const bool restride = stride != svc->stride;
 
-  // ---- per mesh, on the host: entry order, word layout, light data, copies -----------------
+  // ---- per mesh, on the host: entry order, word layout, light data ------------------------
   struct Plan {
     bool whole = false;            // words unchanged: one copy of the mesh
     std::vector<uint32_t> ld;      // headers and light data (the mesh's first words)
@@ -182,17 +185,23 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     std::vector<uint64_t> sh, bo;  // per entry: shadow and bounce payload offset in the mesh
   };
   std::vector<Plan> plan(nm);
+  std::atomic<uint32_t> restaged{0};
   run_parallel(nm, ctx->host_threads, [&](uint64_t mi) {
     const LiveMesh& m = svc->mesh[mi];
     Plan& p = plan[mi];
     bool same = !dirty[mi] && !restride && svc->W;
-    for (const LivePair& q : m.pairs) same = same && svc->slot.count(q.light_id) && svc->slot.at(q.light_id) == slot.at(q.light_id);
+    for (const LivePair& q : m.pairs) {
+      if (!same) break;
+      auto it = svc->slot.find(q.light_id);
+      same = it != svc->slot.end() && it->second == slot.at(q.light_id);
+    }
     const uint32_t ft = res.first_tri[mi], nt = res.first_tri[mi + 1] - ft;
     if (same) {
       p.whole = true;
       p.len = m.w_len;
       return;
     }
+    if (!dirty[mi]) restaged++;
     for (uint32_t pi = 0; pi < m.pairs.size(); ++pi) {
       const LivePair& q = m.pairs[pi];
       const uint32_t sl_q = slot.at(q.light_id);
@@ -258,11 +267,11 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       const LivePair& q = m.pairs[e.pair];
       if (!dirty[i]) {
         const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
-        words.add(svc->W->p, q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits));
+        words.add(svc->W->p, m.w_off + q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits));
       }
       if (q.bounce_words[e.k]) {
-        const uint32_t* src = q.src_words ? q.src_words->p : svc->W->p;
-        words.add(src, q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
+        if (q.src_words) words.add(q.src_words->p, q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
+        else words.add(svc->W->p, m.w_off + q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
       }
     }
   }
@@ -276,29 +285,9 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   HIP_TRY(words.run(ctx, W2->p));
   clock.mark("word_copies", s);
 
-  // ---- the new level store: every pair's levels and visited bits, contiguous ---------------
-  uint64_t gtotal = 0;
-  for (const LiveMesh& m : svc->mesh)
-    for (const LivePair& q : m.pairs) gtotal += q.grid_words + q.vis_words;
-  std::unique_ptr<DevBlock> G2(new DevBlock());
-  HIP_TRY(G2->alloc(gtotal + 1, s));
-  std::vector<uint64_t> g_new;  // new grid offset of every pair, mesh by mesh
-  {
-    CopyPlan levels;
-    uint64_t at = 0;
-    for (const LiveMesh& m : svc->mesh)
-      for (const LivePair& q : m.pairs) {
-        const uint32_t* src = q.src_block ? q.src_block->p : svc->G->p;
-        levels.add(src, q.grid_off, at, q.grid_words);
-        levels.add(src, q.vis_off, at + q.grid_words, q.vis_words);
-        g_new.push_back(at);
-        at += q.grid_words + q.vis_words;
-      }
-    HIP_TRY(levels.run(ctx, G2->p));
-  }
-  clock.mark("levels", s);
-
   // ---- dirty meshes: visited from their pairs, seam padding, crop and dilation -------------
+  // The levels of the dirty meshes' pairs are gathered into a work buffer (padding changes them
+  // in place; the pairs keep their unpadded levels).
   {
     std::vector<uint32_t> dm;
     for (uint32_t i = 0; i < nm; ++i)
@@ -308,12 +297,8 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     std::vector<uint32_t> kc_crop;  // 4 per kept entry
     std::vector<uint32_t> k_region;
     std::vector<uint64_t> k_start(1, 0), k_out;
-    size_t gi = 0;
-    std::vector<size_t> pair_g(nm + 1, 0);  // first g_new index of each mesh
-    for (uint32_t i = 0; i < nm; ++i) {
-      pair_g[i] = gi;
-      gi += svc->mesh[i].pairs.size();
-    }
+    CopyPlan levels;
+    uint64_t at = 0;
     for (uint32_t d = 0; d < dm.size(); ++d) {
       const uint32_t i = dm[d];
       const LiveMesh& m = svc->mesh[i];
@@ -326,9 +311,11 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
         r_rect.insert(r_rect.end(), q.rect, q.rect + 4);
         r_bits.push_back(q.bits);
         r_mesh.push_back(d);
-        const uint64_t at = g_new[pair_g[i] + pi];
-        r_off.push_back(at);  // levels are padded in a copy of the new store, below
+        levels.add(q.levels->p, q.grid_off, at, q.grid_words);
+        levels.add(q.levels->p, q.vis_off, at + q.grid_words, q.vis_words);
+        r_off.push_back(at);
         r_vis.push_back(at + q.grid_words);
+        at += q.grid_words + q.vis_words;
         vis_start.push_back(vis_start.back() + q.vis_words);
         if (!q.kept.empty()) {
           used.push_back(r0 + (uint32_t)pi);
@@ -348,10 +335,9 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     const uint32_t nreg = (uint32_t)r_bits.size();
     const uint64_t nkept = k_out.size();
     if (nreg) {
-      // seam padding changes levels in place: pad a copy of the store, keep the store unpadded
       std::unique_ptr<DevBlock> work(new DevBlock());
-      HIP_TRY(work->alloc(gtotal + 1, s));
-      HIP_TRY(hipMemcpyAsync(work->p, G2->p, gtotal * 4, hipMemcpyDeviceToDevice, s));
+      HIP_TRY(work->alloc(at + 1, s));
+      HIP_TRY(levels.run(ctx, work->p));
       DBuf<uint32_t> d_rect, d_bits, d_mesh, d_used, d_visited, d_kcrop, d_kregion;
       DBuf<uint64_t> d_off, d_vis, d_vstart, d_uarea, d_mvoff, d_kstart, d_kout, d_kid;
       DBuf<uint32_t> d_mres32;
@@ -405,63 +391,55 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   clock.mark("dirty");
 
   // ---- the new state ------------------------------------------------------------------------
-  size_t gi = 0;
   for (uint32_t i = 0; i < nm; ++i) {
     LiveMesh& m = svc->mesh[i];
     const Plan& p = plan[i];
-    if (p.whole) {
-      for (LivePair& q : m.pairs) {
-        for (auto& o : q.shadow_off) o = o - m.w_off + base[i];
-        for (size_t k = 0; k < q.bounce_off.size(); ++k)
-          if (q.bounce_words[k]) q.bounce_off[k] = q.bounce_off[k] - m.w_off + base[i];
-      }
-    } else {
-      for (LivePair& q : m.pairs) q.shadow_off.assign(q.kept.size(), 0);
+    if (!p.whole) {
       for (size_t j = 0; j < p.ent.size(); ++j) {
         LivePair& q = m.pairs[p.ent[j].pair];
-        q.shadow_off[p.ent[j].k] = base[i] + p.sh[j];
-        if (q.bounce_words[p.ent[j].k]) q.bounce_off[p.ent[j].k] = base[i] + p.bo[j];
+        q.shadow_off[p.ent[j].k] = p.sh[j];
+        if (q.bounce_words[p.ent[j].k]) q.bounce_off[p.ent[j].k] = p.bo[j];
       }
-    }
-    for (LivePair& q : m.pairs) {
-      q.grid_off = g_new[gi++];
-      q.vis_off = q.grid_off + q.grid_words;
-      q.src_block.reset();
-      q.src_words.reset();
+      for (LivePair& q : m.pairs) q.src_words.reset();
     }
     m.w_off = base[i];
     m.w_len = p.len;
   }
   svc->W = std::move(W2);
-  svc->G = std::move(G2);
   svc->slot = std::move(slot);
   svc->baked = std::move(baked);
   svc->bvh = sl.bvh;
   svc->stride = stride;
-  dl_bake_stats st{};
-  st.meshes = nm;
-  st.triangles = res.ntri;
-  for (const LiveMesh& m : svc->mesh) {
-    st.texels += m.texels;
-    for (const LivePair& q : m.pairs) {
-      st.rays += q.rays;
-      st.samples += q.samples;
-      st.bounce_rays += q.bounce_rays;
-      st.lights_removed += q.entries - q.kept.size();
-    }
-  }
-  st.triangle_words = total;
-  svc->stats = st;
+  svc->stats.triangle_words = total;
   clock.mark("state");
+  if (clock.on) clock.line += " restaged_meshes=" + std::to_string(restaged.load());
   return DL_OK;
 }
 
+// Adds (sign +1) or removes (-1) a pair's share of the live stats.
+void count_pair(dl_bake_stats& st, const LivePair& q, int sign) {
+  const uint64_t removed = q.entries - q.kept.size();
+  if (sign > 0) {
+    st.rays += q.rays;
+    st.samples += q.samples;
+    st.bounce_rays += q.bounce_rays;
+    st.lights_removed += removed;
+  } else {
+    st.rays -= q.rays;
+    st.samples -= q.samples;
+    st.bounce_rays -= q.bounce_rays;
+    st.lights_removed -= removed;
+  }
+}
+
 // The captured pairs of a bake of `lights` replace the live pairs of those lights (and of
-// `removed`); marks the meshes whose pairs changed.
+// `removed`); marks the meshes whose pairs changed and keeps the stats up to date.
 void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& replaced, std::vector<uint8_t>& dirty) {
   for (uint32_t i = 0; i < svc->mesh.size(); ++i) {
     auto& pairs = svc->mesh[i].pairs;
     const size_t before = pairs.size();
+    for (const LivePair& q : pairs)
+      if (replaced.count(q.light_id)) count_pair(svc->stats, q, -1);
     pairs.erase(std::remove_if(pairs.begin(), pairs.end(), [&](const LivePair& q) { return replaced.count(q.light_id) > 0; }),
                 pairs.end());
     if (pairs.size() != before) dirty[i] = 1;
@@ -483,8 +461,9 @@ void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& re
     q.bounce_off = std::move(c.bounce_off);
     q.bounce_words = std::move(c.bounce_words);
     q.shadow_off.assign(q.kept.size(), 0);
-    q.src_block = c.block;
+    q.levels = c.block;
     q.src_words = c.words;
+    count_pair(svc->stats, q, +1);
     auto& pairs = svc->mesh[c.mesh].pairs;
     pairs.insert(std::upper_bound(pairs.begin(), pairs.end(), q.light_id,
                                   [](uint64_t id, const LivePair& p) { return id < p.light_id; }),
@@ -549,9 +528,12 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   for (size_t i = 0; i < svc->mesh.size(); ++i) svc->mesh[i].texels = cap.texels[i];
   svc->crops = cap.crops;
   svc->W.reset();
-  svc->G.reset();
   svc->slot.clear();
   svc->stride = 0;  // forces every mesh to be written
+  svc->stats = dl_bake_stats{};
+  svc->stats.meshes = (uint32_t)svc->mesh.size();
+  svc->stats.triangles = svc->res.ntri;
+  for (const LiveMesh& m : svc->mesh) svc->stats.texels += m.texels;
   std::vector<uint8_t> dirty(svc->mesh.size(), 1);
   merge(svc, cap, {}, dirty);
   svc->base.reset(new dl_baked_scene());
