@@ -357,6 +357,7 @@ struct dl_service {
   std::unique_ptr<dl::DevBlock> W;              // live words
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
   std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words
+  std::unordered_map<uint64_t, std::vector<uint32_t>> light_meshes;  // light id -> meshes holding its pairs
   std::unique_ptr<dl_baked_scene> base;         // settings, sun maps and sky probes of the first bake
   std::vector<dl_baked_light> baked;            // the live scene's lights in slot order
   std::vector<dl_light_bvh_node> bvh;

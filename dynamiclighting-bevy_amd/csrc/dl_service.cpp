@@ -114,7 +114,15 @@ struct Ranges {
 // Runs word copies grouped by source buffer on the context's stream.
 struct CopyPlan {
   std::unordered_map<const uint32_t*, Ranges> by_src;
-  void add(const uint32_t* src, uint64_t s, uint64_t d, uint64_t len) { by_src[src].add(s, d, len); }
+  const uint32_t* last = nullptr;  // the source of the previous add (runs of adds share one)
+  Ranges* last_ranges = nullptr;
+  void add(const uint32_t* src, uint64_t s, uint64_t d, uint64_t len) {
+    if (!last_ranges || src != last) {
+      last = src;
+      last_ranges = &by_src[src];
+    }
+    last_ranges->add(s, d, len);
+  }
   hipError_t run(dl_ctx* ctx, uint32_t* dst) {
     for (auto& kv : by_src) {
       const std::vector<uint64_t>& r = kv.second.r;
@@ -278,8 +286,9 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   DBuf<uint32_t> d_staged;
   HIP_TRY(upload(ctx, d_staged, staged));
   if (words.by_src.count(nullptr)) {
-    words.by_src[d_staged.p] = words.by_src[nullptr];
+    words.by_src[d_staged.p] = std::move(words.by_src[nullptr]);
     words.by_src.erase(nullptr);
+    words.last_ranges = nullptr;
   }
   clock.mark("word_ranges");
   HIP_TRY(words.run(ctx, W2->p));
@@ -297,6 +306,8 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     std::vector<uint32_t> kc_crop;  // 4 per kept entry
     std::vector<uint32_t> k_region;
     std::vector<uint64_t> k_start(1, 0), k_out;
+    std::vector<uint32_t> dm_r0(dm.size());
+    std::vector<uint64_t> dm_e0(dm.size() + 1, 0);  // first kept entry of each dirty mesh
     CopyPlan levels;
     uint64_t at = 0;
     for (uint32_t d = 0; d < dm.size(); ++d) {
@@ -322,22 +333,38 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
           used_area.push_back(used_area.back() + (uint64_t)(q.rect[2] - q.rect[0] + 1) * (q.rect[3] - q.rect[1] + 1));
         }
       }
+      dm_r0[d] = r0;
+      dm_e0[d + 1] = dm_e0[d] + plan[i].ent.size();
+    }
+    // the kept entries of the dirty meshes, filled mesh by mesh in parallel
+    const uint64_t nkept = dm_e0.back();
+    kc_crop.resize(4 * nkept);
+    k_region.resize(nkept);
+    k_out.resize(nkept);
+    std::vector<uint64_t> k_words(nkept);
+    run_parallel(dm.size(), ctx->host_threads, [&](uint64_t d) {
+      const uint32_t i = dm[d];
+      const LiveMesh& m = svc->mesh[i];
       const Plan& p = plan[i];
       for (size_t j = 0; j < p.ent.size(); ++j) {
         const Entry& e = p.ent[j];
+        const uint64_t k = dm_e0[d] + j;
         const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
-        kc_crop.insert(kc_crop.end(), c, c + 4);
-        k_region.push_back(r0 + e.pair);
-        k_start.push_back(k_start.back() + shadow_words(c, m.pairs[e.pair].bits));
-        k_out.push_back(base[i] + p.sh[j]);
+        std::memcpy(&kc_crop[4 * k], c, 16);
+        k_region[k] = dm_r0[d] + e.pair;
+        k_words[k] = shadow_words(c, m.pairs[e.pair].bits);
+        k_out[k] = base[i] + p.sh[j];
       }
-    }
+    });
+    k_start.resize(nkept + 1);
+    for (uint64_t k = 0; k < nkept; ++k) k_start[k + 1] = k_start[k] + k_words[k];
     const uint32_t nreg = (uint32_t)r_bits.size();
-    const uint64_t nkept = k_out.size();
+    clock.mark("dirty_host");
     if (nreg) {
       std::unique_ptr<DevBlock> work(new DevBlock());
       HIP_TRY(work->alloc(at + 1, s));
       HIP_TRY(levels.run(ctx, work->p));
+      clock.mark("dirty_levels", s);
       DBuf<uint32_t> d_rect, d_bits, d_mesh, d_used, d_visited, d_kcrop, d_kregion;
       DBuf<uint64_t> d_off, d_vis, d_vstart, d_uarea, d_mvoff, d_kstart, d_kout, d_kid;
       DBuf<uint32_t> d_mres32;
@@ -360,10 +387,12 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       ms.visited_off = d_mvoff.p;
       ms.owned_off = d_mvoff.p;
       ms.visited = d_visited.p;
+      clock.mark("dirty_uploads", s);
       HIP_TRY(launch_region_visited_or(s, ms, rg, nreg, d_vstart.p, vis_start.back()));
       HIP_TRY(upload(ctx, d_used, used));
       HIP_TRY(upload(ctx, d_uarea, used_area));
       HIP_TRY(launch_pad_seams(s, ms, rg, d_used.p, d_uarea.p, (uint32_t)used.size(), used_area.back()));
+      clock.mark("dirty_pad", s);
       if (nkept) {
         std::vector<uint64_t> kid(nkept);
         std::vector<uint32_t> ktri(nkept);
@@ -435,7 +464,17 @@ void count_pair(dl_bake_stats& st, const LivePair& q, int sign) {
 // The captured pairs of a bake of `lights` replace the live pairs of those lights (and of
 // `removed`); marks the meshes whose pairs changed and keeps the stats up to date.
 void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& replaced, std::vector<uint8_t>& dirty) {
-  for (uint32_t i = 0; i < svc->mesh.size(); ++i) {
+  // the meshes holding pairs of the replaced lights (svc->light_meshes indexes them)
+  std::vector<uint32_t> hit;
+  for (uint64_t id : replaced) {
+    auto it = svc->light_meshes.find(id);
+    if (it == svc->light_meshes.end()) continue;
+    hit.insert(hit.end(), it->second.begin(), it->second.end());
+    svc->light_meshes.erase(it);
+  }
+  std::sort(hit.begin(), hit.end());
+  hit.erase(std::unique(hit.begin(), hit.end()), hit.end());
+  for (uint32_t i : hit) {
     auto& pairs = svc->mesh[i].pairs;
     const size_t before = pairs.size();
     for (const LivePair& q : pairs)
@@ -464,6 +503,7 @@ void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& re
     q.levels = c.block;
     q.src_words = c.words;
     count_pair(svc->stats, q, +1);
+    svc->light_meshes[c.light_id].push_back(c.mesh);
     auto& pairs = svc->mesh[c.mesh].pairs;
     pairs.insert(std::upper_bound(pairs.begin(), pairs.end(), q.light_id,
                                   [](uint64_t id, const LivePair& p) { return id < p.light_id; }),
@@ -529,6 +569,7 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->crops = cap.crops;
   svc->W.reset();
   svc->slot.clear();
+  svc->light_meshes.clear();
   svc->stride = 0;  // forces every mesh to be written
   svc->stats = dl_bake_stats{};
   svc->stats.meshes = (uint32_t)svc->mesh.size();
