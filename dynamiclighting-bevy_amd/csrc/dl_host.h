@@ -338,6 +338,14 @@ struct LivePair {
   std::shared_ptr<DevBlock> levels;     // holds the pair's unpadded levels and visited bits
   std::shared_ptr<DevBlock> src_words;
 };
+// A pair's visited bits to add to or take from its mesh's visit counts (the block holding them
+// lives until the counts are updated).
+struct VisDelta {
+  uint32_t mesh = 0;
+  uint32_t rect[4] = {0, 0, 0, 0};
+  std::shared_ptr<DevBlock> block;
+  uint64_t vis_off = 0, vis_words = 0;
+};
 struct LiveMesh {
   std::vector<LivePair> pairs;         // sorted by light id
   uint64_t w_off = 0, w_len = 0;       // the mesh's words in the live words
@@ -355,6 +363,12 @@ struct dl_service {
   std::vector<dl::LiveMesh> mesh;               // resident mesh order
   std::vector<uint32_t> crops;                  // 4 per receiver triangle
   std::unique_ptr<dl::DevBlock> W;              // live words
+  // per texel of every mesh's grid: how many of its pairs visited it (16 bits, two per word),
+  // from word vc_off[mesh]; and the changes of the pairs merged since the last assembly
+  std::unique_ptr<dl::DevBlock> vc;
+  std::vector<uint64_t> vc_off;
+  std::vector<dl::VisDelta> vis_add, vis_sub;
+  std::unique_ptr<dl::DevBlock> d_crops;        // the receiver triangles' crops on the device
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
   std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words
   std::unordered_map<uint64_t, std::vector<uint32_t>> light_meshes;  // light id -> meshes holding its pairs

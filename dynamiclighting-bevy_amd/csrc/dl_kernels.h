@@ -112,6 +112,10 @@ hipError_t launch_owned(hipStream_t s, DevTris tris, DevMeshes meshes, const uin
 hipError_t launch_texels(hipStream_t s, DevTris tris, DevMeshes meshes, const uint64_t* trace_count,
                          const uint64_t* cand_start, uint64_t total, int fill, unsigned long long* counts,
                          const uint64_t* tex_start, DevTexels out);
+// *out += a fingerprint of n words (order-free sum of splitmix64(index << 32 | word)).
+hipError_t launch_words_hash(hipStream_t s, const uint32_t* words, uint64_t n, unsigned long long* out);
+// out[i] = i for i < n.
+hipError_t launch_iota_u64(hipStream_t s, uint64_t* out, uint64_t n);
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta);
 // The burial test with one lane per texel (block threads per block).
 hipError_t launch_burial_lanes(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
@@ -302,6 +306,13 @@ hipError_t launch_bounce_filter(hipStream_t s, DevBounce b, const uint32_t* win_
                                 uint32_t nwin, uint64_t ntiles, BlurWeights weights);
 hipError_t launch_bounce_has(hipStream_t s, DevTris tris, DevEntries ent, DevBounce b, uint8_t* has_bounce);
 // ORs every region's region-local visited bits into its mesh's visited grid.
+// Visit counts of a rebake service's meshes: adds or subtracts n pairs' visited bits (pair p:
+// its bits at device address vis[p], rect, mesh resolution, counts from word count_off[p]; start:
+// word offsets, total words); then the visited bit grids of n meshes from their counts.
+hipError_t launch_vis_count(hipStream_t s, const uint64_t* vis, const uint32_t* rect, const uint32_t* res, const uint64_t* count_off,
+                            uint32_t n, const uint64_t* start, uint64_t total, uint32_t* counts, int sign);
+hipError_t launch_visited_from_counts(hipStream_t s, const uint32_t* res, const uint64_t* count_off, uint32_t n, const uint64_t* start,
+                                      uint64_t total, const uint32_t* counts, uint32_t* visited);
 hipError_t launch_region_visited_or(hipStream_t s, DevMeshes meshes, DevRegions rg, uint32_t nregions, const uint64_t* vis_words_start,
                                     uint64_t total_words);
 // Copies n word ranges: dst[r.y .. r.y + r.z) = src[r.x .. r.x + r.z) for r = ranges[i] (x, y, z as

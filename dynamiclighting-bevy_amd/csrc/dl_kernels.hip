@@ -1568,7 +1568,6 @@ __global__ void k_trig_eval(TrigTables trig, const float* __restrict__ x, uint64
   }
 }
 
-// out[i] = in[i] + delta (mod 2^32): moves triangle indices between numberings.
 // Region-local visited bits into the mesh visited grids: one thread per visited word.
 __global__ void k_region_visited_or(DevMeshes ms, DevRegions rg, uint32_t nreg, const uint64_t* __restrict__ start, uint64_t total) {
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
@@ -1585,6 +1584,49 @@ __global__ void k_region_visited_or(DevMeshes ms, DevRegions rg, uint32_t nreg, 
       const uint64_t vi = (rc[0] + ri % rw) + (rc[1] + ri / rw) * res;
       atomicOr(&ms.visited[ms.visited_off[m] + vi / 32], 1u << (vi % 32));
     }
+  }
+}
+
+// Per-texel visit counts of a rebake service's meshes (16 bits per texel, two per word): for
+// each visited bit of n pairs' region-local bits (pair p's words at vis[p], region rect[4p..],
+// mesh resolution res[p], counts from word count_off[p]), adds (sign > 0) or subtracts one.
+// One thread per visited word; start holds the pairs' word offsets (n + 1).
+__global__ void k_vis_count(const uint64_t* __restrict__ vis, const uint32_t* __restrict__ rect, const uint32_t* __restrict__ res,
+                            const uint64_t* __restrict__ count_off, uint32_t n, const uint64_t* __restrict__ start, uint64_t total,
+                            uint32_t* counts, int sign) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = (uint32_t)owner_of(start, n, w);
+    const uint32_t bits = reinterpret_cast<const uint32_t*>(vis[p])[w - start[p]];
+    if (!bits) continue;
+    const uint32_t* rc = rect + 4 * (size_t)p;
+    const uint64_t rw = (uint64_t)(rc[2] - rc[0] + 1), texels = rw * (rc[3] - rc[1] + 1);
+    for (uint32_t b = 0; b < 32; ++b) {
+      const uint64_t ri = (w - start[p]) * 32 + b;
+      if (ri >= texels || !((bits >> b) & 1u)) continue;
+      const uint64_t vi = (rc[0] + ri % rw) + (rc[1] + ri / rw) * (uint64_t)res[p];
+      const uint32_t one = 1u << (16u * (uint32_t)(vi & 1u));
+      if (sign > 0) atomicAdd(&counts[count_off[p] + vi / 2], one);
+      else atomicSub(&counts[count_off[p] + vi / 2], one);
+    }
+  }
+}
+
+// Visited grids of n meshes from their visit counts: bit i of mesh m's grid (from word
+// visited_off[m]) is set when texel i's count (from word count_off[m]) is nonzero.  One thread
+// per grid word; start holds the meshes' grid word offsets (n + 1).
+__global__ void k_visited_from_counts(const uint32_t* __restrict__ res, const uint64_t* __restrict__ count_off, uint32_t n,
+                                      const uint64_t* __restrict__ start, uint64_t total, const uint32_t* __restrict__ counts,
+                                      uint32_t* visited) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t m = (uint32_t)owner_of(start, n, w);
+    const uint64_t texels = (uint64_t)res[m] * res[m], i0 = (w - start[m]) * 32;
+    uint32_t out = 0;
+    for (uint32_t b = 0; b < 32 && i0 + b < texels; b += 2) {
+      const uint32_t c = counts[count_off[m] + (i0 + b) / 2];
+      if (c & 0xffffu) out |= 1u << b;
+      if ((c >> 16) && i0 + b + 1 < texels) out |= 2u << b;
+    }
+    visited[w] = out;
   }
 }
 
@@ -1661,6 +1703,35 @@ hipError_t launch_burial_decide(hipStream_t s, DevScene scene, DevTexels tx, uin
   if (max_n == 0 || nk == 0 || k0 + nk > 8) return max_n == 0 ? hipSuccess : hipErrorInvalidValue;
   hipLaunchKernelGGL(k_burial_decide, dim3(grid_for(max_n, kThreads)), dim3(kThreads), 0, s, scene, tx, first, list, count, max_n,
                      k0, nk, rays, hits, back_count, buried, kept, next, next_n);
+  return hipGetLastError();
+}
+// A fingerprint of n words: the sum (mod 2^64) of splitmix64(index << 32 | word) over the words,
+// added to *out (an order-free sum, so the result does not depend on the launch).
+__global__ void k_words_hash(const uint32_t* __restrict__ w, uint64_t n, unsigned long long* out) {
+  unsigned long long acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = (i << 32) ^ (uint64_t)w[i];
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    acc += z ^ (z >> 31);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63u) == 0) atomicAdd(out, acc);
+}
+hipError_t launch_words_hash(hipStream_t s, const uint32_t* words, uint64_t n, unsigned long long* out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_words_hash, dim3((uint32_t)std::min<uint64_t>(grid_for(n, kThreads), 4096)), dim3(kThreads), 0, s, words, n,
+                     out);
+  return hipGetLastError();
+}
+__global__ void k_iota_u64(uint64_t* out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) out[i] = i;
+}
+hipError_t launch_iota_u64(hipStream_t s, uint64_t* out, uint64_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u64, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, out, n);
   return hipGetLastError();
 }
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta) {
@@ -1820,6 +1891,20 @@ hipError_t launch_region_visited_or(hipStream_t s, DevMeshes meshes, DevRegions 
   if (total_words == 0) return hipSuccess;
   hipLaunchKernelGGL(k_region_visited_or, dim3(grid_for(total_words, kThreads)), dim3(kThreads), 0, s, meshes, rg, nregions,
                      vis_words_start, total_words);
+  return hipGetLastError();
+}
+hipError_t launch_vis_count(hipStream_t s, const uint64_t* vis, const uint32_t* rect, const uint32_t* res, const uint64_t* count_off,
+                            uint32_t n, const uint64_t* start, uint64_t total, uint32_t* counts, int sign) {
+  if (total == 0 || n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vis_count, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, vis, rect, res, count_off, n, start, total,
+                     counts, sign);
+  return hipGetLastError();
+}
+hipError_t launch_visited_from_counts(hipStream_t s, const uint32_t* res, const uint64_t* count_off, uint32_t n, const uint64_t* start,
+                                      uint64_t total, const uint32_t* counts, uint32_t* visited) {
+  if (total == 0 || n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_visited_from_counts, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, res, count_off, n, start, total,
+                     counts, visited);
   return hipGetLastError();
 }
 hipError_t launch_copy_ranges(hipStream_t s, const uint32_t* src, uint32_t* dst, const uint64_t* ranges, const uint64_t* range_start,

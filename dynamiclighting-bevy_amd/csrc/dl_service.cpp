@@ -168,6 +168,8 @@ struct PhaseClock {
 int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   dl_ctx* ctx = svc->ctx;
   hipStream_t s = ctx->stream;
+  ctx->cache.stream = s;
+  const CacheScope cache_scope(&ctx->cache);  // stream-ordered buffers: no device-wide frees
   const dl_bake_options& opt = svc->options;
   const Resident& res = svc->res;
   const uint32_t nm = (uint32_t)svc->mesh.size();
@@ -259,7 +261,35 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   // ---- the copies into the new words ------------------------------------------------------
   std::unique_ptr<DevBlock> W2(new DevBlock());
   HIP_TRY(W2->alloc(total + 1, s));
-  std::vector<uint32_t> staged;  // light data of the rewritten meshes, uploaded once
+  // light data of the rewritten meshes, staged in one buffer; the copies of each rewritten mesh
+  // are listed in parallel and then grouped by source
+  std::vector<uint64_t> st_off(nm + 1, 0);
+  for (uint32_t i = 0; i < nm; ++i) st_off[i + 1] = st_off[i] + (plan[i].whole ? 0 : plan[i].ld.size());
+  std::vector<uint32_t> staged(st_off[nm]);
+  struct Copy {
+    const uint32_t* src;
+    uint64_t s, d, len;
+  };
+  std::vector<std::vector<Copy>> mesh_copies(nm);
+  run_parallel(nm, ctx->host_threads, [&](uint64_t i) {
+    const Plan& p = plan[i];
+    const LiveMesh& m = svc->mesh[i];
+    if (p.whole) return;
+    std::copy(p.ld.begin(), p.ld.end(), staged.begin() + (long)st_off[i]);
+    std::vector<Copy>& out = mesh_copies[i];
+    for (size_t j = 0; j < p.ent.size(); ++j) {
+      const Entry& e = p.ent[j];
+      const LivePair& q = m.pairs[e.pair];
+      if (!dirty[i]) {
+        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
+        out.push_back({svc->W->p, m.w_off + q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits)});
+      }
+      if (q.bounce_words[e.k]) {
+        if (q.src_words) out.push_back({q.src_words->p, q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]});
+        else out.push_back({svc->W->p, m.w_off + q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]});
+      }
+    }
+  });
   CopyPlan words;
   for (uint32_t i = 0; i < nm; ++i) {
     const Plan& p = plan[i];
@@ -268,20 +298,8 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       words.add(svc->W->p, m.w_off, base[i], m.w_len);
       continue;
     }
-    words.add(nullptr, staged.size(), base[i], p.ld.size());  // resolved to the staging buffer below
-    staged.insert(staged.end(), p.ld.begin(), p.ld.end());
-    for (size_t j = 0; j < p.ent.size(); ++j) {
-      const Entry& e = p.ent[j];
-      const LivePair& q = m.pairs[e.pair];
-      if (!dirty[i]) {
-        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
-        words.add(svc->W->p, m.w_off + q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits));
-      }
-      if (q.bounce_words[e.k]) {
-        if (q.src_words) words.add(q.src_words->p, q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
-        else words.add(svc->W->p, m.w_off + q.bounce_off[e.k], base[i] + p.bo[j], q.bounce_words[e.k]);
-      }
-    }
+    words.add(nullptr, st_off[i], base[i], p.ld.size());  // resolved to the staging buffer below
+    for (const Copy& c : mesh_copies[i]) words.add(c.src, c.s, c.d, c.len);
   }
   DBuf<uint32_t> d_staged;
   HIP_TRY(upload(ctx, d_staged, staged));
@@ -291,136 +309,156 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     words.last_ranges = nullptr;
   }
   clock.mark("word_ranges");
+  if (clock.on) {
+    size_t nr = 0;
+    for (auto& kv : words.by_src) nr += kv.second.r.size() / 3;
+    clock.line += " (ranges " + std::to_string(nr) + ", staged words " + std::to_string(staged.size()) + ")";
+  }
   HIP_TRY(words.run(ctx, W2->p));
   clock.mark("word_copies", s);
 
-  // ---- dirty meshes: visited from their pairs, seam padding, crop and dilation -------------
-  // The levels of the dirty meshes' pairs are gathered into a work buffer (padding changes them
-  // in place; the pairs keep their unpadded levels).
+  // ---- visit counts: the pending changes of merge, additions first ---------------------------
+  // (so that no count passes below zero on the way)
+  for (int sign : {+1, -1}) {
+    std::vector<VisDelta>& list = sign > 0 ? svc->vis_add : svc->vis_sub;
+    if (list.empty()) continue;
+    std::vector<uint64_t> vaddr, coff, start(1, 0);
+    std::vector<uint32_t> rect, vres;
+    for (const VisDelta& v : list) {
+      vaddr.push_back(reinterpret_cast<uint64_t>(v.block->p + v.vis_off));
+      rect.insert(rect.end(), v.rect, v.rect + 4);
+      vres.push_back(res.res[v.mesh]);
+      coff.push_back(svc->vc_off[v.mesh]);
+      start.push_back(start.back() + v.vis_words);
+    }
+    DBuf<uint64_t> d_vaddr, d_coff, d_start;
+    DBuf<uint32_t> d_rect, d_vres;
+    HIP_TRY(upload(ctx, d_vaddr, vaddr));
+    HIP_TRY(upload(ctx, d_coff, coff));
+    HIP_TRY(upload(ctx, d_start, start));
+    HIP_TRY(upload(ctx, d_rect, rect));
+    HIP_TRY(upload(ctx, d_vres, vres));
+    HIP_TRY(launch_vis_count(s, d_vaddr.p, d_rect.p, d_vres.p, d_coff.p, (uint32_t)list.size(), d_start.p, start.back(), svc->vc->p,
+                             sign));
+    HIP_TRY(hipStreamSynchronize(s));  // the deltas' blocks may be freed after this
+    list.clear();
+  }
+  clock.mark("counts");
+
+  // ---- dirty meshes: visited grids from the counts, seam padding, crop and dilation ---------
+  // The levels of the dirty meshes' used pairs (those with kept entries, the only ones padded
+  // and cropped) are gathered into a work buffer: padding changes them in place, and the pairs
+  // keep their unpadded levels.
   {
     std::vector<uint32_t> dm;
     for (uint32_t i = 0; i < nm; ++i)
       if (dirty[i] && !plan[i].whole) dm.push_back(i);
-    std::vector<uint32_t> r_rect, r_bits, r_mesh, used;
-    std::vector<uint64_t> r_off, r_vis, vis_start(1, 0), used_area(1, 0), m_res, m_voff(1, 0);
-    std::vector<uint32_t> kc_crop;  // 4 per kept entry
-    std::vector<uint32_t> k_region;
-    std::vector<uint64_t> k_start(1, 0), k_out;
-    std::vector<uint32_t> dm_r0(dm.size());
-    std::vector<uint64_t> dm_e0(dm.size() + 1, 0);  // first kept entry of each dirty mesh
+    const uint32_t nd = (uint32_t)dm.size();
+    std::vector<uint32_t> r_rect, r_bits, r_mesh, m_res32, used;
+    std::vector<uint64_t> r_off, used_area(1, 0), m_voff(1, 0), m_coff;
+    std::vector<uint32_t> dm_r0(nd);
+    std::vector<uint64_t> dm_e0(nd + 1, 0);  // first kept entry of each dirty mesh
+    std::vector<std::vector<uint32_t>> used_of(nd);  // per dirty mesh: pair -> used region
     CopyPlan levels;
     uint64_t at = 0;
-    for (uint32_t d = 0; d < dm.size(); ++d) {
+    for (uint32_t d = 0; d < nd; ++d) {
       const uint32_t i = dm[d];
       const LiveMesh& m = svc->mesh[i];
-      const uint32_t r0 = (uint32_t)r_bits.size();
       const uint32_t rs = res.res[i];
-      m_res.push_back(rs);
+      m_res32.push_back(rs);
       m_voff.push_back(m_voff.back() + ((uint64_t)rs * rs + 31) / 32);
+      m_coff.push_back(svc->vc_off[i]);
+      dm_r0[d] = (uint32_t)r_bits.size();
+      used_of[d].assign(m.pairs.size(), 0);
       for (size_t pi = 0; pi < m.pairs.size(); ++pi) {
         const LivePair& q = m.pairs[pi];
+        if (q.kept.empty()) continue;
+        used_of[d][pi] = (uint32_t)r_bits.size() - dm_r0[d];
+        used.push_back((uint32_t)r_bits.size());
         r_rect.insert(r_rect.end(), q.rect, q.rect + 4);
         r_bits.push_back(q.bits);
         r_mesh.push_back(d);
         levels.add(q.levels->p, q.grid_off, at, q.grid_words);
-        levels.add(q.levels->p, q.vis_off, at + q.grid_words, q.vis_words);
         r_off.push_back(at);
-        r_vis.push_back(at + q.grid_words);
-        at += q.grid_words + q.vis_words;
-        vis_start.push_back(vis_start.back() + q.vis_words);
-        if (!q.kept.empty()) {
-          used.push_back(r0 + (uint32_t)pi);
-          used_area.push_back(used_area.back() + (uint64_t)(q.rect[2] - q.rect[0] + 1) * (q.rect[3] - q.rect[1] + 1));
-        }
+        at += q.grid_words;
+        used_area.push_back(used_area.back() + (uint64_t)(q.rect[2] - q.rect[0] + 1) * (q.rect[3] - q.rect[1] + 1));
       }
-      dm_r0[d] = r0;
       dm_e0[d + 1] = dm_e0[d] + plan[i].ent.size();
     }
+    clock.mark("dirty_pairs");
     // the kept entries of the dirty meshes, filled mesh by mesh in parallel
     const uint64_t nkept = dm_e0.back();
-    kc_crop.resize(4 * nkept);
-    k_region.resize(nkept);
-    k_out.resize(nkept);
-    std::vector<uint64_t> k_words(nkept);
-    run_parallel(dm.size(), ctx->host_threads, [&](uint64_t d) {
+    std::vector<uint32_t> k_tri(nkept), k_region(nkept);
+    std::vector<uint64_t> k_words(nkept), k_out(nkept), k_start(nkept + 1, 0);
+    run_parallel(nd, ctx->host_threads, [&](uint64_t d) {
       const uint32_t i = dm[d];
       const LiveMesh& m = svc->mesh[i];
       const Plan& p = plan[i];
       for (size_t j = 0; j < p.ent.size(); ++j) {
         const Entry& e = p.ent[j];
         const uint64_t k = dm_e0[d] + j;
-        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
-        std::memcpy(&kc_crop[4 * k], c, 16);
-        k_region[k] = dm_r0[d] + e.pair;
-        k_words[k] = shadow_words(c, m.pairs[e.pair].bits);
+        k_tri[k] = res.first_tri[i] + e.tri;
+        k_region[k] = dm_r0[d] + used_of[d][e.pair];
+        k_words[k] = shadow_words(&svc->crops[4 * (size_t)k_tri[k]], m.pairs[e.pair].bits);
         k_out[k] = base[i] + p.sh[j];
       }
     });
-    k_start.resize(nkept + 1);
     for (uint64_t k = 0; k < nkept; ++k) k_start[k + 1] = k_start[k] + k_words[k];
     const uint32_t nreg = (uint32_t)r_bits.size();
     clock.mark("dirty_host");
-    if (nreg) {
-      std::unique_ptr<DevBlock> work(new DevBlock());
-      HIP_TRY(work->alloc(at + 1, s));
-      HIP_TRY(levels.run(ctx, work->p));
-      clock.mark("dirty_levels", s);
-      DBuf<uint32_t> d_rect, d_bits, d_mesh, d_used, d_visited, d_kcrop, d_kregion;
-      DBuf<uint64_t> d_off, d_vis, d_vstart, d_uarea, d_mvoff, d_kstart, d_kout, d_kid;
-      DBuf<uint32_t> d_mres32;
-      std::vector<uint32_t> mres32(m_res.begin(), m_res.end());
-      HIP_TRY(upload(ctx, d_rect, r_rect));
-      HIP_TRY(upload(ctx, d_bits, r_bits));
-      HIP_TRY(upload(ctx, d_mesh, r_mesh));
-      HIP_TRY(upload(ctx, d_off, r_off));
-      HIP_TRY(upload(ctx, d_vis, r_vis));
-      HIP_TRY(upload(ctx, d_vstart, vis_start));
-      HIP_TRY(upload(ctx, d_mres32, mres32));
+    if (clock.on) clock.line += " (used regions " + std::to_string(nreg) + ", kept " + std::to_string(nkept) + ")";
+    if (nd) {
+      DBuf<uint32_t> d_visited, d_mres32;
+      DBuf<uint64_t> d_mvoff, d_mcoff;
+      HIP_TRY(upload(ctx, d_mres32, m_res32));
       HIP_TRY(upload(ctx, d_mvoff, m_voff));
-      HIP_TRY(zeroed(ctx, d_visited, m_voff.back() + 1));
-      DevRegions rg{d_rect.p, d_bits.p, d_off.p, d_mesh.p, work->p};
-      rg.vis = work->p;
-      rg.vis_off = d_vis.p;
-      DevMeshes ms{};
-      ms.n = (uint32_t)m_res.size();
-      ms.res = d_mres32.p;
-      ms.visited_off = d_mvoff.p;
-      ms.owned_off = d_mvoff.p;
-      ms.visited = d_visited.p;
-      clock.mark("dirty_uploads", s);
-      HIP_TRY(launch_region_visited_or(s, ms, rg, nreg, d_vstart.p, vis_start.back()));
-      HIP_TRY(upload(ctx, d_used, used));
-      HIP_TRY(upload(ctx, d_uarea, used_area));
-      HIP_TRY(launch_pad_seams(s, ms, rg, d_used.p, d_uarea.p, (uint32_t)used.size(), used_area.back()));
-      clock.mark("dirty_pad", s);
-      if (nkept) {
-        std::vector<uint64_t> kid(nkept);
-        std::vector<uint32_t> ktri(nkept);
-        for (uint64_t k = 0; k < nkept; ++k) {
-          kid[k] = k;
-          ktri[k] = (uint32_t)k;
+      HIP_TRY(upload(ctx, d_mcoff, m_coff));
+      HIP_TRY(d_visited.alloc(m_voff.back() + 1));
+      HIP_TRY(launch_visited_from_counts(s, d_mres32.p, d_mcoff.p, nd, d_mvoff.p, m_voff.back(), svc->vc->p, d_visited.p));
+      if (nreg) {
+        std::unique_ptr<DevBlock> work(new DevBlock());
+        HIP_TRY(work->alloc(at + 1, s));
+        HIP_TRY(levels.run(ctx, work->p));
+        DBuf<uint32_t> d_rect, d_bits, d_mesh, d_used, d_ktri, d_kregion;
+        DBuf<uint64_t> d_off, d_uarea, d_kstart, d_kout, d_kid;
+        HIP_TRY(upload(ctx, d_rect, r_rect));
+        HIP_TRY(upload(ctx, d_bits, r_bits));
+        HIP_TRY(upload(ctx, d_mesh, r_mesh));
+        HIP_TRY(upload(ctx, d_off, r_off));
+        HIP_TRY(upload(ctx, d_used, used));
+        HIP_TRY(upload(ctx, d_uarea, used_area));
+        DevRegions rg{d_rect.p, d_bits.p, d_off.p, d_mesh.p, work->p};
+        DevMeshes ms{};
+        ms.n = nd;
+        ms.res = d_mres32.p;
+        ms.visited_off = d_mvoff.p;
+        ms.owned_off = d_mvoff.p;
+        ms.visited = d_visited.p;
+        clock.mark("dirty_uploads", s);
+        HIP_TRY(launch_pad_seams(s, ms, rg, d_used.p, d_uarea.p, (uint32_t)used.size(), used_area.back()));
+        clock.mark("dirty_pad", s);
+        if (nkept) {
+          HIP_TRY(upload(ctx, d_ktri, k_tri));
+          HIP_TRY(upload(ctx, d_kregion, k_region));
+          HIP_TRY(upload(ctx, d_kstart, k_start));
+          HIP_TRY(upload(ctx, d_kout, k_out));
+          HIP_TRY(d_kid.alloc(nkept));
+          HIP_TRY(launch_iota_u64(s, d_kid.p, nkept));
+          DevTris tr{};
+          tr.n = res.ntri;
+          tr.crop = svc->d_crops->p;
+          DevEntries ent{nkept, nullptr, d_kregion.p, d_ktri.p, nullptr};
+          HIP_TRY(launch_crop(s, tr, ent, rg, d_kid.p, d_kstart.p, nkept, k_start.back(), d_kout.p, W2->p));
         }
-        DBuf<uint32_t> d_ktri;
-        HIP_TRY(upload(ctx, d_kcrop, kc_crop));
-        HIP_TRY(upload(ctx, d_kregion, k_region));
-        HIP_TRY(upload(ctx, d_ktri, ktri));
-        HIP_TRY(upload(ctx, d_kstart, k_start));
-        HIP_TRY(upload(ctx, d_kout, k_out));
-        HIP_TRY(upload(ctx, d_kid, kid));
-        DevTris tr{};
-        tr.n = (uint32_t)nkept;
-        tr.crop = d_kcrop.p;
-        DevEntries ent{nkept, nullptr, d_kregion.p, d_ktri.p, nullptr};
-        HIP_TRY(launch_crop(s, tr, ent, rg, d_kid.p, d_kstart.p, nkept, k_start.back(), d_kout.p, W2->p));
+        HIP_TRY(hipStreamSynchronize(s));
       }
-      HIP_TRY(hipStreamSynchronize(s));
     }
   }
   HIP_TRY(hipStreamSynchronize(s));
   clock.mark("dirty");
 
   // ---- the new state ------------------------------------------------------------------------
-  for (uint32_t i = 0; i < nm; ++i) {
+  run_parallel(nm, ctx->host_threads, [&](uint64_t i) {
     LiveMesh& m = svc->mesh[i];
     const Plan& p = plan[i];
     if (!p.whole) {
@@ -433,7 +471,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     }
     m.w_off = base[i];
     m.w_len = p.len;
-  }
+  });
   svc->W = std::move(W2);
   svc->slot = std::move(slot);
   svc->baked = std::move(baked);
@@ -443,6 +481,17 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   clock.mark("state");
   if (clock.on) clock.line += " restaged_meshes=" + std::to_string(restaged.load());
   return DL_OK;
+}
+
+// A pair's visited bits, to add to or take from its mesh's visit counts.
+VisDelta vis_delta(uint32_t mesh, const LivePair& q) {
+  VisDelta v;
+  v.mesh = mesh;
+  std::memcpy(v.rect, q.rect, 16);
+  v.block = q.levels;
+  v.vis_off = q.vis_off;
+  v.vis_words = q.vis_words;
+  return v;
 }
 
 // Adds (sign +1) or removes (-1) a pair's share of the live stats.
@@ -478,7 +527,10 @@ void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& re
     auto& pairs = svc->mesh[i].pairs;
     const size_t before = pairs.size();
     for (const LivePair& q : pairs)
-      if (replaced.count(q.light_id)) count_pair(svc->stats, q, -1);
+      if (replaced.count(q.light_id)) {
+        count_pair(svc->stats, q, -1);
+        svc->vis_sub.push_back(vis_delta(i, q));
+      }
     pairs.erase(std::remove_if(pairs.begin(), pairs.end(), [&](const LivePair& q) { return replaced.count(q.light_id) > 0; }),
                 pairs.end());
     if (pairs.size() != before) dirty[i] = 1;
@@ -503,6 +555,7 @@ void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& re
     q.levels = c.block;
     q.src_words = c.words;
     count_pair(svc->stats, q, +1);
+    svc->vis_add.push_back(vis_delta(c.mesh, q));
     svc->light_meshes[c.light_id].push_back(c.mesh);
     auto& pairs = svc->mesh[c.mesh].pairs;
     pairs.insert(std::upper_bound(pairs.begin(), pairs.end(), q.light_id,
@@ -570,6 +623,21 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->W.reset();
   svc->slot.clear();
   svc->light_meshes.clear();
+  svc->vis_add.clear();
+  svc->vis_sub.clear();
+  {
+    hipStream_t s = ctx->stream;
+    svc->vc_off.assign(svc->mesh.size() + 1, 0);
+    for (size_t i = 0; i < svc->mesh.size(); ++i)
+      svc->vc_off[i + 1] = svc->vc_off[i] + ((uint64_t)svc->res.res[i] * svc->res.res[i] + 1) / 2;
+    svc->vc.reset(new DevBlock());
+    HIP_TRY(svc->vc->alloc(svc->vc_off.back() + 1, s));
+    HIP_TRY(hipMemsetAsync(svc->vc->p, 0, (svc->vc_off.back() + 1) * 4, s));
+    svc->d_crops.reset(new DevBlock());
+    HIP_TRY(svc->d_crops->alloc(svc->crops.size() + 1, s));
+    HIP_TRY(hipMemcpyAsync(svc->d_crops->p, svc->crops.data(), svc->crops.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   svc->stride = 0;  // forces every mesh to be written
   svc->stats = dl_bake_stats{};
   svc->stats.meshes = (uint32_t)svc->mesh.size();
@@ -631,6 +699,11 @@ int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_up
       info->version = svc->version;
       for (uint8_t d : dirty) info->dirty_meshes += d;
       info->traced_pairs = cap.pairs.size();
+      for (const CapturedPair& c : cap.pairs) {
+        info->traced_samples += c.samples;
+        info->traced_rays += c.rays;
+        info->traced_bounce_rays += c.bounce_rays;
+      }
       info->trace_ms = ms_between(t0, t1);
       info->assemble_ms = ms_between(t1, std::chrono::steady_clock::now());
       info->total_ms = ms_between(t0, std::chrono::steady_clock::now());
@@ -704,13 +777,20 @@ int dl_service_words(dl_service* svc, const uint32_t** device_words, uint64_t* c
 }  // extern "C"
 
 namespace {
-// FNV-1a over the live words and stats (a test's fingerprint of the live scene after an edit).
+// A fingerprint of the live scene after an edit: of its words (hashed on the device), stats and
+// light order.
 uint64_t live_checksum(dl_service* svc, int* st) {
   dl_ctx* ctx = svc->ctx;
-  std::vector<uint32_t> w;
   *st = DL_OK;
+  unsigned long long words_hash = 0;
   if (svc->W && svc->stats.triangle_words) {
-    hipError_t e = download(ctx, w, svc->W->p, svc->stats.triangle_words);
+    ctx->cache.stream = ctx->stream;
+    const CacheScope cache_scope(&ctx->cache);
+    DBuf<unsigned long long> d;
+    hipError_t e = zeroed(ctx, d, 1);
+    if (e == hipSuccess) e = launch_words_hash(ctx->stream, svc->W->p, svc->stats.triangle_words, d.p);
+    if (e == hipSuccess) e = hipMemcpyAsync(&words_hash, d.p, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
       *st = set_error(ctx, DL_E_DEVICE, hipGetErrorString(e));
       return 0;
@@ -721,7 +801,7 @@ uint64_t live_checksum(dl_service* svc, int* st) {
     const uint8_t* b = static_cast<const uint8_t*>(p);
     for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
   };
-  put(w.data(), w.size() * 4);
+  put(&words_hash, 8);
   put(&svc->stats, sizeof svc->stats);
   for (const dl_baked_light& l : svc->baked) put(&l.id, 8);
   return h;
@@ -764,6 +844,7 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
       bool done = false;
       uint64_t bytes = 0;
       double trace_ms = 0;
+      double queue_ms = 0;
     };
     std::vector<Slot> slots(count);
     std::mutex mu;
@@ -796,6 +877,7 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
           cv.notify_all();
         }
         const auto a = std::chrono::steady_clock::now();
+        sl.queue_ms = std::chrono::duration<double, std::milli>(a - arrival(i)).count();
         std::vector<dl_light> changed(edits[i].lights, edits[i].lights + edits[i].light_count);
         int st;
         try {
@@ -819,51 +901,79 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
     };
     std::vector<std::thread> ts;
     for (uint32_t k = 0; k < workers; ++k) ts.emplace_back(work, ctx->workers[k]);
+    // Edits join the live scene in arrival order.  Those whose tracing has ended by the time the
+    // scene is free join together: their pairs are merged in order and the scene is assembled
+    // once for all of them, which gives the scene of applying them one by one.
+    constexpr uint32_t kMaxJoin = 16;
     int result = DL_OK;
-    for (uint32_t i = 0; i < count && result == DL_OK; ++i) {  // edits join the live scene in order
-      Slot& sl = slots[i];
+    uint32_t i = 0;
+    while (i < count && result == DL_OK) {
+      uint32_t j = i + 1;
       {
         std::unique_lock<std::mutex> l(mu);
-        cv.wait(l, [&] { return sl.done; });
-      }
-      if (sl.status != DL_OK) {
-        result = set_error(ctx, sl.status, sl.err);
-        break;
+        cv.wait(l, [&] { return slots[i].done; });
+        while (j < count && j - i < kMaxJoin && slots[j].done) ++j;
       }
       const auto a = std::chrono::steady_clock::now();
-      std::vector<dl_light> next_lights;
-      std::unordered_set<uint64_t> replaced;
-      int st = edited_lights(svc, edits[i], next_lights, replaced);
       std::vector<uint8_t> dirty(svc->mesh.size(), 0);
-      if (st == DL_OK) {
+      uint32_t applied = i;
+      int st = DL_OK;
+      for (uint32_t k = i; k < j; ++k) {
+        Slot& sl = slots[k];
+        if (sl.status != DL_OK) {
+          st = set_error(ctx, sl.status, sl.err);
+          break;
+        }
+        std::vector<dl_light> next_lights;
+        std::unordered_set<uint64_t> replaced;
+        if ((st = edited_lights(svc, edits[k], next_lights, replaced)) != DL_OK) break;
         merge(svc, sl.cap, replaced, dirty);
         for (uint64_t id : replaced)
-          if (!std::any_of(edits[i].lights, edits[i].lights + edits[i].light_count, [&](const dl_light& l) { return l.id == id; }))
+          if (!std::any_of(edits[k].lights, edits[k].lights + edits[k].light_count, [&](const dl_light& l) { return l.id == id; }))
             svc->cubes.erase(id);
         svc->lights = std::move(next_lights);
-        st = assemble(svc, dirty);
+        applied = k + 1;
       }
-      if (st != DL_OK) {
-        result = st;
-        break;
+      if (applied > i) {
+        const int sa = assemble(svc, dirty);
+        if (sa != DL_OK) {
+          result = sa;
+          break;
+        }
       }
-      svc->version++;
       const auto done = std::chrono::steady_clock::now();
-      latency_ms[i] = std::chrono::duration<double, std::milli>(done - arrival(i)).count();
-      if (info) {
-        info[i] = dl_service_update{};
-        info[i].version = svc->version;
-        for (uint8_t d : dirty) info[i].dirty_meshes += d;
-        info[i].traced_pairs = sl.cap.pairs.size();
-        info[i].trace_ms = sl.trace_ms;
-        info[i].assemble_ms = ms_between(a, done);
-        info[i].total_ms = info[i].trace_ms + info[i].assemble_ms;
+      uint32_t ndirty = 0;
+      for (uint8_t d : dirty) ndirty += d;
+      for (uint32_t k = i; k < applied; ++k) {
+        Slot& sl = slots[k];
+        latency_ms[k] = std::chrono::duration<double, std::milli>(done - arrival(k)).count();
+        if (info) {
+          info[k] = dl_service_update{};
+          info[k].version = svc->version + (k - i) + 1;
+          info[k].dirty_meshes = ndirty;
+          info[k].reserved = applied - i;  // edits that joined the scene together
+          info[k].traced_pairs = sl.cap.pairs.size();
+          for (const CapturedPair& c : sl.cap.pairs) {
+            info[k].traced_samples += c.samples;
+            info[k].traced_rays += c.rays;
+            info[k].traced_bounce_rays += c.bounce_rays;
+          }
+          info[k].queue_ms = sl.queue_ms;
+          info[k].trace_ms = sl.trace_ms;
+          info[k].assemble_ms = ms_between(a, done);
+          info[k].total_ms = info[k].trace_ms + info[k].assemble_ms;
+        }
+        if (checksum) checksum[k] = 0;
+        sl.cap = Capture{};
       }
-      sl.cap = Capture{};
-      if (checksum) {
-        checksum[i] = live_checksum(svc, &st);
-        if (st != DL_OK) result = st;
+      svc->version += applied - i;
+      if (checksum && applied > i) {
+        int sc = DL_OK;
+        checksum[applied - 1] = live_checksum(svc, &sc);
+        if (sc != DL_OK) st = sc;
       }
+      if (st != DL_OK) result = st;
+      i = j;
     }
     {
       std::lock_guard<std::mutex> l(mu);

@@ -103,7 +103,8 @@ class dl_service_edit(C.Structure):
 class dl_service_update(C.Structure):
     _fields_ = [("version", C.c_uint64), ("dirty_meshes", C.c_uint32), ("reserved", C.c_uint32),
                 ("traced_pairs", C.c_uint64), ("trace_ms", C.c_double), ("assemble_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("traced_samples", C.c_uint64), ("traced_rays", C.c_uint64),
+                ("traced_bounce_rays", C.c_uint64), ("queue_ms", C.c_double)]
 
 
 class dl_service_request(C.Structure):

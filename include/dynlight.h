@@ -345,15 +345,23 @@ typedef struct dl_service_update {
     uint32_t reserved;
     uint64_t traced_pairs;    /* (mesh, light) pairs traced for the edit                       */
     double trace_ms, assemble_ms, total_ms;
+    uint64_t traced_samples;  /* texel-light samples, shadow rays and bounce rays the edit's     */
+    uint64_t traced_rays;     /* tracing cast (the dl_bake_stats of those pairs)               */
+    uint64_t traced_bounce_rays;
+    double queue_ms;          /* dl_service_replay_edits: arrival to the start of its tracing  */
 } dl_service_update;
 int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_update* info);
 /* Serves edits as they arrive, with continuous batching: each edit's changed lights are traced on
- * the first of `workers` (1-16) worker streams that is free, admitted while the device memory the
- * edits being traced may hold fits `budget_bytes` (0: half the free device memory), so edits
- * that arrive while others run overlap on the device; edits join the live scene in arrival order.
- * latency_ms[i] runs from edit i's arrival to the live scene holding it; checksum (may be NULL)
- * receives a fingerprint of the live words, stats and light order after each edit (the same
- * whether the edits ran alone or in the mix); info (may be NULL) each edit's update record. */
+ * the first of `workers` (1-16) worker streams that is free, admitted in arrival order while the
+ * device memory the edits being traced may hold fits `budget_bytes` (0: half the free device
+ * memory), so edits that arrive while others run overlap on the device.  Edits join the live scene
+ * in arrival order; those whose tracing has ended by the time the scene is free join together
+ * (up to 16), in one assembly that gives the scene of applying them one by one.
+ * latency_ms[i] runs from edit i's arrival to the live scene holding it.  checksum (may be NULL)
+ * receives, for the last edit of each joint assembly, a fingerprint of the live words, stats and
+ * light order (the same as after that edit applied alone), and 0 for the others.  info (may be
+ * NULL) receives each edit's update record (dirty_meshes and assemble_ms are those of its joint
+ * assembly; `reserved` is the number of edits in it). */
 int dl_service_replay_edits(dl_service* svc, const dl_service_edit* edits, uint32_t count, uint32_t workers,
                             uint64_t budget_bytes, double* latency_ms, uint64_t* checksum, dl_service_update* info);
 /* A copy of the live scene on the host (downloads its words). */

@@ -152,35 +152,44 @@ def _edit_stream(lights, n, seed):
 
 def test_replayed_edits_overlap_and_match_edits_one_at_a_time(ctx):
     """Continuous batching of edits (dl_service_replay_edits): edits arriving together trace on
-    several streams at once, yet after each edit the live scene is the same as when the edits run
-    one at a time, under a budget that admits one edit at a time, and the last one equals a bake."""
+    several streams at once and join the live scene together, yet wherever the mix forms the live
+    scene it equals the scene after the same edits applied one at a time (edits far apart, one
+    stream, a budget that admits one edit at a time), and the last one equals a bake."""
     inp, o = small()
     edits, after = _edit_stream(list(inp.lights), 10, seed=3)
-    stream = [(c, r, 0.002 * i) for i, (c, r) in enumerate(edits)]
     one = Service(ctx, inp, o)
     try:
-        lat1, up1, sum1 = one.replay_edits(stream, workers=1, budget_bytes=1, checksums=True)
+        spaced = [(c, r, 0.15 * i) for i, (c, r) in enumerate(edits)]
+        lat1, up1, sum1 = one.replay_edits(spaced, workers=1, budget_bytes=1, checksums=True)
     finally:
         one.close()
+    assert all(sum1), "edits far apart each form the live scene"
+    assert [u["reserved"] for u in up1] == [1] * len(edits)
     many = Service(ctx, inp, o)
     try:
-        lat4, up4, sum4 = many.replay_edits(stream, workers=4, checksums=True)
+        together = [(c, r, 0.0) for c, r in edits]
+        lat4, up4, sum4 = many.replay_edits(together, workers=4, checksums=True)
         print("latency ms, one at a time:", [round(x, 1) for x in lat1], flush=True)
-        print("latency ms, four streams:", [round(x, 1) for x in lat4], flush=True)
-        assert sum1 == sum4
+        print("latency ms, all at once on four streams:", [round(x, 1) for x in lat4], flush=True)
+        print("joint assemblies:", [u["reserved"] for u in up4], flush=True)
+        formed = [k for k, x in enumerate(sum4) if x]
+        assert formed and formed[-1] == len(edits) - 1
+        assert all(sum4[k] == sum1[k] for k in formed)
         assert [u["version"] for u in up4] == list(range(1, len(edits) + 1))
-        assert [u["dirty_meshes"] for u in up1] == [u["dirty_meshes"] for u in up4]
         assert all(x > 0 for x in lat4)
         final = inp.with_lights(sorted(after[-1].values(), key=lambda l: l.id))
         assert_scenes_equal(many.scene(), ctx.bake(final, o))
     finally:
         many.close()
-    # one edit applied alone gives the checksum the replay recorded after that edit
+    # the first edit applied alone gives the fingerprint recorded after it
     alone = Service(ctx, inp, o)
     try:
         c, r = edits[0]
         _, _, s = alone.replay_edits([(c, r, 0.0)], workers=1, checksums=True)
         assert s[0] == sum1[0]
+        alone.apply(*edits[1])
+        _, _, s = alone.replay_edits([(edits[2][0], edits[2][1], 0.0)], workers=2, checksums=True)
+        assert s[0] == sum1[2]
     finally:
         alone.close()
 
