@@ -50,13 +50,16 @@ uint32_t bounce_bits(int32_t c) { return c == DL_BOUNCE_BITS8 ? 8u : c == DL_BOU
 // texels those leave undecided.  Writes buried[i] and counts the kept texels per triangle.
 // With `counters` (6 words, counting on), counters[4..5] receive the node fetches and triangle
 // tests of the burial rays; *rays counts the rays cast and *ms the device time.
+// With `esc`, the rays that hit nothing are appended there (Capture::escaping's layout).
 hipError_t burial_streamed(dl_ctx* ctx, DevScene sc, const DevTris& tr, const DevMeshes& ms, const DevTexels& tx, uint64_t ntex,
                            uint64_t seed, uint8_t* buried, unsigned long long* kept, unsigned long long* counters,
-                           uint64_t* rays_cast, double* ms_spent) {
+                           uint64_t* rays_cast, double* ms_spent, const Capture* esc) {
   if (ntex == 0) return hipSuccess;
   if (ntex >= (1ull << 32)) return hipErrorInvalidValue;  // texels are listed by 32-bit index
   hipStream_t s = ctx->stream;
-  if (ctx->burial_lanes) {  // one lane per texel instead (counters: node fetches and triangle tests only)
+  float* esc_out = esc && esc->escaping ? reinterpret_cast<float*>(esc->escaping->p) : nullptr;
+  unsigned long long* esc_count = esc_out ? reinterpret_cast<unsigned long long*>(esc->escaping->p + 8 * esc->escaping_cap) : nullptr;
+  if (ctx->burial_lanes && !esc_out) {  // one lane per texel instead (counters: node fetches and triangle tests only)
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess) (void)hipEventRecord(ev[0], s);
     sc.counters = counters ? counters + 4 : nullptr;
@@ -92,9 +95,11 @@ hipError_t burial_streamed(dl_ctx* ctx, DevScene sc, const DevTris& tr, const De
     if ((e = hipMemsetAsync(ctr.p, 0, 3 * sizeof(unsigned long long), s)) ||
         (e = launch_burial_rays(s, tr, ms, tx, c0, nullptr, nullptr, n, 0, 4, seed, ctx->trig, rays.p, hits.p, total.p)) ||
         (e = launch_closest_stream(s, sc, rays.p, total.p, 4 * n, n, hits.p, q1, gl)) ||
+        (esc_out && (e = launch_burial_misses(s, rays.p, hits.p, total.p, 4 * n, esc_out, esc->escaping_cap, esc_count))) ||
         (e = launch_burial_decide(s, sc, tx, c0, nullptr, nullptr, n, 0, 4, rays.p, hits.p, back.p, buried, kept, list.p, ctr.p + 2)) ||
         (e = launch_burial_rays(s, tr, ms, tx, 0, list.p, ctr.p + 2, n, 4, 4, seed, ctx->trig, rays.p, hits.p, total.p)) ||
         (e = launch_closest_stream(s, sc, rays.p, total.p, 4 * n, n, hits.p, q2, gl)) ||
+        (esc_out && (e = launch_burial_misses(s, rays.p, hits.p, total.p, 4 * n, esc_out, esc->escaping_cap, esc_count))) ||
         (e = launch_burial_decide(s, sc, tx, 0, list.p, ctr.p + 2, n, 4, 4, rays.p, hits.p, back.p, buried, kept, nullptr, nullptr)))
       break;
     if (counters) {
@@ -517,6 +522,8 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     cap->cubes.clear();
     cap->crops = h_crop;
     cap->texels.assign(nm, 0);
+    cap->escaping_count = 0;
+    if (cap->escaping) HIP_TRY(hipMemsetAsync(cap->escaping->p + 8 * cap->escaping_cap, 0, 8, s));
   }
 
   // ---- regions (mesh_bake.rs:175-192) and bounce windows (bounce_map.rs:30-43) -----------------
@@ -757,7 +764,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(d_buried.alloc(ntex0 + 1));
       HIP_TRY(hipMemsetAsync(d_cnt.p, 0, ((size_t)nbt + 1) * 8, s));
       HIP_TRY(burial_streamed(ctx, sc, tr, ms, tx0, ntex0, opt.seed, d_buried.p, d_cnt.p, ctx->counting ? d_counters.p : nullptr,
-                              &burial_rays, &burial_ms));
+                              &burial_rays, &burial_ms, cap));
       HIP_TRY(d_tex_start.alloc((size_t)nbt + 1));
       HIP_TRY(scan_u64(s, reinterpret_cast<const uint64_t*>(d_cnt.p), d_tex_start.p, nbt, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
       HIP_TRY(download(ctx, h_tex_start, d_tex_start.p, (size_t)nbt + 1));
@@ -1551,6 +1558,11 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     resident->occ9 = std::move(occ9_local);
     resident->has_alpha = mesh_alpha;
     resident->ready = true;
+  }
+  if (cap && cap->escaping) {
+    unsigned long long n = 0;
+    HIP_TRY(copy_to_host(ctx, &n, cap->escaping->p + 8 * cap->escaping_cap, 8));
+    cap->escaping_count = n;
   }
   stage(4, 0);
   ctx->timings.total_ms = ms_since(t_start);

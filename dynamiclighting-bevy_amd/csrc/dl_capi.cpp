@@ -325,6 +325,10 @@ int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_opt
     svc->options = *options;
     dl_baked_scene* first = nullptr;
     dl::Capture cap;
+    cap.escaping = std::make_shared<dl::DevBlock>();  // for the isolation checks of rooms added later
+    cap.escaping_cap = dl::kEscapingCap;
+    if (cap.escaping->alloc(8 * cap.escaping_cap + 2, ctx->stream) != hipSuccess)
+      return set_err(ctx, DL_E_OOM, "allocating the burial ray record failed");
     int st = dl_bake_impl(ctx, scene, options, nullptr, &first, &svc->res, &cap);
     if (st != DL_OK) return st;
     std::unique_ptr<dl_baked_scene> hold(first);
@@ -347,6 +351,7 @@ int dl_service_bake(dl_service* svc, const dl_light* lights, uint32_t light_coun
   std::lock_guard<std::mutex> lock(ctx->mu);
   try {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (svc->parts.size() > 1) return set_err(ctx, DL_E_ARGUMENT, "dl_service_bake bakes services without added rooms");
     const dl_bake_input in{nullptr, 0, lights, light_count};
     return dl_bake_impl(ctx, &in, &svc->options, nullptr, out, &svc->res);
   } catch (const std::bad_alloc&) {
@@ -370,6 +375,7 @@ int dl_service_replay(dl_service* svc, const dl_service_request* requests, uint3
   std::lock_guard<std::mutex> lock(ctx->mu);
   try {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (svc->parts.size() > 1) return set_err(ctx, DL_E_ARGUMENT, "dl_service_replay serves services without added rooms");
     if (!prepare_workers(ctx, workers)) return set_err(ctx, DL_E_DEVICE, "creating a worker stream failed");
     for (uint32_t i = 0; i < count; ++i) {
       status[i] = DL_E_CANCELLED;
@@ -432,6 +438,7 @@ int dl_service_ray_queries_noalloc(dl_service* svc, const float* rays8, uint64_t
   if (ray_count && (!rays8 || !hit_or_index || (closest && !t_out))) return set_err(ctx, DL_E_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+  if (svc->parts.size() > 1) return set_err(ctx, DL_E_ARGUMENT, "ray queries see services without added rooms");
   const Resident& r = svc->res;
   DevScene sc{};
   sc.nodes = r.d_nodes.p;

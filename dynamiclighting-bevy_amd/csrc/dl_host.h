@@ -313,11 +313,18 @@ struct CapturedPair {
   std::vector<uint64_t> bounce_off, bounce_words;  // per kept triangle: its bounce payload in `words`
   std::shared_ptr<DevBlock> block, words;
 };
+// Burial rays that hit nothing kept per part of a rebake service (8 floats each: 128 MiB).
+constexpr uint64_t kEscapingCap = 1ull << 22;
 struct Capture {
   std::vector<CapturedPair> pairs;
   std::vector<std::pair<uint64_t, std::vector<uint32_t>>> cubes;  // distance cube words per local light id
   std::vector<uint32_t> crops;    // 4 per receiver triangle of the bake (scene order)
   std::vector<uint64_t> texels;   // per mesh
+  // Burial rays that hit nothing (8 floats each: origin, direction, +inf, 0), when `escaping` is
+  // set before the bake: it holds 8 * escaping_cap floats, then the 64-bit count of such rays
+  // (all of them, also those past the cap), which the bake leaves in escaping_count.
+  std::shared_ptr<DevBlock> escaping;
+  uint64_t escaping_cap = 0, escaping_count = 0;
 };
 }  // namespace dl
 
@@ -346,6 +353,13 @@ struct VisDelta {
   std::shared_ptr<DevBlock> block;
   uint64_t vis_off = 0, vis_words = 0;
 };
+// A part of a service's live scene: its first scene, or a room added later.  Its meshes are
+// live meshes mesh0 .. mesh0 + res->meshes.size() - 1; the bounds hold all its vertices.
+struct ScenePart {
+  Resident* res = nullptr;
+  uint32_t mesh0 = 0;
+  float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {0.0f, 0.0f, 0.0f};
+};
 struct LiveMesh {
   std::vector<LivePair> pairs;         // sorted by light id
   uint64_t w_off = 0, w_len = 0;       // the mesh's words in the live words
@@ -369,6 +383,18 @@ struct dl_service {
   std::vector<uint64_t> vc_off;
   std::vector<dl::VisDelta> vis_add, vis_sub;
   std::unique_ptr<dl::DevBlock> d_crops;        // the receiver triangles' crops on the device
+  // the live meshes of every part in mesh order (the first scene's, then each added room's):
+  // their meta, grid resolutions and first triangles (a scene-wide numbering, mesh count + 1)
+  std::vector<dl::MeshMeta> mesh_meta;
+  std::vector<uint32_t> mesh_res, first_tri;
+  uint64_t ntri = 0;
+  std::vector<dl::ScenePart> parts;              // part 0 is `res`
+  std::vector<std::unique_ptr<dl::Resident>> rooms;  // the residents of the added rooms
+  // burial rays of the parts that hit nothing in their own part (8 floats each, see
+  // Capture::escaping); esc_open when a part had more than its capture kept
+  std::shared_ptr<dl::DevBlock> esc;
+  uint64_t nesc = 0;
+  bool esc_open = false;
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
   std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words
   std::unordered_map<uint64_t, std::vector<uint32_t>> light_meshes;  // light id -> meshes holding its pairs

@@ -125,6 +125,12 @@ hipError_t launch_burial_lanes(hipStream_t s, DevScene scene, DevTris tris, DevM
 hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t first, const uint32_t* list,
                               const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, uint64_t seed,
                               TrigTables trig, float4* rays, float4* hits, uint64_t* total);
+// The rays of a burial pass (slots below *total, at most max_rays) that hit nothing, appended to
+// `out` as 8 floats each while fewer than `cap` are there; *count counts them all.
+hipError_t launch_burial_misses(hipStream_t s, const float4* rays, const float4* hits, const uint64_t* total, uint64_t max_rays,
+                                float* out, uint64_t cap, unsigned long long* count);
+// *count += the nonzero flags among n.
+hipError_t launch_count_nonzero(hipStream_t s, const uint32_t* flags, uint64_t n, unsigned long long* count);
 hipError_t launch_burial_decide(hipStream_t s, DevScene scene, DevTexels tx, uint64_t first, const uint32_t* list,
                                 const unsigned long long* count, uint64_t max_n, uint32_t k0, uint32_t nk, const float4* rays,
                                 const float4* hits, uint8_t* back_count, uint8_t* buried, unsigned long long* kept, uint32_t* next,

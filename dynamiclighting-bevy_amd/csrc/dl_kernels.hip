@@ -612,6 +612,35 @@ __global__ void k_burial_rays(DevTris tr, DevMeshes ms, DevTexels tx, uint64_t f
   }
 }
 
+// The rays of a burial pass that hit nothing (slots below *total), appended to `out` as 8 floats
+// (origin, direction, +inf, 0) while fewer than `cap` are there; *count counts them all.
+__global__ void k_burial_misses(const float4* __restrict__ rays, const float4* __restrict__ hits, const uint64_t* __restrict__ total,
+                                float* out, uint64_t cap, unsigned long long* count) {
+  const uint64_t n = *total;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    if (__float_as_uint(hits[r].w) != 0xffffffffu) continue;
+    const unsigned long long k = atomicAdd(count, 1ull);
+    if (k >= cap) continue;
+    const float4 a = rays[2 * r], b = rays[2 * r + 1];
+    float* o = out + 8 * k;
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = b.x;
+    o[4] = b.y;
+    o[5] = b.z;
+    o[6] = INFINITY;
+    o[7] = 0.0f;
+  }
+}
+
+// *count += the nonzero flags among n.
+__global__ void k_count_nonzero(const uint32_t* __restrict__ flags, uint64_t n, unsigned long long* count) {
+  uint32_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) c += flags[i] != 0u;
+  wave_add(count, c);
+}
+
 // Tallies rays k0 .. k0+nk-1 of each texel of a pass on the streamed hits: a hit whose triangle faces along the ray is a back face.  back[i] carries the back-face
 // count of the earlier rays (their front faces are the rest).  Decided texels get buried[i] and,
 // when kept, a count in kept[triangle]; undecided ones are appended to `next` (*next_n).
@@ -1694,6 +1723,18 @@ hipError_t launch_burial_rays(hipStream_t s, DevTris tris, DevMeshes meshes, Dev
   if (max_n == 0 || nk == 0 || k0 + nk > 8) return max_n == 0 ? hipSuccess : hipErrorInvalidValue;
   hipLaunchKernelGGL(k_burial_rays, dim3(grid_for(max_n, kThreads)), dim3(kThreads), 0, s, tris, meshes, tx, first, list, count,
                      max_n, k0, nk, seed, trig, rays, hits, total);
+  return hipGetLastError();
+}
+hipError_t launch_burial_misses(hipStream_t s, const float4* rays, const float4* hits, const uint64_t* total, uint64_t max_rays,
+                                float* out, uint64_t cap, unsigned long long* count) {
+  if (max_rays == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_burial_misses, dim3(grid_for(max_rays, kThreads)), dim3(kThreads), 0, s, rays, hits, total, out, cap, count);
+  return hipGetLastError();
+}
+hipError_t launch_count_nonzero(hipStream_t s, const uint32_t* flags, uint64_t n, unsigned long long* count) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_nonzero, dim3((uint32_t)std::min<uint64_t>(grid_for(n, kThreads), 4096)), dim3(kThreads), 0, s, flags, n,
+                     count);
   return hipGetLastError();
 }
 hipError_t launch_burial_decide(hipStream_t s, DevScene scene, DevTexels tx, uint64_t first, const uint32_t* list,

@@ -20,6 +20,7 @@
 // which gives exactly the words, stats and lights of bake() of the edited light set.
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -171,7 +172,6 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   ctx->cache.stream = s;
   const CacheScope cache_scope(&ctx->cache);  // stream-ordered buffers: no device-wide frees
   const dl_bake_options& opt = svc->options;
-  const Resident& res = svc->res;
   const uint32_t nm = (uint32_t)svc->mesh.size();
   PhaseClock clock;
   const Slots sl = slots_of(svc->lights);
@@ -205,7 +205,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       auto it = svc->slot.find(q.light_id);
       same = it != svc->slot.end() && it->second == slot.at(q.light_id);
     }
-    const uint32_t ft = res.first_tri[mi], nt = res.first_tri[mi + 1] - ft;
+    const uint32_t ft = svc->first_tri[mi], nt = svc->first_tri[mi + 1] - ft;
     if (same) {
       p.whole = true;
       p.len = m.w_len;
@@ -281,7 +281,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       const Entry& e = p.ent[j];
       const LivePair& q = m.pairs[e.pair];
       if (!dirty[i]) {
-        const uint32_t* c = &svc->crops[4 * (size_t)(res.first_tri[i] + e.tri)];
+        const uint32_t* c = &svc->crops[4 * (size_t)(svc->first_tri[i] + e.tri)];
         out.push_back({svc->W->p, m.w_off + q.shadow_off[e.k], base[i] + p.sh[j], shadow_words(c, q.bits)});
       }
       if (q.bounce_words[e.k]) {
@@ -327,7 +327,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     for (const VisDelta& v : list) {
       vaddr.push_back(reinterpret_cast<uint64_t>(v.block->p + v.vis_off));
       rect.insert(rect.end(), v.rect, v.rect + 4);
-      vres.push_back(res.res[v.mesh]);
+      vres.push_back(svc->mesh_res[v.mesh]);
       coff.push_back(svc->vc_off[v.mesh]);
       start.push_back(start.back() + v.vis_words);
     }
@@ -364,7 +364,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     for (uint32_t d = 0; d < nd; ++d) {
       const uint32_t i = dm[d];
       const LiveMesh& m = svc->mesh[i];
-      const uint32_t rs = res.res[i];
+      const uint32_t rs = svc->mesh_res[i];
       m_res32.push_back(rs);
       m_voff.push_back(m_voff.back() + ((uint64_t)rs * rs + 31) / 32);
       m_coff.push_back(svc->vc_off[i]);
@@ -397,7 +397,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       for (size_t j = 0; j < p.ent.size(); ++j) {
         const Entry& e = p.ent[j];
         const uint64_t k = dm_e0[d] + j;
-        k_tri[k] = res.first_tri[i] + e.tri;
+        k_tri[k] = svc->first_tri[i] + e.tri;
         k_region[k] = dm_r0[d] + used_of[d][e.pair];
         k_words[k] = shadow_words(&svc->crops[4 * (size_t)k_tri[k]], m.pairs[e.pair].bits);
         k_out[k] = base[i] + p.sh[j];
@@ -445,7 +445,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
           HIP_TRY(d_kid.alloc(nkept));
           HIP_TRY(launch_iota_u64(s, d_kid.p, nkept));
           DevTris tr{};
-          tr.n = res.ntri;
+          tr.n = (uint32_t)svc->ntri;
           tr.crop = svc->d_crops->p;
           DevEntries ent{nkept, nullptr, d_kregion.p, d_ktri.p, nullptr};
           HIP_TRY(launch_crop(s, tr, ent, rg, d_kid.p, d_kstart.p, nkept, k_start.back(), d_kout.p, W2->p));
@@ -567,13 +567,236 @@ void merge(dl_service* svc, Capture& cap, const std::unordered_set<uint64_t>& re
 }
 
 // Bakes `lights` (the changed ones) against the resident geometry and captures their pairs.
+// Bounds of every vertex of a bake input; false when it has none.
+bool bounds_of(const dl_bake_input* in, float bmin[3], float bmax[3]) {
+  bool any = false;
+  for (int a = 0; a < 3; ++a) {
+    bmin[a] = INFINITY;
+    bmax[a] = -INFINITY;
+  }
+  for (uint32_t m = 0; m < in->mesh_count; ++m) {
+    const dl_mesh& mesh = in->meshes[m];
+    if (!mesh.positions) continue;
+    for (uint32_t v = 0; v < mesh.vertex_count; ++v)
+      for (int a = 0; a < 3; ++a) {
+        const float x = mesh.positions[3 * (size_t)v + a];
+        if (!std::isfinite(x)) continue;
+        bmin[a] = std::min(bmin[a], x);
+        bmax[a] = std::max(bmax[a], x);
+        any = true;
+      }
+  }
+  return any;
+}
+
+// Whether the ball of a light's range (which holds its emitter, its rays and its photons) comes
+// within a small margin of a part's bounds.
+bool reaches(const dl_light& l, const ScenePart& part) {
+  double d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double margin = 1e-3 + 1e-5 * std::fabs((double)part.bmin[a]) + 1e-5 * std::fabs((double)part.bmax[a]);
+    const double lo = (double)part.bmin[a] - margin, hi = (double)part.bmax[a] + margin, x = l.position[a];
+    const double g = x < lo ? lo - x : x > hi ? x - hi : 0.0;
+    d2 += g * g;
+  }
+  const double r = (double)l.radius * (1.0 + 1e-5) + 1e-3;
+  return d2 <= r * r;
+}
+
+bool boxes_meet(const ScenePart& a, const ScenePart& b) {
+  for (int k = 0; k < 3; ++k)
+    if (a.bmax[k] + 1e-3f < b.bmin[k] || b.bmax[k] + 1e-3f < a.bmin[k]) return false;
+  return true;
+}
+
+// The part whose bounds a light reaches (0 when it reaches none: it then meets no geometry at
+// all, so any part traces it alike); -1 when it reaches two or more.
+int part_of(const dl_service* svc, const dl_light& l) {
+  int found = -2;
+  for (size_t k = 0; k < svc->parts.size(); ++k)
+    if (reaches(l, svc->parts[k])) found = found == -2 ? (int)k : -1;
+  return found == -2 ? 0 : found;
+}
+
+DevScene scene_of(const Resident& r) {
+  DevScene sc{};
+  sc.nodes = r.d_nodes.p;
+  sc.tris = reinterpret_cast<const float4*>(r.d_occ.p);
+  sc.ntris = r.nocc;
+  sc.stack = 3 * std::max(1u, r.depth) + 1;
+  sc.stack16 = r.nodes < 65536 ? 1u : 0u;
+  return sc;
+}
+
+// How many of n rays (8 floats each, from device address rays8) hit some occluder of a part.
+int rays_hitting(dl_ctx* ctx, const Resident& r, const float* rays8, uint64_t n, uint64_t* hits) {
+  *hits = 0;
+  if (n == 0 || r.nocc == 0) return DL_OK;
+  ctx->cache.stream = ctx->stream;
+  const CacheScope cache_scope(&ctx->cache);
+  DBuf<uint32_t> flags;
+  DBuf<unsigned long long> count;
+  DBuf<float> unused_t;
+  HIP_TRY(flags.alloc(n));
+  HIP_TRY(zeroed(ctx, count, 1));
+  HIP_TRY(launch_ray_queries(ctx->stream, scene_of(r), rays8, n, 0, unused_t.p, flags.p));
+  HIP_TRY(launch_count_nonzero(ctx->stream, flags.p, n, count.p));
+  unsigned long long c = 0;
+  HIP_TRY(copy_to_host(ctx, &c, count.p, 8));
+  *hits = c;
+  return DL_OK;
+}
+
+// Bakes `lights` (the changed ones) against the resident geometry of the parts they reach and
+// captures their pairs (mesh indices are live ones).
 int trace_lights(dl_ctx* w, dl_service* svc, const std::vector<dl_light>& lights, Capture& cap) {
   if (lights.empty()) return DL_OK;
-  const dl_bake_input in{nullptr, 0, lights.data(), (uint32_t)lights.size()};
+  std::vector<std::vector<dl_light>> by_part(svc->parts.size());
+  for (const dl_light& l : lights) {
+    const int k = part_of(svc, l);
+    if (k < 0) return set_error(w, DL_E_ARGUMENT, "light " + hex_id(l.id) + " reaches two parts of the live scene (its rooms)");
+    by_part[k].push_back(l);
+  }
+  for (size_t k = 0; k < by_part.size(); ++k) {
+    if (by_part[k].empty()) continue;
+    const dl_bake_input in{nullptr, 0, by_part[k].data(), (uint32_t)by_part[k].size()};
+    dl_baked_scene* sc = nullptr;
+    Capture part;
+    const int st = dl_bake_impl(w, &in, &svc->options, nullptr, &sc, svc->parts[k].res, &part);
+    delete sc;
+    if (st != DL_OK) return st;
+    for (CapturedPair& c : part.pairs) {
+      c.mesh += svc->parts[k].mesh0;
+      cap.pairs.push_back(std::move(c));
+    }
+    for (auto& c : part.cubes) cap.cubes.push_back(std::move(c));
+  }
+  return DL_OK;
+}
+
+// ---- rooms -----------------------------------------------------------------------------------
+// A room joins the live scene as a part of its own.  The bake of the whole scene equals the live
+// scene with the room's bake beside it when nothing that bake computes for one part reaches the
+// other: no light's range reaches the other part's bounds (shadow rays, area samples, photons and
+// gather rays stay inside a light's range ball) and no burial ray (unbounded) that leaves its own
+// part hits the other.  join_room checks exactly these, with the burial rays that hit nothing in
+// their own part recorded when each part was baked, and refuses the room otherwise.
+
+struct RoomTrace {
+  std::unique_ptr<Resident> res;
+  Capture cap;
+  ScenePart part;
+};
+
+// Phase 1 of a room: its bake alone (on a worker), with its pairs and escaping rays captured.
+int trace_room(dl_ctx* w, dl_service* svc, const dl_bake_input* room, RoomTrace& t) {
+  dl_ctx* ctx = w;  // errors are the worker's
+  if (!room || room->mesh_count == 0) return set_error(w, DL_E_ARGUMENT, "a room needs meshes");
+  if (!bounds_of(room, t.part.bmin, t.part.bmax)) return set_error(w, DL_E_ARGUMENT, "the room has no finite vertex");
+  t.res.reset(new Resident());
+  t.cap.escaping = std::make_shared<DevBlock>();
+  t.cap.escaping_cap = kEscapingCap;
+  HIP_TRY(t.cap.escaping->alloc(8 * kEscapingCap + 2, w->stream));
   dl_baked_scene* sc = nullptr;
-  const int st = dl_bake_impl(w, &in, &svc->options, nullptr, &sc, &svc->res, &cap);
+  const int st = dl_bake_impl(w, room, &svc->options, nullptr, &sc, t.res.get(), &t.cap);
   delete sc;
-  return st;
+  if (st != DL_OK) return st;
+  HIP_TRY(hipStreamSynchronize(w->stream));
+  t.part.res = t.res.get();
+  return DL_OK;
+}
+
+// Phase 2 of a room: the isolation checks, then its meshes, lights and pairs join the live state
+// (the caller assembles).  Marks the room's meshes dirty (`dirty` grows with the meshes).
+int join_room(dl_service* svc, const dl_bake_input* room, RoomTrace& t, std::vector<uint8_t>& dirty,
+              std::unordered_set<uint64_t>& added_lights) {
+  dl_ctx* ctx = svc->ctx;
+  const Resident& rr = *t.res;
+  if (rr.meshes.empty()) return set_error(ctx, DL_E_ARGUMENT, "the room has no meshes");
+  if (!svc->mesh_meta.empty() && rr.meshes.front().id <= svc->mesh_meta.back().id)
+    return set_error(ctx, DL_E_ARGUMENT, "a room's mesh ids must be above every live mesh id");
+  std::unordered_set<uint64_t> ids;
+  for (const dl_light& l : svc->lights) ids.insert(l.id);
+  for (uint32_t k = 0; k < room->light_count; ++k) {
+    const dl_light& l = room->lights[k];
+    if (l.shape == DL_SHAPE_SUN || l.shape == DL_SHAPE_SKY) return set_error(ctx, DL_E_ARGUMENT, "a room's lights must be local lights");
+    if (!ids.insert(l.id).second) return set_error(ctx, DL_E_DUPLICATE_ID, "two lights share BakeId " + hex_id(l.id));
+    for (const ScenePart& p : svc->parts)
+      if (reaches(l, p)) return set_error(ctx, DL_E_ARGUMENT, "the room's light " + hex_id(l.id) + " reaches the live scene");
+  }
+  for (const dl_light& l : svc->lights) {
+    if (l.shape == DL_SHAPE_SUN || l.shape == DL_SHAPE_SKY)
+      return set_error(ctx, DL_E_ARGUMENT, "rooms join live scenes of local lights only (light " + hex_id(l.id) + ")");
+    if (reaches(l, t.part)) return set_error(ctx, DL_E_ARGUMENT, "live light " + hex_id(l.id) + " reaches the room");
+  }
+  for (const ScenePart& p : svc->parts)
+    if (boxes_meet(p, t.part)) return set_error(ctx, DL_E_ARGUMENT, "the room's bounds meet the live scene's");
+  if (svc->esc_open || t.cap.escaping_count > t.cap.escaping_cap)
+    return set_error(ctx, DL_E_ARGUMENT, "too many burial rays leave a part of the scene to check a room's isolation");
+  uint64_t hits = 0;
+  int st = rays_hitting(ctx, rr, reinterpret_cast<const float*>(svc->esc ? svc->esc->p : nullptr), svc->nesc, &hits);
+  if (st != DL_OK) return st;
+  if (hits) return set_error(ctx, DL_E_ARGUMENT, "burial rays of the live scene reach the room");
+  for (const ScenePart& p : svc->parts) {
+    st = rays_hitting(ctx, *p.res, reinterpret_cast<const float*>(t.cap.escaping->p), t.cap.escaping_count, &hits);
+    if (st != DL_OK) return st;
+    if (hits) return set_error(ctx, DL_E_ARGUMENT, "burial rays of the room reach the live scene");
+  }
+
+  // the room's meshes, triangles and visit counts join the live scene's
+  hipStream_t s = ctx->stream;
+  const uint32_t mesh0 = (uint32_t)svc->mesh.size(), nrm = (uint32_t)rr.meshes.size();
+  const uint64_t tri0 = svc->ntri;
+  for (uint32_t i = 0; i < nrm; ++i) {
+    svc->mesh_meta.push_back(rr.meshes[i]);
+    svc->mesh_res.push_back(rr.res[i]);
+    svc->first_tri.push_back((uint32_t)(tri0 + rr.first_tri[i + 1]));
+    LiveMesh m;
+    m.texels = t.cap.texels[i];
+    svc->mesh.push_back(std::move(m));
+    svc->stats.texels += t.cap.texels[i];
+  }
+  svc->ntri += rr.ntri;
+  svc->stats.meshes += nrm;
+  svc->stats.triangles += rr.ntri;
+  svc->crops.insert(svc->crops.end(), t.cap.crops.begin(), t.cap.crops.end());
+  {
+    std::vector<uint64_t> off(svc->vc_off);
+    for (uint32_t i = 0; i < nrm; ++i) off.push_back(off.back() + ((uint64_t)rr.res[i] * rr.res[i] + 1) / 2);
+    std::unique_ptr<DevBlock> vc(new DevBlock()), crops(new DevBlock());
+    HIP_TRY(vc->alloc(off.back() + 1, s));
+    HIP_TRY(hipMemcpyAsync(vc->p, svc->vc->p, svc->vc_off.back() * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemsetAsync(vc->p + svc->vc_off.back(), 0, (off.back() + 1 - svc->vc_off.back()) * 4, s));
+    HIP_TRY(crops->alloc(svc->crops.size() + 1, s));
+    HIP_TRY(hipMemcpyAsync(crops->p, svc->crops.data(), svc->crops.size() * 4, hipMemcpyHostToDevice, s));
+    // escaping rays: the live scene's, then the room's
+    std::shared_ptr<DevBlock> esc = std::make_shared<DevBlock>();
+    const uint64_t nesc = svc->nesc + t.cap.escaping_count;
+    HIP_TRY(esc->alloc(8 * nesc + 1, s));
+    if (svc->nesc) HIP_TRY(hipMemcpyAsync(esc->p, svc->esc->p, svc->nesc * 32, hipMemcpyDeviceToDevice, s));
+    if (t.cap.escaping_count)
+      HIP_TRY(hipMemcpyAsync(esc->p + 8 * svc->nesc, t.cap.escaping->p, t.cap.escaping_count * 32, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    svc->vc = std::move(vc);
+    svc->vc_off = std::move(off);
+    svc->d_crops = std::move(crops);
+    svc->esc = std::move(esc);
+    svc->nesc = nesc;
+  }
+  t.part.mesh0 = mesh0;
+  svc->parts.push_back(t.part);
+  svc->rooms.push_back(std::move(t.res));
+  // its lights and pairs
+  for (uint32_t k = 0; k < room->light_count; ++k) {
+    svc->lights.push_back(room->lights[k]);
+    added_lights.insert(room->lights[k].id);
+  }
+  std::stable_sort(svc->lights.begin(), svc->lights.end(), [](const dl_light& a, const dl_light& b) { return a.id < b.id; });
+  for (CapturedPair& c : t.cap.pairs) c.mesh += mesh0;
+  dirty.resize(svc->mesh.size(), 0);
+  for (uint32_t i = mesh0; i < svc->mesh.size(); ++i) dirty[i] = 1;
+  merge(svc, t.cap, {}, dirty);
+  return DL_OK;
 }
 
 // Checks an edit and returns the edited light set and the ids whose pairs it replaces.
@@ -620,6 +843,18 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->mesh.assign(svc->res.meshes.size(), LiveMesh{});
   for (size_t i = 0; i < svc->mesh.size(); ++i) svc->mesh[i].texels = cap.texels[i];
   svc->crops = cap.crops;
+  svc->mesh_meta = svc->res.meshes;
+  svc->mesh_res = svc->res.res;
+  svc->first_tri = svc->res.first_tri;
+  svc->ntri = svc->res.ntri;
+  svc->rooms.clear();
+  svc->parts.assign(1, ScenePart{});
+  svc->parts[0].res = &svc->res;
+  if (!bounds_of(scene, svc->parts[0].bmin, svc->parts[0].bmax))
+    for (int a = 0; a < 3; ++a) svc->parts[0].bmin[a] = svc->parts[0].bmax[a] = 0.0f;
+  svc->esc = cap.escaping;
+  svc->nesc = cap.escaping ? std::min(cap.escaping_count, cap.escaping_cap) : 0;
+  svc->esc_open = !cap.escaping || cap.escaping_count > cap.escaping_cap;
   svc->W.reset();
   svc->slot.clear();
   svc->light_meshes.clear();
@@ -629,7 +864,7 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
     hipStream_t s = ctx->stream;
     svc->vc_off.assign(svc->mesh.size() + 1, 0);
     for (size_t i = 0; i < svc->mesh.size(); ++i)
-      svc->vc_off[i + 1] = svc->vc_off[i] + ((uint64_t)svc->res.res[i] * svc->res.res[i] + 1) / 2;
+      svc->vc_off[i + 1] = svc->vc_off[i] + ((uint64_t)svc->mesh_res[i] * svc->mesh_res[i] + 1) / 2;
     svc->vc.reset(new DevBlock());
     HIP_TRY(svc->vc->alloc(svc->vc_off.back() + 1, s));
     HIP_TRY(hipMemsetAsync(svc->vc->p, 0, (svc->vc_off.back() + 1) * 4, s));
@@ -641,7 +876,7 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->stride = 0;  // forces every mesh to be written
   svc->stats = dl_bake_stats{};
   svc->stats.meshes = (uint32_t)svc->mesh.size();
-  svc->stats.triangles = svc->res.ntri;
+  svc->stats.triangles = svc->ntri;
   for (const LiveMesh& m : svc->mesh) svc->stats.texels += m.texels;
   std::vector<uint8_t> dirty(svc->mesh.size(), 1);
   merge(svc, cap, {}, dirty);
@@ -679,6 +914,33 @@ int dl_service_apply(dl_service* svc, const dl_service_edit* edit, dl_service_up
   try {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, DL_E_DEVICE, "hipSetDevice failed");
     const auto t0 = std::chrono::steady_clock::now();
+    if (edit->room) {
+      if (edit->light_count || edit->remove_count) return set_error(ctx, DL_E_ARGUMENT, "an edit adds a room or edits lights, not both");
+      RoomTrace t;
+      int st = trace_room(ctx, svc, edit->room, t);
+      if (st != DL_OK) return st;
+      const auto t1 = std::chrono::steady_clock::now();
+      std::vector<uint8_t> dirty(svc->mesh.size(), 0);
+      std::unordered_set<uint64_t> added;
+      if ((st = join_room(svc, edit->room, t, dirty, added)) != DL_OK) return st;
+      if ((st = assemble(svc, dirty)) != DL_OK) return st;
+      svc->version++;
+      if (info) {
+        *info = dl_service_update{};
+        info->version = svc->version;
+        for (uint8_t d : dirty) info->dirty_meshes += d;
+        info->traced_pairs = t.cap.pairs.size();
+        for (const CapturedPair& c : t.cap.pairs) {
+          info->traced_samples += c.samples;
+          info->traced_rays += c.rays;
+          info->traced_bounce_rays += c.bounce_rays;
+        }
+        info->trace_ms = ms_between(t0, t1);
+        info->assemble_ms = ms_between(t1, std::chrono::steady_clock::now());
+        info->total_ms = ms_between(t0, std::chrono::steady_clock::now());
+      }
+      return DL_OK;
+    }
     std::vector<dl_light> next;
     std::unordered_set<uint64_t> replaced;
     int st = edited_lights(svc, *edit, next, replaced);
@@ -738,10 +1000,10 @@ int dl_service_scene(dl_service* svc, dl_baked_scene** out) {
     sc->meshes.resize(svc->mesh.size());
     for (size_t i = 0; i < svc->mesh.size(); ++i) {
       dl_baked_mesh_out& mo = sc->meshes[i];
-      mo.id = svc->res.meshes[i].id;
-      mo.content_hash = svc->res.meshes[i].content_hash;
-      mo.resolution = svc->res.res[i];
-      mo.occludes = svc->res.meshes[i].occludes;
+      mo.id = svc->mesh_meta[i].id;
+      mo.content_hash = svc->mesh_meta[i].content_hash;
+      mo.resolution = svc->mesh_res[i];
+      mo.occludes = svc->mesh_meta[i].occludes;
       mo.words.share(words, words->data() + svc->mesh[i].w_off, svc->mesh[i].w_len);
     }
     if (svc->options.dynamic_geometry != DL_DYNAMIC_LIGHTING_ONLY)
@@ -839,6 +1101,7 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
     }
     struct Slot {
       Capture cap;
+      RoomTrace room;
       int status = DL_E_CANCELLED;
       std::string err;
       bool done = false;
@@ -881,7 +1144,13 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
         std::vector<dl_light> changed(edits[i].lights, edits[i].lights + edits[i].light_count);
         int st;
         try {
-          st = trace_lights(w, svc, changed, sl.cap);
+          if (edits[i].room) {
+            st = edits[i].light_count || edits[i].remove_count
+                     ? set_error(w, DL_E_ARGUMENT, "an edit adds a room or edits lights, not both")
+                     : trace_room(w, svc, edits[i].room, sl.room);
+          } else {
+            st = trace_lights(w, svc, changed, sl.cap);
+          }
           if (st == DL_OK && hipStreamSynchronize(w->stream) != hipSuccess) st = set_error(w, DL_E_DEVICE, "tracing an edit failed");
         } catch (const std::bad_alloc&) {
           w->err = "host allocation failed";
@@ -924,6 +1193,17 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
           st = set_error(ctx, sl.status, sl.err);
           break;
         }
+        if (edits[k].room) {
+          std::unordered_set<uint64_t> added;
+          if ((st = join_room(svc, edits[k].room, sl.room, dirty, added)) != DL_OK) break;
+          applied = k + 1;
+          continue;
+        }
+        // the parts the edit's lights reach may have grown since they were traced
+        for (uint32_t q = 0; q < edits[k].light_count && st == DL_OK; ++q)
+          if (svc->parts.size() > 1 && part_of(svc, edits[k].lights[q]) < 0)
+            st = set_error(ctx, DL_E_ARGUMENT, "light " + hex_id(edits[k].lights[q].id) + " reaches two parts of the live scene (its rooms)");
+        if (st != DL_OK) break;
         std::vector<dl_light> next_lights;
         std::unordered_set<uint64_t> replaced;
         if ((st = edited_lights(svc, edits[k], next_lights, replaced)) != DL_OK) break;
@@ -952,8 +1232,9 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
           info[k].version = svc->version + (k - i) + 1;
           info[k].dirty_meshes = ndirty;
           info[k].reserved = applied - i;  // edits that joined the scene together
-          info[k].traced_pairs = sl.cap.pairs.size();
-          for (const CapturedPair& c : sl.cap.pairs) {
+          const Capture& kc = edits[k].room ? sl.room.cap : sl.cap;
+          info[k].traced_pairs = kc.pairs.size();
+          for (const CapturedPair& c : kc.pairs) {
             info[k].traced_samples += c.samples;
             info[k].traced_rays += c.rays;
             info[k].traced_bounce_rays += c.bounce_rays;
@@ -965,6 +1246,7 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
         }
         if (checksum) checksum[k] = 0;
         sl.cap = Capture{};
+        sl.room = RoomTrace{};
       }
       svc->version += applied - i;
       if (checksum && applied > i) {

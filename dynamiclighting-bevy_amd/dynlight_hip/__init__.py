@@ -480,7 +480,17 @@ class Service:
         id, `remove` lists ids to drop.  Returns the update record."""
         packed = BakeInput([], list(lights))
         rm = (C.c_uint64 * max(1, len(remove)))(*remove)
-        e = abi.dl_service_edit(packed.struct.lights, packed.struct.light_count, rm, len(remove), 0.0)
+        e = abi.dl_service_edit(packed.struct.lights, packed.struct.light_count, rm, len(remove), 0.0, None)
+        u = abi.dl_service_update()
+        st = self.lib.dl_service_apply(self.handle, C.byref(e), C.byref(u))
+        if st != 0:
+            raise BakeError(st, self.ctx.error())
+        return {name: getattr(u, name) for name, _ in u._fields_}
+
+    def add_room(self, room: BakeInput) -> dict:
+        """Adds a room to the live scene (dl_service_apply with a room): a scene of its own that
+        joins as a part of the live scene when isolated from it.  Returns the update record."""
+        e = abi.dl_service_edit(None, 0, None, 0, 0.0, C.cast(C.pointer(room.struct), C.c_void_p))
         u = abi.dl_service_update()
         st = self.lib.dl_service_apply(self.handle, C.byref(e), C.byref(u))
         if st != 0:
@@ -488,15 +498,18 @@ class Service:
         return {name: getattr(u, name) for name, _ in u._fields_}
 
     def replay_edits(self, edits, workers: int = 4, budget_bytes: int = 0, checksums: bool = False):
-        """Serves edits (lights, remove, arrival_s) as they arrive (dl_service_replay_edits): tracing
-        on `workers` streams under a device-memory budget, joining the live scene in arrival order.
-        Returns (latencies in ms, update records, checksums or None)."""
+        """Serves edits as they arrive (dl_service_replay_edits): (lights, remove, arrival_s) light
+        edits, or (room BakeInput, None, arrival_s) room additions; tracing on `workers` streams
+        under a device-memory budget, joining the live scene in arrival order.  Returns
+        (latencies in ms, update records, checksums or None)."""
         n = len(edits)
-        packed = [BakeInput([], list(l)) for l, _, _ in edits]
-        rms = [(C.c_uint64 * max(1, len(r)))(*r) for _, r, _ in edits]
+        rooms = [e[0] if isinstance(e[0], BakeInput) else None for e in edits]
+        packed = [BakeInput([], []) if rm is not None else BakeInput([], list(e[0])) for rm, e in zip(rooms, edits)]
+        rms = [(C.c_uint64 * max(1, len(e[1] or ())))(*(e[1] or ())) for e in edits]
         arr = (abi.dl_service_edit * max(1, n))()
-        for i, (p, rm, (_, r, t)) in enumerate(zip(packed, rms, edits)):
-            arr[i] = abi.dl_service_edit(p.struct.lights, p.struct.light_count, rm, len(r), float(t))
+        for i, (p, rm, e) in enumerate(zip(packed, rms, edits)):
+            room = C.cast(C.pointer(rooms[i].struct), C.c_void_p) if rooms[i] is not None else None
+            arr[i] = abi.dl_service_edit(p.struct.lights, p.struct.light_count, rm, len(e[1] or ()), float(e[2]), room)
         lat = (C.c_double * max(1, n))()
         sums = (C.c_uint64 * max(1, n))()
         info = (abi.dl_service_update * max(1, n))()

@@ -97,7 +97,7 @@ class dl_id_candidate(C.Structure):
 
 class dl_service_edit(C.Structure):
     _fields_ = [("lights", C.c_void_p), ("light_count", C.c_uint32), ("remove", C.POINTER(C.c_uint64)),
-                ("remove_count", C.c_uint32), ("arrival_s", C.c_double)]
+                ("remove_count", C.c_uint32), ("arrival_s", C.c_double), ("room", C.c_void_p)]
 
 
 class dl_service_update(C.Structure):
@@ -388,6 +388,20 @@ class BakeInput:
         other._pack_lights()
         other._make_struct()
         return other
+
+    def joined(self, other: "BakeInput") -> "BakeInput":
+        """This input's meshes and lights followed by `other`'s (buffers are shared)."""
+        j = BakeInput.__new__(BakeInput)
+        j.meshes = None
+        j.lights = list(self.lights) + list(other.lights)
+        j._keep = [self, other]
+        j._alphas = []
+        j._mesh_arr = np.zeros(len(self._mesh_arr) + len(other._mesh_arr), dtype=self._mesh_arr.dtype)  # the C layout
+        j._mesh_arr[:len(self._mesh_arr)] = self._mesh_arr
+        j._mesh_arr[len(self._mesh_arr):] = other._mesh_arr
+        j._pack_lights()
+        j._make_struct()
+        return j
 
     def subset(self, keep: np.ndarray) -> "BakeInput":
         """A view with only the meshes where `keep` is true (buffers are shared)."""

@@ -508,6 +508,65 @@ def batch(count: int = 512, seed: int = 4):
     return [j[0] for j in jobs], [j[1] for j in jobs]
 
 
+def service_room(k: int, seed: int = 6, origin=None) -> BakeInput:
+    """Room k of the serve workload's room requests: one room of the batch workload's generator
+    (``batch_job``), a closed 6x3x6 m shell filled with boxes and pillars to a log-uniform
+    triangle count in [500, 3000] (a batch room's share), a log-uniform light count in [1, 64]
+    (40% point, 30% spot baked as point, 30% rectangle/tube), range U(4, 12), single bounce for
+    odd k.  It stands 100 + 80k m along -x from the interior, so that no light or burial ray of
+    one reaches the other; mesh ids from 10^7 + 10^5 k, light ids from 2 * 10^7 + 10^4 k (above
+    the interior's and every earlier room's).  `origin` (x, z) places the room elsewhere."""
+    rng = Pcg32.seeded(splitmix64((seed ^ (k * 0x9E3779B1)) & MASK64))
+    tris = int(round(math.exp(rng.uniform(math.log(500.0), math.log(3000.0)))))
+    nlights = min(64, int(math.exp(rng.uniform(0.0, math.log(65.0)))))
+    W, H, D = 6.0, 3.0, 6.0
+    ox, oz = (-100.0 - 80.0 * k, 0.0) if origin is None else (float(origin[0]), float(origin[1]))
+    p, n, idx, uv = _room_shell(ox, oz, W, H, D, {})
+    positions, normals, indices, uvs = [p], [n], [idx], [uv]
+    boxes = max(1, (tris - len(idx) // 3) // 12)
+    u = rng.uniforms(boxes * 9).reshape(boxes, 9)
+    pillar = u[:, 1] < 0.3
+    ext = np.where(pillar[:, None], np.stack([0.2 + 0.8 * u[:, 2], np.full(boxes, H), 0.2 + 0.8 * u[:, 4]], 1),
+                   0.2 + 2.8 * u[:, 2:5])
+    cxs = ox + ext[:, 0] / 2 + u[:, 5] * (W - ext[:, 0])
+    czs = oz + ext[:, 2] / 2 + u[:, 6] * (D - ext[:, 2])
+    mn = np.stack([cxs - ext[:, 0] / 2, np.zeros(boxes), czs - ext[:, 2] / 2], 1).astype(np.float32)
+    mx = np.stack([cxs + ext[:, 0] / 2, ext[:, 1], czs + ext[:, 2] / 2], 1).astype(np.float32)
+    albedo = np.concatenate([np.full((1, 3), 0.8, np.float32),
+                             (0.1 + 0.8 * rng.uniforms(boxes * 3).reshape(boxes, 3)).astype(np.float32)])
+    local, tnrm, tidx, tuv = _TEMPLATE
+    box_pos = ((mn + mx)[:, None, :] * np.float32(0.5) + local[None] * (mx - mn)[:, None, :] * np.float32(0.5))
+    vcount = np.concatenate([[0, len(p)], len(p) + len(local) * np.arange(1, boxes + 1)])
+    icount = np.concatenate([[0, len(idx)], len(idx) + len(tidx) * np.arange(1, boxes + 1)])
+    positions.append(box_pos.reshape(-1, 3).astype(np.float32))
+    normals.append(np.tile(tnrm, (boxes, 1)))
+    indices.append(np.tile(tidx, boxes))
+    uvs.append(np.tile(tuv, (boxes, 1)))
+    ids = np.arange(10_000_000 + 100_000 * k, 10_000_000 + 100_000 * k + 1 + boxes, dtype=np.uint64)
+    light_list = []
+    for q in range(nlights):
+        kind = rng.uniform()
+        lp = (ox + rng.uniform(0.1, W - 0.1), rng.uniform(0.1, H - 0.1), oz + rng.uniform(0.1, D - 0.1))
+        radius = rng.uniform(4.0, 12.0)
+        lid = 20_000_000 + 10_000 * k + q
+        if kind < 0.7:
+            light = point_light(lid, lp, radius)
+        else:
+            length = rng.uniform(0.5, 2.0)
+            if rng.uniform() < 0.5:
+                light = area_light(lid, lp, radius, quat_from_rotation_x(-math.pi / 2), (length, rng.uniform(0.5, 2.0)))
+            else:
+                yaw = rng.uniform(0.0, math.pi)
+                light = area_light(lid, lp, radius, (0.0, math.sin(yaw / 2), 0.0, math.cos(yaw / 2)), (length, 0.0))
+        if k % 2 == 1:
+            light.illumination = ILLUMINATION_SINGLE_BOUNCE
+        light_list.append(light)
+    return BakeInput.from_arrays(
+        ids=ids, positions=np.concatenate(positions), indices=np.concatenate(indices), uv1=np.concatenate(uvs),
+        normals=np.concatenate(normals), vertex_offsets=vcount.astype(np.int64), index_offsets=icount.astype(np.int64),
+        albedo=albedo, content_hash=np.array([splitmix64(int(i)) for i in ids], np.uint64), lights=light_list)
+
+
 # ---- serve: rebake requests against the resident interior -------------------------------------
 def serve_requests(inp: BakeInput, count: int = 2000, seed: int = 5, rate: float = 20.0, grid: int = 8,
                    room=(6.0, 3.0, 6.0), room_share: float = 0.2):

@@ -338,6 +338,13 @@ typedef struct dl_service_edit {
     const uint64_t* remove;   /* ids of lights to remove                                       */
     uint32_t remove_count;
     double arrival_s;         /* for dl_service_replay_edits: arrival in seconds (sorted)      */
+    /* Non-NULL (and no lights or removals): the edit adds this room, a scene of its own (local
+     * lights only, mesh ids above every live one) that joins the live scene as a part of its own.
+     * The result is the bake of the whole scene: the room is refused (DL_E_ARGUMENT) unless no
+     * light of one part reaches the other's bounds and no burial ray that leaves its own part
+     * hits the other.  Light edits then trace each light against the part it reaches (a light
+     * that reaches two parts is refused). */
+    const dl_bake_input* room;
 } dl_service_edit;
 typedef struct dl_service_update {
     uint64_t version;         /* edits applied to the live scene so far                        */

@@ -214,3 +214,59 @@ def test_replay_errors_keep_the_edits_before_them(ctx):
         assert_scenes_equal(svc.scene(), ctx.bake(inp.with_lights(sorted(current.values(), key=lambda l: l.id)), o))
     finally:
         svc.close()
+
+
+def test_added_rooms_join_the_live_scene_as_a_bake_of_the_whole(ctx):
+    """Room requests: a room (a scene of its own, isolated from the live scene) joins it; the live
+    scene then equals the bake of the whole scene (the oracle's too, for the first room), and
+    light edits that follow, in the room and in the first scene, keep it so."""
+    inp, o = small()
+    svc = Service(ctx, inp, o)
+    try:
+        room0, room1 = scenes.service_room(0), scenes.service_room(1)
+        info = svc.add_room(room0)
+        print(f"room 0: {room0.mesh_count} meshes, {len(room0.lights)} lights, {info['total_ms']:.1f} ms", flush=True)
+        whole = inp.joined(room0)
+        assert_scenes_equal(svc.scene(), ctx.bake(whole, o))
+        assert_scenes_equal(svc.scene(), oracle.bake(whole, o, threads=8))
+        info = svc.add_room(room1)
+        whole = whole.joined(room1)
+        assert_scenes_equal(svc.scene(), ctx.bake(whole, o))
+        # edits after the rooms: a room light moved, an interior light moved, a light added in a room
+        lights = {l.id: l for l in whole.lights}
+        a = moved(room0.lights[0], 0.5, 0.0, 0.3)
+        b = moved(inp.lights[2], -0.7, 0.0, 0.4)
+        c = scenes.point_light(20_000_900, (-100.0 - 80.0 + 3.0, 1.5, 3.0), 5.0, bounce=True)
+        svc.apply([a, b, c])
+        for l in (a, b, c):
+            lights[l.id] = l
+        assert_scenes_equal(svc.scene(), ctx.bake(whole.with_lights(sorted(lights.values(), key=lambda l: l.id)), o))
+    finally:
+        svc.close()
+
+
+def test_rooms_that_are_not_isolated_are_refused(ctx):
+    inp, o = small()
+    svc = Service(ctx, inp, o)
+    try:
+        before = svc.scene()
+        # a room inside the live scene's bounds, and one whose light reaches the live scene
+        for origin in ((3.0, 3.0), (14.0, 0.0)):  # inside the live bounds; its lights reach them
+            with pytest.raises(BakeError) as e:
+                svc.add_room(scenes.service_room(0, origin=origin))
+            assert e.value.code == abi.DL_E_ARGUMENT
+        near = scenes.service_room(0)
+        # mesh ids not above the live ones
+        low = scenes.box_room(3)[0]
+        with pytest.raises(BakeError) as e:
+            svc.add_room(low)
+        assert e.value.code == abi.DL_E_ARGUMENT
+        assert_scenes_equal(svc.scene(), before)
+        # a light edit that reaches two parts is refused once a room is there
+        svc.add_room(near)
+        far = scenes.point_light(70_001, (-60.0, 1.5, 3.0), 80.0, bounce=False)
+        with pytest.raises(BakeError) as e:
+            svc.apply([far])
+        assert e.value.code == abi.DL_E_ARGUMENT
+    finally:
+        svc.close()
