@@ -536,7 +536,7 @@ def main():
                      "unit": "GB/s", "frac": r_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_gather_rays"),
                      "alg_bytes_per_launch": r_alg, "launch_ms": r_launch_s * 1000.0, "launches_per_step": r_launches,
                      "jobs_per_step": counts["gather_jobs"], "samples_per_step": counts["gather_samples"],
-                     "traced_per_step": counts["gather_traced"],
+                     "traced_per_step": counts["gather_traced"], "photon_reads_per_step": counts["gather_photon_reads"],
                      "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per sample (photon record), "
                               f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written)"},
         # The other two traversal kernels, priced per use of each node and triangle record: most of

@@ -483,11 +483,20 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   stage(1, cube_slot.size());
   const uint32_t cres = opt.photon_cube_resolution;
   DBuf<uint32_t> d_cube_slot, d_photons;
+  DBuf<unsigned long long> d_tile_live;
+  DBuf<float4> d_tile_box;
   if (!cube_slot.empty()) {
     uint64_t photons = 6ull * cres * cres * cube_slot.size();
     HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
     HIP_TRY(d_photons.alloc(photons * 4));  // one 16-byte record per photon (12 bytes used)
     HIP_TRY(launch_photons(s, sc, occ, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, 128));
+    if (ctx->photon_tiles) {
+      const uint64_t tpf = photon_tiles_per_face(cres), tiles = 6 * tpf * tpf * cube_slot.size();
+      HIP_TRY(d_tile_live.alloc(tiles));
+      HIP_TRY(d_tile_box.alloc(3 * tiles));
+      HIP_TRY(launch_photon_tiles(s, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, d_tile_live.p,
+                                  d_tile_box.p));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     advance(cube_slot.size());
   }
@@ -634,7 +643,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   std::unique_ptr<dl_baked_scene> scene(new dl_baked_scene());
   scene->meshes.resize(nm);
   DBuf<unsigned long long> d_counters;
-  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 6));
+  if (ctx->counting) HIP_TRY(zeroed(ctx, d_counters, 7));
   uint64_t burial_rays = 0;
   double burial_ms = 0.0;
   float bounce_kernel_ms = 0.0f;
@@ -994,6 +1003,12 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bp.cube_res = cres;
       bp.photons = d_photons.p;
       bp.sun_only = 0;
+      bp.photon_reads = ctx->counting ? d_counters.p + 6 : nullptr;
+      if (d_tile_live.p) {
+        bp.tiles_per_face = photon_tiles_per_face(cres);
+        bp.tile_live = d_tile_live.p;
+        bp.tile_box = d_tile_box.p;
+      }
       HIP_TRY(launch_bounce_owner(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, be_units.back()));
       DevScene scb = sc;
       scb.counters = ctx->counting ? d_counters.p + 2 : nullptr;
@@ -1403,8 +1418,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   ctx->timings.bounce_kernel_ms = bounce_kernel_ms;
   ctx->timings.burial_ms = burial_ms;
   if (ctx->counting) {
-    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
-    HIP_TRY(copy_to_host(ctx, c, d_counters.p, 48));
+    unsigned long long c[7] = {0, 0, 0, 0, 0, 0, 0};
+    HIP_TRY(copy_to_host(ctx, c, d_counters.p, 56));
+    ctx->timings.gather_photon_reads = c[6];
     ctx->timings.burial_rays = burial_rays;
     ctx->timings.burial_node_visits = c[4];
     ctx->timings.burial_tri_tests = c[5];

@@ -46,7 +46,7 @@ int upload_trig(dl_ctx* ctx) {
   ctx->trig.cos.vals = put(t.cos_vals);
   ctx->trig.sin.any = put(t.sin_any);
   ctx->trig.cos.any = put(t.cos_any);
-  ctx->trig.band = t.band;
+  ctx->trig.tol = t.tol;
   if (copy != hipSuccess || off > total) return set_err(ctx, DL_E_DEVICE, "uploading the sin/cos tables failed");
   return DL_OK;
 }
@@ -127,6 +127,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     w->gather.chunk_rays = std::max<uint64_t>(1ull << 22, ctx->gather.chunk_rays / streams);
     w->gather_mode = ctx->gather_mode;
     w->trace_stream = ctx->trace_stream;
+    w->photon_tiles = ctx->photon_tiles;
     w->counting = false;
   }
   return true;
@@ -659,7 +660,7 @@ int dl_test_trig_exhaustive(uint32_t lo, uint32_t hi, uint64_t* mismatches) {
   TrigTables t;
   t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data(), host.sin_any.data()};
   t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data(), host.cos_any.data()};
-  t.band = host.band;
+  t.tol = host.tol;
   const unsigned n = std::max(1u, std::thread::hardware_concurrency());
   std::vector<uint64_t> bad(n, 0);
   std::vector<std::thread> ts;
@@ -690,7 +691,7 @@ int dl_test_trig_host(const float* x, uint64_t n, float* sin_out, float* cos_out
   TrigTables t;
   t.sin = {reinterpret_cast<const uint2*>(host.sin_runs.data()), host.sin_vals.data(), host.sin_any.data()};
   t.cos = {reinterpret_cast<const uint2*>(host.cos_runs.data()), host.cos_vals.data(), host.cos_any.data()};
-  t.band = host.band;
+  t.tol = host.tol;
   for (uint64_t i = 0; i < n; ++i) {
     sin_out[i] = lsinf(t, x[i]);
     cos_out[i] = lcosf(t, x[i]);
@@ -764,6 +765,13 @@ int dl_set_trace_stream(dl_ctx* ctx, int enable) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->trace_stream = enable ? 1u : 0u;
+  return DL_OK;
+}
+
+int dl_set_photon_tiles(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->photon_tiles = enable ? 1u : 0u;
   return DL_OK;
 }
 

@@ -77,33 +77,12 @@ void build_trig_tables(TrigHost& out, unsigned threads) {
     for (size_t r = 0; r < kTrigRuns; ++r)
       if (runs[2 * r]) any[r / 32] |= 1u << (r % 32);
   }
-  // The band: every exception's double value must lie within it of a rounding midpoint, so the
-  // kernels search exactly when an exception is possible.  Start from the largest relative
-  // distance among the exceptions and widen until the kernels' own test accepts every one.
-  double band = 0.0;
+  // The distance: every exception's double value lies within it of a rounding midpoint (see
+  // trig_midpoint_distance), so the kernels search exactly when an exception is possible.
+  uint32_t tol = 0;
   for (int w = 0; w < 2; ++w)
-    for (uint32_t k : w ? out.cos_keys : out.sin_keys) {
-      const double v = trig_value(u2f(k), w);
-      const float f = (float)v;
-      const uint32_t b = f2u(f);
-      if ((b & 0x7fffffffu) == 0u || (b & 0x7f800000u) == 0x7f800000u) continue;
-      const double m1 = ((double)f + (double)u2f(b + 1u)) * 0.5, m0 = ((double)f + (double)u2f(b - 1u)) * 0.5;
-      const double lo = std::min(m0, m1), hi = std::max(m0, m1);
-      band = std::max(band, std::min(v - lo, hi - v) / (hi - lo));
-    }
-  for (bool ok = false; !ok;) {
-    ok = true;
-    for (int w = 0; w < 2 && ok; ++w)
-      for (uint32_t k : w ? out.cos_keys : out.sin_keys) {
-        const double v = trig_value(u2f(k), w);
-        if (!trig_near_midpoint(v, (float)v, band)) {
-          ok = false;
-          band = band * (1.0 + 1.0 / 1048576.0) + 1e-300;
-          break;
-        }
-      }
-  }
-  out.band = band;
+    for (uint32_t k : w ? out.cos_keys : out.sin_keys) tol = std::max(tol, trig_midpoint_distance(trig_value(u2f(k), w)));
+  out.tol = tol;
 }
 
 // ---- light BVH: SAH split at light positions, leaves of at most two (light_bvh.rs:59-189) --------

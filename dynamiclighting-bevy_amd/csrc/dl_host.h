@@ -135,7 +135,7 @@ struct TrigHost {
   std::vector<uint32_t> sin_keys, sin_vals, cos_keys, cos_vals;  // exceptions in ascending order
   std::vector<uint32_t> sin_runs, cos_runs;  // see TrigTable::runs (bitmap, rank) word pairs
   std::vector<uint32_t> sin_any, cos_any;    // see TrigTable::any
-  double band = 0.0;  // see TrigTables::band
+  uint32_t tol = 0;  // see TrigTables::tol
 };
 void build_trig_tables(TrigHost& out, unsigned threads);
 
@@ -165,6 +165,7 @@ struct dl_ctx {
   dl::GatherLaunch gather{};
   uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
   uint32_t trace_stream = 1;  // 1: area lights, soft suns and skies through the streamed any-hit path
+  uint32_t photon_tiles = 1;  // 1: the point-light gather reads photons only where the photon tiles leave a sample open
   unsigned host_threads = 1;
   bool counting = false;
   // capture mode (test hook): later bakes keep each kept bounce entry's pre-pack f32 crop

@@ -198,7 +198,17 @@ struct BounceParams {
   uint32_t cube_res;
   const uint32_t* photons;  // 4 words per photon (distance, normal, albedo, 0), 6 * res^2 per cube
   uint32_t sun_only;        // k_bounce_gather: point lights are gathered by the streamed kernel
+  // Photon tiles (launch_photon_tiles): per 8x8 tile of a cube face, tile_live holds one bit per
+  // texel whose photon has a lit surface, tile_box three float4 (shared normal | uniform flag,
+  // box minimum | coordinate scale, box maximum); kPhotonTile^2 texels per tile, tiles_per_face^2
+  // tiles per face.  Null tile_live: the gather fetches every sample's photon.
+  uint32_t tiles_per_face = 0;
+  const unsigned long long* tile_live = nullptr;
+  const float4* tile_box = nullptr;
+  unsigned long long* photon_reads = nullptr;  // counting: photon records read by launch_gather_rays
 };
+static constexpr uint32_t kPhotonTile = 8;  // photon tile side (64 texels: one wave, one 64-bit mask)
+inline uint32_t photon_tiles_per_face(uint32_t res) { return (res + kPhotonTile - 1) / kPhotonTile; }
 // Bounce work: entries (triangle-light pairs) whose light bakes bounce, each with its window.
 struct DevBounce {
   uint64_t n;                 // bounce entries
@@ -217,6 +227,10 @@ struct DevBounce {
 };
 hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
                           uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block);
+// Tile summaries of built photon cubes (BounceParams::tile_live / tile_box): 6 * tpf^2 tiles per
+// cube with tpf = photon_tiles_per_face(res); live holds one u64 per tile, box three float4.
+hipError_t launch_photon_tiles(hipStream_t s, const DevLight* lights, const uint32_t* cube_slot, uint32_t ncubes, uint32_t res,
+                               const uint32_t* photons, unsigned long long* live, float4* box);
 hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                                DevEntries ent, DevBounce b, uint64_t units);
 hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,

@@ -2029,6 +2029,144 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
   }
 }
 
+// Photon tiles: what a gather sample can learn about its photon without reading it.  A sample
+// whose direction s falls in cube texel i reconstructs the photon as lp + s * distance_i
+// (bounce.rs:262-286), so every such point lies in the set {lp + w_i * f / |f|} over the texel's
+// square f of the face plane (major axis 1), widened by 1e-6 in face coordinates for the f32
+// roundings that pick the texel.  That set lies in the box of the eight points
+// lp + (w_i / |f|max) c and lp + (w_i / |f|min) c over the square's corners c (|f| over the
+// square lies in [|f|min, |f|max]).  Per 8x8 tile the box spans the texels whose photon has a lit
+// surface (non-zero normal), and is padded by 1e-5 of its coordinates and of |lp| + distance for
+// the f32 roundings of lp + s * distance.  One wave per tile, lane = texel (bit
+// (y & 7) * 8 + (x & 7) of the tile's live mask).
+__device__ __forceinline__ void face_plane(uint32_t face, double sc, double tc, double* f) {
+  switch (face) {
+    case 0: f[0] = 1.0; f[1] = -tc; f[2] = -sc; break;
+    case 1: f[0] = -1.0; f[1] = -tc; f[2] = sc; break;
+    case 2: f[0] = sc; f[1] = 1.0; f[2] = tc; break;
+    case 3: f[0] = sc; f[1] = -1.0; f[2] = -tc; break;
+    case 4: f[0] = sc; f[1] = -tc; f[2] = 1.0; break;
+    default: f[0] = -sc; f[1] = -tc; f[2] = -1.0; break;
+  }
+}
+__device__ __forceinline__ float f32_down(double v) {
+  float f = (float)v;
+  return (double)f > v ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float f32_up(double v) {
+  float f = (float)v;
+  return (double)f < v ? nextafterf(f, INFINITY) : f;
+}
+__global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32_t* __restrict__ cube_slot, uint32_t ncubes,
+                               uint32_t res, uint32_t tpf, const uint32_t* __restrict__ photons, unsigned long long* live,
+                               float4* box) {
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  const uint64_t per = 6ull * tpf * tpf, total = per * ncubes;
+  for (uint64_t q = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); q < total; q += (uint64_t)gridDim.x * waves) {
+    const uint32_t c = (uint32_t)(q / per), rem = (uint32_t)(q % per);
+    const uint32_t face = rem / (tpf * tpf), t = rem % (tpf * tpf);
+    const uint32_t x = (t % tpf) * kPhotonTile + (lane & 7u), y = (t / tpf) * kPhotonTile + (lane >> 3);
+    const DevLight& L = lights[cube_slot[c]];
+    const double lp[3] = {(double)L.pos[0], (double)L.pos[1], (double)L.pos[2]};
+    bool lit = false;
+    uint32_t code = 0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, wmax = 0.0;
+    if (x < res && y < res) {
+      const uint4 ph = reinterpret_cast<const uint4*>(photons)[(uint64_t)c * 6 * res * res + (uint64_t)face * res * res +
+                                                               (uint64_t)y * res + x];
+      code = ph.y & 0xffffffu;
+      lit = code != 0u;
+      if (lit) {
+        const double w = (double)__uint_as_float(ph.x), e = 1.0e-6;
+        wmax = w;
+        const double s0 = ((double)x / res - e) * 2.0 - 1.0, s1 = ((double)(x + 1) / res + e) * 2.0 - 1.0;
+        const double t0 = ((double)y / res - e) * 2.0 - 1.0, t1 = ((double)(y + 1) / res + e) * 2.0 - 1.0;
+        const double sn = fmin(fmax(0.0, s0), s1), tn = fmin(fmax(0.0, t0), t1);
+        const double fmin_ = sqrt(1.0 + sn * sn + tn * tn);
+        double fmax2 = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          const double sc = (k & 1) ? s1 : s0, tc = (k & 2) ? t1 : t0;
+          fmax2 = fmax(fmax2, 1.0 + sc * sc + tc * tc);
+        }
+        const double near_ = w / sqrt(fmax2), far_ = w / fmin_;
+        for (int k = 0; k < 4; ++k) {
+          double f[3];
+          face_plane(face, (k & 1) ? s1 : s0, (k & 2) ? t1 : t0, f);
+          for (int a = 0; a < 3; ++a) {
+            const double p0 = lp[a] + f[a] * near_, p1 = lp[a] + f[a] * far_;
+            lo[a] = fmin(lo[a], fmin(p0, p1));
+            hi[a] = fmax(hi[a], fmax(p0, p1));
+          }
+        }
+      }
+    }
+    const uint64_t mask = __ballot(lit);
+    const uint32_t first = mask ? (uint32_t)__ffsll((unsigned long long)mask) - 1u : 0u;
+    const uint32_t code0 = __shfl(code, (int)first);
+    const bool uniform = mask != 0 && __ballot(lit && code != code0) == 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fmin(lo[a], __shfl_xor(lo[a], off));
+        hi[a] = fmax(hi[a], __shfl_xor(hi[a], off));
+      }
+      wmax = fmax(wmax, __shfl_xor(wmax, off));
+    }
+    if (lane == 0) {
+      live[q] = (unsigned long long)mask;
+      float4 b0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b1 = b0, b2 = b0;
+      if (mask) {
+        const f3 n = normalize_or_zero(mk3((float)(int8_t)(code0 & 0xffu) / 127.0f, (float)(int8_t)((code0 >> 8) & 0xffu) / 127.0f,
+                                           (float)(int8_t)((code0 >> 16) & 0xffu) / 127.0f));
+        b0 = make_float4(n.x, n.y, n.z, uniform ? 1.0f : 0.0f);
+        const double lpm = fmax(fabs(lp[0]), fmax(fabs(lp[1]), fabs(lp[2])));
+        double scale = 0.0;
+        float l[3], h[3];
+        for (int a = 0; a < 3; ++a) {
+          const double pad = 1.0e-5 * (fmax(fabs(lo[a]), fabs(hi[a])) + lpm + wmax) + 1.0e-6;
+          l[a] = f32_down(lo[a] - pad);
+          h[a] = f32_up(hi[a] + pad);
+          scale += fmax(fabs((double)l[a]), fabs((double)h[a]));
+        }
+        b1 = make_float4(l[0], l[1], l[2], f32_up(scale));
+        b2 = make_float4(h[0], h[1], h[2], 0.0f);
+      }
+      box[3 * q] = b0;
+      box[3 * q + 1] = b1;
+      box[3 * q + 2] = b2;
+    }
+  }
+}
+
+// Whether a sample in a lit texel of tile `tb` (three float4 of tile_box) certainly brings no
+// light: with o the ray origin, n the receiver's face normal and m a margin well above the f32
+// roundings of the gather's own arithmetic (1e-5 of the coordinates involved), either the tile's
+// photons share the receiver's normal (the weight max(n.d, 0) max(pn.-d, 0) is then zero for
+// every d) and o is outside the box (so the distance is positive), or the whole box lies behind
+// the receiver's plane (n.d < 0), or the tile's photons share a normal whose plane has o behind it
+// (pn.-d < 0).  In all three cases the ray counts as cast and its contribution is zero.
+__device__ __forceinline__ bool tile_dark(const float4& b0, const float4& b1, const float4& b2, f3 o, f3 n, float o_scale) {
+  const float m = 1.0e-5f * (o_scale + b1.w) + 1.0e-6f;
+  const bool uniform = b0.w != 0.0f;
+  const bool inside = o.x >= b1.x && o.x <= b2.x && o.y >= b1.y && o.y <= b2.y && o.z >= b1.z && o.z <= b2.z;
+  if (uniform && b0.x == n.x && b0.y == n.y && b0.z == n.z && !inside) return true;
+  const float front = n.x * ((n.x >= 0.0f ? b2.x : b1.x) - o.x) + n.y * ((n.y >= 0.0f ? b2.y : b1.y) - o.y) +
+                      n.z * ((n.z >= 0.0f ? b2.z : b1.z) - o.z);
+  if (front < -m) return true;
+  if (!uniform) return false;
+  const float back = b0.x * ((b0.x >= 0.0f ? b1.x : b2.x) - o.x) + b0.y * ((b0.y >= 0.0f ? b1.y : b2.y) - o.y) +
+                     b0.z * ((b0.z >= 0.0f ? b1.z : b2.z) - o.z);
+  return back > m;
+}
+__device__ __forceinline__ bool tile_outside(const float4& b1, const float4& b2, f3 o) {
+  return !(o.x >= b1.x && o.x <= b2.x && o.y >= b1.y && o.y <= b2.y && o.z >= b1.z && o.z <= b2.z);
+}
+// Tile index and bit of cube texel (face, x, y).
+__device__ __forceinline__ uint32_t photon_tile(uint32_t face, uint32_t x, uint32_t y, uint32_t tpf, uint32_t* bit) {
+  *bit = (y % kPhotonTile) * kPhotonTile + (x % kPhotonTile);
+  return face * tpf * tpf + (y / kPhotonTile) * tpf + x / kPhotonTile;
+}
+
 // glam acos_approx (DirectXMath XMScalarAcos polynomial)
 __device__ __forceinline__ float acos_approx(float v) {
   bool nonneg = v >= 0.0f;
@@ -2317,6 +2455,8 @@ struct GatherJob {
   float step;
   uint32_t samples;
   const uint32_t* cube;
+  uint64_t tile0;  // the cube's first photon tile
+  float o_scale;   // |origin.x| + |origin.y| + |origin.z|
 };
 
 __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMeshes& ms, const DevLight* __restrict__ lights,
@@ -2334,6 +2474,8 @@ __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMesh
   g.dir0 = normalize_or(sub(pos, g.lp), mk3(0.0f, 1.0f, 0.0f));
   g.origin = add(pos, mul(g.normal, kRayOffset));
   g.cube = p.photons + (size_t)b.cube_of_slot[slot] * 6 * p.cube_res * p.cube_res * 4;
+  g.tile0 = (uint64_t)b.cube_of_slot[slot] * 6 * p.tiles_per_face * p.tiles_per_face;
+  g.o_scale = fabsf(g.origin.x) + fabsf(g.origin.y) + fabsf(g.origin.z);
   const uint32_t m = tr.mesh[t];
   const uint32_t xy = tx.xy[j];
   *rng = stream5(p.seed, ms.id[m], L.id, xy & 0xffffu, xy >> 16);
@@ -2343,28 +2485,28 @@ __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMesh
 
 // Sample k of a job: true when it casts a visibility ray (direction, length and the light it
 // brings if unoccluded), false when the photon is empty (bounce.rs:262-286).
-__device__ __forceinline__ bool gather_sample(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k, uint32_t res,
-                                              uint32_t colored, f3* d, float* distance, f3* contrib) {
+// Whether sample k of a job casts a ray (bounce.rs:262-286: its photon has a lit surface and a
+// positive distance), reading the photon only when its tile leaves it open: an unlit texel casts
+// none, a lit one whose tile box excludes the origin casts one.
+__device__ __forceinline__ bool gather_sample_casts(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k,
+                                                    const BounceParams& p) {
   float spread = 0.1f + (float)k * g.step;
   f3 rnd = on_unit_sphere(trig, rng);
   f3 sample = normalize_or(slerp3(trig, g.dir0, rnd, spread), g.dir0);
-  uint32_t ci = cube_texel_of(sample, res);
+  uint32_t face, x, y;
+  cube_texel_parts(sample, p.cube_res, &face, &x, &y);
+  const uint32_t ci = face * p.cube_res * p.cube_res + y * p.cube_res + x;
+  if (p.tile_live) {
+    uint32_t bit;
+    const uint64_t tile = g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit);
+    if (!((p.tile_live[tile] >> bit) & 1ull)) return false;
+    if (tile_outside(p.tile_box[3 * tile + 1], p.tile_box[3 * tile + 2], g.origin)) return true;
+  }
   const uint4 ph = reinterpret_cast<const uint4*>(g.cube)[ci];
-  const uint32_t w0 = ph.x, w1 = ph.y, w2 = ph.z;
-  f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
-                                (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
-  f3 photon = add(g.lp, mul(sample, __uint_as_float(w0)));
-  f3 to_photon = sub(photon, g.origin);
-  float dist = len(to_photon);
-  if (is_zero(pn) || dist <= 0.0f) return false;
-  f3 dir = divs(to_photon, dist);
-  float weight = maxf_(dot(g.normal, dir), 0.0f) * maxf_(dot(pn, neg(dir)), 0.0f);
-  f3 alb = colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f, (float)((w2 >> 16) & 0xffu) / 255.0f)
-                   : mk3(1.0f, 1.0f, 1.0f);
-  *d = dir;
-  *distance = dist;
-  *contrib = mul(alb, weight);
-  return true;
+  f3 pn = normalize_or_zero(mk3((float)(int8_t)(ph.y & 0xffu) / 127.0f, (float)(int8_t)((ph.y >> 8) & 0xffu) / 127.0f,
+                                (float)(int8_t)((ph.y >> 16) & 0xffu) / 127.0f));
+  f3 photon = add(g.lp, mul(sample, __uint_as_float(ph.x)));
+  return !is_zero(pn) && len(sub(photon, g.origin)) > 0.0f;
 }
 
 // ---- batched sample generation ------------------------------------------------------------
@@ -2377,7 +2519,7 @@ __device__ __forceinline__ void trig_finish_n(const TrigTables& t, const TrigTab
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     out[i] = (float)v[i];
-    need[i] = trig_in_domain(f2u(x[i])) && trig_near_midpoint(v[i], out[i], t.band);
+    need[i] = trig_in_domain(f2u(x[i])) && trig_near_midpoint(v[i], t.tol);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {  // runs without an exception are cleared by the cached bitmap
@@ -2396,12 +2538,17 @@ __device__ __forceinline__ void trig_finish_n(const TrigTables& t, const TrigTab
   }
 }
 
-// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples), the same values as
-// G calls of gather_sample in order: random draws in sample order, then the sin/cos searches
-// and the photon fetches of the G samples issued together.  la = |g.dir0|.
+// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples), each as in
+// bounce.rs:262-286 (spread, random direction, slerp, photon, distance, weight): random draws in
+// sample order, then the sin/cos searches, the photon tile reads and the photon fetches of the G
+// samples issued together.  A sample whose tile shows that it casts no ray, or a ray that brings
+// no light, skips its photon fetch; its outputs are those the photon would have given (ray false,
+// or ray true with a zero contribution, which is never traced).  la = |g.dir0|.
 template <int G>
 __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k0, float la,
-                                             uint32_t res, uint32_t colored, f3* dout, float* dist, f3* cout, bool* ray) {
+                                             const BounceParams& p, f3* dout, float* dist, f3* cout, bool* ray,
+                                             uint32_t* reads) {
+  const uint32_t res = p.cube_res, colored = p.colored;
   const f3 a = g.dir0;
   // on_unit_sphere (random.rs:22-28): the draws, then sin/cos of the azimuths
   float z[G], ang[G];
@@ -2445,7 +2592,7 @@ __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTable
   float sv[3 * G];
   trig_finish_n<3 * G>(trig, trig.sin, arg, va, sv);
   // sample direction, cube texel, photon
-  uint32_t ci[G];
+  uint32_t ci[G], tile[G], bit[G];
   f3 sample[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
@@ -2457,13 +2604,47 @@ __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTable
       sl = slerp3(trig, a, rnd[j], sp[j]);
     }
     sample[j] = normalize_or(sl, a);
-    ci[j] = cube_texel_of(sample[j], res);
+    uint32_t face, x, y;
+    cube_texel_parts(sample[j], res, &face, &x, &y);
+    ci[j] = face * res * res + y * res + x;
+    tile[j] = photon_tile(face, x, y, p.tiles_per_face, &bit[j]);
+  }
+  // what the photon tiles decide without the photon: 0 read the photon, 1 unlit texel (no ray),
+  // 2 a ray that brings no light
+  uint32_t known[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) known[j] = 0u;
+  if (p.tile_live) {
+    unsigned long long lw[G];
+    float4 b0[G], b1[G], b2[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const uint64_t t = g.tile0 + tile[j];
+      lw[j] = p.tile_live[t];
+      b0[j] = p.tile_box[3 * t];
+      b1[j] = p.tile_box[3 * t + 1];
+      b2[j] = p.tile_box[3 * t + 2];
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      known[j] = !((lw[j] >> bit[j]) & 1ull) ? 1u : tile_dark(b0[j], b1[j], b2[j], g.origin, g.normal, g.o_scale) ? 2u : 0u;
   }
   uint4 ph[G];
 #pragma unroll
-  for (int j = 0; j < G; ++j) ph[j] = k0 + j < g.samples ? reinterpret_cast<const uint4*>(g.cube)[ci[j]] : make_uint4(0u, 0u, 0u, 0u);
+  for (int j = 0; j < G; ++j) {
+    const bool read = k0 + j < g.samples && known[j] == 0u;
+    ph[j] = read ? reinterpret_cast<const uint4*>(g.cube)[ci[j]] : make_uint4(0u, 0u, 0u, 0u);
+    *reads += read ? 1u : 0u;
+  }
 #pragma unroll
   for (int j = 0; j < G; ++j) {
+    if (known[j] != 0u) {
+      ray[j] = k0 + j < g.samples && known[j] == 2u;
+      dout[j] = mk3(0.0f, 0.0f, 0.0f);
+      cout[j] = mk3(0.0f, 0.0f, 0.0f);
+      dist[j] = 0.0f;
+      continue;
+    }
     const uint32_t w1 = ph[j].y, w2 = ph[j].z;
     const f3 pn = normalize_or_zero(mk3((float)(int8_t)(w1 & 0xffu) / 127.0f, (float)(int8_t)((w1 >> 8) & 0xffu) / 127.0f,
                                         (float)(int8_t)((w1 >> 16) & 0xffu) / 127.0f));
@@ -2567,11 +2748,7 @@ __global__ void k_gather_count(DevTris tr, DevMeshes ms, const DevLight* __restr
       const uint2 jb = jobs[i];
       Pcg rng;
       GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
-      for (uint32_t k = 0; k < g.samples; ++k) {
-        f3 d, c;
-        float dist;
-        rays += gather_sample(g, trig, rng, k, p.cube_res, p.colored, &d, &dist, &c) ? 1u : 0u;
-      }
+      for (uint32_t k = 0; k < g.samples; ++k) rays += gather_sample_casts(g, trig, rng, k, p) ? 1u : 0u;
       win = b.ent_window[jb.y];
     }
     wave_add_at(b.rays, m, rays, i < njobs);
@@ -2613,7 +2790,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) tcount[n] = 0;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
     const uint64_t i = i0 + lane;
-    uint32_t m = 0xffffffffu, nr = 0, win = 0;
+    uint32_t m = 0xffffffffu, nr = 0, win = 0, reads = 0;
     if (i < n) {
       const uint2 jb = jobs[i];
       win = b.ent_window[jb.y];
@@ -2628,7 +2805,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
         f3 d[G], c[G];
         float dist[G];
         bool ray[G];
-        gather_group<G>(g, trig, rng, k0, la, p.cube_res, p.colored, d, dist, c, ray);
+        gather_group<G>(g, trig, rng, k0, la, p, d, dist, c, ray, &reads);
 #pragma unroll
         for (int j = 0; j < G; ++j) {
           if (k0 + j >= g.samples) continue;
@@ -2660,6 +2837,7 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
     }
     wave_add_at(b.rays, m, nr, i < n);
     if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
+    if (p.photon_reads) wave_add(p.photon_reads, reads);
   }
 }
 
@@ -3282,6 +3460,15 @@ hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLi
   if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_photons, dim3(grid_for(total, block)), dim3(block), stack_bytes(scene, block), s, scene, occ, lights,
                      cube_slot, ncubes, res, photons);
+  return hipGetLastError();
+}
+hipError_t launch_photon_tiles(hipStream_t s, const DevLight* lights, const uint32_t* cube_slot, uint32_t ncubes, uint32_t res,
+                               const uint32_t* photons, unsigned long long* live, float4* box) {
+  const uint32_t tpf = photon_tiles_per_face(res);
+  const uint64_t tiles = 6ull * tpf * tpf * ncubes;
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_photon_tiles, dim3(grid_for(tiles, kThreads / 64)), dim3(kThreads), 0, s, lights, cube_slot, ncubes, res, tpf,
+                     photons, live, box);
   return hipGetLastError();
 }
 hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,

@@ -346,7 +346,7 @@ DLI f3 cube_texel_dir(uint32_t face, uint32_t x, uint32_t y, uint32_t res) {
   }
   return normalize(d);
 }
-DLI uint32_t cube_texel_of(f3 d, uint32_t res) {
+DLI void cube_texel_parts(f3 d, uint32_t res, uint32_t* face_out, uint32_t* x_out, uint32_t* y_out) {
   float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   uint32_t face;
   float sc, tc, ma;
@@ -363,8 +363,13 @@ DLI uint32_t cube_texel_of(f3 d, uint32_t res) {
   float u = 0.5f, v = 0.5f;
   if (ma != 0.0f) { u = (sc / ma) * 0.5f + 0.5f; v = (tc / ma) * 0.5f + 0.5f; }
   float mx = (float)(res == 0 ? 0u : res - 1u);
-  uint32_t x = sat_u32(clampf_(floorf(u * (float)res), 0.0f, mx));
-  uint32_t y = sat_u32(clampf_(floorf(v * (float)res), 0.0f, mx));
+  *face_out = face;
+  *x_out = sat_u32(clampf_(floorf(u * (float)res), 0.0f, mx));
+  *y_out = sat_u32(clampf_(floorf(v * (float)res), 0.0f, mx));
+}
+DLI uint32_t cube_texel_of(f3 d, uint32_t res) {
+  uint32_t face, x, y;
+  cube_texel_parts(d, res, &face, &x, &y);
   return face * res * res + y * res + x;
 }
 

@@ -3,12 +3,13 @@
 // The reference computes f32::sin / f32::cos, which Rust std sends to glibc sinf/cosf; glibc's
 // results are not correctly rounded (about 1% of inputs in [-0.8, 8] differ from the nearest
 // float).  Kernels therefore evaluate sin/cos in double precision (argument reduction by a
-// two-part pi/2 split and the fdlibm polynomial kernels, fixed operation order, no FMA) and
-// round once to f32; the host compares that value with its own sinf/cosf for every float in the
-// domain the bake uses and stores the inputs where they differ.  Every such input lies close to
-// a rounding midpoint of the double value (glibc is off by less than 0.07 ulp), so a kernel only
-// looks its input up in a bitmap of the exceptions (with per-32-input ranks into their results)
-// when the double value is within the host-measured band around a midpoint.
+// two-part pi/2 split and the fdlibm polynomial kernels in fused multiply-adds, a fixed
+// operation sequence that the host and the device evaluate identically) and round once to f32;
+// the host compares that value with its own sinf/cosf for every float in the domain the bake uses
+// and stores the inputs where they differ.  Every such input lies close to a rounding midpoint of
+// the double value (glibc is off by less than 0.07 ulp), so a kernel only looks its input up in a
+// bitmap of the exceptions (with per-32-input ranks into their results) when the double value's
+// bits below f32 precision lie within the host-measured distance of the midpoint pattern.
 #pragma once
 
 #include "dl_math.h"
@@ -27,23 +28,25 @@ static constexpr double kC1 = 4.16666666666666019037e-02, kC2 = -1.3888888888874
                         kC3 = 2.48015872894767294178e-05, kC4 = -2.75573143513906633035e-07,
                         kC5 = 2.08757232129817482790e-09, kC6 = -1.13596475577881948265e-11;
 
+// x + x^3 (S1 + x^2 (S2 + ... + x^2 S6)) and 1 - x^2 / 2 + x^4 (C1 + ... + x^2 C5), by Horner's
+// rule in fused multiply-adds (std::fma on the host, v_fma_f64 on the device: both exact).
 DLI double poly_sin(double x) {
-  double z = x * x;
-  double v = z * x;
-  double r = kS2 + z * (kS3 + z * (kS4 + z * (kS5 + z * kS6)));
-  return x + v * (kS1 + z * r);
+  const double z = x * x;
+  const double v = z * x;
+  const double r = fma(z, fma(z, fma(z, fma(z, kS6, kS5), kS4), kS3), kS2);
+  return fma(v, fma(z, r, kS1), x);
 }
 DLI double poly_cos(double x) {
-  double z = x * x;
-  double r = z * (kC1 + z * (kC2 + z * (kC3 + z * (kC4 + z * (kC5 + z * kC6)))));
-  return 1.0 - (0.5 * z - z * r);
+  const double z = x * x;
+  const double r = fma(z, fma(z, fma(z, fma(z, fma(z, kC6, kC5), kC4), kC3), kC2), kC1);
+  return fma(z * z, r, fma(-0.5, z, 1.0));
 }
 // Argument reduction: x = fn * pi/2 + r with |r| <= pi/4 (for the domain's |x| <= 8).
 DLI double trig_reduce(float xf, int* quadrant) {
   const double x = (double)xf;
-  const double fn = floor(x * kInvPio2 + 0.5);
+  const double fn = floor(fma(x, kInvPio2, 0.5));
   *quadrant = (int)fn;
-  return (x - fn * kPio2_1) - fn * kPio2_1t;
+  return fma(-fn, kPio2_1t, fma(-fn, kPio2_1, x));
 }
 // sin (which = 0) or cos (which = 1) of the reduced argument, before rounding: both kernels are
 // evaluated and the quadrant selects, so lanes do not diverge.
@@ -74,11 +77,11 @@ struct TrigTable {
 
 struct TrigTables {
   TrigTable sin, cos;
-  // Inputs whose double value lies farther than band * (midpoint gap) from both rounding
-  // midpoints of its f32 rounding are never exceptions (checked on the host for every float of
-  // the domain); only the others are searched.  Domain: bits [kDomPosLo, kDomPosHi] then
+  // Inputs whose double value has its 29 bits below f32 precision farther than `tol` from the
+  // midpoint pattern 2^28 are never exceptions (checked on the host for every float of the
+  // domain); only the others are searched.  Domain: bits [kDomPosLo, kDomPosHi] then
   // [kDomNegLo, kDomNegHi].
-  double band;
+  uint32_t tol;
 };
 static constexpr uint32_t kDomPosLo = 0x37800000u, kDomPosHi = 0x41000000u;  // [2^-16, 8]
 static constexpr uint32_t kDomNegLo = 0xB7800000u, kDomNegHi = 0xBF4CCCCDu;  // [-0.8, -2^-16]
@@ -114,20 +117,21 @@ DLI float trig_lookup(const TrigTable& t, float x, float computed) {
   if (!((run.x >> bit) & 1u)) return computed;
   return u2f(t.vals[run.y + (uint32_t)__builtin_popcount(run.x & ((1u << bit) - 1u))]);
 }
-// Whether the double value v (rounded to f) lies within band * gap of a rounding midpoint,
-// where gap is the distance between the midpoints below and above f.
-DLI bool trig_near_midpoint(double v, float f, double band) {
-  const uint32_t b = f2u(f);
-  if ((b & 0x7fffffffu) == 0u || (b & 0x7f800000u) == 0x7f800000u) return true;
-  const float away = u2f(b + 1u), toward = u2f(b - 1u);  // larger and smaller magnitude
-  const double m1 = ((double)f + (double)away) * 0.5, m0 = ((double)f + (double)toward) * 0.5;
-  const double lo = m0 < m1 ? m0 : m1, hi = m0 < m1 ? m1 : m0;
-  const double a = v - lo, c = hi - v;
-  return (a < c ? a : c) <= band * (hi - lo);
+// Distance of the double v from the nearest rounding midpoint of its f32 rounding, in units of
+// 2^-29 f32 ulp: the 29 mantissa bits below f32 precision against the midpoint pattern 2^28.  Only
+// meaningful when v rounds to a normal float of its own binade; values below 2^-125 in magnitude
+// (f32 denormal range and its edge) report 0, so they always count as near.
+DLI uint32_t trig_midpoint_distance(double v) {
+  uint64_t b;
+  __builtin_memcpy(&b, &v, 8);
+  if (((uint32_t)(b >> 52) & 0x7ffu) < 1023u - 125u) return 0u;
+  const uint32_t low = (uint32_t)b & 0x1fffffffu;
+  return low >= 0x10000000u ? low - 0x10000000u : 0x10000000u - low;
 }
+DLI bool trig_near_midpoint(double v, uint32_t tol) { return trig_midpoint_distance(v) <= tol; }
 DLI float trig_finish(const TrigTables& t, const TrigTable& tab, float x, double v) {
   const float f = (float)v;
-  if (!trig_in_domain(f2u(x)) || !trig_near_midpoint(v, f, t.band)) return f;
+  if (!trig_in_domain(f2u(x)) || !trig_near_midpoint(v, t.tol)) return f;
   return trig_lookup(tab, x, f);
 }
 DLI float lsinf(const TrigTables& t, float x) { return trig_finish(t, t.sin, x, trig_value(x, 0)); }

@@ -147,6 +147,8 @@ def load() -> C.CDLL:
                                         C.c_uint32]
     lib.dl_set_trace_stream.restype = C.c_int
     lib.dl_set_trace_stream.argtypes = [P, C.c_int]
+    lib.dl_set_photon_tiles.restype = C.c_int
+    lib.dl_set_photon_tiles.argtypes = [P, C.c_int]
     lib.dl_set_gather.restype = C.c_int
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32]
@@ -435,6 +437,11 @@ class Context:
     def set_trace_stream(self, enable: bool) -> None:
         """Area lights, soft suns and skies through the streamed any-hit path (default) or per lane."""
         self.lib.dl_set_trace_stream(self.handle, 1 if enable else 0)
+
+    def set_photon_tiles(self, enable: bool) -> None:
+        """The point-light gather skips the photon reads its cube's tile summaries decide (default)
+        or reads every sample's photon; the results are the same."""
+        self.lib.dl_set_photon_tiles(self.handle, 1 if enable else 0)
 
     def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, chunk_rays: int = 0, blocks_per_cu: int = 0,
                    min_waves: int = 0, refill: int = 0, order: int = 0) -> None:

@@ -74,6 +74,12 @@ uint32_t dl_test_batch_trace(dl_ctx* ctx, dl_timings* timings, uint32_t* worker,
  * any-hit kernel; 0: each lane of the shadow-ray kernel casts its texel's rays in turn. */
 int dl_set_trace_stream(dl_ctx* ctx, int enable);
 
+/* 1 (default): the point-light gather summarizes each photon cube in 8x8-texel tiles (which
+ * texels hold a lit photon, their shared normal, a box around the photon points a sample can
+ * reconstruct) and reads a sample's photon only when its tile cannot show that the sample casts
+ * no ray or brings no light; 0: every sample reads its photon.  Results are identical. */
+int dl_set_photon_tiles(dl_ctx* ctx, int enable);
+
 /* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
  * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
  * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per

@@ -70,3 +70,37 @@ def test_capture_is_off_unless_asked(ctx):
     inp, o = scene_cases.bounce_point_case()
     ctx.bake(inp, o)
     assert ctx.lib.dl_test_bounce_capture(ctx.handle, 0, None, None) == 0
+
+
+@pytest.mark.parametrize("res", [5, 37, 512])
+def test_photon_tiles_change_no_result(ctx, res):
+    """The gather skips the photon reads its tile summaries decide; bakes with and without the
+    tiles are identical (words, bounce crops, stats) and equal the oracle, at cube sizes that are
+    not multiples of the 8-texel tile too.  Counting mode shows that fewer photons were read."""
+    inp, o = scene_cases.bounce_point_case()
+    o.photon_cube_resolution = res
+    print(f"photon tiles at cube resolution {res}", flush=True)
+    ctx.set_counting(True)
+    try:
+        ctx.set_photon_tiles(False)
+        try:
+            plain, plain_crops = ctx.bake_bounce_crops(inp, o)
+            plain_t = ctx.timings()
+        finally:
+            ctx.set_photon_tiles(True)
+        tiled, tiled_crops = ctx.bake_bounce_crops(inp, o)
+        tiled_t = ctx.timings()
+    finally:
+        ctx.set_counting(False)
+    assert_scenes_equal(tiled, plain)
+    assert tiled_crops.keys() == plain_crops.keys()
+    for k in plain_crops:
+        assert np.array_equal(tiled_crops[k].view(np.uint32), plain_crops[k].view(np.uint32)), k
+    assert plain_t["gather_samples"] > 0
+    assert plain_t["gather_photon_reads"] == plain_t["gather_samples"]
+    assert tiled_t["gather_samples"] == plain_t["gather_samples"]
+    assert tiled_t["gather_photon_reads"] < tiled_t["gather_samples"]
+    print(f"  photon reads {tiled_t['gather_photon_reads']} of {tiled_t['gather_samples']} samples", flush=True)
+    want_scene, want = oracle.bounce_crops(inp, o)
+    assert_scenes_equal(tiled, want_scene)
+    compare_crops(tiled_crops, want)

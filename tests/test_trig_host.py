@@ -1,7 +1,7 @@
 """The sin/cos table path on the host, against the host libm for every float of the domain.
 
 The kernels evaluate sin/cos in double precision, round to f32 and search the exception list
-only within the host-measured band around rounding midpoints; the host evaluation here runs the
+only when the double value lies within the host-measured distance of a rounding midpoint; the host evaluation here runs the
 same code (dl_trig.h), so an exhaustive pass pins the lookup logic without a GPU.  The device
 side is compared with this host path in tests/test_gpu_parity.py.
 """
