@@ -39,26 +39,38 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: one wave64 VALU instruction per cycle per CU (4 SIMDs of 16 lanes), 256 CUs at
+# 2.4 GHz (the same rate gives MI355X's 78.6 TFLOP/s of FP64 FMA)
+VALU_PEAK_GIPS = 256 * 2.4
+TILE_BYTES = 56.0    # one photon tile: 8 B of lit bits and a 48-B box (8x8 texels)
 NODE_BYTES = 112.0  # a 4-wide node as the kernels read it: six float4 of child bounds, one int4 of child slots
 TRI_BYTES = 48.0    # v0 | occluder index, e1, e2 as three float4
 RAY_BYTES = 32.0    # origin | length, direction | flags
 JOB_BYTES = 48.0    # gather job: (texel, entry) pair, texel position and coordinates, ray offset, traced count and bits
-PHOTON_BYTES = 16.0  # one photon record per gather sample
+PHOTON_BYTES = 16.0  # one photon record, read by the samples the photon tiles leave open
 TRACED_BYTES = 44.0  # ray record and RGB contribution written per traced sample
 
 
-def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "interior_bake_kernels.md")):
-    """HBM bytes per dispatch of `kernel` as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by
-    a plain-copy calibration) and recorded in the profile summary, or None when absent."""
+PROFILE = os.path.join(ROOT, "profiles", "interior_bake_kernels.md")
+
+
+def profile_figure(tag, kernel, path=PROFILE):
+    """A per-dispatch figure recorded in the profile summary as a line `TAG kernel value`: TRAFFIC is
+    the HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a plain-copy
+    calibration), VALU the VALU instructions per dispatch (SQ_INSTS_VALU); None when absent."""
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == tag and parts[1] == kernel:
                     return float(parts[2])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel):
+    return profile_figure("TRAFFIC", kernel)
 
 
 def host_cpu():
@@ -194,6 +206,17 @@ def batch_main(args):
     streams = [int(x) for x in args.streams.split(",")]
     ctx.bake_batch(ins[:16], ops[:16], streams=max(streams), read=False)
     log("warmup done")
+    # launch shapes from the autotuner (timed on a 32-job sample; not part of the timed passes)
+    tuned, tune_s = None, 0.0
+    if not args.no_tune:
+        import tempfile
+        t0 = time.perf_counter()
+        tuned = ctx.autotune_batch(ins, ops, max_streams=16,
+                                   cache_path=os.path.join(tempfile.gettempdir(), "dynlight_tune.txt"))
+        tune_s = time.perf_counter() - t0
+        log(f"autotune ({tune_s:.1f} s): {tuned}")
+        if tuned["streams"] not in streams:
+            streams.append(tuned["streams"])
     runs = {}
     ref = None
     for s in [1] + streams:
@@ -231,6 +254,11 @@ def batch_main(args):
         "plain_ms": 1000.0 * runs[1],
         "batched_ms_by_streams": {str(k): 1000.0 * runs[k] for k in streams},
         "speedup_vs_plain": runs[1] / runs[best_s],
+        "autotune": None if tuned is None else {
+            **{k: tuned[k] for k in ("streams", "trace_block", "stream_block", "packet", "burial_block", "burial_lanes",
+                                     "trace_stream", "trials", "from_cache")},
+            "sample_ms": tuned["ms"], "tune_s": tune_s, "tuned_streams_ms": 1000.0 * runs[tuned["streams"]],
+            "tuned_speedup_vs_plain": runs[1] / runs[tuned["streams"]]},
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on a job sample)")
@@ -436,6 +464,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serve workload: request arrivals per second")
     ap.add_argument("--jobs", type=int, default=512, help="batch workload: number of jobs")
     ap.add_argument("--streams", default="2,4,8,16", help="batch workload: worker stream counts to time (the fastest is reported)")
+    ap.add_argument("--no-tune", action="store_true", help="batch workload: keep the default launch shapes (no autotuner)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -466,7 +495,7 @@ def main():
         return
 
     import dynlight_hip
-    from dynlight_hip import scenes
+    from dynlight_hip import abi, scenes
 
     inp, opts = scenes.interior(2)
     ctx = dynlight_hip.Context(0)
@@ -505,12 +534,22 @@ def main():
     g_alg = (NODE_BYTES * bnodes + TRI_BYTES * btris + (RAY_BYTES + 4.0) * counts["gather_traced"]) / launches
     g_launch_s = tm["gather_trace_ms"] / launches / 1000.0
     g_achieved = g_alg / g_launch_s / 1e9 if g_launch_s > 0 else 0.0
-    # The sample setup of the gather (the largest kernel): per job, per sample and per traced sample.
+    # The sample setup of the gather (the largest kernel): per job, per photon read (the samples the
+    # photon tiles leave open) and per traced sample, plus each bounce light's photon tiles once.
     r_launches = max(1, tm["gather_rays_launches"])
-    r_alg = (JOB_BYTES * counts["gather_jobs"] + PHOTON_BYTES * counts["gather_samples"]
-             + TRACED_BYTES * counts["gather_traced"]) / r_launches
+    res = opts.photon_cube_resolution
+    tiles = 6 * ((res + 7) // 8) ** 2 * sum(1 for l in inp.lights if l.illumination == abi.ILLUMINATION_SINGLE_BOUNCE)
+    r_alg = (JOB_BYTES * counts["gather_jobs"] + PHOTON_BYTES * counts["gather_photon_reads"]
+             + TRACED_BYTES * counts["gather_traced"] + TILE_BYTES * tiles) / r_launches
     r_launch_s = tm["gather_rays_ms"] / r_launches / 1000.0
     r_achieved = r_alg / r_launch_s / 1e9 if r_launch_s > 0 else 0.0
+    r_valu = profile_figure("VALU", "k_gather_rays")
+    valu = None
+    if r_valu and r_launch_s > 0:
+        gips = r_valu / r_launch_s / 1e9
+        valu = {"bound": "valu", "achieved": gips, "peak": VALU_PEAK_GIPS, "unit": "G wave-instructions/s",
+                "frac": gips / VALU_PEAK_GIPS, "valu_per_launch": r_valu,
+                "note": "SQ_INSTS_VALU per dispatch from rocprofv3 over the HIP-event launch time; the kernel's limit"}
     out = {
         "metric": "texel-light visibility samples/s (1 GPU)",
         "value": value,
@@ -537,8 +576,10 @@ def main():
                      "alg_bytes_per_launch": r_alg, "launch_ms": r_launch_s * 1000.0, "launches_per_step": r_launches,
                      "jobs_per_step": counts["gather_jobs"], "samples_per_step": counts["gather_samples"],
                      "traced_per_step": counts["gather_traced"], "photon_reads_per_step": counts["gather_photon_reads"],
-                     "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per sample (photon record), "
-                              f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written)"},
+                     "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per photon read, "
+                              f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written), "
+                              f"{TILE_BYTES:.0f} B per photon tile of each bounce light",
+                     "valu_roofline": valu},
         # The other two traversal kernels, priced per use of each node and triangle record: most of
         # those reads are served from L2, so this is a work rate, not an HBM bandwidth; the HBM
         # bytes they move come from rocprofv3 (profiles/interior_bake_kernels.md).
@@ -571,7 +612,7 @@ def main():
                           {"GPU_MAX_HW_QUEUES": q}, 900)
             if b:
                 out["batch"][f"hw_queues_{q}"] = {k: b[k] for k in ("value", "plain_ms", "ms_per_step", "speedup_vs_plain",
-                                                                   "batched_ms_by_streams")}
+                                                                   "batched_ms_by_streams", "autotune")}
                 out["batch"][f"hw_queues_{q}"]["streams"] = b["config"]["streams"]
     if not args.no_serve:
         log("serve workload (child process)")

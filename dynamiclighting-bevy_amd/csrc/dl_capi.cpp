@@ -146,6 +146,7 @@ void dl_set_thread_error(const std::string& msg) { g_null_ctx_err = msg; }
 namespace dl {
 int set_error(dl_ctx* ctx, int code, const std::string& msg) { return set_err(ctx, code, msg); }
 bool make_workers(dl_ctx* ctx, uint32_t streams) { return prepare_workers(ctx, streams); }
+double batch_job_cost(const dl_bake_input& in, const dl_bake_options& o) { return job_cost(in, o); }
 }  // namespace dl
 
 extern "C" {

@@ -411,6 +411,9 @@ namespace dl {
 // The C ABI's error bookkeeping and worker contexts, for the service's entry points.
 int set_error(dl_ctx* ctx, int code, const std::string& msg);
 bool make_workers(dl_ctx* ctx, uint32_t streams);
+// A bake job's rough cost (texels times rays per texel, plus photons), by which dl_bake_batch
+// hands jobs out largest first.
+double batch_job_cost(const dl_bake_input& in, const dl_bake_options& o);
 // Builds the live scene of a service from the capture of its first bake.
 int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap, const dl_baked_scene& first);
 }  // namespace dl

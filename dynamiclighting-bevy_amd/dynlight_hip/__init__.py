@@ -147,6 +147,9 @@ def load() -> C.CDLL:
                                         C.c_uint32]
     lib.dl_set_trace_stream.restype = C.c_int
     lib.dl_set_trace_stream.argtypes = [P, C.c_int]
+    lib.dl_autotune_batch.restype = C.c_int
+    lib.dl_autotune_batch.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
+                                      C.c_uint32, C.c_char_p, C.POINTER(abi.dl_tuned_launch)]
     lib.dl_set_photon_tiles.restype = C.c_int
     lib.dl_set_photon_tiles.argtypes = [P, C.c_int]
     lib.dl_set_gather.restype = C.c_int
@@ -359,6 +362,23 @@ class Context:
             for i in range(n):
                 if outs[i]:
                     self.lib.dl_baked_scene_free(outs[i])
+
+    def autotune_batch(self, inputs: list[BakeInput], options: list[abi.dl_bake_options] | abi.dl_bake_options | None = None,
+                       max_streams: int = 16, cache_path: str | None = None) -> dict:
+        """dl_autotune_batch: times launch shapes on a sample of the jobs, applies the fastest to
+        this context and returns them (`streams` is for bake_batch); cache_path keeps them per
+        batch signature."""
+        n = len(inputs)
+        if options is None or isinstance(options, abi.dl_bake_options):
+            options = [options if options is not None else default_options()] * n
+        ins = (abi.dl_bake_input * max(1, n))(*[i.struct for i in inputs])
+        ops = (abi.dl_bake_options * max(1, n))(*options)
+        t = abi.dl_tuned_launch()
+        st = self.lib.dl_autotune_batch(self.handle, ins, ops, n, max_streams,
+                                        cache_path.encode() if cache_path else None, C.byref(t))
+        if st != 0:
+            raise BakeError(st, self.error())
+        return {name: getattr(t, name) for name, _ in t._fields_}
 
     def batch_trace(self) -> list[dict]:
         """Per job of the last bake_batch: worker, start/end ms from the batch start, timings."""

@@ -167,6 +167,13 @@ class dl_baked_settings(C.Structure):
                 ("max_lightmap_size", C.c_uint32), ("distance_cube_resolution", C.c_uint32)]
 
 
+class dl_tuned_launch(C.Structure):
+    _fields_ = [("streams", C.c_uint32), ("trace_block", C.c_uint32), ("stream_block", C.c_uint32),
+                ("packet", C.c_uint32), ("burial_block", C.c_uint32), ("burial_lanes", C.c_uint32),
+                ("trace_stream", C.c_uint32), ("trials", C.c_uint32), ("ms", C.c_double),
+                ("from_cache", C.c_uint32)]
+
+
 class dl_timings(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("texels_ms", C.c_double),
                 ("trace_ms", C.c_double), ("bounce_ms", C.c_double), ("post_ms", C.c_double),

@@ -400,6 +400,33 @@ int dl_service_ray_queries_noalloc(dl_service* svc, const float* rays8, uint64_t
 int dl_bake_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_options* options, uint32_t count,
                   uint32_t streams, dl_baked_scene** out, int* status);
 
+/* Launch settings of a batch, picked by dl_autotune_batch. */
+typedef struct dl_tuned_launch {
+    uint32_t streams;       /* worker streams to pass to dl_bake_batch                          */
+    uint32_t trace_block;   /* threads per block of the shadow-ray kernel                       */
+    uint32_t stream_block;  /* threads per block of the persistent ray-stream kernel            */
+    uint32_t packet;        /* rays a wave of the ray-stream kernel takes from its queue at once */
+    uint32_t burial_block;  /* threads per block of the burial test's ray stream                */
+    uint32_t burial_lanes;  /* 0: burial rays streamed; else one lane per texel, this block size */
+    uint32_t trace_stream;  /* 1: area, soft-sun and sky rays streamed; 0: per lane             */
+    uint32_t trials;        /* settings timed (0 when read from the cache file)                 */
+    double ms;              /* the sample's batch time with these settings                      */
+    uint32_t from_cache;    /* 1: the settings came from cache_path                             */
+} dl_tuned_launch;
+
+/* Autotuner of dl_bake_batch's launch shapes (no counterpart in the reference): times
+ * dl_bake_batch on a deterministic sample of the jobs (up to 32, spread over the jobs' cost order)
+ * for candidate worker-stream counts (up to max_streams), shadow-ray and ray-stream block sizes,
+ * ray packet widths, burial block sizes and the forms of the burial test and of multi-ray lights
+ * (streamed or one lane per texel), one parameter at a time from the current settings, and keeps
+ * the fastest.  The result is applied to ctx (block sizes and packet) and returned;
+ * `streams` is for the caller's dl_bake_batch.  With cache_path, settings are looked up there by a
+ * signature of the batch (job count, sizes and options), the device and GPU_MAX_HW_QUEUES, and
+ * stored there after tuning.  The stack depth of the traversal kernels is not tuned: it is the
+ * BVH's need (3 entries per level), below which traversal would fail. */
+int dl_autotune_batch(dl_ctx* ctx, const dl_bake_input* inputs, const dl_bake_options* options, uint32_t count,
+                      uint32_t max_streams, const char* cache_path, dl_tuned_launch* out);
+
 /* ---- BakedScene accessors (src/bake/format/types.rs:13-37) --------------------------------- */
 void dl_scene_settings(const dl_baked_scene* scene, dl_baked_settings* out);
 void dl_scene_stats(const dl_baked_scene* scene, dl_bake_stats* out);

@@ -57,7 +57,8 @@ def test_default_options_match_the_reference_defaults():
 
 STRUCTS = ["dl_alpha_texture", "dl_mesh", "dl_light", "dl_bake_input", "dl_bake_options", "dl_progress",
            "dl_bake_stats", "dl_light_bvh_node", "dl_baked_light", "dl_baked_mesh_view", "dl_sun_map_view",
-           "dl_sky_probes_view", "dl_baked_settings", "dl_timings", "dl_service_request", "dl_receiver"]
+           "dl_sky_probes_view", "dl_baked_settings", "dl_timings", "dl_service_request", "dl_receiver",
+           "dl_tuned_launch"]
 
 
 def test_ctypes_structs_match_the_c_layout(tmp_path):

@@ -86,3 +86,33 @@ def test_batch_errors(ctx):
     assert e.value.code == abi.DL_E_ARGUMENT
     # the context still bakes afterwards
     assert_scenes_equal(ctx.bake_batch([good], [o], streams=1)[0], ctx.bake(good, o))
+
+
+def test_autotuned_launches_keep_results_and_are_cached(ctx, tmp_path):
+    """dl_autotune_batch picks launch shapes on a sample of the jobs; a batch baked with them equals
+    single bakes, and the second call reads the same settings from the cache file."""
+    pick, ins, ops = _spanning_jobs(16, pool=64)
+    print(f"autotune on jobs {pick}", flush=True)
+    cache = str(tmp_path / "tune.txt")
+    try:
+        t = ctx.autotune_batch(ins, ops, max_streams=8, cache_path=cache)
+        print(f"  tuned {t}", flush=True)
+        assert t["from_cache"] == 0 and t["trials"] >= 5
+        assert 1 <= t["streams"] <= 8
+        assert t["trace_block"] in (64, 128, 256) and t["stream_block"] in (128, 256, 512)
+        assert t["packet"] in (64, 128, 256) and t["burial_block"] in (64, 128, 256)
+        assert t["burial_lanes"] in (0, 64, 128) and t["trace_stream"] in (0, 1)
+        again = ctx.autotune_batch(ins, ops, max_streams=8, cache_path=cache)
+        assert again["from_cache"] == 1
+        keys = ("streams", "trace_block", "stream_block", "packet", "burial_block", "burial_lanes", "trace_stream")
+        assert {k: again[k] for k in keys} == {k: t[k] for k in keys}
+        got = ctx.bake_batch(ins, ops, streams=t["streams"])
+    finally:
+        ctx.set_launch(128, 256)
+        ctx.set_gather(1, block=256, chunk=128)
+        ctx.set_burial(0)
+        ctx.set_trace_stream(True)
+    for j, inp, o, g in zip(pick, ins, ops, got):
+        assert_scenes_equal(g, ctx.bake(inp, o))
+    with pytest.raises(BakeError):
+        ctx.autotune_batch([], None)
