@@ -6,13 +6,16 @@ rooms with door openings, seeded boxes and pillars to 200k triangles, 256 point/
 range 4-12 m, 32 texels/m, 1-bit shadows and one colored bounce from every light.  A sample is one (texel, light) pair for which the
 reference's texel_shadow returns Some (src/bake/mesh_bake.rs:241-247).
 
-Prints one JSON line.  `roofline` prices the dominant kernel, the bounce gather's sample setup
-(k_gather_rays, one launch per chunk of gather jobs), at its algorithmic bytes: 48 B per job (job
-record, texel position and coordinates, ray offset, traced count and bits), 16 B per sample (its
-photon record) and 44 B per traced sample (ray and contribution written), with job, sample and
-traced counts from an instrumented (untimed) bake, divided by the kernel's average launch time
-from HIP events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes per launch
-recorded in profiles/interior_bake_kernels.md.  `other_kernels` gives the any-hit traversal of the
+Prints one JSON line.  `roofline` prices the dominant stage, the bounce gather's sample setup (per
+chunk of gather jobs: the fast pass k_gather_split, the list of the samples it leaves open, the
+exact pass k_gather_open), at its algorithmic bytes: 48 B per job (job record, texel position and
+coordinates, ray offset, traced count and bits), 40 B per open sample, 16 B per photon read, 44 B
+per traced sample (ray and contribution written) and 56 B per photon tile of each bounce light,
+with the counts from an instrumented (untimed) bake, divided by the span's average time from HIP
+events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes of the two kernels per
+launch recorded in profiles/interior_bake_kernels.md; `valu_roofline` their VALU instructions per
+launch (rocprofv3 SQ_INSTS_VALU, also recorded there) over the same span against the issue peak,
+the bound these kernels actually meet.  `other_kernels` gives the any-hit traversal of the
 gather rays (k_any_stream: per ray, 112 B per 4-wide BVH node read plus 48 B per triangle tested
 plus its 32-byte ray record and 4-byte list entry) and the direct shadow-ray kernel (k_trace) as
 work rates counted per use of each record (mostly L2-served, so not HBM figures), beside the HBM
@@ -49,6 +52,7 @@ RAY_BYTES = 32.0    # origin | length, direction | flags
 JOB_BYTES = 48.0    # gather job: (texel, entry) pair, texel position and coordinates, ray offset, traced count and bits
 PHOTON_BYTES = 16.0  # one photon record, read by the samples the photon tiles leave open
 TRACED_BYTES = 44.0  # ray record and RGB contribution written per traced sample
+OPEN_BYTES = 40.0    # per sample left to the exact pass: its draws written and read (32 B), its list entry (8 B)
 
 
 PROFILE = os.path.join(ROOT, "profiles", "interior_bake_kernels.md")
@@ -534,16 +538,19 @@ def main():
     g_alg = (NODE_BYTES * bnodes + TRI_BYTES * btris + (RAY_BYTES + 4.0) * counts["gather_traced"]) / launches
     g_launch_s = tm["gather_trace_ms"] / launches / 1000.0
     g_achieved = g_alg / g_launch_s / 1e9 if g_launch_s > 0 else 0.0
-    # The sample setup of the gather (the largest kernel): per job, per photon read (the samples the
-    # photon tiles leave open) and per traced sample, plus each bounce light's photon tiles once.
+    # The sample setup of the gather (the largest stage, one span per chunk of jobs: the fast pass
+    # k_gather_split, the list of open samples and the exact pass k_gather_open): per job, per open
+    # sample, per photon read and per traced sample, plus each bounce light's photon tiles once.
     r_launches = max(1, tm["gather_rays_launches"])
     res = opts.photon_cube_resolution
     tiles = 6 * ((res + 7) // 8) ** 2 * sum(1 for l in inp.lights if l.illumination == abi.ILLUMINATION_SINGLE_BOUNCE)
-    r_alg = (JOB_BYTES * counts["gather_jobs"] + PHOTON_BYTES * counts["gather_photon_reads"]
-             + TRACED_BYTES * counts["gather_traced"] + TILE_BYTES * tiles) / r_launches
+    r_alg = (JOB_BYTES * counts["gather_jobs"] + OPEN_BYTES * counts["gather_open"]
+             + PHOTON_BYTES * counts["gather_photon_reads"] + TRACED_BYTES * counts["gather_traced"]
+             + TILE_BYTES * tiles) / r_launches
     r_launch_s = tm["gather_rays_ms"] / r_launches / 1000.0
     r_achieved = r_alg / r_launch_s / 1e9 if r_launch_s > 0 else 0.0
-    r_valu = profile_figure("VALU", "k_gather_rays")
+    vs, vo = profile_figure("VALU", "k_gather_split"), profile_figure("VALU", "k_gather_open")
+    r_valu = vs + vo if vs and vo else None
     valu = None
     if r_valu and r_launch_s > 0:
         gips = r_valu / r_launch_s / 1e9
@@ -571,13 +578,16 @@ def main():
                    "rays_per_step": scene.stats["rays"], "bounce_rays_per_step": scene.stats["bounce_rays"]},
         "trace_kernel_ms": trace_ms / args.steps,
         "trace_samples_per_s": samples / kernel_s,
-        "roofline": {"kernel": "k_gather_rays", "bound": "hbm", "achieved": r_achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": r_achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_gather_rays"),
+        "roofline": {"kernel": "gather sample setup (k_gather_split + k_gather_open)", "bound": "hbm", "achieved": r_achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r_achieved / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("k_gather_split") + measured_traffic("k_gather_open"))
+                     if measured_traffic("k_gather_split") and measured_traffic("k_gather_open") else None,
                      "alg_bytes_per_launch": r_alg, "launch_ms": r_launch_s * 1000.0, "launches_per_step": r_launches,
                      "jobs_per_step": counts["gather_jobs"], "samples_per_step": counts["gather_samples"],
                      "traced_per_step": counts["gather_traced"], "photon_reads_per_step": counts["gather_photon_reads"],
-                     "bytes": f"{JOB_BYTES:.0f} B per job, {PHOTON_BYTES:.0f} B per photon read, "
-                              f"{TRACED_BYTES:.0f} B per traced sample (ray and contribution written), "
+                     "open_per_step": counts["gather_open"],
+                     "bytes": f"{JOB_BYTES:.0f} B per job, {OPEN_BYTES:.0f} B per open sample, {PHOTON_BYTES:.0f} B per "
+                              f"photon read, {TRACED_BYTES:.0f} B per traced sample (ray and contribution written), "
                               f"{TILE_BYTES:.0f} B per photon tile of each bounce light",
                      "valu_roofline": valu},
         # The other two traversal kernels, priced per use of each node and triangle record: most of

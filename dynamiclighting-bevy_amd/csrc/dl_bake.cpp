@@ -1054,7 +1054,15 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         DBuf<uint64_t> d_nr, d_rs, d_tcount, d_toff;
         DBuf<float4> d_rays;
         DBuf<float> d_contrib;
-        DBuf<uint32_t> d_tmask, d_list;
+        DBuf<uint32_t> d_tmask, d_list, d_omask;
+        DBuf<uint64_t> d_ocount, d_ooff;
+        // the split sample setup needs the photon tiles (phase A decides samples by them)
+        const bool split = ctx->gather_split && bp.tile_live != nullptr;
+        if (split) {
+          HIP_TRY(d_omask.alloc(cj * mask_words));
+          HIP_TRY(d_ocount.alloc(cj + 1));
+          HIP_TRY(d_ooff.alloc(cj + 1));
+        }
         HIP_TRY(d_nr.alloc(cj + 1));
         HIP_TRY(d_rs.alloc(cj + 1));
         HIP_TRY(d_tcount.alloc(cj + 1));
@@ -1079,19 +1087,32 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           HIP_TRY(hipEventCreate(&g1));
           gev.push_back(g1);
           HIP_TRY(hipEventRecord(g0, s));
-          HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
-                                     ctx->gather.order, d_tmask.p, mask_words, d_tcount.p));
+          if (split) {
+            // phase A decides most samples; the open ones (listed from their bits) get the exact chain
+            HIP_TRY(launch_gather_split(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, d_rays.p, ctx->gather.order,
+                                        d_omask.p, d_tmask.p, mask_words, d_ocount.p, d_tcount.p));
+            HIP_TRY(scan_u64(s, d_ocount.p, d_ooff.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+            HIP_TRY(launch_gather_list(s, d_omask.p, mask_words, d_ooff.p, d_rs.p, n, ctx->gather.order, d_list.p));
+            HIP_TRY(launch_gather_open(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
+                                       ctx->gather.order, d_list.p, d_ooff.p + n, n * max_samples, d_tmask.p, mask_words,
+                                       d_tcount.p));
+          } else {
+            HIP_TRY(launch_gather_rays(s, tr, ms, d_lights.p, tx, ent, bb, jobs, n, d_rs.p, bp, ctx->trig, d_rays.p, d_contrib.p,
+                                       ctx->gather.order, d_tmask.p, mask_words, d_tcount.p));
+          }
           HIP_TRY(hipEventRecord(g1, s));
           // the traced rays only, job by job
           HIP_TRY(scan_u64(s, d_tcount.p, d_toff.p, n, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
-          if (ctx->counting) {  // the chunk's samples and traced samples, for the instrumented figures
-            uint64_t both[2] = {0, 0};
-            HIP_TRY(hipMemcpyAsync(&both[0], d_rs.p + n, 8, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(&both[1], d_toff.p + n, 8, hipMemcpyDeviceToHost, s));
+          if (ctx->counting) {  // the chunk's samples, traced and open samples, for the instrumented figures
+            uint64_t three[3] = {0, 0, 0};
+            HIP_TRY(hipMemcpyAsync(&three[0], d_rs.p + n, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(&three[1], d_toff.p + n, 8, hipMemcpyDeviceToHost, s));
+            if (split) HIP_TRY(hipMemcpyAsync(&three[2], d_ooff.p + n, 8, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             ctx->timings.gather_jobs += n;
-            ctx->timings.gather_samples += both[0];
-            ctx->timings.gather_traced += both[1];
+            ctx->timings.gather_samples += three[0];
+            ctx->timings.gather_traced += three[1];
+            ctx->timings.gather_open += three[2];
           }
           HIP_TRY(launch_gather_list(s, d_tmask.p, mask_words, d_toff.p, d_rs.p, n, ctx->gather.order, d_list.p));
           HIP_TRY(hipMemsetAsync(d_jobc.p + 2, 0, sizeof(unsigned long long), s));

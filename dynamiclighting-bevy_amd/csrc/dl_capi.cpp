@@ -128,6 +128,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     w->gather_mode = ctx->gather_mode;
     w->trace_stream = ctx->trace_stream;
     w->photon_tiles = ctx->photon_tiles;
+    w->gather_split = ctx->gather_split;
     w->counting = false;
   }
   return true;
@@ -776,12 +777,45 @@ int dl_set_photon_tiles(dl_ctx* ctx, int enable) {
   return DL_OK;
 }
 
+int dl_set_gather_split(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->gather_split = enable ? 1u : 0u;
+  return DL_OK;
+}
+
+int dl_test_gather_directions(dl_ctx* ctx, const float* dir3, const float* z, const float* azimuth, const float* spread, uint64_t n,
+                              float* exact3, float* fast3, uint32_t* fast_ok) {
+  using namespace dl;
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  if (n && (!dir3 || !z || !azimuth || !spread || !exact3 || !fast3 || !fast_ok)) return set_err(ctx, DL_E_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "hipSetDevice failed");
+    if (n == 0) return DL_OK;
+    const CacheScope scope(&ctx->cache);
+    DBuf<float> a, zz, aa, ss, e, f;
+    DBuf<uint32_t> ok;
+    hipStream_t s = ctx->stream;
+    if (a.alloc(3 * n) || zz.alloc(n) || aa.alloc(n) || ss.alloc(n) || e.alloc(3 * n) || f.alloc(3 * n) || ok.alloc(n) ||
+        copy_to_device(ctx, a.p, dir3, 12 * n) || copy_to_device(ctx, zz.p, z, 4 * n) ||
+        copy_to_device(ctx, aa.p, azimuth, 4 * n) || copy_to_device(ctx, ss.p, spread, 4 * n) ||
+        launch_test_directions(s, ctx->trig, a.p, zz.p, aa.p, ss.p, n, e.p, f.p, ok.p) ||
+        copy_to_host(ctx, exact3, e.p, 12 * n) || copy_to_host(ctx, fast3, f.p, 12 * n) || copy_to_host(ctx, fast_ok, ok.p, 4 * n))
+      return set_err(ctx, DL_E_DEVICE, "device pass failed");
+    return DL_OK;
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, DL_E_OOM, "host allocation failed");
+  }
+}
+
 int dl_set_gather(dl_ctx* ctx, uint32_t mode, uint32_t block, uint32_t chunk, uint64_t chunk_rays, uint32_t blocks_per_cu,
                   uint32_t min_waves, uint32_t refill, uint32_t order) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   if (mode > 1) return set_err(ctx, DL_E_ARGUMENT, "gather mode must be 0 or 1");
   if (block && (block % 64 != 0 || block > 1024)) return set_err(ctx, DL_E_ARGUMENT, "block must be a multiple of 64 up to 1024");
   if (chunk_rays && chunk_rays < 4096) return set_err(ctx, DL_E_ARGUMENT, "chunks need at least 4096 rays");
+  if (chunk_rays > (1ull << 31)) return set_err(ctx, DL_E_ARGUMENT, "at most 2^31 rays per chunk (ray slots are 32-bit)");
   if (blocks_per_cu > 64) return set_err(ctx, DL_E_ARGUMENT, "at most 64 blocks per CU");
   if (chunk > (1u << 20)) return set_err(ctx, DL_E_ARGUMENT, "at most 2^20 rays per queue run");
   if (min_waves != 0 && min_waves != 8) return set_err(ctx, DL_E_ARGUMENT, "min_waves must be 0 or 8");

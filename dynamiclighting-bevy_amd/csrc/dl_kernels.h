@@ -261,6 +261,22 @@ hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* li
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
                               float4* rays, float* contrib, uint32_t order, uint32_t* tmask, uint32_t words, uint64_t* tcount);
+// The split sample setup (instead of launch_gather_rays, with the photon tiles): phase A decides
+// most samples with fast arithmetic and marks the others open (omask, ocount per job, n + 1
+// entries; tmask and tcount zeroed); after a scan of ocount and launch_gather_list over omask,
+// phase B evaluates the `*count` open slots of `list` exactly and fills tmask and tcount.
+hipError_t launch_gather_split(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, float4* rays,
+                               uint32_t order, uint32_t* omask, uint32_t* tmask, uint32_t words, uint64_t* ocount,
+                               uint64_t* tcount);
+hipError_t launch_gather_open(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
+                              float4* rays, float* contrib, uint32_t order, const uint32_t* list, const uint64_t* count,
+                              uint64_t max_count, uint32_t* tmask, uint32_t words, uint64_t* tcount);
+// Exact and fast gather sample directions (3 floats each per input) and whether the fast one
+// applies, for n inputs (a3: 3 floats each).
+hipError_t launch_test_directions(hipStream_t s, TrigTables trig, const float* a3, const float* z, const float* ang,
+                                  const float* sp, uint64_t n, float* exact3, float* fast3, uint32_t* fast_ok);
 // The slots of the traced rays of a chunk (a job's in sample order, jobs in order) from the
 // per-job bits `tmask` (words per job) and the scanned traced counts `toff`.
 hipError_t launch_gather_list(hipStream_t s, const uint32_t* tmask, uint32_t words, const uint64_t* toff, const uint64_t* rs,

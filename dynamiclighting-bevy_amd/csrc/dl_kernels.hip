@@ -7,6 +7,7 @@
 // trace neighbouring texels toward the same light.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <mutex>
 
 #include "dl_kernels.h"
@@ -1295,11 +1296,14 @@ __global__ void k_vis_finish(DevTris tr, DevMeshes ms, const DevLight* __restric
 }
 
 // ---- optimize: shadows.rs:54-74 with any_light shadows.rs:136-146 -------------------------------
+// One wave per entry: whether any texel of the triangle's centred rect (within its region) holds a
+// non-zero level, 64 texels per step, stopping at the first step that finds one.
 __global__ void k_optimize(DevTris tr, DevMeshes ms, DevEntries ent, DevRegions rg, const uint8_t* __restrict__ has_bounce,
                            uint8_t* keep) {
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ent.n; e += (uint64_t)gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t e = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); e < ent.n; e += (uint64_t)gridDim.x * waves) {
     if (has_bounce && has_bounce[e]) {
-      keep[e] = 1;
+      if (lane == 0) keep[e] = 1;
       continue;
     }
     uint32_t t = ent.tri[e], r = ent.region[e];
@@ -1310,14 +1314,16 @@ __global__ void k_optimize(DevTris tr, DevMeshes ms, DevEntries ent, DevRegions 
     uint32_t x0 = max(o.x0, rc[0]), x1 = min(o.x1, rc[2]), y0 = max(o.y0, rc[1]), y1 = min(o.y1, rc[3]);
     bool any = false;
     if (x0 <= x1 && y0 <= y1) {
-      for (uint32_t y = y0; y <= y1 && !any; ++y)
-        for (uint32_t x = x0; x <= x1; ++x)
-          if (region_level(rg, r, x, y)) {
-            any = true;
-            break;
-          }
+      const uint32_t w = x1 - x0 + 1;
+      const uint64_t cells = (uint64_t)w * (y1 - y0 + 1);
+      for (uint64_t c0 = 0; c0 < cells && !any; c0 += 64) {
+        const uint64_t c = c0 + lane;
+        bool lit = false;
+        if (c < cells) lit = region_level(rg, r, x0 + (uint32_t)(c % w), y0 + (uint32_t)(c / w)) != 0;
+        any = __ballot(lit) != 0;
+      }
     }
-    keep[e] = any ? 1 : 0;
+    if (lane == 0) keep[e] = any ? 1 : 0;
   }
 }
 
@@ -1843,7 +1849,7 @@ hipError_t launch_vis_finish(hipStream_t s, DevTris tris, DevMeshes meshes, cons
 hipError_t launch_optimize(hipStream_t s, DevTris tris, DevMeshes meshes, DevEntries ent, DevRegions reg,
                            const uint8_t* has_bounce, uint8_t* keep) {
   if (ent.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_optimize, dim3(grid_for(ent.n, kThreads)), dim3(kThreads), 0, s, tris, meshes, ent, reg, has_bounce, keep);
+  hipLaunchKernelGGL(k_optimize, dim3(grid_for(ent.n, kThreads / 64)), dim3(kThreads), 0, s, tris, meshes, ent, reg, has_bounce, keep);
   return hipGetLastError();
 }
 hipError_t launch_pad_seams(hipStream_t s, DevMeshes meshes, DevRegions reg, const uint32_t* used_regions,
@@ -2462,6 +2468,7 @@ struct GatherJob {
 __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMeshes& ms, const DevLight* __restrict__ lights,
                                                 const DevTexels& tx, const DevEntries& ent, const DevBounce& b,
                                                 const BounceParams& p, uint32_t j, uint32_t e, Pcg* rng, uint32_t* mesh) {
+  // rng: the job's sample stream (null when the caller brings the draws)
   const uint32_t t = ent.tri[e], slot = ent.light[e];
   const DevLight& L = lights[slot];
   GatherJob g;
@@ -2477,8 +2484,10 @@ __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMesh
   g.tile0 = (uint64_t)b.cube_of_slot[slot] * 6 * p.tiles_per_face * p.tiles_per_face;
   g.o_scale = fabsf(g.origin.x) + fabsf(g.origin.y) + fabsf(g.origin.z);
   const uint32_t m = tr.mesh[t];
-  const uint32_t xy = tx.xy[j];
-  *rng = stream5(p.seed, ms.id[m], L.id, xy & 0xffffu, xy >> 16);
+  if (rng) {
+    const uint32_t xy = tx.xy[j];
+    *rng = stream5(p.seed, ms.id[m], L.id, xy & 0xffffu, xy >> 16);
+  }
   *mesh = m;
   return g;
 }
@@ -2538,21 +2547,10 @@ __device__ __forceinline__ void trig_finish_n(const TrigTables& t, const TrigTab
   }
 }
 
-// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples), each as in
-// bounce.rs:262-286 (spread, random direction, slerp, photon, distance, weight): random draws in
-// sample order, then the sin/cos searches, the photon tile reads and the photon fetches of the G
-// samples issued together.  A sample whose tile shows that it casts no ray, or a ray that brings
-// no light, skips its photon fetch; its outputs are those the photon would have given (ray false,
-// or ray true with a zero contribution, which is never traced).  la = |g.dir0|.
+// The random draws of on_unit_sphere (random.rs:22-28) for samples k0 .. k0 + G - 1 (those below
+// g.samples; the others get 0): z in [-1, 1) and the azimuth, in sample order.
 template <int G>
-__device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k0, float la,
-                                             const BounceParams& p, f3* dout, float* dist, f3* cout, bool* ray,
-                                             uint32_t* reads) {
-  const uint32_t res = p.cube_res, colored = p.colored;
-  const f3 a = g.dir0;
-  // on_unit_sphere (random.rs:22-28): the draws, then sin/cos of the azimuths
-  float z[G], ang[G];
-  double vs[G], vc[G];
+__device__ __forceinline__ void gather_draws(const GatherJob& g, Pcg& rng, uint32_t k0, float* z, float* ang) {
 #pragma unroll
   for (int j = 0; j < G; ++j) {
     z[j] = 0.0f;
@@ -2561,6 +2559,25 @@ __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTable
       z[j] = pcg_f32(rng) * 2.0f - 1.0f;
       ang[j] = pcg_f32(rng) * kTau;
     }
+  }
+}
+
+// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples) from their draws, each
+// as in bounce.rs:262-286 (spread, random direction, slerp, photon, distance, weight): the sin/cos
+// searches, the photon tile reads and the photon fetches of the G samples issued together.  A
+// sample whose tile shows that it casts no ray, or a ray that brings no light, skips its photon
+// fetch; its outputs are those the photon would have given (ray false, or ray true with a zero
+// contribution, which is never traced).  la = |g.dir0|.
+template <int G>
+__device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
+                                             uint32_t k0, float la, const BounceParams& p, f3* dout, float* dist, f3* cout,
+                                             bool* ray, uint32_t* reads) {
+  const uint32_t res = p.cube_res, colored = p.colored;
+  const f3 a = g.dir0;
+  // on_unit_sphere: sin/cos of the azimuths
+  double vs[G], vc[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
     int q;
     const double r = trig_reduce(ang[j], &q);
     vs[j] = trig_select(r, q, 0);
@@ -2666,6 +2683,15 @@ __device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTable
     dist[j] = dl;
     cout[j] = mul(alb, weight);
   }
+}
+// Draws and evaluates samples k0 .. k0 + G - 1 in order.
+template <int G>
+__device__ __forceinline__ void gather_group(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k0, float la,
+                                             const BounceParams& p, f3* dout, float* dist, f3* cout, bool* ray,
+                                             uint32_t* reads) {
+  float z[G], ang[G];
+  gather_draws<G>(g, rng, k0, z, ang);
+  gather_exact<G>(g, trig, z, ang, k0, la, p, dout, dist, cout, ray, reads);
 }
 
 // Job lists: pass 0 counts, pass 1 writes.  counts[0..1]: traced / count-only jobs (pass 0),
@@ -2838,6 +2864,273 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
     wave_add_at(b.rays, m, nr, i < n);
     if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
     if (p.photon_reads) wave_add(p.photon_reads, reads);
+  }
+}
+
+// ---- split sample setup: a fast pass over every sample, the exact chain for the few it leaves --
+// Phase A (k_gather_split) follows each sample with fast f32 arithmetic (hardware sin, cos,
+// reciprocal and square roots): where that direction's cube texel is certain (away from the face
+// and tile edges by more than its error can move it) and the photon tiles decide the sample (no
+// lit photon in any texel it can fall in, or a ray that brings no light), the sample needs nothing
+// more.  Every other sample is open: phase A keeps its random draws in its ray slot, and phase B
+// (k_gather_open) evaluates it with the exact chain (gather_exact), so every result is that of
+// gather_group.
+//
+// kSplitDirErr bounds the difference of each component of the fast unit direction from the
+// exact one wherever |cos| of the slerp angle is at most kSplitCos (samples beyond are open: the
+// slerp's 1 / sin(angle) amplifies the fast sines' errors there).  The GPU test of
+// dl_test_gather_directions measures that difference on 16M samples drawn as the bake draws them
+// and requires it below a tenth of the bound (about 6e-6 is measured).
+static constexpr float kSplitCos = 0.99f;
+static constexpr float kSplitDirErr = 1.0e-4f;
+
+// glam's slerp and normalize of gather_exact in fast arithmetic; false beyond kSplitCos.
+__device__ __forceinline__ bool fast_direction(f3 a, float la, float inv_la, float z, float ang, float sp, f3* out) {
+  float sa, ca;
+  __sincosf(ang, &sa, &ca);
+  const float r = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-z, z, 1.0f), 0.0f));
+  const f3 rnd = mk3(r * ca, r * sa, z);
+  const float lb2 = fmaf(rnd.x, rnd.x, fmaf(rnd.y, rnd.y, rnd.z * rnd.z));
+  const float ilb = __builtin_amdgcn_rsqf(lb2);
+  const float lb = lb2 * ilb;
+  const float dd = fmaf(a.x, rnd.x, fmaf(a.y, rnd.y, a.z * rnd.z)) * (inv_la * ilb);
+  if (!(fabsf(dd) <= kSplitCos)) return false;
+  const float x = fabsf(dd);
+  float t = fmaf(fmaf(fmaf(fmaf(fmaf(fmaf(fmaf(-0.0012624911f, x, 0.0066700901f), x, -0.0170881256f), x, 0.0308918810f), x,
+                                -0.0501743046f), x, 0.0889789874f), x, -0.2145988016f), x, 1.5707963050f);
+  t *= __builtin_amdgcn_sqrtf(1.0f - x);
+  const float theta = dd >= 0.0f ? t : kPi - t;
+  const float inv = __builtin_amdgcn_rcpf(__sinf(theta));
+  const float rl = fmaf(lb - la, sp, la);
+  const float wa = rl * inv_la * __sinf(theta * (1.0f - sp)) * inv, wb = rl * ilb * __sinf(theta * sp) * inv;
+  const f3 sl = mk3(fmaf(a.x, wa, rnd.x * wb), fmaf(a.y, wa, rnd.y * wb), fmaf(a.z, wa, rnd.z * wb));
+  const float il = __builtin_amdgcn_rsqf(fmaf(sl.x, sl.x, fmaf(sl.y, sl.y, sl.z * sl.z)));
+  *out = mk3(sl.x * il, sl.y * il, sl.z * il);
+  return true;
+}
+
+// The face of unit direction d (cube_texel_parts' choice) and its continuous texel coordinates
+// (u * res, v * res); *gap: how far d is from choosing another face (the largest absolute
+// component minus the next one).
+__device__ __forceinline__ void fast_face(f3 d, float res, uint32_t* face, float* tu, float* tv, float* gap) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  float sc, tc, ma;
+  if (ax > ay && ax > az) {
+    if (d.x > 0.0f) { *face = 0; sc = -d.z; tc = -d.y; } else { *face = 1; sc = d.z; tc = -d.y; }
+    ma = ax;
+    *gap = ax - fmaxf(ay, az);
+  } else if (ay > az) {
+    if (d.y > 0.0f) { *face = 2; sc = d.x; tc = d.z; } else { *face = 3; sc = d.x; tc = -d.z; }
+    ma = ay;
+    *gap = ay - fmaxf(ax, az);
+  } else {
+    if (d.z > 0.0f) { *face = 4; sc = d.x; tc = -d.y; } else { *face = 5; sc = -d.x; tc = -d.y; }
+    ma = az;
+    *gap = az - fmaxf(ax, ay);
+  }
+  const float im = __builtin_amdgcn_rcpf(ma);
+  *tu = fmaf(sc * im, 0.5f, 0.5f) * res;
+  *tv = fmaf(tc * im, 0.5f, 0.5f) * res;
+}
+
+// The texels a sample's exact direction can fall in, from its fast direction: {x0, x1} x {y0, y1}
+// of one face (their coordinates widened by the margin, clamped as cube_texel_parts clamps).
+struct FastCand {
+  bool sure;        // the face is certain (else the sample is open)
+  uint64_t tile[4];
+  uint32_t bit[4];
+  unsigned long long w0;  // lit bits of tile[0]
+};
+// The candidate texels of fast direction d and the lit word of the first one's tile.
+__device__ __forceinline__ FastCand fast_candidates(const GatherJob& g, const BounceParams& p, f3 d, float fres, float margin) {
+  FastCand c;
+  uint32_t face;
+  float tu, tv, gap;
+  fast_face(d, fres, &face, &tu, &tv, &gap);
+  c.sure = gap > 2.5f * kSplitDirErr;
+  const int32_t hi = (int32_t)p.cube_res - 1;
+  const uint32_t x0 = (uint32_t)min(max((int32_t)floorf(tu - margin), 0), hi), x1 = (uint32_t)min(max((int32_t)floorf(tu + margin), 0), hi);
+  const uint32_t y0 = (uint32_t)min(max((int32_t)floorf(tv - margin), 0), hi), y1 = (uint32_t)min(max((int32_t)floorf(tv + margin), 0), hi);
+  c.tile[0] = g.tile0 + photon_tile(face, x0, y0, p.tiles_per_face, &c.bit[0]);
+  c.tile[1] = g.tile0 + photon_tile(face, x1, y0, p.tiles_per_face, &c.bit[1]);
+  c.tile[2] = g.tile0 + photon_tile(face, x0, y1, p.tiles_per_face, &c.bit[2]);
+  c.tile[3] = g.tile0 + photon_tile(face, x1, y1, p.tiles_per_face, &c.bit[3]);
+  c.w0 = p.tile_live[c.tile[0]];
+  return c;
+}
+// Whether the tiles decide a sample whose exact texel is one of its candidates: 1 no ray (none of
+// them holds a lit photon), 2 a ray that brings no light (all of them hold one, and every tile
+// holding one of them is dark for this receiver), 0 open.
+__device__ __forceinline__ uint32_t fast_decide(const GatherJob& g, const BounceParams& p, const FastCand& c) {
+  if (!c.sure) return 0u;
+  const uint64_t* tile = c.tile;
+  const uint32_t* bit = c.bit;
+  const unsigned long long w0 = c.w0;
+  // the candidates' lit bits: one word when they share a tile (the usual case)
+  const bool one = tile[1] == tile[0] && tile[2] == tile[0] && tile[3] == tile[0];
+  uint32_t lit = 0;
+  if (one) {
+    lit = (uint32_t)__popcll(w0 & ((1ull << bit[0]) | (1ull << bit[1]) | (1ull << bit[2]) | (1ull << bit[3])));
+    lit = lit == 0u ? 0u : lit == (uint32_t)__popcll((1ull << bit[0]) | (1ull << bit[1]) | (1ull << bit[2]) | (1ull << bit[3])) ? 4u : 1u;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) lit += (uint32_t)(((c == 0 || tile[c] == tile[0] ? w0 : p.tile_live[tile[c]]) >> bit[c]) & 1ull);
+  }
+  if (lit == 0u) return 1u;
+  if (lit != 4u) return 0u;
+  if (!tile_dark(p.tile_box[3 * tile[0]], p.tile_box[3 * tile[0] + 1], p.tile_box[3 * tile[0] + 2], g.origin, g.normal, g.o_scale))
+    return 0u;
+  if (!one) {
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+      if (tile[c] == tile[0]) continue;
+      if (!tile_dark(p.tile_box[3 * tile[c]], p.tile_box[3 * tile[c] + 1], p.tile_box[3 * tile[c] + 2], g.origin, g.normal,
+                     g.o_scale))
+        return 0u;
+    }
+  }
+  return 2u;
+}
+
+// Phase A: per job its samples in order; open samples keep (z, azimuth) in their ray slot and set
+// their bit of omask (ocount: open samples per job); tmask and tcount start at zero for phase B.
+// Counts the rays of the samples it decides.  Needs the photon tiles (p.tile_live).
+__global__ void k_gather_split(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
+                               BounceParams p, float4* rays, uint32_t order, uint32_t* omask, uint32_t* tmask, uint32_t words,
+                               uint64_t* ocount, uint64_t* tcount) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const float fres = (float)p.cube_res;
+  const float margin = 3.0f * kSplitDirErr * fres + 1.0e-3f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ocount[n] = 0;
+    tcount[n] = 0;
+  }
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    const uint64_t i = i0 + lane;
+    uint32_t m = 0xffffffffu, nr = 0, win = 0;
+    if (i < n) {
+      const uint2 jb = jobs[i];
+      win = b.ent_window[jb.y];
+      Pcg rng;
+      const GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
+      const uint64_t base = rs[i];
+      const float la = len(g.dir0), inv_la = 1.0f / la;
+      uint32_t word = 0, open = 0;
+      // two samples per trip: their directions, candidate texels and tile reads side by side
+      for (uint32_t k0 = 0; k0 < g.samples; k0 += 2) {
+        float z[2], ang[2];
+        gather_draws<2>(g, rng, k0, z, ang);
+        FastCand c[2];
+        bool fast[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          f3 d = mk3(0.0f, 0.0f, 1.0f);
+          fast[j] = fast_direction(g.dir0, la, inv_la, z[j], ang[j], 0.1f + (float)(k0 + j) * g.step, &d);
+          c[j] = fast_candidates(g, p, d, fres, margin);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const uint32_t k = k0 + j;
+          if (k >= g.samples) break;
+          const uint32_t v = fast[j] ? fast_decide(g, p, c[j]) : 0u;
+          nr += v == 2u ? 1u : 0u;
+          if (v == 0u) {
+            rays[2 * ray_slot(order, base, i, n, k)] = make_float4(z[j], ang[j], 0.0f, 0.0f);
+            word |= 1u << (k & 31u);
+            open++;
+          }
+          if ((k & 31u) == 31u) {
+            omask[i * words + (k >> 5)] = word;
+            tmask[i * words + (k >> 5)] = 0u;
+            word = 0;
+          }
+        }
+      }
+      for (uint32_t w = g.samples >> 5; w < words; ++w) {
+        omask[i * words + w] = w == (g.samples >> 5) ? word : 0u;
+        tmask[i * words + w] = 0u;
+      }
+      ocount[i] = open;
+      tcount[i] = 0;
+    }
+    wave_add_at(b.rays, m, nr, i < n);
+    if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
+  }
+}
+
+// Phase B: the open samples (slots list[0 .. *count)), each with the exact chain from its draws;
+// traced ones write their ray and contribution, set their bit of tmask and add to tcount.
+__global__ void k_gather_open(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* __restrict__ jobs, uint64_t n, const uint64_t* __restrict__ rs,
+                              BounceParams p, TrigTables trig, float4* rays, float* contrib, uint32_t order,
+                              const uint32_t* __restrict__ list, const uint64_t* __restrict__ count, uint32_t* tmask,
+                              uint32_t words, unsigned long long* tcount) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, total = *count;
+  for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); q0 < total; q0 += stride) {
+    const uint64_t q = q0 + lane;
+    uint32_t m = 0xffffffffu, nr = 0, win = 0, reads = 0;
+    if (q < total) {
+      const uint32_t r = list[q];
+      uint64_t i;
+      uint32_t k;
+      if (order) {  // slot k * n + i (slots and n are below 2^32)
+        k = r / (uint32_t)n;
+        i = r - k * (uint32_t)n;
+      } else {
+        i = owner_of(rs, n, r);
+        k = (uint32_t)(r - rs[i]);
+      }
+      const uint2 jb = jobs[i];
+      win = b.ent_window[jb.y];
+      const GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, nullptr, &m);
+      const float4 dr = rays[2 * r];
+      const float z = dr.x, ang = dr.y;
+      f3 d, c;
+      float dist;
+      bool ray;
+      gather_exact<1>(g, trig, &z, &ang, k, len(g.dir0), p, &d, &dist, &c, &ray, &reads);
+      nr = ray ? 1u : 0u;
+      if (ray && !(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
+        const uint32_t alpha = lights[ent.light[jb.y]].alpha ? 1u : 0u;
+        rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist);
+        rays[2 * r + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(alpha | 2u));
+        contrib[3 * r] = c.x;
+        contrib[3 * r + 1] = c.y;
+        contrib[3 * r + 2] = c.z;
+        atomicOr(&tmask[i * words + (k >> 5)], 1u << (k & 31u));
+        atomicAdd(&tcount[i], 1ull);
+      }
+    }
+    wave_add_at(b.rays, m, nr, q < total);
+    if (b.wrays) wave_add_at(b.wrays, win, nr, q < total);
+    if (p.photon_reads) wave_add(p.photon_reads, reads);
+  }
+}
+
+// The exact and the fast direction of gather samples (a3: the job's direction to the texel, 3
+// floats; z, ang: the draws; sp: the spread), for the test of kSplitDirErr.
+__global__ void k_test_directions(TrigTables trig, const float* __restrict__ a3, const float* __restrict__ z,
+                                  const float* __restrict__ ang, const float* __restrict__ sp, uint64_t n, float* exact3,
+                                  float* fast3, uint32_t* fast_ok) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const f3 a = mk3(a3[3 * i], a3[3 * i + 1], a3[3 * i + 2]);
+    const float r = sqrtf(maxf_(1.0f - z[i] * z[i], 0.0f));
+    float sa, ca;
+    lsincosf(trig, ang[i], &sa, &ca);
+    const f3 rnd = mk3(r * ca, r * sa, z[i]);
+    const f3 e = normalize_or(slerp3(trig, a, rnd, sp[i]), a);
+    const float la = len(a);
+    f3 f = mk3(0.0f, 0.0f, 0.0f);
+    fast_ok[i] = fast_direction(a, la, 1.0f / la, z[i], ang[i], sp[i], &f) ? 1u : 0u;
+    exact3[3 * i] = e.x;
+    exact3[3 * i + 1] = e.y;
+    exact3[3 * i + 2] = e.z;
+    fast3[3 * i] = f.x;
+    fast3[3 * i + 1] = f.y;
+    fast3[3 * i + 2] = f.z;
   }
 }
 
@@ -3537,6 +3830,32 @@ hipError_t launch_gather_count(hipStream_t s, DevTris tris, DevMeshes meshes, co
 }
 hipError_t launch_gather_nrays(hipStream_t s, DevEntries ent, const DevLight* lights, const uint2* jobs, uint64_t n, uint64_t* out) {
   hipLaunchKernelGGL(k_gather_nrays, dim3(grid_for(n + 1, kThreads)), dim3(kThreads), 0, s, ent, lights, jobs, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_gather_split(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                               DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, float4* rays,
+                               uint32_t order, uint32_t* omask, uint32_t* tmask, uint32_t words, uint64_t* ocount,
+                               uint64_t* tcount) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_split, dim3(grid_for(n, 128)), dim3(128), 0, s, tris, meshes, lights, tx, ent, b, jobs, n, rs, p, rays,
+                     order, omask, tmask, words, ocount, tcount);
+  return hipGetLastError();
+}
+hipError_t launch_gather_open(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, const uint64_t* rs, BounceParams p, TrigTables trig,
+                              float4* rays, float* contrib, uint32_t order, const uint32_t* list, const uint64_t* count,
+                              uint64_t max_count, uint32_t* tmask, uint32_t words, uint64_t* tcount) {
+  if (n == 0 || max_count == 0) return hipSuccess;
+  // a grid for at most 2^20 lanes (the loop strides over the device-side count)
+  hipLaunchKernelGGL(k_gather_open, dim3(grid_for(std::min<uint64_t>(max_count, 1ull << 20), 128)), dim3(128), 0, s, tris, meshes,
+                     lights, tx, ent, b, jobs, n, rs, p, trig, rays, contrib, order, list, count, tmask, words,
+                     reinterpret_cast<unsigned long long*>(tcount));
+  return hipGetLastError();
+}
+hipError_t launch_test_directions(hipStream_t s, TrigTables trig, const float* a3, const float* z, const float* ang,
+                                  const float* sp, uint64_t n, float* exact3, float* fast3, uint32_t* fast_ok) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_test_directions, dim3(grid_for(n, 256)), dim3(256), 0, s, trig, a3, z, ang, sp, n, exact3, fast3, fast_ok);
   return hipGetLastError();
 }
 hipError_t launch_gather_rays(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,

@@ -150,6 +150,11 @@ def load() -> C.CDLL:
     lib.dl_autotune_batch.restype = C.c_int
     lib.dl_autotune_batch.argtypes = [P, C.POINTER(abi.dl_bake_input), C.POINTER(abi.dl_bake_options), C.c_uint32,
                                       C.c_uint32, C.c_char_p, C.POINTER(abi.dl_tuned_launch)]
+    lib.dl_set_gather_split.restype = C.c_int
+    lib.dl_set_gather_split.argtypes = [P, C.c_int]
+    f32p = C.POINTER(C.c_float)
+    lib.dl_test_gather_directions.restype = C.c_int
+    lib.dl_test_gather_directions.argtypes = [P, f32p, f32p, f32p, f32p, C.c_uint64, f32p, f32p, C.POINTER(C.c_uint32)]
     lib.dl_set_photon_tiles.restype = C.c_int
     lib.dl_set_photon_tiles.argtypes = [P, C.c_int]
     lib.dl_set_gather.restype = C.c_int
@@ -457,6 +462,26 @@ class Context:
     def set_trace_stream(self, enable: bool) -> None:
         """Area lights, soft suns and skies through the streamed any-hit path (default) or per lane."""
         self.lib.dl_set_trace_stream(self.handle, 1 if enable else 0)
+
+    def set_gather_split(self, enable: bool) -> None:
+        """The gather's sample setup as a fast pass plus an exact pass for the samples it leaves open
+        (default), or the exact chain for every sample; the results are the same."""
+        self.lib.dl_set_gather_split(self.handle, 1 if enable else 0)
+
+    def gather_directions(self, dirs, z, azimuth, spread):
+        """dl_test_gather_directions: (exact (n, 3), fast (n, 3), fast_ok (n,)) float32/bool arrays."""
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        z, azimuth, spread = (np.ascontiguousarray(v, np.float32) for v in (z, azimuth, spread))
+        n = len(z)
+        exact = np.zeros((n, 3), np.float32)
+        fast = np.zeros((n, 3), np.float32)
+        ok = np.zeros(n, np.uint32)
+        f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        st = self.lib.dl_test_gather_directions(self.handle, f(dirs), f(z), f(azimuth), f(spread), n, f(exact), f(fast),
+                                                ok.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if st != 0:
+            raise BakeError(st, self.error())
+        return exact, fast, ok.astype(bool)
 
     def set_photon_tiles(self, enable: bool) -> None:
         """The point-light gather skips the photon reads its cube's tile summaries decide (default)

@@ -80,6 +80,19 @@ int dl_set_trace_stream(dl_ctx* ctx, int enable);
  * no ray or brings no light; 0: every sample reads its photon.  Results are identical. */
 int dl_set_photon_tiles(dl_ctx* ctx, int enable);
 
+/* 1 (default): with the photon tiles, the gather's sample setup runs in two passes: every sample
+ * once in fast f32 arithmetic (hardware sin, cos, reciprocal and square roots), which decides the
+ * samples whose cube texel is certain and whose tile settles them, and the exact chain for the
+ * others; 0: the exact chain for every sample.  Results are identical. */
+int dl_set_gather_split(dl_ctx* ctx, int enable);
+
+/* The sample directions of the gather for n inputs (dir3: the job's direction to its texel, 3
+ * floats; z and azimuth: on_unit_sphere's draws; spread: the slerp factor): exact3 as the bake
+ * computes them (bounce.rs:262-270), fast3 as the split setup's fast pass does, fast_ok[i] = 1
+ * when the fast pass applies to input i.  For the test of the fast pass's error bound. */
+int dl_test_gather_directions(dl_ctx* ctx, const float* dir3, const float* z, const float* azimuth, const float* spread,
+                              uint64_t n, float* exact3, float* fast3, uint32_t* fast_ok);
+
 /* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
  * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
  * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per

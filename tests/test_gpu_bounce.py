@@ -104,3 +104,50 @@ def test_photon_tiles_change_no_result(ctx, res):
     want_scene, want = oracle.bounce_crops(inp, o)
     assert_scenes_equal(tiled, want_scene)
     compare_crops(tiled_crops, want)
+
+
+SPLIT_DIR_ERR = 1.0e-4  # kSplitDirErr: the bound the split setup's margins assume on each component
+
+
+def test_fast_directions_stay_far_within_the_split_bound(ctx):
+    """The split setup's fast pass (hardware sin/cos/rcp/sqrt) against the exact chain on 16M samples
+    drawn as the bake draws them (z and azimuth from 24-bit uniforms, the 32 spreads of a 32-sample
+    light, random unit directions to the texel): every component within a tenth of the bound."""
+    rng = np.random.default_rng(7)
+    n = 1 << 24
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    u1 = (rng.integers(0, 1 << 24, n) * np.float32(1.0 / 16777216.0)).astype(np.float32)
+    u2 = (rng.integers(0, 1 << 24, n) * np.float32(1.0 / 16777216.0)).astype(np.float32)
+    z = u1 * np.float32(2.0) - np.float32(1.0)
+    ang = u2 * np.float32(6.2831855)
+    step = np.float32(0.9) / np.float32(31)
+    sp = (np.float32(0.1) + rng.integers(0, 32, n).astype(np.float32) * step).astype(np.float32)
+    exact, fast, ok = ctx.gather_directions(d.astype(np.float32), z, ang, sp)
+    assert ok.mean() > 0.98
+    err = np.abs(exact[ok].astype(np.float64) - fast[ok]).max()
+    print(f"fast pass: {ok.mean():.4f} of samples, largest component error {err:.3e} (bound {SPLIT_DIR_ERR:.1e})", flush=True)
+    assert err < SPLIT_DIR_ERR / 10
+    assert np.all(np.abs(np.linalg.norm(exact.astype(np.float64), axis=1) - 1.0) < 1e-5)
+
+
+@pytest.mark.parametrize("res", [5, 37, 512])
+def test_split_setup_changes_no_result(ctx, res):
+    """Bakes with the split sample setup (default) and with the exact chain for every sample are
+    identical (words, bounce crops, stats) and equal the oracle."""
+    inp, o = scene_cases.bounce_point_case()
+    o.photon_cube_resolution = res
+    print(f"split setup at cube resolution {res}", flush=True)
+    ctx.set_gather_split(False)
+    try:
+        plain, plain_crops = ctx.bake_bounce_crops(inp, o)
+    finally:
+        ctx.set_gather_split(True)
+    split, split_crops = ctx.bake_bounce_crops(inp, o)
+    assert_scenes_equal(split, plain)
+    assert split_crops.keys() == plain_crops.keys()
+    for k in plain_crops:
+        assert np.array_equal(split_crops[k].view(np.uint32), plain_crops[k].view(np.uint32)), k
+    want_scene, want = oracle.bounce_crops(inp, o)
+    assert_scenes_equal(split, want_scene)
+    compare_crops(split_crops, want)
