@@ -315,8 +315,11 @@ struct CapturedPair {
   std::vector<uint64_t> bounce_off, bounce_words;  // per kept triangle: its bounce payload in `words`
   std::shared_ptr<DevBlock> block, words;
 };
-// Burial rays that hit nothing kept per part of a rebake service (8 floats each: 128 MiB).
-constexpr uint64_t kEscapingCap = 1ull << 22;
+// Burial rays that hit nothing kept per part of a rebake service (8 floats each): up to 2^27 for
+// the scene a service is created with (4 GiB; the 200k-triangle interior sends 119 M of them out),
+// up to 2^22 for a room that joins it (128 MiB).
+constexpr uint64_t kEscapingCap = 1ull << 27;
+constexpr uint64_t kRoomEscapingCap = 1ull << 22;
 struct Capture {
   std::vector<CapturedPair> pairs;
   std::vector<std::pair<uint64_t, std::vector<uint32_t>>> cubes;  // distance cube words per local light id
@@ -396,6 +399,7 @@ struct dl_service {
   // Capture::escaping); esc_open when a part had more than its capture kept
   std::shared_ptr<dl::DevBlock> esc;
   uint64_t nesc = 0;
+  uint64_t esc_total = 0;  // escaping rays of the scene the service was created with (also past the cap)
   bool esc_open = false;
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id
   std::unordered_map<uint64_t, uint32_t> slot;  // light id -> slot in the live words

@@ -695,8 +695,8 @@ int trace_room(dl_ctx* w, dl_service* svc, const dl_bake_input* room, RoomTrace&
   if (!bounds_of(room, t.part.bmin, t.part.bmax)) return set_error(w, DL_E_ARGUMENT, "the room has no finite vertex");
   t.res.reset(new Resident());
   t.cap.escaping = std::make_shared<DevBlock>();
-  t.cap.escaping_cap = kEscapingCap;
-  HIP_TRY(t.cap.escaping->alloc(8 * kEscapingCap + 2, w->stream));
+  t.cap.escaping_cap = kRoomEscapingCap;
+  HIP_TRY(t.cap.escaping->alloc(8 * kRoomEscapingCap + 2, w->stream));
   dl_baked_scene* sc = nullptr;
   const int st = dl_bake_impl(w, room, &svc->options, nullptr, &sc, t.res.get(), &t.cap);
   delete sc;
@@ -731,8 +731,13 @@ int join_room(dl_service* svc, const dl_bake_input* room, RoomTrace& t, std::vec
   }
   for (const ScenePart& p : svc->parts)
     if (boxes_meet(p, t.part)) return set_error(ctx, DL_E_ARGUMENT, "the room's bounds meet the live scene's");
-  if (svc->esc_open || t.cap.escaping_count > t.cap.escaping_cap)
-    return set_error(ctx, DL_E_ARGUMENT, "too many burial rays leave a part of the scene to check a room's isolation");
+  if (svc->esc_open)
+    return set_error(ctx, DL_E_ARGUMENT, "too many burial rays leave the live scene to check a room's isolation (" +
+                                             std::to_string(svc->esc_total) + ", at most " + std::to_string(kEscapingCap) + ")");
+  if (t.cap.escaping_count > t.cap.escaping_cap)
+    return set_error(ctx, DL_E_ARGUMENT, "too many burial rays leave the room to check its isolation (" +
+                                             std::to_string(t.cap.escaping_count) + ", at most " +
+                                             std::to_string(t.cap.escaping_cap) + ")");
   uint64_t hits = 0;
   int st = rays_hitting(ctx, rr, reinterpret_cast<const float*>(svc->esc ? svc->esc->p : nullptr), svc->nesc, &hits);
   if (st != DL_OK) return st;
@@ -855,6 +860,7 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->esc = cap.escaping;
   svc->nesc = cap.escaping ? std::min(cap.escaping_count, cap.escaping_cap) : 0;
   svc->esc_open = !cap.escaping || cap.escaping_count > cap.escaping_cap;
+  svc->esc_total = cap.escaping_count;
   svc->W.reset();
   svc->slot.clear();
   svc->light_meshes.clear();
