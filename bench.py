@@ -19,7 +19,7 @@ the bound these kernels actually meet.  `other_kernels` gives the any-hit traver
 gather rays (k_any_stream: per ray, 112 B per 4-wide BVH node read plus 48 B per triangle tested
 plus its 32-byte ray record and 4-byte list entry) and the direct shadow-ray kernel (k_trace) as
 work rates counted per use of each record (mostly L2-served, so not HBM figures), beside the HBM
-bytes rocprofv3 measured for them.  `cpu_baseline` times the CPU oracle on a 1/256 receiver sample
+bytes rocprofv3 measured for them.  `cpu_baseline` times the CPU oracle on a 1/64 receiver sample
 of the same scene (every mesh still occludes) and extrapolates to the whole bake, and records the
 host CPU.  `batch` runs the 512-job batch workload in child processes at 4 (HIP's default) and 16
 hardware queues, and `serve` the rebake service at 20 requests/s (see --workload).
@@ -131,7 +131,7 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(inp, opts, full_samples, modulus=256, cube_lights=2):
+def cpu_baseline(inp, opts, full_samples, modulus=64, cube_lights=2):
     """The CPU oracle on a bounded sample, extrapolated to the whole bake.
 
     Meshes: receivers with splitmix64(id) % modulus == 0 baked against every occluder and light
