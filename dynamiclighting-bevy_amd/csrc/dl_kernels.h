@@ -117,6 +117,10 @@ hipError_t launch_words_hash(hipStream_t s, const uint32_t* words, uint64_t n, u
 // out[i] = i for i < n.
 hipError_t launch_iota_u64(hipStream_t s, uint64_t* out, uint64_t n);
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta);
+// Renumbers in place the light slots (through `table`, old slot -> new) of n meshes' words, mesh j
+// at base[j] with nt[j] triangles and `stride` words per light entry.
+hipError_t launch_remap_slots(hipStream_t s, uint32_t* words, const uint64_t* base, const uint32_t* nt, uint32_t n, uint32_t stride,
+                              const uint32_t* table);
 // The burial test with one lane per texel (block threads per block).
 hipError_t launch_burial_lanes(hipStream_t s, DevScene scene, DevTris tris, DevMeshes meshes, DevTexels tx, uint64_t ntex,
                                uint64_t seed, TrigTables trig, uint8_t* buried, unsigned long long* kept, uint32_t block);

@@ -1674,6 +1674,23 @@ __global__ void k_copy_ranges(const uint32_t* __restrict__ src, uint32_t* __rest
   }
 }
 
+// Renumbers the light slots of meshes' words in place: mesh j's words start at base[j] with nt[j]
+// triangle headers (4 words: the offset of its light data, then the crop), light data = entry
+// count, then per entry `stride` words whose first is the slot (triangle_words.rs:47-89).  One wave
+// per mesh, lanes over its triangles.
+__global__ void k_remap_slots(uint32_t* words, const uint64_t* __restrict__ base, const uint32_t* __restrict__ nt, uint32_t n,
+                              uint32_t stride, const uint32_t* __restrict__ table) {
+  const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+  for (uint64_t j = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); j < n; j += (uint64_t)gridDim.x * waves) {
+    uint32_t* w = words + base[j];
+    for (uint32_t t = lane; t < nt[j]; t += 64) {
+      uint32_t* ld = w + w[4 * (size_t)t];
+      const uint32_t count = ld[0];
+      for (uint32_t e = 0; e < count; ++e) ld[1 + (size_t)stride * e] = table[ld[1 + (size_t)stride * e]];
+    }
+  }
+}
+
 __global__ void k_add_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n, uint32_t delta) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = in[i] + delta;
@@ -1779,6 +1796,12 @@ __global__ void k_iota_u64(uint64_t* out, uint64_t n) {
 hipError_t launch_iota_u64(hipStream_t s, uint64_t* out, uint64_t n) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_iota_u64, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_remap_slots(hipStream_t s, uint32_t* words, const uint64_t* base, const uint32_t* nt, uint32_t n, uint32_t stride,
+                              const uint32_t* table) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_remap_slots, dim3(grid_for(n, kThreads / 64)), dim3(kThreads), 0, s, words, base, nt, n, stride, table);
   return hipGetLastError();
 }
 hipError_t launch_add_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t delta) {

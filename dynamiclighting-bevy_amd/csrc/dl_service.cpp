@@ -188,14 +188,15 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
 
   // ---- per mesh, on the host: entry order, word layout, light data ------------------------
   struct Plan {
-    bool whole = false;            // words unchanged: one copy of the mesh
+    bool whole = false;            // words unchanged (or only their slot fields): one copy of the mesh
+    bool remap = false;            // whole, with the slot fields of its light data renumbered
     std::vector<uint32_t> ld;      // headers and light data (the mesh's first words)
     uint64_t len = 0;
     std::vector<Entry> ent;        // kept entries in word order
     std::vector<uint64_t> sh, bo;  // per entry: shadow and bounce payload offset in the mesh
   };
   std::vector<Plan> plan(nm);
-  std::atomic<uint32_t> restaged{0};
+  std::atomic<uint32_t> restaged{0}, remapped{0};
   run_parallel(nm, ctx->host_threads, [&](uint64_t mi) {
     const LiveMesh& m = svc->mesh[mi];
     Plan& p = plan[mi];
@@ -210,6 +211,33 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       p.whole = true;
       p.len = m.w_len;
       return;
+    }
+    if (!dirty[mi] && !restride && svc->W) {
+      // pairs unchanged, slots renumbered: when the mesh's lights keep their order, every
+      // triangle's entries (slots ascending) keep theirs, so the words change in the slot fields
+      // only (renumbered on the device after the copy)
+      std::vector<std::pair<uint32_t, uint32_t>> os;
+      os.reserve(m.pairs.size());
+      bool known = true;
+      for (const LivePair& q : m.pairs) {
+        auto it = svc->slot.find(q.light_id);
+        if (it == svc->slot.end()) {
+          known = false;
+          break;
+        }
+        os.push_back({it->second, slot.at(q.light_id)});
+      }
+      if (known) {
+        std::sort(os.begin(), os.end());
+        bool kept = true;
+        for (size_t j = 1; j < os.size() && kept; ++j) kept = os[j - 1].second < os[j].second;
+        if (kept) {
+          p.whole = p.remap = true;
+          p.len = m.w_len;
+          remapped++;
+          return;
+        }
+      }
     }
     if (!dirty[mi]) restaged++;
     for (uint32_t pi = 0; pi < m.pairs.size(); ++pi) {
@@ -315,6 +343,30 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
     clock.line += " (ranges " + std::to_string(nr) + ", staged words " + std::to_string(staged.size()) + ")";
   }
   HIP_TRY(words.run(ctx, W2->p));
+  {  // the slot fields of the meshes copied whole with renumbered slots
+    std::vector<uint64_t> rbase;
+    std::vector<uint32_t> rnt;
+    for (uint32_t i = 0; i < nm; ++i)
+      if (plan[i].remap) {
+        rbase.push_back(base[i]);
+        rnt.push_back(svc->first_tri[i + 1] - svc->first_tri[i]);
+      }
+    if (!rbase.empty()) {
+      uint32_t old_n = 0;
+      for (const auto& kv : svc->slot) old_n = std::max(old_n, kv.second + 1);
+      std::vector<uint32_t> table(old_n, 0u);
+      for (const auto& kv : svc->slot) {
+        auto it = slot.find(kv.first);
+        if (it != slot.end()) table[kv.second] = it->second;
+      }
+      DBuf<uint64_t> d_rbase;
+      DBuf<uint32_t> d_rnt, d_table;
+      HIP_TRY(upload(ctx, d_rbase, rbase));
+      HIP_TRY(upload(ctx, d_rnt, rnt));
+      HIP_TRY(upload(ctx, d_table, table));
+      HIP_TRY(launch_remap_slots(s, W2->p, d_rbase.p, d_rnt.p, (uint32_t)rbase.size(), stride, d_table.p));
+    }
+  }
   clock.mark("word_copies", s);
 
   // ---- visit counts: the pending changes of merge, additions first ---------------------------
@@ -479,7 +531,7 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
   svc->stride = stride;
   svc->stats.triangle_words = total;
   clock.mark("state");
-  if (clock.on) clock.line += " restaged_meshes=" + std::to_string(restaged.load());
+  if (clock.on) clock.line += " restaged_meshes=" + std::to_string(restaged.load()) + " remapped_meshes=" + std::to_string(remapped.load());
   return DL_OK;
 }
 
