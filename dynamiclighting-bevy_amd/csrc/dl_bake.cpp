@@ -485,18 +485,31 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   DBuf<uint32_t> d_cube_slot, d_photons;
   DBuf<unsigned long long> d_tile_live;
   DBuf<float4> d_tile_box;
-  if (!cube_slot.empty()) {
-    uint64_t photons = 6ull * cres * cres * cube_slot.size();
-    HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
-    HIP_TRY(d_photons.alloc(photons * 4));  // one 16-byte record per photon (12 bytes used)
+  const uint64_t cube_texels = 6ull * cres * cres * cube_slot.size();
+  const uint64_t cube_tiles = 6ull * photon_tiles_per_face(cres) * photon_tiles_per_face(cres) * cube_slot.size();
+  // every texel of every cube, and their tiles
+  auto build_cubes = [&]() -> int {
+    HIP_TRY(d_photons.alloc(cube_texels * 4));  // one 16-byte record per photon (12 bytes used)
     HIP_TRY(launch_photons(s, sc, occ, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, 128));
     if (ctx->photon_tiles) {
-      const uint64_t tpf = photon_tiles_per_face(cres), tiles = 6 * tpf * tpf * cube_slot.size();
-      HIP_TRY(d_tile_live.alloc(tiles));
-      HIP_TRY(d_tile_box.alloc(3 * tiles));
+      HIP_TRY(d_tile_live.alloc(cube_tiles));
+      HIP_TRY(d_tile_box.alloc(3 * cube_tiles));
       HIP_TRY(launch_photon_tiles(s, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, d_tile_live.p,
                                   d_tile_box.p));
     }
+    ctx->timings.photon_texels = cube_texels;
+    return DL_OK;
+  };
+  // Lazy cubes wait for the gather's job lists (one mesh batch, the point-light gather with the
+  // split setup), which decide between tracing the texels they can read and every texel; their
+  // marks number texels 64 * tile + bit in 32 bits.
+  bool cubes_pending = ctx->lazy_photons && ctx->gather_mode == 1 && ctx->gather_split && ctx->photon_tiles &&
+                       !cube_slot.empty() && cube_tiles < (1ull << 26);
+  const uint64_t cube_bytes = cube_texels * 16 + cube_tiles * 56;
+  if (!cube_slot.empty()) {
+    HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
+    if (!cubes_pending)
+      if (int rc = build_cubes()) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     advance(cube_slot.size());
   }
@@ -610,6 +623,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
           hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
           hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used)
         avail += reserved - used;
+      if (cubes_pending) avail -= std::min(avail, cube_bytes);  // the cubes are built later, from this memory
       if (avail > (2ull << 30)) budget = avail / 2 / std::max(1u, ctx->share);
     }
   }
@@ -637,6 +651,10 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       acc += est;
     }
     if (nm > start) batches.push_back({start, nm});
+  }
+  if (cubes_pending && batches.size() > 1) {  // the gathers of several batches read the same cubes
+    cubes_pending = false;
+    if (int rc = build_cubes()) return rc;
   }
   ctx->timings.texels_ms = ms_since(t_tex);
 
@@ -1040,6 +1058,48 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
         HIP_TRY(d_sjobs.alloc(std::max<uint64_t>(1, jc[2])));
         HIP_TRY(launch_gather_jobs(s, tr, d_lights.p, tx, d_tex_start.p, ent, bb, 1, nullptr, d_joff.p, d_tjobs.p, d_cjobs.p,
                                    d_sjobs.p));
+        if (cubes_pending) {
+          // The cubes, now that the jobs are known: when they set up fewer samples than twice the
+          // cubes' texels, only the texels their samples can fall in (marked from the samples'
+          // fast directions, exact where those are unsure), else every texel.
+          cubes_pending = false;
+          uint32_t most = 2;
+          for (uint64_t e : be) most = std::max(most, hlights[b_ent_light[e]].bounce_samples);
+          if ((jc[0] + jc[1]) * (uint64_t)most > 2 * cube_texels) {
+            if (int rc = build_cubes()) return rc;
+          } else {
+            bp.tiles_per_face = photon_tiles_per_face(cres);
+            DBuf<uint8_t> d_need;
+            DBuf<unsigned long long> d_need_mask;
+            DBuf<uint64_t> d_ncnt, d_noff;
+            DBuf<uint32_t> d_nlist;
+            HIP_TRY(zeroed(ctx, d_need, 64 * cube_tiles));
+            HIP_TRY(launch_photon_mark(s, tr, ms, d_lights.p, tx, ent, bb, d_tjobs.p, jc[0], bp, ctx->trig, d_need.p));
+            HIP_TRY(launch_photon_mark(s, tr, ms, d_lights.p, tx, ent, bb, d_cjobs.p, jc[1], bp, ctx->trig, d_need.p));
+            HIP_TRY(d_need_mask.alloc(cube_tiles));
+            HIP_TRY(d_ncnt.alloc(cube_tiles + 1));
+            HIP_TRY(d_noff.alloc(cube_tiles + 1));
+            HIP_TRY(launch_need_count(s, d_need.p, cube_tiles, d_need_mask.p, d_ncnt.p));
+            HIP_TRY(scan_u64(s, d_ncnt.p, d_noff.p, cube_tiles, &ctx->scan_tmp, &ctx->scan_tmp_bytes));
+            uint64_t needed = 0;
+            HIP_TRY(hipMemcpyAsync(&needed, d_noff.p + cube_tiles, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(d_nlist.alloc(std::max<uint64_t>(1, needed)));
+            HIP_TRY(launch_need_list(s, d_need_mask.p, d_noff.p, cube_tiles, d_nlist.p));
+            HIP_TRY(d_photons.alloc(cube_texels * 4));
+            HIP_TRY(launch_photons_listed(s, sc, occ, d_lights.p, d_cube_slot.p, cres, d_nlist.p, needed, d_photons.p, 128));
+            HIP_TRY(d_tile_live.alloc(cube_tiles));
+            HIP_TRY(d_tile_box.alloc(3 * cube_tiles));
+            HIP_TRY(launch_photon_tiles(s, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, d_tile_live.p,
+                                        d_tile_box.p, d_need_mask.p));
+            HIP_TRY(hipStreamSynchronize(s));  // before the marks and lists go
+            ctx->timings.photon_texels = needed;
+          }
+          bp.photons = d_photons.p;
+          bp.tiles_per_face = photon_tiles_per_face(cres);
+          bp.tile_live = d_tile_live.p;
+          bp.tile_box = d_tile_box.p;
+        }
         HIP_TRY(launch_gather_count(s, tr, ms, d_lights.p, tx, ent, bb, d_cjobs.p, jc[1], bp, ctx->trig));
         bp.sun_only = 1;
       }

@@ -129,6 +129,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     w->trace_stream = ctx->trace_stream;
     w->photon_tiles = ctx->photon_tiles;
     w->gather_split = ctx->gather_split;
+    w->lazy_photons = ctx->lazy_photons;
     w->counting = false;
   }
   return true;
@@ -781,6 +782,13 @@ int dl_set_gather_split(dl_ctx* ctx, int enable) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->gather_split = enable ? 1u : 0u;
+  return DL_OK;
+}
+
+int dl_set_lazy_photons(dl_ctx* ctx, int enable) {
+  if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->lazy_photons = enable ? 1u : 0u;
   return DL_OK;
 }
 

@@ -166,6 +166,7 @@ struct dl_ctx {
   uint32_t gather_mode = 1;  // 0: one wave per (triangle, light, 64 texels); 1: streamed jobs
   uint32_t trace_stream = 1;  // 1: area lights, soft suns and skies through the streamed any-hit path
   uint32_t photon_tiles = 1;  // 1: the point-light gather reads photons only where the photon tiles leave a sample open
+  uint32_t lazy_photons = 1;  // 1: bakes with few gather samples per cube texel trace only the texels the samples can read
   uint32_t gather_split = 1;  // 1: the gather's sample setup in a fast pass and an exact pass for the samples it leaves
   unsigned host_threads = 1;
   bool counting = false;

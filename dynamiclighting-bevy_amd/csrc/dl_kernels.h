@@ -233,8 +233,22 @@ hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLi
                           uint32_t ncubes, uint32_t res, uint32_t* photons, uint32_t block);
 // Tile summaries of built photon cubes (BounceParams::tile_live / tile_box): 6 * tpf^2 tiles per
 // cube with tpf = photon_tiles_per_face(res); live holds one u64 per tile, box three float4.
+// need_mask (lazy cubes, else null): per tile the texels that hold a photon; the others count as
+// unlit.
 hipError_t launch_photon_tiles(hipStream_t s, const DevLight* lights, const uint32_t* cube_slot, uint32_t ncubes, uint32_t res,
-                               const uint32_t* photons, unsigned long long* live, float4* box);
+                               const uint32_t* photons, unsigned long long* live, float4* box,
+                               const unsigned long long* need_mask = nullptr);
+// Lazy photon cubes, for bakes whose gathers read few of their cubes' texels.  need: one byte per
+// cube texel in tile order (64 * tile + bit), zeroed; launch_photon_mark sets the bytes of the
+// texels a list of gather jobs can read (p.tiles_per_face set); launch_need_count turns them into
+// per-tile masks and counts (tiles + 1 entries, for a scan), launch_need_list lists the texels
+// (64 * tile + bit) at the scanned offsets, and launch_photons_listed traces those texels only.
+hipError_t launch_photon_mark(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, BounceParams p, TrigTables trig, uint8_t* need);
+hipError_t launch_need_count(hipStream_t s, const uint8_t* need, uint64_t tiles, unsigned long long* need_mask, uint64_t* cnt);
+hipError_t launch_need_list(hipStream_t s, const unsigned long long* need_mask, const uint64_t* off, uint64_t tiles, uint32_t* list);
+hipError_t launch_photons_listed(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
+                                 uint32_t res, const uint32_t* list, uint64_t total, uint32_t* photons, uint32_t block);
 hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,
                                DevEntries ent, DevBounce b, uint64_t units);
 hipError_t launch_bounce_gather(hipStream_t s, DevScene scene, DevOcc occ, DevTris tris, DevMeshes meshes,

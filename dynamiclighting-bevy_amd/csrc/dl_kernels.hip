@@ -2019,7 +2019,34 @@ __device__ __forceinline__ f3 surface_normal(const DevScene& sc, const DevOcc& o
   return try_normalize(s, &r) ? r : face;
 }
 
-// PhotonCube::build / photon (bounce.rs:48-115)
+// The photon of texel (face, x, y) of a light's cube (PhotonCube::photon, bounce.rs:48-115): the
+// 16-byte record (distance, packed lit-side normal, packed albedo, 0).
+__device__ __forceinline__ uint4 photon_record(const DevScene& sc, const DevOcc& occ, const DevLight& L, uint32_t face, uint32_t x,
+                                               uint32_t y, uint32_t res, uint32_t* stack, Tally& tl) {
+  f3 lp = ld3(L.pos);
+  f3 dir = cube_texel_dir(face, x, y, res);
+  Hit h;
+  uint32_t w0, w1 = 0, w2 = 0;
+  if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h, tl, L.alpha)) {
+    w0 = __float_as_uint(L.radius);
+  } else {
+    f3 normal = surface_normal(sc, occ, h);
+    f3 point = add(lp, mul(dir, h.t));
+    float bias = maxf_(h.t * 0.001f, 0.001f);
+    f3 nudged = add(sub(point, mul(dir, bias)), mul(normal, bias));
+    bool lit_side = dot(tri_face_normal(sc, h.pos), dir) < 0.0f;
+    w0 = __float_as_uint(len(sub(nudged, lp)));
+    if (lit_side)
+      w1 = ((uint32_t)sat_i8(roundf(normal.x * 127.0f)) & 0xffu) | (((uint32_t)sat_i8(roundf(normal.y * 127.0f)) & 0xffu) << 8) |
+           (((uint32_t)sat_i8(roundf(normal.z * 127.0f)) & 0xffu) << 16);
+    const float* a = occ.albedo + 3 * (size_t)h.occ;
+    w2 = sat_u8(roundf(clampf_(a[0], 0.0f, 1.0f) * 255.0f)) | (sat_u8(roundf(clampf_(a[1], 0.0f, 1.0f) * 255.0f)) << 8) |
+         (sat_u8(roundf(clampf_(a[2], 0.0f, 1.0f) * 255.0f)) << 16);
+  }
+  return make_uint4(w0, w1, w2, 0u);
+}
+
+// PhotonCube::build (bounce.rs:126-151): every texel of every cube.
 __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ lights, const uint32_t* __restrict__ cube_slot,
                           uint32_t ncubes, uint32_t res, uint32_t* photons) {
   extern __shared__ uint32_t lds[];
@@ -2033,28 +2060,62 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
     uint32_t face = (uint32_t)(rem / ((uint64_t)res * res));
     uint32_t yx = (uint32_t)(rem % ((uint64_t)res * res));
     uint32_t y = yx / res, x = yx % res;
-    const DevLight& L = lights[cube_slot[c]];
-    f3 lp = ld3(L.pos);
-    f3 dir = cube_texel_dir(face, x, y, res);
-    Hit h;
-    uint32_t w0, w1 = 0, w2 = 0;
-    if (!traverse<false>(sc, stack, blockDim.x, lp, dir, L.radius, &h, tl, L.alpha)) {
-      w0 = __float_as_uint(L.radius);
-    } else {
-      f3 normal = surface_normal(sc, occ, h);
-      f3 point = add(lp, mul(dir, h.t));
-      float bias = maxf_(h.t * 0.001f, 0.001f);
-      f3 nudged = add(sub(point, mul(dir, bias)), mul(normal, bias));
-      bool lit_side = dot(tri_face_normal(sc, h.pos), dir) < 0.0f;
-      w0 = __float_as_uint(len(sub(nudged, lp)));
-      if (lit_side)
-        w1 = ((uint32_t)sat_i8(roundf(normal.x * 127.0f)) & 0xffu) | (((uint32_t)sat_i8(roundf(normal.y * 127.0f)) & 0xffu) << 8) |
-             (((uint32_t)sat_i8(roundf(normal.z * 127.0f)) & 0xffu) << 16);
-      const float* a = occ.albedo + 3 * (size_t)h.occ;
-      w2 = sat_u8(roundf(clampf_(a[0], 0.0f, 1.0f) * 255.0f)) | (sat_u8(roundf(clampf_(a[1], 0.0f, 1.0f) * 255.0f)) << 8) |
-           (sat_u8(roundf(clampf_(a[2], 0.0f, 1.0f) * 255.0f)) << 16);
+    reinterpret_cast<uint4*>(photons)[i] = photon_record(sc, occ, lights[cube_slot[c]], face, x, y, res, stack, tl);
+  }
+}
+
+// Lazy photon cubes: only the texels a gather reads.  need holds one byte per texel in tile order
+// (byte 64 * tile + bit, tile and bit as in the tile masks of k_photon_tiles); the listed form
+// traces texels list[0 .. total) of that numbering (64 * tile + bit, below 2^32).
+__global__ void k_photons_listed(DevScene sc, DevOcc occ, const DevLight* __restrict__ lights,
+                                 const uint32_t* __restrict__ cube_slot, uint32_t res, uint32_t tpf,
+                                 const uint32_t* __restrict__ list, uint64_t total, uint32_t* photons) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stack = lds + threadIdx.x;
+  Tally tl;
+  const uint32_t per_face = tpf * tpf, per_cube = 6 * per_face;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = list[i], tile = v >> 6, bit = v & 63u;
+    const uint32_t c = tile / per_cube, rem = tile % per_cube, face = rem / per_face, t = rem % per_face;
+    const uint32_t x = (t % tpf) * kPhotonTile + (bit & 7u), y = (t / tpf) * kPhotonTile + (bit >> 3);
+    reinterpret_cast<uint4*>(photons)[(uint64_t)c * 6 * res * res + (uint64_t)face * res * res + (uint64_t)y * res + x] =
+        photon_record(sc, occ, lights[cube_slot[c]], face, x, y, res, stack, tl);
+  }
+}
+
+// Per tile, the mask of its needed texels (need_mask) and their count (cnt; entry `tiles` is 0,
+// for the scan).  One thread per tile, reading its 64 need bytes.
+__global__ void k_need_count(const uint4* __restrict__ need, uint64_t tiles, unsigned long long* need_mask, uint64_t* cnt) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= tiles; t += (uint64_t)gridDim.x * blockDim.x) {
+    if (t == tiles) {
+      cnt[t] = 0;
+      continue;
     }
-    reinterpret_cast<uint4*>(photons)[i] = make_uint4(w0, w1, w2, 0u);
+    unsigned long long m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 v = need[4 * t + q];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) m |= 1ull << (16 * q + k);
+    }
+    need_mask[t] = m;
+    cnt[t] = (uint64_t)__popcll(m);
+  }
+}
+
+// The needed texels of each tile, in bit order, at its offset.
+__global__ void k_need_list(const unsigned long long* __restrict__ need_mask, const uint64_t* __restrict__ off, uint64_t tiles,
+                            uint32_t* list) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long m = need_mask[t];
+    uint64_t o = off[t];
+    while (m) {
+      const uint32_t bit = (uint32_t)__ffsll(m) - 1u;
+      list[o++] = (uint32_t)(t << 6) | bit;
+      m &= m - 1ull;
+    }
   }
 }
 
@@ -2067,7 +2128,9 @@ __global__ void k_photons(DevScene sc, DevOcc occ, const DevLight* __restrict__ 
 // square lies in [|f|min, |f|max]).  Per 8x8 tile the box spans the texels whose photon has a lit
 // surface (non-zero normal), and is padded by 1e-5 of its coordinates and of |lp| + distance for
 // the f32 roundings of lp + s * distance.  One wave per tile, lane = texel (bit
-// (y & 7) * 8 + (x & 7) of the tile's live mask).
+// (y & 7) * 8 + (x & 7) of the tile's live mask).  With need_mask (lazy cubes) only the needed
+// texels count: the others hold no photon and are left out of the mask and the box, which then
+// describe the needed texels only (the gather asks only about those).
 __device__ __forceinline__ void face_plane(uint32_t face, double sc, double tc, double* f) {
   switch (face) {
     case 0: f[0] = 1.0; f[1] = -tc; f[2] = -sc; break;
@@ -2087,8 +2150,8 @@ __device__ __forceinline__ float f32_up(double v) {
   return (double)f < v ? nextafterf(f, INFINITY) : f;
 }
 __global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32_t* __restrict__ cube_slot, uint32_t ncubes,
-                               uint32_t res, uint32_t tpf, const uint32_t* __restrict__ photons, unsigned long long* live,
-                               float4* box) {
+                               uint32_t res, uint32_t tpf, const uint32_t* __restrict__ photons,
+                               const unsigned long long* __restrict__ need_mask, unsigned long long* live, float4* box) {
   const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
   const uint64_t per = 6ull * tpf * tpf, total = per * ncubes;
   for (uint64_t q = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); q < total; q += (uint64_t)gridDim.x * waves) {
@@ -2100,7 +2163,7 @@ __global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32
     bool lit = false;
     uint32_t code = 0;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, wmax = 0.0;
-    if (x < res && y < res) {
+    if (x < res && y < res && (!need_mask || ((need_mask[q] >> lane) & 1ull))) {
       const uint4 ph = reinterpret_cast<const uint4*>(photons)[(uint64_t)c * 6 * res * res + (uint64_t)face * res * res +
                                                                (uint64_t)y * res + x];
       code = ph.y & 0xffffffu;
@@ -2515,32 +2578,6 @@ __device__ __forceinline__ GatherJob gather_job(const DevTris& tr, const DevMesh
   return g;
 }
 
-// Sample k of a job: true when it casts a visibility ray (direction, length and the light it
-// brings if unoccluded), false when the photon is empty (bounce.rs:262-286).
-// Whether sample k of a job casts a ray (bounce.rs:262-286: its photon has a lit surface and a
-// positive distance), reading the photon only when its tile leaves it open: an unlit texel casts
-// none, a lit one whose tile box excludes the origin casts one.
-__device__ __forceinline__ bool gather_sample_casts(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k,
-                                                    const BounceParams& p) {
-  float spread = 0.1f + (float)k * g.step;
-  f3 rnd = on_unit_sphere(trig, rng);
-  f3 sample = normalize_or(slerp3(trig, g.dir0, rnd, spread), g.dir0);
-  uint32_t face, x, y;
-  cube_texel_parts(sample, p.cube_res, &face, &x, &y);
-  const uint32_t ci = face * p.cube_res * p.cube_res + y * p.cube_res + x;
-  if (p.tile_live) {
-    uint32_t bit;
-    const uint64_t tile = g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit);
-    if (!((p.tile_live[tile] >> bit) & 1ull)) return false;
-    if (tile_outside(p.tile_box[3 * tile + 1], p.tile_box[3 * tile + 2], g.origin)) return true;
-  }
-  const uint4 ph = reinterpret_cast<const uint4*>(g.cube)[ci];
-  f3 pn = normalize_or_zero(mk3((float)(int8_t)(ph.y & 0xffu) / 127.0f, (float)(int8_t)((ph.y >> 8) & 0xffu) / 127.0f,
-                                (float)(int8_t)((ph.y >> 16) & 0xffu) / 127.0f));
-  f3 photon = add(g.lp, mul(sample, __uint_as_float(ph.x)));
-  return !is_zero(pn) && len(sub(photon, g.origin)) > 0.0f;
-}
-
 // ---- batched sample generation ------------------------------------------------------------
 // The exception lookups of N values of one function (see trig_finish), with the N lookups'
 // loads issued together: the run words first, then the results of the actual exceptions.
@@ -2585,17 +2622,11 @@ __device__ __forceinline__ void gather_draws(const GatherJob& g, Pcg& rng, uint3
   }
 }
 
-// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples) from their draws, each
-// as in bounce.rs:262-286 (spread, random direction, slerp, photon, distance, weight): the sin/cos
-// searches, the photon tile reads and the photon fetches of the G samples issued together.  A
-// sample whose tile shows that it casts no ray, or a ray that brings no light, skips its photon
-// fetch; its outputs are those the photon would have given (ray false, or ray true with a zero
-// contribution, which is never traced).  la = |g.dir0|.
+// The exact sample directions of samples k0 .. k0 + G - 1 from their draws (random.rs:22-28,
+// bounce.rs:262-270: on_unit_sphere, glam's slerp from the direction to the texel, normalize).
 template <int G>
-__device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
-                                             uint32_t k0, float la, const BounceParams& p, f3* dout, float* dist, f3* cout,
-                                             bool* ray, uint32_t* reads) {
-  const uint32_t res = p.cube_res, colored = p.colored;
+__device__ __forceinline__ void exact_directions(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
+                                                 uint32_t k0, float la, f3* sample) {
   const f3 a = g.dir0;
   // on_unit_sphere: sin/cos of the azimuths
   double vs[G], vc[G];
@@ -2631,9 +2662,6 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
   for (int i = 0; i < 3 * G; ++i) va[i] = trig_value(arg[i], 0);
   float sv[3 * G];
   trig_finish_n<3 * G>(trig, trig.sin, arg, va, sv);
-  // sample direction, cube texel, photon
-  uint32_t ci[G], tile[G], bit[G];
-  f3 sample[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
     f3 sl;
@@ -2644,6 +2672,50 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
       sl = slerp3(trig, a, rnd[j], sp[j]);
     }
     sample[j] = normalize_or(sl, a);
+  }
+}
+// Whether sample k of a job casts a ray (bounce.rs:262-286: its photon has a lit surface and a
+// positive distance), reading the photon only when its tile leaves it open: an unlit texel casts
+// none, a lit one whose tile box excludes the origin casts one.
+__device__ __forceinline__ bool gather_sample_casts(const GatherJob& g, const TrigTables& trig, Pcg& rng, uint32_t k,
+                                                    const BounceParams& p) {
+  float z, ang;
+  gather_draws<1>(g, rng, k, &z, &ang);
+  f3 sample;
+  exact_directions<1>(g, trig, &z, &ang, k, len(g.dir0), &sample);
+  uint32_t face, x, y;
+  cube_texel_parts(sample, p.cube_res, &face, &x, &y);
+  const uint32_t ci = face * p.cube_res * p.cube_res + y * p.cube_res + x;
+  if (p.tile_live) {
+    uint32_t bit;
+    const uint64_t tile = g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit);
+    if (!((p.tile_live[tile] >> bit) & 1ull)) return false;
+    if (tile_outside(p.tile_box[3 * tile + 1], p.tile_box[3 * tile + 2], g.origin)) return true;
+  }
+  const uint4 ph = reinterpret_cast<const uint4*>(g.cube)[ci];
+  f3 pn = normalize_or_zero(mk3((float)(int8_t)(ph.y & 0xffu) / 127.0f, (float)(int8_t)((ph.y >> 8) & 0xffu) / 127.0f,
+                                (float)(int8_t)((ph.y >> 16) & 0xffu) / 127.0f));
+  f3 photon = add(g.lp, mul(sample, __uint_as_float(ph.x)));
+  return !is_zero(pn) && len(sub(photon, g.origin)) > 0.0f;
+}
+
+// G consecutive samples k0 .. k0 + G - 1 of a job (those below g.samples) from their draws, each
+// as in bounce.rs:262-286 (spread, random direction, slerp, photon, distance, weight): the sin/cos
+// searches, the photon tile reads and the photon fetches of the G samples issued together.  A
+// sample whose tile shows that it casts no ray, or a ray that brings no light, skips its photon
+// fetch; its outputs are those the photon would have given (ray false, or ray true with a zero
+// contribution, which is never traced).  la = |g.dir0|.
+template <int G>
+__device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
+                                             uint32_t k0, float la, const BounceParams& p, f3* dout, float* dist, f3* cout,
+                                             bool* ray, uint32_t* reads) {
+  const uint32_t res = p.cube_res, colored = p.colored;
+  // sample direction, cube texel, photon
+  uint32_t ci[G], tile[G], bit[G];
+  f3 sample[G];
+  exact_directions<G>(g, trig, z, ang, k0, la, sample);
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
     uint32_t face, x, y;
     cube_texel_parts(sample[j], res, &face, &x, &y);
     ci[j] = face * res * res + y * res + x;
@@ -2962,7 +3034,7 @@ struct FastCand {
   bool sure;        // the face is certain (else the sample is open)
   uint64_t tile[4];
   uint32_t bit[4];
-  unsigned long long w0;  // lit bits of tile[0]
+  uint32_t face, x0, x1, y0, y1;
 };
 // The candidate texels of fast direction d and the lit word of the first one's tile.
 __device__ __forceinline__ FastCand fast_candidates(const GatherJob& g, const BounceParams& p, f3 d, float fres, float margin) {
@@ -2978,7 +3050,11 @@ __device__ __forceinline__ FastCand fast_candidates(const GatherJob& g, const Bo
   c.tile[1] = g.tile0 + photon_tile(face, x1, y0, p.tiles_per_face, &c.bit[1]);
   c.tile[2] = g.tile0 + photon_tile(face, x0, y1, p.tiles_per_face, &c.bit[2]);
   c.tile[3] = g.tile0 + photon_tile(face, x1, y1, p.tiles_per_face, &c.bit[3]);
-  c.w0 = p.tile_live[c.tile[0]];
+  c.face = face;
+  c.x0 = x0;
+  c.x1 = x1;
+  c.y0 = y0;
+  c.y1 = y1;
   return c;
 }
 // Whether the tiles decide a sample whose exact texel is one of its candidates: 1 no ray (none of
@@ -2988,7 +3064,7 @@ __device__ __forceinline__ uint32_t fast_decide(const GatherJob& g, const Bounce
   if (!c.sure) return 0u;
   const uint64_t* tile = c.tile;
   const uint32_t* bit = c.bit;
-  const unsigned long long w0 = c.w0;
+  const unsigned long long w0 = p.tile_live[tile[0]];
   // the candidates' lit bits: one word when they share a tile (the usual case)
   const bool one = tile[1] == tile[0] && tile[2] == tile[0] && tile[3] == tile[0];
   uint32_t lit = 0;
@@ -3080,6 +3156,43 @@ __global__ void k_gather_split(DevTris tr, DevMeshes ms, const DevLight* __restr
     }
     wave_add_at(b.rays, m, nr, i < n);
     if (b.wrays) wave_add_at(b.wrays, win, nr, i < n);
+  }
+}
+
+// Lazy photon cubes: the texels the gather of a list of n jobs can read (traced jobs: k_gather_split
+// and k_gather_open; count-only jobs: k_gather_count).  Per sample, when its fast direction's
+// candidate texels are sure, all of them (its exact texel is one of them, and phase A asks the
+// tiles about no other); else its exact texel (the only one phase B and the count read).  Sets
+// need[64 * tile + bit] = 1 for each, tile and bit as in the photon tiles; p.tiles_per_face must be
+// set, the tiles themselves need not exist yet.
+__global__ void k_photon_mark(DevTris tr, DevMeshes ms, const DevLight* __restrict__ lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* __restrict__ jobs, uint64_t n, BounceParams p, TrigTables trig,
+                              uint8_t* need) {
+  const float fres = (float)p.cube_res;
+  const float margin = 3.0f * kSplitDirErr * fres + 1.0e-3f;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 jb = jobs[i];
+    Pcg rng;
+    uint32_t m;
+    const GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, &rng, &m);
+    const float la = len(g.dir0), inv_la = 1.0f / la;
+    for (uint32_t k = 0; k < g.samples; ++k) {
+      float z, ang;
+      gather_draws<1>(g, rng, k, &z, &ang);
+      f3 d = mk3(0.0f, 0.0f, 1.0f);
+      const bool fast = fast_direction(g.dir0, la, inv_la, z, ang, 0.1f + (float)k * g.step, &d);
+      const FastCand c = fast_candidates(g, p, d, fres, margin);
+      if (fast && c.sure) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) need[64 * c.tile[j] + c.bit[j]] = 1;
+      } else {
+        f3 sample;
+        exact_directions<1>(g, trig, &z, &ang, k, la, &sample);
+        uint32_t face, x, y, bit;
+        cube_texel_parts(sample, p.cube_res, &face, &x, &y);
+        need[64 * (g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit)) + bit] = 1;
+      }
+    }
   }
 }
 
@@ -3779,12 +3892,37 @@ hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLi
   return hipGetLastError();
 }
 hipError_t launch_photon_tiles(hipStream_t s, const DevLight* lights, const uint32_t* cube_slot, uint32_t ncubes, uint32_t res,
-                               const uint32_t* photons, unsigned long long* live, float4* box) {
+                               const uint32_t* photons, unsigned long long* live, float4* box,
+                               const unsigned long long* need_mask) {
   const uint32_t tpf = photon_tiles_per_face(res);
   const uint64_t tiles = 6ull * tpf * tpf * ncubes;
   if (tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_photon_tiles, dim3(grid_for(tiles, kThreads / 64)), dim3(kThreads), 0, s, lights, cube_slot, ncubes, res, tpf,
-                     photons, live, box);
+                     photons, need_mask, live, box);
+  return hipGetLastError();
+}
+hipError_t launch_photon_mark(hipStream_t s, DevTris tris, DevMeshes meshes, const DevLight* lights, DevTexels tx, DevEntries ent,
+                              DevBounce b, const uint2* jobs, uint64_t n, BounceParams p, TrigTables trig, uint8_t* need) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_photon_mark, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, tris, meshes, lights, tx, ent, b, jobs, n, p,
+                     trig, need);
+  return hipGetLastError();
+}
+hipError_t launch_need_count(hipStream_t s, const uint8_t* need, uint64_t tiles, unsigned long long* need_mask, uint64_t* cnt) {
+  hipLaunchKernelGGL(k_need_count, dim3(grid_for(tiles + 1, kThreads)), dim3(kThreads), 0, s,
+                     reinterpret_cast<const uint4*>(need), tiles, need_mask, cnt);
+  return hipGetLastError();
+}
+hipError_t launch_need_list(hipStream_t s, const unsigned long long* need_mask, const uint64_t* off, uint64_t tiles, uint32_t* list) {
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_need_list, dim3(grid_for(tiles, kThreads)), dim3(kThreads), 0, s, need_mask, off, tiles, list);
+  return hipGetLastError();
+}
+hipError_t launch_photons_listed(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,
+                                 uint32_t res, const uint32_t* list, uint64_t total, uint32_t* photons, uint32_t block) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_photons_listed, dim3(grid_for(total, block)), dim3(block), stack_bytes(scene, block), s, scene, occ, lights,
+                     cube_slot, res, photon_tiles_per_face(res), list, total, photons);
   return hipGetLastError();
 }
 hipError_t launch_bounce_owner(hipStream_t s, DevTris tris, const DevLight* lights, DevTexels tx, const uint64_t* tex_start,

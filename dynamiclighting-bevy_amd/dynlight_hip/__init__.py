@@ -157,6 +157,8 @@ def load() -> C.CDLL:
     lib.dl_test_gather_directions.argtypes = [P, f32p, f32p, f32p, f32p, C.c_uint64, f32p, f32p, C.POINTER(C.c_uint32)]
     lib.dl_set_photon_tiles.restype = C.c_int
     lib.dl_set_photon_tiles.argtypes = [P, C.c_int]
+    lib.dl_set_lazy_photons.restype = C.c_int
+    lib.dl_set_lazy_photons.argtypes = [P, C.c_int]
     lib.dl_set_gather.restype = C.c_int
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32]
@@ -487,6 +489,11 @@ class Context:
         """The point-light gather skips the photon reads its cube's tile summaries decide (default)
         or reads every sample's photon; the results are the same."""
         self.lib.dl_set_photon_tiles(self.handle, 1 if enable else 0)
+
+    def set_lazy_photons(self, enable: bool) -> None:
+        """Bakes with few gather samples per photon-cube texel trace only the texels their samples
+        can read (default), or every texel of every cube; the results are the same."""
+        self.lib.dl_set_lazy_photons(self.handle, 1 if enable else 0)
 
     def set_gather(self, mode: int = 1, block: int = 0, chunk: int = 0, chunk_rays: int = 0, blocks_per_cu: int = 0,
                    min_waves: int = 0, refill: int = 0, order: int = 0) -> None:

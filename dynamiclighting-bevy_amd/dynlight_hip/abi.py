@@ -188,7 +188,8 @@ class dl_timings(C.Structure):
                 ("gather_rays_ms", C.c_double), ("gather_rays_launches", C.c_uint64), ("gather_jobs", C.c_uint64),
                 ("gather_samples", C.c_uint64), ("gather_traced", C.c_uint64), ("burial_ms", C.c_double),
                 ("burial_rays", C.c_uint64), ("burial_node_visits", C.c_uint64), ("burial_tri_tests", C.c_uint64),
-                ("gather_photon_reads", C.c_uint64), ("gather_open", C.c_uint64)]
+                ("gather_photon_reads", C.c_uint64), ("gather_open", C.c_uint64),
+                ("photon_texels", C.c_uint64)]
 
 
 def _np_dtype(struct) -> np.dtype:

@@ -476,6 +476,7 @@ typedef struct dl_timings {
     uint64_t burial_tri_tests;  /* triangles tested by those rays (counting on)                */
     uint64_t gather_photon_reads; /* photon records the traced gather jobs read (counting on)  */
     uint64_t gather_open;       /* samples the split setup's fast pass left to the exact one (counting on) */
+    uint64_t photon_texels;     /* photon cube texels traced (all of every cube, or the marked ones of lazy cubes) */
 } dl_timings;
 int dl_last_timings(const dl_ctx* ctx, dl_timings* out);
 

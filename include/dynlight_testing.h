@@ -86,6 +86,12 @@ int dl_set_photon_tiles(dl_ctx* ctx, int enable);
  * others; 0: the exact chain for every sample.  Results are identical. */
 int dl_set_gather_split(dl_ctx* ctx, int enable);
 
+/* 1 (default): a bake whose gathers set up fewer samples than twice its photon cubes' texels (one
+ * mesh batch, split setup) traces only the cube texels its gather samples can fall in, marked from
+ * the samples' fast directions before the cubes are built; 0: every texel of every cube.  Results
+ * are identical. */
+int dl_set_lazy_photons(dl_ctx* ctx, int enable);
+
 /* The sample directions of the gather for n inputs (dir3: the job's direction to its texel, 3
  * floats; z and azimuth: on_unit_sphere's draws; spread: the slerp factor): exact3 as the bake
  * computes them (bounce.rs:262-270), fast3 as the split setup's fast pass does, fast_ok[i] = 1

@@ -47,6 +47,31 @@ def test_batched_jobs_equal_single_bakes(ctx):
         assert_scenes_equal(a, g)
 
 
+def test_lazy_photon_cubes_keep_batched_results(ctx):
+    """The bounce jobs of the batch workload with lazy photon cubes (the default: jobs that set up
+    few gather samples per cube texel trace only the texels their samples can read) equal the same
+    jobs with every texel traced."""
+    pick, ins, ops = _spanning_jobs(32)
+    sel = [i for i, j in enumerate(pick) if j % 2]
+    pick, ins, ops = [pick[i] for i in sel], [ins[i] for i in sel], [ops[i] for i in sel]
+    print(f"bounce jobs {pick}", flush=True)
+    ctx.set_lazy_photons(False)
+    try:
+        full = ctx.bake_batch(ins, ops, streams=4)
+        full_t = ctx.batch_trace()
+    finally:
+        ctx.set_lazy_photons(True)
+    lazy = ctx.bake_batch(ins, ops, streams=4)
+    lazy_t = ctx.batch_trace()
+    for j, a, b in zip(pick, lazy, full):
+        assert_scenes_equal(a, b)
+    traced = sum(t["photon_texels"] for t in lazy_t)
+    total = sum(t["photon_texels"] for t in full_t)
+    print(f"  traced {traced} of {total} cube texels", flush=True)
+    assert 0 < traced < total
+    assert any(a["photon_texels"] < b["photon_texels"] for a, b in zip(lazy_t, full_t))
+
+
 def test_cheapest_batched_jobs_equal_the_oracle(ctx):
     pick, ins, ops = _cheap_jobs(6)
     got = ctx.bake_batch(ins, ops, streams=2)

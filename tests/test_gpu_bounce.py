@@ -106,6 +106,39 @@ def test_photon_tiles_change_no_result(ctx, res):
     compare_crops(tiled_crops, want)
 
 
+@pytest.mark.parametrize("res", [5, 37, 512])
+def test_lazy_photon_cubes_change_no_result(ctx, res):
+    """A bake whose gathers set up few samples per cube texel traces only the texels its samples can
+    read: the result equals that of full cubes (words, bounce crops, stats) and the oracle; at 512^2
+    fewer texels are traced, at 5^2 (more samples than texels) every texel."""
+    inp, o = scene_cases.bounce_point_case()
+    o.photon_cube_resolution = res
+    print(f"lazy photon cubes at cube resolution {res}", flush=True)
+    ctx.set_lazy_photons(False)
+    try:
+        full, full_crops = ctx.bake_bounce_crops(inp, o)
+        full_t = ctx.timings()
+    finally:
+        ctx.set_lazy_photons(True)
+    lazy, lazy_crops = ctx.bake_bounce_crops(inp, o)
+    lazy_t = ctx.timings()
+    assert_scenes_equal(lazy, full)
+    assert lazy_crops.keys() == full_crops.keys()
+    for k in full_crops:
+        assert np.array_equal(lazy_crops[k].view(np.uint32), full_crops[k].view(np.uint32)), k
+    total = full_t["photon_texels"]
+    assert total > 0 and total % (6 * res * res) == 0
+    print(f"  traced {lazy_t['photon_texels']} of {total} cube texels", flush=True)
+    assert 0 < lazy_t["photon_texels"] <= total
+    if res == 512:
+        assert lazy_t["photon_texels"] < total // 4
+    if res == 5:
+        assert lazy_t["photon_texels"] == total
+    want_scene, want = oracle.bounce_crops(inp, o)
+    assert_scenes_equal(lazy, want_scene)
+    compare_crops(lazy_crops, want)
+
+
 SPLIT_DIR_ERR = 1.0e-4  # kSplitDirErr: the bound the split setup's margins assume on each component
 
 
