@@ -240,14 +240,27 @@ int assemble(dl_service* svc, const std::vector<uint8_t>& dirty) {
       }
     }
     if (!dirty[mi]) restaged++;
-    for (uint32_t pi = 0; pi < m.pairs.size(); ++pi) {
-      const LivePair& q = m.pairs[pi];
-      const uint32_t sl_q = slot.at(q.light_id);
-      for (uint32_t k = 0; k < q.kept.size(); ++k) p.ent.push_back({q.kept[k], sl_q, pi, k});
+    // entries in (triangle, slot) order: the pairs in slot order, then a counting sort by
+    // triangle (a pair keeps a triangle at most once)
+    const uint32_t np = (uint32_t)m.pairs.size();
+    std::vector<uint32_t> pslot(np), porder(np);
+    for (uint32_t pi = 0; pi < np; ++pi) {
+      pslot[pi] = slot.at(m.pairs[pi].light_id);
+      porder[pi] = pi;
     }
-    std::sort(p.ent.begin(), p.ent.end(), [](const Entry& a, const Entry& b) { return a.tri != b.tri ? a.tri < b.tri : a.slot < b.slot; });
-    std::vector<uint32_t> count(nt, 0);
-    for (const Entry& e : p.ent) count[e.tri]++;
+    std::sort(porder.begin(), porder.end(), [&](uint32_t a, uint32_t b) { return pslot[a] < pslot[b]; });
+    std::vector<uint32_t> count(nt, 0), at(nt, 0);
+    size_t ne = 0;
+    for (const LivePair& q : m.pairs) {
+      for (uint32_t t : q.kept) count[t]++;
+      ne += q.kept.size();
+    }
+    for (uint32_t t = 1; t < nt; ++t) at[t] = at[t - 1] + count[t - 1];
+    p.ent.resize(ne);
+    for (uint32_t pi : porder) {
+      const LivePair& q = m.pairs[pi];
+      for (uint32_t k = 0; k < q.kept.size(); ++k) p.ent[at[q.kept[k]]++] = Entry{q.kept[k], pslot[pi], pi, k};
+    }
     std::vector<uint32_t> ld_off(nt);
     uint64_t w = 4ull * nt;
     for (uint32_t t = 0; t < nt; ++t) {
@@ -1272,6 +1285,7 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
         svc->lights = std::move(next_lights);
         applied = k + 1;
       }
+      const auto merged = std::chrono::steady_clock::now();
       if (applied > i) {
         const int sa = assemble(svc, dirty);
         if (sa != DL_OK) {
@@ -1282,6 +1296,9 @@ extern "C" int dl_service_replay_edits(dl_service* svc, const dl_service_edit* e
       const auto done = std::chrono::steady_clock::now();
       uint32_t ndirty = 0;
       for (uint8_t d : dirty) ndirty += d;
+      if (std::getenv("DL_SERVICE_TIMING"))
+        std::fprintf(stderr, "[join] edits=%u merge=%f assemble=%f dirty=%u\n", applied - i, ms_between(a, merged),
+                     ms_between(merged, done), ndirty);
       for (uint32_t k = i; k < applied; ++k) {
         Slot& sl = slots[k];
         latency_ms[k] = std::chrono::duration<double, std::milli>(done - arrival(k)).count();
