@@ -117,7 +117,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     dl_ctx* w = ctx->workers[k];
     w->host_threads = std::max(1u, hw / streams);
     w->share = streams;
-    w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
+    w->cache.cap = std::max<size_t>(4ull << 30, ctx->cache.cap / streams);
     w->trace_block = ctx->trace_block;
     w->burial_block = ctx->burial_block;
     w->burial_lanes = ctx->burial_lanes;
@@ -173,6 +173,11 @@ int dl_ctx_create(int device, dl_ctx** out) {
     // stalls bakes running side by side), and it reuses memory freed on another stream only once
     // that free has completed, so no stream is made to wait for another.  Contexts trim the pool
     // after a batch and when they are destroyed.
+    // The context's buffer cache keeps up to half of the device's memory: a bake's buffers (tens
+    // of GB for a scene of 10^8 texels with bounce) then come back from it whole at the next bake,
+    // and no bake waits for the pool to map fresh memory.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) ctx->cache.cap = std::max<size_t>(ctx->cache.cap, total_b / 2);
     hipMemPool_t pool = nullptr;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
       uint64_t keep = UINT64_MAX;

@@ -45,7 +45,13 @@ struct DevCache {
       cached -= *cls;
       return hipSuccess;
     }
-    return hipMallocAsync(p, *cls, stream);
+    hipError_t e = hipMallocAsync(p, *cls, stream);
+    if (e == hipErrorOutOfMemory && cached) {  // the cache's buffers go back to the pool, and the request is tried again
+      (void)hipGetLastError();
+      clear();
+      e = hipMallocAsync(p, *cls, stream);
+    }
+    return e;
   }
   void put(void* p, size_t cls) {
     if (cached + cls > cap) {

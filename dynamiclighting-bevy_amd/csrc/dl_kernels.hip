@@ -2519,25 +2519,70 @@ __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __re
   return 0u;
 }
 
-// One BVH node of a closest-hit traversal (the loop body of traverse<false>): keeps the lowest
-// (t, occluder index) hit in `hit` (`have` once there is one).  Returns 0 while the traversal
-// continues, 2 once it has ended.
-__device__ __forceinline__ uint32_t visit_closest(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
-                                                  const SlabRay& sr, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
-                                                  uint32_t alpha, bool& have, Hit& hit) {
-  tl.nodes++;
-  float tn[4];
+// The hit leaf children of the last node a closest-hit stream step visited, still to be tested:
+// `mask` over the node's children (refs `ch`, box entries `tn`), and triangles [i, end) of the
+// leaf under test.
+struct PendingLeaves {
   int4 ch;
-  const uint32_t mask = node_hits(sc, node, sr, (have ? hit.t : tmax) * (1.0f + kBoxEps), tn, &ch);
-  closest_leaves(sc, mask & leaf_bits(ch), ch, tn, o, d, tmax, alpha, have, hit, tl);
-  const int32_t next = push_children_sorted(sc, mask, ch, tn, stack, stride, sp);
-  if (next >= 0) {
-    node = next;
+  float tn[4];
+  uint32_t mask, i, end;
+};
+
+// One step of a closest-hit traversal in a ray stream: one triangle of the pending leaves (the
+// nearest leaf first; a leaf whose box starts beyond the best hit is dropped), or, with none
+// left, one BVH node, whose hit leaf children become pending and whose inner children are pushed
+// (the nearest is visited next; node -1: pop).  The node and leaf order is that of traverse<false>
+// (a node's leaves before the next node), one triangle per step, so that the lanes of a wave do
+// similar work per step.  Keeps the lowest (t, occluder index) hit.  Returns 0 while the
+// traversal continues, 2 once it has ended.
+__device__ __forceinline__ uint32_t step_closest(const DevScene& sc, uint32_t* __restrict__ stack, uint32_t stride, f3 o, f3 d,
+                                                 const SlabRay& sr, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
+                                                 uint32_t alpha, bool& have, Hit& hit, PendingLeaves& q) {
+  while (q.i == q.end && q.mask) {
+    int kb = 0;
+    float tb = INFINITY;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (((q.mask >> k) & 1u) && (!any || q.tn[k] < tb)) {
+        kb = k;
+        tb = q.tn[k];
+        any = true;
+      }
+    q.mask &= ~(1u << kb);
+    if (have && tb > hit.t * (1.0f + kBoxEps)) continue;
+    const uint32_t v = (uint32_t)(-1 - sel4(q.ch, kb));
+    q.i = v >> 2;
+    q.end = q.i + (v & 3u) + 1u;
+  }
+  if (q.i < q.end) {
+    const uint32_t i = q.i++;
+    const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+    float t, u, vv;
+    tl.tris++;
+    if (mt_hit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), o, d, tmax, &t, &u, &vv) && finite_(t) &&
+        (!alpha || alpha_blocks(sc, __float_as_uint(a.w), u, vv))) {
+      const uint32_t occ = __float_as_uint(a.w);
+      if (!have || t < hit.t || (t == hit.t && occ < hit.occ)) {
+        have = true;
+        hit.t = t;
+        hit.u = u;
+        hit.v = vv;
+        hit.occ = occ;
+        hit.pos = i;
+      }
+    }
     return 0u;
   }
-  if (sp == 0) return 2u;
-  sp--;
-  node = pop_node(sc, stack, stride, sp);
+  if (node < 0) {
+    if (sp == 0) return 2u;
+    sp--;
+    node = pop_node(sc, stack, stride, sp);
+  }
+  tl.nodes++;
+  const uint32_t mask = node_hits(sc, node, sr, (have ? hit.t : tmax) * (1.0f + kBoxEps), q.tn, &q.ch);
+  q.mask = mask & leaf_bits(q.ch);
+  node = push_children_sorted(sc, mask, q.ch, q.tn, stack, stride, sp);
   return 0u;
 }
 
@@ -3337,6 +3382,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
   int32_t node = 0;
   bool have = false;
   Hit hit{};
+  PendingLeaves pend{};
   for (;;) {
     // Finished lanes take their prefetched ray, and lanes without one request another, only
     // once at least `refill` lanes are idle: these steps then run with many lanes at a time.
@@ -3355,6 +3401,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
         node = 0;
         sp = 0;
         have = false;
+        pend.mask = 0;
+        pend.i = pend.end = 0;
         live = true;
       }
     }
@@ -3398,7 +3446,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(MIN_WAVES, 8))) void k_any_stream(
     }
     if (live) {
       if constexpr (CLOSEST) {
-        if (visit_closest(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha, have, hit) != 0u) {
+        if (step_closest(sc, stack, blockDim.x, o, d, sr, tmax, node, sp, tl, alpha, have, hit, pend) != 0u) {
           live = false;
           if (have) reinterpret_cast<float4*>(contrib)[cidx] = make_float4(hit.t, hit.u, hit.v, __uint_as_float(hit.pos));
         }
