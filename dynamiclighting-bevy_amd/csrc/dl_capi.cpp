@@ -117,7 +117,7 @@ bool prepare_workers(dl_ctx* ctx, uint32_t streams) {
     dl_ctx* w = ctx->workers[k];
     w->host_threads = std::max(1u, hw / streams);
     w->share = streams;
-    w->cache.cap = std::max<size_t>(4ull << 30, ctx->cache.cap / streams);
+    w->cache.cap = std::max<size_t>(4ull << 30, (32ull << 30) / streams);
     w->trace_block = ctx->trace_block;
     w->burial_block = ctx->burial_block;
     w->burial_lanes = ctx->burial_lanes;
@@ -173,11 +173,13 @@ int dl_ctx_create(int device, dl_ctx** out) {
     // stalls bakes running side by side), and it reuses memory freed on another stream only once
     // that free has completed, so no stream is made to wait for another.  Contexts trim the pool
     // after a batch and when they are destroyed.
-    // The context's buffer cache keeps up to half of the device's memory: a bake's buffers (tens
-    // of GB for a scene of 10^8 texels with bounce) then come back from it whole at the next bake,
-    // and no bake waits for the pool to map fresh memory.
+    // The context's buffer cache keeps up to a quarter of the device's memory: a bake's buffers
+    // (about 40 GB for a scene of 10^8 texels with bounce) then come back from it whole at the next
+    // bake, and no bake waits for the pool to map (and clear) fresh memory.  Worker contexts keep
+    // 32 GB between them.
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) ctx->cache.cap = std::max<size_t>(ctx->cache.cap, total_b / 2);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) ctx->cache.cap = std::max<size_t>(ctx->cache.cap, total_b / 4);
+    if (const char* env = std::getenv("DL_CACHE_BYTES")) ctx->cache.cap = strtoull(env, nullptr, 10);
     hipMemPool_t pool = nullptr;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
       uint64_t keep = UINT64_MAX;
@@ -342,6 +344,13 @@ int dl_service_create(dl_ctx* ctx, const dl_bake_input* scene, const dl_bake_opt
     if (st != DL_OK) return st;
     std::unique_ptr<dl_baked_scene> hold(first);
     if ((st = dl::service_init_live(svc.get(), scene, cap, *first)) != DL_OK) return st;
+    // The service's later tracing runs on worker contexts (with buffers of other sizes) and its
+    // context assembles: its cache keeps 32 GB of the initial bake's buffers, the rest goes back
+    // to the device's pool for the workers.
+    ctx->cache.stream = ctx->stream;
+    ctx->cache.cap = std::min<size_t>(ctx->cache.cap, 32ull << 30);
+    ctx->cache.trim(ctx->cache.cap);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return set_err(ctx, DL_E_DEVICE, "releasing the initial bake's buffers failed");
     if (initial) *initial = hold.release();
     *out = svc.release();
     return DL_OK;

@@ -61,6 +61,18 @@ struct DevCache {
     lists[cls].push_back(p);
     cached += cls;
   }
+  // Returns cached buffers to the pool, the largest first, until at most `keep` bytes are cached.
+  void trim(size_t keep) {
+    while (cached > keep) {
+      auto big = lists.end();
+      for (auto it = lists.begin(); it != lists.end(); ++it)
+        if (!it->second.empty() && (big == lists.end() || it->first > big->first)) big = it;
+      if (big == lists.end()) break;
+      (void)hipFreeAsync(big->second.back(), stream);
+      big->second.pop_back();
+      cached -= big->first;
+    }
+  }
   void clear() {
     for (auto& l : lists)
       for (void* q : l.second) (void)hipFreeAsync(q, stream);
