@@ -799,6 +799,24 @@ int dl_set_gather_split(dl_ctx* ctx, int enable) {
   return DL_OK;
 }
 
+uint32_t dl_test_light_bvh(const float* positions, const float* radii, uint32_t n, dl_light_bvh_node* nodes, uint32_t cap,
+                           uint32_t* order) {
+  if (n && (!positions || !radii || !order)) return 0;
+  try {
+    std::vector<dl::f3> pos(n);
+    std::vector<float> rad(radii, radii + n);
+    for (uint32_t i = 0; i < n; ++i) pos[i] = dl::mk3(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]);
+    dl::LightBvhOut out;
+    dl::build_light_bvh(pos, rad, out);
+    if (out.nodes.size() > cap || (!out.nodes.empty() && !nodes)) return 0;
+    std::copy(out.nodes.begin(), out.nodes.end(), nodes);
+    std::copy(out.order.begin(), out.order.end(), order);
+    return (uint32_t)out.nodes.size();
+  } catch (...) {
+    return 0;
+  }
+}
+
 int dl_set_lazy_photons(dl_ctx* ctx, int enable) {
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);

@@ -113,17 +113,6 @@ struct LightBvhBuild {
     nodes[n].aabb_min[0] = a.lo.x; nodes[n].aabb_min[1] = a.lo.y; nodes[n].aabb_min[2] = a.lo.z;
     nodes[n].aabb_max[0] = a.hi.x; nodes[n].aabb_max[1] = a.hi.y; nodes[n].aabb_max[2] = a.hi.z;
   }
-  float cost(size_t first, size_t count, int ax, float split) const {
-    Aabb3 l = empty_box(), r = empty_box();
-    uint32_t lc = 0, rc = 0;
-    for (size_t k = first; k < first + count; ++k) {
-      uint32_t li = idx[k];
-      if (axis(pos[li], ax) < split) { lc++; l = unite(l, bounds(li)); }
-      else { rc++; r = unite(r, bounds(li)); }
-    }
-    if (lc == 0 || rc == 0) return 3.40282347e+38f;
-    return (float)lc * half_area_x2(l) + (float)rc * half_area_x2(r);
-  }
   void subdivide(size_t n) {
     uint32_t first = nodes[n].left_first, count = nodes[n].count;
     if (count <= 2) return;
@@ -133,12 +122,33 @@ struct LightBvhBuild {
     bool found = false;
     int best_axis = 0;
     float best_split = 0.0f, best_cost = 3.40282347e+38f;
-    for (int a = 0; a < 3; ++a)
+    // Every light's coordinate is a candidate split, tried axis by axis in the node's order (the
+    // first lowest cost wins).  A split at v puts the lights whose coordinate is below v left, and
+    // costs lc * area(left bounds) + rc * area(right bounds) (infinite with an empty side): with the
+    // lights sorted along the axis the left ones are a prefix, so prefix and suffix bounds give
+    // every candidate's cost in O(log n) instead of a pass over the node.  Bounds are unions by
+    // min/max, exact in any order, so the costs are those of the pass bit for bit.
+    std::vector<std::pair<float, uint32_t>> sv(count);
+    std::vector<Aabb3> pre(count + 1), suf(count + 1);
+    for (int a = 0; a < 3; ++a) {
+      for (uint32_t k = 0; k < count; ++k) sv[k] = {axis(pos[idx[first + k]], a), idx[first + k]};
+      std::sort(sv.begin(), sv.end(), [](const std::pair<float, uint32_t>& x, const std::pair<float, uint32_t>& y) {
+        return x.first < y.first;
+      });
+      pre[0] = empty_box();
+      for (uint32_t k = 0; k < count; ++k) pre[k + 1] = unite(pre[k], bounds(sv[k].second));
+      suf[count] = empty_box();
+      for (uint32_t k = count; k-- > 0;) suf[k] = unite(suf[k + 1], bounds(sv[k].second));
       for (uint32_t k = 0; k < count; ++k) {
-        float cand = axis(pos[idx[first + k]], a);
-        float c = cost(first, count, a, cand);
+        const float cand = axis(pos[idx[first + k]], a);
+        const uint32_t lc = (uint32_t)(std::lower_bound(sv.begin(), sv.end(), cand,
+                                                        [](const std::pair<float, uint32_t>& x, float v) { return x.first < v; }) -
+                                       sv.begin());
+        const uint32_t rc = count - lc;
+        const float c = (lc == 0 || rc == 0) ? 3.40282347e+38f : (float)lc * half_area_x2(pre[lc]) + (float)rc * half_area_x2(suf[lc]);
         if (c < best_cost) { best_cost = c; found = true; best_axis = a; best_split = cand; }
       }
+    }
     if (!found || best_cost >= parent) return;
     int64_t i = first, j = (int64_t)first + count - 1;
     while (i <= j) {

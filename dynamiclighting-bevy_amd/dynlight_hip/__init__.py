@@ -159,6 +159,9 @@ def load() -> C.CDLL:
     lib.dl_set_photon_tiles.argtypes = [P, C.c_int]
     lib.dl_set_lazy_photons.restype = C.c_int
     lib.dl_set_lazy_photons.argtypes = [P, C.c_int]
+    lib.dl_test_light_bvh.restype = C.c_uint32
+    lib.dl_test_light_bvh.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
+                                      C.POINTER(abi.dl_light_bvh_node), C.c_uint32, C.POINTER(C.c_uint32)]
     lib.dl_set_gather.restype = C.c_int
     lib.dl_set_gather.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32]

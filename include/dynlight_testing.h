@@ -99,6 +99,12 @@ int dl_set_lazy_photons(dl_ctx* ctx, int enable);
 int dl_test_gather_directions(dl_ctx* ctx, const float* dir3, const float* z, const float* azimuth, const float* spread,
                               uint64_t n, float* exact3, float* fast3, uint32_t* fast_ok);
 
+/* The light BVH the bake builds on the host (LightBvh::build, light_bvh.rs:59-189) over n point
+ * lights (positions: 3 floats each, radii): writes up to `cap` nodes and the lights' slot order
+ * (n entries) and returns the node count (0 when cap is too small).  Needs no device. */
+uint32_t dl_test_light_bvh(const float* positions, const float* radii, uint32_t n, dl_light_bvh_node* nodes, uint32_t cap,
+                           uint32_t* order);
+
 /* Point-light bounce gather: mode 0 = one wave per (triangle, light, 64 texels) tracing its
  * samples in turn; mode 1 = split gather (rays, persistent any-hit traversal, ordered sums;
  * default).  For mode 1: threads per block and rays per queue run of the any-hit kernel, rays per
