@@ -415,9 +415,9 @@ struct dl_service {
   std::vector<dl::ScenePart> parts;              // part 0 is `res`
   std::vector<std::unique_ptr<dl::Resident>> rooms;  // the residents of the added rooms
   // burial rays of the parts that hit nothing in their own part (8 floats each, see
-  // Capture::escaping); esc_open when a part had more than its capture kept
-  std::shared_ptr<dl::DevBlock> esc;
-  uint64_t nesc = 0;
+  // Capture::escaping), one block and count per part; esc_open when a part had more than its
+  // capture kept
+  std::vector<std::pair<std::shared_ptr<dl::DevBlock>, uint64_t>> esc;
   uint64_t esc_total = 0;  // escaping rays of the scene the service was created with (also past the cap)
   bool esc_open = false;
   std::unordered_map<uint64_t, std::vector<uint32_t>> cubes;  // distance cube words per local light id

@@ -804,9 +804,11 @@ int join_room(dl_service* svc, const dl_bake_input* room, RoomTrace& t, std::vec
                                              std::to_string(t.cap.escaping_count) + ", at most " +
                                              std::to_string(t.cap.escaping_cap) + ")");
   uint64_t hits = 0;
-  int st = rays_hitting(ctx, rr, reinterpret_cast<const float*>(svc->esc ? svc->esc->p : nullptr), svc->nesc, &hits);
-  if (st != DL_OK) return st;
-  if (hits) return set_error(ctx, DL_E_ARGUMENT, "burial rays of the live scene reach the room");
+  int st = DL_OK;
+  for (const auto& blk : svc->esc) {
+    if ((st = rays_hitting(ctx, rr, reinterpret_cast<const float*>(blk.first->p), blk.second, &hits)) != DL_OK) return st;
+    if (hits) return set_error(ctx, DL_E_ARGUMENT, "burial rays of the live scene reach the room");
+  }
   for (const ScenePart& p : svc->parts) {
     st = rays_hitting(ctx, *p.res, reinterpret_cast<const float*>(t.cap.escaping->p), t.cap.escaping_count, &hits);
     if (st != DL_OK) return st;
@@ -831,27 +833,29 @@ int join_room(dl_service* svc, const dl_bake_input* room, RoomTrace& t, std::vec
   svc->stats.triangles += rr.ntri;
   svc->crops.insert(svc->crops.end(), t.cap.crops.begin(), t.cap.crops.end());
   {
+    // visit counts: the room's zeroed after the live scene's, in a buffer grown by doubling
     std::vector<uint64_t> off(svc->vc_off);
     for (uint32_t i = 0; i < nrm; ++i) off.push_back(off.back() + ((uint64_t)rr.res[i] * rr.res[i] + 1) / 2);
-    std::unique_ptr<DevBlock> vc(new DevBlock()), crops(new DevBlock());
-    HIP_TRY(vc->alloc(off.back() + 1, s));
-    HIP_TRY(hipMemcpyAsync(vc->p, svc->vc->p, svc->vc_off.back() * 4, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemsetAsync(vc->p + svc->vc_off.back(), 0, (off.back() + 1 - svc->vc_off.back()) * 4, s));
+    if (off.back() + 1 > svc->vc->words) {
+      std::unique_ptr<DevBlock> vc(new DevBlock());
+      HIP_TRY(vc->alloc(std::max<uint64_t>(off.back() + 1, 2 * svc->vc->words), s));
+      HIP_TRY(hipMemcpyAsync(vc->p, svc->vc->p, svc->vc_off.back() * 4, hipMemcpyDeviceToDevice, s));
+      svc->vc = std::move(vc);
+    }
+    HIP_TRY(hipMemsetAsync(svc->vc->p + svc->vc_off.back(), 0, (off.back() + 1 - svc->vc_off.back()) * 4, s));
+    std::unique_ptr<DevBlock> crops(new DevBlock());
     HIP_TRY(crops->alloc(svc->crops.size() + 1, s));
     HIP_TRY(hipMemcpyAsync(crops->p, svc->crops.data(), svc->crops.size() * 4, hipMemcpyHostToDevice, s));
-    // escaping rays: the live scene's, then the room's
-    std::shared_ptr<DevBlock> esc = std::make_shared<DevBlock>();
-    const uint64_t nesc = svc->nesc + t.cap.escaping_count;
-    HIP_TRY(esc->alloc(8 * nesc + 1, s));
-    if (svc->nesc) HIP_TRY(hipMemcpyAsync(esc->p, svc->esc->p, svc->nesc * 32, hipMemcpyDeviceToDevice, s));
-    if (t.cap.escaping_count)
-      HIP_TRY(hipMemcpyAsync(esc->p + 8 * svc->nesc, t.cap.escaping->p, t.cap.escaping_count * 32, hipMemcpyDeviceToDevice, s));
+    // the room's escaping rays, in a block of their own size
+    if (t.cap.escaping_count) {
+      std::shared_ptr<DevBlock> esc = std::make_shared<DevBlock>();
+      HIP_TRY(esc->alloc(8 * t.cap.escaping_count, s));
+      HIP_TRY(hipMemcpyAsync(esc->p, t.cap.escaping->p, t.cap.escaping_count * 32, hipMemcpyDeviceToDevice, s));
+      svc->esc.emplace_back(std::move(esc), t.cap.escaping_count);
+    }
     HIP_TRY(hipStreamSynchronize(s));
-    svc->vc = std::move(vc);
     svc->vc_off = std::move(off);
     svc->d_crops = std::move(crops);
-    svc->esc = std::move(esc);
-    svc->nesc = nesc;
   }
   t.part.mesh0 = mesh0;
   svc->parts.push_back(t.part);
@@ -922,8 +926,8 @@ int service_init_live(dl_service* svc, const dl_bake_input* scene, Capture& cap,
   svc->parts[0].res = &svc->res;
   if (!bounds_of(scene, svc->parts[0].bmin, svc->parts[0].bmax))
     for (int a = 0; a < 3; ++a) svc->parts[0].bmin[a] = svc->parts[0].bmax[a] = 0.0f;
-  svc->esc = cap.escaping;
-  svc->nesc = cap.escaping ? std::min(cap.escaping_count, cap.escaping_cap) : 0;
+  svc->esc.clear();
+  if (cap.escaping && cap.escaping_count) svc->esc.emplace_back(cap.escaping, std::min(cap.escaping_count, cap.escaping_cap));
   svc->esc_open = !cap.escaping || cap.escaping_count > cap.escaping_cap;
   svc->esc_total = cap.escaping_count;
   svc->W.reset();
