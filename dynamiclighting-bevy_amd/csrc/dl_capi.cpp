@@ -521,6 +521,12 @@ int dl_scene_mesh(const dl_baked_scene* s, uint32_t i, dl_baked_mesh_view* out) 
   out->triangle_word_count = m.words.size();
   return DL_OK;
 }
+uint32_t dl_scene_meshes(const dl_baked_scene* s, uint32_t first, uint32_t count, dl_baked_mesh_view* out) {
+  if (!s || !out || first >= s->meshes.size()) return 0;
+  const uint32_t n = (uint32_t)std::min<uint64_t>(count, s->meshes.size() - first);
+  for (uint32_t k = 0; k < n; ++k) (void)dl_scene_mesh(s, first + k, out + k);
+  return n;
+}
 uint64_t dl_scene_distance_cubes(const dl_baked_scene* s, const uint32_t** words) {
   *words = s->distance_cubes.data();
   return s->distance_cubes.size();

@@ -470,6 +470,7 @@ def bind_scene_api(lib: C.CDLL, prefix: str):
         "scene_light_bvh": (C.c_uint32, [H, C.POINTER(C.c_void_p)]),
         "scene_mesh_count": (C.c_uint32, [H]),
         "scene_mesh": (C.c_int, [H, C.c_uint32, C.POINTER(dl_baked_mesh_view)]),
+        "scene_meshes": (C.c_uint32, [H, C.c_uint32, C.c_uint32, C.POINTER(dl_baked_mesh_view)]),
         "scene_distance_cubes": (C.c_uint64, [H, C.POINTER(C.c_void_p)]),
         "scene_sun_map_count": (C.c_uint32, [H]),
         "scene_sun_map": (C.c_int, [H, C.c_uint32, C.POINTER(dl_sun_map_view)]),
@@ -509,11 +510,32 @@ def read_scene(lib: C.CDLL, prefix: str, handle) -> BakedScene:
     if n:
         bvh = np.frombuffer((C.c_uint32 * (8 * n)).from_address(p.value), dtype=np.uint32).reshape(n, 8).copy()
     meshes = []
-    for i in range(g("scene_mesh_count")(handle)):
-        mv = dl_baked_mesh_view()
-        g("scene_mesh")(handle, i, C.byref(mv))
-        meshes.append(BakedMesh(mv.id, mv.content_hash, mv.resolution, bool(mv.occludes),
-                                _words(mv.triangles, mv.triangle_word_count)))
+    nm = g("scene_mesh_count")(handle)
+    if hasattr(lib, prefix + "scene_meshes") and nm:
+        # every view in one call, read as a numpy record array; words copied once per run of
+        # meshes whose words lie end to end
+        views = (dl_baked_mesh_view * nm)()
+        got = g("scene_meshes")(handle, 0, nm, views)
+        assert got == nm
+        rec = np.frombuffer(views, dtype=np.dtype({"names": ["id", "hash", "res", "occ", "ptr", "n"],
+                                                   "formats": ["<u8", "<u8", "<u4", "<u4", "<u8", "<u8"],
+                                                   "offsets": [0, 8, 16, 20, 24, 32], "itemsize": C.sizeof(dl_baked_mesh_view)}))
+        ptr, cnt = rec["ptr"].astype(np.int64), rec["n"].astype(np.int64)
+        brk = np.ones(nm, bool)
+        brk[1:] = (ptr[1:] != ptr[:-1] + 4 * cnt[:-1]) | (ptr[1:] == 0) | (ptr[:-1] == 0)
+        starts = np.flatnonzero(brk).tolist() + [nm]
+        ids, hashes, res, occ = rec["id"].tolist(), rec["hash"].tolist(), rec["res"].tolist(), rec["occ"].tolist()
+        off = np.concatenate([[0], np.cumsum(cnt)]).tolist()
+        for a, b in zip(starts[:-1], starts[1:]):
+            block = _words(int(ptr[a]), off[b] - off[a])
+            for k in range(a, b):
+                meshes.append(BakedMesh(ids[k], hashes[k], res[k], bool(occ[k]), block[off[k] - off[a]:off[k + 1] - off[a]]))
+    else:
+        for i in range(nm):
+            mv = dl_baked_mesh_view()
+            g("scene_mesh")(handle, i, C.byref(mv))
+            meshes.append(BakedMesh(mv.id, mv.content_hash, mv.resolution, bool(mv.occludes),
+                                    _words(mv.triangles, mv.triangle_word_count)))
     p = C.c_void_p()
     n = g("scene_distance_cubes")(handle, C.byref(p))
     cubes = _words(p.value, n)

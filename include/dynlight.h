@@ -435,6 +435,9 @@ int dl_scene_light(const dl_baked_scene* scene, uint32_t index, dl_baked_light* 
 uint32_t dl_scene_light_bvh(const dl_baked_scene* scene, const dl_light_bvh_node** nodes);
 uint32_t dl_scene_mesh_count(const dl_baked_scene* scene);
 int dl_scene_mesh(const dl_baked_scene* scene, uint32_t index, dl_baked_mesh_view* out);
+/* Views of meshes first .. first + count - 1 (those that exist) in one call; returns how many
+ * were written.  Same views as dl_scene_mesh, for callers that read every mesh. */
+uint32_t dl_scene_meshes(const dl_baked_scene* scene, uint32_t first, uint32_t count, dl_baked_mesh_view* out);
 uint64_t dl_scene_distance_cubes(const dl_baked_scene* scene, const uint32_t** words);
 uint32_t dl_scene_sun_map_count(const dl_baked_scene* scene);
 int dl_scene_sun_map(const dl_baked_scene* scene, uint32_t index, dl_sun_map_view* out);
