@@ -10,7 +10,7 @@ Prints one JSON line.  `roofline` prices the dominant stage, the bounce gather's
 chunk of gather jobs: the fast pass k_gather_split, the list of the samples it leaves open, the
 exact pass k_gather_open), at its algorithmic bytes: 48 B per job (job record, texel position and
 coordinates, ray offset, traced count and bits), 40 B per open sample, 16 B per photon read, 44 B
-per traced sample (ray and contribution written) and 56 B per photon tile of each bounce light,
+per traced sample (ray and contribution written) and 64 B per photon tile of each bounce light,
 with the counts from an instrumented (untimed) bake, divided by the span's average time from HIP
 events on the bake stream.  `traffic` is the rocprofv3-measured HBM bytes of the two kernels per
 launch recorded in profiles/interior_bake_kernels.md; `valu_roofline` their VALU instructions per
@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue peak: one wave64 VALU instruction per cycle per CU (4 SIMDs of 16 lanes), 256 CUs at
 # 2.4 GHz (the same rate gives MI355X's 78.6 TFLOP/s of FP64 FMA)
 VALU_PEAK_GIPS = 256 * 2.4
-TILE_BYTES = 56.0    # one photon tile: 8 B of lit bits and a 48-B box (8x8 texels)
+TILE_BYTES = 64.0    # one photon tile: 16 B of lit bits (9x9 texels) and a 48-B box
 NODE_BYTES = 112.0  # a 4-wide node as the kernels read it: six float4 of child bounds, one int4 of child slots
 TRI_BYTES = 48.0    # v0 | occluder index, e1, e2 as three float4
 RAY_BYTES = 32.0    # origin | length, direction | flags

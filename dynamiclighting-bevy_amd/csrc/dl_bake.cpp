@@ -496,7 +496,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     HIP_TRY(d_photons.alloc(cube_texels * 4));  // one 16-byte record per photon (12 bytes used)
     HIP_TRY(launch_photons(s, sc, occ, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, 128));
     if (ctx->photon_tiles) {
-      HIP_TRY(d_tile_live.alloc(cube_tiles));
+      HIP_TRY(d_tile_live.alloc(2 * cube_tiles));  // two summary words per tile
       HIP_TRY(d_tile_box.alloc(3 * cube_tiles));
       HIP_TRY(launch_photon_tiles(s, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, d_tile_live.p,
                                   d_tile_box.p));
@@ -509,7 +509,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   // marks number texels 64 * tile + bit in 32 bits.
   bool cubes_pending = ctx->lazy_photons && ctx->gather_mode == 1 && ctx->gather_split && ctx->photon_tiles &&
                        !cube_slot.empty() && cube_tiles < (1ull << 26);
-  const uint64_t cube_bytes = cube_texels * 16 + cube_tiles * 56;
+  const uint64_t cube_bytes = cube_texels * 16 + cube_tiles * 64;
   if (!cube_slot.empty()) {
     HIP_TRY(upload(ctx, d_cube_slot, cube_slot));
     if (!cubes_pending)
@@ -1092,7 +1092,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
             HIP_TRY(launch_need_list(s, d_need_mask.p, d_noff.p, cube_tiles, d_nlist.p));
             HIP_TRY(d_photons.alloc(cube_texels * 4));
             HIP_TRY(launch_photons_listed(s, sc, occ, d_lights.p, d_cube_slot.p, cres, d_nlist.p, needed, d_photons.p, 128));
-            HIP_TRY(d_tile_live.alloc(cube_tiles));
+            HIP_TRY(d_tile_live.alloc(2 * cube_tiles));  // two summary words per tile
             HIP_TRY(d_tile_box.alloc(3 * cube_tiles));
             HIP_TRY(launch_photon_tiles(s, d_lights.p, d_cube_slot.p, (uint32_t)cube_slot.size(), cres, d_photons.p, d_tile_live.p,
                                         d_tile_box.p, d_need_mask.p));

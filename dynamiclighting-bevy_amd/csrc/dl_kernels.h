@@ -202,16 +202,20 @@ struct BounceParams {
   uint32_t cube_res;
   const uint32_t* photons;  // 4 words per photon (distance, normal, albedo, 0), 6 * res^2 per cube
   uint32_t sun_only;        // k_bounce_gather: point lights are gathered by the streamed kernel
-  // Photon tiles (launch_photon_tiles): per 8x8 tile of a cube face, tile_live holds one bit per
-  // texel whose photon has a lit surface, tile_box three float4 (shared normal | uniform flag,
-  // box minimum | coordinate scale, box maximum); kPhotonTile^2 texels per tile, tiles_per_face^2
-  // tiles per face.  Null tile_live: the gather fetches every sample's photon.
+  // Photon tiles (launch_photon_tiles): the tiles step by kPhotonTile texels over a cube face
+  // (tiles_per_face^2 per face), and each summarizes the kTileSpan^2 texels from its corner (its
+  // own 8x8 and the first row and column of its neighbours'): tile_live holds two words per tile,
+  // one bit per texel whose photon has a lit surface (bit dy * kTileSpan + dx), tile_box three
+  // float4 (shared normal | uniform flag, box minimum | coordinate scale, box maximum).  Any 2x2
+  // block of texels lies in the summary of the tile holding its first texel.  Null tile_live: the
+  // gather fetches every sample's photon.
   uint32_t tiles_per_face = 0;
   const unsigned long long* tile_live = nullptr;
   const float4* tile_box = nullptr;
   unsigned long long* photon_reads = nullptr;  // counting: photon records read by launch_gather_rays
 };
-static constexpr uint32_t kPhotonTile = 8;  // photon tile side (64 texels: one wave, one 64-bit mask)
+static constexpr uint32_t kPhotonTile = 8;  // photon tile step (its own texels: 8x8)
+static constexpr uint32_t kTileSpan = kPhotonTile + 1;  // texels a tile summarizes along each axis
 inline uint32_t photon_tiles_per_face(uint32_t res) { return (res + kPhotonTile - 1) / kPhotonTile; }
 // Bounce work: entries (triangle-light pairs) whose light bakes bounce, each with its window.
 struct DevBounce {

@@ -2125,12 +2125,14 @@ __global__ void k_need_list(const unsigned long long* __restrict__ need_mask, co
 // square f of the face plane (major axis 1), widened by 1e-6 in face coordinates for the f32
 // roundings that pick the texel.  That set lies in the box of the eight points
 // lp + (w_i / |f|max) c and lp + (w_i / |f|min) c over the square's corners c (|f| over the
-// square lies in [|f|min, |f|max]).  Per 8x8 tile the box spans the texels whose photon has a lit
-// surface (non-zero normal), and is padded by 1e-5 of its coordinates and of |lp| + distance for
-// the f32 roundings of lp + s * distance.  One wave per tile, lane = texel (bit
-// (y & 7) * 8 + (x & 7) of the tile's live mask).  With need_mask (lazy cubes) only the needed
-// texels count: the others hold no photon and are left out of the mask and the box, which then
-// describe the needed texels only (the gather asks only about those).
+// square lies in [|f|min, |f|max]).  Per tile (a step of 8 texels) the summary covers the 9x9
+// texels from its corner, so that the at most 2x2 candidate texels of a fast-pass sample always
+// lie in one summary: the box spans those of its texels whose photon has a lit surface (non-zero
+// normal), padded by 1e-5 of its coordinates and of |lp| + distance for the f32 roundings of
+// lp + s * distance.  One wave per tile, two texels per lane (index lane and 64 + lane, bit
+// dy * 9 + dx of the tile's two live words).  With need_mask (lazy cubes) only the needed texels
+// count: the others hold no photon and are left out of the mask and the box, which then describe
+// the needed texels only (the gather asks only about those).
 __device__ __forceinline__ void face_plane(uint32_t face, double sc, double tc, double* f) {
   switch (face) {
     case 0: f[0] = 1.0; f[1] = -tc; f[2] = -sc; break;
@@ -2157,20 +2159,30 @@ __global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32
   for (uint64_t q = (uint64_t)blockIdx.x * waves + (threadIdx.x >> 6); q < total; q += (uint64_t)gridDim.x * waves) {
     const uint32_t c = (uint32_t)(q / per), rem = (uint32_t)(q % per);
     const uint32_t face = rem / (tpf * tpf), t = rem % (tpf * tpf);
-    const uint32_t x = (t % tpf) * kPhotonTile + (lane & 7u), y = (t / tpf) * kPhotonTile + (lane >> 3);
     const DevLight& L = lights[cube_slot[c]];
     const double lp[3] = {(double)L.pos[0], (double)L.pos[1], (double)L.pos[2]};
-    bool lit = false;
-    uint32_t code = 0;
+    // the tile's kTileSpan^2 texels, two per lane: index lane and 64 + lane (dy * kTileSpan + dx)
+    bool lit2[2] = {false, false};
+    uint32_t code2[2] = {0u, 0u};
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, wmax = 0.0;
-    if (x < res && y < res && (!need_mask || ((need_mask[q] >> lane) & 1ull))) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t idx = (uint32_t)h * 64u + lane;
+      if (idx >= kTileSpan * kTileSpan) continue;
+      const uint32_t x = (t % tpf) * kPhotonTile + idx % kTileSpan, y = (t / tpf) * kPhotonTile + idx / kTileSpan;
+      if (x >= res || y >= res) continue;
+      if (need_mask) {  // lazy cubes: the texel's need bit, in its own tile's 8x8 mask
+        const uint64_t own = (uint64_t)c * per + (uint64_t)face * tpf * tpf + (y / kPhotonTile) * tpf + x / kPhotonTile;
+        if (!((need_mask[own] >> ((y % kPhotonTile) * kPhotonTile + x % kPhotonTile)) & 1ull)) continue;
+      }
       const uint4 ph = reinterpret_cast<const uint4*>(photons)[(uint64_t)c * 6 * res * res + (uint64_t)face * res * res +
                                                                (uint64_t)y * res + x];
-      code = ph.y & 0xffffffu;
-      lit = code != 0u;
-      if (lit) {
+      const uint32_t code = ph.y & 0xffffffu;
+      code2[h] = code;
+      lit2[h] = code != 0u;
+      if (lit2[h]) {
         const double w = (double)__uint_as_float(ph.x), e = 1.0e-6;
-        wmax = w;
+        wmax = fmax(wmax, w);
         const double s0 = ((double)x / res - e) * 2.0 - 1.0, s1 = ((double)(x + 1) / res + e) * 2.0 - 1.0;
         const double t0 = ((double)y / res - e) * 2.0 - 1.0, t1 = ((double)(y + 1) / res + e) * 2.0 - 1.0;
         const double sn = fmin(fmax(0.0, s0), s1), tn = fmin(fmax(0.0, t0), t1);
@@ -2192,10 +2204,14 @@ __global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32
         }
       }
     }
-    const uint64_t mask = __ballot(lit);
-    const uint32_t first = mask ? (uint32_t)__ffsll((unsigned long long)mask) - 1u : 0u;
-    const uint32_t code0 = __shfl(code, (int)first);
-    const bool uniform = mask != 0 && __ballot(lit && code != code0) == 0;
+    const uint64_t mask0 = __ballot(lit2[0]), mask1 = __ballot(lit2[1]);
+    // the first lit texel's normal code; uniform when every lit texel has it
+    const uint32_t first = mask0 ? (uint32_t)__ffsll((unsigned long long)mask0) - 1u
+                                 : mask1 ? (uint32_t)__ffsll((unsigned long long)mask1) - 1u : 0u;
+    const uint32_t code0 = mask0 ? __shfl(code2[0], (int)first) : __shfl(code2[1], (int)first);
+    const bool uniform =
+        (mask0 | mask1) != 0 && __ballot((lit2[0] && code2[0] != code0) || (lit2[1] && code2[1] != code0)) == 0;
+    const bool any = (mask0 | mask1) != 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       for (int a = 0; a < 3; ++a) {
@@ -2205,9 +2221,10 @@ __global__ void k_photon_tiles(const DevLight* __restrict__ lights, const uint32
       wmax = fmax(wmax, __shfl_xor(wmax, off));
     }
     if (lane == 0) {
-      live[q] = (unsigned long long)mask;
+      live[2 * q] = (unsigned long long)mask0;
+      live[2 * q + 1] = (unsigned long long)mask1;  // texels 64 .. kTileSpan^2 - 1
       float4 b0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b1 = b0, b2 = b0;
-      if (mask) {
+      if (any) {
         const f3 n = normalize_or_zero(mk3((float)(int8_t)(code0 & 0xffu) / 127.0f, (float)(int8_t)((code0 >> 8) & 0xffu) / 127.0f,
                                            (float)(int8_t)((code0 >> 16) & 0xffu) / 127.0f));
         b0 = make_float4(n.x, n.y, n.z, uniform ? 1.0f : 0.0f);
@@ -2253,10 +2270,19 @@ __device__ __forceinline__ bool tile_dark(const float4& b0, const float4& b1, co
 __device__ __forceinline__ bool tile_outside(const float4& b1, const float4& b2, f3 o) {
   return !(o.x >= b1.x && o.x <= b2.x && o.y >= b1.y && o.y <= b2.y && o.z >= b1.z && o.z <= b2.z);
 }
-// Tile index and bit of cube texel (face, x, y).
+// The tile holding cube texel (face, x, y) and the texel's bit in its summary (dy * kTileSpan + dx).
 __device__ __forceinline__ uint32_t photon_tile(uint32_t face, uint32_t x, uint32_t y, uint32_t tpf, uint32_t* bit) {
+  *bit = (y % kPhotonTile) * kTileSpan + (x % kPhotonTile);
+  return face * tpf * tpf + (y / kPhotonTile) * tpf + x / kPhotonTile;
+}
+// The same tile and the texel's bit in the tile's own 8x8 texels (need marks, lazy cubes).
+__device__ __forceinline__ uint32_t photon_tile8(uint32_t face, uint32_t x, uint32_t y, uint32_t tpf, uint32_t* bit) {
   *bit = (y % kPhotonTile) * kPhotonTile + (x % kPhotonTile);
   return face * tpf * tpf + (y / kPhotonTile) * tpf + x / kPhotonTile;
+}
+// Bit b of a tile's summary (two words).
+__device__ __forceinline__ bool tile_bit(const unsigned long long* live, uint64_t tile, uint32_t b) {
+  return (live[2 * tile + (b >> 6)] >> (b & 63u)) & 1ull;
 }
 
 // glam acos_approx (DirectXMath XMScalarAcos polynomial)
@@ -2734,7 +2760,7 @@ __device__ __forceinline__ bool gather_sample_casts(const GatherJob& g, const Tr
   if (p.tile_live) {
     uint32_t bit;
     const uint64_t tile = g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit);
-    if (!((p.tile_live[tile] >> bit) & 1ull)) return false;
+    if (!tile_bit(p.tile_live, tile, bit)) return false;
     if (tile_outside(p.tile_box[3 * tile + 1], p.tile_box[3 * tile + 2], g.origin)) return true;
   }
   const uint4 ph = reinterpret_cast<const uint4*>(g.cube)[ci];
@@ -2772,19 +2798,19 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
 #pragma unroll
   for (int j = 0; j < G; ++j) known[j] = 0u;
   if (p.tile_live) {
-    unsigned long long lw[G];
+    bool lw[G];
     float4 b0[G], b1[G], b2[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const uint64_t t = g.tile0 + tile[j];
-      lw[j] = p.tile_live[t];
+      lw[j] = tile_bit(p.tile_live, t, bit[j]);
       b0[j] = p.tile_box[3 * t];
       b1[j] = p.tile_box[3 * t + 1];
       b2[j] = p.tile_box[3 * t + 2];
     }
 #pragma unroll
     for (int j = 0; j < G; ++j)
-      known[j] = !((lw[j] >> bit[j]) & 1ull) ? 1u : tile_dark(b0[j], b1[j], b2[j], g.origin, g.normal, g.o_scale) ? 2u : 0u;
+      known[j] = !lw[j] ? 1u : tile_dark(b0[j], b1[j], b2[j], g.origin, g.normal, g.o_scale) ? 2u : 0u;
   }
   uint4 ph[G];
 #pragma unroll
@@ -3074,27 +3100,32 @@ __device__ __forceinline__ void fast_face(f3 d, float res, uint32_t* face, float
 }
 
 // The texels a sample's exact direction can fall in, from its fast direction: {x0, x1} x {y0, y1}
-// of one face (their coordinates widened by the margin, clamped as cube_texel_parts clamps).
+// of one face (their coordinates widened by the margin, clamped as cube_texel_parts clamps), all
+// in the summary of the tile holding (x0, y0).
 struct FastCand {
-  bool sure;        // the face is certain (else the sample is open)
-  uint64_t tile[4];
+  bool sure;      // the face is certain and the candidates span at most 2x2 texels (else the sample is open)
+  uint64_t tile;  // the tile whose summary holds every candidate
   uint32_t bit[4];
   uint32_t face, x0, x1, y0, y1;
 };
-// The candidate texels of fast direction d and the lit word of the first one's tile.
 __device__ __forceinline__ FastCand fast_candidates(const GatherJob& g, const BounceParams& p, f3 d, float fres, float margin) {
   FastCand c;
   uint32_t face;
   float tu, tv, gap;
   fast_face(d, fres, &face, &tu, &tv, &gap);
-  c.sure = gap > 2.5f * kSplitDirErr;
   const int32_t hi = (int32_t)p.cube_res - 1;
   const uint32_t x0 = (uint32_t)min(max((int32_t)floorf(tu - margin), 0), hi), x1 = (uint32_t)min(max((int32_t)floorf(tu + margin), 0), hi);
   const uint32_t y0 = (uint32_t)min(max((int32_t)floorf(tv - margin), 0), hi), y1 = (uint32_t)min(max((int32_t)floorf(tv + margin), 0), hi);
-  c.tile[0] = g.tile0 + photon_tile(face, x0, y0, p.tiles_per_face, &c.bit[0]);
-  c.tile[1] = g.tile0 + photon_tile(face, x1, y0, p.tiles_per_face, &c.bit[1]);
-  c.tile[2] = g.tile0 + photon_tile(face, x0, y1, p.tiles_per_face, &c.bit[2]);
-  c.tile[3] = g.tile0 + photon_tile(face, x1, y1, p.tiles_per_face, &c.bit[3]);
+  // (at cube resolutions where the margin reaches half a texel the span can exceed 2x2: open)
+  c.sure = gap > 2.5f * kSplitDirErr && x1 <= x0 + 1u && y1 <= y0 + 1u;
+  const uint32_t tpf = p.tiles_per_face, tx = x0 / kPhotonTile, ty = y0 / kPhotonTile;
+  c.tile = g.tile0 + (uint64_t)(face * tpf * tpf + ty * tpf + tx);
+  const uint32_t dx0 = x0 - tx * kPhotonTile, dx1 = x1 - tx * kPhotonTile;
+  const uint32_t dy0 = y0 - ty * kPhotonTile, dy1 = y1 - ty * kPhotonTile;
+  c.bit[0] = dy0 * kTileSpan + dx0;
+  c.bit[1] = dy0 * kTileSpan + dx1;
+  c.bit[2] = dy1 * kTileSpan + dx0;
+  c.bit[3] = dy1 * kTileSpan + dx1;
   c.face = face;
   c.x0 = x0;
   c.x1 = x1;
@@ -3102,38 +3133,25 @@ __device__ __forceinline__ FastCand fast_candidates(const GatherJob& g, const Bo
   c.y1 = y1;
   return c;
 }
-// Whether the tiles decide a sample whose exact texel is one of its candidates: 1 no ray (none of
-// them holds a lit photon), 2 a ray that brings no light (all of them hold one, and every tile
-// holding one of them is dark for this receiver), 0 open.
+// Whether the tile decides a sample whose exact texel is one of its candidates: 1 no ray (none of
+// them holds a lit photon), 2 a ray that brings no light (all of them hold one and the tile is
+// dark for this receiver), 0 open.
 __device__ __forceinline__ uint32_t fast_decide(const GatherJob& g, const BounceParams& p, const FastCand& c) {
   if (!c.sure) return 0u;
-  const uint64_t* tile = c.tile;
-  const uint32_t* bit = c.bit;
-  const unsigned long long w0 = p.tile_live[tile[0]];
-  // the candidates' lit bits: one word when they share a tile (the usual case)
-  const bool one = tile[1] == tile[0] && tile[2] == tile[0] && tile[3] == tile[0];
+  const uint4 w = reinterpret_cast<const uint4*>(p.tile_live)[c.tile];
+  const unsigned long long w0 = (unsigned long long)w.x | ((unsigned long long)w.y << 32);
+  const unsigned long long w1 = (unsigned long long)w.z | ((unsigned long long)w.w << 32);
   uint32_t lit = 0;
-  if (one) {
-    lit = (uint32_t)__popcll(w0 & ((1ull << bit[0]) | (1ull << bit[1]) | (1ull << bit[2]) | (1ull << bit[3])));
-    lit = lit == 0u ? 0u : lit == (uint32_t)__popcll((1ull << bit[0]) | (1ull << bit[1]) | (1ull << bit[2]) | (1ull << bit[3])) ? 4u : 1u;
-  } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) lit += (uint32_t)(((c == 0 || tile[c] == tile[0] ? w0 : p.tile_live[tile[c]]) >> bit[c]) & 1ull);
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = c.bit[k];
+    lit += (uint32_t)(((b < 64u ? w0 : w1) >> (b & 63u)) & 1ull);
   }
   if (lit == 0u) return 1u;
   if (lit != 4u) return 0u;
-  if (!tile_dark(p.tile_box[3 * tile[0]], p.tile_box[3 * tile[0] + 1], p.tile_box[3 * tile[0] + 2], g.origin, g.normal, g.o_scale))
-    return 0u;
-  if (!one) {
-#pragma unroll
-    for (int c = 1; c < 4; ++c) {
-      if (tile[c] == tile[0]) continue;
-      if (!tile_dark(p.tile_box[3 * tile[c]], p.tile_box[3 * tile[c] + 1], p.tile_box[3 * tile[c] + 2], g.origin, g.normal,
-                     g.o_scale))
-        return 0u;
-    }
-  }
-  return 2u;
+  return tile_dark(p.tile_box[3 * c.tile], p.tile_box[3 * c.tile + 1], p.tile_box[3 * c.tile + 2], g.origin, g.normal, g.o_scale)
+             ? 2u
+             : 0u;
 }
 
 // Phase A: per job its samples in order; open samples keep (z, azimuth) in their ray slot and set
@@ -3229,13 +3247,17 @@ __global__ void k_photon_mark(DevTris tr, DevMeshes ms, const DevLight* __restri
       const FastCand c = fast_candidates(g, p, d, fres, margin);
       if (fast && c.sure) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) need[64 * c.tile[j] + c.bit[j]] = 1;
+        for (int j = 0; j < 4; ++j) {
+          uint32_t bit;
+          const uint32_t t = photon_tile8(c.face, (j & 1) ? c.x1 : c.x0, (j & 2) ? c.y1 : c.y0, p.tiles_per_face, &bit);
+          need[64 * (g.tile0 + t) + bit] = 1;
+        }
       } else {
         f3 sample;
         exact_directions<1>(g, trig, &z, &ang, k, la, &sample);
         uint32_t face, x, y, bit;
         cube_texel_parts(sample, p.cube_res, &face, &x, &y);
-        need[64 * (g.tile0 + photon_tile(face, x, y, p.tiles_per_face, &bit)) + bit] = 1;
+        need[64 * (g.tile0 + photon_tile8(face, x, y, p.tiles_per_face, &bit)) + bit] = 1;
       }
     }
   }
