@@ -947,6 +947,36 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       }
     }
     HIP_TRY(hipEventRecord(ev1, s));
+    // The host side of the bounce stage's setup, while the shadow-ray kernels run: bounce entries
+    // grouped by light slot (triangle order within a light; the gather's results do not depend on
+    // the order, and consecutive work then shares one light's photon cube and the BVH nodes around
+    // it, which keeps the working set in L2/MALL), their 64-texel units, and the windows' offsets
+    // and filter tiles.
+    std::vector<uint64_t> win_off, be, be_units(1, 0), win_tiles;
+    if (bounce_in_scene && nbw > 0) {
+      win_off.assign(nbw + 1, 0);
+      for (uint32_t w = 0; w < nbw; ++w) win_off[w + 1] = win_off[w] + win_area[w0 + w];
+      std::vector<uint64_t> per_light(nl + 1, 0);
+      for (uint32_t t = 0; t < nbt; ++t)
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
+          if (hlights[b_ent_light[e]].bakes_bounce) per_light[b_ent_light[e] + 1]++;
+      for (uint32_t l = 0; l < nl; ++l) per_light[l + 1] += per_light[l];
+      be.resize(per_light[nl]);
+      for (uint32_t t = 0; t < nbt; ++t)
+        for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
+          if (hlights[b_ent_light[e]].bakes_bounce) be[per_light[b_ent_light[e]]++] = e;
+      be_units.resize(be.size() + 1);
+      for (size_t i = 0; i < be.size(); ++i) {
+        const uint32_t t = b_ent_tri[be[i]];
+        be_units[i + 1] = be_units[i] + (h_tex_start[t + 1] - h_tex_start[t] + 63) / 64;
+      }
+      win_tiles.assign(nbw + 1, 0);  // square tiles of the filter per window
+      for (uint32_t w = 0; w < nbw; ++w) {
+        const uint32_t* q = &win_rect[4 * (size_t)(w0 + w)];
+        win_tiles[w + 1] = win_tiles[w] + (uint64_t)((q[2] - q[0] + kFilterTile) / kFilterTile) *
+                                              ((q[3] - q[1] + kFilterTile) / kFilterTile);
+      }
+    }
     HIP_TRY(hipStreamSynchronize(s));
     float kms = 0.0f;
     (void)hipEventElapsedTime(&kms, ev0, ev1);
@@ -964,41 +994,13 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     DBuf<uint8_t> d_tile_lit;
     DevBounce bb{};
     if (bounce_in_scene && nbw > 0) {
-      std::vector<uint64_t> win_off(nbw + 1, 0);
-      for (uint32_t w = 0; w < nbw; ++w) win_off[w + 1] = win_off[w] + win_area[w0 + w];
       std::vector<uint32_t> b_win_rect(win_rect.begin() + 4 * (size_t)w0, win_rect.begin() + 4 * (size_t)w1);
       std::vector<uint32_t> b_win_res(win_res.begin() + w0, win_res.begin() + w1);
-      // Bounce entries grouped by light slot (triangle order within a light): the gather's
-      // results do not depend on the order, and consecutive work then shares one light's photon
-      // cube and the BVH nodes around it, which keeps the working set in L2/MALL.
-      std::vector<uint64_t> be, be_units(1, 0);
-      {
-        std::vector<uint64_t> per_light(nl + 1, 0);
-        for (uint32_t t = 0; t < nbt; ++t)
-          for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
-            if (hlights[b_ent_light[e]].bakes_bounce) per_light[b_ent_light[e] + 1]++;
-        for (uint32_t l = 0; l < nl; ++l) per_light[l + 1] += per_light[l];
-        be.resize(per_light[nl]);
-        for (uint32_t t = 0; t < nbt; ++t)
-          for (uint64_t e = b_ent_start[t]; e < b_ent_start[t + 1]; ++e)
-            if (hlights[b_ent_light[e]].bakes_bounce) be[per_light[b_ent_light[e]]++] = e;
-        be_units.resize(be.size() + 1);
-        for (size_t i = 0; i < be.size(); ++i) {
-          const uint32_t t = b_ent_tri[be[i]];
-          be_units[i + 1] = be_units[i] + (h_tex_start[t + 1] - h_tex_start[t] + 63) / 64;
-        }
-      }
       bool sun_bounce = false;
       for (uint64_t e : be) sun_bounce |= hlights[b_ent_light[e]].shape == 3;
       HIP_TRY(upload(ctx, d_win_rect, b_win_rect));
       HIP_TRY(upload(ctx, d_win_res, b_win_res));
       HIP_TRY(upload(ctx, d_win_off, win_off));
-      std::vector<uint64_t> win_tiles(nbw + 1, 0);  // square tiles of the filter per window
-      for (uint32_t w = 0; w < nbw; ++w) {
-        const uint32_t* q = &b_win_rect[4 * (size_t)w];
-        win_tiles[w + 1] = win_tiles[w] + (uint64_t)((q[2] - q[0] + kFilterTile) / kFilterTile) *
-                                              ((q[3] - q[1] + kFilterTile) / kFilterTile);
-      }
       HIP_TRY(upload(ctx, d_win_start, win_tiles));
       HIP_TRY(zeroed(ctx, d_tile_lit, win_tiles[nbw] + 1));
       HIP_TRY(upload(ctx, d_ent_window, b_ent_window));
