@@ -499,7 +499,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the bake shards receiver meshes across GPUs only by design, see DESIGN.md)", flush=True)
+              "(a sharded bake, receiver meshes split across GPUs, is designed but not built)", flush=True)
         sys.exit(2)
 
     if args.workload == "batch":
