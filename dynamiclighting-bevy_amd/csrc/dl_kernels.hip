@@ -3043,14 +3043,18 @@ __global__ void k_gather_rays(DevTris tr, DevMeshes ms, const DevLight* __restri
 // gather_group.
 //
 // kSplitDirErr bounds the difference of each component of the fast unit direction from the
-// exact one wherever |cos| of the slerp angle is at most kSplitCos (samples beyond are open: the
-// slerp's 1 / sin(angle) amplifies the fast sines' errors there).  The GPU test of
-// dl_test_gather_directions measures that difference on 16M samples drawn as the bake draws them
-// and requires it below a tenth of the bound (about 6e-6 is measured).
+// exact one wherever the cosine of the slerp angle lies in [-kSplitCos, kSplitCosNear] (samples
+// beyond are open: near -1 the slerp's 1 / sin(angle) amplifies the fast sines' errors; near +1
+// the slerp tends to the lerp and stays well conditioned, up to a margin from glam's lerp branch
+// at 1 - 3e-7).  The GPU test of dl_test_gather_directions measures that difference on 16M
+// samples drawn as the bake draws them and requires it below a tenth of the bound (about 6e-6 is
+// measured, 5e-7 for cosines above 0.99).
 static constexpr float kSplitCos = 0.99f;
+static constexpr float kSplitCosNear = 0.9999f;
 static constexpr float kSplitDirErr = 1.0e-4f;
 
-// glam's slerp and normalize of gather_exact in fast arithmetic; false beyond kSplitCos.
+// glam's slerp and normalize of gather_exact in fast arithmetic; false outside
+// [-kSplitCos, kSplitCosNear].
 __device__ __forceinline__ bool fast_direction(f3 a, float la, float inv_la, float z, float ang, float sp, f3* out) {
   float sa, ca;
   __sincosf(ang, &sa, &ca);
@@ -3060,7 +3064,7 @@ __device__ __forceinline__ bool fast_direction(f3 a, float la, float inv_la, flo
   const float ilb = __builtin_amdgcn_rsqf(lb2);
   const float lb = lb2 * ilb;
   const float dd = fmaf(a.x, rnd.x, fmaf(a.y, rnd.y, a.z * rnd.z)) * (inv_la * ilb);
-  if (!(fabsf(dd) <= kSplitCos)) return false;
+  if (!(dd <= kSplitCosNear && dd >= -kSplitCos)) return false;
   const float x = fabsf(dd);
   float t = fmaf(fmaf(fmaf(fmaf(fmaf(fmaf(fmaf(-0.0012624911f, x, 0.0066700901f), x, -0.0170881256f), x, 0.0308918810f), x,
                                 -0.0501743046f), x, 0.0889789874f), x, -0.2145988016f), x, 1.5707963050f);
