@@ -811,6 +811,8 @@ int dl_set_gather_split(dl_ctx* ctx, int enable) {
 uint32_t dl_test_light_bvh(const float* positions, const float* radii, uint32_t n, dl_light_bvh_node* nodes, uint32_t cap,
                            uint32_t* order) {
   if (n && (!positions || !radii || !order)) return 0;
+  for (uint64_t i = 0; i < 3ull * n; ++i)
+    if (!std::isfinite(positions[i])) return 0;  // (the bake validates its lights before the build)
   try {
     std::vector<dl::f3> pos(n);
     std::vector<float> rad(radii, radii + n);

@@ -101,7 +101,8 @@ int dl_test_gather_directions(dl_ctx* ctx, const float* dir3, const float* z, co
 
 /* The light BVH the bake builds on the host (LightBvh::build, light_bvh.rs:59-189) over n point
  * lights (positions: 3 floats each, radii): writes up to `cap` nodes and the lights' slot order
- * (n entries) and returns the node count (0 when cap is too small).  Needs no device. */
+ * (n entries) and returns the node count (0 when cap is too small or a position is not
+ * finite).  Needs no device. */
 uint32_t dl_test_light_bvh(const float* positions, const float* radii, uint32_t n, dl_light_bvh_node* nodes, uint32_t cap,
                            uint32_t* order);
 
