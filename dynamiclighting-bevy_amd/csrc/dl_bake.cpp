@@ -991,7 +991,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     DBuf<float> d_map_a, d_map_b;
     DBuf<uint32_t> d_win_rect, d_win_res, d_ent_window;
     DBuf<uint64_t> d_win_off, d_win_start, d_be, d_be_units;  // d_win_start: filter tiles per window
-    DBuf<uint8_t> d_tile_lit;
+    DBuf<uint8_t> d_tile_lit, d_tile_nb;
     DevBounce bb{};
     if (bounce_in_scene && nbw > 0) {
       std::vector<uint32_t> b_win_rect(win_rect.begin() + 4 * (size_t)w0, win_rect.begin() + 4 * (size_t)w1);
@@ -1003,6 +1003,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       HIP_TRY(upload(ctx, d_win_off, win_off));
       HIP_TRY(upload(ctx, d_win_start, win_tiles));
       HIP_TRY(zeroed(ctx, d_tile_lit, win_tiles[nbw] + 1));
+      HIP_TRY(d_tile_nb.alloc(win_tiles[nbw] + 1));  // written by the filter, tile by tile
       HIP_TRY(upload(ctx, d_ent_window, b_ent_window));
       HIP_TRY(upload(ctx, d_be, be));
       HIP_TRY(upload(ctx, d_be_units, be_units));
@@ -1020,6 +1021,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
       bb.rays = d_mbrays.p;
       bb.tile_start = d_win_start.p;
       bb.tile_lit = d_tile_lit.p;
+      bb.tile_nb = d_tile_nb.p;
       bb.wrays = cap ? d_wcount.p : nullptr;
       BounceParams bp;
       bp.seed = opt.seed;

@@ -231,6 +231,8 @@ struct DevBounce {
   unsigned long long* rays;   // per mesh
   const uint64_t* tile_start; // filter tiles (kFilterTile squares) before each window, windows + 1
   uint8_t* tile_lit;          // per filter tile, zeroed: 1 once a gathered value in it has a positive component
+  uint8_t* tile_nb;           // per filter tile, written by the filter: 1 when it wrote the tile's values in
+                              // map_b, 0 when they are all zero (no mark among its 3x3 tiles; left unwritten)
   unsigned long long* wrays = nullptr;  // per window: bounce rays of its (mesh, light) (capture; may be null)
 };
 hipError_t launch_photons(hipStream_t s, DevScene scene, DevOcc occ, const DevLight* lights, const uint32_t* cube_slot,

@@ -3666,15 +3666,24 @@ __global__ void k_sun_sum(DevTris tr, DevEntries ent, const DevLight* __restrict
   }
 }
 
+// A window of the filtered map: texels outside it, and texels of filter tiles the filter left
+// unwritten (all zero), read as zero.
 struct WinView {
   const float* map;
   uint32_t x0, y0, x1, y1;
+  const uint8_t* nb;  // the window's tile_nb flags
   __device__ f3 get(int64_t x, int64_t y) const {
     if (x < (int64_t)x0 || y < (int64_t)y0 || x > (int64_t)x1 || y > (int64_t)y1) return mk3(0.0f, 0.0f, 0.0f);
+    const uint64_t tiles_x = (x1 - x0 + kFilterTile) / kFilterTile;
+    if (!nb[(uint64_t)(y - y0) / kFilterTile * tiles_x + (uint64_t)(x - x0) / kFilterTile]) return mk3(0.0f, 0.0f, 0.0f);
     const float* p = map + 3 * ((uint64_t)(x - x0) + (uint64_t)(y - y0) * (x1 - x0 + 1));
     return mk3(p[0], p[1], p[2]);
   }
 };
+__device__ __forceinline__ WinView win_view(const DevBounce& b, uint32_t w) {
+  const uint32_t* wr = b.win_rect + 4 * (size_t)w;
+  return WinView{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3], b.tile_nb + b.tile_start[w]};
+}
 
 // BounceMap::dilate, then blur along x, then along y (bounce_map.rs:60-125), fused: a block takes
 // a kFilterTile-square tile of one window, loads the gather's values (map_a) with a halo of 4 into LDS,
@@ -3708,17 +3717,8 @@ __global__ __launch_bounds__(256) void k_bounce_filter(DevBounce b, const uint32
       for (int64_t v = ty - 1; v <= ty + 1; ++v)
         for (int64_t u = tx - 1; u <= tx + 1; ++u)
           if (u >= 0 && v >= 0 && u < (int64_t)tiles_x && v < tiles_y) lit |= b.tile_lit[tile_start[w] + (uint64_t)(v * (int64_t)tiles_x + u)] != 0;
-      if (!lit) {
-        for (int q = threadIdx.x; q < T * T; q += blockDim.x) {
-          const int64_t x = X0 + q % T, y = Y0 + q / T;
-          if (!inside(x, y)) continue;
-          float* d = b.map_b + 3 * (b.win_off[w] + (uint64_t)(x - x0) + (uint64_t)(y - y0) * ww);
-          d[0] = 0.0f;
-          d[1] = 0.0f;
-          d[2] = 0.0f;
-        }
-        continue;
-      }
+      if (threadIdx.x == 0) b.tile_nb[tile] = lit ? 1 : 0;
+      if (!lit) continue;  // left unwritten: readers see the flag and take zeros
     }
     for (int q = threadIdx.x; q < TI * TI; q += blockDim.x) {
       const int64_t x = X0 - 4 + q % TI, y = Y0 - 4 + q / TI;
@@ -3800,7 +3800,7 @@ __global__ void k_bounce_has(DevTris tr, DevEntries ent, DevBounce b, uint8_t* h
     if (w != 0xffffffffu) {
       const uint32_t* c = tr.crop + 4 * (size_t)ent.tri[e];
       const uint32_t* wr = b.win_rect + 4 * (size_t)w;
-      const WinView v{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+      const WinView v = win_view(b, w);
       const uint32_t cw = c[2] >= c[0] ? c[2] - c[0] + 1u : 0u, ch = c[3] >= c[1] ? c[3] - c[1] + 1u : 0u;
       uint32_t count = cw * ch;  // crop sides are at most 65535
       // A filtered texel is positive only in a tile with a marked tile among its 3x3 neighbours:
@@ -3861,9 +3861,8 @@ struct PackView {
 
 __device__ __forceinline__ PackView pack_view(const DevTris& tr, const DevEntries& ent, const DevBounce& b, uint64_t e) {
   uint32_t t = ent.tri[e], w = b.ent_window[e];
-  const uint32_t* wr = b.win_rect + 4 * (size_t)w;
   PackView pv;
-  pv.v = WinView{b.map_b + 3 * b.win_off[w], wr[0], wr[1], wr[2], wr[3]};
+  pv.v = win_view(b, w);
   pv.cx0 = tr.crop[4 * (size_t)t];
   pv.cy0 = tr.crop[4 * (size_t)t + 1];
   pv.W = tr.crop[4 * (size_t)t + 2] - pv.cx0 + 5;
