@@ -2693,12 +2693,12 @@ __device__ __forceinline__ void gather_draws(const GatherJob& g, Pcg& rng, uint3
   }
 }
 
-// The exact sample directions of samples k0 .. k0 + G - 1 from their draws (random.rs:22-28,
-// bounce.rs:262-270: on_unit_sphere, glam's slerp from the direction to the texel, normalize).
+// The exact sample directions of G samples from their draws (random.rs:22-28, bounce.rs:262-270:
+// on_unit_sphere, glam's slerp from the direction to the texel, normalize): sample j is sample
+// k[j] of job *gj[j] (la[j] = |gj[j]->dir0|).
 template <int G>
-__device__ __forceinline__ void exact_directions(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
-                                                 uint32_t k0, float la, f3* sample) {
-  const f3 a = g.dir0;
+__device__ __forceinline__ void exact_directions_of(const GatherJob* const* gj, const TrigTables& trig, const float* z,
+                                                    const float* ang, const uint32_t* k, const float* la, f3* sample) {
   // on_unit_sphere: sin/cos of the azimuths
   double vs[G], vc[G];
 #pragma unroll
@@ -2717,11 +2717,12 @@ __device__ __forceinline__ void exact_directions(const GatherJob& g, const TrigT
   bool common[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
+    const f3 a = gj[j]->dir0;
     const float r = sqrtf(maxf_(1.0f - z[j] * z[j], 0.0f));
     rnd[j] = mk3(r * ca[j], r * sa[j], z[j]);
-    sp[j] = 0.1f + (float)(k0 + j) * g.step;
+    sp[j] = 0.1f + (float)k[j] * gj[j]->step;
     lb[j] = len(rnd[j]);
-    const float dd = dot(a, rnd[j]) / (la * lb[j]);
+    const float dd = dot(a, rnd[j]) / (la[j] * lb[j]);
     common[j] = fabsf(dd) < 1.0f - 3.0e-7f;
     const float theta = common[j] ? acos_approx(dd) : 0.0f;
     arg[3 * j] = theta;
@@ -2735,16 +2736,33 @@ __device__ __forceinline__ void exact_directions(const GatherJob& g, const TrigT
   trig_finish_n<3 * G>(trig, trig.sin, arg, va, sv);
 #pragma unroll
   for (int j = 0; j < G; ++j) {
+    const f3 a = gj[j]->dir0;
     f3 sl;
     if (common[j]) {
-      const float rl = la + (lb[j] - la) * sp[j];
-      sl = mul(add(mul(mul(a, rl / la), sv[3 * j + 1]), mul(mul(rnd[j], rl / lb[j]), sv[3 * j + 2])), 1.0f / sv[3 * j]);
+      const float rl = la[j] + (lb[j] - la[j]) * sp[j];
+      sl = mul(add(mul(mul(a, rl / la[j]), sv[3 * j + 1]), mul(mul(rnd[j], rl / lb[j]), sv[3 * j + 2])), 1.0f / sv[3 * j]);
     } else {
       sl = slerp3(trig, a, rnd[j], sp[j]);
     }
     sample[j] = normalize_or(sl, a);
   }
 }
+// Samples k0 .. k0 + G - 1 of one job (la = |g.dir0|).
+template <int G>
+__device__ __forceinline__ void exact_directions(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
+                                                 uint32_t k0, float la, f3* sample) {
+  const GatherJob* gj[G];
+  uint32_t k[G];
+  float las[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    gj[j] = &g;
+    k[j] = k0 + (uint32_t)j;
+    las[j] = la;
+  }
+  exact_directions_of<G>(gj, trig, z, ang, k, las, sample);
+}
+
 // Whether sample k of a job casts a ray (bounce.rs:262-286: its photon has a lit surface and a
 // positive distance), reading the photon only when its tile leaves it open: an unlit texel casts
 // none, a lit one whose tile box excludes the origin casts one.
@@ -2777,14 +2795,14 @@ __device__ __forceinline__ bool gather_sample_casts(const GatherJob& g, const Tr
 // fetch; its outputs are those the photon would have given (ray false, or ray true with a zero
 // contribution, which is never traced).  la = |g.dir0|.
 template <int G>
-__device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
-                                             uint32_t k0, float la, const BounceParams& p, f3* dout, float* dist, f3* cout,
-                                             bool* ray, uint32_t* reads) {
+__device__ __forceinline__ void gather_exact_of(const GatherJob* const* gj, const TrigTables& trig, const float* z,
+                                                const float* ang, const uint32_t* k, const float* la, const BounceParams& p,
+                                                f3* dout, float* dist, f3* cout, bool* ray, uint32_t* reads) {
   const uint32_t res = p.cube_res, colored = p.colored;
   // sample direction, cube texel, photon
   uint32_t ci[G], tile[G], bit[G];
   f3 sample[G];
-  exact_directions<G>(g, trig, z, ang, k0, la, sample);
+  exact_directions_of<G>(gj, trig, z, ang, k, la, sample);
 #pragma unroll
   for (int j = 0; j < G; ++j) {
     uint32_t face, x, y;
@@ -2802,7 +2820,7 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
     float4 b0[G], b1[G], b2[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      const uint64_t t = g.tile0 + tile[j];
+      const uint64_t t = gj[j]->tile0 + tile[j];
       lw[j] = tile_bit(p.tile_live, t, bit[j]);
       b0[j] = p.tile_box[3 * t];
       b1[j] = p.tile_box[3 * t + 1];
@@ -2810,19 +2828,20 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
     }
 #pragma unroll
     for (int j = 0; j < G; ++j)
-      known[j] = !lw[j] ? 1u : tile_dark(b0[j], b1[j], b2[j], g.origin, g.normal, g.o_scale) ? 2u : 0u;
+      known[j] = !lw[j] ? 1u : tile_dark(b0[j], b1[j], b2[j], gj[j]->origin, gj[j]->normal, gj[j]->o_scale) ? 2u : 0u;
   }
   uint4 ph[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
-    const bool read = k0 + j < g.samples && known[j] == 0u;
-    ph[j] = read ? reinterpret_cast<const uint4*>(g.cube)[ci[j]] : make_uint4(0u, 0u, 0u, 0u);
+    const bool read = k[j] < gj[j]->samples && known[j] == 0u;
+    ph[j] = read ? reinterpret_cast<const uint4*>(gj[j]->cube)[ci[j]] : make_uint4(0u, 0u, 0u, 0u);
     *reads += read ? 1u : 0u;
   }
 #pragma unroll
   for (int j = 0; j < G; ++j) {
+    const GatherJob& g = *gj[j];
     if (known[j] != 0u) {
-      ray[j] = k0 + j < g.samples && known[j] == 2u;
+      ray[j] = k[j] < g.samples && known[j] == 2u;
       dout[j] = mk3(0.0f, 0.0f, 0.0f);
       cout[j] = mk3(0.0f, 0.0f, 0.0f);
       dist[j] = 0.0f;
@@ -2838,7 +2857,7 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
     dout[j] = mk3(0.0f, 0.0f, 0.0f);
     cout[j] = mk3(0.0f, 0.0f, 0.0f);
     dist[j] = 0.0f;
-    if (k0 + j >= g.samples || is_zero(pn) || dl <= 0.0f) continue;
+    if (k[j] >= g.samples || is_zero(pn) || dl <= 0.0f) continue;
     const f3 dir = divs(to_photon, dl);
     const float weight = maxf_(dot(g.normal, dir), 0.0f) * maxf_(dot(pn, neg(dir)), 0.0f);
     const f3 alb = colored ? mk3((float)(w2 & 0xffu) / 255.0f, (float)((w2 >> 8) & 0xffu) / 255.0f,
@@ -2849,6 +2868,22 @@ __device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTable
     dist[j] = dl;
     cout[j] = mul(alb, weight);
   }
+}
+// Samples k0 .. k0 + G - 1 of one job (la = |g.dir0|).
+template <int G>
+__device__ __forceinline__ void gather_exact(const GatherJob& g, const TrigTables& trig, const float* z, const float* ang,
+                                             uint32_t k0, float la, const BounceParams& p, f3* dout, float* dist, f3* cout,
+                                             bool* ray, uint32_t* reads) {
+  const GatherJob* gj[G];
+  uint32_t k[G];
+  float las[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    gj[j] = &g;
+    k[j] = k0 + (uint32_t)j;
+    las[j] = la;
+  }
+  gather_exact_of<G>(gj, trig, z, ang, k, las, p, dout, dist, cout, ray, reads);
 }
 // Draws and evaluates samples k0 .. k0 + G - 1 in order.
 template <int G>
@@ -3274,45 +3309,66 @@ __global__ void k_gather_open(DevTris tr, DevMeshes ms, const DevLight* __restri
                               BounceParams p, TrigTables trig, float4* rays, float* contrib, uint32_t order,
                               const uint32_t* __restrict__ list, const uint64_t* __restrict__ count, uint32_t* tmask,
                               uint32_t words, unsigned long long* tcount) {
+  // two list entries per lane (2q and 2q + 1, often samples of one job): their exact chains run
+  // side by side, so that each one's dependent loads overlap the other's
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, total = *count;
-  for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); q0 < total; q0 += stride) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, total = *count, pairs = (total + 1) / 2;
+  for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); q0 < pairs; q0 += stride) {
     const uint64_t q = q0 + lane;
-    uint32_t m = 0xffffffffu, nr = 0, win = 0, reads = 0;
-    if (q < total) {
-      const uint32_t r = list[q];
-      uint64_t i;
-      uint32_t k;
-      if (order) {  // slot k * n + i (slots and n are below 2^32)
-        k = r / (uint32_t)n;
-        i = r - k * (uint32_t)n;
-      } else {
-        i = owner_of(rs, n, r);
-        k = (uint32_t)(r - rs[i]);
+    uint32_t m[2] = {0xffffffffu, 0xffffffffu}, nr[2] = {0u, 0u}, win[2] = {0u, 0u}, reads = 0;
+    bool valid[2] = {false, false};
+    if (q < pairs) {
+      uint32_t r[2], k[2];
+      uint64_t i[2];
+      uint2 jb[2];
+      GatherJob g[2];
+      float z[2], ang[2], la[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        valid[h] = 2 * q + h < total;
+        r[h] = list[valid[h] ? 2 * q + h : 2 * q];
+        if (order) {  // slot k * n + i (slots and n are below 2^32)
+          k[h] = r[h] / (uint32_t)n;
+          i[h] = r[h] - k[h] * (uint32_t)n;
+        } else {
+          i[h] = owner_of(rs, n, r[h]);
+          k[h] = (uint32_t)(r[h] - rs[i[h]]);
+        }
+        jb[h] = jobs[i[h]];
+        win[h] = b.ent_window[jb[h].y];
+        g[h] = gather_job(tr, ms, lights, tx, ent, b, p, jb[h].x, jb[h].y, nullptr, &m[h]);
+        if (!valid[h]) g[h].samples = 0u;  // a copy of entry 2q past the end: reads and casts nothing
+        const float4 dr = rays[2 * r[h]];
+        z[h] = dr.x;
+        ang[h] = dr.y;
+        la[h] = len(g[h].dir0);
       }
-      const uint2 jb = jobs[i];
-      win = b.ent_window[jb.y];
-      const GatherJob g = gather_job(tr, ms, lights, tx, ent, b, p, jb.x, jb.y, nullptr, &m);
-      const float4 dr = rays[2 * r];
-      const float z = dr.x, ang = dr.y;
-      f3 d, c;
-      float dist;
-      bool ray;
-      gather_exact<1>(g, trig, &z, &ang, k, len(g.dir0), p, &d, &dist, &c, &ray, &reads);
-      nr = ray ? 1u : 0u;
-      if (ray && !(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) {
-        const uint32_t alpha = lights[ent.light[jb.y]].alpha ? 1u : 0u;
-        rays[2 * r] = make_float4(g.origin.x, g.origin.y, g.origin.z, dist);
-        rays[2 * r + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(alpha | 2u));
-        contrib[3 * r] = c.x;
-        contrib[3 * r + 1] = c.y;
-        contrib[3 * r + 2] = c.z;
-        atomicOr(&tmask[i * words + (k >> 5)], 1u << (k & 31u));
-        atomicAdd(&tcount[i], 1ull);
+      const GatherJob* gj[2] = {&g[0], &g[1]};
+      f3 d[2], c[2];
+      float dist[2];
+      bool ray[2];
+      gather_exact_of<2>(gj, trig, z, ang, k, la, p, d, dist, c, ray, &reads);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!valid[h]) continue;
+        nr[h] = ray[h] ? 1u : 0u;
+        if (ray[h] && !(c[h].x == 0.0f && c[h].y == 0.0f && c[h].z == 0.0f)) {
+          const uint32_t alpha = lights[ent.light[jb[h].y]].alpha ? 1u : 0u;
+          rays[2 * r[h]] = make_float4(g[h].origin.x, g[h].origin.y, g[h].origin.z, dist[h]);
+          rays[2 * r[h] + 1] = make_float4(d[h].x, d[h].y, d[h].z, __uint_as_float(alpha | 2u));
+          contrib[3 * r[h]] = c[h].x;
+          contrib[3 * r[h] + 1] = c[h].y;
+          contrib[3 * r[h] + 2] = c[h].z;
+          atomicOr(&tmask[i[h] * words + (k[h] >> 5)], 1u << (k[h] & 31u));
+          atomicAdd(&tcount[i[h]], 1ull);
+        }
       }
     }
-    wave_add_at(b.rays, m, nr, q < total);
-    if (b.wrays) wave_add_at(b.wrays, win, nr, q < total);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      wave_add_at(b.rays, m[h], nr[h], valid[h]);
+      if (b.wrays) wave_add_at(b.wrays, win[h], nr[h], valid[h]);
+    }
     if (p.photon_reads) wave_add(p.photon_reads, reads);
   }
 }
