@@ -308,9 +308,9 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
     }
   });
   const uint32_t nocc = use_res ? resident->nocc : (uint32_t)occ_flags.size();
-  Bvh4 bvh;
+  Bvh bvh;
   const auto t_bvh = std::chrono::steady_clock::now();
-  if (!use_res) build_bvh4(occ9.data(), nocc, ctx->host_threads, bvh);
+  if (!use_res) build_bvh(occ9.data(), nocc, ctx->host_threads, bvh);
   ctx->timings.bvh_ms = ms_since(t_bvh);
   ctx->timings.bvh_nodes = use_res ? resident->nodes : bvh.nodes.size();
   ctx->timings.occluder_triangles = nocc;
@@ -428,7 +428,7 @@ int dl_bake_impl(dl_ctx* ctx, const dl_bake_input* input, const dl_bake_options*
   sc.nodes = geo.d_nodes.p;
   sc.tris = reinterpret_cast<const float4*>(geo.d_occ.p);
   sc.ntris = nocc;
-  sc.stack = 3 * std::max(1u, geo.depth) + 1;
+  sc.stack = bvh_stack_entries(geo.depth);
   sc.stack16 = geo.nodes < 65536 ? 1u : 0u;
   sc.overflow = d_overflow.p;
   sc.counters = nullptr;

@@ -1,8 +1,8 @@
 // Host build of the occluder BVH: large triangles are first split into several references, each
 // bounded by the part of the triangle inside half of its parent reference's box (clipped in
 // double precision, bounds rounded outward); then binned SAH over reference centroids into a
-// binary tree with leaves of at most four references, collapsed to 4-wide nodes by opening the
-// largest children first.  Reference cutting, subtrees above a size threshold, leaf records and
+// binary tree with leaves of at most four references, collapsed to kBvhWidth-wide nodes by
+// opening the largest children first.  Reference cutting, subtrees above a size threshold, leaf records and
 // quantization run on worker threads.  A triangle split into several references appears in
 // several leaves; any-hit and closest-hit (lowest t, then lowest occluder index) results do not
 // depend on that.
@@ -474,14 +474,14 @@ int step_exponent(float lo, float hi) {
   return std::min(e, 127);
 }
 
-// Node4 (padded float child boxes) -> CNode; checks that every decoded box contains its child's.
-void quantize(const Node4& nd, CNode* out) {
+// NodeW (padded float child boxes) -> CNode; checks that every decoded box contains its child's.
+void quantize(const NodeW& nd, CNode* out) {
   std::memset(out, 0, sizeof *out);
   const float* los[3] = {nd.lox, nd.loy, nd.loz};
   const float* his[3] = {nd.hix, nd.hiy, nd.hiz};
   float origin[3];
-  uint32_t lo_q[3] = {0, 0, 0}, hi_q[3] = {0, 0, 0};
   int exps[3];
+  constexpr int kw = kBvhWidth / 4;  // words per bound row
   for (int a = 0; a < 3; ++a) {
     float mn = INFINITY, mx = -INFINITY;
     for (int k = 0; k < nd.count; ++k) {
@@ -492,7 +492,7 @@ void quantize(const Node4& nd, CNode* out) {
     origin[a] = mn;
     exps[a] = step_exponent(mn, mx);
     const double step = std::ldexp(1.0, exps[a]);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kBvhWidth; ++k) {
       uint32_t ql = 255, qh = 0;
       if (k < nd.count) {
         const double l = std::floor(((double)los[a][k] - (double)mn) / step) - 1.0;
@@ -507,39 +507,33 @@ void quantize(const Node4& nd, CNode* out) {
           qh = 255;
         }
       }
-      lo_q[a] |= ql << (8 * k);
-      hi_q[a] |= qh << (8 * k);
+      out->q[(2 * a) * kw + k / 4] |= ql << (8 * (k % 4));
+      out->q[(2 * a + 1) * kw + k / 4] |= qh << (8 * (k % 4));
     }
   }
   out->ox = origin[0];
   out->oy = origin[1];
   out->oz = origin[2];
   out->exps = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16);
-  out->lox = lo_q[0];
-  out->hix = hi_q[0];
-  out->loy = lo_q[1];
-  out->hiy = hi_q[1];
-  out->loz = lo_q[2];
-  out->hiz = hi_q[2];
-  for (int k = 0; k < 4; ++k) out->child[k] = nd.child[k];
+  for (int k = 0; k < kBvhWidth; ++k) out->child[k] = nd.child[k];
 }
 
 struct Pending {
   uint32_t n2, n4, depth;
 };
 
-// The 4-wide node `p.n4` from binary node `p.n2`: its two children, then the inner child with the
-// largest box opened into its two, up to four children; inner children get the next indices of
-// `wide` and are queued in `work`.
-void collapse_node(const Node2* n2, const Pending& p, std::vector<Node4>& wide, std::vector<Pending>& work) {
-  uint32_t kids[4];
+// The wide node `p.n4` from binary node `p.n2`: its two children, then the inner child with the
+// largest box opened into its two, up to kBvhWidth children; inner children get the next indices
+// of `wide` and are queued in `work`.
+void collapse_node(const Node2* n2, const Pending& p, std::vector<NodeW>& wide, std::vector<Pending>& work) {
+  uint32_t kids[kBvhWidth];
   int nk = 0;
   if (n2[p.n2].left < 0) {
     kids[nk++] = p.n2;  // whole tree is one leaf
   } else {
     kids[nk++] = (uint32_t)n2[p.n2].left;
     kids[nk++] = (uint32_t)n2[p.n2].right;
-    while (nk < 4) {
+    while (nk < kBvhWidth) {
       int open = -1;
       float best = -1.0f;
       for (int k = 0; k < nk; ++k)
@@ -553,7 +547,7 @@ void collapse_node(const Node2* n2, const Pending& p, std::vector<Node4>& wide, 
       kids[nk++] = (uint32_t)n2[o].right;
     }
   }
-  Node4 nd;
+  NodeW nd;
   std::memset(&nd, 0, sizeof nd);
   nd.count = nk;
   for (int k = 0; k < nk; ++k) {
@@ -580,7 +574,7 @@ void collapse_node(const Node2* n2, const Pending& p, std::vector<Node4>& wide, 
 
 }  // namespace
 
-void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
+void build_bvh(const float* tris9, uint32_t n, unsigned threads, Bvh& out) {
   out.nodes.clear();
   out.tris.clear();
   out.order.clear();
@@ -626,10 +620,10 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
     o[4] = t[3] - t[0]; o[5] = t[4] - t[1]; o[6] = t[5] - t[2]; o[7] = 0.0f;
     o[8] = t[6] - t[0]; o[9] = t[7] - t[1]; o[10] = t[8] - t[2]; o[11] = 0.0f;
   });
-  // Collapse to 4-wide nodes: the top of the tree here, subtrees below `cut` references on
+  // Collapse to wide nodes: the top of the tree here, subtrees below `cut` references on
   // worker threads; the nodes of each subtree follow the top's nodes in one block.
   const uint32_t cut = threads > 1 ? std::max<uint32_t>(4096, nr / (threads * 8)) : 0u;
-  std::vector<Node4> wide(1);
+  std::vector<NodeW> wide(1);
   std::vector<Pending> work{{0, 0, 1}}, deferred;
   while (!work.empty()) {
     const Pending p = work.back();
@@ -641,10 +635,10 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
     out.depth = std::max(out.depth, p.depth);
     collapse_node(n2, p, wide, work);
   }
-  std::vector<std::vector<Node4>> sub(deferred.size());
+  std::vector<std::vector<NodeW>> sub(deferred.size());
   std::vector<uint32_t> sub_depth(deferred.size(), 0);
   parallel_blocks(deferred.size(), threads, [&](uint64_t i) {
-    std::vector<Node4>& local = sub[i];
+    std::vector<NodeW>& local = sub[i];
     local.resize(1);
     std::vector<Pending> w{{deferred[i].n2, 0, deferred[i].depth}};
     while (!w.empty()) {
@@ -666,8 +660,8 @@ void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out) {
   // a subtree's root takes the slot the top reserved for it; its other nodes follow at base[i]
   parallel_blocks(sub.size(), threads, [&](uint64_t i) {
     for (size_t j = 0; j < sub[i].size(); ++j) {
-      Node4 nd = sub[i][j];
-      for (int k = 0; k < 4; ++k)
+      NodeW nd = sub[i][j];
+      for (int k = 0; k < kBvhWidth; ++k)
         if (nd.child[k] > 0) nd.child[k] = (int32_t)(base[i] + (size_t)nd.child[k] - 1);
       quantize(nd, &out.nodes[j == 0 ? deferred[i].n4 : base[i] + j - 1]);
     }

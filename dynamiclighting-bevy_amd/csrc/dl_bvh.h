@@ -1,6 +1,5 @@
-// Occluder BVH in device layout: 4-wide nodes of 128 bytes (one cache line pair), child boxes in
-// SoA order so one node is eight float4 loads, and triangles in leaf order as three float4 each
-// (v0 | occluder index, e1 = v1 - v0, e2 = v2 - v0).
+// Occluder BVH in device layout: wide nodes with 8-bit quantized child boxes (CNode), and
+// triangles in leaf order as three float4 each (v0 | occluder index, e1 = v1 - v0, e2 = v2 - v0).
 //
 // Boxes are grown by a relative 2^-15 of their coordinate magnitude, so the slab test, itself
 // widened by a relative 2^-16, never culls a triangle that Moller-Trumbore reports as hit: any-hit
@@ -8,6 +7,7 @@
 // over all occluders (occluder.rs:180-223 with ties resolved as at occluder.rs:205-209).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include <memory>
@@ -16,27 +16,36 @@
 
 namespace dl {
 
-// A collapsed 4-wide node with float child boxes (builder intermediate, quantized to CNode).
-struct alignas(16) Node4 {
-  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
-  int32_t child[4];  // > 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1)); 0 unused slot
-  int32_t count;     // children in use (1..4); kernels read the child slots instead
-  int32_t pad[3];
-};
-static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
+// Children per node (4 or 8, fixed at build time by DL_BVH_WIDTH).
+#ifndef DL_BVH_WIDTH
+#define DL_BVH_WIDTH 4
+#endif
+constexpr int kBvhWidth = DL_BVH_WIDTH;
+static_assert(kBvhWidth == 4 || kBvhWidth == 8, "BVH nodes are 4 or 8 wide");
 
-// The 4-wide node the kernels read: 64 bytes.  Child bounds are 8-bit steps from the node's
-// origin (its padded minimum corner) with a power-of-two step per axis, rounded outward by one
-// extra step, so each decoded box (origin + q * step, one float rounding) contains the padded
-// box of the child it stands for.
+// A collapsed wide node with float child boxes (builder intermediate, quantized to CNode).
+struct NodeW {
+  float lox[kBvhWidth], hix[kBvhWidth], loy[kBvhWidth], hiy[kBvhWidth], loz[kBvhWidth], hiz[kBvhWidth];
+  int32_t child[kBvhWidth];  // > 0 inner node; < 0 leaf: -1 - ((first << 2) | (count - 1)); 0 unused slot
+  int32_t count;             // children in use (1..kBvhWidth); kernels read the child slots instead
+};
+
+// The node the kernels read: 64 bytes when 4-wide, 128 when 8-wide.  Child bounds are 8-bit steps
+// from the node's origin (its padded minimum corner) with a power-of-two step per axis, rounded
+// outward by one extra step, so each decoded box (origin + q * step, one float rounding) contains
+// the padded box of the child it stands for.  q holds lo x, hi x, lo y, hi y, lo z, hi z in that
+// order, kBvhWidth / 4 words each, child k in byte k % 4 of word k / 4; the child slots start on a
+// 16-byte boundary.
+constexpr int kNodeQWords = 6 * kBvhWidth / 4;
+constexpr int kNodeQPad = (4 - (4 + kNodeQWords) % 4) % 4;
 struct alignas(64) CNode {
   float ox, oy, oz;
-  uint32_t exps;        // biased float exponents of the x, y, z steps in bits 0-7, 8-15, 16-23
-  uint32_t lox, hix, loy, hiy;  // byte k: child k
-  uint32_t loz, hiz, pad0, pad1;
-  int32_t child[4];     // as Node4::child; 0 marks an unused slot
+  uint32_t exps;  // biased float exponents of the x, y, z steps in bits 0-7, 8-15, 16-23
+  uint32_t q[kNodeQWords + kNodeQPad];
+  int32_t child[kBvhWidth];  // as NodeW::child; 0 marks an unused slot
 };
-static_assert(sizeof(CNode) == 64, "CNode is 64 bytes");
+static_assert(sizeof(CNode) == 16 * kBvhWidth, "CNode is 64 bytes when 4-wide, 128 when 8-wide");
+static_assert(offsetof(CNode, child) % 16 == 0, "child slots are 16-byte aligned");
 
 // An allocator that leaves value-initialized elements uninitialized: for large buffers that are
 // written in full right after they are sized.
@@ -59,14 +68,18 @@ struct NoInit : std::allocator<T> {
   }
 };
 
-struct Bvh4 {
+struct Bvh {
   std::vector<CNode> nodes;
   std::vector<float, NoInit<float>> tris;      // 12 floats per reference, leaf order
   std::vector<uint32_t, NoInit<uint32_t>> order;  // leaf position -> occluder index (a split triangle appears more than once)
   uint32_t depth = 0;           // node levels on the longest root-to-leaf path
 };
 
+// Traversal stack entries a BVH of `depth` node levels needs: a node visit pushes its hit inner
+// children except the nearest (at most kBvhWidth - 1) on top of what its ancestors pushed.
+inline uint32_t bvh_stack_entries(uint32_t depth) { return (uint32_t)(kBvhWidth - 1) * (depth > 1u ? depth : 1u) + 1u; }
+
 // Builds over `n` triangles given as 9 floats each (v0, v1, v2).
-void build_bvh4(const float* tris9, uint32_t n, unsigned threads, Bvh4& out);
+void build_bvh(const float* tris9, uint32_t n, unsigned threads, Bvh& out);
 
 }  // namespace dl

@@ -465,7 +465,7 @@ int dl_service_ray_queries_noalloc(dl_service* svc, const float* rays8, uint64_t
   sc.nodes = r.d_nodes.p;
   sc.tris = reinterpret_cast<const float4*>(r.d_occ.p);
   sc.ntris = r.nocc;
-  sc.stack = 3 * std::max(1u, r.depth) + 1;
+  sc.stack = bvh_stack_entries(r.depth);
   sc.stack16 = r.nodes < 65536 ? 1u : 0u;
   uint32_t overflow = 0;
   const hipError_t e = launch_ray_queries_noalloc(ctx->stream, ctx->device, sc, rays8, ray_count, closest ? 1 : 0, t_out,
@@ -565,8 +565,8 @@ int dl_last_timings(const dl_ctx* ctx, dl_timings* out) {
 int dl_test_bvh_structure(const float* tris9, uint32_t tri_count, uint32_t threads, uint64_t out[4]) {
   using namespace dl;
   if ((!tris9 && tri_count) || !out) return set_err(nullptr, DL_E_ARGUMENT, "null argument");
-  Bvh4 bvh;
-  build_bvh4(tris9, tri_count, threads, bvh);
+  Bvh bvh;
+  build_bvh(tris9, tri_count, threads, bvh);
   const size_t nn = bvh.nodes.size(), nr = bvh.order.size();
   auto bad = [](const char* what) { return set_err(nullptr, DL_E_ARGUMENT, std::string("BVH structure: ") + what); };
   if (tri_count == 0) {
@@ -581,7 +581,7 @@ int dl_test_bvh_structure(const float* tris9, uint32_t tri_count, uint32_t threa
       const uint32_t n = stack.back();
       stack.pop_back();
       deepest = std::max(deepest, depth[n]);
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kBvhWidth; ++k) {
         const int32_t c = bvh.nodes[n].child[k];
         if (c > 0) {
           if ((size_t)c >= nn || node_seen[c]++) return bad("an inner child out of range or reached twice");
@@ -626,8 +626,8 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   if (!ctx) return set_err(nullptr, DL_E_ARGUMENT, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  Bvh4 bvh;
-  build_bvh4(tris9, tri_count, ctx->host_threads, bvh);
+  Bvh bvh;
+  build_bvh(tris9, tri_count, ctx->host_threads, bvh);
   DBuf<CNode> n;
   DBuf<float> t, r, tout;
   DBuf<uint32_t> idx, ovf;
@@ -644,7 +644,7 @@ int dl_test_ray_queries(dl_ctx* ctx, const float* tris9, uint32_t tri_count, con
   sc.nodes = n.p;
   sc.tris = reinterpret_cast<const float4*>(t.p);
   sc.ntris = tri_count;
-  sc.stack = 3 * std::max(1u, bvh.depth) + 1;
+  sc.stack = bvh_stack_entries(bvh.depth);
   sc.stack16 = bvh.nodes.size() < 65536 ? 1u : 0u;
   sc.overflow = ovf.p;
   sc.counters = nullptr;

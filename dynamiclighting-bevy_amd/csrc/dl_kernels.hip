@@ -87,8 +87,6 @@ __device__ __forceinline__ bool alpha_blocks(const DevScene& sc, uint32_t occ, f
   return (sc.tex_bits[ti[2] + bit / 32] >> (bit % 32)) & 1u;
 }
 
-__device__ __forceinline__ int32_t sel4(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
-__device__ __forceinline__ float sel4(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
 // A ray prepared for slab tests: origin, reciprocal direction and origin * reciprocal.
 struct SlabRay {
@@ -105,8 +103,20 @@ __device__ __forceinline__ SlabRay slab_ray(f3 o, f3 d) {
   return r;
 }
 
-// Box tests of the (up to four) children of `node`: bit k of the result is set when child k's
-// decoded slab interval meets [0, limit], with tn[k] its entry distance.  A child bound is
+// The child slots of a node (> 0 inner node, < 0 leaf, 0 unused).
+struct Kids {
+  int32_t c[kBvhWidth];
+};
+// Child slot k of a node for a k known only at run time (a select chain, no indexed registers).
+__device__ __forceinline__ int32_t kid(const Kids& ch, int k) {
+  int32_t r = ch.c[0];
+#pragma unroll
+  for (int j = 1; j < kBvhWidth; ++j) r = k == j ? ch.c[j] : r;
+  return r;
+}
+
+// Box tests of the (up to kBvhWidth) children of `node`: bit k of the result is set when child
+// k's decoded slab interval meets [0, limit], with tn[k] its entry distance.  A child bound is
 // origin + q * step (8-bit q, power-of-two step); its plane distance is evaluated as
 // q * (step / d) + (origin / d - o / d) with fused multiply-adds.  step / d is exact, and the
 // roundings of the other terms are relative to the node's coordinates, the ray origin and the
@@ -114,11 +124,29 @@ __device__ __forceinline__ SlabRay slab_ray(f3 o, f3 d) {
 // coordinates), the boxes' padding (2^-15 of the coordinate) and the interval's widening
 // (2^-16 of the distance) cover them, so the test only culls boxes the ray misses.  The node
 // record is dead once this returns, which keeps it out of the registers of the leaf tests.
-__device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, const SlabRay& r, float limit, float tn[4],
-                                              int4* ch) {
+__device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, const SlabRay& r, float limit,
+                                              float tn[kBvhWidth], Kids* ch) {
+  constexpr int kQv = (kNodeQWords + kNodeQPad) / 4;  // float4 rows of quantized bounds
+  constexpr int kw = kBvhWidth / 4;                   // words per bound row
   const uint4* nb = reinterpret_cast<const uint4*>(sc.nodes + node);
-  const uint4 q0 = nb[0], q1 = nb[1], q2 = nb[2];
-  *ch = *reinterpret_cast<const int4*>(nb + 3);  // 0 marks an unused child slot (the root is no child)
+  const uint4 q0 = nb[0];
+  uint32_t qw[4 * kQv];
+#pragma unroll
+  for (int v = 0; v < kQv; ++v) {
+    const uint4 t = nb[1 + v];
+    qw[4 * v] = t.x;
+    qw[4 * v + 1] = t.y;
+    qw[4 * v + 2] = t.z;
+    qw[4 * v + 3] = t.w;
+  }
+#pragma unroll
+  for (int v = 0; v < kw; ++v) {  // 0 marks an unused child slot (the root is no child)
+    const int4 c = reinterpret_cast<const int4*>(nb + 1 + kQv)[v];
+    ch->c[4 * v] = c.x;
+    ch->c[4 * v + 1] = c.y;
+    ch->c[4 * v + 2] = c.z;
+    ch->c[4 * v + 3] = c.w;
+  }
   const float ax = __uint_as_float((q0.w & 0xffu) << 23) * r.ix;
   const float ay = __uint_as_float(((q0.w >> 8) & 0xffu) << 23) * r.iy;
   const float az = __uint_as_float(((q0.w >> 16) & 0xffu) << 23) * r.iz;
@@ -127,17 +155,20 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
   const float bz = __builtin_fmaf(__uint_as_float(q0.z), r.iz, -r.oz);
   uint32_t mask = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int sh = 8 * k;
-    const float t0x = __builtin_fmaf((float)((q1.x >> sh) & 0xffu), ax, bx), t1x = __builtin_fmaf((float)((q1.y >> sh) & 0xffu), ax, bx);
-    const float t0y = __builtin_fmaf((float)((q1.z >> sh) & 0xffu), ay, by), t1y = __builtin_fmaf((float)((q1.w >> sh) & 0xffu), ay, by);
-    const float t0z = __builtin_fmaf((float)((q2.x >> sh) & 0xffu), az, bz), t1z = __builtin_fmaf((float)((q2.y >> sh) & 0xffu), az, bz);
+  for (int k = 0; k < kBvhWidth; ++k) {
+    const int sh = 8 * (k % 4), w = k / 4;
+    const float t0x = __builtin_fmaf((float)((qw[w] >> sh) & 0xffu), ax, bx);
+    const float t1x = __builtin_fmaf((float)((qw[kw + w] >> sh) & 0xffu), ax, bx);
+    const float t0y = __builtin_fmaf((float)((qw[2 * kw + w] >> sh) & 0xffu), ay, by);
+    const float t1y = __builtin_fmaf((float)((qw[3 * kw + w] >> sh) & 0xffu), ay, by);
+    const float t0z = __builtin_fmaf((float)((qw[4 * kw + w] >> sh) & 0xffu), az, bz);
+    const float t1z = __builtin_fmaf((float)((qw[5 * kw + w] >> sh) & 0xffu), az, bz);
     float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
     float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
     n = __builtin_fmaf(fabsf(n), -kBoxEps, n);
     f = __builtin_fmaf(fabsf(f), kBoxEps, f);
     tn[k] = n;
-    if (sel4(*ch, k) != 0 && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
+    if (ch->c[k] != 0 && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
   }
   return mask;
 }
@@ -160,20 +191,23 @@ __device__ __forceinline__ void wave_add_at(unsigned long long* base, uint32_t k
 }
 
 // Bit k set when child k is a leaf (a negative child reference).
-__device__ __forceinline__ uint32_t leaf_bits(const int4& ch) {
-  return (ch.x < 0 ? 1u : 0u) | (ch.y < 0 ? 2u : 0u) | (ch.z < 0 ? 4u : 0u) | (ch.w < 0 ? 8u : 0u);
+__device__ __forceinline__ uint32_t leaf_bits(const Kids& ch) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < kBvhWidth; ++k) m |= ch.c[k] < 0 ? 1u << k : 0u;
+  return m;
 }
 
 // Pushes the hit inner children of a node except the nearest, which it returns (-1 if none).
-__device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t mask, const int4& ch, const float tn[4],
+__device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t mask, const Kids& ch, const float tn[kBvhWidth],
                                                  uint32_t* __restrict__ stack, uint32_t stride, uint32_t& sp) {
   uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
   const bool s16 = sc.stack16 != 0;
   int32_t next = -1;
   float next_t = INFINITY;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int32_t c = sel4(ch, k);
+  for (int k = 0; k < kBvhWidth; ++k) {
+    const int32_t c = ch.c[k];
     if (!((mask >> k) & 1u) || c < 0) continue;
     if (next < 0) {
       next = c;
@@ -196,15 +230,16 @@ __device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t ma
 
 // Closest-hit order: pushes the hit inner children of a node except the nearest, farthest first
 // (so that pops come nearest first), and returns the nearest (-1 if none).
-__device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint32_t mask, const int4& ch, const float tn[4],
-                                                        uint32_t* __restrict__ stack, uint32_t stride, uint32_t& sp) {
+__device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint32_t mask, const Kids& ch,
+                                                        const float tn[kBvhWidth], uint32_t* __restrict__ stack,
+                                                        uint32_t stride, uint32_t& sp) {
   uint16_t* const stack16 = reinterpret_cast<uint16_t*>(stack - threadIdx.x) + threadIdx.x;
   const bool s16 = sc.stack16 != 0;
-  float key[4];
-  int32_t val[4];
+  float key[kBvhWidth];
+  int32_t val[kBvhWidth];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int32_t c = sel4(ch, k);
+  for (int k = 0; k < kBvhWidth; ++k) {
+    const int32_t c = ch.c[k];
     const bool push = ((mask >> k) & 1u) && c > 0;
     val[k] = push ? c : 0;  // 0: nothing to push (the order is a heuristic; no child is dropped)
     key[k] = push ? tn[k] : INFINITY;
@@ -219,10 +254,16 @@ __device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint
     val[a] = val[b];                                     \
     val[b] = tv;                                         \
   }
-  DL_CSWAP(0, 1) DL_CSWAP(2, 3) DL_CSWAP(0, 2) DL_CSWAP(1, 3) DL_CSWAP(1, 2)
+  if constexpr (kBvhWidth == 4) {
+    DL_CSWAP(0, 1) DL_CSWAP(2, 3) DL_CSWAP(0, 2) DL_CSWAP(1, 3) DL_CSWAP(1, 2)
+  } else {  // 19 compare-exchanges (Batcher's network for eight)
+    DL_CSWAP(0, 2) DL_CSWAP(1, 3) DL_CSWAP(4, 6) DL_CSWAP(5, 7) DL_CSWAP(0, 4) DL_CSWAP(1, 5) DL_CSWAP(2, 6)
+    DL_CSWAP(3, 7) DL_CSWAP(0, 1) DL_CSWAP(2, 3) DL_CSWAP(4, 5) DL_CSWAP(6, 7) DL_CSWAP(2, 4) DL_CSWAP(3, 5)
+    DL_CSWAP(1, 4) DL_CSWAP(3, 6) DL_CSWAP(1, 2) DL_CSWAP(3, 4) DL_CSWAP(5, 6)
+  }
 #undef DL_CSWAP
 #pragma unroll
-  for (int k = 3; k >= 1; --k) {
+  for (int k = kBvhWidth - 1; k >= 1; --k) {
     if (val[k] == 0) continue;
     if (s16) stack16[sp * stride] = (uint16_t)val[k];
     else stack[sp * stride] = (uint32_t)val[k];
@@ -235,14 +276,14 @@ __device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint
 // The triangles of the hit leaf children of a closest-hit node visit, nearest box first; a leaf
 // whose box starts beyond the best hit so far (the bound of node_hits with the hit's distance)
 // holds no hit at or below it and is skipped.  Keeps the lowest (t, occluder index) hit.
-__device__ __forceinline__ void closest_leaves(const DevScene& sc, uint32_t leaves, const int4& ch, const float tn[4], f3 o, f3 d,
+__device__ __forceinline__ void closest_leaves(const DevScene& sc, uint32_t leaves, const Kids& ch, const float tn[kBvhWidth], f3 o, f3 d,
                                                float tmax, uint32_t alpha, bool& have, Hit& hit, Tally& tl) {
   while (leaves) {
     int kb = 0;
     float tb = INFINITY;
     bool any = false;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kBvhWidth; ++k)
       if (((leaves >> k) & 1u) && (!any || tn[k] < tb)) {
         kb = k;
         tb = tn[k];
@@ -250,7 +291,7 @@ __device__ __forceinline__ void closest_leaves(const DevScene& sc, uint32_t leav
       }
     leaves &= ~(1u << kb);
     if (have && tb > hit.t * (1.0f + kBoxEps)) continue;
-    const uint32_t v = (uint32_t)(-1 - sel4(ch, kb));
+    const uint32_t v = (uint32_t)(-1 - kid(ch, kb));
     for (uint32_t i = v >> 2, end = (v >> 2) + (v & 3u) + 1u; i < end; ++i) {
       const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
       float t, u, vv;
@@ -291,8 +332,8 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
   int32_t node = 0;
   for (;;) {
     tl.nodes++;
-    float tn[4];
-    int4 ch;
+    float tn[kBvhWidth];
+    Kids ch;
     const float limit = (ANY || !have ? tmax : bt) * (1.0f + kBoxEps);
     const uint32_t mask = node_hits(sc, node, sr, limit, tn, &ch);
     if constexpr (!ANY) {
@@ -313,7 +354,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, uint32_t* __restric
     for (;;) {
       if (i == end) {
         if (!leaves) break;
-        const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
+        const uint32_t v = (uint32_t)(-1 - kid(ch, __builtin_ctz(leaves)));
         leaves &= leaves - 1;
         i = v >> 2;
         end = i + (v & 3u) + 1u;
@@ -2513,14 +2554,14 @@ __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __re
                                               const SlabRay& sr, float tmax, int32_t& node, uint32_t& sp, Tally& tl,
                                               uint32_t alpha) {
   tl.nodes++;
-  float tn[4];
-  int4 ch;
+  float tn[kBvhWidth];
+  Kids ch;
   const uint32_t mask = node_hits(sc, node, sr, tmax * (1.0f + kBoxEps), tn, &ch);
   uint32_t leaves = mask & leaf_bits(ch), i = 0, end = 0;
   for (;;) {
     if (i == end) {
       if (!leaves) break;
-      const uint32_t v = (uint32_t)(-1 - sel4(ch, __builtin_ctz(leaves)));
+      const uint32_t v = (uint32_t)(-1 - kid(ch, __builtin_ctz(leaves)));
       leaves &= leaves - 1;
       i = v >> 2;
       end = i + (v & 3u) + 1u;
@@ -2549,8 +2590,8 @@ __device__ __forceinline__ uint32_t visit_any(const DevScene& sc, uint32_t* __re
 // `mask` over the node's children (refs `ch`, box entries `tn`), and triangles [i, end) of the
 // leaf under test.
 struct PendingLeaves {
-  int4 ch;
-  float tn[4];
+  Kids ch;
+  float tn[kBvhWidth];
   uint32_t mask, i, end;
 };
 
@@ -2569,7 +2610,7 @@ __device__ __forceinline__ uint32_t step_closest(const DevScene& sc, uint32_t* _
     float tb = INFINITY;
     bool any = false;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kBvhWidth; ++k)
       if (((q.mask >> k) & 1u) && (!any || q.tn[k] < tb)) {
         kb = k;
         tb = q.tn[k];
@@ -2577,7 +2618,7 @@ __device__ __forceinline__ uint32_t step_closest(const DevScene& sc, uint32_t* _
       }
     q.mask &= ~(1u << kb);
     if (have && tb > hit.t * (1.0f + kBoxEps)) continue;
-    const uint32_t v = (uint32_t)(-1 - sel4(q.ch, kb));
+    const uint32_t v = (uint32_t)(-1 - kid(q.ch, kb));
     q.i = v >> 2;
     q.end = q.i + (v & 3u) + 1u;
   }

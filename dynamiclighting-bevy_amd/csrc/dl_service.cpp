@@ -688,7 +688,7 @@ DevScene scene_of(const Resident& r) {
   sc.nodes = r.d_nodes.p;
   sc.tris = reinterpret_cast<const float4*>(r.d_occ.p);
   sc.ntris = r.nocc;
-  sc.stack = 3 * std::max(1u, r.depth) + 1;
+  sc.stack = bvh_stack_entries(r.depth);
   sc.stack16 = r.nodes < 65536 ? 1u : 0u;
   return sc;
 }
