@@ -103,16 +103,22 @@ __device__ __forceinline__ SlabRay slab_ray(f3 o, f3 d) {
   return r;
 }
 
-// The child slots of a node (> 0 inner node, < 0 leaf, 0 unused).
+// The child slots of a node (> 0 inner node, < 0 leaf, 0 unused), held as int4 vectors: a select
+// over the elements of an array would be turned back into an indexed load from scratch memory.
 struct Kids {
-  int32_t c[kBvhWidth];
+  int4 lo;
+#if DL_BVH_WIDTH == 8
+  int4 hi;
+#endif
 };
-// Child slot k of a node for a k known only at run time (a select chain, no indexed registers).
+__device__ __forceinline__ int32_t sel4(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+// Child slot k of a node (k known at compile time or not).
 __device__ __forceinline__ int32_t kid(const Kids& ch, int k) {
-  int32_t r = ch.c[0];
-#pragma unroll
-  for (int j = 1; j < kBvhWidth; ++j) r = k == j ? ch.c[j] : r;
-  return r;
+#if DL_BVH_WIDTH == 8
+  return k < 4 ? sel4(ch.lo, k) : sel4(ch.hi, k - 4);
+#else
+  return sel4(ch.lo, k);
+#endif
 }
 
 // Box tests of the (up to kBvhWidth) children of `node`: bit k of the result is set when child
@@ -139,14 +145,11 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
     qw[4 * v + 2] = t.z;
     qw[4 * v + 3] = t.w;
   }
-#pragma unroll
-  for (int v = 0; v < kw; ++v) {  // 0 marks an unused child slot (the root is no child)
-    const int4 c = reinterpret_cast<const int4*>(nb + 1 + kQv)[v];
-    ch->c[4 * v] = c.x;
-    ch->c[4 * v + 1] = c.y;
-    ch->c[4 * v + 2] = c.z;
-    ch->c[4 * v + 3] = c.w;
-  }
+  // 0 marks an unused child slot (the root is no child)
+  ch->lo = reinterpret_cast<const int4*>(nb + 1 + kQv)[0];
+#if DL_BVH_WIDTH == 8
+  ch->hi = reinterpret_cast<const int4*>(nb + 1 + kQv)[1];
+#endif
   const float ax = __uint_as_float((q0.w & 0xffu) << 23) * r.ix;
   const float ay = __uint_as_float(((q0.w >> 8) & 0xffu) << 23) * r.iy;
   const float az = __uint_as_float(((q0.w >> 16) & 0xffu) << 23) * r.iz;
@@ -168,7 +171,7 @@ __device__ __forceinline__ uint32_t node_hits(const DevScene& sc, int32_t node, 
     n = __builtin_fmaf(fabsf(n), -kBoxEps, n);
     f = __builtin_fmaf(fabsf(f), kBoxEps, f);
     tn[k] = n;
-    if (ch->c[k] != 0 && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
+    if (kid(*ch, k) != 0 && n <= f && f >= 0.0f && n <= limit) mask |= 1u << k;
   }
   return mask;
 }
@@ -194,7 +197,7 @@ __device__ __forceinline__ void wave_add_at(unsigned long long* base, uint32_t k
 __device__ __forceinline__ uint32_t leaf_bits(const Kids& ch) {
   uint32_t m = 0;
 #pragma unroll
-  for (int k = 0; k < kBvhWidth; ++k) m |= ch.c[k] < 0 ? 1u << k : 0u;
+  for (int k = 0; k < kBvhWidth; ++k) m |= kid(ch, k) < 0 ? 1u << k : 0u;
   return m;
 }
 
@@ -207,7 +210,7 @@ __device__ __forceinline__ int32_t push_children(const DevScene& sc, uint32_t ma
   float next_t = INFINITY;
 #pragma unroll
   for (int k = 0; k < kBvhWidth; ++k) {
-    const int32_t c = ch.c[k];
+    const int32_t c = kid(ch, k);
     if (!((mask >> k) & 1u) || c < 0) continue;
     if (next < 0) {
       next = c;
@@ -239,7 +242,7 @@ __device__ __forceinline__ int32_t push_children_sorted(const DevScene& sc, uint
   int32_t val[kBvhWidth];
 #pragma unroll
   for (int k = 0; k < kBvhWidth; ++k) {
-    const int32_t c = ch.c[k];
+    const int32_t c = kid(ch, k);
     const bool push = ((mask >> k) & 1u) && c > 0;
     val[k] = push ? c : 0;  // 0: nothing to push (the order is a heuristic; no child is dropped)
     key[k] = push ? tn[k] : INFINITY;
