@@ -31,13 +31,15 @@ KERNELS = {
 }
 
 
-@pytest.fixture(scope="module")
-def kernel_meta(tmp_path_factory):
+@pytest.fixture(scope="module", params=[4, 8], ids=["bvh4", "bvh8"])
+def kernel_meta(request, tmp_path_factory):
+    """Kernel metadata of the default build (4-wide BVH nodes) and of the DL_BVH_WIDTH=8 build."""
     if not (os.path.exists(HIPCC) or shutil.which(HIPCC)):
         pytest.skip("hipcc not found")
     out = str(tmp_path_factory.mktemp("isa") / "dl_kernels.s")
     cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
-           "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument", "--cuda-device-only", "-S", SRC, "-o", out]
+           "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-DDL_BVH_WIDTH={request.param}",
+           "--cuda-device-only", "-S", SRC, "-o", out]
     subprocess.run(cmd, check=True, capture_output=True, timeout=600)
     text = open(out).read()
     meta = {}
